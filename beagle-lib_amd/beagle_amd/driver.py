@@ -1,0 +1,159 @@
+"""Runs a whole likelihood (and adjoint) evaluation of a synthetic problem
+through any library that exports the BEAGLE API, issuing the calls a
+client such as BEAST or the reference's examples/hmctest makes:
+
+  set tips / eigen-system / frequencies / category weights and rates /
+  pattern weights -> update transition matrices -> post-order partials ->
+  (accumulate scale factors) -> root log-likelihood -> root pre-order
+  partials = frequencies -> pre-order partials -> edge derivatives with
+  D = r_c Q and D = r_c^2 Q^2 -> cross products.
+
+Buffer numbering: tips 0..n-1, post-order partials of internal node v at
+v (n..2n-2), pre-order partials of node v at 2n-1+v.  Matrix of the
+branch above v is v; the two derivative matrices follow at 2n-1 and 2n
+(their transposes, for the S>4 manual-transpose convention, at 2n+1, 2n+2).
+The scale buffer of internal node v is v-n; the cumulative buffer is n-1.
+"""
+import numpy as np
+
+from . import api, synth
+
+
+class Evaluator:
+    def __init__(self, lib, prob, *, tips_as="states", scaling="none", preorder=True,
+                 precision="double", extra_flags=0, resources=None, transpose="auto"):
+        self.lib = lib
+        self.prob = prob
+        self.scaling = scaling
+        self.preorder = preorder
+        self.transpose = transpose
+        n = prob.tree.n
+        self.n = n
+        nodes = 2 * n - 1
+        self.pre_base = nodes
+        pre = nodes if preorder else 0
+        if tips_as == "states":
+            compact = n
+            partials = (n - 1) + pre
+        else:
+            compact = 0
+            partials = nodes + pre
+        self.tips_as = tips_as
+        self.n_matrices = nodes + 4
+        self.d1_index = nodes
+        self.d2_index = nodes + 1
+        pref = api.FLAG_EIGEN_REAL
+        req = 0
+        if precision == "double":
+            req |= api.FLAG_PRECISION_DOUBLE
+        else:
+            req |= api.FLAG_PRECISION_SINGLE
+        if scaling == "manual_log":
+            pref |= api.FLAG_SCALERS_LOG
+        elif scaling == "always":
+            pref |= api.FLAG_SCALING_ALWAYS
+        elif scaling == "auto":
+            pref |= api.FLAG_SCALING_AUTO
+        if transpose == "auto":
+            pref |= api.FLAG_PREORDER_TRANSPOSE_AUTO
+        pref |= extra_flags
+        self.inst = lib.create(n, partials, compact, prob.S, prob.P, 1, self.n_matrices, prob.C,
+                               n, resources=resources, preference=pref, requirement=req)
+        self._setup()
+
+    # -- data ---------------------------------------------------------------
+    def _setup(self):
+        p, inst = self.prob, self.inst
+        for t in range(self.n):
+            if self.tips_as == "states":
+                inst.set_tip_states(t, p.tips[t])
+            else:
+                inst.set_tip_partials(t, synth.states_to_partials(p.tips[t], p.S).reshape(-1))
+        m = p.model
+        inst.set_eigen(0, m.evec.reshape(-1), m.ivec.reshape(-1), m.evals)
+        inst.set_state_frequencies(0, m.freqs)
+        inst.set_category_weights(0, p.weights)
+        inst.set_category_rates(p.rates)
+        inst.set_pattern_weights(p.pattern_weights)
+        # derivative matrices r_c Q and r_c^2 Q^2 with pad 0
+        # (examples/hmctest/hmctest.cpp:398-427)
+        Q = m.Q
+        d1 = np.concatenate([(r * Q).reshape(-1) for r in p.rates])
+        d2 = np.concatenate([(r * r * (Q @ Q)).reshape(-1) for r in p.rates])
+        inst.set_differential_matrix(self.d1_index, d1)
+        inst.set_differential_matrix(self.d2_index, d2)
+        self.manual_transpose = self.transpose == "manual" and p.S > 4
+        if self.manual_transpose:
+            inst.transpose_transition_matrices([self.d1_index, self.d2_index],
+                                               [self.d1_index + 2, self.d2_index + 2])
+
+    def _scale_write(self, v):
+        return v - self.n
+
+    # -- evaluation ---------------------------------------------------------
+    def update_matrices(self, lengths=None):
+        tree = self.prob.tree
+        if lengths is None:
+            lengths = tree.lengths
+        idx = np.arange(2 * self.n - 2)
+        self.inst.update_transition_matrices(0, idx, None, None, lengths[:2 * self.n - 2])
+        if self.manual_transpose:
+            # pre-order "self" matrices are passed transposed (the reference
+            # GPU convention for S > 4, examples/hmctest/hmctest.cpp:495)
+            raise NotImplementedError("manual transpose driver path")
+
+    def post_order(self):
+        tree = self.prob.tree
+        manual = self.scaling in ("manual", "manual_log")
+        ops = synth.postorder_ops(tree, self._scale_write if manual else None)
+        cum = self.n - 1
+        if manual:
+            self.inst.reset_scale_factors(cum)
+            self.inst.update_partials(ops, cum)
+        else:
+            self.inst.update_partials(ops, api.OP_NONE)
+        return ops
+
+    def root(self):
+        tree = self.prob.tree
+        manual = self.scaling in ("manual", "manual_log")
+        cum = self.n - 1 if manual else api.OP_NONE
+        return self.inst.root_log_likelihood(tree.root, 0, 0, cum)
+
+    def log_likelihood(self, lengths=None):
+        self.update_matrices(lengths)
+        self.post_order()
+        return self.root()
+
+    def pre_order(self):
+        tree = self.prob.tree
+        self.inst.set_partials(self.pre_base + tree.root,
+                               np.tile(self.prob.model.freqs, self.prob.C * self.prob.P))
+        ops = synth.preorder_ops(tree, self.pre_base)
+        self.inst.update_pre_partials(ops, api.OP_NONE)
+        return ops
+
+    def branch_nodes(self):
+        return [v for v in range(2 * self.n - 1) if v != self.prob.tree.root]
+
+    def edge_derivatives(self, second=False, per_site=True):
+        nodes = self.branch_nodes()
+        d = self.d2_index if second else self.d1_index
+        return self.inst.edge_derivatives(nodes, [self.pre_base + v for v in nodes],
+                                          [d] * len(nodes), per_site=per_site)
+
+    def cross_products(self):
+        nodes = self.branch_nodes()
+        return self.inst.cross_products(nodes, [self.pre_base + v for v in nodes], 0, 0,
+                                        self.prob.tree.lengths[nodes])
+
+    def gradient(self, lengths=None):
+        """logL, dlogL/dt per branch, d2logL/dt2 per branch."""
+        logl = self.log_likelihood(lengths)
+        self.pre_order()
+        _, g1, sq1 = self.edge_derivatives(False, per_site=False)
+        _, g2, _ = self.edge_derivatives(True, per_site=False)
+        return logl, g1, g2 - sq1
+
+    def close(self):
+        self.inst.finalize()
