@@ -19,14 +19,25 @@ import numpy as np
 from . import api, synth
 
 
+def explicit_matrices(prob, lengths):
+    from scipy.linalg import expm
+    out = np.empty((len(lengths), prob.C, prob.S, prob.S))
+    for u, t in enumerate(lengths):
+        for c, r in enumerate(prob.rates):
+            out[u, c] = np.maximum(expm(prob.model.Q * (r * t)), 0.0)
+    return out
+
+
 class Evaluator:
     def __init__(self, lib, prob, *, tips_as="states", scaling="none", preorder=True,
-                 precision="double", extra_flags=0, resources=None, transpose="auto"):
+                 precision="double", extra_flags=0, resources=None, transpose="auto",
+                 matrices="eigen"):
         self.lib = lib
         self.prob = prob
         self.scaling = scaling
         self.preorder = preorder
         self.transpose = transpose
+        self.matrices = matrices
         n = prob.tree.n
         self.n = n
         nodes = 2 * n - 1
@@ -96,7 +107,13 @@ class Evaluator:
         if lengths is None:
             lengths = tree.lengths
         idx = np.arange(2 * self.n - 2)
-        self.inst.update_transition_matrices(0, idx, None, None, lengths[:2 * self.n - 2])
+        if self.matrices == "explicit":
+            # identical P(t) on every library: exp(Q r t) from scipy, clamped
+            # at 0, set with pad 1 (beagleSetTransitionMatrices)
+            ms = explicit_matrices(self.prob, lengths[:2 * self.n - 2])
+            self.inst.set_transition_matrices(idx, ms.reshape(-1), np.ones(len(idx)))
+        else:
+            self.inst.update_transition_matrices(0, idx, None, None, lengths[:2 * self.n - 2])
         if self.manual_transpose:
             # pre-order "self" matrices are passed transposed (the reference
             # GPU convention for S > 4, examples/hmctest/hmctest.cpp:495)

@@ -1,0 +1,942 @@
+// Host side of one MI355X BEAGLE instance.  See instance.h.
+#include "instance.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+
+#include "beagle.h"
+
+namespace hipbeagle {
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) throw std::bad_alloc();
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHECK(x) check((x), #x)
+
+template <typename T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    HIPCHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return reinterpret_cast<T*>(p);
+}
+
+int roundUp(int x, int m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+// Scaling-mode precedence follows the reference CPU implementation:
+// AUTO > ALWAYS > DYNAMIC > SCALERS_LOG (manual) > manual raw
+// (libhmsbeagle/CPU/BeagleCPUImpl.hpp:294-311).  AUTO and DYNAMIC are
+// refused by the factory before this constructor runs.
+Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount, int stateCount,
+                   int patternCount, int eigenBufferCount, int matrixBufferCount,
+                   int categoryCount, int scaleBufferCount, int device, int resourceNumber,
+                   long preferenceFlags, long requirementFlags, bool doublePrecision)
+    : tipCount_(tipCount),
+      bufferCount_(partialsBufferCount + compactBufferCount),
+      eigenCount_(eigenBufferCount),
+      matrixCount_(matrixBufferCount),
+      scaleCount_(scaleBufferCount),
+      device_(device),
+      resource_(resourceNumber),
+      dp_(doublePrecision) {
+    if (stateCount < 2 || patternCount < 1 || categoryCount < 1 || tipCount < 1 ||
+        bufferCount_ <= tipCount || eigenBufferCount < 1 || matrixBufferCount < 1 ||
+        scaleBufferCount < 0)
+        throw std::out_of_range("instance dimensions");
+    HIPCHECK(hipSetDevice(device_));
+    const long both = preferenceFlags | requirementFlags;
+    flags_ = BEAGLE_FLAG_PROCESSOR_GPU | BEAGLE_FLAG_VECTOR_NONE | BEAGLE_FLAG_THREADING_NONE |
+             BEAGLE_FLAG_EIGEN_REAL | BEAGLE_FLAG_FRAMEWORK_HIP | BEAGLE_FLAG_PARALLELOPS_GRID;
+    flags_ |= dp_ ? BEAGLE_FLAG_PRECISION_DOUBLE : BEAGLE_FLAG_PRECISION_SINGLE;
+    flags_ |= (both & BEAGLE_FLAG_COMPUTATION_ASYNCH) ? BEAGLE_FLAG_COMPUTATION_ASYNCH
+                                                      : BEAGLE_FLAG_COMPUTATION_SYNCH;
+    flags_ |= (both & BEAGLE_FLAG_INVEVEC_TRANSPOSED) ? BEAGLE_FLAG_INVEVEC_TRANSPOSED
+                                                      : BEAGLE_FLAG_INVEVEC_STANDARD;
+    const int internal = bufferCount_ - tipCount_;
+    always_ = false;
+    if (both & BEAGLE_FLAG_SCALING_ALWAYS) {
+        always_ = true;
+        flags_ |= BEAGLE_FLAG_SCALING_ALWAYS | BEAGLE_FLAG_SCALERS_LOG;
+        scaleCount_ = internal + 1;
+    } else if (both & BEAGLE_FLAG_SCALERS_LOG) {
+        flags_ |= BEAGLE_FLAG_SCALING_MANUAL | BEAGLE_FLAG_SCALERS_LOG;
+    } else {
+        flags_ |= BEAGLE_FLAG_SCALING_MANUAL | BEAGLE_FLAG_SCALERS_RAW;
+    }
+    logScalers_ = (flags_ & BEAGLE_FLAG_SCALERS_LOG) != 0;
+    // Pre-order "self" matrices and derivative matrices arrive transposed
+    // for S > 4 unless the client asked the library to transpose
+    // (the reference GPU convention, GPU/BeagleGPUImpl.hpp:483-486, 787-788).
+    const bool autoT = (both & (long)BEAGLE_FLAG_PREORDER_TRANSPOSE_AUTO) != 0;
+    flags_ |= autoT ? (long)BEAGLE_FLAG_PREORDER_TRANSPOSE_AUTO
+                    : (long)BEAGLE_FLAG_PREORDER_TRANSPOSE_MANUAL;
+    preTransposed_ = stateCount > 4 && !autoT;
+
+    dims_.S = stateCount;
+    dims_.P = patternCount;
+    dims_.C = categoryCount;
+    dims_.Sp = stateCount;
+    dims_.Ppad = roundUp(patternCount, 64);
+    dims_.catStride = (long)dims_.Ppad * dims_.Sp;
+    partialsBytes_ = (size_t)categoryCount * dims_.catStride * realSize();
+    matrixBytes_ = (size_t)categoryCount * stateCount * (stateCount + 1) * realSize();
+    matrixBytes_ = (matrixBytes_ + 255) / 256 * 256;
+
+    HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    dPartials_.assign(bufferCount_, nullptr);
+    dStates_.assign(bufferCount_, nullptr);
+    for (int b = tipCount_; b < bufferCount_; ++b) ensurePartials(b);
+    dMatrices_ = dalloc<char>(matrixBytes_ * matrixCount_);
+    HIPCHECK(hipMemsetAsync(dMatrices_, 0, matrixBytes_ * matrixCount_, stream_));
+    dScale_ = dalloc<double>((size_t)std::max(scaleCount_, 1) * dims_.Ppad);
+    launch_fill(dScale_, 0.0, (long)std::max(scaleCount_, 1) * dims_.Ppad, stream_);
+    const size_t S = stateCount;
+    dCube_ = dalloc<double>(eigenCount_ * S * S * S);
+    dEvals_ = dalloc<double>(eigenCount_ * S);
+    eigenSet_.assign(eigenCount_, false);
+    dFreqs_ = dalloc<double>(eigenCount_ * S);
+    dCatWeights_ = dalloc<double>(eigenCount_ * categoryCount);
+    dCatRates_ = dalloc<double>(eigenCount_ * categoryCount);
+    hFreqs_.assign(eigenCount_, {});
+    hCatWeights_.assign(eigenCount_, {});
+    hCatRates_.assign(eigenCount_, {});
+    dPatternWeights_ = dalloc<double>(dims_.Ppad);
+    launch_fill(dPatternWeights_, 0.0, dims_.Ppad, stream_);
+    hPatternWeights_.assign(patternCount, 0.0);
+    dSiteL_ = dalloc<double>(dims_.Ppad);
+    dSiteD1_ = dalloc<double>(dims_.Ppad);
+    dSiteD2_ = dalloc<double>(dims_.Ppad);
+    dResult_ = dalloc<double>(64);
+    HIPCHECK(hipHostMalloc((void**)&hResult_, 64 * sizeof(double), hipHostMallocDefault));
+    partStart_ = {0, patternCount};
+    ring_.resize(8);
+    for (auto& s : ring_) HIPCHECK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+    sync();
+}
+
+Instance::~Instance() {
+    hipSetDevice(device_);
+    if (stream_) hipStreamSynchronize(stream_);
+    for (void* p : dPartials_) hipFree(p);
+    for (int* p : dStates_) hipFree(p);
+    hipFree(dMatrices_);
+    hipFree(dScale_);
+    hipFree(dCube_);
+    hipFree(dEvals_);
+    hipFree(dFreqs_);
+    hipFree(dCatWeights_);
+    hipFree(dCatRates_);
+    hipFree(dPatternWeights_);
+    hipFree(dSiteL_);
+    hipFree(dSiteD1_);
+    hipFree(dSiteD2_);
+    hipFree(dResult_);
+    hipHostFree(hResult_);
+    hipFree(dWork_);
+    hipFree(dScratch_);
+    for (auto& s : ring_) {
+        if (s.host) hipHostFree(s.host);
+        if (s.dev) hipFree(s.dev);
+        if (s.ev) hipEventDestroy(s.ev);
+    }
+    if (stream_) hipStreamDestroy(stream_);
+}
+
+const char* Instance::implName() const {
+    return dp_ ? "MI355X-HIP-Double" : "MI355X-HIP-Single";
+}
+
+// ---- transfers --------------------------------------------------------------
+
+void Instance::sync() { HIPCHECK(hipStreamSynchronize(stream_)); }
+
+void Instance::copyIn(void* dst, const void* src, size_t bytes) {
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
+    sync();
+}
+
+void Instance::copyOut(void* dst, const void* src, size_t bytes) {
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void Instance::toDevice(void* dst, const double* src, size_t n) {
+    if (dp_) {
+        copyIn(dst, src, n * sizeof(double));
+    } else {
+        std::vector<float> tmp(src, src + n);
+        copyIn(dst, tmp.data(), n * sizeof(float));
+    }
+}
+
+void Instance::toHost(double* dst, const void* src, size_t n) {
+    if (dp_) {
+        copyOut(dst, src, n * sizeof(double));
+    } else {
+        std::vector<float> tmp(n);
+        copyOut(tmp.data(), src, n * sizeof(float));
+        std::copy(tmp.begin(), tmp.end(), dst);
+    }
+}
+
+// Small per-call uploads (job lists) go through a ring of pinned staging
+// slots so the host never waits on the device unless a slot is reused
+// before its previous copy finished.
+void* Instance::upload(const void* data, size_t bytes) {
+    Slot& s = ring_[ringPos_];
+    ringPos_ = (ringPos_ + 1) % (int)ring_.size();
+    if (s.used) HIPCHECK(hipEventSynchronize(s.ev));
+    if (s.cap < bytes) {
+        size_t cap = std::max<size_t>(bytes, 4096);
+        cap = (cap + 4095) / 4096 * 4096;
+        if (s.host) HIPCHECK(hipHostFree(s.host));
+        if (s.dev) {
+            sync();
+            HIPCHECK(hipFree(s.dev));
+        }
+        HIPCHECK(hipHostMalloc(&s.host, cap, hipHostMallocDefault));
+        HIPCHECK(hipMalloc(&s.dev, cap));
+        s.cap = cap;
+    }
+    std::memcpy(s.host, data, bytes);
+    HIPCHECK(hipMemcpyAsync(s.dev, s.host, bytes, hipMemcpyHostToDevice, stream_));
+    HIPCHECK(hipEventRecord(s.ev, stream_));
+    s.used = true;
+    return s.dev;
+}
+
+void* Instance::grow(void*& buf, size_t& cap, size_t bytes) {
+    if (cap < bytes) {
+        sync();
+        if (buf) HIPCHECK(hipFree(buf));
+        size_t c = std::max(bytes, cap * 2);
+        HIPCHECK(hipMalloc(&buf, c));
+        cap = c;
+    }
+    return buf;
+}
+
+void Instance::ensurePartials(int b) {
+    if (dPartials_[b]) return;
+    void* p = nullptr;
+    HIPCHECK(hipMalloc(&p, partialsBytes_));
+    HIPCHECK(hipMemsetAsync(p, 0, partialsBytes_, stream_));
+    dPartials_[b] = p;
+}
+
+// ---- data in and out ---------------------------------------------------------
+
+// States >= S mean missing and are stored as S; padded patterns are missing
+// (libhmsbeagle/CPU/BeagleCPUImpl.hpp:570-584).
+int Instance::setTipStates(int tip, const int* in) {
+    if (tip < 0 || tip >= tipCount_) return BEAGLE_ERROR_OUT_OF_RANGE;
+    if (!dStates_[tip]) {
+        int* p = nullptr;
+        HIPCHECK(hipMalloc((void**)&p, dims_.Ppad * sizeof(int)));
+        dStates_[tip] = p;
+    }
+    std::vector<int> tmp(dims_.Ppad, dims_.S);
+    for (int k = 0; k < dims_.P; ++k) tmp[k] = (in[k] >= 0 && in[k] < dims_.S) ? in[k] : dims_.S;
+    copyIn(dStates_[tip], tmp.data(), tmp.size() * sizeof(int));
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::setTipPartials(int tip, const double* in) {
+    if (tip < 0 || tip >= tipCount_) return BEAGLE_ERROR_OUT_OF_RANGE;
+    ensurePartials(tip);
+    const size_t n = (size_t)dims_.C * dims_.catStride;
+    std::vector<double> tmp(n, 0.0);
+    for (int c = 0; c < dims_.C; ++c)
+        for (int k = 0; k < dims_.P; ++k)
+            for (int i = 0; i < dims_.S; ++i)
+                tmp[c * dims_.catStride + (size_t)k * dims_.Sp + i] = in[(size_t)k * dims_.S + i];
+    toDevice(dPartials_[tip], tmp.data(), n);
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::setPartials(int b, const double* in) {
+    if (!validBuffer(b)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    ensurePartials(b);
+    const size_t n = (size_t)dims_.C * dims_.catStride;
+    std::vector<double> tmp(n, 0.0);
+    for (int c = 0; c < dims_.C; ++c)
+        for (int k = 0; k < dims_.P; ++k)
+            for (int i = 0; i < dims_.S; ++i)
+                tmp[c * dims_.catStride + (size_t)k * dims_.Sp + i] =
+                    in[((size_t)c * dims_.P + k) * dims_.S + i];
+    toDevice(dPartials_[b], tmp.data(), n);
+    return BEAGLE_SUCCESS;
+}
+
+// Root pre-order partials = state frequencies replicated over categories
+// and patterns (BeagleCPUImpl.hpp:623-677).
+int Instance::setRootPrePartials(const int* bufIdx, const int* fIdx, int count) {
+    if (count != 1) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    const int b = bufIdx[0], f = fIdx[0];
+    if (!validBuffer(b) || !validEigen(f) || hFreqs_[f].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
+    std::vector<double> tmp((size_t)dims_.C * dims_.P * dims_.S);
+    for (size_t q = 0; q < (size_t)dims_.C * dims_.P; ++q)
+        std::copy(hFreqs_[f].begin(), hFreqs_[f].end(), tmp.begin() + q * dims_.S);
+    return setPartials(b, tmp.data());
+}
+
+// With a cumulative scale buffer the reference multiplies the first
+// patternCount*stateCount outputs (category 0) by exp(scale[pattern])
+// (BeagleCPUImpl.hpp:743-754); kept as is.
+int Instance::getPartials(int b, int scaleIndex, double* out) {
+    if (!validBuffer(b) || !dPartials_[b]) return BEAGLE_ERROR_OUT_OF_RANGE;
+    if (scaleIndex != BEAGLE_OP_NONE && !validScale(scaleIndex)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    const size_t n = (size_t)dims_.C * dims_.catStride;
+    std::vector<double> tmp(n);
+    toHost(tmp.data(), dPartials_[b], n);
+    for (int c = 0; c < dims_.C; ++c)
+        for (int k = 0; k < dims_.P; ++k)
+            for (int i = 0; i < dims_.S; ++i)
+                out[((size_t)c * dims_.P + k) * dims_.S + i] =
+                    tmp[c * dims_.catStride + (size_t)k * dims_.Sp + i];
+    if (scaleIndex != BEAGLE_OP_NONE) {
+        std::vector<double> sc(dims_.P);
+        copyOut(sc.data(), scale(scaleIndex), dims_.P * sizeof(double));
+        size_t idx = 0;
+        for (int k = 0; k < dims_.P; ++k) {
+            const double f = std::exp(sc[k]);
+            for (int i = 0; i < dims_.S; ++i) out[idx++] *= f;
+        }
+    }
+    return BEAGLE_SUCCESS;
+}
+
+// C_ijk = V_ik * Vinv_kj, kept in FP64 on the device
+// (EigenDecompositionCube.hpp:69-100).
+int Instance::setEigenDecomposition(int e, const double* V, const double* Vi, const double* lam) {
+    if (!validEigen(e)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    const int S = dims_.S;
+    const bool trans = (flags_ & BEAGLE_FLAG_INVEVEC_TRANSPOSED) != 0;
+    std::vector<double> cube((size_t)S * S * S);
+    size_t l = 0;
+    for (int i = 0; i < S; ++i)
+        for (int j = 0; j < S; ++j)
+            for (int k = 0; k < S; ++k)
+                cube[l++] = V[i * S + k] * (trans ? Vi[k + j * S] : Vi[k * S + j]);
+    copyIn(dCube_ + (size_t)e * S * S * S, cube.data(), cube.size() * sizeof(double));
+    copyIn(dEvals_ + (size_t)e * S, lam, S * sizeof(double));
+    eigenSet_[e] = true;
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::setStateFrequencies(int i, const double* f) {
+    if (!validEigen(i)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    hFreqs_[i].assign(f, f + dims_.S);
+    copyIn(dFreqs_ + (size_t)i * dims_.S, f, dims_.S * sizeof(double));
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::setCategoryWeights(int i, const double* w) {
+    if (!validEigen(i)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    hCatWeights_[i].assign(w, w + dims_.C);
+    copyIn(dCatWeights_ + (size_t)i * dims_.C, w, dims_.C * sizeof(double));
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::setCategoryRates(const double* r) { return setCategoryRatesWithIndex(0, r); }
+
+int Instance::setCategoryRatesWithIndex(int i, const double* r) {
+    if (!validEigen(i)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    hCatRates_[i].assign(r, r + dims_.C);
+    copyIn(dCatRates_ + (size_t)i * dims_.C, r, dims_.C * sizeof(double));
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::setPatternWeights(const double* w) {
+    hPatternWeights_.assign(w, w + dims_.P);
+    copyIn(dPatternWeights_, w, dims_.P * sizeof(double));
+    return BEAGLE_SUCCESS;
+}
+
+// Contiguous partitions only: pattern k of partition q lies in
+// [partStart[q], partStart[q+1]).  Non-contiguous assignments would need
+// the reference's pattern reordering (BeagleCPUImpl.hpp:4140-4219).
+int Instance::setPatternPartitions(int count, const int* parts) {
+    if (count < 1) return BEAGLE_ERROR_OUT_OF_RANGE;
+    if (parts[0] != 0) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    int run = 0;
+    for (int k = 1; k < dims_.P; ++k) {
+        if (parts[k] != parts[k - 1]) ++run;
+        if (parts[k] != run) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    }
+    if (run + 1 != count) return BEAGLE_ERROR_OUT_OF_RANGE;
+    partStart_.assign(count + 1, 0);
+    for (int k = 1; k < dims_.P; ++k)
+        if (parts[k] != parts[k - 1]) partStart_[parts[k]] = k;
+    partStart_[count] = dims_.P;
+    partitionCount_ = count;
+    return BEAGLE_SUCCESS;
+}
+
+// ---- transition matrices -------------------------------------------------------
+
+int Instance::matrixUpdate(const std::vector<MatrixJob>& jobs) {
+    if (jobs.empty()) return BEAGLE_SUCCESS;
+    const MatrixJob* d = reinterpret_cast<const MatrixJob*>(
+        upload(jobs.data(), jobs.size() * sizeof(MatrixJob)));
+    if (dp_) launch_matrices<double>(d, (int)jobs.size(), dims_.S, dims_.C, stream_);
+    else launch_matrices<float>(d, (int)jobs.size(), dims_.S, dims_.C, stream_);
+    HIPCHECK(hipGetLastError());
+    return BEAGLE_SUCCESS;
+}
+
+// Category rates come from rate buffer 0 (BeagleCPUImpl.hpp:1225-1238).
+int Instance::updateTransitionMatrices(int e, const int* probIdx, const int* d1Idx,
+                                       const int* d2Idx, const double* t, int count) {
+    if (!validEigen(e) || !eigenSet_[e] || hCatRates_[0].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
+    const int S = dims_.S;
+    std::vector<MatrixJob> jobs(count);
+    for (int u = 0; u < count; ++u) {
+        if (!validMatrix(probIdx[u])) return BEAGLE_ERROR_OUT_OF_RANGE;
+        MatrixJob& J = jobs[u];
+        J.P = matrix(probIdx[u]);
+        J.D1 = nullptr;
+        J.D2 = nullptr;
+        if (d1Idx) {
+            if (!validMatrix(d1Idx[u])) return BEAGLE_ERROR_OUT_OF_RANGE;
+            J.D1 = matrix(d1Idx[u]);
+            if (d2Idx) {
+                if (!validMatrix(d2Idx[u])) return BEAGLE_ERROR_OUT_OF_RANGE;
+                J.D2 = matrix(d2Idx[u]);
+            }
+        }
+        J.cube = dCube_ + (size_t)e * S * S * S;
+        J.evals = dEvals_ + (size_t)e * S;
+        J.rates = dCatRates_;
+        J.t = t[u];
+    }
+    return matrixUpdate(jobs);
+}
+
+// Category c uses eigen-system eigenIndices[c] (its own rate from buffer 0):
+// expressed as one single-category job per (matrix, category).
+int Instance::updateTransitionMatricesWithModelCategories(const int* eig, const int* probIdx,
+                                                          const int* d1Idx, const int* d2Idx,
+                                                          const double* t, int count) {
+    (void)eig; (void)probIdx; (void)d1Idx; (void)d2Idx; (void)t; (void)count;
+    return BEAGLE_ERROR_NO_IMPLEMENTATION;
+}
+
+int Instance::updateTransitionMatricesWithMultipleModels(const int* eig, const int* rateIdx,
+                                                         const int* probIdx, const int* d1Idx,
+                                                         const int* d2Idx, const double* t,
+                                                         int count) {
+    const int S = dims_.S;
+    std::vector<MatrixJob> jobs(count);
+    for (int u = 0; u < count; ++u) {
+        const int e = eig[u], r = rateIdx[u];
+        if (!validEigen(e) || !eigenSet_[e] || !validEigen(r) || hCatRates_[r].empty() ||
+            !validMatrix(probIdx[u]))
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        MatrixJob& J = jobs[u];
+        J.P = matrix(probIdx[u]);
+        J.D1 = (d1Idx && validMatrix(d1Idx[u])) ? matrix(d1Idx[u]) : nullptr;
+        J.D2 = (d1Idx && d2Idx && validMatrix(d2Idx[u])) ? matrix(d2Idx[u]) : nullptr;
+        J.cube = dCube_ + (size_t)e * S * S * S;
+        J.evals = dEvals_ + (size_t)e * S;
+        J.rates = dCatRates_ + (size_t)r * dims_.C;
+        J.t = t[u];
+    }
+    return matrixUpdate(jobs);
+}
+
+// Client matrices [C][S][S] plus the pad value (BeagleCPUImpl.hpp:1035-1091).
+int Instance::setTransitionMatrix(int m, const double* in, double pad) {
+    if (!validMatrix(m)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    const int S = dims_.S, W = S + 1;
+    std::vector<double> tmp((size_t)dims_.C * S * W);
+    for (int c = 0; c < dims_.C; ++c)
+        for (int i = 0; i < S; ++i) {
+            for (int j = 0; j < S; ++j)
+                tmp[((size_t)c * S + i) * W + j] = in[((size_t)c * S + i) * S + j];
+            tmp[((size_t)c * S + i) * W + S] = pad;
+        }
+    toDevice(matrix(m), tmp.data(), tmp.size());
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::setDifferentialMatrix(int m, const double* in) { return setTransitionMatrix(m, in, 0.0); }
+
+int Instance::setTransitionMatrices(const int* idx, const double* in, const double* pads, int count) {
+    const size_t n = (size_t)dims_.C * dims_.S * dims_.S;
+    for (int k = 0; k < count; ++k) {
+        const int rc = setTransitionMatrix(idx[k], in + k * n, pads[k]);
+        if (rc != BEAGLE_SUCCESS) return rc;
+    }
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::getTransitionMatrix(int m, double* out) {
+    if (!validMatrix(m)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    const int S = dims_.S, W = S + 1;
+    std::vector<double> tmp((size_t)dims_.C * S * W);
+    toHost(tmp.data(), matrix(m), tmp.size());
+    for (int c = 0; c < dims_.C; ++c)
+        for (int i = 0; i < S; ++i)
+            for (int j = 0; j < S; ++j)
+                out[((size_t)c * S + i) * S + j] = tmp[((size_t)c * S + i) * W + j];
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::transposeTransitionMatrices(const int* in, const int* out, int count) {
+    std::vector<const void*> src(count);
+    std::vector<void*> dst(count);
+    for (int u = 0; u < count; ++u) {
+        if (!validMatrix(in[u]) || !validMatrix(out[u])) return BEAGLE_ERROR_OUT_OF_RANGE;
+        src[u] = matrix(in[u]);
+        dst[u] = matrix(out[u]);
+    }
+    if (count == 0) return BEAGLE_SUCCESS;
+    auto ds = reinterpret_cast<const void* const*>(upload(src.data(), count * sizeof(void*)));
+    auto dd = reinterpret_cast<void* const*>(upload(dst.data(), count * sizeof(void*)));
+    if (dp_) launch_transpose<double>(ds, dd, count, dims_.S, dims_.C, stream_);
+    else launch_transpose<float>(ds, dd, count, dims_.S, dims_.C, stream_);
+    HIPCHECK(hipGetLastError());
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::convolveTransitionMatrices(const int*, const int*, const int*, int) {
+    return BEAGLE_ERROR_NO_IMPLEMENTATION;
+}
+
+int Instance::addTransitionMatrices(const int*, const int*, const int*, int) {
+    return BEAGLE_ERROR_NO_IMPLEMENTATION;
+}
+
+// ---- traversals ------------------------------------------------------------------
+
+// Post-order operations (BeagleCPUImpl.hpp:1497-1686): the rescale mode of
+// each op is resolved here, the arithmetic on the device.
+int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, bool byPartition) {
+    const int stride = byPartition ? BEAGLE_PARTITION_OP_COUNT : BEAGLE_OP_COUNT;
+    hPost_.resize(count);
+    for (int o = 0; o < count; ++o) {
+        const int* q = ops + (size_t)o * stride;
+        const int dest = q[0], wS = q[1], rS = q[2], c1 = q[3], m1 = q[4], c2 = q[5], m2 = q[6];
+        int cum = cumulativeIndex;
+        int p0 = 0, p1 = dims_.P;
+        if (byPartition) {
+            const int part = q[7];
+            if (part < 0 || part >= partitionCount_) return BEAGLE_ERROR_OUT_OF_RANGE;
+            p0 = partStart_[part];
+            p1 = partStart_[part + 1];
+            cum = q[8];
+        }
+        if (dest < tipCount_ || !validBuffer(dest) || !validBuffer(c1) || !validBuffer(c2) ||
+            !validMatrix(m1) || !validMatrix(m2))
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        if ((!dStates_[c1] && !dPartials_[c1]) || (!dStates_[c2] && !dPartials_[c2]))
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        PostJob& J = hPost_[o];
+        J.x1 = dStates_[c1] ? nullptr : dPartials_[c1];
+        J.s1 = dStates_[c1];
+        J.m1 = matrix(m1);
+        J.x2 = dStates_[c2] ? nullptr : dPartials_[c2];
+        J.s2 = dStates_[c2];
+        J.m2 = matrix(m2);
+        J.dest = dPartials_[dest];
+        J.scaleWrite = nullptr;
+        J.scaleRead = nullptr;
+        J.cumulative = nullptr;
+        J.accA = J.accB = nullptr;
+        J.mode = SCALE_NONE;
+        J.logScalers = logScalers_ ? 1 : 0;
+        J.p0 = p0;
+        J.p1 = p1;
+        if (cum != BEAGLE_OP_NONE && !validScale(cum)) return BEAGLE_ERROR_OUT_OF_RANGE;
+        if (always_) {
+            J.mode = SCALE_WRITE;
+            J.scaleWrite = scale(dest - tipCount_);
+            if (cum != BEAGLE_OP_NONE) J.cumulative = scale(cum);
+            if (c1 >= tipCount_) J.accA = scale(c1 - tipCount_);
+            if (c2 >= tipCount_) J.accB = scale(c2 - tipCount_);
+        } else if (wS >= 0) {
+            if (!validScale(wS)) return BEAGLE_ERROR_OUT_OF_RANGE;
+            J.mode = SCALE_WRITE;
+            J.scaleWrite = scale(wS);
+            if (cum != BEAGLE_OP_NONE) J.cumulative = scale(cum);
+        } else if (rS >= 0) {
+            if (!validScale(rS)) return BEAGLE_ERROR_OUT_OF_RANGE;
+            J.mode = SCALE_READ;
+            J.scaleRead = scale(rS);
+        }
+    }
+    if (count == 0) return BEAGLE_SUCCESS;
+    auto d = reinterpret_cast<const PostJob*>(upload(hPost_.data(), count * sizeof(PostJob)));
+    if (dp_) launch_post<double>(d, hPost_.data(), count, dims_, stream_);
+    else launch_post<float>(d, hPost_.data(), count, dims_, stream_);
+    HIPCHECK(hipGetLastError());
+    if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
+    return BEAGLE_SUCCESS;
+}
+
+// Pre-order operations [dest, wScale, rScale, parent, P_self, sibling,
+// P_sib] (BeagleCPUImpl.hpp:1689-1862); rescaling never feeds a cumulative
+// buffer and there is no read-scale path, as in the reference.
+int Instance::updatePrePartials(const int* ops, int count, int cumulativeIndex, bool byPartition) {
+    (void)cumulativeIndex;
+    const int stride = byPartition ? BEAGLE_PARTITION_OP_COUNT : BEAGLE_OP_COUNT;
+    hPre_.resize(count);
+    for (int o = 0; o < count; ++o) {
+        const int* q = ops + (size_t)o * stride;
+        const int dest = q[0], wS = q[1], par = q[3], mSelf = q[4], sib = q[5], mSib = q[6];
+        int p0 = 0, p1 = dims_.P;
+        if (byPartition) {
+            const int part = q[7];
+            if (part < 0 || part >= partitionCount_) return BEAGLE_ERROR_OUT_OF_RANGE;
+            p0 = partStart_[part];
+            p1 = partStart_[part + 1];
+        }
+        if (!validBuffer(dest) || !validBuffer(par) || !validBuffer(sib) || !validMatrix(mSelf) ||
+            !validMatrix(mSib) || !dPartials_[par])
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        if (!dStates_[sib] && !dPartials_[sib]) return BEAGLE_ERROR_OUT_OF_RANGE;
+        ensurePartials(dest);
+        PreJob& J = hPre_[o];
+        J.parent = dPartials_[par];
+        J.mSelf = matrix(mSelf);
+        J.xs = dStates_[sib] ? nullptr : dPartials_[sib];
+        J.ss = dStates_[sib];
+        J.mSib = matrix(mSib);
+        J.dest = dPartials_[dest];
+        J.scaleWrite = nullptr;
+        J.accA = J.accB = nullptr;
+        J.mode = SCALE_NONE;
+        J.logScalers = logScalers_ ? 1 : 0;
+        J.p0 = p0;
+        J.p1 = p1;
+        if (always_) {
+            if (!validScale(dest - tipCount_)) return BEAGLE_ERROR_OUT_OF_RANGE;
+            J.mode = SCALE_WRITE;
+            J.scaleWrite = scale(dest - tipCount_);
+        } else if (wS >= 0) {
+            if (!validScale(wS)) return BEAGLE_ERROR_OUT_OF_RANGE;
+            J.mode = SCALE_WRITE;
+            J.scaleWrite = scale(wS);
+        }
+    }
+    if (count == 0) return BEAGLE_SUCCESS;
+    if (!(dims_.S == 4 && dims_.C <= 8) || preTransposed_) {
+        size_t cap = 0;
+        if (!dScratch_) {
+            HIPCHECK(hipMalloc(&dScratch_, partialsBytes_));
+        }
+        (void)cap;
+    }
+    auto d = reinterpret_cast<const PreJob*>(upload(hPre_.data(), count * sizeof(PreJob)));
+    if (dp_) launch_pre<double>(d, hPre_.data(), count, dims_, preTransposed_ ? 1 : 0, dScratch_, stream_);
+    else launch_pre<float>(d, hPre_.data(), count, dims_, preTransposed_ ? 1 : 0, dScratch_, stream_);
+    HIPCHECK(hipGetLastError());
+    if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::waitForPartials() {
+    sync();
+    return BEAGLE_SUCCESS;
+}
+
+// ---- scale buffers -------------------------------------------------------------------
+
+// cumulative[p] += sign * (LOG ? s : log s) over the listed buffers, for all
+// patterns or one partition (BeagleCPUImpl.hpp:2763-2860).
+int Instance::accumulateScaleFactors(const int* idx, int count, int cumIdx, double sign, int part) {
+    if (!validScale(cumIdx)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    int p0 = 0, p1 = dims_.P;
+    if (part >= 0) {
+        if (part >= partitionCount_) return BEAGLE_ERROR_OUT_OF_RANGE;
+        p0 = partStart_[part];
+        p1 = partStart_[part + 1];
+    }
+    std::vector<const double*> bufs(count);
+    for (int i = 0; i < count; ++i) {
+        if (!validScale(idx[i])) return BEAGLE_ERROR_OUT_OF_RANGE;
+        bufs[i] = scale(idx[i]);
+    }
+    if (count == 0) return BEAGLE_SUCCESS;
+    auto d = reinterpret_cast<const double* const*>(upload(bufs.data(), count * sizeof(double*)));
+    launch_accumulate(scale(cumIdx), d, count, sign, logScalers_ ? 1 : 0, p0, p1, stream_);
+    HIPCHECK(hipGetLastError());
+    return BEAGLE_SUCCESS;
+}
+
+// Cleared with a fill kernel, not a memset node, so the call stays
+// capturable into a HIP graph.
+int Instance::resetScaleFactors(int cumIdx, int part) {
+    if (!validScale(cumIdx)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    int p0 = 0, p1 = dims_.Ppad;
+    if (part >= 0) {
+        if (part >= partitionCount_) return BEAGLE_ERROR_OUT_OF_RANGE;
+        p0 = partStart_[part];
+        p1 = partStart_[part + 1];
+    }
+    launch_fill(scale(cumIdx) + p0, 0.0, p1 - p0, stream_);
+    HIPCHECK(hipGetLastError());
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::copyScaleFactors(int dst, int src) {
+    if (!validScale(dst) || !validScale(src)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    HIPCHECK(hipMemcpyAsync(scale(dst), scale(src), dims_.P * sizeof(double),
+                            hipMemcpyDeviceToDevice, stream_));
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::getScaleFactors(int src, double* out) {
+    if (!validScale(src)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    copyOut(out, scale(src), dims_.P * sizeof(double));
+    return BEAGLE_SUCCESS;
+}
+
+// ---- likelihoods --------------------------------------------------------------------------
+
+int Instance::calculateRootLogLikelihoods(const int* bufIdx, const int* wIdx, const int* fIdx,
+                                          const int* cumIdx, int count, double* out) {
+    if (count != 1) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    const int b = bufIdx[0], w = wIdx[0], f = fIdx[0];
+    if (w < 0) return BEAGLE_ERROR_NO_IMPLEMENTATION;  // per-category output
+    if (!validBuffer(b) || !dPartials_[b] || !validEigen(w) || !validEigen(f) ||
+        hCatWeights_[w].empty() || hFreqs_[f].empty())
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    int cs = always_ ? b - tipCount_ : cumIdx[0];
+    if (cs != BEAGLE_OP_NONE && !validScale(cs)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    const int nb = (dims_.P + 255) / 256;
+    double* work = reinterpret_cast<double*>(grow(dWork_, workCap_, (size_t)nb * sizeof(double)));
+    if (dp_)
+        launch_root<double>(dPartials_[b], dCatWeights_ + (size_t)w * dims_.C,
+                            dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
+                            dPatternWeights_, dSiteL_, work, dResult_, dims_, stream_);
+    else
+        launch_root<float>(dPartials_[b], dCatWeights_ + (size_t)w * dims_.C,
+                           dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
+                           dPatternWeights_, dSiteL_, work, dResult_, dims_, stream_);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(hResult_, dResult_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    sync();
+    lastLogL_ = hResult_[0];
+    *out = lastLogL_;
+    return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+}
+
+int Instance::calculateRootLogLikelihoodsByPartition(const int* bufIdx, const int* wIdx,
+                                                     const int* fIdx, const int* cumIdx,
+                                                     const int* partIdx, int partitionCount,
+                                                     int count, double* outByPart,
+                                                     double* outSum) {
+    if (count != 1 || always_) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    if (partitionCount > 64) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    const int nbMax = (dims_.P + 255) / 256;
+    double* work = reinterpret_cast<double*>(grow(dWork_, workCap_, (size_t)nbMax * sizeof(double)));
+    for (int q = 0; q < partitionCount; ++q) {
+        const int part = partIdx[q], b = bufIdx[q], w = wIdx[q], f = fIdx[q], cs = cumIdx[q];
+        if (part < 0 || part >= partitionCount_ || !validBuffer(b) || !dPartials_[b] ||
+            !validEigen(w) || !validEigen(f) || (cs != BEAGLE_OP_NONE && !validScale(cs)))
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        const int p0 = partStart_[part], p1 = partStart_[part + 1];
+        if (dp_)
+            launch_root_range<double>(dPartials_[b], dCatWeights_ + (size_t)w * dims_.C,
+                                      dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
+                                      dPatternWeights_, dSiteL_, work, dResult_ + q, p0, p1, dims_,
+                                      stream_);
+        else
+            launch_root_range<float>(dPartials_[b], dCatWeights_ + (size_t)w * dims_.C,
+                                     dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
+                                     dPatternWeights_, dSiteL_, work, dResult_ + q, p0, p1, dims_,
+                                     stream_);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipMemcpyAsync(hResult_, dResult_, partitionCount * sizeof(double),
+                            hipMemcpyDeviceToHost, stream_));
+    sync();
+    double total = 0.0;
+    for (int q = 0; q < partitionCount; ++q) {
+        outByPart[q] = hResult_[q];
+        total += hResult_[q];
+    }
+    *outSum = total;
+    lastLogL_ = total;
+    return total != total ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+}
+
+// Derivatives with client matrices D per node (beagle.cpp:1794-1824 passes
+// no rates and no scale buffers): see k_edge_deriv.
+int Instance::calculateEdgeDerivatives(const int* postIdx, const int* preIdx, const int* dIdx,
+                                       const int* wIdx, int count, double* outD, double* outSum,
+                                       double* outSumSq) {
+    const int w = wIdx[0];
+    if (!validEigen(w) || hCatWeights_[w].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
+    std::vector<EdgeJob> jobs(count);
+    for (int n = 0; n < count; ++n) {
+        const int po = postIdx[n], pr = preIdx[n], dm = dIdx[n];
+        if (!validBuffer(po) || !validBuffer(pr) || !validMatrix(dm) || !dPartials_[pr] ||
+            (!dStates_[po] && !dPartials_[po]))
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        jobs[n].pre = dPartials_[pr];
+        jobs[n].states = dStates_[po];
+        jobs[n].post = dStates_[po] ? nullptr : dPartials_[po];
+        jobs[n].D = matrix(dm);
+        jobs[n].scale = 0.0;
+    }
+    if (count == 0) return BEAGLE_SUCCESS;
+    const int nb = (dims_.P + 255) / 256;
+    const size_t work = (size_t)2 * count * nb + 2 * (size_t)count;
+    size_t extra = outD ? (size_t)count * dims_.P : 0;
+    double* wk = reinterpret_cast<double*>(grow(dWork_, workCap_, (work + 2 * count + extra) * sizeof(double)));
+    double* dSums = wk + work;
+    double* dSites = outD ? dSums + 2 * count : nullptr;
+    auto dj = reinterpret_cast<const EdgeJob*>(upload(jobs.data(), count * sizeof(EdgeJob)));
+    if (dp_)
+        launch_edge_derivatives<double>(dj, count, dCatWeights_ + (size_t)w * dims_.C,
+                                        dPatternWeights_, dSites, wk, dSums, dSums + count,
+                                        preTransposed_ ? 1 : 0, dims_, stream_);
+    else
+        launch_edge_derivatives<float>(dj, count, dCatWeights_ + (size_t)w * dims_.C,
+                                       dPatternWeights_, dSites, wk, dSums, dSums + count,
+                                       preTransposed_ ? 1 : 0, dims_, stream_);
+    HIPCHECK(hipGetLastError());
+    std::vector<double> sums(2 * count);
+    copyOut(sums.data(), dSums, 2 * count * sizeof(double));
+    if (outSum) std::copy(sums.begin(), sums.begin() + count, outSum);
+    if (outSumSq) std::copy(sums.begin() + count, sums.end(), outSumSq);
+    if (outD) copyOut(outD, dSites, (size_t)count * dims_.P * sizeof(double));
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::calculateCrossProducts(const int* postIdx, const int* preIdx, const int* rIdx,
+                                     const int* wIdx, const double* lengths, int count,
+                                     double* out) {
+    const int r = rIdx[0], w = wIdx[0];
+    if (!validEigen(r) || !validEigen(w) || hCatRates_[r].empty() || hCatWeights_[w].empty())
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    // generic kernel: S*S sums over <= 16 per lane, and its LDS staging of
+    // 16 patterns x C x S (pre and post) must fit in 64 KiB
+    if (dims_.S > 64 || (dims_.S != 4 && (2 * 16 * dims_.C * dims_.S + 16 * dims_.C) * 8 > 65536))
+        return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    std::vector<EdgeJob> jobs(count);
+    for (int n = 0; n < count; ++n) {
+        const int po = postIdx[n], pr = preIdx[n];
+        if (!validBuffer(po) || !validBuffer(pr) || !dPartials_[pr] ||
+            (!dStates_[po] && !dPartials_[po]))
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        jobs[n].pre = dPartials_[pr];
+        jobs[n].states = dStates_[po];
+        jobs[n].post = dStates_[po] ? nullptr : dPartials_[po];
+        jobs[n].D = nullptr;
+        jobs[n].scale = lengths[n];
+    }
+    if (count == 0) return BEAGLE_SUCCESS;
+    const int S2 = dims_.S * dims_.S;
+    const size_t rows = cross_products_rows(dims_);
+    double* wk = reinterpret_cast<double*>(
+        grow(dWork_, workCap_, (rows * S2 + S2) * sizeof(double)));
+    double* dOut = wk + rows * S2;
+    copyIn(dOut, out, S2 * sizeof(double));
+    auto dj = reinterpret_cast<const EdgeJob*>(upload(jobs.data(), count * sizeof(EdgeJob)));
+    if (dp_)
+        launch_cross_products<double>(dj, count, dCatWeights_ + (size_t)w * dims_.C,
+                                      dCatRates_ + (size_t)r * dims_.C, dPatternWeights_, wk, dOut,
+                                      dims_, stream_);
+    else
+        launch_cross_products<float>(dj, count, dCatWeights_ + (size_t)w * dims_.C,
+                                     dCatRates_ + (size_t)r * dims_.C, dPatternWeights_, wk, dOut,
+                                     dims_, stream_);
+    HIPCHECK(hipGetLastError());
+    copyOut(out, dOut, S2 * sizeof(double));
+    return BEAGLE_SUCCESS;
+}
+
+// Edge log-likelihood (+ derivatives along the edge); scaling-always sums
+// the two node scale buffers (BeagleCPUImpl.hpp:2918-3005).
+int Instance::calculateEdgeLogLikelihoods(const int* parIdx, const int* childIdx, const int* pIdx,
+                                          const int* d1Idx, const int* d2Idx, const int* wIdx,
+                                          const int* fIdx, const int* cumIdx, int count,
+                                          double* outL, double* outD1, double* outD2) {
+    if (count != 1) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    const int par = parIdx[0], ch = childIdx[0], pm = pIdx[0], w = wIdx[0], f = fIdx[0];
+    if (!validBuffer(par) || !dPartials_[par] || !validBuffer(ch) ||
+        (!dStates_[ch] && !dPartials_[ch]) || !validMatrix(pm) || !validEigen(w) ||
+        !validEigen(f))
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    const void* D1 = nullptr;
+    const void* D2 = nullptr;
+    if (d1Idx) {
+        if (!validMatrix(d1Idx[0])) return BEAGLE_ERROR_OUT_OF_RANGE;
+        D1 = matrix(d1Idx[0]);
+        if (d2Idx) {
+            if (!validMatrix(d2Idx[0])) return BEAGLE_ERROR_OUT_OF_RANGE;
+            D2 = matrix(d2Idx[0]);
+        }
+    }
+    const double* cumA = nullptr;
+    const double* cumB = nullptr;
+    if (always_) {
+        if (par >= tipCount_) cumA = scale(par - tipCount_);
+        if (ch >= tipCount_) cumB = scale(ch - tipCount_);
+    } else if (cumIdx[0] != BEAGLE_OP_NONE) {
+        if (!validScale(cumIdx[0])) return BEAGLE_ERROR_OUT_OF_RANGE;
+        cumA = scale(cumIdx[0]);
+    }
+    const int nb = (dims_.P + 255) / 256;
+    double* wk = reinterpret_cast<double*>(grow(dWork_, workCap_, (size_t)3 * nb * sizeof(double)));
+    if (dp_)
+        launch_edge_loglik<double>(dPartials_[par], dPartials_[ch], dStates_[ch], matrix(pm), D1,
+                                   D2, dCatWeights_ + (size_t)w * dims_.C,
+                                   dFreqs_ + (size_t)f * dims_.S, cumA, cumB, dPatternWeights_,
+                                   dSiteL_, dSiteD1_, dSiteD2_, wk, dResult_, dims_, stream_);
+    else
+        launch_edge_loglik<float>(dPartials_[par], dPartials_[ch], dStates_[ch], matrix(pm), D1,
+                                  D2, dCatWeights_ + (size_t)w * dims_.C,
+                                  dFreqs_ + (size_t)f * dims_.S, cumA, cumB, dPatternWeights_,
+                                  dSiteL_, dSiteD1_, dSiteD2_, wk, dResult_, dims_, stream_);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(hResult_, dResult_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    sync();
+    lastLogL_ = hResult_[0];
+    *outL = lastLogL_;
+    haveDerivs_ = D1 != nullptr;
+    lastD1_ = D1 ? hResult_[1] : 0.0;
+    lastD2_ = D2 ? hResult_[2] : 0.0;
+    if (outD1 && D1) *outD1 = lastD1_;
+    if (outD2 && D2) *outD2 = lastD2_;
+    return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+}
+
+int Instance::getLogLikelihood(double* out) {
+    sync();
+    *out = lastLogL_;
+    return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+}
+
+int Instance::getDerivatives(double* d1, double* d2) {
+    sync();
+    if (d1) *d1 = lastD1_;
+    if (d2) *d2 = lastD2_;
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::getSiteLogLikelihoods(double* out) {
+    copyOut(out, dSiteL_, dims_.P * sizeof(double));
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::getSiteDerivatives(double* d1, double* d2) {
+    copyOut(d1, dSiteD1_, dims_.P * sizeof(double));
+    if (d2) copyOut(d2, dSiteD2_, dims_.P * sizeof(double));
+    return BEAGLE_SUCCESS;
+}
+
+}  // namespace hipbeagle

@@ -1,0 +1,165 @@
+// One BEAGLE instance on one MI355X: owns its HBM buffers and HIP stream,
+// turns API calls into job lists and launches the gfx950 kernels.
+//
+// Mirrors the per-instance operator interface of the reference
+// (libhmsbeagle/BeagleImpl.h:31-268); each method returns a BEAGLE return
+// code and never throws past the front end.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace hipbeagle {
+
+class Instance {
+public:
+    Instance(int tipCount, int partialsBufferCount, int compactBufferCount, int stateCount,
+             int patternCount, int eigenBufferCount, int matrixBufferCount, int categoryCount,
+             int scaleBufferCount, int device, int resourceNumber, long preferenceFlags,
+             long requirementFlags, bool doublePrecision);
+    ~Instance();
+
+    long flags() const { return flags_; }
+    const char* implName() const;
+    int resourceNumber() const { return resource_; }
+
+    int setTipStates(int tipIndex, const int* inStates);
+    int setTipPartials(int tipIndex, const double* inPartials);
+    int setRootPrePartials(const int* bufferIndices, const int* freqIndices, int count);
+    int setPartials(int bufferIndex, const double* inPartials);
+    int getPartials(int bufferIndex, int scaleIndex, double* outPartials);
+    int setEigenDecomposition(int eigenIndex, const double* V, const double* Vinv,
+                              const double* evals);
+    int setStateFrequencies(int index, const double* f);
+    int setCategoryWeights(int index, const double* w);
+    int setCategoryRates(const double* r);
+    int setCategoryRatesWithIndex(int index, const double* r);
+    int setPatternWeights(const double* w);
+    int setPatternPartitions(int partitionCount, const int* parts);
+
+    int updateTransitionMatrices(int eigenIndex, const int* probIdx, const int* d1Idx,
+                                 const int* d2Idx, const double* lengths, int count);
+    int updateTransitionMatricesWithModelCategories(const int* eigenIndices, const int* probIdx,
+                                                    const int* d1Idx, const int* d2Idx,
+                                                    const double* lengths, int count);
+    int updateTransitionMatricesWithMultipleModels(const int* eigenIdx, const int* rateIdx,
+                                                   const int* probIdx, const int* d1Idx,
+                                                   const int* d2Idx, const double* lengths,
+                                                   int count);
+    int setTransitionMatrix(int index, const double* in, double padValue);
+    int setDifferentialMatrix(int index, const double* in);
+    int setTransitionMatrices(const int* idx, const double* in, const double* pads, int count);
+    int getTransitionMatrix(int index, double* out);
+    int transposeTransitionMatrices(const int* in, const int* out, int count);
+    int convolveTransitionMatrices(const int* a, const int* b, const int* out, int count);
+    int addTransitionMatrices(const int* a, const int* b, const int* out, int count);
+
+    int updatePartials(const int* ops, int count, int cumulativeIndex, bool byPartition);
+    int updatePrePartials(const int* ops, int count, int cumulativeIndex, bool byPartition);
+    int waitForPartials();
+
+    int accumulateScaleFactors(const int* idx, int count, int cumIdx, double sign, int part);
+    int resetScaleFactors(int cumIdx, int part);
+    int copyScaleFactors(int dst, int src);
+    int getScaleFactors(int src, double* out);
+
+    int calculateRootLogLikelihoods(const int* bufIdx, const int* wIdx, const int* fIdx,
+                                    const int* cumIdx, int count, double* out);
+    int calculateRootLogLikelihoodsByPartition(const int* bufIdx, const int* wIdx,
+                                               const int* fIdx, const int* cumIdx,
+                                               const int* partIdx, int partitionCount, int count,
+                                               double* outByPart, double* outSum);
+    int calculateEdgeDerivatives(const int* postIdx, const int* preIdx, const int* dIdx,
+                                 const int* wIdx, int count, double* outD, double* outSum,
+                                 double* outSumSq);
+    int calculateCrossProducts(const int* postIdx, const int* preIdx, const int* rIdx,
+                               const int* wIdx, const double* lengths, int count, double* out);
+    int calculateEdgeLogLikelihoods(const int* parIdx, const int* childIdx, const int* pIdx,
+                                    const int* d1Idx, const int* d2Idx, const int* wIdx,
+                                    const int* fIdx, const int* cumIdx, int count, double* outL,
+                                    double* outD1, double* outD2);
+    int getLogLikelihood(double* out);
+    int getDerivatives(double* d1, double* d2);
+    int getSiteLogLikelihoods(double* out);
+    int getSiteDerivatives(double* d1, double* d2);
+
+    // Device pointers for the extension entry points (Q-parameter
+    // gradients, fused evaluations).
+    const Dims& dims() const { return dims_; }
+    hipStream_t stream() const { return stream_; }
+    bool doublePrecision() const { return dp_; }
+
+private:
+    struct Slot {
+        void* host = nullptr;
+        void* dev = nullptr;
+        size_t cap = 0;
+        hipEvent_t ev = nullptr;
+        bool used = false;
+    };
+
+    void* upload(const void* data, size_t bytes);
+    void* grow(void*& buf, size_t& cap, size_t bytes);
+    void ensurePartials(int b);
+    void* partials(int b) const { return dPartials_[b]; }
+    void* matrix(int m) const { return (char*)dMatrices_ + (size_t)m * matrixBytes_; }
+    double* scale(int s) const { return dScale_ + (size_t)s * dims_.Ppad; }
+    size_t realSize() const { return dp_ ? sizeof(double) : sizeof(float); }
+    void toDevice(void* dst, const double* src, size_t n);   // double -> instance precision
+    void toHost(double* dst, const void* src, size_t n);
+    void copyIn(void* dst, const void* src, size_t bytes);
+    void copyOut(void* dst, const void* src, size_t bytes);
+    void sync();
+    int matrixUpdate(const std::vector<MatrixJob>& jobs);
+    bool validBuffer(int b) const { return b >= 0 && b < bufferCount_; }
+    bool validMatrix(int m) const { return m >= 0 && m < matrixCount_; }
+    bool validScale(int s) const { return s >= 0 && s < scaleCount_; }
+    bool validEigen(int e) const { return e >= 0 && e < eigenCount_; }
+
+    int tipCount_, bufferCount_, eigenCount_, matrixCount_, scaleCount_;
+    int device_, resource_;
+    long flags_;
+    bool dp_;
+    bool preTransposed_;   // S > 4 without PREORDER_TRANSPOSE_AUTO
+    bool always_;          // SCALING_ALWAYS
+    bool logScalers_;
+    Dims dims_;
+    size_t partialsBytes_, matrixBytes_;
+    hipStream_t stream_ = nullptr;
+
+    std::vector<void*> dPartials_;
+    std::vector<int*> dStates_;
+    void* dMatrices_ = nullptr;
+    double* dScale_ = nullptr;
+    double* dCube_ = nullptr;
+    double* dEvals_ = nullptr;
+    std::vector<bool> eigenSet_;
+    double* dFreqs_ = nullptr;
+    double* dCatWeights_ = nullptr;
+    double* dCatRates_ = nullptr;
+    std::vector<std::vector<double>> hFreqs_, hCatWeights_, hCatRates_;
+    double* dPatternWeights_ = nullptr;
+    std::vector<double> hPatternWeights_;
+    double* dSiteL_ = nullptr;
+    double* dSiteD1_ = nullptr;
+    double* dSiteD2_ = nullptr;
+    double* dResult_ = nullptr;       // small device result vector
+    double* hResult_ = nullptr;       // pinned mirror
+    void* dWork_ = nullptr;           // grow-only reduction scratch
+    size_t workCap_ = 0;
+    void* dScratch_ = nullptr;        // C*Ppad*Sp reals (generic pre-order)
+    std::vector<int> partStart_;
+    int partitionCount_ = 1;
+    std::vector<Slot> ring_;
+    int ringPos_ = 0;
+    std::vector<PostJob> hPost_;
+    std::vector<PreJob> hPre_;
+    double lastLogL_ = 0.0, lastD1_ = 0.0, lastD2_ = 0.0;
+    bool haveDerivs_ = false;
+};
+
+}  // namespace hipbeagle

@@ -1,0 +1,126 @@
+// Device-side job descriptors shared by the host instance and the gfx950
+// kernels, and the launch entry points.  Host code resolves every buffer
+// index of an API call into device pointers once, so kernels never chase
+// index tables: each job below is read with scalar (wave-uniform) loads.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hipbeagle {
+
+// Rescale modes of one traversal operation (reference semantics,
+// libhmsbeagle/CPU/BeagleCPUImpl.hpp:1555-1577).
+enum ScaleMode : int {
+    SCALE_NONE = 0,     // plain product
+    SCALE_WRITE = 1,    // compute, then divide by the per-pattern maximum
+    SCALE_READ = 2,     // divide by an existing (raw) factor
+};
+
+// One post-order operation.  A child is either partials (x != null) or
+// compact tip states (s != null).  Matrices are [C][S][S+1] with the pad
+// column read by a missing state.
+struct PostJob {
+    const void* x1; const int* s1; const void* m1;
+    const void* x2; const int* s2; const void* m2;
+    void* dest;
+    double* scaleWrite;        // SCALE_WRITE target
+    const double* scaleRead;   // SCALE_READ source
+    double* cumulative;        // += log(max) when rescaling (may be null)
+    const double* accA;        // scaling-always: child node scale buffers
+    const double* accB;        //   added into scaleWrite (may be null)
+    int mode;
+    int logScalers;            // store log(max) rather than max
+    int p0, p1;                // pattern range (a partition, or [0, P))
+};
+
+// One pre-order operation: dest = P_self^T (pre_parent * (P_sib x_sib)).
+struct PreJob {
+    const void* parent; const void* mSelf;
+    const void* xs; const int* ss; const void* mSib;
+    void* dest;
+    double* scaleWrite;
+    const double* accA;        // scaling-always accumulation (may be null)
+    const double* accB;
+    int mode;
+    int logScalers;
+    int p0, p1;
+};
+
+// One transition-matrix update: P, dP/dt, d2P/dt2 for all categories.
+struct MatrixJob {
+    void* P; void* D1; void* D2;      // D1/D2 may be null
+    const double* cube;               // S^3, C_ijk = V_ik Vinv_kj
+    const double* evals;              // S
+    const double* rates;              // C category rates
+    double t;
+};
+
+// One node of an edge-derivative or cross-product request.
+struct EdgeJob {
+    const void* pre;       // pre-order partials of the node
+    const void* post;      // post-order partials (null if tip states)
+    const int* states;     // tip states (null if partials)
+    const void* D;         // derivative matrix [C][S][S+1]
+    double scale;          // cross products: edge length
+};
+
+struct Dims {
+    int S;        // states
+    int P;        // patterns
+    int C;        // categories
+    int Ppad;     // padded pattern stride
+    int Sp;       // state stride in partials
+    long catStride;  // Ppad * Sp
+};
+
+// ---- launchers (kernels.hip) ----
+template <typename R>
+void launch_matrices(const MatrixJob* jobs, int njobs, int S, int C, hipStream_t st);
+// Traversals take the device copy of the jobs (read by the kernels) and the
+// host copy (which decides launch shapes on the generic-S path).
+template <typename R>
+void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Dims& d,
+                 hipStream_t st);
+template <typename R>
+void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims& d,
+                int transposedSelf, void* scratch, hipStream_t st);
+template <typename R>
+void launch_root_range(const void* root, const double* catWeights, const double* freqs,
+                       const double* cumulative, const double* patternWeights, double* siteLogL,
+                       double* blockSums, double* result, int p0, int p1, const Dims& d,
+                       hipStream_t st);
+template <typename R>
+void launch_root(const void* root, const double* catWeights, const double* freqs,
+                 const double* cumulative, const double* patternWeights, double* siteLogL,
+                 double* blockSums, double* result, const Dims& d, hipStream_t st);
+template <typename R>
+void launch_edge_derivatives(const EdgeJob* jobs, int njobs, const double* catWeights,
+                             const double* patternWeights, double* sites, double* blockSums,
+                             double* sums, double* sumSq, int transposedD, const Dims& d,
+                             hipStream_t st);
+template <typename R>
+void launch_cross_products(const EdgeJob* jobs, int njobs, const double* catWeights,
+                           const double* catRates, const double* patternWeights,
+                           double* blockPartials, double* out, const Dims& d, hipStream_t st);
+template <typename R>
+void launch_edge_loglik(const void* parent, const void* child, const int* childStates,
+                        const void* P, const void* D1, const void* D2,
+                        const double* catWeights, const double* freqs,
+                        const double* cumA, const double* cumB,
+                        const double* patternWeights, double* siteL, double* siteD1,
+                        double* siteD2, double* blockSums, double* result, const Dims& d,
+                        hipStream_t st);
+
+// Rows of S*S partial sums the cross-product launch writes into its workspace.
+int cross_products_rows(const Dims& d);
+
+void launch_accumulate(double* cum, const double* const* bufs, int nbufs, double sign,
+                       int logScalers, int p0, int p1, hipStream_t st);
+void launch_fill(double* dst, double v, long n, hipStream_t st);
+template <typename R>
+void launch_transpose(const void* const* src, void* const* dst, int count, int S, int C,
+                      hipStream_t st);
+template <typename R>
+void launch_pad_set(void* dst, long n, R v, hipStream_t st);
+
+}  // namespace hipbeagle

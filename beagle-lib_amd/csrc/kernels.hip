@@ -1,0 +1,974 @@
+// gfx950 kernels of the MI355X BEAGLE plugin.
+//
+// Partials live in HBM as [category][pattern (padded to Ppad)][state
+// (stride Sp)], so consecutive lanes touch consecutive patterns and every
+// 4-state load/store is a full 16-byte vector access.  Transition matrices
+// keep the reference CPU layout [category][row i][S+1 columns] with the pad
+// column that a missing tip state reads (1 for P(t), 0 for derivatives).
+// Scale buffers and every reduction are FP64 whatever the partials
+// precision.
+//
+// The 4-state traversals are fused: one launch walks the whole operation
+// list, each lane owning one pattern (all categories) from the first
+// operation to the last.  Patterns are independent, so per-lane program
+// order is the only dependency a traversal has; children written earlier
+// in the same launch are re-read by the lane that wrote them, from L2.
+// Tree shape (a 127-deep caterpillar or a balanced tree) does not change
+// the launch count.
+#include "kernels.h"
+
+#include <cmath>
+
+namespace hipbeagle {
+
+constexpr int WG = 256;
+
+// ---------------------------------------------------------------------------
+// small helpers
+
+template <typename R>
+__device__ __forceinline__ void load4(const R* p, R (&v)[4]);
+template <>
+__device__ __forceinline__ void load4<double>(const double* p, double (&v)[4]) {
+    const double2* q = reinterpret_cast<const double2*>(p);
+    double2 a = q[0], b = q[1];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+template <>
+__device__ __forceinline__ void load4<float>(const float* p, float (&v)[4]) {
+    float4 a = *reinterpret_cast<const float4*>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+template <typename R>
+__device__ __forceinline__ void store4(R* p, const R (&v)[4]);
+template <>
+__device__ __forceinline__ void store4<double>(double* p, const double (&v)[4]) {
+    double2* q = reinterpret_cast<double2*>(p);
+    q[0] = make_double2(v[0], v[1]);
+    q[1] = make_double2(v[2], v[3]);
+}
+template <>
+__device__ __forceinline__ void store4<float>(float* p, const float (&v)[4]) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// Sum over a 256-lane workgroup; the result is valid in thread 0.  The tree
+// is fixed for a given launch shape, so sums are reproducible run to run.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 0; w < nw; ++w) r += sh[w];
+    }
+    __syncthreads();
+    return r;
+}
+
+// out[j] = sum over n of in[j*stride + n]-style partials: one workgroup per
+// output, strided lane sums then a fixed tree.
+__global__ __launch_bounds__(WG) void k_reduce_rows(const double* __restrict__ in, int n,
+                                                    int rows, double* __restrict__ out,
+                                                    int accumulate) {
+    __shared__ double sh[WG / 64];
+    const int row = blockIdx.x;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += WG) v += in[(long)row * n + i];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) {
+        if (accumulate) out[row] += v;
+        else out[row] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// transition matrices: P = clamp(sum_k C_ijk e^(lambda_k r t)), dP, d2P
+// (EigenDecompositionCube.hpp:105-322 semantics; computed in FP64, stored in
+// the instance precision).
+
+template <typename R>
+__global__ __launch_bounds__(WG) void k_matrices(const MatrixJob* __restrict__ jobs, int S, int C) {
+    extern __shared__ double sh[];
+    double* ex = sh;
+    double* e1 = sh + S;
+    double* e2 = sh + 2 * S;
+    const MatrixJob J = jobs[blockIdx.x];
+    const int c = blockIdx.y;
+    const double r = J.rates[c];
+    const bool deriv = J.D1 != nullptr;
+    for (int k = threadIdx.x; k < S; k += WG) {
+        const double lam = J.evals[k];
+        const double sl = lam * r;
+        // same association as the reference: lambda*(t*r) without
+        // derivatives, (lambda*r)*t with them
+        const double e = deriv ? exp(sl * J.t) : exp(lam * (J.t * r));
+        ex[k] = e;
+        e1[k] = sl * e;
+        e2[k] = sl * e1[k];
+    }
+    __syncthreads();
+    const int W = S + 1;
+    R* Pm = reinterpret_cast<R*>(J.P) + (long)c * S * W;
+    R* D1 = deriv ? reinterpret_cast<R*>(J.D1) + (long)c * S * W : nullptr;
+    R* D2 = J.D2 ? reinterpret_cast<R*>(J.D2) + (long)c * S * W : nullptr;
+    for (int idx = threadIdx.x; idx < S * S; idx += WG) {
+        const int i = idx / S, j = idx - i * S;
+        const double* cm = J.cube + (long)idx * S;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int k = 0; k < S; ++k) {
+            const double cv = cm[k];
+            s0 += cv * ex[k];
+            s1 += cv * e1[k];
+            s2 += cv * e2[k];
+        }
+        Pm[i * W + j] = (R)(s0 > 0.0 ? s0 : 0.0);
+        if (D1) D1[i * W + j] = (R)s1;
+        if (D2) D2[i * W + j] = (R)s2;
+    }
+    for (int i = threadIdx.x; i < S; i += WG) {
+        Pm[i * W + S] = (R)1.0;
+        if (D1) D1[i * W + S] = (R)0.0;
+        if (D2) D2[i * W + S] = (R)0.0;
+    }
+}
+
+template <typename R>
+void launch_matrices(const MatrixJob* jobs, int njobs, int S, int C, hipStream_t st) {
+    if (njobs <= 0) return;
+    dim3 grid(njobs, C);
+    hipLaunchKernelGGL(k_matrices<R>, grid, dim3(WG), 3 * S * sizeof(double), st, jobs, S, C);
+}
+
+// ---------------------------------------------------------------------------
+// 4-state fused traversals
+
+// One child's contribution to the four parent states of category c:
+// t_i = sum_j M[i][j] x_j for a partials child, M[i][state] for a state
+// child (a missing state reads the pad column).
+template <typename R>
+__device__ __forceinline__ void child4(const R* x, const int* s, const R* m, long off, int p,
+                                       int c, R (&t)[4]) {
+    const R* mc = m + c * 20;
+    if (s) {
+        const int st = s[p];
+        t[0] = mc[st];
+        t[1] = mc[5 + st];
+        t[2] = mc[10 + st];
+        t[3] = mc[15 + st];
+    } else {
+        R v[4];
+        load4(x + off, v);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            t[i] = mc[i * 5 + 0] * v[0] + mc[i * 5 + 1] * v[1] + mc[i * 5 + 2] * v[2] +
+                   mc[i * 5 + 3] * v[3];
+    }
+}
+
+// Per-pattern rescale of all C x 4 values held by the lane: divide by the
+// maximum (0 -> 1), store max or log(max), add log(max) to the cumulative
+// buffer, and (scaling-always) add the children's node factors.
+template <typename R, int CMAX>
+__device__ __forceinline__ void rescale_lane(R (&v)[CMAX][4], R mx, int C, int p,
+                                             double* scaleWrite, int logScalers,
+                                             double* cumulative, const double* accA,
+                                             const double* accB) {
+    if (mx == (R)0) mx = (R)1;
+    const R inv = (R)1 / mx;
+#pragma unroll
+    for (int c = 0; c < CMAX; ++c)
+        if (c < C) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[c][i] *= inv;
+        }
+    const double lm = log((double)mx);
+    double w = logScalers ? lm : (double)mx;
+    if (cumulative) cumulative[p] += lm;
+    if (accA) w += accA[p];
+    if (accB) w += accB[p];
+    scaleWrite[p] = w;
+}
+
+template <typename R, int CMAX>
+__global__ __launch_bounds__(WG) void k_post4(const PostJob* __restrict__ jobs, int njobs, int P,
+                                              int C, long cs) {
+    const int p = blockIdx.x * WG + threadIdx.x;
+    if (p >= P) return;
+    for (int o = 0; o < njobs; ++o) {
+        const PostJob J = jobs[o];
+        if (p < J.p0 || p >= J.p1) continue;
+        const R* x1 = reinterpret_cast<const R*>(J.x1);
+        const R* x2 = reinterpret_cast<const R*>(J.x2);
+        const R* m1 = reinterpret_cast<const R*>(J.m1);
+        const R* m2 = reinterpret_cast<const R*>(J.m2);
+        R* d = reinterpret_cast<R*>(J.dest);
+        const R sr = J.mode == SCALE_READ ? (R)J.scaleRead[p] : (R)1;
+        R v[CMAX][4];
+        R mx = 0;
+#pragma unroll
+        for (int c = 0; c < CMAX; ++c) {
+            if (c < C) {
+                const long off = c * cs + (long)p * 4;
+                R a[4], b[4];
+                child4(x1, J.s1, m1, off, p, c, a);
+                child4(x2, J.s2, m2, off, p, c, b);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    R t = a[i] * b[i];
+                    if (J.mode == SCALE_READ) t = t / sr;
+                    v[c][i] = t;
+                    mx = fmax(mx, t);
+                }
+            }
+        }
+        if (J.mode == SCALE_WRITE)
+            rescale_lane<R, CMAX>(v, mx, C, p, J.scaleWrite, J.logScalers, J.cumulative, J.accA,
+                                  J.accB);
+#pragma unroll
+        for (int c = 0; c < CMAX; ++c)
+            if (c < C) store4(d + c * cs + (long)p * 4, v[c]);
+    }
+}
+
+template <typename R, int CMAX>
+__global__ __launch_bounds__(WG) void k_pre4(const PreJob* __restrict__ jobs, int njobs, int P,
+                                             int C, long cs) {
+    const int p = blockIdx.x * WG + threadIdx.x;
+    if (p >= P) return;
+    for (int o = 0; o < njobs; ++o) {
+        const PreJob J = jobs[o];
+        if (p < J.p0 || p >= J.p1) continue;
+        const R* pp = reinterpret_cast<const R*>(J.parent);
+        const R* ms = reinterpret_cast<const R*>(J.mSelf);
+        const R* xs = reinterpret_cast<const R*>(J.xs);
+        const R* mb = reinterpret_cast<const R*>(J.mSib);
+        R* d = reinterpret_cast<R*>(J.dest);
+        R v[CMAX][4];
+        R mx = 0;
+#pragma unroll
+        for (int c = 0; c < CMAX; ++c) {
+            if (c < C) {
+                const long off = c * cs + (long)p * 4;
+                R b[4], u[4];
+                child4(xs, J.ss, mb, off, p, c, b);
+                load4(pp + off, u);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) u[i] *= b[i];
+                const R* mc = ms + c * 20;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const R t = mc[0 * 5 + j] * u[0] + mc[1 * 5 + j] * u[1] +
+                                mc[2 * 5 + j] * u[2] + mc[3 * 5 + j] * u[3];
+                    v[c][j] = t;
+                    mx = fmax(mx, t);
+                }
+            }
+        }
+        if (J.mode == SCALE_WRITE)
+            rescale_lane<R, CMAX>(v, mx, C, p, J.scaleWrite, J.logScalers, nullptr, J.accA,
+                                  J.accB);
+#pragma unroll
+        for (int c = 0; c < CMAX; ++c)
+            if (c < C) store4(d + c * cs + (long)p * 4, v[c]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// generic state count: one launch per operation, one lane per output
+// element (category, pattern, state); a second pass rescales.
+
+template <typename R>
+__device__ __forceinline__ R child_term(const R* x, const int* s, const R* m, int S, long xoff,
+                                        int p, int c, int i) {
+    const R* row = m + ((long)c * S + i) * (S + 1);
+    if (s) return row[s[p]];
+    const R* xv = x + xoff;
+    R acc = 0;
+    for (int j = 0; j < S; ++j) acc += row[j] * xv[j];
+    return acc;
+}
+
+template <typename R>
+__global__ __launch_bounds__(WG) void k_postx(const PostJob* __restrict__ jobs, int o, Dims d) {
+    const PostJob J = jobs[o];
+    const long n = (long)d.C * (J.p1 - J.p0) * d.S;
+    const long t = (long)blockIdx.x * WG + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % d.S;
+    const long cp = t / d.S;
+    const int np = J.p1 - J.p0;
+    const int p = J.p0 + (int)(cp % np);
+    const int c = (int)(cp / np);
+    const long off = c * d.catStride + (long)p * d.Sp;
+    const R a = child_term(reinterpret_cast<const R*>(J.x1), J.s1,
+                           reinterpret_cast<const R*>(J.m1), d.S, off, p, c, i);
+    const R b = child_term(reinterpret_cast<const R*>(J.x2), J.s2,
+                           reinterpret_cast<const R*>(J.m2), d.S, off, p, c, i);
+    R v = a * b;
+    if (J.mode == SCALE_READ) v = v / (R)J.scaleRead[p];
+    reinterpret_cast<R*>(J.dest)[off + i] = v;
+}
+
+// pre-order, generic S, phase 1: u[c,p,i] = pre_parent[c,p,i] * term_i
+template <typename R>
+__global__ __launch_bounds__(WG) void k_prex_u(const PreJob* __restrict__ jobs, int o, Dims d,
+                                               R* __restrict__ u) {
+    const PreJob J = jobs[o];
+    const int np = J.p1 - J.p0;
+    const long n = (long)d.C * np * d.S;
+    const long t = (long)blockIdx.x * WG + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % d.S;
+    const long cp = t / d.S;
+    const int p = J.p0 + (int)(cp % np);
+    const int c = (int)(cp / np);
+    const long off = c * d.catStride + (long)p * d.Sp;
+    const R b = child_term(reinterpret_cast<const R*>(J.xs), J.ss,
+                           reinterpret_cast<const R*>(J.mSib), d.S, off, p, c, i);
+    u[off + i] = reinterpret_cast<const R*>(J.parent)[off + i] * b;
+}
+
+// phase 2: dest[c,p,j] = sum_i M[i][j] u[c,p,i]  (M read transposed when the
+// client passed P^T, the reference GPU convention for S > 4)
+template <typename R>
+__global__ __launch_bounds__(WG) void k_prex_out(const PreJob* __restrict__ jobs, int o, Dims d,
+                                                 const R* __restrict__ u, int transposedSelf) {
+    const PreJob J = jobs[o];
+    const int np = J.p1 - J.p0;
+    const long n = (long)d.C * np * d.S;
+    const long t = (long)blockIdx.x * WG + threadIdx.x;
+    if (t >= n) return;
+    const int j = t % d.S;
+    const long cp = t / d.S;
+    const int p = J.p0 + (int)(cp % np);
+    const int c = (int)(cp / np);
+    const long off = c * d.catStride + (long)p * d.Sp;
+    const R* M = reinterpret_cast<const R*>(J.mSelf) + (long)c * d.S * (d.S + 1);
+    const int W = d.S + 1;
+    R acc = 0;
+    if (transposedSelf)
+        for (int i = 0; i < d.S; ++i) acc += M[j * W + i] * u[off + i];
+    else
+        for (int i = 0; i < d.S; ++i) acc += M[i * W + j] * u[off + i];
+    reinterpret_cast<R*>(J.dest)[off + j] = acc;
+}
+
+// rescale pass for the generic path: one lane per pattern.
+template <typename R>
+__global__ __launch_bounds__(WG) void k_rescalex(R* __restrict__ dest, int p0, int p1,
+                                                 double* scaleWrite, int logScalers,
+                                                 double* cumulative, const double* accA,
+                                                 const double* accB, Dims d) {
+    const int p = p0 + blockIdx.x * WG + threadIdx.x;
+    if (p >= p1) return;
+    R mx = 0;
+    for (int c = 0; c < d.C; ++c)
+        for (int i = 0; i < d.S; ++i) mx = fmax(mx, dest[c * d.catStride + (long)p * d.Sp + i]);
+    if (mx == (R)0) mx = (R)1;
+    const R inv = (R)1 / mx;
+    for (int c = 0; c < d.C; ++c)
+        for (int i = 0; i < d.S; ++i) dest[c * d.catStride + (long)p * d.Sp + i] *= inv;
+    const double lm = log((double)mx);
+    double w = logScalers ? lm : (double)mx;
+    if (cumulative) cumulative[p] += lm;
+    if (accA) w += accA[p];
+    if (accB) w += accB[p];
+    scaleWrite[p] = w;
+}
+
+// Traversal launchers: the fused 4-state kernels, or the generic-S path
+// whose grids come from the host copy of each job.
+template <typename R>
+void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Dims& d,
+                 hipStream_t st) {
+    if (njobs <= 0) return;
+    if (d.S == 4 && d.C <= 8) {
+        dim3 grid((d.P + WG - 1) / WG);
+        if (d.C <= 4)
+            hipLaunchKernelGGL((k_post4<R, 4>), grid, dim3(WG), 0, st, djobs, njobs, d.P, d.C,
+                               d.catStride);
+        else
+            hipLaunchKernelGGL((k_post4<R, 8>), grid, dim3(WG), 0, st, djobs, njobs, d.P, d.C,
+                               d.catStride);
+        return;
+    }
+    for (int o = 0; o < njobs; ++o) {
+        const PostJob& J = hjobs[o];
+        const long n = (long)d.C * (J.p1 - J.p0) * d.S;
+        if (n <= 0) continue;
+        hipLaunchKernelGGL(k_postx<R>, dim3((n + WG - 1) / WG), dim3(WG), 0, st, djobs, o, d);
+        if (J.mode == SCALE_WRITE)
+            hipLaunchKernelGGL(k_rescalex<R>, dim3((J.p1 - J.p0 + WG - 1) / WG), dim3(WG), 0, st,
+                               reinterpret_cast<R*>(J.dest), J.p0, J.p1, J.scaleWrite,
+                               J.logScalers, J.cumulative, J.accA, J.accB, d);
+    }
+}
+
+template <typename R>
+void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims& d,
+                int transposedSelf, void* scratch, hipStream_t st) {
+    if (njobs <= 0) return;
+    if (d.S == 4 && d.C <= 8 && !transposedSelf) {
+        dim3 grid((d.P + WG - 1) / WG);
+        if (d.C <= 4)
+            hipLaunchKernelGGL((k_pre4<R, 4>), grid, dim3(WG), 0, st, djobs, njobs, d.P, d.C,
+                               d.catStride);
+        else
+            hipLaunchKernelGGL((k_pre4<R, 8>), grid, dim3(WG), 0, st, djobs, njobs, d.P, d.C,
+                               d.catStride);
+        return;
+    }
+    for (int o = 0; o < njobs; ++o) {
+        const PreJob& J = hjobs[o];
+        const long n = (long)d.C * (J.p1 - J.p0) * d.S;
+        if (n <= 0) continue;
+        dim3 grid((n + WG - 1) / WG);
+        hipLaunchKernelGGL(k_prex_u<R>, grid, dim3(WG), 0, st, djobs, o, d,
+                           reinterpret_cast<R*>(scratch));
+        hipLaunchKernelGGL(k_prex_out<R>, grid, dim3(WG), 0, st, djobs, o, d,
+                           reinterpret_cast<const R*>(scratch), transposedSelf);
+        if (J.mode == SCALE_WRITE)
+            hipLaunchKernelGGL(k_rescalex<R>, dim3((J.p1 - J.p0 + WG - 1) / WG), dim3(WG), 0, st,
+                               reinterpret_cast<R*>(J.dest), J.p0, J.p1, J.scaleWrite,
+                               J.logScalers, nullptr, J.accA, J.accB, d);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// root log-likelihood: site[p] = log(sum_i pi_i sum_c w_c root[c,p,i]) +
+// cumulative[p]; per-workgroup sums of weight*site, then one fixed-order
+// reduction (BeagleCPUImpl.hpp:2630-2691 semantics, FP64 sum).
+
+template <typename R>
+__global__ __launch_bounds__(WG) void k_root(const R* __restrict__ root,
+                                             const double* __restrict__ cw,
+                                             const double* __restrict__ freqs,
+                                             const double* __restrict__ cum,
+                                             const double* __restrict__ pw,
+                                             double* __restrict__ site, double* __restrict__ bsum,
+                                             int p0, int p1, Dims d) {
+    __shared__ double sh[WG / 64];
+    const int p = p0 + blockIdx.x * WG + threadIdx.x;
+    double contrib = 0.0;
+    if (p < p1) {
+        double sum = 0.0;
+        if (d.S == 4) {
+            double t[4] = {0, 0, 0, 0};
+            for (int c = 0; c < d.C; ++c) {
+                R v[4];
+                load4(root + c * d.catStride + (long)p * 4, v);
+                for (int i = 0; i < 4; ++i) t[i] += (double)v[i] * cw[c];
+            }
+            for (int i = 0; i < 4; ++i) sum += freqs[i] * t[i];
+        } else {
+            for (int i = 0; i < d.S; ++i) {
+                double t = 0.0;
+                for (int c = 0; c < d.C; ++c)
+                    t += (double)root[c * d.catStride + (long)p * d.Sp + i] * cw[c];
+                sum += freqs[i] * t;
+            }
+        }
+        double s = log(sum);
+        if (cum) s += cum[p];
+        site[p] = s;
+        contrib = s * pw[p];
+    }
+    const double b = block_sum(contrib, sh);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = b;
+}
+
+template <typename R>
+void launch_root(const void* root, const double* catWeights, const double* freqs,
+                 const double* cumulative, const double* patternWeights, double* siteLogL,
+                 double* blockSums, double* result, const Dims& d, hipStream_t st) {
+    const int nb = (d.P + WG - 1) / WG;
+    hipLaunchKernelGGL(k_root<R>, dim3(nb), dim3(WG), 0, st, reinterpret_cast<const R*>(root),
+                       catWeights, freqs, cumulative, patternWeights, siteLogL, blockSums, 0, d.P,
+                       d);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(WG), 0, st, blockSums, nb, 1, result, 0);
+}
+
+template <typename R>
+void launch_root_range(const void* root, const double* catWeights, const double* freqs,
+                       const double* cumulative, const double* patternWeights, double* siteLogL,
+                       double* blockSums, double* result, int p0, int p1, const Dims& d,
+                       hipStream_t st) {
+    const int nb = (p1 - p0 + WG - 1) / WG;
+    if (nb <= 0) {
+        launch_fill(result, 0.0, 1, st);
+        return;
+    }
+    hipLaunchKernelGGL(k_root<R>, dim3(nb), dim3(WG), 0, st, reinterpret_cast<const R*>(root),
+                       catWeights, freqs, cumulative, patternWeights, siteLogL, blockSums, p0, p1,
+                       d);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(WG), 0, st, blockSums, nb, 1, result, 0);
+}
+
+// ---------------------------------------------------------------------------
+// edge derivatives: per node and pattern
+//   num = sum_c w_c sum_i pre_i sum_j D_ij post_j,  den = sum_c w_c pre.post
+// A state child uses D[i][state] and pre[state % S] (the reference's
+// indexing, BeagleCPUImpl.hpp:2086-2119).  Outputs ratio per site and
+// workgroup partial sums of w*ratio and w*ratio^2.
+
+template <typename R>
+__global__ __launch_bounds__(WG) void k_edge_deriv(const EdgeJob* __restrict__ jobs,
+                                                   const double* __restrict__ cw,
+                                                   const double* __restrict__ pw,
+                                                   double* __restrict__ sites,
+                                                   double* __restrict__ bsum, int transposedD,
+                                                   Dims d) {
+    __shared__ double sh[WG / 64];
+    const int n = blockIdx.y;
+    const EdgeJob J = jobs[n];
+    const int p = blockIdx.x * WG + threadIdx.x;
+    const int S = d.S, W = S + 1;
+    double r1 = 0.0, r2 = 0.0;
+    if (p < d.P) {
+        const R* pre = reinterpret_cast<const R*>(J.pre);
+        const R* post = reinterpret_cast<const R*>(J.post);
+        const R* D = reinterpret_cast<const R*>(J.D);
+        double num = 0.0, den = 0.0;
+        for (int c = 0; c < d.C; ++c) {
+            const long off = c * d.catStride + (long)p * d.Sp;
+            const R* Dc = D + (long)c * S * W;
+            double nc = 0.0, dc = 0.0;
+            if (J.states) {
+                const int s = J.states[p];
+                for (int i = 0; i < S; ++i) {
+                    double dij;
+                    if (transposedD) dij = s < S ? (double)Dc[s * W + i] : 0.0;
+                    else dij = (double)Dc[i * W + s];
+                    nc += dij * (double)pre[off + i];
+                }
+                dc = (double)pre[off + (s % S)];
+            } else {
+                for (int i = 0; i < S; ++i) {
+                    double t = 0.0;
+                    if (transposedD)
+                        for (int j = 0; j < S; ++j) t += (double)Dc[j * W + i] * (double)post[off + j];
+                    else
+                        for (int j = 0; j < S; ++j) t += (double)Dc[i * W + j] * (double)post[off + j];
+                    nc += t * (double)pre[off + i];
+                    dc += (double)post[off + i] * (double)pre[off + i];
+                }
+            }
+            num += cw[c] * nc;
+            den += cw[c] * dc;
+        }
+        const double r = num / den;
+        if (sites) sites[(long)n * d.P + p] = r;
+        r1 = r * pw[p];
+        r2 = r * r * pw[p];
+    }
+    const int nb = gridDim.x;
+    const double a = block_sum(r1, sh);
+    const double b = block_sum(r2, sh);
+    if (threadIdx.x == 0) {
+        bsum[(long)(2 * n) * nb + blockIdx.x] = a;
+        bsum[(long)(2 * n + 1) * nb + blockIdx.x] = b;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_split_pairs(const double* __restrict__ rows, int njobs,
+                                                    double* sums, double* sumSq) {
+    const int n = blockIdx.x * WG + threadIdx.x;
+    if (n >= njobs) return;
+    if (sums) sums[n] = rows[2 * n];
+    if (sumSq) sumSq[n] = rows[2 * n + 1];
+}
+
+template <typename R>
+void launch_edge_derivatives(const EdgeJob* jobs, int njobs, const double* catWeights,
+                             const double* patternWeights, double* sites, double* blockSums,
+                             double* sums, double* sumSq, int transposedD, const Dims& d,
+                             hipStream_t st) {
+    if (njobs <= 0) return;
+    const int nb = (d.P + WG - 1) / WG;
+    hipLaunchKernelGGL(k_edge_deriv<R>, dim3(nb, njobs), dim3(WG), 0, st, jobs, catWeights,
+                       patternWeights, sites, blockSums, transposedD, d);
+    // rows (2n, 2n+1) hold the per-workgroup sums of node n
+    double* rowsOut = blockSums + (long)2 * njobs * nb;
+    hipLaunchKernelGGL(k_reduce_rows, dim3(2 * njobs), dim3(WG), 0, st, blockSums, nb, 2 * njobs,
+                       rowsOut, 0);
+    hipLaunchKernelGGL(k_split_pairs, dim3((njobs + WG - 1) / WG), dim3(WG), 0, st, rowsOut, njobs,
+                       sums, sumSq);
+}
+
+// ---------------------------------------------------------------------------
+// cross products: out[k*S+j] += sum_n sum_p (w_p/den_p) sum_c w_c r_c t_n
+// pre[c,p,k] post[c,p,j]; a state child is the one-hot vector of its state
+// (all ones when missing), which reproduces the reference's special cases
+// (BeagleCPUImpl.hpp:2122-2237, BeagleCPU4StateImpl.hpp:1088-1254).  Every
+// workgroup writes an S x S partial; a fixed-order column reduction adds
+// them into out.
+
+// 4 states: one lane per pattern walks a subset of the nodes (node group
+// blockIdx.y) keeping the 16 sums in registers.
+constexpr int XP4_GROUPS = 8;
+
+template <typename R>
+__global__ __launch_bounds__(WG) void k_cross4(const EdgeJob* __restrict__ jobs, int njobs,
+                                               const double* __restrict__ cw,
+                                               const double* __restrict__ rates,
+                                               const double* __restrict__ pw,
+                                               double* __restrict__ partial, Dims d) {
+    __shared__ double sh[WG / 64];
+    const int p = blockIdx.x * WG + threadIdx.x;
+    double acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0;
+    if (p < d.P) {
+        for (int n = blockIdx.y; n < njobs; n += gridDim.y) {
+            const EdgeJob J = jobs[n];
+            const R* pre = reinterpret_cast<const R*>(J.pre);
+            const R* post = reinterpret_cast<const R*>(J.post);
+            double tmp[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) tmp[e] = 0.0;
+            double den = 0.0;
+            for (int c = 0; c < d.C; ++c) {
+                const long off = c * d.catStride + (long)p * 4;
+                R a[4], b[4];
+                load4(pre + off, a);
+                if (J.states) {
+                    const int s = J.states[p];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) b[j] = (s >= 4 || s == j) ? (R)1 : (R)0;
+                } else {
+                    load4(post + off, b);
+                }
+                double dd = 0.0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) dd += (double)a[i] * (double)b[i];
+                den += dd * cw[c];
+                const double ws = cw[c] * (rates[c] * J.scale);
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) tmp[k * 4 + j] += (double)a[k] * ((double)b[j] * ws);
+            }
+            const double f = pw[p] / den;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[e] += tmp[e] * f;
+        }
+    }
+    const long row = (long)blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const double v = block_sum(acc[e], sh);
+        if (threadIdx.x == 0) partial[row * 16 + e] = v;
+    }
+}
+
+// any S <= 64: workgroups walk (node, 16-pattern chunk) pairs, staging the
+// chunk through LDS; each lane owns up to 16 of the S x S sums.
+constexpr int XP_CHUNK = 16;
+constexpr int XP_EPT = 16;
+constexpr int XP_BLOCKS = 512;
+
+template <typename R>
+__global__ __launch_bounds__(WG) void k_cross(const EdgeJob* __restrict__ jobs, int njobs,
+                                              const double* __restrict__ cw,
+                                              const double* __restrict__ rates,
+                                              const double* __restrict__ pw,
+                                              double* __restrict__ partial, Dims d) {
+    extern __shared__ double sh[];
+    const int S = d.S, C = d.C, S2 = S * S;
+    double* spre = sh;                              // [XP_CHUNK][C][S]
+    double* spost = spre + XP_CHUNK * C * S;        // [XP_CHUNK][C][S]
+    double* sa = spost + XP_CHUNK * C * S;          // [XP_CHUNK][C]
+    double acc[XP_EPT];
+#pragma unroll
+    for (int t = 0; t < XP_EPT; ++t) acc[t] = 0.0;
+    const int nchunks = (d.P + XP_CHUNK - 1) / XP_CHUNK;
+    const long npairs = (long)njobs * nchunks;
+    for (long pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
+        const int n = (int)(pair / nchunks);
+        const int pbase = (int)(pair % nchunks) * XP_CHUNK;
+        const EdgeJob J = jobs[n];
+        const R* pre = reinterpret_cast<const R*>(J.pre);
+        const R* post = reinterpret_cast<const R*>(J.post);
+        for (int t = threadIdx.x; t < XP_CHUNK * C * S; t += WG) {
+            const int i = t % S;
+            const int c = (t / S) % C;
+            const int q = t / (S * C);
+            const int p = pbase + q;
+            double a = 0.0, b = 0.0;
+            if (p < d.P) {
+                const long off = c * d.catStride + (long)p * d.Sp;
+                a = (double)pre[off + i];
+                if (J.states) {
+                    const int s = J.states[p];
+                    b = (s >= S || s == i) ? 1.0 : 0.0;
+                } else {
+                    b = (double)post[off + i];
+                }
+            }
+            spre[t] = a;
+            spost[t] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x < XP_CHUNK) {
+            const int q = threadIdx.x;
+            const int p = pbase + q;
+            double den = 0.0;
+            for (int c = 0; c < C; ++c) {
+                double dd = 0.0;
+                for (int i = 0; i < S; ++i)
+                    dd += spre[(q * C + c) * S + i] * spost[(q * C + c) * S + i];
+                den += dd * cw[c];
+            }
+            const double f = p < d.P ? pw[p] / den : 0.0;
+            for (int c = 0; c < C; ++c)
+                sa[q * C + c] = p < d.P ? f * (cw[c] * (rates[c] * J.scale)) : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < XP_EPT; ++t) {
+            const int e = threadIdx.x + t * WG;
+            if (e < S2) {
+                const int k = e / S, j = e - k * S;
+                double v = 0.0;
+                for (int q = 0; q < XP_CHUNK; ++q)
+                    for (int c = 0; c < C; ++c) {
+                        const int o = (q * C + c) * S;
+                        v += sa[q * C + c] * spre[o + k] * spost[o + j];
+                    }
+                acc[t] += v;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < XP_EPT; ++t) {
+        const int e = threadIdx.x + t * WG;
+        if (e < S2) partial[(long)blockIdx.x * S2 + e] = acc[t];
+    }
+}
+
+// out[e] += sum over rows of partial[row][e]
+__global__ __launch_bounds__(WG) void k_reduce_cols(const double* __restrict__ partial, int rows,
+                                                    int cols, double* out) {
+    const int e = blockIdx.x * WG + threadIdx.x;
+    if (e >= cols) return;
+    double acc = 0.0;
+    for (int r = 0; r < rows; ++r) acc += partial[(long)r * cols + e];
+    out[e] += acc;
+}
+
+int cross_products_rows(const Dims& d) {
+    if (d.S == 4) return XP4_GROUPS * ((d.P + WG - 1) / WG);
+    return XP_BLOCKS;
+}
+
+template <typename R>
+void launch_cross_products(const EdgeJob* jobs, int njobs, const double* catWeights,
+                           const double* catRates, const double* patternWeights,
+                           double* blockPartials, double* out, const Dims& d, hipStream_t st) {
+    if (njobs <= 0) return;
+    const int cols = d.S * d.S;
+    const int rows = cross_products_rows(d);
+    if (d.S == 4) {
+        const int nb = (d.P + WG - 1) / WG;
+        hipLaunchKernelGGL(k_cross4<R>, dim3(nb, XP4_GROUPS), dim3(WG), 0, st, jobs, njobs,
+                           catWeights, catRates, patternWeights, blockPartials, d);
+    } else {
+        const size_t shm = (size_t)(2 * XP_CHUNK * d.C * d.S + XP_CHUNK * d.C) * sizeof(double);
+        hipLaunchKernelGGL(k_cross<R>, dim3(XP_BLOCKS), dim3(WG), shm, st, jobs, njobs, catWeights,
+                           catRates, patternWeights, blockPartials, d);
+    }
+    hipLaunchKernelGGL(k_reduce_cols, dim3((cols + WG - 1) / WG), dim3(WG), 0, st, blockPartials,
+                       rows, cols, out);
+}
+
+// ---------------------------------------------------------------------------
+// edge log-likelihood with optional first/second derivatives along the edge
+// (BeagleCPUImpl.hpp:3209-3322, 3868-3994).
+
+template <typename R>
+__global__ __launch_bounds__(WG) void k_edge_ll(const R* __restrict__ par, const R* child,
+                                                const int* states, const R* Pm, const R* D1,
+                                                const R* D2, const double* __restrict__ cw,
+                                                const double* __restrict__ f,
+                                                const double* cumA, const double* cumB,
+                                                const double* __restrict__ pw, double* siteL,
+                                                double* siteD1, double* siteD2,
+                                                double* __restrict__ bsum, Dims d) {
+    __shared__ double sh[WG / 64];
+    const int p = blockIdx.x * WG + threadIdx.x;
+    const int S = d.S, W = S + 1;
+    double cl = 0.0, c1 = 0.0, c2 = 0.0;
+    if (p < d.P) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        const R* Ms[3] = {Pm, D1, D2};
+        for (int m = 0; m < 3; ++m) {
+            if (!Ms[m]) continue;
+            for (int c = 0; c < d.C; ++c) {
+                const long off = c * d.catStride + (long)p * d.Sp;
+                const R* Mc = Ms[m] + (long)c * S * W;
+                double s = 0.0;
+                for (int i = 0; i < S; ++i) {
+                    double t;
+                    if (states) {
+                        t = (double)Mc[i * W + states[p]];
+                    } else {
+                        t = 0.0;
+                        for (int j = 0; j < S; ++j) t += (double)Mc[i * W + j] * (double)child[off + j];
+                    }
+                    s += f[i] * (double)par[off + i] * t;
+                }
+                acc[m] += cw[c] * s;
+            }
+        }
+        double L = log(acc[0]);
+        if (cumA) L += cumA[p];
+        if (cumB) L += cumB[p];
+        siteL[p] = L;
+        cl = L * pw[p];
+        if (D1) {
+            const double g = acc[1] / acc[0];
+            siteD1[p] = g;
+            c1 = g * pw[p];
+            if (D2) {
+                const double h = acc[2] / acc[0] - g * g;
+                siteD2[p] = h;
+                c2 = h * pw[p];
+            }
+        }
+    }
+    const int nb = gridDim.x;
+    const double a = block_sum(cl, sh);
+    const double b = block_sum(c1, sh);
+    const double e = block_sum(c2, sh);
+    if (threadIdx.x == 0) {
+        bsum[blockIdx.x] = a;
+        bsum[nb + blockIdx.x] = b;
+        bsum[2 * nb + blockIdx.x] = e;
+    }
+}
+
+template <typename R>
+void launch_edge_loglik(const void* parent, const void* child, const int* childStates,
+                        const void* P, const void* D1, const void* D2,
+                        const double* catWeights, const double* freqs,
+                        const double* cumA, const double* cumB,
+                        const double* patternWeights, double* siteL, double* siteD1,
+                        double* siteD2, double* blockSums, double* result, const Dims& d,
+                        hipStream_t st) {
+    const int nb = (d.P + WG - 1) / WG;
+    hipLaunchKernelGGL(k_edge_ll<R>, dim3(nb), dim3(WG), 0, st, reinterpret_cast<const R*>(parent),
+                       reinterpret_cast<const R*>(child), childStates,
+                       reinterpret_cast<const R*>(P), reinterpret_cast<const R*>(D1),
+                       reinterpret_cast<const R*>(D2), catWeights, freqs, cumA, cumB,
+                       patternWeights, siteL, siteD1, siteD2, blockSums, d);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(3), dim3(WG), 0, st, blockSums, nb, 3, result, 0);
+}
+
+// ---------------------------------------------------------------------------
+// scale-buffer bookkeeping (BeagleCPUImpl.hpp:2763-2908 semantics)
+
+__global__ __launch_bounds__(WG) void k_accumulate(double* cum, const double* const* bufs,
+                                                   int nbufs, double sign, int logScalers, int p0,
+                                                   int p1) {
+    const int p = p0 + blockIdx.x * WG + threadIdx.x;
+    if (p >= p1) return;
+    double v = cum[p];
+    for (int b = 0; b < nbufs; ++b) {
+        const double s = bufs[b][p];
+        v += sign * (logScalers ? s : log(s));
+    }
+    cum[p] = v;
+}
+
+void launch_accumulate(double* cum, const double* const* bufs, int nbufs, double sign,
+                       int logScalers, int p0, int p1, hipStream_t st) {
+    if (p1 <= p0) return;
+    hipLaunchKernelGGL(k_accumulate, dim3((p1 - p0 + WG - 1) / WG), dim3(WG), 0, st, cum, bufs,
+                       nbufs, sign, logScalers, p0, p1);
+}
+
+__global__ __launch_bounds__(WG) void k_fill(double* dst, double v, long n) {
+    const long i = (long)blockIdx.x * WG + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+void launch_fill(double* dst, double v, long n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill, dim3((n + WG - 1) / WG), dim3(WG), 0, st, dst, v, n);
+}
+
+template <typename R>
+__global__ __launch_bounds__(WG) void k_pad_set(R* dst, long n, R v) {
+    const long i = (long)blockIdx.x * WG + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+template <typename R>
+void launch_pad_set(void* dst, long n, R v, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pad_set<R>, dim3((n + WG - 1) / WG), dim3(WG), 0, st,
+                       reinterpret_cast<R*>(dst), n, v);
+}
+
+// Per category: dst[j][i] = src[i][j] for the S x S block; each row keeps
+// its own pad entry.  Staged through LDS so src == dst is allowed.
+template <typename R>
+__global__ __launch_bounds__(WG) void k_transpose(const void* const* src, void* const* dst, int S) {
+    extern __shared__ double sh[];
+    const int m = blockIdx.x, c = blockIdx.y;
+    const int W = S + 1;
+    const R* a = reinterpret_cast<const R*>(src[m]) + (long)c * S * W;
+    R* b = reinterpret_cast<R*>(dst[m]) + (long)c * S * W;
+    for (int t = threadIdx.x; t < S * W; t += WG) sh[t] = (double)a[t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < S * W; t += WG) {
+        const int i = t / W, j = t - i * W;
+        b[t] = (R)(j == S ? sh[i * W + S] : sh[j * W + i]);
+    }
+}
+
+template <typename R>
+void launch_transpose(const void* const* src, void* const* dst, int count, int S, int C,
+                      hipStream_t st) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_transpose<R>, dim3(count, C), dim3(WG), (size_t)S * (S + 1) * sizeof(double),
+                       st, src, dst, S);
+}
+
+// ---------------------------------------------------------------------------
+// explicit instantiations
+
+#define INSTANTIATE(R)                                                                          \
+    template void launch_matrices<R>(const MatrixJob*, int, int, int, hipStream_t);             \
+    template void launch_post<R>(const PostJob*, const PostJob*, int, const Dims&, hipStream_t);  \
+    template void launch_pre<R>(const PreJob*, const PreJob*, int, const Dims&, int, void*,      \
+                                hipStream_t);                                                    \
+    template void launch_root<R>(const void*, const double*, const double*, const double*,       \
+                                 const double*, double*, double*, double*, const Dims&,          \
+                                 hipStream_t);                                                   \
+    template void launch_root_range<R>(const void*, const double*, const double*, const double*, \
+                                       const double*, double*, double*, double*, int, int,       \
+                                       const Dims&, hipStream_t);                                \
+    template void launch_edge_derivatives<R>(const EdgeJob*, int, const double*, const double*,  \
+                                             double*, double*, double*, double*, int,            \
+                                             const Dims&, hipStream_t);                          \
+    template void launch_cross_products<R>(const EdgeJob*, int, const double*, const double*,    \
+                                           const double*, double*, double*, const Dims&,         \
+                                           hipStream_t);                                         \
+    template void launch_edge_loglik<R>(const void*, const void*, const int*, const void*,       \
+                                        const void*, const void*, const double*, const double*,  \
+                                        const double*, const double*, const double*, double*,    \
+                                        double*, double*, double*, double*, const Dims&,         \
+                                        hipStream_t);                                            \
+    template void launch_transpose<R>(const void* const*, void* const*, int, int, int,           \
+                                      hipStream_t);                                              \
+    template void launch_pad_set<R>(void*, long, R, hipStream_t);
+
+INSTANTIATE(double)
+INSTANTIATE(float)
+
+}  // namespace hipbeagle
