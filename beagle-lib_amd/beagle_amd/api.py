@@ -140,6 +140,8 @@ SIGNATURES = {
     "GetDerivatives": (_I, [_I, _DP, _DP]),
     "GetSiteLogLikelihoods": (_I, [_I, _DP]),
     "GetSiteDerivatives": (_I, [_I, _DP, _DP]),
+    # extensions (include/beagle_amd_ext.h)
+    "ExtGetStream": (_I, [_I, ctypes.POINTER(ctypes.c_void_p)]),
 }
 
 
@@ -420,6 +422,12 @@ class Instance:
                 e2[1] if e2 else None, w[1], f[1], s[1], 1, ctypes.byref(out), ctypes.byref(o1),
                 ctypes.byref(o2))
         return out.value, o1.value, o2.value
+
+    def stream(self):
+        """hipStream_t of the instance (extension entry point)."""
+        s = ctypes.c_void_p(0)
+        self._c("ExtGetStream", ctypes.byref(s))
+        return s.value
 
     def site_derivatives(self):
         d1 = np.zeros(self.P)

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "beagle.h"
+#include "beagle_amd_ext.h"
 #include "instance.h"
 
 using hipbeagle::Instance;
@@ -43,6 +44,7 @@ bool gResInit = false;
 const long kGpuSupport =
     BEAGLE_FLAG_COMPUTATION_SYNCH | BEAGLE_FLAG_COMPUTATION_ASYNCH | BEAGLE_FLAG_PRECISION_SINGLE |
     BEAGLE_FLAG_PRECISION_DOUBLE | BEAGLE_FLAG_SCALING_MANUAL | BEAGLE_FLAG_SCALING_ALWAYS |
+    BEAGLE_FLAG_SCALING_AUTO |
     BEAGLE_FLAG_THREADING_NONE | BEAGLE_FLAG_VECTOR_NONE | BEAGLE_FLAG_PROCESSOR_GPU |
     BEAGLE_FLAG_SCALERS_LOG | BEAGLE_FLAG_SCALERS_RAW | BEAGLE_FLAG_EIGEN_REAL |
     BEAGLE_FLAG_INVEVEC_STANDARD | BEAGLE_FLAG_INVEVEC_TRANSPOSED |
@@ -201,7 +203,7 @@ int beagleCreateInstance(int tipCount, int partialsBufferCount, int compactBuffe
         int err = BEAGLE_ERROR_NO_RESOURCE;
         for (const Cand& c : cands) {
             const long both = preferenceFlags | requirementFlags;
-            if (both & (BEAGLE_FLAG_SCALING_AUTO | BEAGLE_FLAG_SCALING_DYNAMIC | BEAGLE_FLAG_EIGEN_COMPLEX)) {
+            if (both & (BEAGLE_FLAG_SCALING_DYNAMIC | BEAGLE_FLAG_EIGEN_COMPLEX)) {
                 err = BEAGLE_ERROR_NO_IMPLEMENTATION;
                 continue;
             }
@@ -580,6 +582,13 @@ int beagleGetSiteLogLikelihoods(int instance, double* out) {
 
 int beagleGetSiteDerivatives(int instance, double* d1, double* d2) {
     return withInstance(instance, [&](Instance* in) { return in->getSiteDerivatives(d1, d2); });
+}
+
+int beagleExtGetStream(int instance, void** outStream) {
+    return withInstance(instance, [&](Instance* in) {
+        *outStream = reinterpret_cast<void*>(in->stream());
+        return BEAGLE_SUCCESS;
+    });
 }
 
 }  // extern "C"

@@ -34,8 +34,12 @@ int roundUp(int x, int m) { return (x + m - 1) / m * m; }
 
 // Scaling-mode precedence follows the reference CPU implementation:
 // AUTO > ALWAYS > DYNAMIC > SCALERS_LOG (manual) > manual raw
-// (libhmsbeagle/CPU/BeagleCPUImpl.hpp:294-311).  AUTO and DYNAMIC are
-// refused by the factory before this constructor runs.
+// (libhmsbeagle/CPU/BeagleCPUImpl.hpp:294-311).  DYNAMIC is refused by the
+// factory before this constructor runs.  AUTO is this plugin's own exact
+// scheme: every operation divides its output by a power of two per pattern
+// and stores the exponent with the buffer; exponents add up the tree, so
+// the root (and any edge) log-likelihood adds ln2 * exponent and no client
+// scale-buffer bookkeeping is needed.
 Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount, int stateCount,
                    int patternCount, int eigenBufferCount, int matrixBufferCount,
                    int categoryCount, int scaleBufferCount, int device, int resourceNumber,
@@ -63,7 +67,11 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
                                                       : BEAGLE_FLAG_INVEVEC_STANDARD;
     const int internal = bufferCount_ - tipCount_;
     always_ = false;
-    if (both & BEAGLE_FLAG_SCALING_ALWAYS) {
+    auto_ = false;
+    if (both & BEAGLE_FLAG_SCALING_AUTO) {
+        auto_ = true;
+        flags_ |= BEAGLE_FLAG_SCALING_AUTO | BEAGLE_FLAG_SCALERS_LOG;
+    } else if (both & BEAGLE_FLAG_SCALING_ALWAYS) {
         always_ = true;
         flags_ |= BEAGLE_FLAG_SCALING_ALWAYS | BEAGLE_FLAG_SCALERS_LOG;
         scaleCount_ = internal + 1;
@@ -87,12 +95,14 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
     dims_.Sp = stateCount;
     dims_.Ppad = roundUp(patternCount, 64);
     dims_.catStride = (long)dims_.Ppad * dims_.Sp;
+    dims_.autoScale = auto_ ? 1 : 0;
     partialsBytes_ = (size_t)categoryCount * dims_.catStride * realSize();
     matrixBytes_ = (size_t)categoryCount * stateCount * (stateCount + 1) * realSize();
     matrixBytes_ = (matrixBytes_ + 255) / 256 * 256;
 
     HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     dPartials_.assign(bufferCount_, nullptr);
+    dExp_.assign(bufferCount_, nullptr);
     dStates_.assign(bufferCount_, nullptr);
     for (int b = tipCount_; b < bufferCount_; ++b) ensurePartials(b);
     dMatrices_ = dalloc<char>(matrixBytes_ * matrixCount_);
@@ -127,6 +137,7 @@ Instance::~Instance() {
     hipSetDevice(device_);
     if (stream_) hipStreamSynchronize(stream_);
     for (void* p : dPartials_) hipFree(p);
+    for (short* p : dExp_) hipFree(p);
     for (int* p : dStates_) hipFree(p);
     hipFree(dMatrices_);
     hipFree(dScale_);
@@ -231,6 +242,16 @@ void Instance::ensurePartials(int b) {
     HIPCHECK(hipMalloc(&p, partialsBytes_));
     HIPCHECK(hipMemsetAsync(p, 0, partialsBytes_, stream_));
     dPartials_[b] = p;
+    if (auto_) {
+        short* e = nullptr;
+        HIPCHECK(hipMalloc((void**)&e, dims_.Ppad * sizeof(short)));
+        HIPCHECK(hipMemsetAsync(e, 0, dims_.Ppad * sizeof(short), stream_));
+        dExp_[b] = e;
+    }
+}
+
+void Instance::clearExponent(int b) {
+    if (auto_ && dExp_[b]) HIPCHECK(hipMemsetAsync(dExp_[b], 0, dims_.Ppad * sizeof(short), stream_));
 }
 
 // ---- data in and out ---------------------------------------------------------
@@ -260,6 +281,7 @@ int Instance::setTipPartials(int tip, const double* in) {
             for (int i = 0; i < dims_.S; ++i)
                 tmp[c * dims_.catStride + (size_t)k * dims_.Sp + i] = in[(size_t)k * dims_.S + i];
     toDevice(dPartials_[tip], tmp.data(), n);
+    clearExponent(tip);
     return BEAGLE_SUCCESS;
 }
 
@@ -274,6 +296,7 @@ int Instance::setPartials(int b, const double* in) {
                 tmp[c * dims_.catStride + (size_t)k * dims_.Sp + i] =
                     in[((size_t)c * dims_.P + k) * dims_.S + i];
     toDevice(dPartials_[b], tmp.data(), n);
+    clearExponent(b);
     return BEAGLE_SUCCESS;
 }
 
@@ -553,10 +576,14 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
         J.scaleRead = nullptr;
         J.cumulative = nullptr;
         J.accA = J.accB = nullptr;
+        J.e1 = dStates_[c1] ? nullptr : dExp_[c1];
+        J.e2 = dStates_[c2] ? nullptr : dExp_[c2];
+        J.ed = dExp_[dest];
         J.mode = SCALE_NONE;
         J.logScalers = logScalers_ ? 1 : 0;
         J.p0 = p0;
         J.p1 = p1;
+        if (auto_) continue;  // every op rescales by powers of two
         if (cum != BEAGLE_OP_NONE && !validScale(cum)) return BEAGLE_ERROR_OUT_OF_RANGE;
         if (always_) {
             J.mode = SCALE_WRITE;
@@ -615,10 +642,14 @@ int Instance::updatePrePartials(const int* ops, int count, int cumulativeIndex, 
         J.dest = dPartials_[dest];
         J.scaleWrite = nullptr;
         J.accA = J.accB = nullptr;
+        J.e1 = dExp_[par];
+        J.e2 = dStates_[sib] ? nullptr : dExp_[sib];
+        J.ed = dExp_[dest];
         J.mode = SCALE_NONE;
         J.logScalers = logScalers_ ? 1 : 0;
         J.p0 = p0;
         J.p1 = p1;
+        if (auto_) continue;
         if (always_) {
             if (!validScale(dest - tipCount_)) return BEAGLE_ERROR_OUT_OF_RANGE;
             J.mode = SCALE_WRITE;
@@ -655,6 +686,7 @@ int Instance::waitForPartials() {
 // cumulative[p] += sign * (LOG ? s : log s) over the listed buffers, for all
 // patterns or one partition (BeagleCPUImpl.hpp:2763-2860).
 int Instance::accumulateScaleFactors(const int* idx, int count, int cumIdx, double sign, int part) {
+    if (auto_) return BEAGLE_SUCCESS;  // exponents already propagate with the partials
     if (!validScale(cumIdx)) return BEAGLE_ERROR_OUT_OF_RANGE;
     int p0 = 0, p1 = dims_.P;
     if (part >= 0) {
@@ -677,6 +709,7 @@ int Instance::accumulateScaleFactors(const int* idx, int count, int cumIdx, doub
 // Cleared with a fill kernel, not a memset node, so the call stays
 // capturable into a HIP graph.
 int Instance::resetScaleFactors(int cumIdx, int part) {
+    if (auto_) return BEAGLE_SUCCESS;
     if (!validScale(cumIdx)) return BEAGLE_ERROR_OUT_OF_RANGE;
     int p0 = 0, p1 = dims_.Ppad;
     if (part >= 0) {
@@ -712,16 +745,17 @@ int Instance::calculateRootLogLikelihoods(const int* bufIdx, const int* wIdx, co
     if (!validBuffer(b) || !dPartials_[b] || !validEigen(w) || !validEigen(f) ||
         hCatWeights_[w].empty() || hFreqs_[f].empty())
         return BEAGLE_ERROR_OUT_OF_RANGE;
-    int cs = always_ ? b - tipCount_ : cumIdx[0];
+    int cs = always_ ? b - tipCount_ : (auto_ ? BEAGLE_OP_NONE : cumIdx[0]);
     if (cs != BEAGLE_OP_NONE && !validScale(cs)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    const short* ex = auto_ ? dExp_[b] : nullptr;
     const int nb = (dims_.P + 255) / 256;
     double* work = reinterpret_cast<double*>(grow(dWork_, workCap_, (size_t)nb * sizeof(double)));
     if (dp_)
-        launch_root<double>(dPartials_[b], dCatWeights_ + (size_t)w * dims_.C,
+        launch_root<double>(dPartials_[b], ex, dCatWeights_ + (size_t)w * dims_.C,
                             dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
                             dPatternWeights_, dSiteL_, work, dResult_, dims_, stream_);
     else
-        launch_root<float>(dPartials_[b], dCatWeights_ + (size_t)w * dims_.C,
+        launch_root<float>(dPartials_[b], ex, dCatWeights_ + (size_t)w * dims_.C,
                            dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
                            dPatternWeights_, dSiteL_, work, dResult_, dims_, stream_);
     HIPCHECK(hipGetLastError());
@@ -742,18 +776,19 @@ int Instance::calculateRootLogLikelihoodsByPartition(const int* bufIdx, const in
     const int nbMax = (dims_.P + 255) / 256;
     double* work = reinterpret_cast<double*>(grow(dWork_, workCap_, (size_t)nbMax * sizeof(double)));
     for (int q = 0; q < partitionCount; ++q) {
-        const int part = partIdx[q], b = bufIdx[q], w = wIdx[q], f = fIdx[q], cs = cumIdx[q];
+        const int part = partIdx[q], b = bufIdx[q], w = wIdx[q], f = fIdx[q];
+        const int cs = auto_ ? BEAGLE_OP_NONE : cumIdx[q];
         if (part < 0 || part >= partitionCount_ || !validBuffer(b) || !dPartials_[b] ||
             !validEigen(w) || !validEigen(f) || (cs != BEAGLE_OP_NONE && !validScale(cs)))
             return BEAGLE_ERROR_OUT_OF_RANGE;
         const int p0 = partStart_[part], p1 = partStart_[part + 1];
         if (dp_)
-            launch_root_range<double>(dPartials_[b], dCatWeights_ + (size_t)w * dims_.C,
+            launch_root_range<double>(dPartials_[b], auto_ ? dExp_[b] : nullptr, dCatWeights_ + (size_t)w * dims_.C,
                                       dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
                                       dPatternWeights_, dSiteL_, work, dResult_ + q, p0, p1, dims_,
                                       stream_);
         else
-            launch_root_range<float>(dPartials_[b], dCatWeights_ + (size_t)w * dims_.C,
+            launch_root_range<float>(dPartials_[b], auto_ ? dExp_[b] : nullptr, dCatWeights_ + (size_t)w * dims_.C,
                                      dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
                                      dPatternWeights_, dSiteL_, work, dResult_ + q, p0, p1, dims_,
                                      stream_);
@@ -886,22 +921,26 @@ int Instance::calculateEdgeLogLikelihoods(const int* parIdx, const int* childIdx
     if (always_) {
         if (par >= tipCount_) cumA = scale(par - tipCount_);
         if (ch >= tipCount_) cumB = scale(ch - tipCount_);
-    } else if (cumIdx[0] != BEAGLE_OP_NONE) {
+    } else if (!auto_ && cumIdx[0] != BEAGLE_OP_NONE) {
         if (!validScale(cumIdx[0])) return BEAGLE_ERROR_OUT_OF_RANGE;
         cumA = scale(cumIdx[0]);
     }
+    const short* exA = auto_ ? dExp_[par] : nullptr;
+    const short* exB = (auto_ && !dStates_[ch]) ? dExp_[ch] : nullptr;
     const int nb = (dims_.P + 255) / 256;
     double* wk = reinterpret_cast<double*>(grow(dWork_, workCap_, (size_t)3 * nb * sizeof(double)));
     if (dp_)
         launch_edge_loglik<double>(dPartials_[par], dPartials_[ch], dStates_[ch], matrix(pm), D1,
                                    D2, dCatWeights_ + (size_t)w * dims_.C,
-                                   dFreqs_ + (size_t)f * dims_.S, cumA, cumB, dPatternWeights_,
-                                   dSiteL_, dSiteD1_, dSiteD2_, wk, dResult_, dims_, stream_);
+                                   dFreqs_ + (size_t)f * dims_.S, cumA, cumB, exA, exB,
+                                   dPatternWeights_, dSiteL_, dSiteD1_, dSiteD2_, wk, dResult_,
+                                   dims_, stream_);
     else
         launch_edge_loglik<float>(dPartials_[par], dPartials_[ch], dStates_[ch], matrix(pm), D1,
                                   D2, dCatWeights_ + (size_t)w * dims_.C,
-                                  dFreqs_ + (size_t)f * dims_.S, cumA, cumB, dPatternWeights_,
-                                  dSiteL_, dSiteD1_, dSiteD2_, wk, dResult_, dims_, stream_);
+                                  dFreqs_ + (size_t)f * dims_.S, cumA, cumB, exA, exB,
+                                  dPatternWeights_, dSiteL_, dSiteD1_, dSiteD2_, wk, dResult_,
+                                  dims_, stream_);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(hResult_, dResult_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_));
     sync();

@@ -105,6 +105,7 @@ private:
     void* upload(const void* data, size_t bytes);
     void* grow(void*& buf, size_t& cap, size_t bytes);
     void ensurePartials(int b);
+    void clearExponent(int b);
     void* partials(int b) const { return dPartials_[b]; }
     void* matrix(int m) const { return (char*)dMatrices_ + (size_t)m * matrixBytes_; }
     double* scale(int s) const { return dScale_ + (size_t)s * dims_.Ppad; }
@@ -126,6 +127,7 @@ private:
     bool dp_;
     bool preTransposed_;   // S > 4 without PREORDER_TRANSPOSE_AUTO
     bool always_;          // SCALING_ALWAYS
+    bool auto_;            // SCALING_AUTO: power-of-two exponents per buffer
     bool logScalers_;
     Dims dims_;
     size_t partialsBytes_, matrixBytes_;
@@ -133,6 +135,7 @@ private:
 
     std::vector<void*> dPartials_;
     std::vector<int*> dStates_;
+    std::vector<short*> dExp_;        // SCALING_AUTO exponents per buffer
     void* dMatrices_ = nullptr;
     double* dScale_ = nullptr;
     double* dCube_ = nullptr;

@@ -28,6 +28,9 @@ struct PostJob {
     double* cumulative;        // += log(max) when rescaling (may be null)
     const double* accA;        // scaling-always: child node scale buffers
     const double* accB;        //   added into scaleWrite (may be null)
+    const short* e1;           // automatic scaling: children's and the
+    const short* e2;           //   destination's per-pattern power-of-two
+    short* ed;                 //   exponents (null for tips)
     int mode;
     int logScalers;            // store log(max) rather than max
     int p0, p1;                // pattern range (a partition, or [0, P))
@@ -41,6 +44,9 @@ struct PreJob {
     double* scaleWrite;
     const double* accA;        // scaling-always accumulation (may be null)
     const double* accB;
+    const short* e1;           // automatic scaling exponents: parent, sibling,
+    const short* e2;           //   destination
+    short* ed;
     int mode;
     int logScalers;
     int p0, p1;
@@ -71,6 +77,7 @@ struct Dims {
     int Ppad;     // padded pattern stride
     int Sp;       // state stride in partials
     long catStride;  // Ppad * Sp
+    int autoScale;   // automatic power-of-two rescaling of every operation
 };
 
 // ---- launchers (kernels.hip) ----
@@ -85,14 +92,15 @@ template <typename R>
 void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims& d,
                 int transposedSelf, void* scratch, hipStream_t st);
 template <typename R>
-void launch_root_range(const void* root, const double* catWeights, const double* freqs,
-                       const double* cumulative, const double* patternWeights, double* siteLogL,
-                       double* blockSums, double* result, int p0, int p1, const Dims& d,
-                       hipStream_t st);
+void launch_root_range(const void* root, const short* rootExp, const double* catWeights,
+                       const double* freqs, const double* cumulative,
+                       const double* patternWeights, double* siteLogL, double* blockSums,
+                       double* result, int p0, int p1, const Dims& d, hipStream_t st);
 template <typename R>
-void launch_root(const void* root, const double* catWeights, const double* freqs,
-                 const double* cumulative, const double* patternWeights, double* siteLogL,
-                 double* blockSums, double* result, const Dims& d, hipStream_t st);
+void launch_root(const void* root, const short* rootExp, const double* catWeights,
+                 const double* freqs, const double* cumulative, const double* patternWeights,
+                 double* siteLogL, double* blockSums, double* result, const Dims& d,
+                 hipStream_t st);
 template <typename R>
 void launch_edge_derivatives(const EdgeJob* jobs, int njobs, const double* catWeights,
                              const double* patternWeights, double* sites, double* blockSums,
@@ -106,10 +114,10 @@ template <typename R>
 void launch_edge_loglik(const void* parent, const void* child, const int* childStates,
                         const void* P, const void* D1, const void* D2,
                         const double* catWeights, const double* freqs,
-                        const double* cumA, const double* cumB,
-                        const double* patternWeights, double* siteL, double* siteD1,
-                        double* siteD2, double* blockSums, double* result, const Dims& d,
-                        hipStream_t st);
+                        const double* cumA, const double* cumB, const short* exA,
+                        const short* exB, const double* patternWeights, double* siteL,
+                        double* siteD1, double* siteD2, double* blockSums, double* result,
+                        const Dims& d, hipStream_t st);
 
 // Rows of S*S partial sums the cross-product launch writes into its workspace.
 int cross_products_rows(const Dims& d);

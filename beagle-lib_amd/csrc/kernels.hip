@@ -168,111 +168,148 @@ __device__ __forceinline__ void child4(const R* x, const int* s, const R* m, lon
     }
 }
 
-// Per-pattern rescale of all C x 4 values held by the lane: divide by the
-// maximum (0 -> 1), store max or log(max), add log(max) to the cumulative
-// buffer, and (scaling-always) add the children's node factors.
-template <typename R, int CMAX>
-__device__ __forceinline__ void rescale_lane(R (&v)[CMAX][4], R mx, int C, int p,
-                                             double* scaleWrite, int logScalers,
-                                             double* cumulative, const double* accA,
-                                             const double* accB) {
-    if (mx == (R)0) mx = (R)1;
-    const R inv = (R)1 / mx;
-#pragma unroll
-    for (int c = 0; c < CMAX; ++c)
-        if (c < C) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) v[c][i] *= inv;
-        }
-    const double lm = log((double)mx);
-    double w = logScalers ? lm : (double)mx;
-    if (cumulative) cumulative[p] += lm;
-    if (accA) w += accA[p];
-    if (accB) w += accB[p];
-    scaleWrite[p] = w;
+// Workgroup layout of the 4-state traversals: C waves over one tile of 64
+// patterns; wave w owns category w, lane l owns pattern tile*64+l.  A
+// wave's matrices are wave-uniform (scalar loads), its partials loads and
+// stores are 64 consecutive 16/32-byte vectors.  The per-pattern maximum
+// over categories that rescaling needs is exchanged through LDS; only
+// rescaled operations pay that barrier.
+constexpr int TILE = 64;
+
+// Cross-category per-pattern maximum; `k` counts rescaled operations so
+// consecutive exchanges alternate LDS buffers and one barrier suffices.
+template <typename R>
+__device__ __forceinline__ R tile_max(R m, R (*smax)[8][TILE], int& k, int c, int lane, int C) {
+    R (*buf)[TILE] = smax[k & 1];
+    ++k;
+    buf[c][lane] = m;
+    __syncthreads();
+    R mx = buf[0][lane];
+    for (int cc = 1; cc < C; ++cc) mx = fmax(mx, buf[cc][lane]);
+    return mx;
 }
 
-template <typename R, int CMAX>
-__global__ __launch_bounds__(WG) void k_post4(const PostJob* __restrict__ jobs, int njobs, int P,
-                                              int C, long cs) {
-    const int p = blockIdx.x * WG + threadIdx.x;
-    if (p >= P) return;
-    for (int o = 0; o < njobs; ++o) {
-        const PostJob J = jobs[o];
-        if (p < J.p0 || p >= J.p1) continue;
-        const R* x1 = reinterpret_cast<const R*>(J.x1);
-        const R* x2 = reinterpret_cast<const R*>(J.x2);
-        const R* m1 = reinterpret_cast<const R*>(J.m1);
-        const R* m2 = reinterpret_cast<const R*>(J.m2);
-        R* d = reinterpret_cast<R*>(J.dest);
-        const R sr = J.mode == SCALE_READ ? (R)J.scaleRead[p] : (R)1;
-        R v[CMAX][4];
-        R mx = 0;
+__device__ __forceinline__ int exponent_of(double x) {
+    int e;
+    frexp(x, &e);
+    return e;
+}
+__device__ __forceinline__ int exponent_of(float x) {
+    int e;
+    frexpf(x, &e);
+    return e;
+}
+__device__ __forceinline__ double pow2(double x, int e) { return ldexp(x, e); }
+__device__ __forceinline__ float pow2(float x, int e) { return ldexpf(x, e); }
+
+// Rescale one lane's four values given the tile maximum.  Manual modes
+// (reference semantics): divide by max (0 -> 1), wave 0 stores max or
+// log(max), adds log(max) to the cumulative buffer and, under
+// scaling-always, the children's node factors.  Automatic mode: multiply
+// by 2^-e where max = f 2^e, f in [0.5,1) (exact), and wave 0 stores the
+// node's exponent e + e(child1) + e(child2).
+template <typename R, bool AUTO>
+__device__ __forceinline__ void rescale4(R (&v)[4], R mx, int c, int p, double* scaleWrite,
+                                         int logScalers, double* cumulative, const double* accA,
+                                         const double* accB, const short* e1, const short* e2,
+                                         short* ed) {
+    if (AUTO) {
+        const int e = mx > (R)0 ? exponent_of(mx) : 0;
 #pragma unroll
-        for (int c = 0; c < CMAX; ++c) {
-            if (c < C) {
-                const long off = c * cs + (long)p * 4;
-                R a[4], b[4];
-                child4(x1, J.s1, m1, off, p, c, a);
-                child4(x2, J.s2, m2, off, p, c, b);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    R t = a[i] * b[i];
-                    if (J.mode == SCALE_READ) t = t / sr;
-                    v[c][i] = t;
-                    mx = fmax(mx, t);
-                }
-            }
+        for (int i = 0; i < 4; ++i) v[i] = pow2(v[i], -e);
+        if (c == 0) {
+            int t = e;
+            if (e1) t += e1[p];
+            if (e2) t += e2[p];
+            ed[p] = (short)t;
         }
-        if (J.mode == SCALE_WRITE)
-            rescale_lane<R, CMAX>(v, mx, C, p, J.scaleWrite, J.logScalers, J.cumulative, J.accA,
-                                  J.accB);
+    } else {
+        if (mx == (R)0) mx = (R)1;
+        const R inv = (R)1 / mx;
 #pragma unroll
-        for (int c = 0; c < CMAX; ++c)
-            if (c < C) store4(d + c * cs + (long)p * 4, v[c]);
+        for (int i = 0; i < 4; ++i) v[i] *= inv;
+        if (c == 0) {
+            const double lm = log((double)mx);
+            double w = logScalers ? lm : (double)mx;
+            if (cumulative) cumulative[p] += lm;
+            if (accA) w += accA[p];
+            if (accB) w += accB[p];
+            scaleWrite[p] = w;
+        }
     }
 }
 
-template <typename R, int CMAX>
-__global__ __launch_bounds__(WG) void k_pre4(const PreJob* __restrict__ jobs, int njobs, int P,
-                                             int C, long cs) {
-    const int p = blockIdx.x * WG + threadIdx.x;
-    if (p >= P) return;
+template <typename R, bool AUTO>
+__global__ __launch_bounds__(512) void k_post4(const PostJob* __restrict__ jobs, int njobs, int P,
+                                               int C, long cs) {
+    __shared__ R smax[2][8][TILE];
+    const int c = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * TILE + lane;
+    const long off = c * cs + (long)p * 4;
+    int k = 0;
     for (int o = 0; o < njobs; ++o) {
-        const PreJob J = jobs[o];
-        if (p < J.p0 || p >= J.p1) continue;
-        const R* pp = reinterpret_cast<const R*>(J.parent);
-        const R* ms = reinterpret_cast<const R*>(J.mSelf);
-        const R* xs = reinterpret_cast<const R*>(J.xs);
-        const R* mb = reinterpret_cast<const R*>(J.mSib);
-        R* d = reinterpret_cast<R*>(J.dest);
-        R v[CMAX][4];
-        R mx = 0;
+        const PostJob J = jobs[o];
+        const bool in = p < P && p >= J.p0 && p < J.p1;
+        R v[4] = {0, 0, 0, 0};
+        if (in) {
+            R a[4], b[4];
+            child4(reinterpret_cast<const R*>(J.x1), J.s1, reinterpret_cast<const R*>(J.m1), off,
+                   p, c, a);
+            child4(reinterpret_cast<const R*>(J.x2), J.s2, reinterpret_cast<const R*>(J.m2), off,
+                   p, c, b);
+            const R sr = J.mode == SCALE_READ ? (R)J.scaleRead[p] : (R)1;
 #pragma unroll
-        for (int c = 0; c < CMAX; ++c) {
-            if (c < C) {
-                const long off = c * cs + (long)p * 4;
-                R b[4], u[4];
-                child4(xs, J.ss, mb, off, p, c, b);
-                load4(pp + off, u);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) u[i] *= b[i];
-                const R* mc = ms + c * 20;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const R t = mc[0 * 5 + j] * u[0] + mc[1 * 5 + j] * u[1] +
-                                mc[2 * 5 + j] * u[2] + mc[3 * 5 + j] * u[3];
-                    v[c][j] = t;
-                    mx = fmax(mx, t);
-                }
+            for (int i = 0; i < 4; ++i) {
+                v[i] = a[i] * b[i];
+                if (J.mode == SCALE_READ) v[i] = v[i] / sr;
             }
         }
-        if (J.mode == SCALE_WRITE)
-            rescale_lane<R, CMAX>(v, mx, C, p, J.scaleWrite, J.logScalers, nullptr, J.accA,
-                                  J.accB);
+        if (AUTO || J.mode == SCALE_WRITE) {
+            const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+            const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
+            if (in)
+                rescale4<R, AUTO>(v, mx, c, p, J.scaleWrite, J.logScalers, J.cumulative, J.accA,
+                                  J.accB, J.e1, J.e2, J.ed);
+        }
+        if (in) store4(reinterpret_cast<R*>(J.dest) + off, v);
+    }
+}
+
+template <typename R, bool AUTO>
+__global__ __launch_bounds__(512) void k_pre4(const PreJob* __restrict__ jobs, int njobs, int P,
+                                              int C, long cs) {
+    __shared__ R smax[2][8][TILE];
+    const int c = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * TILE + lane;
+    const long off = c * cs + (long)p * 4;
+    int k = 0;
+    for (int o = 0; o < njobs; ++o) {
+        const PreJob J = jobs[o];
+        const bool in = p < P && p >= J.p0 && p < J.p1;
+        R v[4] = {0, 0, 0, 0};
+        if (in) {
+            R b[4], u[4];
+            child4(reinterpret_cast<const R*>(J.xs), J.ss, reinterpret_cast<const R*>(J.mSib), off,
+                   p, c, b);
+            load4(reinterpret_cast<const R*>(J.parent) + off, u);
 #pragma unroll
-        for (int c = 0; c < CMAX; ++c)
-            if (c < C) store4(d + c * cs + (long)p * 4, v[c]);
+            for (int i = 0; i < 4; ++i) u[i] *= b[i];
+            const R* mc = reinterpret_cast<const R*>(J.mSelf) + c * 20;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                v[j] = mc[0 * 5 + j] * u[0] + mc[1 * 5 + j] * u[1] + mc[2 * 5 + j] * u[2] +
+                       mc[3 * 5 + j] * u[3];
+        }
+        if (AUTO || J.mode == SCALE_WRITE) {
+            const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+            const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
+            if (in)
+                rescale4<R, AUTO>(v, mx, c, p, J.scaleWrite, J.logScalers, nullptr, J.accA,
+                                  J.accB, J.e1, J.e2, J.ed);
+        }
+        if (in) store4(reinterpret_cast<R*>(J.dest) + off, v);
     }
 }
 
@@ -361,12 +398,26 @@ template <typename R>
 __global__ __launch_bounds__(WG) void k_rescalex(R* __restrict__ dest, int p0, int p1,
                                                  double* scaleWrite, int logScalers,
                                                  double* cumulative, const double* accA,
-                                                 const double* accB, Dims d) {
+                                                 const double* accB, const short* e1,
+                                                 const short* e2, short* ed, Dims d) {
     const int p = p0 + blockIdx.x * WG + threadIdx.x;
     if (p >= p1) return;
     R mx = 0;
     for (int c = 0; c < d.C; ++c)
         for (int i = 0; i < d.S; ++i) mx = fmax(mx, dest[c * d.catStride + (long)p * d.Sp + i]);
+    if (d.autoScale) {
+        const int e = mx > (R)0 ? exponent_of(mx) : 0;
+        for (int c = 0; c < d.C; ++c)
+            for (int i = 0; i < d.S; ++i) {
+                R& x = dest[c * d.catStride + (long)p * d.Sp + i];
+                x = pow2(x, -e);
+            }
+        int t = e;
+        if (e1) t += e1[p];
+        if (e2) t += e2[p];
+        ed[p] = (short)t;
+        return;
+    }
     if (mx == (R)0) mx = (R)1;
     const R inv = (R)1 / mx;
     for (int c = 0; c < d.C; ++c)
@@ -386,13 +437,13 @@ void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Di
                  hipStream_t st) {
     if (njobs <= 0) return;
     if (d.S == 4 && d.C <= 8) {
-        dim3 grid((d.P + WG - 1) / WG);
-        if (d.C <= 4)
-            hipLaunchKernelGGL((k_post4<R, 4>), grid, dim3(WG), 0, st, djobs, njobs, d.P, d.C,
-                               d.catStride);
+        dim3 grid((d.P + TILE - 1) / TILE);
+        if (d.autoScale)
+            hipLaunchKernelGGL((k_post4<R, true>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
+                               d.P, d.C, d.catStride);
         else
-            hipLaunchKernelGGL((k_post4<R, 8>), grid, dim3(WG), 0, st, djobs, njobs, d.P, d.C,
-                               d.catStride);
+            hipLaunchKernelGGL((k_post4<R, false>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
+                               d.P, d.C, d.catStride);
         return;
     }
     for (int o = 0; o < njobs; ++o) {
@@ -400,10 +451,10 @@ void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Di
         const long n = (long)d.C * (J.p1 - J.p0) * d.S;
         if (n <= 0) continue;
         hipLaunchKernelGGL(k_postx<R>, dim3((n + WG - 1) / WG), dim3(WG), 0, st, djobs, o, d);
-        if (J.mode == SCALE_WRITE)
+        if (J.mode == SCALE_WRITE || d.autoScale)
             hipLaunchKernelGGL(k_rescalex<R>, dim3((J.p1 - J.p0 + WG - 1) / WG), dim3(WG), 0, st,
                                reinterpret_cast<R*>(J.dest), J.p0, J.p1, J.scaleWrite,
-                               J.logScalers, J.cumulative, J.accA, J.accB, d);
+                               J.logScalers, J.cumulative, J.accA, J.accB, J.e1, J.e2, J.ed, d);
     }
 }
 
@@ -412,13 +463,13 @@ void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims&
                 int transposedSelf, void* scratch, hipStream_t st) {
     if (njobs <= 0) return;
     if (d.S == 4 && d.C <= 8 && !transposedSelf) {
-        dim3 grid((d.P + WG - 1) / WG);
-        if (d.C <= 4)
-            hipLaunchKernelGGL((k_pre4<R, 4>), grid, dim3(WG), 0, st, djobs, njobs, d.P, d.C,
-                               d.catStride);
+        dim3 grid((d.P + TILE - 1) / TILE);
+        if (d.autoScale)
+            hipLaunchKernelGGL((k_pre4<R, true>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
+                               d.P, d.C, d.catStride);
         else
-            hipLaunchKernelGGL((k_pre4<R, 8>), grid, dim3(WG), 0, st, djobs, njobs, d.P, d.C,
-                               d.catStride);
+            hipLaunchKernelGGL((k_pre4<R, false>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
+                               d.P, d.C, d.catStride);
         return;
     }
     for (int o = 0; o < njobs; ++o) {
@@ -430,10 +481,10 @@ void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims&
                            reinterpret_cast<R*>(scratch));
         hipLaunchKernelGGL(k_prex_out<R>, grid, dim3(WG), 0, st, djobs, o, d,
                            reinterpret_cast<const R*>(scratch), transposedSelf);
-        if (J.mode == SCALE_WRITE)
+        if (J.mode == SCALE_WRITE || d.autoScale)
             hipLaunchKernelGGL(k_rescalex<R>, dim3((J.p1 - J.p0 + WG - 1) / WG), dim3(WG), 0, st,
                                reinterpret_cast<R*>(J.dest), J.p0, J.p1, J.scaleWrite,
-                               J.logScalers, nullptr, J.accA, J.accB, d);
+                               J.logScalers, nullptr, J.accA, J.accB, J.e1, J.e2, J.ed, d);
     }
 }
 
@@ -448,6 +499,7 @@ __global__ __launch_bounds__(WG) void k_root(const R* __restrict__ root,
                                              const double* __restrict__ freqs,
                                              const double* __restrict__ cum,
                                              const double* __restrict__ pw,
+                                             const short* __restrict__ ex,
                                              double* __restrict__ site, double* __restrict__ bsum,
                                              int p0, int p1, Dims d) {
     __shared__ double sh[WG / 64];
@@ -473,6 +525,7 @@ __global__ __launch_bounds__(WG) void k_root(const R* __restrict__ root,
         }
         double s = log(sum);
         if (cum) s += cum[p];
+        if (ex) s += 0.69314718055994530942 * (double)ex[p];
         site[p] = s;
         contrib = s * pw[p];
     }
@@ -481,29 +534,30 @@ __global__ __launch_bounds__(WG) void k_root(const R* __restrict__ root,
 }
 
 template <typename R>
-void launch_root(const void* root, const double* catWeights, const double* freqs,
-                 const double* cumulative, const double* patternWeights, double* siteLogL,
-                 double* blockSums, double* result, const Dims& d, hipStream_t st) {
+void launch_root(const void* root, const short* rootExp, const double* catWeights,
+                 const double* freqs, const double* cumulative, const double* patternWeights,
+                 double* siteLogL, double* blockSums, double* result, const Dims& d,
+                 hipStream_t st) {
     const int nb = (d.P + WG - 1) / WG;
     hipLaunchKernelGGL(k_root<R>, dim3(nb), dim3(WG), 0, st, reinterpret_cast<const R*>(root),
-                       catWeights, freqs, cumulative, patternWeights, siteLogL, blockSums, 0, d.P,
-                       d);
+                       catWeights, freqs, cumulative, patternWeights, rootExp, siteLogL, blockSums,
+                       0, d.P, d);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(WG), 0, st, blockSums, nb, 1, result, 0);
 }
 
 template <typename R>
-void launch_root_range(const void* root, const double* catWeights, const double* freqs,
-                       const double* cumulative, const double* patternWeights, double* siteLogL,
-                       double* blockSums, double* result, int p0, int p1, const Dims& d,
-                       hipStream_t st) {
+void launch_root_range(const void* root, const short* rootExp, const double* catWeights,
+                       const double* freqs, const double* cumulative,
+                       const double* patternWeights, double* siteLogL, double* blockSums,
+                       double* result, int p0, int p1, const Dims& d, hipStream_t st) {
     const int nb = (p1 - p0 + WG - 1) / WG;
     if (nb <= 0) {
         launch_fill(result, 0.0, 1, st);
         return;
     }
     hipLaunchKernelGGL(k_root<R>, dim3(nb), dim3(WG), 0, st, reinterpret_cast<const R*>(root),
-                       catWeights, freqs, cumulative, patternWeights, siteLogL, blockSums, p0, p1,
-                       d);
+                       catWeights, freqs, cumulative, patternWeights, rootExp, siteLogL, blockSums,
+                       p0, p1, d);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(WG), 0, st, blockSums, nb, 1, result, 0);
 }
 
@@ -795,6 +849,7 @@ __global__ __launch_bounds__(WG) void k_edge_ll(const R* __restrict__ par, const
                                                 const R* D2, const double* __restrict__ cw,
                                                 const double* __restrict__ f,
                                                 const double* cumA, const double* cumB,
+                                                const short* exA, const short* exB,
                                                 const double* __restrict__ pw, double* siteL,
                                                 double* siteD1, double* siteD2,
                                                 double* __restrict__ bsum, Dims d) {
@@ -827,6 +882,8 @@ __global__ __launch_bounds__(WG) void k_edge_ll(const R* __restrict__ par, const
         double L = log(acc[0]);
         if (cumA) L += cumA[p];
         if (cumB) L += cumB[p];
+        if (exA) L += 0.69314718055994530942 * (double)exA[p];
+        if (exB) L += 0.69314718055994530942 * (double)exB[p];
         siteL[p] = L;
         cl = L * pw[p];
         if (D1) {
@@ -855,15 +912,15 @@ template <typename R>
 void launch_edge_loglik(const void* parent, const void* child, const int* childStates,
                         const void* P, const void* D1, const void* D2,
                         const double* catWeights, const double* freqs,
-                        const double* cumA, const double* cumB,
-                        const double* patternWeights, double* siteL, double* siteD1,
-                        double* siteD2, double* blockSums, double* result, const Dims& d,
-                        hipStream_t st) {
+                        const double* cumA, const double* cumB, const short* exA,
+                        const short* exB, const double* patternWeights, double* siteL,
+                        double* siteD1, double* siteD2, double* blockSums, double* result,
+                        const Dims& d, hipStream_t st) {
     const int nb = (d.P + WG - 1) / WG;
     hipLaunchKernelGGL(k_edge_ll<R>, dim3(nb), dim3(WG), 0, st, reinterpret_cast<const R*>(parent),
                        reinterpret_cast<const R*>(child), childStates,
                        reinterpret_cast<const R*>(P), reinterpret_cast<const R*>(D1),
-                       reinterpret_cast<const R*>(D2), catWeights, freqs, cumA, cumB,
+                       reinterpret_cast<const R*>(D2), catWeights, freqs, cumA, cumB, exA, exB,
                        patternWeights, siteL, siteD1, siteD2, blockSums, d);
     hipLaunchKernelGGL(k_reduce_rows, dim3(3), dim3(WG), 0, st, blockSums, nb, 3, result, 0);
 }
@@ -947,12 +1004,12 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
     template void launch_post<R>(const PostJob*, const PostJob*, int, const Dims&, hipStream_t);  \
     template void launch_pre<R>(const PreJob*, const PreJob*, int, const Dims&, int, void*,      \
                                 hipStream_t);                                                    \
-    template void launch_root<R>(const void*, const double*, const double*, const double*,       \
-                                 const double*, double*, double*, double*, const Dims&,          \
-                                 hipStream_t);                                                   \
-    template void launch_root_range<R>(const void*, const double*, const double*, const double*, \
-                                       const double*, double*, double*, double*, int, int,       \
-                                       const Dims&, hipStream_t);                                \
+    template void launch_root<R>(const void*, const short*, const double*, const double*,        \
+                                 const double*, const double*, double*, double*, double*,        \
+                                 const Dims&, hipStream_t);                                      \
+    template void launch_root_range<R>(const void*, const short*, const double*, const double*,  \
+                                       const double*, const double*, double*, double*, double*,  \
+                                       int, int, const Dims&, hipStream_t);                      \
     template void launch_edge_derivatives<R>(const EdgeJob*, int, const double*, const double*,  \
                                              double*, double*, double*, double*, int,            \
                                              const Dims&, hipStream_t);                          \
@@ -961,9 +1018,9 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
                                            hipStream_t);                                         \
     template void launch_edge_loglik<R>(const void*, const void*, const int*, const void*,       \
                                         const void*, const void*, const double*, const double*,  \
-                                        const double*, const double*, const double*, double*,    \
-                                        double*, double*, double*, double*, const Dims&,         \
-                                        hipStream_t);                                            \
+                                        const double*, const double*, const short*, const short*, \
+                                        const double*, double*, double*, double*, double*,       \
+                                        double*, const Dims&, hipStream_t);                      \
     template void launch_transpose<R>(const void* const*, void* const*, int, int, int,           \
                                       hipStream_t);                                              \
     template void launch_pad_set<R>(void*, long, R, hipStream_t);

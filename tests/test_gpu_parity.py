@@ -123,6 +123,31 @@ def test_eigen_path_matches_oracle(amd, oracle, S, n, P, shape, tips_as, scaling
     close(g["cross"], o["cross"], 1e-8, 1e-8)
 
 
+@pytest.mark.parametrize("precision,shape,n", [("double", "caterpillar", 64),
+                                              ("single", "caterpillar", 64),
+                                              ("single", "balanced", 128),
+                                              ("double", "yule", 40)])
+def test_auto_scaling_matches_oracle(amd, oracle, precision, shape, n):
+    """SCALING_AUTO: exact power-of-two rescaling inside every operation.
+    FP64 must equal the oracle's (manually rescaled) FP64 result to 1e-10;
+    FP32 must stay within 1e-6 although unscaled FP32 partials underflow."""
+    prob = synth.make_problem(41 + n, n, 3000, S=4, shape=shape, missing=0.03)
+    ev = driver.Evaluator(amd, prob, precision=precision, scaling="auto")
+    assert ev.inst.flags & api.FLAG_SCALING_AUTO
+    lg = ev.log_likelihood()
+    ev.pre_order()
+    _, s1, _ = ev.edge_derivatives(False, per_site=False)
+    ev.close()
+    ev = driver.Evaluator(oracle, prob, scaling="manual")
+    lo = ev.log_likelihood()
+    ev.pre_order()
+    _, o1, _ = ev.edge_derivatives(False, per_site=False)
+    ev.close()
+    tol = 1e-10 if precision == "double" else 1e-6
+    assert rel(lg, lo) < tol, (lg, lo)
+    close(s1, o1, 1e-8 if precision == "double" else 1e-4, 1e-6 if precision == "double" else 1e-2)
+
+
 def test_single_precision_close_to_oracle(amd, oracle):
     prob = synth.make_problem(5, 16, 2000, S=4, missing=0.02)
     ev = driver.Evaluator(amd, prob, precision="single", scaling="manual")
