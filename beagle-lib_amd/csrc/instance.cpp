@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -126,8 +127,13 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
     dSiteD1_ = dalloc<double>(dims_.Ppad);
     dSiteD2_ = dalloc<double>(dims_.Ppad);
     dResult_ = dalloc<double>(64);
+    // one-tile stand-ins read by unused operand slots of the pipelined
+    // traversal (in bounds for every lane and category, never stored to)
+    dDummy_ = dalloc<char>(dummy_bytes());
+    HIPCHECK(hipMemsetAsync(dDummy_, 0, dummy_bytes(), stream_));
     HIPCHECK(hipHostMalloc((void**)&hResult_, 64 * sizeof(double), hipHostMallocDefault));
     partStart_ = {0, patternCount};
+    if (const char* m = std::getenv("BEAGLE_AMD_TRAVERSAL")) traversalMode_ = std::atoi(m);
     ring_.resize(8);
     for (auto& s : ring_) HIPCHECK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
     sync();
@@ -151,6 +157,7 @@ Instance::~Instance() {
     hipFree(dSiteD1_);
     hipFree(dSiteD2_);
     hipFree(dResult_);
+    hipFree(dDummy_);
     hipHostFree(hResult_);
     hipFree(dWork_);
     hipFree(dScratch_);
@@ -571,6 +578,10 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
         J.x2 = dStates_[c2] ? nullptr : dPartials_[c2];
         J.s2 = dStates_[c2];
         J.m2 = matrix(m2);
+        J.mo1 = (int)(m1 * (matrixBytes_ / realSize()));
+        J.mo2 = (int)(m2 * (matrixBytes_ / realSize()));
+        J.k1 = dStates_[c1] ? 1 : 0;
+        J.k2 = dStates_[c2] ? 1 : 0;
         J.dest = dPartials_[dest];
         J.scaleWrite = nullptr;
         J.scaleRead = nullptr;
@@ -603,9 +614,88 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
         }
     }
     if (count == 0) return BEAGLE_SUCCESS;
-    auto d = reinterpret_cast<const PostJob*>(upload(hPost_.data(), count * sizeof(PostJob)));
-    if (dp_) launch_post<double>(d, hPost_.data(), count, dims_, stream_);
-    else launch_post<float>(d, hPost_.data(), count, dims_, stream_);
+    // Dependency levels: an op runs one level after the latest earlier op
+    // that writes a buffer it reads or writes, or reads the buffer it writes.
+    // Bushy lists (several ops per level) run one launch per level over
+    // (tiles x ops); deep chains run the fused whole-list kernel, one serial
+    // walk per pattern tile.
+    bool levels = false;
+    std::vector<int> level;
+    int nl = 0;
+    if (dims_.S == 4 && dims_.C <= 8 && traversalMode_ != 2) {
+        level.assign(count, 0);
+        std::vector<int> lastWrite(bufferCount_, -1);   // level of the last op writing b
+        std::vector<int> lastRead(bufferCount_, -1);    // deepest level reading b
+        for (int o = 0; o < count; ++o) {
+            const int* q = ops + (size_t)o * stride;
+            int lv = 0;
+            for (int b : {q[3], q[5]})
+                if (lastWrite[b] >= 0) lv = std::max(lv, lastWrite[b] + 1);
+            if (lastWrite[q[0]] >= 0) lv = std::max(lv, lastWrite[q[0]] + 1);
+            if (lastRead[q[0]] >= 0) lv = std::max(lv, lastRead[q[0]] + 1);
+            level[o] = lv;
+            lastWrite[q[0]] = lv;
+            for (int b : {q[3], q[5]}) lastRead[b] = std::max(lastRead[b], lv);
+            nl = std::max(nl, lv + 1);
+        }
+        // ops of one level run concurrently on the same patterns, so a shared
+        // cumulative scale buffer (read-modify-write per op) forces the fused path
+        bool shared = false;
+        for (int o = 0; o < count; ++o) shared |= hPost_[o].cumulative != nullptr;
+        levels = !shared;
+    }
+    // Wide levels run level by level; once a level holds fewer than
+    // kNarrow ops, everything from that level on (the narrow top of the
+    // tree, or a whole caterpillar) runs as one fused launch in list order.
+    const int kNarrow = traversalMode_ == 1 ? 1 : 8;
+    int wide = 0;  // number of leading levels run level by level
+    std::vector<int> start;
+    if (levels) {
+        start.assign(nl + 1, 0);
+        for (int o = 0; o < count; ++o) start[level[o] + 1]++;
+        for (int l = 0; l < nl; ++l) start[l + 1] += start[l];
+        while (wide < nl && start[wide + 1] - start[wide] >= kNarrow) ++wide;
+    }
+    std::vector<PostJob> tail;   // ops left for the fused launch, in list order
+    if (wide > 0) {
+        std::vector<PostJob> sorted(start[wide]);
+        std::vector<int> fill(start.begin(), start.begin() + wide);
+        for (int o = 0; o < count; ++o) {
+            if (level[o] < wide) sorted[fill[level[o]]++] = hPost_[o];
+            else tail.push_back(hPost_[o]);
+        }
+        auto d = reinterpret_cast<const PostJob*>(upload(sorted.data(), sorted.size() * sizeof(PostJob)));
+        if (dp_) launch_post_levels<double>(d, start.data(), wide, dims_, dMatrices_, stream_);
+        else launch_post_levels<float>(d, start.data(), wide, dims_, dMatrices_, stream_);
+    } else {
+        tail = hPost_;
+    }
+    if (!tail.empty()) {
+        // forward the previous op's output from registers when it is this op's
+        // operand over the same pattern range; a read of it over a different
+        // range (partitions) needs the stored values, so such lists run one
+        // operation per launch
+        bool late = false;
+        for (size_t o = 1; o < tail.size(); ++o) {
+            PostJob& J = tail[o];
+            const PostJob& Pv = tail[o - 1];
+            const bool same = J.p0 == Pv.p0 && J.p1 == Pv.p1;
+            if (J.x1 && J.x1 == Pv.dest) { if (same) J.k1 = 2; else late = true; }
+            if (J.x2 && J.x2 == Pv.dest) { if (same) J.k2 = 2; else late = true; }
+        }
+        const int n = (int)tail.size();
+        if (late && dims_.S == 4 && dims_.C <= 8) {
+            for (int o = 0; o < n; ++o) {
+                auto d = reinterpret_cast<const PostJob*>(upload(&tail[o], sizeof(PostJob)));
+                if (dp_) launch_post<double>(d, &tail[o], 1, dims_, dMatrices_, dDummy_, stream_);
+                else launch_post<float>(d, &tail[o], 1, dims_, dMatrices_, dDummy_, stream_);
+            }
+        } else {
+            auto d = reinterpret_cast<const PostJob*>(upload(tail.data(), n * sizeof(PostJob)));
+            if (dp_) launch_post<double>(d, tail.data(), n, dims_, dMatrices_, dDummy_, stream_);
+            else launch_post<float>(d, tail.data(), n, dims_, dMatrices_, dDummy_, stream_);
+        }
+    }
     HIPCHECK(hipGetLastError());
     if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
     return BEAGLE_SUCCESS;
@@ -639,6 +729,8 @@ int Instance::updatePrePartials(const int* ops, int count, int cumulativeIndex, 
         J.xs = dStates_[sib] ? nullptr : dPartials_[sib];
         J.ss = dStates_[sib];
         J.mSib = matrix(mSib);
+        J.moSelf = (int)(mSelf * (matrixBytes_ / realSize()));
+        J.moSib = (int)(mSib * (matrixBytes_ / realSize()));
         J.dest = dPartials_[dest];
         J.scaleWrite = nullptr;
         J.accA = J.accB = nullptr;
@@ -669,8 +761,8 @@ int Instance::updatePrePartials(const int* ops, int count, int cumulativeIndex, 
         (void)cap;
     }
     auto d = reinterpret_cast<const PreJob*>(upload(hPre_.data(), count * sizeof(PreJob)));
-    if (dp_) launch_pre<double>(d, hPre_.data(), count, dims_, preTransposed_ ? 1 : 0, dScratch_, stream_);
-    else launch_pre<float>(d, hPre_.data(), count, dims_, preTransposed_ ? 1 : 0, dScratch_, stream_);
+    if (dp_) launch_pre<double>(d, hPre_.data(), count, dims_, preTransposed_ ? 1 : 0, dScratch_, dMatrices_, stream_);
+    else launch_pre<float>(d, hPre_.data(), count, dims_, preTransposed_ ? 1 : 0, dScratch_, dMatrices_, stream_);
     HIPCHECK(hipGetLastError());
     if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
     return BEAGLE_SUCCESS;

@@ -129,6 +129,7 @@ private:
     bool always_;          // SCALING_ALWAYS
     bool auto_;            // SCALING_AUTO: power-of-two exponents per buffer
     bool logScalers_;
+    int traversalMode_ = 0;   // 0 automatic, 1 always per dependency level, 2 always fused
     Dims dims_;
     size_t partialsBytes_, matrixBytes_;
     hipStream_t stream_ = nullptr;
@@ -151,6 +152,7 @@ private:
     double* dSiteD1_ = nullptr;
     double* dSiteD2_ = nullptr;
     double* dResult_ = nullptr;       // small device result vector
+    char* dDummy_ = nullptr;          // scratch tile for the pipelined traversal
     double* hResult_ = nullptr;       // pinned mirror
     void* dWork_ = nullptr;           // grow-only reduction scratch
     size_t workCap_ = 0;

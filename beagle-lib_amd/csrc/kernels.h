@@ -34,6 +34,9 @@ struct PostJob {
     int mode;
     int logScalers;            // store log(max) rather than max
     int p0, p1;                // pattern range (a partition, or [0, P))
+    int mo1, mo2;              // m1, m2 as element offsets into the matrix slab
+    int k1, k2;                // operand kinds: 0 partials, 1 tip states, 2 the
+                               //   previous operation's output (same pattern range)
 };
 
 // One pre-order operation: dest = P_self^T (pre_parent * (P_sib x_sib)).
@@ -50,6 +53,7 @@ struct PreJob {
     int mode;
     int logScalers;
     int p0, p1;
+    int moSelf, moSib;         // mSelf, mSib as element offsets into the matrix slab
 };
 
 // One transition-matrix update: P, dP/dt, d2P/dt2 for all categories.
@@ -86,11 +90,19 @@ void launch_matrices(const MatrixJob* jobs, int njobs, int S, int C, hipStream_t
 // Traversals take the device copy of the jobs (read by the kernels) and the
 // host copy (which decides launch shapes on the generic-S path).
 template <typename R>
+// dummy: the instance's scratch tile (dummy_bytes()), read and written by
+// operand slots and lanes that have no real data.
 void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Dims& d,
-                 hipStream_t st);
+                 const void* mats, void* dummy, hipStream_t st);
+inline size_t dummy_bytes() { return 2 * 8 * 64 * 4 * sizeof(double) + 64 * (8 + 4) + 128 * 2; }
+// 4-state post-order by dependency level: jobs are ordered by level and
+// level l occupies [levelStart[l], levelStart[l+1]).
+template <typename R>
+void launch_post_levels(const PostJob* djobs, const int* levelStart, int nlevels, const Dims& d,
+                        const void* mats, hipStream_t st);
 template <typename R>
 void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims& d,
-                int transposedSelf, void* scratch, hipStream_t st);
+                int transposedSelf, void* scratch, const void* mats, hipStream_t st);
 template <typename R>
 void launch_root_range(const void* root, const short* rootExp, const double* catWeights,
                        const double* freqs, const double* cumulative,

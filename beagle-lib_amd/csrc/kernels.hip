@@ -23,6 +23,19 @@ namespace hipbeagle {
 
 constexpr int WG = 256;
 
+// Pointers that arrive inside job structs are generic; casting them to the
+// global address space turns flat accesses (which wait on both the vector
+// and the scalar counters) into global loads and stores.
+#define GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const GLOBAL T* gin(const void* p) {
+    return (const GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ GLOBAL T* gout(void* p) {
+    return (GLOBAL T*)p;
+}
+
 // ---------------------------------------------------------------------------
 // small helpers
 
@@ -38,6 +51,25 @@ template <>
 __device__ __forceinline__ void load4<float>(const float* p, float (&v)[4]) {
     float4 a = *reinterpret_cast<const float4*>(p);
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void load4(const GLOBAL double* p, double (&v)[4]) {
+    const GLOBAL dvec2* q = (const GLOBAL dvec2*)p;
+    const dvec2 a = q[0], b = q[1];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ __forceinline__ void load4(const GLOBAL float* p, float (&v)[4]) {
+    const fvec4 a = *(const GLOBAL fvec4*)p;
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void store4(GLOBAL double* p, const double (&v)[4]) {
+    GLOBAL dvec2* q = (GLOBAL dvec2*)p;
+    q[0] = dvec2{v[0], v[1]};
+    q[1] = dvec2{v[2], v[3]};
+}
+__device__ __forceinline__ void store4(GLOBAL float* p, const float (&v)[4]) {
+    *(GLOBAL fvec4*)p = fvec4{v[0], v[1], v[2], v[3]};
 }
 template <typename R>
 __device__ __forceinline__ void store4(R* p, const R (&v)[4]);
@@ -145,29 +177,6 @@ void launch_matrices(const MatrixJob* jobs, int njobs, int S, int C, hipStream_t
 // ---------------------------------------------------------------------------
 // 4-state fused traversals
 
-// One child's contribution to the four parent states of category c:
-// t_i = sum_j M[i][j] x_j for a partials child, M[i][state] for a state
-// child (a missing state reads the pad column).
-template <typename R>
-__device__ __forceinline__ void child4(const R* x, const int* s, const R* m, long off, int p,
-                                       int c, R (&t)[4]) {
-    const R* mc = m + c * 20;
-    if (s) {
-        const int st = s[p];
-        t[0] = mc[st];
-        t[1] = mc[5 + st];
-        t[2] = mc[10 + st];
-        t[3] = mc[15 + st];
-    } else {
-        R v[4];
-        load4(x + off, v);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            t[i] = mc[i * 5 + 0] * v[0] + mc[i * 5 + 1] * v[1] + mc[i * 5 + 2] * v[2] +
-                   mc[i * 5 + 3] * v[3];
-    }
-}
-
 // Workgroup layout of the 4-state traversals: C waves over one tile of 64
 // patterns; wave w owns category w, lane l owns pattern tile*64+l.  A
 // wave's matrices are wave-uniform (scalar loads), its partials loads and
@@ -183,7 +192,11 @@ __device__ __forceinline__ R tile_max(R m, R (*smax)[8][TILE], int& k, int c, in
     R (*buf)[TILE] = smax[k & 1];
     ++k;
     buf[c][lane] = m;
-    __syncthreads();
+    // LDS-only release/acquire around the barrier: outstanding global stores
+    // need not complete for this exchange
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     R mx = buf[0][lane];
     for (int cc = 1; cc < C; ++cc) mx = fmax(mx, buf[cc][lane]);
     return mx;
@@ -219,9 +232,9 @@ __device__ __forceinline__ void rescale4(R (&v)[4], R mx, int c, int p, double* 
         for (int i = 0; i < 4; ++i) v[i] = pow2(v[i], -e);
         if (c == 0) {
             int t = e;
-            if (e1) t += e1[p];
-            if (e2) t += e2[p];
-            ed[p] = (short)t;
+            if (e1) t += gin<short>(e1)[p];
+            if (e2) t += gin<short>(e2)[p];
+            gout<short>(ed)[p] = (short)t;
         }
     } else {
         if (mx == (R)0) mx = (R)1;
@@ -231,56 +244,197 @@ __device__ __forceinline__ void rescale4(R (&v)[4], R mx, int c, int p, double* 
         if (c == 0) {
             const double lm = log((double)mx);
             double w = logScalers ? lm : (double)mx;
-            if (cumulative) cumulative[p] += lm;
-            if (accA) w += accA[p];
-            if (accB) w += accB[p];
-            scaleWrite[p] = w;
+            if (cumulative) gout<double>(cumulative)[p] += lm;
+            if (accA) w += gin<double>(accA)[p];
+            if (accB) w += gin<double>(accB)[p];
+            gout<double>(scaleWrite)[p] = w;
         }
     }
+}
+
+// A 4-state operand as held in registers between its load and its use:
+// partials values, or a tip state (expanded at use to a one-hot vector whose
+// fifth entry selects the matrix pad column, so a missing state reads the
+// pad exactly as the reference does).
+template <typename R>
+struct Operand4 {
+    R x[4];
+    int st;
+};
+
+template <typename R>
+__device__ __forceinline__ void fetch4(const void* xp, const int* s, long off, int p,
+                                       Operand4<R>& o) {
+    if (s) o.st = gin<int>(s)[p];
+    else load4(gin<R>(xp) + off, o.x);
+}
+
+// t_i = sum_j M[i][j] x_j over the 4 states (plus the pad column for a
+// missing tip state); M is one category's 4 x 5 block.
+template <typename R>
+__device__ __forceinline__ void apply4(const R* m, bool isState, const Operand4<R>& o,
+                                       R (&t)[4]) {
+    if (isState) {
+        const int st = o.st;
+        const R x0 = st == 0, x1 = st == 1, x2 = st == 2, x3 = st == 3, x4 = st >= 4;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            t[i] = m[i * 5 + 0] * x0 + m[i * 5 + 1] * x1 + m[i * 5 + 2] * x2 +
+                   m[i * 5 + 3] * x3 + m[i * 5 + 4] * x4;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            t[i] = m[i * 5 + 0] * o.x[0] + m[i * 5 + 1] * o.x[1] + m[i * 5 + 2] * o.x[2] +
+                   m[i * 5 + 3] * o.x[3];
+    }
+}
+
+// Post-order traversal, software-pipelined.  While operation o computes,
+// the loads of operation o+1's operands are in flight.  To let the
+// compiler wait only for those loads (gfx9 counts stores in the same
+// vmcnt as loads, so a conservative vmcnt(0) would also wait for the
+// previous operation's stores), every step issues the same loads into the
+// same registers: unused operand slots read a one-tile dummy buffer that
+// stays in cache, and the loop is unrolled by two with alternating register
+// sets so no loaded register is ever copied.  Operand kinds are decided on
+// the host: partials (0), tip states (1), or the previous operation's
+// output forwarded from registers (2).
+template <typename R>
+__device__ __forceinline__ void issue4(const void* x, const int* s, int kind, const void* dx,
+                                       const int* ds, long off, long doff, int p, int lane,
+                                       Operand4<R>& o) {
+    const void* xa = kind == 0 ? x : dx;
+    const long xo = kind == 0 ? off : doff;
+    const int* sa = kind == 1 ? s : ds;
+    const int so = kind == 1 ? p : lane;
+    load4(gin<R>(xa) + xo, o.x);
+    o.st = gin<int>(sa)[so];
+}
+
+// Operand value by kind, formed with selects (no branches, so no register
+// copies of in-flight loads).
+template <typename R>
+__device__ __forceinline__ void operand4(int kind, const Operand4<R>& o, const R (&prev)[4],
+                                         R (&x)[5]) {
+    const int st = o.st;
+    const bool isS = kind == 1, isF = kind == 2;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const R oh = st == j ? (R)1 : (R)0;
+        x[j] = isS ? oh : (isF ? prev[j] : o.x[j]);
+    }
+    x[4] = (isS && st >= 4) ? (R)1 : (R)0;
+}
+
+// t_i = sum_j M[i][j] x_j with x_4 selecting the pad column
+template <typename R>
+__device__ __forceinline__ void mat4(const R* m, const R (&x)[5], R (&t)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        t[i] = m[i * 5 + 0] * x[0] + m[i * 5 + 1] * x[1] + m[i * 5 + 2] * x[2] +
+               m[i * 5 + 3] * x[3] + m[i * 5 + 4] * x[4];
+}
+
+// Automatic (power-of-two) rescale inside the pipelined kernel: every wave
+// of the tile loads both children's exponents and stores the node's (the
+// same value from each wave), so the step's memory operations do not
+// depend on the wave or on which operands are tips.
+template <typename R>
+__device__ __forceinline__ void auto_rescale4(R (&v)[4], R mx, const short* e1, const short* e2,
+                                              short* ed, const short* dummyE, int p, int lane,
+                                              bool in) {
+    const int e = mx > (R)0 ? exponent_of(mx) : 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = pow2(v[i], -e);
+    const int x1 = e1 ? gin<short>(e1)[p] : gin<short>(dummyE)[lane];
+    const int x2 = e2 ? gin<short>(e2)[p] : gin<short>(dummyE)[lane];
+    short* dst = in ? ed + p : const_cast<short*>(dummyE) + 64 + lane;
+    *gout<short>(dst) = (short)(e + (e1 ? x1 : 0) + (e2 ? x2 : 0));
+}
+
+template <typename R, bool AUTO>
+__device__ __forceinline__ void post_compute(const PostJob& J, const Operand4<R>& A,
+                                             const Operand4<R>& B, R (&prev)[4],
+                                             const R* __restrict__ mats, R (*smax)[8][TILE],
+                                             int& k, int c, int lane, int p, int P, int C,
+                                             long off, long doff, R* dummyW,
+                                             const double* dummyD, const short* dummyE) {
+    const bool in = p < P && p >= J.p0 && p < J.p1;
+    R xa[5], xb[5], a[4], b[4], v[4];
+    operand4(J.k1, A, prev, xa);
+    operand4(J.k2, B, prev, xb);
+    mat4(mats + J.mo1 + c * 20, xa, a);
+    mat4(mats + J.mo2 + c * 20, xb, b);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = a[i] * b[i];
+    // the read-scale factor is loaded whether or not it is used
+    const double* srp = J.mode == SCALE_READ ? J.scaleRead + p : dummyD + lane;
+    const R sr = (R)*gin<double>(srp);
+    if (J.mode == SCALE_READ) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = v[i] / sr;
+    }
+    if (AUTO) {
+        const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+        const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
+        auto_rescale4(v, mx, J.e1, J.e2, J.ed, dummyE, p, lane, in);
+    } else if (J.mode == SCALE_WRITE) {
+        const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+        const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
+        if (in)
+            rescale4<R, false>(v, mx, c, p, J.scaleWrite, J.logScalers, J.cumulative, J.accA,
+                               J.accB, J.e1, J.e2, J.ed);
+    }
+    // lanes outside the operation's pattern range store into a scratch tile
+    R* dst = in ? reinterpret_cast<R*>(J.dest) + off : dummyW + doff;
+    store4(gout<R>(dst), v);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) prev[i] = v[i];
 }
 
 template <typename R, bool AUTO>
 __global__ __launch_bounds__(512) void k_post4(const PostJob* __restrict__ jobs, int njobs, int P,
-                                               int C, long cs) {
+                                               int C, long cs, const R* __restrict__ mats,
+                                               const R* dummyX, const int* dummyS, R* dummyW,
+                                               const double* dummyD, const short* dummyE) {
     __shared__ R smax[2][8][TILE];
-    const int c = threadIdx.x >> 6;
+    // wave index = category; readfirstlane makes it an SGPR so matrix
+    // addresses are provably uniform and become scalar loads
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * TILE + lane;
     const long off = c * cs + (long)p * 4;
+    const long doff = (long)(c * TILE + lane) * 4;
     int k = 0;
-    for (int o = 0; o < njobs; ++o) {
-        const PostJob J = jobs[o];
-        const bool in = p < P && p >= J.p0 && p < J.p1;
-        R v[4] = {0, 0, 0, 0};
-        if (in) {
-            R a[4], b[4];
-            child4(reinterpret_cast<const R*>(J.x1), J.s1, reinterpret_cast<const R*>(J.m1), off,
-                   p, c, a);
-            child4(reinterpret_cast<const R*>(J.x2), J.s2, reinterpret_cast<const R*>(J.m2), off,
-                   p, c, b);
-            const R sr = J.mode == SCALE_READ ? (R)J.scaleRead[p] : (R)1;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                v[i] = a[i] * b[i];
-                if (J.mode == SCALE_READ) v[i] = v[i] / sr;
-            }
-        }
-        if (AUTO || J.mode == SCALE_WRITE) {
-            const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
-            const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
-            if (in)
-                rescale4<R, AUTO>(v, mx, c, p, J.scaleWrite, J.logScalers, J.cumulative, J.accA,
-                                  J.accB, J.e1, J.e2, J.ed);
-        }
-        if (in) store4(reinterpret_cast<R*>(J.dest) + off, v);
+    R prev[4] = {0, 0, 0, 0};
+    Operand4<R> A0, B0, A1, B1;
+    PostJob J0 = jobs[0], J1;
+    issue4(J0.x1, J0.s1, J0.k1, dummyX, dummyS, off, doff, p, lane, A0);
+    issue4(J0.x2, J0.s2, J0.k2, dummyX, dummyS, off, doff, p, lane, B0);
+    for (int o = 0;; o += 2) {
+        // operands of the next step; past the end the last operation's are
+        // reloaded (in bounds, unused) so every step issues the same loads
+        J1 = jobs[min(o + 1, njobs - 1)];
+        issue4(J1.x1, J1.s1, J1.k1, dummyX, dummyS, off, doff, p, lane, A1);
+        issue4(J1.x2, J1.s2, J1.k2, dummyX, dummyS, off, doff, p, lane, B1);
+        post_compute<R, AUTO>(J0, A0, B0, prev, mats, smax, k, c, lane, p, P, C, off, doff, dummyW,
+                              dummyD, dummyE);
+        if (o + 1 >= njobs) break;
+        J0 = jobs[min(o + 2, njobs - 1)];
+        issue4(J0.x1, J0.s1, J0.k1, dummyX, dummyS, off, doff, p, lane, A0);
+        issue4(J0.x2, J0.s2, J0.k2, dummyX, dummyS, off, doff, p, lane, B0);
+        post_compute<R, AUTO>(J1, A1, B1, prev, mats, smax, k, c, lane, p, P, C, off, doff, dummyW,
+                              dummyD, dummyE);
+        if (o + 2 >= njobs) break;
     }
 }
 
+// Pre-order traversal, simple form: one operation at a time per tile.
 template <typename R, bool AUTO>
 __global__ __launch_bounds__(512) void k_pre4(const PreJob* __restrict__ jobs, int njobs, int P,
-                                              int C, long cs) {
+                                              int C, long cs, const R* __restrict__ mats) {
     __shared__ R smax[2][8][TILE];
-    const int c = threadIdx.x >> 6;
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * TILE + lane;
     const long off = c * cs + (long)p * 4;
@@ -288,15 +442,16 @@ __global__ __launch_bounds__(512) void k_pre4(const PreJob* __restrict__ jobs, i
     for (int o = 0; o < njobs; ++o) {
         const PreJob J = jobs[o];
         const bool in = p < P && p >= J.p0 && p < J.p1;
-        R v[4] = {0, 0, 0, 0};
-        if (in) {
+        Operand4<R> U, B;
+        fetch4(J.parent, nullptr, off, p, U);
+        fetch4(J.xs, J.ss, off, p, B);
+        R v[4];
+        {
             R b[4], u[4];
-            child4(reinterpret_cast<const R*>(J.xs), J.ss, reinterpret_cast<const R*>(J.mSib), off,
-                   p, c, b);
-            load4(reinterpret_cast<const R*>(J.parent) + off, u);
+            apply4(mats + J.moSib + c * 20, J.ss != nullptr, B, b);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) u[i] *= b[i];
-            const R* mc = reinterpret_cast<const R*>(J.mSelf) + c * 20;
+            for (int i = 0; i < 4; ++i) u[i] = U.x[i] * b[i];
+            const R* mc = mats + J.moSelf + c * 20;
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 v[j] = mc[0 * 5 + j] * u[0] + mc[1 * 5 + j] * u[1] + mc[2 * 5 + j] * u[2] +
@@ -309,8 +464,51 @@ __global__ __launch_bounds__(512) void k_pre4(const PreJob* __restrict__ jobs, i
                 rescale4<R, AUTO>(v, mx, c, p, J.scaleWrite, J.logScalers, nullptr, J.accA,
                                   J.accB, J.e1, J.e2, J.ed);
         }
-        if (in) store4(reinterpret_cast<R*>(J.dest) + off, v);
+        if (in) store4(gout<R>(J.dest) + off, v);
     }
+}
+
+// Level-batched post-order: every operation of one dependency level in a
+// single launch, grid = (pattern tiles, operations); each workgroup runs one
+// operation on one tile.  Used when a level holds many operations (bushy
+// trees), where it exposes ops x tiles independent workgroups instead of
+// one serial chain per tile.
+template <typename R, bool AUTO>
+__global__ __launch_bounds__(512) void k_post4_level(const PostJob* __restrict__ jobs, int P,
+                                                     int C, long cs,
+                                                     const R* __restrict__ mats) {
+    __shared__ R smax[2][8][TILE];
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * TILE + lane;
+    const long off = c * cs + (long)p * 4;
+    int k = 0;
+    const PostJob J = jobs[blockIdx.y];
+    const bool in = p < P && p >= J.p0 && p < J.p1;
+    Operand4<R> A, B;
+    fetch4(J.x1, J.s1, off, p, A);
+    fetch4(J.x2, J.s2, off, p, B);
+    R v[4];
+    {
+        R a[4], b[4];
+        apply4(mats + J.mo1 + c * 20, J.s1 != nullptr, A, a);
+        apply4(mats + J.mo2 + c * 20, J.s2 != nullptr, B, b);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = a[i] * b[i];
+        if (J.mode == SCALE_READ) {
+            const R sr = (R)gin<double>(J.scaleRead)[p];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] = v[i] / sr;
+        }
+    }
+    if (AUTO || J.mode == SCALE_WRITE) {
+        const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+        const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
+        if (in)
+            rescale4<R, AUTO>(v, mx, c, p, J.scaleWrite, J.logScalers, J.cumulative, J.accA,
+                              J.accB, J.e1, J.e2, J.ed);
+    }
+    if (in) store4(gout<R>(J.dest) + off, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -433,17 +631,43 @@ __global__ __launch_bounds__(WG) void k_rescalex(R* __restrict__ dest, int p0, i
 // Traversal launchers: the fused 4-state kernels, or the generic-S path
 // whose grids come from the host copy of each job.
 template <typename R>
+void launch_post_levels(const PostJob* djobs, const int* levelStart, int nlevels, const Dims& d,
+                        const void* mats, hipStream_t st) {
+    for (int l = 0; l < nlevels; ++l) {
+        const int n = levelStart[l + 1] - levelStart[l];
+        dim3 grid((d.P + TILE - 1) / TILE, n);
+        if (d.autoScale)
+            hipLaunchKernelGGL((k_post4_level<R, true>), grid, dim3(TILE * d.C), 0, st,
+                               djobs + levelStart[l], d.P, d.C, d.catStride,
+                               reinterpret_cast<const R*>(mats));
+        else
+            hipLaunchKernelGGL((k_post4_level<R, false>), grid, dim3(TILE * d.C), 0, st,
+                               djobs + levelStart[l], d.P, d.C, d.catStride,
+                               reinterpret_cast<const R*>(mats));
+    }
+}
+
+template <typename R>
 void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Dims& d,
-                 hipStream_t st) {
+                 const void* mats, void* dummy, hipStream_t st) {
+    // dummy: scratch laid out as [read tile 8x64x4 R][write tile 8x64x4 R]
+    //        [64 doubles][64 ints][128 shorts]
+    const R* dummyX = reinterpret_cast<const R*>(dummy);
+    R* dummyW = const_cast<R*>(dummyX) + 8 * TILE * 4;
+    const double* dummyD = reinterpret_cast<const double*>(dummyW + 8 * TILE * 4);
+    const int* dummyS = reinterpret_cast<const int*>(dummyD + TILE);
+    const short* dummyE = reinterpret_cast<const short*>(dummyS + TILE);
     if (njobs <= 0) return;
     if (d.S == 4 && d.C <= 8) {
         dim3 grid((d.P + TILE - 1) / TILE);
         if (d.autoScale)
             hipLaunchKernelGGL((k_post4<R, true>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
-                               d.P, d.C, d.catStride);
+                               d.P, d.C, d.catStride, reinterpret_cast<const R*>(mats), dummyX,
+                               dummyS, dummyW, dummyD, dummyE);
         else
             hipLaunchKernelGGL((k_post4<R, false>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
-                               d.P, d.C, d.catStride);
+                               d.P, d.C, d.catStride, reinterpret_cast<const R*>(mats), dummyX,
+                               dummyS, dummyW, dummyD, dummyE);
         return;
     }
     for (int o = 0; o < njobs; ++o) {
@@ -460,16 +684,16 @@ void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Di
 
 template <typename R>
 void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims& d,
-                int transposedSelf, void* scratch, hipStream_t st) {
+                int transposedSelf, void* scratch, const void* mats, hipStream_t st) {
     if (njobs <= 0) return;
     if (d.S == 4 && d.C <= 8 && !transposedSelf) {
         dim3 grid((d.P + TILE - 1) / TILE);
         if (d.autoScale)
             hipLaunchKernelGGL((k_pre4<R, true>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
-                               d.P, d.C, d.catStride);
+                               d.P, d.C, d.catStride, reinterpret_cast<const R*>(mats));
         else
             hipLaunchKernelGGL((k_pre4<R, false>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
-                               d.P, d.C, d.catStride);
+                               d.P, d.C, d.catStride, reinterpret_cast<const R*>(mats));
         return;
     }
     for (int o = 0; o < njobs; ++o) {
@@ -1001,9 +1225,12 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
 
 #define INSTANTIATE(R)                                                                          \
     template void launch_matrices<R>(const MatrixJob*, int, int, int, hipStream_t);             \
-    template void launch_post<R>(const PostJob*, const PostJob*, int, const Dims&, hipStream_t);  \
+    template void launch_post<R>(const PostJob*, const PostJob*, int, const Dims&, const void*,  \
+                                 void*, hipStream_t);                                            \
+    template void launch_post_levels<R>(const PostJob*, const int*, int, const Dims&,            \
+                                        const void*, hipStream_t);                               \
     template void launch_pre<R>(const PreJob*, const PreJob*, int, const Dims&, int, void*,      \
-                                hipStream_t);                                                    \
+                                const void*, hipStream_t);                                       \
     template void launch_root<R>(const void*, const short*, const double*, const double*,        \
                                  const double*, const double*, double*, double*, double*,        \
                                  const Dims&, hipStream_t);                                      \

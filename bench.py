@@ -97,6 +97,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB -> 1 GiB elementwise copy kernel, the "
+                         "reference size for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
 
     if args.cpu_baseline_only:
@@ -179,6 +182,14 @@ def main():
     torch.cuda.synchronize()
     grad_eps = ng / (time.perf_counter() - t0)
     ev.close()
+
+    if args.calibrate_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     cpu = None
     if not args.no_cpu_baseline:
