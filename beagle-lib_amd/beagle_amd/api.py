@@ -142,6 +142,7 @@ SIGNATURES = {
     "GetSiteDerivatives": (_I, [_I, _DP, _DP]),
     # extensions (include/beagle_amd_ext.h)
     "ExtGetStream": (_I, [_I, ctypes.POINTER(ctypes.c_void_p)]),
+    "ExtUpdatePrePartialsWithDerivatives": (_I, [_I, _IP, _I, _IP, _IP, _IP, _I, _DP, _DP, _DP]),
 }
 
 
@@ -422,6 +423,18 @@ class Instance:
                 e2[1] if e2 else None, w[1], f[1], s[1], 1, ctypes.byref(out), ctypes.byref(o1),
                 ctypes.byref(o2))
         return out.value, o1.value, o2.value
+
+    def pre_partials_with_derivatives(self, ops, post, d1, d2, weights_index=0):
+        """Extension: pre-order pass fused with per-branch derivative sums.
+        Returns (sum w L'/L, sum w (L'/L)^2, sum w L''/L) per operation."""
+        a = _ip(np.asarray(ops).reshape(-1))
+        n = len(a[0]) // 7
+        b, c, d = _ip(post), _ip(d1), _ip(d2)
+        s1, sq, s2 = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._c("ExtUpdatePrePartialsWithDerivatives", a[1], n, b[1], c[1], d[1], weights_index,
+                s1.ctypes.data_as(c_double_p), sq.ctypes.data_as(c_double_p),
+                s2.ctypes.data_as(c_double_p))
+        return s1, sq, s2
 
     def stream(self):
         """hipStream_t of the instance (extension entry point)."""

@@ -142,13 +142,39 @@ class Evaluator:
         self.post_order()
         return self.root()
 
+    def set_root_pre(self):
+        """Root pre-order partials = state frequencies (the reference's CPU
+        path has beagleSetRootPrePartials; its GPU path does not, and
+        examples/hmctest/hmctest.cpp:550-559 uses beagleSetPartials)."""
+        root = self.pre_base + self.prob.tree.root
+        if self.lib.prefix == "beagle" and self.lib.path.endswith("libhmsbeagle_amd.so"):
+            self.inst.set_root_pre_partials([root], [0])
+        else:
+            self.inst.set_partials(root, np.tile(self.prob.model.freqs, self.prob.C * self.prob.P))
+
     def pre_order(self):
         tree = self.prob.tree
-        self.inst.set_partials(self.pre_base + tree.root,
-                               np.tile(self.prob.model.freqs, self.prob.C * self.prob.P))
+        self.set_root_pre()
         ops = synth.preorder_ops(tree, self.pre_base)
         self.inst.update_pre_partials(ops, api.OP_NONE)
         return ops
+
+    def fused_gradient(self, lengths=None):
+        """logL, dlogL/dt and d2logL/dt2 per branch through the fused
+        pre-order + derivative extension (this package's library only)."""
+        logl = self.log_likelihood(lengths)
+        self.set_root_pre()
+        ops = synth.preorder_ops(self.prob.tree, self.pre_base)
+        post = ops[:, 0] - self.pre_base
+        n = len(post)
+        s1, sq, s2 = self.inst.pre_partials_with_derivatives(
+            ops, post, [self.d1_index] * n, [self.d2_index] * n)
+        g1 = np.zeros(2 * self.n - 1)
+        g2 = np.zeros(2 * self.n - 1)
+        g1[post] = s1
+        g2[post] = s2 - sq
+        nodes = self.branch_nodes()
+        return logl, g1[nodes], g2[nodes]
 
     def branch_nodes(self):
         return [v for v in range(2 * self.n - 1) if v != self.prob.tree.root]

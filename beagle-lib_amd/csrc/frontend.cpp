@@ -584,6 +584,22 @@ int beagleGetSiteDerivatives(int instance, double* d1, double* d2) {
     return withInstance(instance, [&](Instance* in) { return in->getSiteDerivatives(d1, d2); });
 }
 
+int beagleExtUpdatePrePartialsWithDerivatives(int instance, const BeagleOperation* operations,
+                                              int operationCount, const int* postBufferIndices,
+                                              const int* firstDerivativeIndices,
+                                              const int* secondDerivativeIndices,
+                                              int categoryWeightsIndex,
+                                              double* outSumFirstDerivatives,
+                                              double* outSumSquaredFirstDerivatives,
+                                              double* outSumSecondDerivatives) {
+    return withInstance(instance, [&](Instance* in) {
+        return in->updatePrePartialsWithDerivatives(
+            reinterpret_cast<const int*>(operations), operationCount, postBufferIndices,
+            firstDerivativeIndices, secondDerivativeIndices, categoryWeightsIndex,
+            outSumFirstDerivatives, outSumSquaredFirstDerivatives, outSumSecondDerivatives);
+    });
+}
+
 int beagleExtGetStream(int instance, void** outStream) {
     return withInstance(instance, [&](Instance* in) {
         *outStream = reinterpret_cast<void*>(in->stream());

@@ -313,10 +313,12 @@ int Instance::setRootPrePartials(const int* bufIdx, const int* fIdx, int count) 
     if (count != 1) return BEAGLE_ERROR_NO_IMPLEMENTATION;
     const int b = bufIdx[0], f = fIdx[0];
     if (!validBuffer(b) || !validEigen(f) || hFreqs_[f].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
-    std::vector<double> tmp((size_t)dims_.C * dims_.P * dims_.S);
-    for (size_t q = 0; q < (size_t)dims_.C * dims_.P; ++q)
-        std::copy(hFreqs_[f].begin(), hFreqs_[f].end(), tmp.begin() + q * dims_.S);
-    return setPartials(b, tmp.data());
+    ensurePartials(b);
+    if (dp_) launch_tile_vector<double>(dPartials_[b], dFreqs_ + (size_t)f * dims_.S, dims_, stream_);
+    else launch_tile_vector<float>(dPartials_[b], dFreqs_ + (size_t)f * dims_.S, dims_, stream_);
+    HIPCHECK(hipGetLastError());
+    clearExponent(b);
+    return BEAGLE_SUCCESS;
 }
 
 // With a cumulative scale buffer the reference multiplies the first
@@ -704,10 +706,10 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
 // Pre-order operations [dest, wScale, rScale, parent, P_self, sibling,
 // P_sib] (BeagleCPUImpl.hpp:1689-1862); rescaling never feeds a cumulative
 // buffer and there is no read-scale path, as in the reference.
-int Instance::updatePrePartials(const int* ops, int count, int cumulativeIndex, bool byPartition) {
-    (void)cumulativeIndex;
+int Instance::buildPreJobs(const int* ops, int count, bool byPartition) {
     const int stride = byPartition ? BEAGLE_PARTITION_OP_COUNT : BEAGLE_OP_COUNT;
     hPre_.resize(count);
+    const size_t mstride = matrixBytes_ / realSize();
     for (int o = 0; o < count; ++o) {
         const int* q = ops + (size_t)o * stride;
         const int dest = q[0], wS = q[1], par = q[3], mSelf = q[4], sib = q[5], mSib = q[6];
@@ -724,16 +726,17 @@ int Instance::updatePrePartials(const int* ops, int count, int cumulativeIndex, 
         if (!dStates_[sib] && !dPartials_[sib]) return BEAGLE_ERROR_OUT_OF_RANGE;
         ensurePartials(dest);
         PreJob& J = hPre_[o];
+        std::memset(&J, 0, sizeof(PreJob));
         J.parent = dPartials_[par];
         J.mSelf = matrix(mSelf);
         J.xs = dStates_[sib] ? nullptr : dPartials_[sib];
         J.ss = dStates_[sib];
         J.mSib = matrix(mSib);
-        J.moSelf = (int)(mSelf * (matrixBytes_ / realSize()));
-        J.moSib = (int)(mSib * (matrixBytes_ / realSize()));
+        J.moSelf = (int)(mSelf * mstride);
+        J.moSib = (int)(mSib * mstride);
+        J.kSib = dStates_[sib] ? 1 : 0;
+        J.kPar = 0;
         J.dest = dPartials_[dest];
-        J.scaleWrite = nullptr;
-        J.accA = J.accB = nullptr;
         J.e1 = dExp_[par];
         J.e2 = dStates_[sib] ? nullptr : dExp_[sib];
         J.ed = dExp_[dest];
@@ -752,19 +755,110 @@ int Instance::updatePrePartials(const int* ops, int count, int cumulativeIndex, 
             J.scaleWrite = scale(wS);
         }
     }
-    if (count == 0) return BEAGLE_SUCCESS;
-    if (!(dims_.S == 4 && dims_.C <= 8) || preTransposed_) {
-        size_t cap = 0;
-        if (!dScratch_) {
-            HIPCHECK(hipMalloc(&dScratch_, partialsBytes_));
+    return BEAGLE_SUCCESS;
+}
+
+// Parent operand kinds for the pipelined 4-state kernel; returns false when
+// an operation reads the previous one's output over a different pattern
+// range (those lists run one operation per launch).
+bool Instance::preKinds() {
+    bool late = false;
+    for (size_t o = 1; o < hPre_.size(); ++o) {
+        PreJob& J = hPre_[o];
+        const PreJob& Pv = hPre_[o - 1];
+        const bool same = J.p0 == Pv.p0 && J.p1 == Pv.p1;
+        if (J.parent == Pv.dest) {
+            if (same) J.kPar = 2;
+            else late = true;
+        } else if (J.parent == Pv.parent) {
+            J.kPar = 3;
         }
-        (void)cap;
+    }
+    return !late;
+}
+
+void Instance::launchPre(int count, double* partial) {
+    const bool fused = dims_.S == 4 && dims_.C <= 8 && !preTransposed_;
+    if (!fused && !dScratch_) HIPCHECK(hipMalloc(&dScratch_, partialsBytes_));
+    const double* cw = partial ? dCatWeights_ + (size_t)preWeights_ * dims_.C : nullptr;
+    if (fused && !preKinds()) {
+        for (int o = 0; o < count; ++o) {
+            hPre_[o].kPar = 0;
+            auto d = reinterpret_cast<const PreJob*>(upload(&hPre_[o], sizeof(PreJob)));
+            const int ntiles = (dims_.P + 63) / 64;
+            double* part = partial ? partial + (size_t)o * 3 * ntiles : nullptr;
+            if (dp_) launch_pre<double>(d, &hPre_[o], 1, dims_, 0, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, part, stream_);
+            else launch_pre<float>(d, &hPre_[o], 1, dims_, 0, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, part, stream_);
+        }
+        return;
     }
     auto d = reinterpret_cast<const PreJob*>(upload(hPre_.data(), count * sizeof(PreJob)));
-    if (dp_) launch_pre<double>(d, hPre_.data(), count, dims_, preTransposed_ ? 1 : 0, dScratch_, dMatrices_, stream_);
-    else launch_pre<float>(d, hPre_.data(), count, dims_, preTransposed_ ? 1 : 0, dScratch_, dMatrices_, stream_);
+    const int t = preTransposed_ ? 1 : 0;
+    if (dp_) launch_pre<double>(d, hPre_.data(), count, dims_, t, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+    else launch_pre<float>(d, hPre_.data(), count, dims_, t, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+}
+
+int Instance::updatePrePartials(const int* ops, int count, int cumulativeIndex, bool byPartition) {
+    (void)cumulativeIndex;
+    const int rc = buildPreJobs(ops, count, byPartition);
+    if (rc != BEAGLE_SUCCESS || count == 0) return rc;
+    launchPre(count, nullptr);
     HIPCHECK(hipGetLastError());
     if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
+    return BEAGLE_SUCCESS;
+}
+
+// Extension: pre-order operations fused with the derivative sums of each
+// operation's branch (see beagle_amd_ext.h).  Where the fused 4-state
+// kernel does not apply it runs the equivalent API sequence.
+int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const int* postIdx,
+                                               const int* d1Idx, const int* d2Idx, int wIdx,
+                                               double* outD1, double* outD1Sq, double* outD2) {
+    if (!validEigen(wIdx) || hCatWeights_[wIdx].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
+    const bool fused = dims_.S == 4 && dims_.C <= 8 && !preTransposed_ && d2Idx != nullptr;
+    if (!fused) {
+        int rc = updatePrePartials(ops, count, BEAGLE_OP_NONE, false);
+        if (rc != BEAGLE_SUCCESS) return rc;
+        std::vector<int> pre(count), w(1, wIdx);
+        for (int o = 0; o < count; ++o) pre[o] = ops[(size_t)o * BEAGLE_OP_COUNT];
+        rc = calculateEdgeDerivatives(postIdx, pre.data(), d1Idx, w.data(), count, nullptr, outD1,
+                                      outD1Sq);
+        if (rc != BEAGLE_SUCCESS || !d2Idx) return rc;
+        std::vector<double> s2(count);
+        rc = calculateEdgeDerivatives(postIdx, pre.data(), d2Idx, w.data(), count, nullptr,
+                                      outD2 ? outD2 : s2.data(), nullptr);
+        return rc;
+    }
+    int rc = buildPreJobs(ops, count, false);
+    if (rc != BEAGLE_SUCCESS || count == 0) return rc;
+    const size_t mstride = matrixBytes_ / realSize();
+    for (int o = 0; o < count; ++o) {
+        const int po = postIdx[o];
+        if (!validBuffer(po) || (!dStates_[po] && !dPartials_[po]) || !validMatrix(d1Idx[o]) ||
+            !validMatrix(d2Idx[o]))
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        PreJob& J = hPre_[o];
+        J.xself = dStates_[po] ? nullptr : dPartials_[po];
+        J.sself = dStates_[po];
+        J.kSelf = dStates_[po] ? 1 : 0;
+        J.moD1 = (int)(d1Idx[o] * mstride);
+        J.moD2 = (int)(d2Idx[o] * mstride);
+    }
+    const int ntiles = (dims_.P + 63) / 64;
+    const size_t nPart = (size_t)count * 3 * ntiles;
+    double* wk = reinterpret_cast<double*>(grow(dWork_, workCap_, (nPart + 3 * (size_t)count) * sizeof(double)));
+    double* sums = wk + nPart;
+    preWeights_ = wIdx;
+    launchPre(count, wk);
+    launch_reduce_rows(wk, ntiles, 3 * count, sums, stream_);
+    HIPCHECK(hipGetLastError());
+    std::vector<double> h(3 * (size_t)count);
+    copyOut(h.data(), sums, h.size() * sizeof(double));
+    for (int o = 0; o < count; ++o) {
+        if (outD1) outD1[o] = h[3 * o];
+        if (outD1Sq) outD1Sq[o] = h[3 * o + 1];
+        if (outD2) outD2[o] = h[3 * o + 2];
+    }
     return BEAGLE_SUCCESS;
 }
 

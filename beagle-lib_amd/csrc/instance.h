@@ -60,6 +60,9 @@ public:
 
     int updatePartials(const int* ops, int count, int cumulativeIndex, bool byPartition);
     int updatePrePartials(const int* ops, int count, int cumulativeIndex, bool byPartition);
+    int updatePrePartialsWithDerivatives(const int* ops, int count, const int* postIdx,
+                                         const int* d1Idx, const int* d2Idx, int wIdx,
+                                         double* outD1, double* outD1Sq, double* outD2);
     int waitForPartials();
 
     int accumulateScaleFactors(const int* idx, int count, int cumIdx, double sign, int part);
@@ -116,6 +119,9 @@ private:
     void copyOut(void* dst, const void* src, size_t bytes);
     void sync();
     int matrixUpdate(const std::vector<MatrixJob>& jobs);
+    int buildPreJobs(const int* ops, int count, bool byPartition);
+    bool preKinds();
+    void launchPre(int count, double* partial);
     bool validBuffer(int b) const { return b >= 0 && b < bufferCount_; }
     bool validMatrix(int m) const { return m >= 0 && m < matrixCount_; }
     bool validScale(int s) const { return s >= 0 && s < scaleCount_; }
@@ -129,6 +135,7 @@ private:
     bool always_;          // SCALING_ALWAYS
     bool auto_;            // SCALING_AUTO: power-of-two exponents per buffer
     bool logScalers_;
+    int preWeights_ = 0;      // category-weights buffer of the fused derivative pass
     int traversalMode_ = 0;   // 0 automatic, 1 always per dependency level, 2 always fused
     Dims dims_;
     size_t partialsBytes_, matrixBytes_;

@@ -54,6 +54,11 @@ struct PreJob {
     int logScalers;
     int p0, p1;
     int moSelf, moSib;         // mSelf, mSib as element offsets into the matrix slab
+    int kPar, kSib;            // operand kinds (see k_pre4)
+    // fused derivatives: the node's own post-order operand and the two
+    // derivative matrices (element offsets)
+    const void* xself; const int* sself;
+    int kSelf, moD1, moD2;
 };
 
 // One transition-matrix update: P, dP/dt, d2P/dt2 for all categories.
@@ -100,9 +105,13 @@ inline size_t dummy_bytes() { return 2 * 8 * 64 * 4 * sizeof(double) + 64 * (8 +
 template <typename R>
 void launch_post_levels(const PostJob* djobs, const int* levelStart, int nlevels, const Dims& d,
                         const void* mats, hipStream_t st);
+// partial != null: fused derivative sums, 3 x ntiles doubles per job
+// (ntiles = ceil(P/64)), reduced afterwards with launch_reduce_rows.
 template <typename R>
 void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims& d,
-                int transposedSelf, void* scratch, const void* mats, hipStream_t st);
+                int transposedSelf, void* scratch, const void* mats, void* dummy,
+                const double* cw, const double* pw, double* partial, hipStream_t st);
+void launch_reduce_rows(const double* in, int n, int rows, double* out, hipStream_t st);
 template <typename R>
 void launch_root_range(const void* root, const short* rootExp, const double* catWeights,
                        const double* freqs, const double* cumulative,
@@ -142,5 +151,7 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
                       hipStream_t st);
 template <typename R>
 void launch_pad_set(void* dst, long n, R v, hipStream_t st);
+template <typename R>
+void launch_tile_vector(void* dst, const double* v, const Dims& d, hipStream_t st);
 
 }  // namespace hipbeagle

@@ -429,50 +429,11 @@ __global__ __launch_bounds__(512) void k_post4(const PostJob* __restrict__ jobs,
     }
 }
 
-// Pre-order traversal, simple form: one operation at a time per tile.
-template <typename R, bool AUTO>
-__global__ __launch_bounds__(512) void k_pre4(const PreJob* __restrict__ jobs, int njobs, int P,
-                                              int C, long cs, const R* __restrict__ mats) {
-    __shared__ R smax[2][8][TILE];
-    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int p = blockIdx.x * TILE + lane;
-    const long off = c * cs + (long)p * 4;
-    int k = 0;
-    for (int o = 0; o < njobs; ++o) {
-        const PreJob J = jobs[o];
-        const bool in = p < P && p >= J.p0 && p < J.p1;
-        Operand4<R> U, B;
-        fetch4(J.parent, nullptr, off, p, U);
-        fetch4(J.xs, J.ss, off, p, B);
-        R v[4];
-        {
-            R b[4], u[4];
-            apply4(mats + J.moSib + c * 20, J.ss != nullptr, B, b);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) u[i] = U.x[i] * b[i];
-            const R* mc = mats + J.moSelf + c * 20;
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                v[j] = mc[0 * 5 + j] * u[0] + mc[1 * 5 + j] * u[1] + mc[2 * 5 + j] * u[2] +
-                       mc[3 * 5 + j] * u[3];
-        }
-        if (AUTO || J.mode == SCALE_WRITE) {
-            const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
-            const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
-            if (in)
-                rescale4<R, AUTO>(v, mx, c, p, J.scaleWrite, J.logScalers, nullptr, J.accA,
-                                  J.accB, J.e1, J.e2, J.ed);
-        }
-        if (in) store4(gout<R>(J.dest) + off, v);
-    }
-}
-
 // Level-batched post-order: every operation of one dependency level in a
 // single launch, grid = (pattern tiles, operations); each workgroup runs one
-// operation on one tile.  Used when a level holds many operations (bushy
-// trees), where it exposes ops x tiles independent workgroups instead of
-// one serial chain per tile.
+// operation on one tile.  Used for the wide levels of bushy trees, where it
+// exposes ops x tiles independent workgroups instead of one serial chain
+// per tile.
 template <typename R, bool AUTO>
 __global__ __launch_bounds__(512) void k_post4_level(const PostJob* __restrict__ jobs, int P,
                                                      int C, long cs,
@@ -509,6 +470,163 @@ __global__ __launch_bounds__(512) void k_post4_level(const PostJob* __restrict__
                               J.accB, J.e1, J.e2, J.ed);
     }
     if (in) store4(gout<R>(J.dest) + off, v);
+}
+
+// Pre-order traversal, pipelined like the post-order one (same fixed
+// per-step loads into alternating register sets).  Operand kinds, decided
+// on the host: parent 0 = load, 2 = the previous operation's output,
+// 3 = the previous operation's parent (the two children of a node are
+// consecutive operations); sibling and (with DERIV) the node's own
+// post-order operand 0 = partials, 1 = tip states.
+//
+// With DERIV each operation also yields its branch's derivative sums:
+// with pre = the new pre-order partials and post = the node's post-order
+// partials, per pattern
+//   den = sum_c w_c pre.post,  n1 = sum_c w_c pre.(D1 post),  n2 likewise
+// (a tip uses column `state` of D and pre[state & 3] for den, as the
+// reference does), then sum_p w_p n1/den, sum_p w_p (n1/den)^2 and
+// sum_p w_p n2/den are accumulated per 64-pattern tile.  This is
+// updatePrePartials followed by two calculateEdgeDerivatives calls, with
+// the pre- and post-order partials read once.
+template <typename R>
+struct PreOps4 {
+    Operand4<R> U, B, Y;   // parent, sibling, own post-order
+};
+
+template <typename R, bool DERIV>
+__device__ __forceinline__ void pre_issue(const PreJob& J, const void* dx, const int* ds, long off,
+                                          long doff, int p, int lane, PreOps4<R>& o) {
+    issue4(J.parent, nullptr, J.kPar == 0 ? 0 : 2, dx, ds, off, doff, p, lane, o.U);
+    issue4(J.xs, J.ss, J.kSib, dx, ds, off, doff, p, lane, o.B);
+    if (DERIV) issue4(J.xself, J.sself, J.kSelf, dx, ds, off, doff, p, lane, o.Y);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <typename R, bool AUTO, bool DERIV>
+__device__ __forceinline__ void pre_compute(const PreJob& J, const PreOps4<R>& o, R (&prev)[4],
+                                            R (&uprev)[4], const R* __restrict__ mats,
+                                            R (*smax)[8][TILE], double (*sdv)[8][3][TILE],
+                                            int& k, int c, int lane, int p, int P, int C,
+                                            long off, long doff, R* dummyW, const short* dummyE,
+                                            const double* __restrict__ cw,
+                                            const double* __restrict__ pw, double* partial,
+                                            int tile, int ntiles, int oidx) {
+    const bool in = p < P && p >= J.p0 && p < J.p1;
+    R u[4], xb[5], b[4], v[4];
+    // parent operand by kind (selects, no copies of in-flight registers)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        u[i] = J.kPar == 2 ? prev[i] : (J.kPar == 3 ? uprev[i] : o.U.x[i]);
+    operand4(J.kSib, o.B, prev, xb);
+    mat4(mats + J.moSib + c * 20, xb, b);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uprev[i] = u[i];
+        u[i] *= b[i];
+    }
+    const R* ms = mats + J.moSelf + c * 20;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        v[j] = ms[0 * 5 + j] * u[0] + ms[1 * 5 + j] * u[1] + ms[2 * 5 + j] * u[2] +
+               ms[3 * 5 + j] * u[3];
+    if (AUTO) {
+        const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+        const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
+        auto_rescale4(v, mx, J.e1, J.e2, J.ed, dummyE, p, lane, in);
+    } else if (J.mode == SCALE_WRITE) {
+        const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+        const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
+        if (in)
+            rescale4<R, false>(v, mx, c, p, J.scaleWrite, J.logScalers, nullptr, J.accA, J.accB,
+                               J.e1, J.e2, J.ed);
+    }
+    R* dst = in ? reinterpret_cast<R*>(J.dest) + off : dummyW + doff;
+    store4(gout<R>(dst), v);
+    if (DERIV) {
+        R y[5], t1[4], t2[4];
+        operand4(J.kSelf, o.Y, prev, y);
+        mat4(mats + J.moD1 + c * 20, y, t1);
+        mat4(mats + J.moD2 + c * 20, y, t2);
+        double den, n1 = 0, n2 = 0;
+        if (J.kSelf == 1) {
+            const int s = o.Y.st & 3;
+            den = (double)(s == 0 ? v[0] : s == 1 ? v[1] : s == 2 ? v[2] : v[3]);
+        } else {
+            den = (double)v[0] * (double)y[0] + (double)v[1] * (double)y[1] +
+                  (double)v[2] * (double)y[2] + (double)v[3] * (double)y[3];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            n1 += (double)v[i] * (double)t1[i];
+            n2 += (double)v[i] * (double)t2[i];
+        }
+        const double wc = cw[c];
+        double (*buf)[3][TILE] = sdv[k & 1];
+        ++k;
+        buf[c][0][lane] = wc * den;
+        buf[c][1][lane] = wc * n1;
+        buf[c][2][lane] = wc * n2;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        if (c == 0) {
+            double D = 0, N1 = 0, N2 = 0;
+            for (int cc = 0; cc < C; ++cc) {
+                D += buf[cc][0][lane];
+                N1 += buf[cc][1][lane];
+                N2 += buf[cc][2][lane];
+            }
+            const double w = in ? pw[p] : 0.0;
+            const double r1 = in ? N1 / D : 0.0, r2 = in ? N2 / D : 0.0;
+            const double a = wave_sum(w * r1), bsq = wave_sum(w * r1 * r1), e = wave_sum(w * r2);
+            if (lane == 0) {
+                GLOBAL double* pt = gout<double>(partial);
+                pt[((long)oidx * 3 + 0) * ntiles + tile] = a;
+                pt[((long)oidx * 3 + 1) * ntiles + tile] = bsq;
+                pt[((long)oidx * 3 + 2) * ntiles + tile] = e;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) prev[i] = v[i];
+}
+
+template <typename R, bool AUTO, bool DERIV>
+__global__ __launch_bounds__(512) void k_pre4(const PreJob* __restrict__ jobs, int njobs, int P,
+                                              int C, long cs, const R* __restrict__ mats,
+                                              const R* dummyX, const int* dummyS, R* dummyW,
+                                              const short* dummyE, const double* __restrict__ cw,
+                                              const double* __restrict__ pw, double* partial) {
+    __shared__ R smax[2][8][TILE];
+    __shared__ double sdv[DERIV ? 2 : 1][8][3][TILE];
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x;
+    const int p = tile * TILE + lane;
+    const long off = c * cs + (long)p * 4;
+    const long doff = (long)(c * TILE + lane) * 4;
+    const int ntiles = gridDim.x;
+    int k = 0;
+    R prev[4] = {0, 0, 0, 0}, uprev[4] = {0, 0, 0, 0};
+    PreOps4<R> S0, S1;
+    PreJob J0 = jobs[0], J1;
+    pre_issue<R, DERIV>(J0, dummyX, dummyS, off, doff, p, lane, S0);
+    for (int o = 0;; o += 2) {
+        J1 = jobs[min(o + 1, njobs - 1)];
+        pre_issue<R, DERIV>(J1, dummyX, dummyS, off, doff, p, lane, S1);
+        pre_compute<R, AUTO, DERIV>(J0, S0, prev, uprev, mats, smax, sdv, k, c, lane, p, P, C, off,
+                                    doff, dummyW, dummyE, cw, pw, partial, tile, ntiles, o);
+        if (o + 1 >= njobs) break;
+        J0 = jobs[min(o + 2, njobs - 1)];
+        pre_issue<R, DERIV>(J0, dummyX, dummyS, off, doff, p, lane, S0);
+        pre_compute<R, AUTO, DERIV>(J1, S1, prev, uprev, mats, smax, sdv, k, c, lane, p, P, C, off,
+                                    doff, dummyW, dummyE, cw, pw, partial, tile, ntiles, o + 1);
+        if (o + 2 >= njobs) break;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -684,16 +802,29 @@ void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Di
 
 template <typename R>
 void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims& d,
-                int transposedSelf, void* scratch, const void* mats, hipStream_t st) {
+                int transposedSelf, void* scratch, const void* mats, void* dummy,
+                const double* cw, const double* pw, double* partial, hipStream_t st) {
     if (njobs <= 0) return;
     if (d.S == 4 && d.C <= 8 && !transposedSelf) {
+        const R* dummyX = reinterpret_cast<const R*>(dummy);
+        R* dummyW = const_cast<R*>(dummyX) + 8 * TILE * 4;
+        const double* dummyD = reinterpret_cast<const double*>(dummyW + 8 * TILE * 4);
+        const int* dummyS = reinterpret_cast<const int*>(dummyD + TILE);
+        const short* dummyE = reinterpret_cast<const short*>(dummyS + TILE);
+        const R* m = reinterpret_cast<const R*>(mats);
         dim3 grid((d.P + TILE - 1) / TILE);
-        if (d.autoScale)
-            hipLaunchKernelGGL((k_pre4<R, true>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
-                               d.P, d.C, d.catStride, reinterpret_cast<const R*>(mats));
-        else
-            hipLaunchKernelGGL((k_pre4<R, false>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
-                               d.P, d.C, d.catStride, reinterpret_cast<const R*>(mats));
+        dim3 block(TILE * d.C);
+#define PRE4(A, D)                                                                                \
+    hipLaunchKernelGGL((k_pre4<R, A, D>), grid, block, 0, st, djobs, njobs, d.P, d.C,             \
+                       d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial)
+        if (partial) {
+            if (d.autoScale) PRE4(true, true);
+            else PRE4(false, true);
+        } else {
+            if (d.autoScale) PRE4(true, false);
+            else PRE4(false, false);
+        }
+#undef PRE4
         return;
     }
     for (int o = 0; o < njobs; ++o) {
@@ -1038,6 +1169,11 @@ __global__ __launch_bounds__(WG) void k_reduce_cols(const double* __restrict__ p
     out[e] += acc;
 }
 
+void launch_reduce_rows(const double* in, int n, int rows, double* out, hipStream_t st) {
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_reduce_rows, dim3(rows), dim3(WG), 0, st, in, n, rows, out, 0);
+}
+
 int cross_products_rows(const Dims& d) {
     if (d.S == 4) return XP4_GROUPS * ((d.P + WG - 1) / WG);
     return XP_BLOCKS;
@@ -1182,6 +1318,26 @@ void launch_fill(double* dst, double v, long n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill, dim3((n + WG - 1) / WG), dim3(WG), 0, st, dst, v, n);
 }
 
+// dst[c][p][i] = v[i] for every category and pattern (root pre-order
+// partials = state frequencies).
+template <typename R>
+__global__ __launch_bounds__(WG) void k_tile_vector(R* dst, const double* __restrict__ v, Dims d) {
+    const long t = (long)blockIdx.x * WG + threadIdx.x;
+    const long n = (long)d.C * d.Ppad * d.S;
+    if (t >= n) return;
+    const int i = t % d.S;
+    const long cp = t / d.S;
+    const long p = cp % d.Ppad, c = cp / d.Ppad;
+    dst[c * d.catStride + p * d.Sp + i] = (R)v[i];
+}
+
+template <typename R>
+void launch_tile_vector(void* dst, const double* v, const Dims& d, hipStream_t st) {
+    const long n = (long)d.C * d.Ppad * d.S;
+    hipLaunchKernelGGL(k_tile_vector<R>, dim3((n + WG - 1) / WG), dim3(WG), 0, st,
+                       reinterpret_cast<R*>(dst), v, d);
+}
+
 template <typename R>
 __global__ __launch_bounds__(WG) void k_pad_set(R* dst, long n, R v) {
     const long i = (long)blockIdx.x * WG + threadIdx.x;
@@ -1230,7 +1386,8 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
     template void launch_post_levels<R>(const PostJob*, const int*, int, const Dims&,            \
                                         const void*, hipStream_t);                               \
     template void launch_pre<R>(const PreJob*, const PreJob*, int, const Dims&, int, void*,      \
-                                const void*, hipStream_t);                                       \
+                                const void*, void*, const double*, const double*, double*,       \
+                                hipStream_t);                                                    \
     template void launch_root<R>(const void*, const short*, const double*, const double*,        \
                                  const double*, const double*, double*, double*, double*,        \
                                  const Dims&, hipStream_t);                                      \
@@ -1250,7 +1407,8 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
                                         double*, const Dims&, hipStream_t);                      \
     template void launch_transpose<R>(const void* const*, void* const*, int, int, int,           \
                                       hipStream_t);                                              \
-    template void launch_pad_set<R>(void*, long, R, hipStream_t);
+    template void launch_pad_set<R>(void*, long, R, hipStream_t);                               \
+    template void launch_tile_vector<R>(void*, const double*, const Dims&, hipStream_t);
 
 INSTANTIATE(double)
 INSTANTIATE(float)

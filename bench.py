@@ -170,18 +170,37 @@ def main():
         ev.close()
     err32 = max(abs(a - b) / abs(b) for a, b in zip(logl32, logl64))
 
-    # ---- logL + full gradient (FP64) ----
+    # ---- logL + full gradient (FP64): API call sequence, and the fused
+    # pre-order + derivative extension ----
     ev = driver.Evaluator(lib, probs[1], extra_flags=async_flag)
-    for _ in range(2):
-        ev.gradient()
     ng = max(args.steps // 5, 3)
+    rates = {}
+    for name, fn in (("api", ev.gradient), ("fused", ev.fused_gradient)):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(ng):
+            res = fn()
+        torch.cuda.synchronize()
+        rates[name] = (ng / (time.perf_counter() - t0), res)
+    g_api, g_fused = rates["api"][1], rates["fused"][1]
+    grad_err = max(float(np.max(np.abs(g_api[1] - g_fused[1]) / np.maximum(np.abs(g_api[1]), 1e-12))),
+                   float(np.max(np.abs(g_api[2] - g_fused[2]) / np.maximum(np.abs(g_api[2]), 1e-12))))
+    ev.close()
+    # FP32 auto-scaled fused gradient vs the FP64 one
+    ev = driver.Evaluator(lib, probs[1], extra_flags=async_flag, precision="single",
+                          scaling="auto")
+    for _ in range(2):
+        ev.fused_gradient()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(ng):
-        ev.gradient()
+        g32 = ev.fused_gradient()
     torch.cuda.synchronize()
-    grad_eps = ng / (time.perf_counter() - t0)
+    grad32_eps = ng / (time.perf_counter() - t0)
     ev.close()
+    grad32_err = float(np.max(np.abs(g32[1] - g_fused[1])) / np.max(np.abs(g_fused[1])))
 
     if args.calibrate_copy:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
@@ -231,9 +250,15 @@ def main():
         "partials_GBps_fp64": achieved,
         "fp32_auto": {"evals_per_s": fp32_eps, "speedup_vs_fp64": fp32_eps / evals_per_s,
                       "logL_rel_err_vs_fp64": err32},
-        "logL_plus_gradient_fp64": {"evals_per_s": grad_eps,
-                                    "what": "post-order + root + pre-order + d1 and d2 edge "
-                                            "derivatives for 254 branches, balanced tree"},
+        "logL_plus_gradient": {
+            "what": "post-order + root + pre-order + dlogL/dt and d2logL/dt2 for all 254 "
+                    "branches, balanced tree",
+            "fp64_api_evals_per_s": rates["api"][0],
+            "fp64_fused_evals_per_s": rates["fused"][0],
+            "fused_vs_api_max_rel_diff": grad_err,
+            "fp32_auto_fused_evals_per_s": grad32_eps,
+            "fp32_gradient_max_err_rel_to_max": grad32_err,
+            "speedup_fp32_fused_vs_fp64_api": grad32_eps / rates["api"][0]},
         "logL": logl64,
         "cpu_baseline": cpu,
     }

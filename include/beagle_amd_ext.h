@@ -6,6 +6,8 @@
 #ifndef BEAGLE_AMD_EXT_H
 #define BEAGLE_AMD_EXT_H
 
+#include "beagle.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -14,6 +16,32 @@ extern "C" {
  * of its work, so a caller can order its own work after it or bracket it
  * with events.  No counterpart in the reference. */
 int beagleExtGetStream(int instance, void** outStream);
+
+/* Pre-order operations fused with the derivative sums of each operation's
+ * branch.  Equivalent to
+ *   beagleUpdatePrePartials(instance, operations, operationCount, OP_NONE);
+ *   beagleCalculateEdgeDerivatives(instance, postBufferIndices, pre, D1, w,
+ *       operationCount, NULL, outSumFirstDerivatives,
+ *       outSumSquaredFirstDerivatives);
+ *   beagleCalculateEdgeDerivatives(instance, postBufferIndices, pre, D2, w,
+ *       operationCount, NULL, outSumSecondDerivatives, NULL);
+ * with pre[o] = operations[o].destinationPartials, D1 =
+ * firstDerivativeIndices, D2 = secondDerivativeIndices and w =
+ * {categoryWeightsIndex}: with D1 = r_c Q and D2 = r_c^2 Q^2 (the caller of
+ * examples/hmctest/hmctest.cpp:398-427 sets them so) the outputs give per
+ * branch dlogL/dt = outSumFirstDerivatives[o] and d2logL/dt2 =
+ * outSumSecondDerivatives[o] - outSumSquaredFirstDerivatives[o].  For
+ * 4 states the derivative sums are computed inside the pre-order traversal
+ * (post- and pre-order partials read once).  Output pointers may be NULL.
+ * No counterpart in the reference. */
+int beagleExtUpdatePrePartialsWithDerivatives(int instance, const BeagleOperation* operations,
+                                              int operationCount, const int* postBufferIndices,
+                                              const int* firstDerivativeIndices,
+                                              const int* secondDerivativeIndices,
+                                              int categoryWeightsIndex,
+                                              double* outSumFirstDerivatives,
+                                              double* outSumSquaredFirstDerivatives,
+                                              double* outSumSecondDerivatives);
 
 #ifdef __cplusplus
 }

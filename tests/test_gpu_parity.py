@@ -211,3 +211,32 @@ def test_partitions_match_oracle(amd, oracle):
         ev.close()
     close(out[0][0], out[1][0], 1e-10, 0)
     assert rel(out[0][1], out[1][1]) < RTOL
+
+
+@pytest.mark.parametrize("S,shape,scaling,precision", [(4, "yule", "none", "double"),
+                                                       (4, "balanced", "manual", "double"),
+                                                       (4, "caterpillar", "auto", "double"),
+                                                       (4, "yule", "auto", "single"),
+                                                       (20, "yule", "none", "double")])
+def test_fused_pre_order_derivatives_match_oracle(amd, oracle, S, shape, scaling, precision):
+    """Extension beagleExtUpdatePrePartialsWithDerivatives against the
+    oracle's beagleUpdatePrePartials + two beagleCalculateEdgeDerivatives
+    calls, per branch: dlogL/dt and d2logL/dt2."""
+    prob = synth.make_problem(77 + S, 20, 1500 if S == 4 else 200, S=S, shape=shape,
+                              missing=0.05)
+    ev = driver.Evaluator(amd, prob, scaling=scaling, precision=precision, matrices="explicit")
+    lg, g1, g2 = ev.fused_gradient()
+    pre_tip = ev.inst.get_partials(ev.pre_base + 3)
+    ev.close()
+    ev = driver.Evaluator(oracle, prob, scaling="manual" if scaling == "auto" else scaling,
+                          matrices="explicit")
+    lo, o1, o2 = ev.gradient()
+    ev.close()
+    if precision == "double":
+        assert rel(lg, lo) < 1e-12
+        close(g1, o1, 1e-9, 1e-8)
+        close(g2, o2, 1e-9, 1e-8)
+    else:
+        assert rel(lg, lo) < 1e-6
+        close(g1, o1, 1e-3, 1e-2)
+        close(g2, o2, 1e-3, 1e-1)
