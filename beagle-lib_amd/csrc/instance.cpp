@@ -31,6 +31,73 @@ T* dalloc(size_t n) {
 
 int roundUp(int x, int m) { return (x + m - 1) / m * m; }
 
+// Dependency level of each operation of a list run in order: one more than
+// the latest earlier operation that writes a resource it reads or writes,
+// or reads a resource it writes.  Resources are buffer indices (partials
+// buffers, then scale buffers offset by the buffer count); -1 = unused.
+struct Access {
+    int reads[3];
+    int writes[2];
+};
+
+int dependencyLevels(const std::vector<Access>& acc, int nres, std::vector<int>& level) {
+    const int count = (int)acc.size();
+    level.assign(count, 0);
+    std::vector<int> lastWrite(nres, -1), lastRead(nres, -1);
+    int nl = 0;
+    for (int o = 0; o < count; ++o) {
+        int lv = 0;
+        for (int b : acc[o].reads)
+            if (b >= 0 && lastWrite[b] >= 0) lv = std::max(lv, lastWrite[b] + 1);
+        for (int b : acc[o].writes) {
+            if (b < 0) continue;
+            if (lastWrite[b] >= 0) lv = std::max(lv, lastWrite[b] + 1);
+            if (lastRead[b] >= 0) lv = std::max(lv, lastRead[b] + 1);
+        }
+        level[o] = lv;
+        for (int b : acc[o].writes)
+            if (b >= 0) lastWrite[b] = lv;
+        for (int b : acc[o].reads)
+            if (b >= 0) lastRead[b] = std::max(lastRead[b], lv);
+        nl = std::max(nl, lv + 1);
+    }
+    return nl;
+}
+
+// Launch plan: levels in order; a level with at least `narrow` operations
+// becomes one batched launch over (tiles x ops); runs of consecutive
+// narrower levels are merged into one fused launch that walks their
+// operations in list order.
+struct Segment {
+    bool batched;
+    std::vector<int> ops;
+};
+
+std::vector<Segment> planSegments(const std::vector<int>& level, int nl, int narrow) {
+    std::vector<std::vector<int>> byLevel(nl);
+    for (int o = 0; o < (int)level.size(); ++o) byLevel[level[o]].push_back(o);
+    std::vector<Segment> plan;
+    std::vector<int> pending;  // levels merged into the next fused segment
+    auto flush = [&]() {
+        if (pending.empty()) return;
+        Segment s{false, {}};
+        for (int o = 0; o < (int)level.size(); ++o)
+            if (level[o] >= pending.front() && level[o] <= pending.back()) s.ops.push_back(o);
+        plan.push_back(std::move(s));
+        pending.clear();
+    };
+    for (int l = 0; l < nl; ++l) {
+        if ((int)byLevel[l].size() >= narrow) {
+            flush();
+            plan.push_back(Segment{true, byLevel[l]});
+        } else {
+            pending.push_back(l);
+        }
+    }
+    flush();
+    return plan;
+}
+
 }  // namespace
 
 // Scaling-mode precedence follows the reference CPU implementation:
@@ -616,91 +683,84 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
         }
     }
     if (count == 0) return BEAGLE_SUCCESS;
-    // Dependency levels: an op runs one level after the latest earlier op
-    // that writes a buffer it reads or writes, or reads the buffer it writes.
-    // Bushy lists (several ops per level) run one launch per level over
-    // (tiles x ops); deep chains run the fused whole-list kernel, one serial
-    // walk per pattern tile.
-    bool levels = false;
-    std::vector<int> level;
-    int nl = 0;
-    if (dims_.S == 4 && dims_.C <= 8 && traversalMode_ != 2) {
-        level.assign(count, 0);
-        std::vector<int> lastWrite(bufferCount_, -1);   // level of the last op writing b
-        std::vector<int> lastRead(bufferCount_, -1);    // deepest level reading b
-        for (int o = 0; o < count; ++o) {
-            const int* q = ops + (size_t)o * stride;
-            int lv = 0;
-            for (int b : {q[3], q[5]})
-                if (lastWrite[b] >= 0) lv = std::max(lv, lastWrite[b] + 1);
-            if (lastWrite[q[0]] >= 0) lv = std::max(lv, lastWrite[q[0]] + 1);
-            if (lastRead[q[0]] >= 0) lv = std::max(lv, lastRead[q[0]] + 1);
-            level[o] = lv;
-            lastWrite[q[0]] = lv;
-            for (int b : {q[3], q[5]}) lastRead[b] = std::max(lastRead[b], lv);
-            nl = std::max(nl, lv + 1);
-        }
-        // ops of one level run concurrently on the same patterns, so a shared
-        // cumulative scale buffer (read-modify-write per op) forces the fused path
+    const bool fourState = dims_.S == 4 && dims_.C <= 8;
+    if (!fourState) {
+        auto d = reinterpret_cast<const PostJob*>(upload(hPost_.data(), count * sizeof(PostJob)));
+        if (dp_) launch_post<double>(d, hPost_.data(), count, dims_, dMatrices_, dDummy_, stream_);
+        else launch_post<float>(d, hPost_.data(), count, dims_, dMatrices_, dDummy_, stream_);
+    } else {
+        // A shared cumulative scale buffer is updated by every rescaling op of
+        // a pattern, so such lists stay in one fused launch (ops of a pattern
+        // in order); otherwise wide levels are batched.
         bool shared = false;
         for (int o = 0; o < count; ++o) shared |= hPost_[o].cumulative != nullptr;
-        levels = !shared;
-    }
-    // Wide levels run level by level; once a level holds fewer than
-    // kNarrow ops, everything from that level on (the narrow top of the
-    // tree, or a whole caterpillar) runs as one fused launch in list order.
-    const int kNarrow = traversalMode_ == 1 ? 1 : 8;
-    int wide = 0;  // number of leading levels run level by level
-    std::vector<int> start;
-    if (levels) {
-        start.assign(nl + 1, 0);
-        for (int o = 0; o < count; ++o) start[level[o] + 1]++;
-        for (int l = 0; l < nl; ++l) start[l + 1] += start[l];
-        while (wide < nl && start[wide + 1] - start[wide] >= kNarrow) ++wide;
-    }
-    std::vector<PostJob> tail;   // ops left for the fused launch, in list order
-    if (wide > 0) {
-        std::vector<PostJob> sorted(start[wide]);
-        std::vector<int> fill(start.begin(), start.begin() + wide);
-        for (int o = 0; o < count; ++o) {
-            if (level[o] < wide) sorted[fill[level[o]]++] = hPost_[o];
-            else tail.push_back(hPost_[o]);
-        }
-        auto d = reinterpret_cast<const PostJob*>(upload(sorted.data(), sorted.size() * sizeof(PostJob)));
-        if (dp_) launch_post_levels<double>(d, start.data(), wide, dims_, dMatrices_, stream_);
-        else launch_post_levels<float>(d, start.data(), wide, dims_, dMatrices_, stream_);
-    } else {
-        tail = hPost_;
-    }
-    if (!tail.empty()) {
-        // forward the previous op's output from registers when it is this op's
-        // operand over the same pattern range; a read of it over a different
-        // range (partitions) needs the stored values, so such lists run one
-        // operation per launch
-        bool late = false;
-        for (size_t o = 1; o < tail.size(); ++o) {
-            PostJob& J = tail[o];
-            const PostJob& Pv = tail[o - 1];
-            const bool same = J.p0 == Pv.p0 && J.p1 == Pv.p1;
-            if (J.x1 && J.x1 == Pv.dest) { if (same) J.k1 = 2; else late = true; }
-            if (J.x2 && J.x2 == Pv.dest) { if (same) J.k2 = 2; else late = true; }
-        }
-        const int n = (int)tail.size();
-        if (late && dims_.S == 4 && dims_.C <= 8) {
-            for (int o = 0; o < n; ++o) {
-                auto d = reinterpret_cast<const PostJob*>(upload(&tail[o], sizeof(PostJob)));
-                if (dp_) launch_post<double>(d, &tail[o], 1, dims_, dMatrices_, dDummy_, stream_);
-                else launch_post<float>(d, &tail[o], 1, dims_, dMatrices_, dDummy_, stream_);
-            }
+        std::vector<Segment> plan;
+        if (shared || traversalMode_ == 2) {
+            Segment s{false, {}};
+            for (int o = 0; o < count; ++o) s.ops.push_back(o);
+            plan.push_back(std::move(s));
         } else {
-            auto d = reinterpret_cast<const PostJob*>(upload(tail.data(), n * sizeof(PostJob)));
-            if (dp_) launch_post<double>(d, tail.data(), n, dims_, dMatrices_, dDummy_, stream_);
-            else launch_post<float>(d, tail.data(), n, dims_, dMatrices_, dDummy_, stream_);
+            std::vector<Access> acc(count);
+            std::vector<int> level;
+            for (int o = 0; o < count; ++o) {
+                const int* q = ops + (size_t)o * stride;
+                const PostJob& J = hPost_[o];
+                const int sw = J.scaleWrite ? (int)((J.scaleWrite - dScale_) / dims_.Ppad) : -1;
+                const int sr = J.scaleRead ? (int)((J.scaleRead - dScale_) / dims_.Ppad) : -1;
+                acc[o] = Access{{q[3], q[5], sr >= 0 ? bufferCount_ + sr : -1},
+                                {q[0], sw >= 0 ? bufferCount_ + sw : -1}};
+            }
+            const int nl = dependencyLevels(acc, bufferCount_ + scaleCount_, level);
+            plan = planSegments(level, nl, traversalMode_ == 1 ? 1 : kNarrowLevel);
+        }
+        for (const Segment& s : plan) {
+            std::vector<PostJob> jobs;
+            for (int o : s.ops) jobs.push_back(hPost_[o]);
+            launchPostSegment(jobs, s.batched);
         }
     }
     HIPCHECK(hipGetLastError());
     if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
     return BEAGLE_SUCCESS;
+}
+
+// One segment of a 4-state post-order plan.  Batched: one launch over
+// (tiles x ops).  Fused: the previous op's output is forwarded from
+// registers when it is an operand over the same pattern range; a read of it
+// over a different range (partitions) needs the stored values, so such a
+// segment runs one operation per launch.
+void Instance::launchPostSegment(std::vector<PostJob>& jobs, bool batched) {
+    const int n = (int)jobs.size();
+    if (n == 0) return;
+    if (batched) {
+        for (PostJob& J : jobs) J.k1 = J.s1 ? 1 : 0, J.k2 = J.s2 ? 1 : 0;
+        const int start[2] = {0, n};
+        auto d = reinterpret_cast<const PostJob*>(upload(jobs.data(), n * sizeof(PostJob)));
+        if (dp_) launch_post_levels<double>(d, start, 1, dims_, dMatrices_, stream_);
+        else launch_post_levels<float>(d, start, 1, dims_, dMatrices_, stream_);
+        return;
+    }
+    bool late = false;
+    for (int o = 1; o < n; ++o) {
+        PostJob& J = jobs[o];
+        const PostJob& Pv = jobs[o - 1];
+        const bool same = J.p0 == Pv.p0 && J.p1 == Pv.p1;
+        if (J.x1 && J.x1 == Pv.dest) { if (same) J.k1 = 2; else late = true; }
+        if (J.x2 && J.x2 == Pv.dest) { if (same) J.k2 = 2; else late = true; }
+    }
+    if (late) {
+        for (int o = 0; o < n; ++o) {
+            jobs[o].k1 = jobs[o].s1 ? 1 : 0;
+            jobs[o].k2 = jobs[o].s2 ? 1 : 0;
+            auto d = reinterpret_cast<const PostJob*>(upload(&jobs[o], sizeof(PostJob)));
+            if (dp_) launch_post<double>(d, &jobs[o], 1, dims_, dMatrices_, dDummy_, stream_);
+            else launch_post<float>(d, &jobs[o], 1, dims_, dMatrices_, dDummy_, stream_);
+        }
+        return;
+    }
+    auto d = reinterpret_cast<const PostJob*>(upload(jobs.data(), n * sizeof(PostJob)));
+    if (dp_) launch_post<double>(d, jobs.data(), n, dims_, dMatrices_, dDummy_, stream_);
+    else launch_post<float>(d, jobs.data(), n, dims_, dMatrices_, dDummy_, stream_);
 }
 
 // Pre-order operations [dest, wScale, rScale, parent, P_self, sibling,
@@ -727,6 +787,7 @@ int Instance::buildPreJobs(const int* ops, int count, bool byPartition) {
         ensurePartials(dest);
         PreJob& J = hPre_[o];
         std::memset(&J, 0, sizeof(PreJob));
+        J.oidx = o;
         J.parent = dPartials_[par];
         J.mSelf = matrix(mSelf);
         J.xs = dStates_[sib] ? nullptr : dPartials_[sib];
@@ -758,51 +819,78 @@ int Instance::buildPreJobs(const int* ops, int count, bool byPartition) {
     return BEAGLE_SUCCESS;
 }
 
-// Parent operand kinds for the pipelined 4-state kernel; returns false when
-// an operation reads the previous one's output over a different pattern
-// range (those lists run one operation per launch).
-bool Instance::preKinds() {
-    bool late = false;
-    for (size_t o = 1; o < hPre_.size(); ++o) {
-        PreJob& J = hPre_[o];
-        const PreJob& Pv = hPre_[o - 1];
-        const bool same = J.p0 == Pv.p0 && J.p1 == Pv.p1;
-        if (J.parent == Pv.dest) {
-            if (same) J.kPar = 2;
-            else late = true;
-        } else if (J.parent == Pv.parent) {
-            J.kPar = 3;
-        }
-    }
-    return !late;
-}
-
-void Instance::launchPre(int count, double* partial) {
-    const bool fused = dims_.S == 4 && dims_.C <= 8 && !preTransposed_;
-    if (!fused && !dScratch_) HIPCHECK(hipMalloc(&dScratch_, partialsBytes_));
+// Launch the pre-order jobs in hPre_ (built from `ops`): the generic-S
+// path one launch per op; for 4 states a level plan of batched and fused
+// segments (see planSegments).  partial != null adds the derivative sums.
+void Instance::launchPre(const int* ops, int stride, int count, double* partial) {
+    const bool fourState = dims_.S == 4 && dims_.C <= 8 && !preTransposed_;
     const double* cw = partial ? dCatWeights_ + (size_t)preWeights_ * dims_.C : nullptr;
-    if (fused && !preKinds()) {
-        for (int o = 0; o < count; ++o) {
-            hPre_[o].kPar = 0;
-            auto d = reinterpret_cast<const PreJob*>(upload(&hPre_[o], sizeof(PreJob)));
-            const int ntiles = (dims_.P + 63) / 64;
-            double* part = partial ? partial + (size_t)o * 3 * ntiles : nullptr;
-            if (dp_) launch_pre<double>(d, &hPre_[o], 1, dims_, 0, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, part, stream_);
-            else launch_pre<float>(d, &hPre_[o], 1, dims_, 0, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, part, stream_);
-        }
+    if (!fourState) {
+        if (!dScratch_) HIPCHECK(hipMalloc(&dScratch_, partialsBytes_));
+        auto d = reinterpret_cast<const PreJob*>(upload(hPre_.data(), count * sizeof(PreJob)));
+        const int t = preTransposed_ ? 1 : 0;
+        if (dp_) launch_pre<double>(d, hPre_.data(), count, dims_, t, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+        else launch_pre<float>(d, hPre_.data(), count, dims_, t, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
         return;
     }
-    auto d = reinterpret_cast<const PreJob*>(upload(hPre_.data(), count * sizeof(PreJob)));
-    const int t = preTransposed_ ? 1 : 0;
-    if (dp_) launch_pre<double>(d, hPre_.data(), count, dims_, t, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
-    else launch_pre<float>(d, hPre_.data(), count, dims_, t, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+    std::vector<Segment> plan;
+    if (traversalMode_ == 2) {
+        Segment s{false, {}};
+        for (int o = 0; o < count; ++o) s.ops.push_back(o);
+        plan.push_back(std::move(s));
+    } else {
+        std::vector<Access> acc(count);
+        std::vector<int> level;
+        for (int o = 0; o < count; ++o) {
+            const int* q = ops + (size_t)o * stride;
+            const PreJob& J = hPre_[o];
+            const int sw = J.scaleWrite ? (int)((J.scaleWrite - dScale_) / dims_.Ppad) : -1;
+            acc[o] = Access{{q[3], q[5], -1}, {q[0], sw >= 0 ? bufferCount_ + sw : -1}};
+        }
+        const int nl = dependencyLevels(acc, bufferCount_ + scaleCount_, level);
+        plan = planSegments(level, nl, traversalMode_ == 1 ? 1 : kNarrowLevel);
+    }
+    for (const Segment& s : plan) {
+        std::vector<PreJob> jobs;
+        for (int o : s.ops) jobs.push_back(hPre_[o]);
+        const int n = (int)jobs.size();
+        if (s.batched) {
+            const int start[2] = {0, n};
+            auto d = reinterpret_cast<const PreJob*>(upload(jobs.data(), n * sizeof(PreJob)));
+            if (dp_) launch_pre_levels<double>(d, start, 1, dims_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+            else launch_pre_levels<float>(d, start, 1, dims_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+            continue;
+        }
+        // parent operand kinds: 2 = previous op's output (same pattern
+        // range), 3 = previous op's parent; a read of the previous output over
+        // a different range runs one op per launch
+        bool late = false;
+        for (int o = 1; o < n; ++o) {
+            PreJob& J = jobs[o];
+            const PreJob& Pv = jobs[o - 1];
+            const bool same = J.p0 == Pv.p0 && J.p1 == Pv.p1;
+            if (J.parent == Pv.dest) {
+                if (same) J.kPar = 2;
+                else late = true;
+            } else if (J.parent == Pv.parent && same) {
+                J.kPar = 3;
+            }
+        }
+        const int step = late ? 1 : n;
+        for (int o = 0; o < n; o += step) {
+            if (late) jobs[o].kPar = 0;
+            auto d = reinterpret_cast<const PreJob*>(upload(&jobs[o], step * sizeof(PreJob)));
+            if (dp_) launch_pre<double>(d, &jobs[o], step, dims_, 0, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+            else launch_pre<float>(d, &jobs[o], step, dims_, 0, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+        }
+    }
 }
 
 int Instance::updatePrePartials(const int* ops, int count, int cumulativeIndex, bool byPartition) {
     (void)cumulativeIndex;
     const int rc = buildPreJobs(ops, count, byPartition);
     if (rc != BEAGLE_SUCCESS || count == 0) return rc;
-    launchPre(count, nullptr);
+    launchPre(ops, byPartition ? BEAGLE_PARTITION_OP_COUNT : BEAGLE_OP_COUNT, count, nullptr);
     HIPCHECK(hipGetLastError());
     if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
     return BEAGLE_SUCCESS;
@@ -849,7 +937,7 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
     double* wk = reinterpret_cast<double*>(grow(dWork_, workCap_, (nPart + 3 * (size_t)count) * sizeof(double)));
     double* sums = wk + nPart;
     preWeights_ = wIdx;
-    launchPre(count, wk);
+    launchPre(ops, BEAGLE_OP_COUNT, count, wk);
     launch_reduce_rows(wk, ntiles, 3 * count, sums, stream_);
     HIPCHECK(hipGetLastError());
     std::vector<double> h(3 * (size_t)count);

@@ -120,8 +120,9 @@ private:
     void sync();
     int matrixUpdate(const std::vector<MatrixJob>& jobs);
     int buildPreJobs(const int* ops, int count, bool byPartition);
-    bool preKinds();
-    void launchPre(int count, double* partial);
+    void launchPre(const int* ops, int stride, int count, double* partial);
+    void launchPostSegment(std::vector<PostJob>& jobs, bool batched);
+    static constexpr int kNarrowLevel = 8;   // levels with fewer ops run fused
     bool validBuffer(int b) const { return b >= 0 && b < bufferCount_; }
     bool validMatrix(int m) const { return m >= 0 && m < matrixCount_; }
     bool validScale(int s) const { return s >= 0 && s < scaleCount_; }

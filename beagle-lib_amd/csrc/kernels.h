@@ -59,6 +59,7 @@ struct PreJob {
     // derivative matrices (element offsets)
     const void* xself; const int* sself;
     int kSelf, moD1, moD2;
+    int oidx;                  // row of this operation in the derivative output
 };
 
 // One transition-matrix update: P, dP/dt, d2P/dt2 for all categories.
@@ -112,6 +113,10 @@ void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims&
                 int transposedSelf, void* scratch, const void* mats, void* dummy,
                 const double* cw, const double* pw, double* partial, hipStream_t st);
 void launch_reduce_rows(const double* in, int n, int rows, double* out, hipStream_t st);
+template <typename R>
+void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, const Dims& d,
+                       const void* mats, void* dummy, const double* cw, const double* pw,
+                       double* partial, hipStream_t st);
 template <typename R>
 void launch_root_range(const void* root, const short* rootExp, const double* catWeights,
                        const double* freqs, const double* cumulative,

@@ -509,12 +509,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <typename R, bool AUTO, bool DERIV>
 __device__ __forceinline__ void pre_compute(const PreJob& J, const PreOps4<R>& o, R (&prev)[4],
                                             R (&uprev)[4], const R* __restrict__ mats,
-                                            R (*smax)[8][TILE], double (*sdv)[8][3][TILE],
-                                            int& k, int c, int lane, int p, int P, int C,
-                                            long off, long doff, R* dummyW, const short* dummyE,
-                                            const double* __restrict__ cw,
+                                            double (*sdv)[8][4][TILE], int& k, int c, int lane,
+                                            int p, int P, int C, long off, long doff, R* dummyW,
+                                            const short* dummyE, const double* __restrict__ cw,
                                             const double* __restrict__ pw, double* partial,
-                                            int tile, int ntiles, int oidx) {
+                                            int tile, int ntiles) {
     const bool in = p < P && p >= J.p0 && p < J.p1;
     R u[4], xb[5], b[4], v[4];
     // parent operand by kind (selects, no copies of in-flight registers)
@@ -533,25 +532,17 @@ __device__ __forceinline__ void pre_compute(const PreJob& J, const PreOps4<R>& o
     for (int j = 0; j < 4; ++j)
         v[j] = ms[0 * 5 + j] * u[0] + ms[1 * 5 + j] * u[1] + ms[2 * 5 + j] * u[2] +
                ms[3 * 5 + j] * u[3];
-    if (AUTO) {
-        const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
-        const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
-        auto_rescale4(v, mx, J.e1, J.e2, J.ed, dummyE, p, lane, in);
-    } else if (J.mode == SCALE_WRITE) {
-        const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
-        const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
-        if (in)
-            rescale4<R, false>(v, mx, c, p, J.scaleWrite, J.logScalers, nullptr, J.accA, J.accB,
-                               J.e1, J.e2, J.ed);
-    }
-    R* dst = in ? reinterpret_cast<R*>(J.dest) + off : dummyW + doff;
-    store4(gout<R>(dst), v);
+    const bool rescale = AUTO || J.mode == SCALE_WRITE;
+    // The per-pattern maximum (for rescaling) and the derivative sums over
+    // categories are exchanged across the tile's category waves in one LDS
+    // round; derivative ratios do not depend on the rescaling, so they are
+    // formed from the values before it.
+    double den = 0, n1 = 0, n2 = 0;
     if (DERIV) {
         R y[5], t1[4], t2[4];
         operand4(J.kSelf, o.Y, prev, y);
         mat4(mats + J.moD1 + c * 20, y, t1);
         mat4(mats + J.moD2 + c * 20, y, t2);
-        double den, n1 = 0, n2 = 0;
         if (J.kSelf == 1) {
             const int s = o.Y.st & 3;
             den = (double)(s == 0 ? v[0] : s == 1 ? v[1] : s == 2 ? v[2] : v[3]);
@@ -565,32 +556,53 @@ __device__ __forceinline__ void pre_compute(const PreJob& J, const PreOps4<R>& o
             n2 += (double)v[i] * (double)t2[i];
         }
         const double wc = cw[c];
-        double (*buf)[3][TILE] = sdv[k & 1];
+        den *= wc;
+        n1 *= wc;
+        n2 *= wc;
+    }
+    if (DERIV || rescale) {
+        double (*buf)[4][TILE] = sdv[k & 1];
         ++k;
-        buf[c][0][lane] = wc * den;
-        buf[c][1][lane] = wc * n1;
-        buf[c][2][lane] = wc * n2;
+        const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+        buf[c][0][lane] = in ? (double)m : 0.0;
+        if (DERIV) {
+            buf[c][1][lane] = den;
+            buf[c][2][lane] = n1;
+            buf[c][3][lane] = n2;
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        if (c == 0) {
+        if (rescale) {
+            double mx = buf[0][0][lane];
+            for (int cc = 1; cc < C; ++cc) mx = fmax(mx, buf[cc][0][lane]);
+            if (AUTO) {
+                auto_rescale4(v, (R)mx, J.e1, J.e2, J.ed, dummyE, p, lane, in);
+            } else if (in) {
+                rescale4<R, false>(v, (R)mx, c, p, J.scaleWrite, J.logScalers, nullptr, J.accA,
+                                   J.accB, J.e1, J.e2, J.ed);
+            }
+        }
+        if (DERIV && c == 0) {
             double D = 0, N1 = 0, N2 = 0;
             for (int cc = 0; cc < C; ++cc) {
-                D += buf[cc][0][lane];
-                N1 += buf[cc][1][lane];
-                N2 += buf[cc][2][lane];
+                D += buf[cc][1][lane];
+                N1 += buf[cc][2][lane];
+                N2 += buf[cc][3][lane];
             }
             const double w = in ? pw[p] : 0.0;
             const double r1 = in ? N1 / D : 0.0, r2 = in ? N2 / D : 0.0;
             const double a = wave_sum(w * r1), bsq = wave_sum(w * r1 * r1), e = wave_sum(w * r2);
             if (lane == 0) {
                 GLOBAL double* pt = gout<double>(partial);
-                pt[((long)oidx * 3 + 0) * ntiles + tile] = a;
-                pt[((long)oidx * 3 + 1) * ntiles + tile] = bsq;
-                pt[((long)oidx * 3 + 2) * ntiles + tile] = e;
+                pt[((long)J.oidx * 3 + 0) * ntiles + tile] = a;
+                pt[((long)J.oidx * 3 + 1) * ntiles + tile] = bsq;
+                pt[((long)J.oidx * 3 + 2) * ntiles + tile] = e;
             }
         }
     }
+    R* dst = in ? reinterpret_cast<R*>(J.dest) + off : dummyW + doff;
+    store4(gout<R>(dst), v);
 #pragma unroll
     for (int i = 0; i < 4; ++i) prev[i] = v[i];
 }
@@ -601,15 +613,14 @@ __global__ __launch_bounds__(512) void k_pre4(const PreJob* __restrict__ jobs, i
                                               const R* dummyX, const int* dummyS, R* dummyW,
                                               const short* dummyE, const double* __restrict__ cw,
                                               const double* __restrict__ pw, double* partial) {
-    __shared__ R smax[2][8][TILE];
-    __shared__ double sdv[DERIV ? 2 : 1][8][3][TILE];
+    __shared__ double sdv[2][8][4][TILE];
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int tile = blockIdx.x;
     const int p = tile * TILE + lane;
     const long off = c * cs + (long)p * 4;
     const long doff = (long)(c * TILE + lane) * 4;
-    const int ntiles = gridDim.x;
+    const int ntiles = (P + TILE - 1) / TILE;
     int k = 0;
     R prev[4] = {0, 0, 0, 0}, uprev[4] = {0, 0, 0, 0};
     PreOps4<R> S0, S1;
@@ -618,15 +629,42 @@ __global__ __launch_bounds__(512) void k_pre4(const PreJob* __restrict__ jobs, i
     for (int o = 0;; o += 2) {
         J1 = jobs[min(o + 1, njobs - 1)];
         pre_issue<R, DERIV>(J1, dummyX, dummyS, off, doff, p, lane, S1);
-        pre_compute<R, AUTO, DERIV>(J0, S0, prev, uprev, mats, smax, sdv, k, c, lane, p, P, C, off,
-                                    doff, dummyW, dummyE, cw, pw, partial, tile, ntiles, o);
+        pre_compute<R, AUTO, DERIV>(J0, S0, prev, uprev, mats, sdv, k, c, lane, p, P, C, off, doff,
+                                    dummyW, dummyE, cw, pw, partial, tile, ntiles);
         if (o + 1 >= njobs) break;
         J0 = jobs[min(o + 2, njobs - 1)];
         pre_issue<R, DERIV>(J0, dummyX, dummyS, off, doff, p, lane, S0);
-        pre_compute<R, AUTO, DERIV>(J1, S1, prev, uprev, mats, smax, sdv, k, c, lane, p, P, C, off,
-                                    doff, dummyW, dummyE, cw, pw, partial, tile, ntiles, o + 1);
+        pre_compute<R, AUTO, DERIV>(J1, S1, prev, uprev, mats, sdv, k, c, lane, p, P, C, off, doff,
+                                    dummyW, dummyE, cw, pw, partial, tile, ntiles);
         if (o + 2 >= njobs) break;
     }
+}
+
+// Level-batched pre-order: one operation per workgroup, grid = (tiles, ops
+// of one dependency level).
+template <typename R, bool AUTO, bool DERIV>
+__global__ __launch_bounds__(512) void k_pre4_level(const PreJob* __restrict__ jobs, int P, int C,
+                                                    long cs, const R* __restrict__ mats,
+                                                    const R* dummyX, const int* dummyS,
+                                                    R* dummyW, const short* dummyE,
+                                                    const double* __restrict__ cw,
+                                                    const double* __restrict__ pw,
+                                                    double* partial) {
+    __shared__ double sdv[2][8][4][TILE];
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x;
+    const int p = tile * TILE + lane;
+    const long off = c * cs + (long)p * 4;
+    const long doff = (long)(c * TILE + lane) * 4;
+    const int ntiles = (P + TILE - 1) / TILE;
+    int k = 0;
+    R prev[4] = {0, 0, 0, 0}, uprev[4] = {0, 0, 0, 0};
+    PreOps4<R> S0;
+    const PreJob J = jobs[blockIdx.y];
+    pre_issue<R, DERIV>(J, dummyX, dummyS, off, doff, p, lane, S0);
+    pre_compute<R, AUTO, DERIV>(J, S0, prev, uprev, mats, sdv, k, c, lane, p, P, C, off, doff,
+                                dummyW, dummyE, cw, pw, partial, tile, ntiles);
 }
 
 // ---------------------------------------------------------------------------
@@ -797,6 +835,35 @@ void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Di
             hipLaunchKernelGGL(k_rescalex<R>, dim3((J.p1 - J.p0 + WG - 1) / WG), dim3(WG), 0, st,
                                reinterpret_cast<R*>(J.dest), J.p0, J.p1, J.scaleWrite,
                                J.logScalers, J.cumulative, J.accA, J.accB, J.e1, J.e2, J.ed, d);
+    }
+}
+
+template <typename R>
+void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, const Dims& d,
+                       const void* mats, void* dummy, const double* cw, const double* pw,
+                       double* partial, hipStream_t st) {
+    const R* dummyX = reinterpret_cast<const R*>(dummy);
+    R* dummyW = const_cast<R*>(dummyX) + 8 * TILE * 4;
+    const double* dummyD = reinterpret_cast<const double*>(dummyW + 8 * TILE * 4);
+    const int* dummyS = reinterpret_cast<const int*>(dummyD + TILE);
+    const short* dummyE = reinterpret_cast<const short*>(dummyS + TILE);
+    const R* m = reinterpret_cast<const R*>(mats);
+    for (int l = 0; l < nlevels; ++l) {
+        const int n = levelStart[l + 1] - levelStart[l];
+        dim3 grid((d.P + TILE - 1) / TILE, n);
+        dim3 block(TILE * d.C);
+        const PreJob* j = djobs + levelStart[l];
+#define PREL(A, D)                                                                                \
+    hipLaunchKernelGGL((k_pre4_level<R, A, D>), grid, block, 0, st, j, d.P, d.C, d.catStride, m,   \
+                       dummyX, dummyS, dummyW, dummyE, cw, pw, partial)
+        if (partial) {
+            if (d.autoScale) PREL(true, true);
+            else PREL(false, true);
+        } else {
+            if (d.autoScale) PREL(true, false);
+            else PREL(false, false);
+        }
+#undef PREL
     }
 }
 
@@ -1388,6 +1455,9 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
     template void launch_pre<R>(const PreJob*, const PreJob*, int, const Dims&, int, void*,      \
                                 const void*, void*, const double*, const double*, double*,       \
                                 hipStream_t);                                                    \
+    template void launch_pre_levels<R>(const PreJob*, const int*, int, const Dims&, const void*, \
+                                       void*, const double*, const double*, double*,             \
+                                       hipStream_t);                                             \
     template void launch_root<R>(const void*, const short*, const double*, const double*,        \
                                  const double*, const double*, double*, double*, double*,        \
                                  const Dims&, hipStream_t);                                      \
