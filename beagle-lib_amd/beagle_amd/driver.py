@@ -159,13 +159,18 @@ class Evaluator:
         self.inst.update_pre_partials(ops, api.OP_NONE)
         return ops
 
-    def fused_gradient(self, lengths=None):
+    def fused_gradient(self, lengths=None, keep_tips=False):
         """logL, dlogL/dt and d2logL/dt2 per branch through the fused
-        pre-order + derivative extension (this package's library only)."""
+        pre-order + derivative extension (this package's library only).
+        For 4 states the tips' pre-order partials are not kept unless
+        keep_tips (no later operation reads them)."""
         logl = self.log_likelihood(lengths)
         self.set_root_pre()
         ops = synth.preorder_ops(self.prob.tree, self.pre_base)
         post = ops[:, 0] - self.pre_base
+        if not keep_tips and self.prob.S == 4 and self.prob.C <= 8:
+            ops = ops.copy()
+            ops[post < self.n, 0] = api.OP_NONE
         n = len(post)
         s1, sq, s2 = self.inst.pre_partials_with_derivatives(
             ops, post, [self.d1_index] * n, [self.d2_index] * n)

@@ -68,11 +68,6 @@ int dependencyLevels(const std::vector<Access>& acc, int nres, std::vector<int>&
 // becomes one batched launch over (tiles x ops); runs of consecutive
 // narrower levels are merged into one fused launch that walks their
 // operations in list order.
-struct Segment {
-    bool batched;
-    std::vector<int> ops;
-};
-
 std::vector<Segment> planSegments(const std::vector<int>& level, int nl, int narrow) {
     std::vector<std::vector<int>> byLevel(nl);
     for (int o = 0; o < (int)level.size(); ++o) byLevel[level[o]].push_back(o);
@@ -713,60 +708,69 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
             const int nl = dependencyLevels(acc, bufferCount_ + scaleCount_, level);
             plan = planSegments(level, nl, traversalMode_ == 1 ? 1 : kNarrowLevel);
         }
-        for (const Segment& s : plan) {
-            std::vector<PostJob> jobs;
-            for (int o : s.ops) jobs.push_back(hPost_[o]);
-            launchPostSegment(jobs, s.batched);
-        }
+        launchPostPlan(plan);
     }
     HIPCHECK(hipGetLastError());
     if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
     return BEAGLE_SUCCESS;
 }
 
-// One segment of a 4-state post-order plan.  Batched: one launch over
-// (tiles x ops).  Fused: the previous op's output is forwarded from
-// registers when it is an operand over the same pattern range; a read of it
-// over a different range (partitions) needs the stored values, so such a
-// segment runs one operation per launch.
-void Instance::launchPostSegment(std::vector<PostJob>& jobs, bool batched) {
-    const int n = (int)jobs.size();
-    if (n == 0) return;
-    if (batched) {
-        for (PostJob& J : jobs) J.k1 = J.s1 ? 1 : 0, J.k2 = J.s2 ? 1 : 0;
-        const int start[2] = {0, n};
-        auto d = reinterpret_cast<const PostJob*>(upload(jobs.data(), n * sizeof(PostJob)));
-        if (dp_) launch_post_levels<double>(d, start, 1, dims_, dMatrices_, stream_);
-        else launch_post_levels<float>(d, start, 1, dims_, dMatrices_, stream_);
-        return;
-    }
-    bool late = false;
-    for (int o = 1; o < n; ++o) {
-        PostJob& J = jobs[o];
-        const PostJob& Pv = jobs[o - 1];
-        const bool same = J.p0 == Pv.p0 && J.p1 == Pv.p1;
-        if (J.x1 && J.x1 == Pv.dest) { if (same) J.k1 = 2; else late = true; }
-        if (J.x2 && J.x2 == Pv.dest) { if (same) J.k2 = 2; else late = true; }
-    }
-    if (late) {
-        for (int o = 0; o < n; ++o) {
-            jobs[o].k1 = jobs[o].s1 ? 1 : 0;
-            jobs[o].k2 = jobs[o].s2 ? 1 : 0;
-            auto d = reinterpret_cast<const PostJob*>(upload(&jobs[o], sizeof(PostJob)));
-            if (dp_) launch_post<double>(d, &jobs[o], 1, dims_, dMatrices_, dDummy_, stream_);
-            else launch_post<float>(d, &jobs[o], 1, dims_, dMatrices_, dDummy_, stream_);
+// Launch a 4-state post-order plan with one upload of all its jobs.
+// Batched segment: one launch over (tiles x ops).  Fused segment: the
+// previous op's output is forwarded from registers when it is an operand
+// over the same pattern range; a read of it over a different range
+// (partitions) needs the stored values, so such a segment runs one
+// operation per launch.
+void Instance::launchPostPlan(const std::vector<Segment>& plan) {
+    std::vector<PostJob> all;
+    struct Launch { int first, n; bool batched; };
+    std::vector<Launch> launches;
+    for (const Segment& s : plan) {
+        const int first = (int)all.size(), n = (int)s.ops.size();
+        if (n == 0) continue;
+        for (int o : s.ops) all.push_back(hPost_[o]);
+        PostJob* jobs = all.data() + first;
+        if (s.batched) {
+            launches.push_back({first, n, true});
+            continue;
         }
-        return;
+        bool late = false;
+        for (int o = 1; o < n; ++o) {
+            PostJob& J = jobs[o];
+            const PostJob& Pv = jobs[o - 1];
+            const bool same = J.p0 == Pv.p0 && J.p1 == Pv.p1;
+            if (J.x1 && J.x1 == Pv.dest) { if (same) J.k1 = 2; else late = true; }
+            if (J.x2 && J.x2 == Pv.dest) { if (same) J.k2 = 2; else late = true; }
+        }
+        if (late) {
+            for (int o = 0; o < n; ++o) {
+                jobs[o].k1 = jobs[o].s1 ? 1 : 0;
+                jobs[o].k2 = jobs[o].s2 ? 1 : 0;
+                launches.push_back({first + o, 1, false});
+            }
+        } else {
+            launches.push_back({first, n, false});
+        }
     }
-    auto d = reinterpret_cast<const PostJob*>(upload(jobs.data(), n * sizeof(PostJob)));
-    if (dp_) launch_post<double>(d, jobs.data(), n, dims_, dMatrices_, dDummy_, stream_);
-    else launch_post<float>(d, jobs.data(), n, dims_, dMatrices_, dDummy_, stream_);
+    if (all.empty()) return;
+    auto d = reinterpret_cast<const PostJob*>(upload(all.data(), all.size() * sizeof(PostJob)));
+    for (const Launch& L : launches) {
+        if (L.batched) {
+            const int start[2] = {0, L.n};
+            if (dp_) launch_post_levels<double>(d + L.first, start, 1, dims_, dMatrices_, stream_);
+            else launch_post_levels<float>(d + L.first, start, 1, dims_, dMatrices_, stream_);
+        } else {
+            PostJob* h = all.data() + L.first;
+            if (dp_) launch_post<double>(d + L.first, h, L.n, dims_, dMatrices_, dDummy_, stream_);
+            else launch_post<float>(d + L.first, h, L.n, dims_, dMatrices_, dDummy_, stream_);
+        }
+    }
 }
 
 // Pre-order operations [dest, wScale, rScale, parent, P_self, sibling,
 // P_sib] (BeagleCPUImpl.hpp:1689-1862); rescaling never feeds a cumulative
 // buffer and there is no read-scale path, as in the reference.
-int Instance::buildPreJobs(const int* ops, int count, bool byPartition) {
+int Instance::buildPreJobs(const int* ops, int count, bool byPartition, bool allowNoDest) {
     const int stride = byPartition ? BEAGLE_PARTITION_OP_COUNT : BEAGLE_OP_COUNT;
     hPre_.resize(count);
     const size_t mstride = matrixBytes_ / realSize();
@@ -780,11 +784,12 @@ int Instance::buildPreJobs(const int* ops, int count, bool byPartition) {
             p0 = partStart_[part];
             p1 = partStart_[part + 1];
         }
-        if (!validBuffer(dest) || !validBuffer(par) || !validBuffer(sib) || !validMatrix(mSelf) ||
-            !validMatrix(mSib) || !dPartials_[par])
+        const bool noDest = allowNoDest && dest == BEAGLE_OP_NONE;
+        if ((!noDest && !validBuffer(dest)) || !validBuffer(par) || !validBuffer(sib) ||
+            !validMatrix(mSelf) || !validMatrix(mSib) || !dPartials_[par])
             return BEAGLE_ERROR_OUT_OF_RANGE;
         if (!dStates_[sib] && !dPartials_[sib]) return BEAGLE_ERROR_OUT_OF_RANGE;
-        ensurePartials(dest);
+        if (!noDest) ensurePartials(dest);
         PreJob& J = hPre_[o];
         std::memset(&J, 0, sizeof(PreJob));
         J.oidx = o;
@@ -797,15 +802,15 @@ int Instance::buildPreJobs(const int* ops, int count, bool byPartition) {
         J.moSib = (int)(mSib * mstride);
         J.kSib = dStates_[sib] ? 1 : 0;
         J.kPar = 0;
-        J.dest = dPartials_[dest];
+        J.dest = noDest ? nullptr : dPartials_[dest];
         J.e1 = dExp_[par];
         J.e2 = dStates_[sib] ? nullptr : dExp_[sib];
-        J.ed = dExp_[dest];
+        J.ed = noDest ? nullptr : dExp_[dest];
         J.mode = SCALE_NONE;
         J.logScalers = logScalers_ ? 1 : 0;
         J.p0 = p0;
         J.p1 = p1;
-        if (auto_) continue;
+        if (auto_ || noDest) continue;  // nothing kept: no scale factors either
         if (always_) {
             if (!validScale(dest - tipCount_)) return BEAGLE_ERROR_OUT_OF_RANGE;
             J.mode = SCALE_WRITE;
@@ -850,20 +855,22 @@ void Instance::launchPre(const int* ops, int stride, int count, double* partial)
         const int nl = dependencyLevels(acc, bufferCount_ + scaleCount_, level);
         plan = planSegments(level, nl, traversalMode_ == 1 ? 1 : kNarrowLevel);
     }
+    // one upload of all segments' jobs, then the launches
+    std::vector<PreJob> all;
+    struct Launch { int first, n; bool batched; };
+    std::vector<Launch> launches;
     for (const Segment& s : plan) {
-        std::vector<PreJob> jobs;
-        for (int o : s.ops) jobs.push_back(hPre_[o]);
-        const int n = (int)jobs.size();
+        const int first = (int)all.size(), n = (int)s.ops.size();
+        if (n == 0) continue;
+        for (int o : s.ops) all.push_back(hPre_[o]);
+        PreJob* jobs = all.data() + first;
         if (s.batched) {
-            const int start[2] = {0, n};
-            auto d = reinterpret_cast<const PreJob*>(upload(jobs.data(), n * sizeof(PreJob)));
-            if (dp_) launch_pre_levels<double>(d, start, 1, dims_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
-            else launch_pre_levels<float>(d, start, 1, dims_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+            launches.push_back({first, n, true});
             continue;
         }
         // parent operand kinds: 2 = previous op's output (same pattern
-        // range), 3 = previous op's parent; a read of the previous output over
-        // a different range runs one op per launch
+        // range), 3 = previous op's parent (same range); a read of the
+        // previous output over a different range runs one op per launch
         bool late = false;
         for (int o = 1; o < n; ++o) {
             PreJob& J = jobs[o];
@@ -876,19 +883,33 @@ void Instance::launchPre(const int* ops, int stride, int count, double* partial)
                 J.kPar = 3;
             }
         }
-        const int step = late ? 1 : n;
-        for (int o = 0; o < n; o += step) {
-            if (late) jobs[o].kPar = 0;
-            auto d = reinterpret_cast<const PreJob*>(upload(&jobs[o], step * sizeof(PreJob)));
-            if (dp_) launch_pre<double>(d, &jobs[o], step, dims_, 0, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
-            else launch_pre<float>(d, &jobs[o], step, dims_, 0, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+        if (late) {
+            for (int o = 0; o < n; ++o) {
+                jobs[o].kPar = 0;
+                launches.push_back({first + o, 1, false});
+            }
+        } else {
+            launches.push_back({first, n, false});
+        }
+    }
+    if (all.empty()) return;
+    auto d = reinterpret_cast<const PreJob*>(upload(all.data(), all.size() * sizeof(PreJob)));
+    for (const Launch& L : launches) {
+        if (L.batched) {
+            const int start[2] = {0, L.n};
+            if (dp_) launch_pre_levels<double>(d + L.first, start, 1, dims_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+            else launch_pre_levels<float>(d + L.first, start, 1, dims_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+        } else {
+            PreJob* h = all.data() + L.first;
+            if (dp_) launch_pre<double>(d + L.first, h, L.n, dims_, 0, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+            else launch_pre<float>(d + L.first, h, L.n, dims_, 0, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
         }
     }
 }
 
 int Instance::updatePrePartials(const int* ops, int count, int cumulativeIndex, bool byPartition) {
     (void)cumulativeIndex;
-    const int rc = buildPreJobs(ops, count, byPartition);
+    const int rc = buildPreJobs(ops, count, byPartition, false);
     if (rc != BEAGLE_SUCCESS || count == 0) return rc;
     launchPre(ops, byPartition ? BEAGLE_PARTITION_OP_COUNT : BEAGLE_OP_COUNT, count, nullptr);
     HIPCHECK(hipGetLastError());
@@ -905,6 +926,8 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
     if (!validEigen(wIdx) || hCatWeights_[wIdx].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
     const bool fused = dims_.S == 4 && dims_.C <= 8 && !preTransposed_ && d2Idx != nullptr;
     if (!fused) {
+        for (int o = 0; o < count; ++o)
+            if (ops[(size_t)o * BEAGLE_OP_COUNT] == BEAGLE_OP_NONE) return BEAGLE_ERROR_OUT_OF_RANGE;
         int rc = updatePrePartials(ops, count, BEAGLE_OP_NONE, false);
         if (rc != BEAGLE_SUCCESS) return rc;
         std::vector<int> pre(count), w(1, wIdx);
@@ -917,7 +940,7 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
                                       outD2 ? outD2 : s2.data(), nullptr);
         return rc;
     }
-    int rc = buildPreJobs(ops, count, false);
+    int rc = buildPreJobs(ops, count, false, true);
     if (rc != BEAGLE_SUCCESS || count == 0) return rc;
     const size_t mstride = matrixBytes_ / realSize();
     for (int o = 0; o < count; ++o) {

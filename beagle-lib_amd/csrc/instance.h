@@ -15,6 +15,14 @@
 
 namespace hipbeagle {
 
+// Operations launched together: batched (one dependency level, grid over
+// pattern tiles x operations) or fused (operations walked in list order by
+// each pattern tile).
+struct Segment {
+    bool batched;
+    std::vector<int> ops;
+};
+
 class Instance {
 public:
     Instance(int tipCount, int partialsBufferCount, int compactBufferCount, int stateCount,
@@ -119,9 +127,9 @@ private:
     void copyOut(void* dst, const void* src, size_t bytes);
     void sync();
     int matrixUpdate(const std::vector<MatrixJob>& jobs);
-    int buildPreJobs(const int* ops, int count, bool byPartition);
+    int buildPreJobs(const int* ops, int count, bool byPartition, bool allowNoDest);
     void launchPre(const int* ops, int stride, int count, double* partial);
-    void launchPostSegment(std::vector<PostJob>& jobs, bool batched);
+    void launchPostPlan(const std::vector<Segment>& plan);
     static constexpr int kNarrowLevel = 8;   // levels with fewer ops run fused
     bool validBuffer(int b) const { return b >= 0 && b < bufferCount_; }
     bool validMatrix(int m) const { return m >= 0 && m < matrixCount_; }

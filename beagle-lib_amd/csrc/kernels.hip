@@ -509,7 +509,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <typename R, bool AUTO, bool DERIV>
 __device__ __forceinline__ void pre_compute(const PreJob& J, const PreOps4<R>& o, R (&prev)[4],
                                             R (&uprev)[4], const R* __restrict__ mats,
-                                            double (*sdv)[8][4][TILE], int& k, int c, int lane,
+                                            double* sdv, int& k, int c, int lane,
                                             int p, int P, int C, long off, long doff, R* dummyW,
                                             const short* dummyE, const double* __restrict__ cw,
                                             const double* __restrict__ pw, double* partial,
@@ -560,24 +560,28 @@ __device__ __forceinline__ void pre_compute(const PreJob& J, const PreOps4<R>& o
         n1 *= wc;
         n2 *= wc;
     }
+    // A job without a destination (extension: derivative sums only) keeps
+    // nothing; its stores go to the dummy tile.
+    const bool keep = in && J.dest != nullptr;
     if (DERIV || rescale) {
-        double (*buf)[4][TILE] = sdv[k & 1];
+        // LDS exchange buffer [2][C][4][TILE], alternating between jobs
+        double* buf = sdv + (k & 1) * C * 4 * TILE;
         ++k;
         const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
-        buf[c][0][lane] = in ? (double)m : 0.0;
+        buf[(c * 4 + 0) * TILE + lane] = in ? (double)m : 0.0;
         if (DERIV) {
-            buf[c][1][lane] = den;
-            buf[c][2][lane] = n1;
-            buf[c][3][lane] = n2;
+            buf[(c * 4 + 1) * TILE + lane] = den;
+            buf[(c * 4 + 2) * TILE + lane] = n1;
+            buf[(c * 4 + 3) * TILE + lane] = n2;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
         if (rescale) {
-            double mx = buf[0][0][lane];
-            for (int cc = 1; cc < C; ++cc) mx = fmax(mx, buf[cc][0][lane]);
+            double mx = buf[lane];
+            for (int cc = 1; cc < C; ++cc) mx = fmax(mx, buf[(cc * 4) * TILE + lane]);
             if (AUTO) {
-                auto_rescale4(v, (R)mx, J.e1, J.e2, J.ed, dummyE, p, lane, in);
+                auto_rescale4(v, (R)mx, J.e1, J.e2, J.ed, dummyE, p, lane, keep);
             } else if (in) {
                 rescale4<R, false>(v, (R)mx, c, p, J.scaleWrite, J.logScalers, nullptr, J.accA,
                                    J.accB, J.e1, J.e2, J.ed);
@@ -586,9 +590,9 @@ __device__ __forceinline__ void pre_compute(const PreJob& J, const PreOps4<R>& o
         if (DERIV && c == 0) {
             double D = 0, N1 = 0, N2 = 0;
             for (int cc = 0; cc < C; ++cc) {
-                D += buf[cc][1][lane];
-                N1 += buf[cc][2][lane];
-                N2 += buf[cc][3][lane];
+                D += buf[(cc * 4 + 1) * TILE + lane];
+                N1 += buf[(cc * 4 + 2) * TILE + lane];
+                N2 += buf[(cc * 4 + 3) * TILE + lane];
             }
             const double w = in ? pw[p] : 0.0;
             const double r1 = in ? N1 / D : 0.0, r2 = in ? N2 / D : 0.0;
@@ -601,7 +605,7 @@ __device__ __forceinline__ void pre_compute(const PreJob& J, const PreOps4<R>& o
             }
         }
     }
-    R* dst = in ? reinterpret_cast<R*>(J.dest) + off : dummyW + doff;
+    R* dst = keep ? reinterpret_cast<R*>(J.dest) + off : dummyW + doff;
     store4(gout<R>(dst), v);
 #pragma unroll
     for (int i = 0; i < 4; ++i) prev[i] = v[i];
@@ -613,7 +617,7 @@ __global__ __launch_bounds__(512) void k_pre4(const PreJob* __restrict__ jobs, i
                                               const R* dummyX, const int* dummyS, R* dummyW,
                                               const short* dummyE, const double* __restrict__ cw,
                                               const double* __restrict__ pw, double* partial) {
-    __shared__ double sdv[2][8][4][TILE];
+    extern __shared__ double sdv[];   // [2][C][4][TILE]
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int tile = blockIdx.x;
@@ -650,7 +654,7 @@ __global__ __launch_bounds__(512) void k_pre4_level(const PreJob* __restrict__ j
                                                     const double* __restrict__ cw,
                                                     const double* __restrict__ pw,
                                                     double* partial) {
-    __shared__ double sdv[2][8][4][TILE];
+    extern __shared__ double sdv[];   // [2][C][4][TILE]
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int tile = blockIdx.x;
@@ -838,6 +842,10 @@ void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Di
     }
 }
 
+// LDS exchange of the 4-state pre-order kernels: [2][C][4][TILE] doubles
+// (used by jobs that rescale or form derivative sums).
+static size_t pre_lds_bytes(int C) { return (size_t)2 * C * 4 * TILE * sizeof(double); }
+
 template <typename R>
 void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, const Dims& d,
                        const void* mats, void* dummy, const double* cw, const double* pw,
@@ -852,9 +860,10 @@ void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, 
         const int n = levelStart[l + 1] - levelStart[l];
         dim3 grid((d.P + TILE - 1) / TILE, n);
         dim3 block(TILE * d.C);
+        const size_t lds = pre_lds_bytes(d.C);
         const PreJob* j = djobs + levelStart[l];
 #define PREL(A, D)                                                                                \
-    hipLaunchKernelGGL((k_pre4_level<R, A, D>), grid, block, 0, st, j, d.P, d.C, d.catStride, m,   \
+    hipLaunchKernelGGL((k_pre4_level<R, A, D>), grid, block, lds, st, j, d.P, d.C, d.catStride, m, \
                        dummyX, dummyS, dummyW, dummyE, cw, pw, partial)
         if (partial) {
             if (d.autoScale) PREL(true, true);
@@ -881,8 +890,9 @@ void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims&
         const R* m = reinterpret_cast<const R*>(mats);
         dim3 grid((d.P + TILE - 1) / TILE);
         dim3 block(TILE * d.C);
+        const size_t lds = pre_lds_bytes(d.C);
 #define PRE4(A, D)                                                                                \
-    hipLaunchKernelGGL((k_pre4<R, A, D>), grid, block, 0, st, djobs, njobs, d.P, d.C,             \
+    hipLaunchKernelGGL((k_pre4<R, A, D>), grid, block, lds, st, djobs, njobs, d.P, d.C,           \
                        d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial)
         if (partial) {
             if (d.autoScale) PRE4(true, true);
@@ -1049,6 +1059,80 @@ __global__ __launch_bounds__(WG) void k_edge_deriv(const EdgeJob* __restrict__ j
     }
 }
 
+// 4-state edge derivatives: one lane per pattern over all categories, 16-
+// or 32-byte vector loads of the pre- and post-order partials, the job's
+// derivative matrix through scalar loads.  Same sums and per-site ratios as
+// k_edge_deriv (a tip's denominator is pre[state % 4], as in the
+// reference).  Lanes past the last pattern read the last pattern and
+// contribute nothing.
+template <typename R>
+__global__ __launch_bounds__(WG) void k_edge_deriv4(const EdgeJob* __restrict__ jobs,
+                                                    const double* __restrict__ cw,
+                                                    const double* __restrict__ pw,
+                                                    double* __restrict__ sites,
+                                                    double* __restrict__ bsum, Dims d) {
+    __shared__ double sh[WG / 64];
+    const int n = blockIdx.y;
+    const EdgeJob J = jobs[n];
+    const int p = blockIdx.x * WG + threadIdx.x;
+    const int pc = min(p, d.P - 1);
+    const R* __restrict__ D = reinterpret_cast<const R*>(J.D);
+    double num = 0.0, den = 0.0;
+    if (J.states) {
+        const int s = gin<int>(J.states)[pc];
+        for (int c = 0; c < d.C; ++c) {
+            R pre[4];
+            load4(gin<R>(J.pre) + c * d.catStride + (long)pc * 4, pre);
+            const R* Dc = D + c * 20;
+            double nc = 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const R* row = Dc + i * 5;
+                const R dij = s == 0 ? row[0] : s == 1 ? row[1] : s == 2 ? row[2]
+                                                : s == 3 ? row[3] : row[4];
+                nc += (double)dij * (double)pre[i];
+            }
+            const int sm = s & 3;
+            const R dc = sm == 0 ? pre[0] : sm == 1 ? pre[1] : sm == 2 ? pre[2] : pre[3];
+            num += cw[c] * nc;
+            den += cw[c] * (double)dc;
+        }
+    } else {
+        for (int c = 0; c < d.C; ++c) {
+            R pre[4], post[4];
+            const long off = c * d.catStride + (long)pc * 4;
+            load4(gin<R>(J.pre) + off, pre);
+            load4(gin<R>(J.post) + off, post);
+            const R* Dc = D + c * 20;
+            double nc = 0.0, dc = 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const R* row = Dc + i * 5;
+                const double t = (double)row[0] * (double)post[0] + (double)row[1] * (double)post[1] +
+                                 (double)row[2] * (double)post[2] + (double)row[3] * (double)post[3];
+                nc += t * (double)pre[i];
+                dc += (double)post[i] * (double)pre[i];
+            }
+            num += cw[c] * nc;
+            den += cw[c] * dc;
+        }
+    }
+    double r1 = 0.0, r2 = 0.0;
+    if (p < d.P) {
+        const double r = num / den;
+        if (sites) sites[(long)n * d.P + p] = r;
+        r1 = r * pw[p];
+        r2 = r * r * pw[p];
+    }
+    const int nb = gridDim.x;
+    const double a = block_sum(r1, sh);
+    const double b = block_sum(r2, sh);
+    if (threadIdx.x == 0) {
+        bsum[(long)(2 * n) * nb + blockIdx.x] = a;
+        bsum[(long)(2 * n + 1) * nb + blockIdx.x] = b;
+    }
+}
+
 __global__ __launch_bounds__(WG) void k_split_pairs(const double* __restrict__ rows, int njobs,
                                                     double* sums, double* sumSq) {
     const int n = blockIdx.x * WG + threadIdx.x;
@@ -1064,8 +1148,12 @@ void launch_edge_derivatives(const EdgeJob* jobs, int njobs, const double* catWe
                              hipStream_t st) {
     if (njobs <= 0) return;
     const int nb = (d.P + WG - 1) / WG;
-    hipLaunchKernelGGL(k_edge_deriv<R>, dim3(nb, njobs), dim3(WG), 0, st, jobs, catWeights,
-                       patternWeights, sites, blockSums, transposedD, d);
+    if (d.S == 4 && d.Sp == 4 && !transposedD)
+        hipLaunchKernelGGL(k_edge_deriv4<R>, dim3(nb, njobs), dim3(WG), 0, st, jobs, catWeights,
+                           patternWeights, sites, blockSums, d);
+    else
+        hipLaunchKernelGGL(k_edge_deriv<R>, dim3(nb, njobs), dim3(WG), 0, st, jobs, catWeights,
+                           patternWeights, sites, blockSums, transposedD, d);
     // rows (2n, 2n+1) hold the per-workgroup sums of node n
     double* rowsOut = blockSums + (long)2 * njobs * nb;
     hipLaunchKernelGGL(k_reduce_rows, dim3(2 * njobs), dim3(WG), 0, st, blockSums, nb, 2 * njobs,

@@ -1,23 +1,29 @@
-"""Benchmark: BEAGLE likelihood evaluations on one MI355X.
+"""Benchmark: BEAGLE log-likelihood + gradient evaluations on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): 4-state
-GTR + Gamma(4 categories, shape 0.5), 128 taxa, 100,000 unique site
-patterns of uniform random tip states, pattern weights 1 + Poisson(2),
-branch lengths ~ Exp(mean 0.1), two tree shapes: a 127-deep caterpillar
-and a balanced tree.  Synthetic data, seed 1.
+Workload: 4-state GTR + Gamma(4 categories, shape 0.5), 128 taxa,
+100,000 unique site patterns of uniform random tip states, pattern
+weights 1 + Poisson(2), branch lengths ~ Exp(mean 0.1), two tree shapes (a
+127-deep caterpillar and a balanced tree).  Synthetic data, seed 1.
 
-One step = one full FP64 log-likelihood evaluation of each tree through
-the beagle.h API: transition matrices for all 254 branches from the
-eigen-system, the 127-operation post-order traversal, the root
-log-likelihood (FP64 sum read back to the host).  value = evaluations per
-second (2 per step).  Inputs are resident in HBM before timing starts.
+One evaluation = the log-likelihood and, for all 254 branches, dlogL/dt
+and d2logL/dt2: transition matrices from the eigen-system, the post-order
+traversal, the root log-likelihood, the root pre-order partials, the
+pre-order traversal and the two derivative sums per branch (derivative
+matrices r_c Q and r_c^2 Q^2).  One step = one evaluation of each tree.
 
-Also reported: the FP32 path with exact power-of-two rescaling (its
-relative logL error vs FP64), a logL + full gradient evaluation (pre-order
-pass plus first and second derivatives for all 254 branches), the fused
-post-order kernel's roofline (algorithmic bytes per launch over its
-duration measured with HIP events on the library's stream), and the
-reference's threaded CPU plugin timed on this host.
+value: evaluations per second of the FP32 path (partials in FP32 with
+exact power-of-two rescaling; derivative sums, scale exponents and the
+log-likelihood in FP64; pre-order fused with the derivative sums), with
+its logL and gradient error against the FP64 path in the same line.
+Also reported: the plain FP64 path through the beagle.h calls
+(beagleUpdatePrePartials + two beagleCalculateEdgeDerivatives), the FP64
+fused path, the FP64 post-order traversal's HBM bandwidth, and the
+reference's threaded CPU plugin on this host's cores.
+
+roofline: the dominant kernel of the value path, the fused FP32 pre-order
++ derivative traversal of the caterpillar (one launch of 254 operations),
+timed with HIP events on the library's stream; algorithmic bytes per
+launch from preorder_bytes().  Inputs are resident in HBM before timing.
 """
 import argparse
 import json
@@ -31,70 +37,157 @@ sys.path.insert(0, os.path.join(REPO, "beagle-lib_amd"))
 
 HBM_PEAK_GBS = 8000.0
 N_TAXA, N_PATTERNS, SEED = 128, 100_000, 1
+SHAPES = ("caterpillar", "balanced")
 METRIC = "logL+gradient evals/s & partials GB/s vs plain FP64, at ≤1e-6 rel err"
+SHORT = 2   # bytes of a power-of-two scale exponent
 
 
-def postorder_bytes(prob, w):
+def postorder_bytes(prob, w, auto=False):
     """Algorithmic HBM bytes of one post-order traversal: per operation the
     destination write C*P*S*w, each child (4*P for tip states, C*P*S*w for
-    partials) and the two C*S*S*w matrices."""
+    partials), the two C*S*(S+1)*w matrices, and with automatic rescaling
+    the children's and the destination's exponents (2 bytes per pattern)."""
     C, P, S = prob.C, prob.P, prob.S
     n = prob.tree.n
+    part = C * P * S * w
     total = 0
     for v in prob.tree.postorder():
         a, b = prob.tree.children[v]
-        total += C * P * S * w + 2 * C * S * S * w
-        total += sum(4 * P if x < n else C * P * S * w for x in (a, b))
+        total += part + 2 * C * S * (S + 1) * w
+        total += sum(4 * P if x < n else part for x in (a, b))
+        if auto:
+            total += SHORT * P * (1 + sum(1 for x in (a, b) if x >= n))
     return total
 
 
-def problems():
+def preorder_bytes(prob, ops, w, auto, post_of):
+    """Algorithmic HBM bytes of one fused pre-order + derivative launch over
+    `ops` (one fused segment).  Per operation: the parent's pre-order
+    partials unless the previous operation's output or parent is reused
+    from registers (the kernel's forwarding rule), the sibling (4*P for tip
+    states), the node's own post-order partials (derivative numerators and
+    denominator), the destination write unless it is BEAGLE_OP_NONE, four
+    C*S*(S+1)*w matrices, the pattern weights (8*P), and with automatic
+    rescaling 2 bytes per pattern for each exponent read or written."""
+    C, P, S = prob.C, prob.P, prob.S
+    n = prob.tree.n
+    part = C * P * S * w
+    total, prev = 0, None
+    for o in ops:
+        dest, par, sib = int(o[0]), int(o[3]), int(o[5])
+        own = post_of(o)
+        forwarded = prev is not None and (par == prev[0] or par == prev[3])
+        total += 0 if forwarded else part
+        total += 4 * P if sib < n else part
+        total += 4 * P if own < n else part
+        total += 4 * C * S * (S + 1) * w + 8 * P
+        if dest >= 0:
+            total += part
+        if auto:
+            total += SHORT * P * (1 + (sib >= n) + (dest >= 0))
+        prev = o
+    return total
+
+
+def problems(patterns=N_PATTERNS):
     from beagle_amd import synth
-    return [synth.make_problem(SEED, N_TAXA, N_PATTERNS, S=4, shape=s)
-            for s in ("caterpillar", "balanced")]
+    return [synth.make_problem(SEED, N_TAXA, patterns, S=4, shape=s) for s in SHAPES]
+
+
+def fused_eval(ev, st=None, timing=None):
+    """One logL + gradient evaluation through the fused extension; with
+    `timing` the extension call is bracketed by HIP events on `st`."""
+    import numpy as np
+    import torch
+    from beagle_amd import api, synth
+    logl = ev.log_likelihood()
+    ev.set_root_pre()
+    ops = synth.preorder_ops(ev.prob.tree, ev.pre_base)
+    post = ops[:, 0] - ev.pre_base
+    ops[post < ev.n, 0] = api.OP_NONE
+    if timing is not None:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+    s1, sq, s2 = ev.inst.pre_partials_with_derivatives(ops, post, [ev.d1_index] * len(post),
+                                                       [ev.d2_index] * len(post))
+    if timing is not None:
+        e1.record(st)
+        timing.append((e0, e1))
+    g1 = np.zeros(2 * ev.n - 1)
+    g2 = np.zeros(2 * ev.n - 1)
+    g1[post] = s1
+    g2[post] = s2 - sq
+    nodes = ev.branch_nodes()
+    return logl, g1[nodes], g2[nodes], ops, post
 
 
 def cpu_baseline(budget_s):
-    """Times the reference CPU plugin (threaded SSE, double) on this host's
-    cores, on the same workload, for about budget_s seconds of evaluations;
-    falls back to the oracle (single thread) if the reference build is not
-    present.  Runs in its own process (both libraries export beagle*)."""
-    import numpy as np
+    """The reference CPU plugin (threaded SSE, double) on this host's cores:
+    logL + gradient evaluations of the balanced tree through the beagle.h
+    calls (post-order, root, beagleUpdatePrePartials, two
+    beagleCalculateEdgeDerivatives) on a `sample`-pattern alignment from the
+    same generators, repeated for about budget_s seconds, scaled to 100,000
+    patterns (the work is linear in the pattern count).  Falls back to the
+    oracle (one
+    thread) without the reference build.  Runs in its own process (both
+    libraries export beagle*)."""
     from beagle_amd import api, driver
     ref_path = os.path.join(REPO, "oracle", "_ref", "libhmsbeagle.so")
     cores = min(len(os.sched_getaffinity(0)), 16)
-    prob = problems()[1]
-    if os.path.exists(ref_path):
-        lib = api.Library(ref_path, "beagle")
-        flags = api.FLAG_PROCESSOR_CPU | api.FLAG_VECTOR_SSE | api.FLAG_THREADING_CPP
-        ev = driver.Evaluator(lib, prob, preorder=False, extra_flags=flags, transpose="none")
-        lib.call("SetCPUThreadCount", ev.inst.h, cores)
-        kind, name = "reference", ev.inst.impl_name
-    else:
+
+    def make(patterns):
+        prob = problems(patterns)[1]
+        if os.path.exists(ref_path):
+            lib = api.Library(ref_path, "beagle")
+            flags = api.FLAG_PROCESSOR_CPU | api.FLAG_VECTOR_SSE | api.FLAG_THREADING_CPP
+            ev = driver.Evaluator(lib, prob, extra_flags=flags, transpose="none")
+            lib.call("SetCPUThreadCount", ev.inst.h, cores)
+            return ev, "reference", ev.inst.impl_name, cores
         lib = api.Library(os.path.join(REPO, "oracle", "liboracle.so"), "orc")
-        ev = driver.Evaluator(lib, prob, preorder=False, transpose="none")
-        kind, name, cores = "port", "oracle", 1
-    logl = ev.log_likelihood()
+        return driver.Evaluator(lib, prob, transpose="none"), "port", "oracle", 1
+
+    # size the sample from one small evaluation
+    ev, kind, name, used = make(1000)
+    t0 = time.perf_counter()
+    ev.gradient()
+    per_pattern = (time.perf_counter() - t0) / 1000
+    ev.close()
+    sample = int(min(N_PATTERNS, max(1000, budget_s / 3 / per_pattern)) // 1000 * 1000)
+    ev, kind, name, used = make(sample)
+    ev.gradient()
     t0 = time.perf_counter()
     n = 0
     while True:
-        ev.log_likelihood()
+        ev.gradient()
         n += 1
-        if time.perf_counter() - t0 > budget_s and n >= 3:
+        if time.perf_counter() - t0 > budget_s and n >= 2:
             break
     dt = time.perf_counter() - t0
     ev.close()
-    return {"value": n / dt, "unit": "evals/s", "cores": cores, "kind": kind,
-            "sample": f"{n} FP64 logL evaluations of the balanced 128-taxon, 100,000-pattern "
-                      f"tree with {name} ({dt:.1f} s)", "logL": logl}
+    rate = n / dt * sample / N_PATTERNS
+    return {"value": rate, "unit": "evals/s", "cores": used, "kind": kind,
+            "sample": f"{n} FP64 logL+gradient evaluations of the balanced 128-taxon tree, "
+                      f"{sample}-pattern alignment from the same generators, {name} ({dt:.1f} s; "
+                      f"threads set with beagleSetCPUThreadCount, the pre-order and "
+                      f"derivative calls run on one), rate scaled by {sample}/100000"}
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+def grad_err(a, b):
+    import numpy as np
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -107,8 +200,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(patterns shard across GPUs with one all-reduce per evaluation; "
-              "see DESIGN.md)", file=sys.stderr)
+              "(patterns shard across GPUs with one all-reduce of the per-pattern sums "
+              "per evaluation; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -119,88 +212,77 @@ def main():
     assert torch.cuda.is_available(), "needs an MI355X"
     lib = beagle_amd.load()
     probs = problems()
-    async_flag = api.FLAG_COMPUTATION_ASYNCH
+    asyn = api.FLAG_COMPUTATION_ASYNCH
 
-    # ---- FP64 plain path (the timed metric) ----
-    evs = [driver.Evaluator(lib, p, preorder=False, extra_flags=async_flag) for p in probs]
+    # ---- value: FP32 fused logL + gradient, both trees ----
+    evs = [driver.Evaluator(lib, p, precision="single", scaling="auto", extra_flags=asyn)
+           for p in probs]
     streams = [torch.cuda.ExternalStream(ev.inst.stream()) for ev in evs]
     for _ in range(args.warmup):
         for ev in evs:
-            ev.log_likelihood()
-    logl64 = [ev.log_likelihood() for ev in evs]
-    ev_pairs = []
+            fused_eval(ev)
+    timing = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        for ev, st in zip(evs, streams):
-            ev.update_matrices()
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(st)
-            ev.post_order()
-            e1.record(st)
-            ev.root()
-            ev_pairs.append((e0, e1))
+        res32 = [fused_eval(evs[0], streams[0], timing), fused_eval(evs[1])]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms_per_step = dt * 1e3 / args.steps
-    evals_per_s = 2 * args.steps / dt
-    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in ev_pairs]))
-    bytes64 = postorder_bytes(probs[0], 8)
-    achieved = bytes64 / (kernel_ms * 1e-3) / 1e9
+    value = 2 * args.steps / dt
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in timing]))
+    cat_ops, cat_post = res32[0][3], res32[0][4]
+    post_idx = dict(zip(map(tuple, cat_ops.tolist()), cat_post.tolist()))
+    pre_bytes = preorder_bytes(probs[0], cat_ops, 4, True, lambda o: post_idx[tuple(o.tolist())])
+    achieved = pre_bytes / (kernel_ms * 1e-3) / 1e9
     for ev in evs:
         ev.close()
 
-    # ---- FP32 with exact power-of-two rescaling ----
-    evs = [driver.Evaluator(lib, p, preorder=False, precision="single", scaling="auto",
-                            extra_flags=async_flag) for p in probs]
-    for _ in range(args.warmup):
-        for ev in evs:
-            ev.log_likelihood()
-    logl32 = [ev.log_likelihood() for ev in evs]
-    n32 = max(args.steps // 2, 5)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(n32):
-        for ev in evs:
-            ev.log_likelihood()
-    torch.cuda.synchronize()
-    fp32_eps = 2 * n32 / (time.perf_counter() - t0)
-    for ev in evs:
-        ev.close()
-    err32 = max(abs(a - b) / abs(b) for a, b in zip(logl32, logl64))
-
-    # ---- logL + full gradient (FP64): API call sequence, and the fused
-    # pre-order + derivative extension ----
-    ev = driver.Evaluator(lib, probs[1], extra_flags=async_flag)
-    ng = max(args.steps // 5, 3)
-    rates = {}
-    for name, fn in (("api", ev.gradient), ("fused", ev.fused_gradient)):
+    # ---- FP64: plain API path and fused path (errors are taken against
+    # the plain path) ----
+    n64 = max(args.steps // 3, 3)
+    rates, res64 = {}, {}
+    for name in ("api", "fused"):
+        evs = [driver.Evaluator(lib, p, extra_flags=asyn) for p in probs]
+        fn = (lambda ev: ev.gradient()) if name == "api" else (lambda ev: fused_eval(ev)[:3])
         for _ in range(2):
-            fn()
+            for ev in evs:
+                fn(ev)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(ng):
-            res = fn()
+        for _ in range(n64):
+            res64[name] = [fn(ev) for ev in evs]
         torch.cuda.synchronize()
-        rates[name] = (ng / (time.perf_counter() - t0), res)
-    g_api, g_fused = rates["api"][1], rates["fused"][1]
-    grad_err = max(float(np.max(np.abs(g_api[1] - g_fused[1]) / np.maximum(np.abs(g_api[1]), 1e-12))),
-                   float(np.max(np.abs(g_api[2] - g_fused[2]) / np.maximum(np.abs(g_api[2]), 1e-12))))
-    ev.close()
-    # FP32 auto-scaled fused gradient vs the FP64 one
-    ev = driver.Evaluator(lib, probs[1], extra_flags=async_flag, precision="single",
-                          scaling="auto")
-    for _ in range(2):
-        ev.fused_gradient()
+        rates[name] = 2 * n64 / (time.perf_counter() - t0)
+        for ev in evs:
+            ev.close()
+    api64 = res64["api"]
+    err = {"logL": max(rel(a[0], b[0]) for a, b in zip(res32, api64)),
+           "dlogL_dt": max(grad_err(a[1], b[1]) for a, b in zip(res32, api64)),
+           "d2logL_dt2": max(grad_err(a[2], b[2]) for a, b in zip(res32, api64))}
+    fused64_diff = max(max(grad_err(a[1], b[1]), grad_err(a[2], b[2]), rel(a[0], b[0]))
+                       for a, b in zip(res64["fused"], api64))
+
+    # ---- plain FP64 post-order traversal bandwidth (caterpillar: one
+    # fused launch of 127 operations) ----
+    ev = driver.Evaluator(lib, probs[0], preorder=False, extra_flags=asyn)
+    st = torch.cuda.ExternalStream(ev.inst.stream())
+    for _ in range(3):
+        ev.log_likelihood()
+    pairs = []
+    for _ in range(max(args.steps // 2, 5)):
+        ev.update_matrices()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        ev.post_order()
+        e1.record(st)
+        ev.root()
+        pairs.append((e0, e1))
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(ng):
-        g32 = ev.fused_gradient()
-    torch.cuda.synchronize()
-    grad32_eps = ng / (time.perf_counter() - t0)
+    post_ms = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
+    post_gbs = postorder_bytes(probs[0], 8) / (post_ms * 1e-3) / 1e9
     ev.close()
-    grad32_err = float(np.max(np.abs(g32[1] - g_fused[1])) / np.max(np.abs(g_fused[1])))
 
     if args.calibrate_copy:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
@@ -214,21 +296,20 @@ def main():
     if not args.no_cpu_baseline:
         r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only",
                             "--cpu-budget", str(args.cpu_budget)], capture_output=True, text=True,
-                           timeout=600)
+                           timeout=900)
         if r.returncode == 0:
             cpu = json.loads(r.stdout.strip().splitlines()[-1])
-            cpu["logL_rel_err_vs_gpu"] = abs(cpu.pop("logL") - logl64[1]) / abs(logl64[1])
         else:
             print(r.stderr[-2000:], file=sys.stderr)
 
     traffic = None
-    tpath = os.path.join(REPO, "profiles", "post4_fp64_traffic.json")
+    tpath = os.path.join(REPO, "profiles", "pre4_fp32_caterpillar_traffic.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
 
     out = {
         "metric": METRIC,
-        "value": evals_per_s,
+        "value": value,
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -237,29 +318,33 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic (seeded generators; no dataset)",
         "config": {"workload": "4-state GTR+G4, 128 taxa, 100,000 patterns, caterpillar + "
-                               "balanced trees, FP64 post-order + root logL",
+                               "balanced trees; logL + dlogL/dt and d2logL/dt2 of all 254 "
+                               "branches per evaluation",
                    "taxa": N_TAXA, "patterns": N_PATTERNS, "categories": 4,
-                   "evals_per_step": 2},
+                   "evals_per_step": 2,
+                   "path": "FP32 partials, power-of-two rescaling, fused pre-order + "
+                           "derivative extension; FP64 sums"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_post4<double> (fused 127-op post-order traversal)",
-                     "kernel_ms": kernel_ms, "algorithmic_bytes": bytes64},
-        "partials_GBps_fp64": achieved,
-        "fp32_auto": {"evals_per_s": fp32_eps, "speedup_vs_fp64": fp32_eps / evals_per_s,
-                      "logL_rel_err_vs_fp64": err32},
-        "logL_plus_gradient": {
-            "what": "post-order + root + pre-order + dlogL/dt and d2logL/dt2 for all 254 "
-                    "branches, balanced tree",
-            "fp64_api_evals_per_s": rates["api"][0],
-            "fp64_fused_evals_per_s": rates["fused"][0],
-            "fused_vs_api_max_rel_diff": grad_err,
-            "fp32_auto_fused_evals_per_s": grad32_eps,
-            "fp32_gradient_max_err_rel_to_max": grad32_err,
-            "speedup_fp32_fused_vs_fp64_api": grad32_eps / rates["api"][0]},
-        "logL": logl64,
+                     "kernel": "k_pre4<float,true,true>: fused FP32 pre-order + derivative "
+                               "traversal of the caterpillar (254 operations, one launch)",
+                     "kernel_ms": kernel_ms, "algorithmic_bytes": pre_bytes},
+        "error_vs_fp64": err,
+        "plain_fp64": {
+            "api_evals_per_s": rates["api"],
+            "what": "beagle.h calls: update matrices, post-order, root, "
+                    "beagleUpdatePrePartials, two beagleCalculateEdgeDerivatives",
+            "speedup_of_value": value / rates["api"],
+            "fused_evals_per_s": rates["fused"],
+            "fused_vs_api_max_rel_diff": fused64_diff,
+            "postorder_GBps": post_gbs,
+            "postorder_frac_of_hbm_peak": post_gbs / HBM_PEAK_GBS,
+            "postorder_ms": post_ms,
+            "postorder_kernel": "k_post4<double,false>: caterpillar, 127 operations, one launch"},
+        "logL": [r[0] for r in api64],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

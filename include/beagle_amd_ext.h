@@ -33,6 +33,11 @@ int beagleExtGetStream(int instance, void** outStream);
  * outSumSecondDerivatives[o] - outSumSquaredFirstDerivatives[o].  For
  * 4 states the derivative sums are computed inside the pre-order traversal
  * (post- and pre-order partials read once).  Output pointers may be NULL.
+ * An operation whose destinationPartials is BEAGLE_OP_NONE yields only its
+ * derivative sums and keeps no pre-order partials (for branches to tips,
+ * whose pre-order partials no later operation reads); this needs the
+ * fused 4-state path (instances with 4 states and at most 8 categories)
+ * and returns BEAGLE_ERROR_OUT_OF_RANGE elsewhere.
  * No counterpart in the reference. */
 int beagleExtUpdatePrePartialsWithDerivatives(int instance, const BeagleOperation* operations,
                                               int operationCount, const int* postBufferIndices,

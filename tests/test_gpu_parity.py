@@ -213,24 +213,28 @@ def test_partitions_match_oracle(amd, oracle):
     assert rel(out[0][1], out[1][1]) < RTOL
 
 
-@pytest.mark.parametrize("S,shape,scaling,precision", [(4, "yule", "none", "double"),
-                                                       (4, "balanced", "manual", "double"),
-                                                       (4, "caterpillar", "auto", "double"),
-                                                       (4, "yule", "auto", "single"),
-                                                       (20, "yule", "none", "double")])
-def test_fused_pre_order_derivatives_match_oracle(amd, oracle, S, shape, scaling, precision):
+@pytest.mark.parametrize("S,shape,scaling,precision,keep", [(4, "yule", "none", "double", True),
+                                                            (4, "balanced", "manual", "double", False),
+                                                            (4, "caterpillar", "auto", "double", True),
+                                                            (4, "yule", "auto", "single", False),
+                                                            (4, "balanced", "auto", "single", True),
+                                                            (20, "yule", "none", "double", True)])
+def test_fused_pre_order_derivatives_match_oracle(amd, oracle, S, shape, scaling, precision, keep):
     """Extension beagleExtUpdatePrePartialsWithDerivatives against the
     oracle's beagleUpdatePrePartials + two beagleCalculateEdgeDerivatives
-    calls, per branch: dlogL/dt and d2logL/dt2."""
+    calls, per branch: dlogL/dt and d2logL/dt2; with keep, also the tips'
+    pre-order partials (otherwise the tips' operations have no
+    destination)."""
     prob = synth.make_problem(77 + S, 20, 1500 if S == 4 else 200, S=S, shape=shape,
                               missing=0.05)
     ev = driver.Evaluator(amd, prob, scaling=scaling, precision=precision, matrices="explicit")
-    lg, g1, g2 = ev.fused_gradient()
-    pre_tip = ev.inst.get_partials(ev.pre_base + 3)
+    lg, g1, g2 = ev.fused_gradient(keep_tips=keep)
+    pre_tip = ev.inst.get_partials(ev.pre_base + 3) if keep else None
     ev.close()
     ev = driver.Evaluator(oracle, prob, scaling="manual" if scaling == "auto" else scaling,
                           matrices="explicit")
     lo, o1, o2 = ev.gradient()
+    pre_tip_o = ev.inst.get_partials(ev.pre_base + 3)
     ev.close()
     if precision == "double":
         assert rel(lg, lo) < 1e-12
@@ -240,3 +244,28 @@ def test_fused_pre_order_derivatives_match_oracle(amd, oracle, S, shape, scaling
         assert rel(lg, lo) < 1e-6
         close(g1, o1, 1e-3, 1e-2)
         close(g2, o2, 1e-3, 1e-1)
+    if keep and scaling == "none":
+        close(pre_tip, pre_tip_o, 1e-10 if precision == "double" else 1e-4, 0)
+
+
+@pytest.mark.parametrize("mode", ["1", "2"])
+@pytest.mark.parametrize("precision,scaling", [("double", "none"), ("double", "always"),
+                                               ("single", "auto")])
+def test_traversal_plans_agree(amd, monkeypatch, mode, precision, scaling):
+    """Every dependency level batched (mode 1) and the whole list fused
+    (mode 2) give the automatic plan's logL, pre-order partials and
+    derivative sums bit for bit on a balanced tree (wide levels) and a
+    caterpillar (one op per level)."""
+    for shape in ("balanced", "caterpillar"):
+        prob = synth.make_problem(5, 32, 3000, S=4, shape=shape)
+        res = []
+        for m in ("0", mode):
+            monkeypatch.setenv("BEAGLE_AMD_TRAVERSAL", m)
+            ev = driver.Evaluator(amd, prob, scaling=scaling, precision=precision)
+            lg, g1, g2 = ev.fused_gradient(keep_tips=True)
+            ev.pre_order()
+            res.append((lg, g1, g2, ev.inst.get_partials(ev.pre_base + 1),
+                        ev.inst.get_partials(ev.pre_base + prob.tree.n + 2)))
+            ev.close()
+        for a, b in zip(res[0], res[1]):
+            np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
