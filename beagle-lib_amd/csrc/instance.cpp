@@ -2,11 +2,13 @@
 #include "instance.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <new>
 #include <stdexcept>
+#include <unordered_map>
 
 #include "beagle.h"
 
@@ -222,6 +224,7 @@ Instance::~Instance() {
     hipFree(dDummy_);
     hipHostFree(hResult_);
     hipFree(dWork_);
+    hipFree(dJobMats_);
     hipFree(dScratch_);
     for (auto& s : ring_) {
         if (s.host) hipHostFree(s.host);
@@ -960,7 +963,7 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
     double* wk = reinterpret_cast<double*>(grow(dWork_, workCap_, (nPart + 3 * (size_t)count) * sizeof(double)));
     double* sums = wk + nPart;
     preWeights_ = wIdx;
-    launchPre(ops, BEAGLE_OP_COUNT, count, wk);
+    if (!launchPreNodes(ops, count, postIdx, wk)) launchPre(ops, BEAGLE_OP_COUNT, count, wk);
     launch_reduce_rows(wk, ntiles, 3 * count, sums, stream_);
     HIPCHECK(hipGetLastError());
     std::vector<double> h(3 * (size_t)count);
@@ -971,6 +974,104 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
         if (outD2) outD2[o] = h[3 * o + 2];
     }
     return BEAGLE_SUCCESS;
+}
+
+// Node-fused path of the derivative extension: the two operations of each
+// node's children (same parent, each the other's sibling, matrices
+// crossed) become one job that reads pre(v), x_a and x_b once.  Returns
+// false, launching nothing, when some operation has no such partner or
+// writes scale factors.  Jobs are planned like operations: wide
+// dependency levels batched, narrow runs fused with the parent forwarded
+// from registers.
+bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, double* partial) {
+    std::unordered_map<int, int> open;   // parent buffer -> operation awaiting its sibling
+    std::vector<std::array<int, 2>> pairs;
+    for (int o = 0; o < count; ++o) {
+        if (hPre_[o].mode != SCALE_NONE) return false;
+        const int* q = ops + (size_t)o * BEAGLE_OP_COUNT;
+        auto it = open.find(q[3]);
+        if (it == open.end()) {
+            open.emplace(q[3], o);
+            continue;
+        }
+        const int a = it->second;
+        const int* r = ops + (size_t)a * BEAGLE_OP_COUNT;
+        if (r[5] != postIdx[o] || q[5] != postIdx[a] || r[4] != q[6] || r[6] != q[4]) return false;
+        pairs.push_back({a, o});
+        open.erase(it);
+    }
+    if (!open.empty() || pairs.empty()) return false;
+
+    const int n = (int)pairs.size();
+    std::vector<NodeJob> nodes(n);
+    std::vector<Access> acc(n);
+    for (int j = 0; j < n; ++j) {
+        const PreJob& A = hPre_[pairs[j][0]];
+        const PreJob& B = hPre_[pairs[j][1]];
+        NodeJob& N = nodes[j];
+        std::memset(&N, 0, sizeof(NodeJob));
+        N.parent = A.parent;
+        N.x[0] = A.xself ? A.xself : (const void*)A.sself;
+        N.k[0] = A.kSelf;
+        N.x[1] = B.xself ? B.xself : (const void*)B.sself;
+        N.k[1] = B.kSelf;
+        N.dest[0] = A.dest;
+        N.dest[1] = B.dest;
+        N.eParent = A.e1;
+        N.ex[0] = B.e2;   // a is b's sibling
+        N.ex[1] = A.e2;
+        N.ed[0] = A.ed;
+        N.ed[1] = B.ed;
+        N.moP[0] = A.moSelf;
+        N.moP[1] = B.moSelf;
+        N.moD1[0] = A.moD1;
+        N.moD2[0] = A.moD2;
+        N.moD1[1] = B.moD1;
+        N.moD2[1] = B.moD2;
+        N.oidx[0] = pairs[j][0];
+        N.oidx[1] = pairs[j][1];
+        const int* qa = ops + (size_t)pairs[j][0] * BEAGLE_OP_COUNT;
+        const int* qb = ops + (size_t)pairs[j][1] * BEAGLE_OP_COUNT;
+        acc[j] = Access{{qa[3], -1, -1}, {qa[0], qb[0]}};
+    }
+    std::vector<int> level;
+    const int nl = dependencyLevels(acc, bufferCount_, level);
+    std::vector<Segment> plan;
+    if (traversalMode_ == 2) {
+        Segment s{false, {}};
+        for (int j = 0; j < n; ++j) s.ops.push_back(j);
+        plan.push_back(std::move(s));
+    } else {
+        plan = planSegments(level, nl, traversalMode_ == 1 ? 1 : kNarrowLevel);
+    }
+    std::vector<NodeJob> all;
+    struct Launch { int first, n; bool batched; };
+    std::vector<Launch> launches;
+    for (const Segment& s : plan) {
+        const int first = (int)all.size();
+        for (int j : s.ops) all.push_back(nodes[j]);
+        if (!s.batched) {
+            for (size_t j = first + 1; j < all.size(); ++j) {
+                NodeJob& J = all[j];
+                const NodeJob& Pv = all[j - 1];
+                if (Pv.dest[0] && J.parent == Pv.dest[0]) J.kPar = 2;
+                else if (Pv.dest[1] && J.parent == Pv.dest[1]) J.kPar = 3;
+            }
+        }
+        launches.push_back({first, (int)s.ops.size(), s.batched});
+    }
+    const double* cw = dCatWeights_ + (size_t)preWeights_ * dims_.C;
+    const size_t perJob = (size_t)dims_.C * 6 * 20;
+    for (size_t j = 0; j < all.size(); ++j) all[j].mbase = (int)(j * perJob);
+    void* jm = grow(dJobMats_, jobMatsCap_, all.size() * perJob * realSize());
+    auto d = reinterpret_cast<const NodeJob*>(upload(all.data(), all.size() * sizeof(NodeJob)));
+    if (dp_) launch_gather_node_matrices<double>(d, (int)all.size(), dims_, dMatrices_, jm, stream_);
+    else launch_gather_node_matrices<float>(d, (int)all.size(), dims_, dMatrices_, jm, stream_);
+    for (const Launch& L : launches) {
+        if (dp_) launch_pre_nodes<double>(d + L.first, L.n, L.batched, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
+        else launch_pre_nodes<float>(d + L.first, L.n, L.batched, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
+    }
+    return true;
 }
 
 int Instance::waitForPartials() {

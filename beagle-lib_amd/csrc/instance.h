@@ -130,6 +130,7 @@ private:
     int buildPreJobs(const int* ops, int count, bool byPartition, bool allowNoDest);
     void launchPre(const int* ops, int stride, int count, double* partial);
     void launchPostPlan(const std::vector<Segment>& plan);
+    bool launchPreNodes(const int* ops, int count, const int* postIdx, double* partial);
     static constexpr int kNarrowLevel = 8;   // levels with fewer ops run fused
     bool validBuffer(int b) const { return b >= 0 && b < bufferCount_; }
     bool validMatrix(int m) const { return m >= 0 && m < matrixCount_; }
@@ -172,6 +173,8 @@ private:
     double* hResult_ = nullptr;       // pinned mirror
     void* dWork_ = nullptr;           // grow-only reduction scratch
     size_t workCap_ = 0;
+    void* dJobMats_ = nullptr;        // grow-only gathered matrices of node jobs
+    size_t jobMatsCap_ = 0;
     void* dScratch_ = nullptr;        // C*Ppad*Sp reals (generic pre-order)
     std::vector<int> partStart_;
     int partitionCount_ = 1;

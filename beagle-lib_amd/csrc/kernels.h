@@ -62,6 +62,25 @@ struct PreJob {
     int oidx;                  // row of this operation in the derivative output
 };
 
+// The two pre-order operations of one node's children a (slot 0) and b
+// (slot 1), fused with both branches' derivative sums:
+//   pre(a) = P_a^T (pre(v) * (P_b x_b)),  pre(b) = P_b^T (pre(v) * (P_a x_a))
+// where x_a, x_b are the children's post-order partials (or tip states).
+// Each of pre(v), x_a and x_b is read once for both operations.
+struct NodeJob {
+    const void* parent;        // pre-order partials of the node (kPar 0)
+    const void* x[2];          // children's post-order partials, or tip states (k == 1)
+    void* dest[2];             // children's pre-order partials; null: not kept
+    const short* eParent;      // automatic scaling exponents: the node's pre-order
+    const short* ex[2];        //   partials, the children's post-order partials
+    short* ed[2];              //   (null for tips), the children's pre-order partials
+    int k[2];                  // 0 partials, 1 tip states
+    int kPar;                  // 0 load; 2, 3: previous job's dest[0], dest[1] (registers)
+    int moP[2], moD1[2], moD2[2];  // matrix element offsets in the matrix slab
+    int oidx[2];               // operation index of each child in the derivative output
+    int mbase;                 // this job's block in the gathered matrices (reals)
+};
+
 // One transition-matrix update: P, dP/dt, d2P/dt2 for all categories.
 struct MatrixJob {
     void* P; void* D1; void* D2;      // D1/D2 may be null
@@ -117,6 +136,19 @@ template <typename R>
 void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, const Dims& d,
                        const void* mats, void* dummy, const double* cw, const double* pw,
                        double* partial, hipStream_t st);
+// Node-fused pre-order + derivative traversal (4 states): batched == true
+// runs the njobs jobs (one dependency level) in one grid over (tiles, jobs),
+// otherwise one launch walks the list in order.
+template <typename R>
+void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, const Dims& d,
+                      const void* jobMats, void* dummy, const double* cw, const double* pw,
+                      double* partial, hipStream_t st);
+// Copies each node job's six matrices (P_a, P_b, D1_a, D2_a, D1_b, D2_b) into
+// jobMats as [job][category][6][20] reals, so a job's matrices for one
+// category are contiguous; job j's block starts at djobs[j].mbase.
+template <typename R>
+void launch_gather_node_matrices(const NodeJob* djobs, int njobs, const Dims& d,
+                                 const void* mats, void* jobMats, hipStream_t st);
 template <typename R>
 void launch_root_range(const void* root, const short* rootExp, const double* catWeights,
                        const double* freqs, const double* cumulative,

@@ -506,6 +506,28 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// v moved across lanes by a DPP control; lanes without a source read 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave with DPP steps (quad swaps, row shifts,
+// row broadcasts); the total ends in lane 63 and is returned to all lanes.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v += dpp_move<0xB1, 0xf>(v);    // quad_perm [1,0,3,2]
+    v += dpp_move<0x4E, 0xf>(v);    // quad_perm [2,3,0,1]: every lane holds its quad's sum
+    v += dpp_move<0x114, 0xf>(v);   // row_shr:4
+    v += dpp_move<0x118, 0xf>(v);   // row_shr:8: lanes 12-15 of each row hold the row sum
+    v += dpp_move<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3: lane 63 holds the total
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+
 template <typename R, bool AUTO, bool DERIV>
 __device__ __forceinline__ void pre_compute(const PreJob& J, const PreOps4<R>& o, R (&prev)[4],
                                             R (&uprev)[4], const R* __restrict__ mats,
@@ -669,6 +691,290 @@ __global__ __launch_bounds__(512) void k_pre4_level(const PreJob* __restrict__ j
     pre_issue<R, DERIV>(J, dummyX, dummyS, off, doff, p, lane, S0);
     pre_compute<R, AUTO, DERIV>(J, S0, prev, uprev, mats, sdv, k, c, lane, p, P, C, off, doff,
                                 dummyW, dummyE, cw, pw, partial, tile, ntiles);
+}
+
+// ---------------------------------------------------------------------------
+// Node-fused pre-order + derivatives (4 states).  One job per internal node
+// v with children a, b: pre(v), x_a and x_b are loaded once (pre(v) comes
+// from registers when the previous job produced it) and both children's
+// pre-order partials and branch derivative sums are formed.  Pipelining as
+// in k_post4: fixed loads into alternating register sets, unused slots read
+// the dummy tile, stores of lanes past P or of outputs that are not kept go
+// to the dummy tile.  With automatic scaling the exponents are loaded with
+// the operands, one job ahead, and the pattern weight is loaded once.
+
+// One 16-byte vector of reals.
+template <typename R>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef fvec4 T;
+    static constexpr int N = 4;
+};
+template <>
+struct Vec16<double> {
+    typedef dvec2 T;
+    static constexpr int N = 2;
+};
+
+// A node job's six matrices for one category (P_a, P_b, D1_a, D2_a, D1_b,
+// D2_b; 20 reals each, contiguous after launch_gather_node_matrices) are
+// loaded one job ahead as one 16-byte vector per lane, staged in the
+// wave's own LDS block and read back as broadcasts, which keeps them out
+// of scalar registers.  Every lane stores its vector (the block holds 64 of
+// them; lanes past the matrices repeat the last one) so no branch
+// separates the lanes.
+constexpr int NODE_MATS = 6 * 20;
+constexpr int NODE_MAT_BLOCK = 64 * 16;   // bytes of one wave's block
+
+template <typename R>
+struct NodeOps4 {
+    R u[4];             // parent pre-order partials (kPar 0)
+    Operand4<R> A, B;   // children's post-order operands
+    int eu, ea, eb;     // exponents (automatic scaling)
+    typename Vec16<R>::T mv;   // this lane's share of the job's matrices
+};
+
+template <typename R, bool AUTO>
+__device__ __forceinline__ void node_issue(const NodeJob& J, const R* dx, const int* ds,
+                                           const short* dE, const R* __restrict__ jm, int c,
+                                           long off, long doff, int p, int lane,
+                                           NodeOps4<R>& o) {
+    const R* up = J.kPar == 0 ? reinterpret_cast<const R*>(J.parent) + off : dx + doff;
+    load4(gin<R>(up), o.u);
+    issue4(J.x[0], reinterpret_cast<const int*>(J.x[0]), J.k[0], dx, ds, off, doff, p, lane, o.A);
+    issue4(J.x[1], reinterpret_cast<const int*>(J.x[1]), J.k[1], dx, ds, off, doff, p, lane, o.B);
+    {
+        constexpr int V = Vec16<R>::N;
+        const int l = min(lane, NODE_MATS / V - 1);
+        o.mv = *(const GLOBAL typename Vec16<R>::T*)gin<R>(jm + J.mbase + c * NODE_MATS + l * V);
+    }
+    if (AUTO) {
+        const short* a = (J.kPar == 0 && J.eParent) ? J.eParent + p : dE + lane;
+        const short* b = J.ex[0] ? J.ex[0] + p : dE + lane;
+        const short* c = J.ex[1] ? J.ex[1] + p : dE + lane;
+        o.eu = gin<short>(a)[0];
+        o.ea = gin<short>(b)[0];
+        o.eb = gin<short>(c)[0];
+    }
+}
+
+// Derivative numerators and denominator of one branch for one category:
+// den = pre . x (a tip: pre[state % 4], as the reference computes it),
+// n1 = pre . (D1 x), n2 = pre . (D2 x).
+template <typename R>
+__device__ __forceinline__ void branch_sums(int kind, int st, const R (&pre)[4], const R (&x)[5],
+                                            const R* d1, const R* d2, double& den, double& n1,
+                                            double& n2) {
+    R t1[4], t2[4];
+    mat4(d1, x, t1);
+    mat4(d2, x, t2);
+    if (kind == 1) {
+        // one-hot weights, not an index: an indexed pick is lowered to a
+        // private-memory load
+        const int s = st & 3;
+        den = (double)pre[0] * (double)(s == 0) + (double)pre[1] * (double)(s == 1) +
+              (double)pre[2] * (double)(s == 2) + (double)pre[3] * (double)(s == 3);
+    } else {
+        den = (double)pre[0] * (double)x[0] + (double)pre[1] * (double)x[1] +
+              (double)pre[2] * (double)x[2] + (double)pre[3] * (double)x[3];
+    }
+    n1 = 0.0;
+    n2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        n1 += (double)pre[i] * (double)t1[i];
+        n2 += (double)pre[i] * (double)t2[i];
+    }
+}
+
+template <typename R, bool AUTO>
+__device__ __forceinline__ void node_compute(const NodeJob& J, const NodeOps4<R>& o, R (&pa)[4],
+                                             R (&pb)[4], int& pea, int& peb, R* smat,
+                                             double* sdv, int& k,
+                                             int c, int lane, int p, int P, int C, long off,
+                                             long doff, R* dummyW, const short* dummyE,
+                                             const double* __restrict__ cw, double w,
+                                             double* partial, int tile, int ntiles) {
+    const bool in = p < P;
+    R u[4], xa[5], xb[5], ya[4], yb[4], va[4], vb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) u[i] = J.kPar == 2 ? pa[i] : (J.kPar == 3 ? pb[i] : o.u[i]);
+    const int eu = J.kPar == 2 ? pea : (J.kPar == 3 ? peb : o.eu);
+    operand4(J.k[0], o.A, pa, xa);   // kinds 0 and 1 only: pa is never selected
+    operand4(J.k[1], o.B, pa, xb);
+    // this wave's LDS block for the job's matrices (two alternating blocks)
+    R* lm = smat + (c * 2 + (k & 1)) * (NODE_MAT_BLOCK / (int)sizeof(R));
+    *(typename Vec16<R>::T*)(lm + lane * Vec16<R>::N) = o.mv;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    const R* Pa = lm;
+    const R* Pb = lm + 20;
+    __builtin_amdgcn_sched_barrier(0);
+    mat4(Pa, xa, ya);
+    mat4(Pb, xb, yb);
+    R ua[4], ub[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        ua[i] = u[i] * yb[i];
+        ub[i] = u[i] * ya[i];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        va[j] = Pa[0 * 5 + j] * ua[0] + Pa[1 * 5 + j] * ua[1] + Pa[2 * 5 + j] * ua[2] +
+                Pa[3 * 5 + j] * ua[3];
+        vb[j] = Pb[0 * 5 + j] * ub[0] + Pb[1 * 5 + j] * ub[1] + Pb[2 * 5 + j] * ub[2] +
+                Pb[3 * 5 + j] * ub[3];
+    }
+    // derivative sums of both branches, formed before rescaling (the
+    // ratios do not depend on it); the scheduling barriers keep one stage's
+    // matrices in registers at a time
+    double dA, n1A, n2A, dB, n1B, n2B;
+    __builtin_amdgcn_sched_barrier(0);
+    branch_sums(J.k[0], o.A.st, va, xa, lm + 40, lm + 60, dA, n1A, n2A);
+    __builtin_amdgcn_sched_barrier(0);
+    branch_sums(J.k[1], o.B.st, vb, xb, lm + 80, lm + 100, dB, n1B, n2B);
+    __builtin_amdgcn_sched_barrier(0);
+    const double wc = cw[c];
+    // LDS exchange [2][C][8][TILE]: max a, max b, den/n1/n2 of a, of b
+    double* buf = sdv + (k & 1) * C * 8 * TILE;
+    ++k;
+    double* mine = buf + c * 8 * TILE + lane;
+    mine[0 * TILE] = in ? (double)fmax(fmax(va[0], va[1]), fmax(va[2], va[3])) : 0.0;
+    mine[1 * TILE] = in ? (double)fmax(fmax(vb[0], vb[1]), fmax(vb[2], vb[3])) : 0.0;
+    mine[2 * TILE] = wc * dA;
+    mine[3 * TILE] = wc * n1A;
+    mine[4 * TILE] = wc * n2A;
+    mine[5 * TILE] = wc * dB;
+    mine[6 * TILE] = wc * n1B;
+    mine[7 * TILE] = wc * n2B;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    const bool keepA = in && J.dest[0] != nullptr;
+    const bool keepB = in && J.dest[1] != nullptr;
+    int outA = 0, outB = 0;
+    if (AUTO) {
+        double ma = buf[lane], mb = buf[TILE + lane];
+        for (int cc = 1; cc < C; ++cc) {
+            ma = fmax(ma, buf[cc * 8 * TILE + lane]);
+            mb = fmax(mb, buf[cc * 8 * TILE + TILE + lane]);
+        }
+        const int ea = ma > 0.0 ? exponent_of((R)ma) : 0;
+        const int eb = mb > 0.0 ? exponent_of((R)mb) : 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            va[i] = pow2(va[i], -ea);
+            vb[i] = pow2(vb[i], -eb);
+        }
+        // pre(a) carries the exponents of pre(v) and of x_b; pre(b) those of
+        // pre(v) and x_a
+        outA = ea + eu + (J.ex[1] ? o.eb : 0);
+        outB = eb + eu + (J.ex[0] ? o.ea : 0);
+        short* da = keepA ? J.ed[0] + p : const_cast<short*>(dummyE) + 64 + lane;
+        short* db = keepB ? J.ed[1] + p : const_cast<short*>(dummyE) + 64 + lane;
+        *gout<short>(da) = (short)outA;
+        *gout<short>(db) = (short)outB;
+    }
+    // per-tile sums of child `ch` by wave ch % C: w r1, w r1^2, w r2 with
+    // r = N / D summed over categories
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+        if (c != ch % C) continue;
+        double D = 0, N1 = 0, N2 = 0;
+        for (int cc = 0; cc < C; ++cc) {
+            const double* q = buf + cc * 8 * TILE + (2 + 3 * ch) * TILE + lane;
+            D += q[0];
+            N1 += q[TILE];
+            N2 += q[2 * TILE];
+        }
+        const double r1 = in ? N1 / D : 0.0, r2 = in ? N2 / D : 0.0;
+        const double s1 = wave_sum_dpp(w * r1), sq = wave_sum_dpp(w * r1 * r1),
+                     s2 = wave_sum_dpp(w * r2);
+        if (lane == 0) {
+            GLOBAL double* pt = gout<double>(partial);
+            const long row = (long)J.oidx[ch] * 3;
+            pt[(row + 0) * ntiles + tile] = s1;
+            pt[(row + 1) * ntiles + tile] = sq;
+            pt[(row + 2) * ntiles + tile] = s2;
+        }
+    }
+    R* dstA = keepA ? reinterpret_cast<R*>(J.dest[0]) + off : dummyW + doff;
+    R* dstB = keepB ? reinterpret_cast<R*>(J.dest[1]) + off : dummyW + doff;
+    store4(gout<R>(dstA), va);
+    store4(gout<R>(dstB), vb);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        pa[i] = va[i];
+        pb[i] = vb[i];
+    }
+    pea = outA;
+    peb = outB;
+}
+
+template <typename R, bool AUTO>
+__global__ __launch_bounds__(512) void k_pren4(const NodeJob* __restrict__ jobs, int njobs, int P,
+                                               int C, long cs, const R* __restrict__ jm,
+                                               const R* dummyX, const int* dummyS, R* dummyW,
+                                               const short* dummyE, const double* __restrict__ cw,
+                                               const double* __restrict__ pw, double* partial) {
+    extern __shared__ double sdv[];   // [2][C][8][TILE] exchange, then [C][2] matrix blocks
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x;
+    const int p = tile * TILE + lane;   // < Ppad: pattern weights are zero-padded
+    R* smat = reinterpret_cast<R*>(sdv + 2 * C * 8 * TILE);
+    const long off = c * cs + (long)p * 4;
+    const long doff = (long)(c * TILE + lane) * 4;
+    const int ntiles = (P + TILE - 1) / TILE;
+    const double w = pw[p];
+    int k = 0, pea = 0, peb = 0;
+    R pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
+    NodeOps4<R> S0, S1;
+    NodeJob J0 = jobs[0], J1;
+    node_issue<R, AUTO>(J0, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
+    for (int o = 0;; o += 2) {
+        J1 = jobs[min(o + 1, njobs - 1)];
+        node_issue<R, AUTO>(J1, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S1);
+        node_compute<R, AUTO>(J0, S0, pa, pb, pea, peb, smat, sdv, k, c, lane, p, P, C, off, doff,
+                              dummyW, dummyE, cw, w, partial, tile, ntiles);
+        if (o + 1 >= njobs) break;
+        J0 = jobs[min(o + 2, njobs - 1)];
+        node_issue<R, AUTO>(J0, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
+        node_compute<R, AUTO>(J1, S1, pa, pb, pea, peb, smat, sdv, k, c, lane, p, P, C, off, doff,
+                              dummyW, dummyE, cw, w, partial, tile, ntiles);
+        if (o + 2 >= njobs) break;
+    }
+}
+
+// One dependency level of node jobs: one job per workgroup, grid (tiles,
+// jobs); every parent is read from memory.
+template <typename R, bool AUTO>
+__global__ __launch_bounds__(512) void k_pren4_level(const NodeJob* __restrict__ jobs, int P,
+                                                     int C, long cs, const R* __restrict__ jm,
+                                                     const R* dummyX, const int* dummyS,
+                                                     R* dummyW, const short* dummyE,
+                                                     const double* __restrict__ cw,
+                                                     const double* __restrict__ pw,
+                                                     double* partial) {
+    extern __shared__ double sdv[];   // [2][C][8][TILE] exchange, then [C][2] matrix blocks
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x;
+    const int p = tile * TILE + lane;
+    R* smat = reinterpret_cast<R*>(sdv + 2 * C * 8 * TILE);
+    const long off = c * cs + (long)p * 4;
+    const long doff = (long)(c * TILE + lane) * 4;
+    const int ntiles = (P + TILE - 1) / TILE;
+    const double w = pw[p];
+    int k = 0, pea = 0, peb = 0;
+    R pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
+    NodeOps4<R> S0;
+    const NodeJob J = jobs[blockIdx.y];
+    node_issue<R, AUTO>(J, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
+    node_compute<R, AUTO>(J, S0, pa, pb, pea, peb, smat, sdv, k, c, lane, p, P, C, off, doff,
+                          dummyW, dummyE, cw, w, partial, tile, ntiles);
 }
 
 // ---------------------------------------------------------------------------
@@ -873,6 +1179,58 @@ void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, 
             else PREL(false, false);
         }
 #undef PREL
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(WG) void k_gather_node_mats(const NodeJob* __restrict__ jobs,
+                                                         int njobs, int C,
+                                                         const R* __restrict__ mats,
+                                                         R* __restrict__ out) {
+    const long t = (long)blockIdx.x * WG + threadIdx.x;
+    const long per = (long)C * NODE_MATS;
+    if (t >= per * njobs) return;
+    const int j = (int)(t / per);
+    const int rem = (int)(t - (long)j * per);
+    const int c = rem / NODE_MATS, r = rem - c * NODE_MATS;
+    const int m = r / 20, e = r - m * 20;
+    const NodeJob& J = jobs[j];
+    const int mo = m == 0 ? J.moP[0] : m == 1 ? J.moP[1] : m == 2 ? J.moD1[0]
+                 : m == 3 ? J.moD2[0] : m == 4 ? J.moD1[1] : J.moD2[1];
+    out[(long)J.mbase + c * NODE_MATS + r] = mats[(long)mo + c * 20 + e];
+}
+
+template <typename R>
+void launch_gather_node_matrices(const NodeJob* djobs, int njobs, const Dims& d,
+                                 const void* mats, void* jobMats, hipStream_t st) {
+    const long n = (long)njobs * d.C * NODE_MATS;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_node_mats<R>, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, st,
+                       djobs, njobs, d.C, reinterpret_cast<const R*>(mats),
+                       reinterpret_cast<R*>(jobMats));
+}
+
+template <typename R>
+void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, const Dims& d,
+                      const void* jobMats, void* dummy, const double* cw, const double* pw,
+                      double* partial, hipStream_t st) {
+    if (njobs <= 0) return;
+    const R* dummyX = reinterpret_cast<const R*>(dummy);
+    R* dummyW = const_cast<R*>(dummyX) + 8 * TILE * 4;
+    const double* dummyD = reinterpret_cast<const double*>(dummyW + 8 * TILE * 4);
+    const int* dummyS = reinterpret_cast<const int*>(dummyD + TILE);
+    const short* dummyE = reinterpret_cast<const short*>(dummyS + TILE);
+    const R* m = reinterpret_cast<const R*>(jobMats);
+    const int tiles = (d.P + TILE - 1) / TILE;
+    const size_t lds = (size_t)2 * d.C * 8 * TILE * sizeof(double) +
+                       (size_t)d.C * 2 * NODE_MAT_BLOCK;
+    dim3 block(TILE * d.C);
+    if (batched) {
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4_level<R, true>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
+        else hipLaunchKernelGGL((k_pren4_level<R, false>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
+    } else {
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4<R, true>), dim3(tiles), block, lds, st, djobs, njobs, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
+        else hipLaunchKernelGGL((k_pren4<R, false>), dim3(tiles), block, lds, st, djobs, njobs, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
     }
 }
 
@@ -1543,6 +1901,11 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
     template void launch_pre<R>(const PreJob*, const PreJob*, int, const Dims&, int, void*,      \
                                 const void*, void*, const double*, const double*, double*,       \
                                 hipStream_t);                                                    \
+    template void launch_gather_node_matrices<R>(const NodeJob*, int, const Dims&, const void*, \
+                                                 void*, hipStream_t);                        \
+    template void launch_pre_nodes<R>(const NodeJob*, int, bool, const Dims&, const void*,     \
+                                      void*, const double*, const double*, double*,         \
+                                      hipStream_t);                                         \
     template void launch_pre_levels<R>(const PreJob*, const int*, int, const Dims&, const void*, \
                                        void*, const double*, const double*, double*,             \
                                        hipStream_t);                                             \

@@ -20,10 +20,11 @@ Also reported: the plain FP64 path through the beagle.h calls
 fused path, the FP64 post-order traversal's HBM bandwidth, and the
 reference's threaded CPU plugin on this host's cores.
 
-roofline: the dominant kernel of the value path, the fused FP32 pre-order
-+ derivative traversal of the caterpillar (one launch of 254 operations),
-timed with HIP events on the library's stream; algorithmic bytes per
-launch from preorder_bytes().  Inputs are resident in HBM before timing.
+roofline: the dominant kernel of the value path, the node-fused FP32
+pre-order + derivative traversal of the caterpillar (one launch of 127
+node jobs, i.e. 254 pre-order operations), timed with HIP events on the
+library's stream; algorithmic bytes per launch from preorder_bytes().
+Inputs are resident in HBM before timing.
 """
 import argparse
 import json
@@ -60,32 +61,39 @@ def postorder_bytes(prob, w, auto=False):
     return total
 
 
-def preorder_bytes(prob, ops, w, auto, post_of):
-    """Algorithmic HBM bytes of one fused pre-order + derivative launch over
-    `ops` (one fused segment).  Per operation: the parent's pre-order
-    partials unless the previous operation's output or parent is reused
-    from registers (the kernel's forwarding rule), the sibling (4*P for tip
-    states), the node's own post-order partials (derivative numerators and
-    denominator), the destination write unless it is BEAGLE_OP_NONE, four
-    C*S*(S+1)*w matrices, the pattern weights (8*P), and with automatic
-    rescaling 2 bytes per pattern for each exponent read or written."""
+def preorder_bytes(prob, ops, post_of, w, auto):
+    """Algorithmic HBM bytes of one node-fused pre-order + derivative
+    launch over `ops` (the library pairs the two operations of each node's
+    children into one job, in the order the second one appears).  Per job:
+    the node's pre-order partials unless the previous job produced them
+    (then they come from registers), each child's post-order partials
+    (4*P for tip states), each kept destination write, six C*S*(S+1)*w
+    matrices and 48 bytes of derivative sums per 64-pattern tile; with
+    automatic rescaling 2 bytes per pattern for each exponent read or
+    written.  Plus the pattern weights (8*P), read once per launch."""
     C, P, S = prob.C, prob.P, prob.S
     n = prob.tree.n
     part = C * P * S * w
-    total, prev = 0, None
+    tiles = (P + 63) // 64
+    waiting, jobs = {}, []
     for o in ops:
-        dest, par, sib = int(o[0]), int(o[3]), int(o[5])
-        own = post_of(o)
-        forwarded = prev is not None and (par == prev[0] or par == prev[3])
-        total += 0 if forwarded else part
-        total += 4 * P if sib < n else part
-        total += 4 * P if own < n else part
-        total += 4 * C * S * (S + 1) * w + 8 * P
-        if dest >= 0:
-            total += part
-        if auto:
-            total += SHORT * P * (1 + (sib >= n) + (dest >= 0))
-        prev = o
+        par = int(o[3])
+        if par in waiting:
+            jobs.append((waiting.pop(par), o))
+        else:
+            waiting[par] = o
+    total, prev = 8 * P, None
+    for a, b in jobs:
+        par = int(a[3])
+        if not (prev is not None and par in (int(prev[0][0]), int(prev[1][0]))):
+            total += part + (SHORT * P if auto else 0)
+        for x in (a, b):
+            own = post_of(x)
+            total += 4 * P if own < n else part + (SHORT * P if auto else 0)
+            if int(x[0]) >= 0:
+                total += part + (SHORT * P if auto else 0)
+        total += 6 * C * S * (S + 1) * w + 6 * 8 * tiles
+        prev = (a, b)
     return total
 
 
@@ -129,9 +137,8 @@ def cpu_baseline(budget_s):
     beagleCalculateEdgeDerivatives) on a `sample`-pattern alignment from the
     same generators, repeated for about budget_s seconds, scaled to 100,000
     patterns (the work is linear in the pattern count).  Falls back to the
-    oracle (one
-    thread) without the reference build.  Runs in its own process (both
-    libraries export beagle*)."""
+    oracle (one thread) without the reference build.  Runs in its own
+    process (both libraries export beagle*)."""
     from beagle_amd import api, driver
     ref_path = os.path.join(REPO, "oracle", "_ref", "libhmsbeagle.so")
     cores = min(len(os.sched_getaffinity(0)), 16)
@@ -218,9 +225,12 @@ def main():
     evs = [driver.Evaluator(lib, p, precision="single", scaling="auto", extra_flags=asyn)
            for p in probs]
     streams = [torch.cuda.ExternalStream(ev.inst.stream()) for ev in evs]
-    for _ in range(args.warmup):
+    # warm-up: at least W steps and one second of work, so clocks have ramped
+    t0, w = time.perf_counter(), 0
+    while w < args.warmup or time.perf_counter() - t0 < 1.0:
         for ev in evs:
             fused_eval(ev)
+        w += 1
     timing = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -233,7 +243,7 @@ def main():
     kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in timing]))
     cat_ops, cat_post = res32[0][3], res32[0][4]
     post_idx = dict(zip(map(tuple, cat_ops.tolist()), cat_post.tolist()))
-    pre_bytes = preorder_bytes(probs[0], cat_ops, 4, True, lambda o: post_idx[tuple(o.tolist())])
+    pre_bytes = preorder_bytes(probs[0], cat_ops, lambda o: post_idx[tuple(o.tolist())], 4, True)
     achieved = pre_bytes / (kernel_ms * 1e-3) / 1e9
     for ev in evs:
         ev.close()
@@ -329,8 +339,9 @@ def main():
                            "derivative extension; FP64 sums"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_pre4<float,true,true>: fused FP32 pre-order + derivative "
-                               "traversal of the caterpillar (254 operations, one launch)",
+                     "kernel": "k_pren4<float,true>: node-fused FP32 pre-order + derivative "
+                               "traversal of the caterpillar (127 node jobs = 254 operations, "
+                               "one launch)",
                      "kernel_ms": kernel_ms, "algorithmic_bytes": pre_bytes},
         "error_vs_fp64": err,
         "plain_fp64": {
