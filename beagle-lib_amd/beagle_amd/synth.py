@@ -353,3 +353,34 @@ def make_problem(seed, n_taxa, P, S=4, shape="yule", C=4, alpha=0.5, missing=0.0
     tips = random_patterns(rng, n_taxa, P, model.S, missing)
     w = pattern_weights(rng, P)
     return Problem(tree=tree, model=model, rates=rates, weights=cw, tips=tips, pattern_weights=w)
+
+
+def problem_arrays(prob, prefix=""):
+    """A Problem as a flat dict of arrays (for fixtures)."""
+    t, m = prob.tree, prob.model
+    n = t.n
+    ch = np.array([t.children[v] for v in range(n, 2 * n - 1)], dtype=np.int64)
+    return {prefix + "children": ch, prefix + "lengths": t.lengths, prefix + "Q": m.Q,
+            prefix + "freqs": m.freqs, prefix + "evec": m.evec, prefix + "ivec": m.ivec,
+            prefix + "evals": m.evals, prefix + "rates": prob.rates,
+            prefix + "weights": prob.weights, prefix + "tips": prob.tips,
+            prefix + "pattern_weights": prob.pattern_weights}
+
+
+def problem_from_arrays(d, prefix=""):
+    """Inverse of problem_arrays."""
+    ch = np.asarray(d[prefix + "children"])
+    n = ch.shape[0] + 1
+    children = {n + i: (int(a), int(b)) for i, (a, b) in enumerate(ch)}
+    parent = np.full(2 * n - 1, -1)
+    for v, (a, b) in children.items():
+        parent[a] = v
+        parent[b] = v
+    tree = Tree(n=n, children=children, parent=parent, lengths=np.asarray(d[prefix + "lengths"]))
+    model = Model(Q=np.asarray(d[prefix + "Q"]), freqs=np.asarray(d[prefix + "freqs"]),
+                  evec=np.asarray(d[prefix + "evec"]), ivec=np.asarray(d[prefix + "ivec"]),
+                  evals=np.asarray(d[prefix + "evals"]))
+    return Problem(tree=tree, model=model, rates=np.asarray(d[prefix + "rates"]),
+                   weights=np.asarray(d[prefix + "weights"]),
+                   tips=np.asarray(d[prefix + "tips"]),
+                   pattern_weights=np.asarray(d[prefix + "pattern_weights"]))

@@ -269,3 +269,79 @@ def test_traversal_plans_agree(amd, monkeypatch, mode, precision, scaling):
             ev.close()
         for a, b in zip(res[0], res[1]):
             np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+
+
+# ---- the reference's golden vectors ---------------------------------------------
+
+def _golden():
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "reference_outputs.npz")
+    return np.load(path, allow_pickle=False)
+
+
+def _cases():
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import cases
+    return cases
+
+
+@pytest.mark.parametrize("name", ["s4_yule_none_states", "s4_balanced_manual_partials",
+                                  "s4_caterpillar_manual_log", "s4_yule_always",
+                                  "s20_yule_none", "s61_yule_manual", "s4_partitions"])
+def test_hip_matches_reference_fixtures(amd, name):
+    """The reference CPU library's outputs (tests/golden/reference_outputs.npz)
+    reproduced by the HIP plugin from the stored inputs.  P(t) comes from
+    each library's own eigen-sum, so arrays are compared relative to their
+    largest entry (near-zero P entries carry ~1e-16 absolute rounding);
+    the log-likelihood within 1e-10 relative."""
+    cases, g = _cases(), _golden()
+    prob = synth.problem_from_arrays(g, name + "/in/")
+    _, opts, parts = cases.CASES[name]
+    out = cases.run(amd, prob, opts, parts)
+    for k in out:
+        want = g[name + "/out/" + k]
+        got = np.asarray(out[k]).reshape(want.shape)
+        if k in ("logL", "partition_logL", "site_logL"):
+            close(got, want, 1e-10)
+        else:
+            scale = np.abs(want).max()
+            tol = 1e-6 if prob.S == 61 else 1e-9
+            assert np.abs(got - want).max() <= tol * scale, (k, np.abs(got - want).max() / scale)
+
+
+# ---- full size (128 taxa, 100,000 patterns): size-independent properties ------
+
+@pytest.mark.parametrize("shape", ["caterpillar", "balanced"])
+def test_full_size_consistency(amd, shape):
+    """At the benchmark size, where the oracle would take minutes: the
+    log-likelihood equals the weighted sum of site log-likelihoods; no
+    scaling, manual scaling and FP64 power-of-two scaling agree; the fused
+    derivative extension equals the beagle.h call sequence; the FP32 path
+    is within 1e-6 of FP64 for logL and the gradient (relative to its
+    largest component)."""
+    prob = synth.make_problem(1, 128, 100_000, S=4, shape=shape)
+    res = {}
+    for key, kw in (("none", {}), ("manual", {"scaling": "manual"}),
+                    ("auto64", {"scaling": "auto"}),
+                    ("auto32", {"scaling": "auto", "precision": "single"})):
+        ev = driver.Evaluator(amd, prob, **kw)
+        lnl = ev.log_likelihood()
+        sites = ev.inst.site_log_likelihoods()
+        if key != "auto32":
+            assert rel(lnl, float(np.dot(sites, prob.pattern_weights))) < 1e-12
+        res[key] = (lnl, ev.fused_gradient(keep_tips=False))
+        if key == "none":
+            res["api"] = (lnl, ev.gradient())
+        ev.close()
+    base = res["api"][1]
+    for key in ("none", "manual", "auto64"):
+        assert rel(res[key][0], res["none"][0]) < 1e-12
+        for a, b in zip(res[key][1][1:], base[1:]):
+            assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+    lnl32, g32 = res["auto32"]
+    assert rel(lnl32, res["none"][0]) < 1e-6
+    for a, b in zip(g32[1:], base[1:]):
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()

@@ -1,0 +1,279 @@
+"""Pins of the FP64 CPU oracle (oracle/beagle_oracle.c), which the GPU
+parity tests use as their checker.  Each test would fail on a plausible
+mistake in the oracle (a dropped term, a wrong sign, index or transpose):
+
+  * known answers of the reference's example programs (PAUP and R values
+    printed by examples/tinytest/tinytest.cpp:332,
+    examples/hmctest/hmctest5.cpp:589,675-679 and
+    examples/hmctest/hmcGapTest.cpp:564);
+  * closed forms: JC69 transition probabilities, P(0) = I, rows of P
+    summing to 1, dP/dt = r Q P(t) and d2P/dt2 = r^2 Q^2 P(t) against
+    scipy's matrix exponential;
+  * brute force: the likelihood of small trees by enumerating every
+    assignment of internal-node states;
+  * central finite differences of the log-likelihood for the branch
+    derivatives (reference CPU pre-order + calculateEdgeDerivatives
+    semantics, BeagleCPUImpl.hpp:1940-2083);
+  * golden vectors of the reference's own CPU library (tests/golden/
+    reference_outputs.npz, written by tests/golden/make_golden.py), and,
+    where the reference build is present, live comparisons for cases the
+    fixtures do not hold (a non-generator derivative matrix, whose tip
+    denominator follows BeagleCPU4StateImpl.hpp:1032).
+"""
+import itertools
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from beagle_amd import api, driver, synth
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import cases  # noqa: E402
+
+GOLDEN = os.path.join(HERE, "golden", "reference_outputs.npz")
+KNOWN = os.path.join(HERE, "golden", "known_answers.json")
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+def seq_states(seq):
+    return np.array(["ACGT".index(ch) if ch in "ACGT" else 4 for ch in seq], dtype=np.int32)
+
+
+def small_problem(spec):
+    """A Problem from a known-answer entry: three tips, ((0,1)3,2)4."""
+    seqs = spec["sequences"]
+    n = len(seqs)
+    children = {n + i: tuple(c) for i, c in enumerate(spec["tree"]["children"])}
+    parent = np.full(2 * n - 1, -1)
+    for v, (a, b) in children.items():
+        parent[a] = v
+        parent[b] = v
+    lengths = np.zeros(2 * n - 1)
+    lengths[:2 * n - 2] = spec["tree"]["lengths"]
+    tree = synth.Tree(n=n, children=children, parent=parent, lengths=lengths)
+    if spec["model"] == "JC69":
+        model = synth.jc69()
+    else:  # an explicit eigen-system, Q = evec diag(evals) ivec
+        e = spec["eigen"]
+        V, Vi, lam = np.array(e["evec"]), np.array(e["ivec"]), np.array(e["evals"])
+        model = synth.Model(Q=V @ np.diag(lam) @ Vi, freqs=np.array(spec["freqs"]), evec=V,
+                            ivec=Vi, evals=lam)
+    rates = np.array(spec["rates"])
+    return synth.Problem(tree=tree, model=model, rates=rates,
+                         weights=np.full(len(rates), 1.0 / len(rates)),
+                         tips=np.stack([seq_states(s) for s in seqs]),
+                         pattern_weights=np.ones(len(seqs[0])))
+
+
+# ---- known answers -------------------------------------------------------------
+
+def test_tinytest_paup_loglikelihood(oracle):
+    spec = json.load(open(KNOWN))["tinytest"]
+    prob = small_problem(spec)
+    ev = driver.Evaluator(oracle, prob, tips_as="partials", preorder=False, transpose="none")
+    lnl = ev.log_likelihood()
+    ev.close()
+    assert abs(lnl - spec["logL"]) < 0.6 * 10 ** -spec["logL_digits"]
+
+
+@pytest.mark.parametrize("name", ["hmctest5", "hmcGapTest"])
+def test_hmc_examples_r_loglikelihood(oracle, name):
+    spec = json.load(open(KNOWN))[name]
+    prob = small_problem(spec)
+    ev = driver.Evaluator(oracle, prob, preorder=False, transpose="none")
+    lnl = ev.log_likelihood()
+    ev.close()
+    assert rel(lnl, spec["logL"]) < 1e-12
+
+
+def test_hmctest5_r_site_gradient(oracle):
+    """Per-site dlogL_p/dt (D = r_c Q) against the R table; rows are the
+    branches above post-order buffers 1, 0, 2, 3, columns the patterns."""
+    spec = json.load(open(KNOWN))["hmctest5"]
+    prob = small_problem(spec)
+    ev = driver.Evaluator(oracle, prob, transpose="none")
+    ev.log_likelihood()
+    ev.pre_order()
+    d1, _, _ = ev.edge_derivatives(False, per_site=True)
+    nodes = ev.branch_nodes()
+    ev.close()
+    d1 = np.asarray(d1).reshape(len(nodes), -1)
+    got = np.stack([d1[nodes.index(v)] for v in spec["gradient_nodes"]])
+    want = np.array(spec["site_dlogL_dt"])
+    np.testing.assert_allclose(got, want, rtol=2e-5, atol=2e-7)
+
+
+# ---- closed forms --------------------------------------------------------------
+
+def matrices(oracle, model, rates, lengths, derivs=True):
+    C, S = len(rates), model.S
+    inst = oracle.create(2, 2, 0, S, 4, 1, 3 * len(lengths), C, 0)
+    inst.set_eigen(0, model.evec.reshape(-1), model.ivec.reshape(-1), model.evals)
+    inst.set_category_rates(rates)
+    n = len(lengths)
+    idx = np.arange(n)
+    inst.update_transition_matrices(0, idx, idx + n if derivs else None,
+                                    idx + 2 * n if derivs else None, lengths)
+    out = [np.stack([inst.get_transition_matrix(i + k * n).reshape(C, S, S) for i in range(n)])
+           for k in range(3 if derivs else 1)]
+    inst.finalize()
+    return out
+
+
+def test_jc69_closed_form(oracle):
+    rates = np.array([0.3, 1.0, 2.5])
+    lengths = np.array([0.0, 0.05, 0.4, 2.0])
+    (P,) = matrices(oracle, synth.jc69(), rates, lengths, derivs=False)
+    for u, t in enumerate(lengths):
+        for c, r in enumerate(rates):
+            e = np.exp(-4.0 * r * t / 3.0)
+            want = np.full((4, 4), 0.25 - 0.25 * e)
+            np.fill_diagonal(want, 0.25 + 0.75 * e)
+            np.testing.assert_allclose(P[u, c], want, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("S,seed", [(4, 1), (20, 2), (61, 3)])
+def test_transition_matrices_against_expm(oracle, S, seed):
+    from scipy.linalg import expm
+    rng = np.random.default_rng(seed)
+    model = synth.gy94(rng) if S == 61 else synth.gtr(rng, S)
+    rates = np.array([0.2, 1.7])
+    lengths = np.array([0.0, 0.013, 0.31])
+    P, D1, D2 = matrices(oracle, model, rates, lengths)
+    Q = model.Q
+    for u, t in enumerate(lengths):
+        for c, r in enumerate(rates):
+            E = expm(Q * r * t)
+            scale = np.abs(E).max()
+            np.testing.assert_allclose(P[u, c], np.maximum(E, 0.0), atol=1e-12 * scale)
+            np.testing.assert_allclose(D1[u, c], r * Q @ E, atol=1e-11 * np.abs(r * Q).max())
+            np.testing.assert_allclose(D2[u, c], r * r * Q @ Q @ E,
+                                       atol=1e-11 * np.abs(r * r * Q @ Q).max())
+    # invariants: P(0) = I, rows of P sum to 1
+    np.testing.assert_allclose(P[0], np.broadcast_to(np.eye(S), P[0].shape), atol=1e-12)
+    np.testing.assert_allclose(P.sum(axis=-1), 1.0, atol=1e-11)
+
+
+# ---- brute force ---------------------------------------------------------------
+
+def brute_force_loglikelihood(prob):
+    from scipy.linalg import expm
+    tree, m = prob.tree, prob.model
+    n, S = tree.n, prob.S
+    internal = list(range(n, 2 * n - 1))
+    Ps = {(v, c): expm(m.Q * r * tree.lengths[v]) for v in range(2 * n - 2)
+          for c, r in enumerate(prob.rates)}
+    total = 0.0
+    for p in range(prob.P):
+        lp = 0.0
+        for c in range(prob.C):
+            acc = 0.0
+            for assign in itertools.product(range(S), repeat=len(internal)):
+                st = dict(zip(internal, assign))
+                prod = m.freqs[st[tree.root]]
+                for v in internal:
+                    for ch in tree.children[v]:
+                        row = Ps[(ch, c)][st[v]]
+                        if ch < n:
+                            s = prob.tips[ch, p]
+                            prod *= row.sum() if s >= S else row[s]
+                        else:
+                            prod *= row[st[ch]]
+                acc += prod
+            lp += prob.weights[c] * acc
+        total += prob.pattern_weights[p] * np.log(lp)
+    return total
+
+
+@pytest.mark.parametrize("shape", ["yule", "caterpillar", "balanced"])
+def test_brute_force_enumeration(oracle, shape):
+    prob = synth.make_problem(21, 4, 6, S=4, shape=shape, C=2, missing=0.15, mean_length=0.3)
+    ev = driver.Evaluator(oracle, prob, preorder=False, transpose="none", matrices="explicit")
+    lnl = ev.log_likelihood()
+    ev.close()
+    assert rel(lnl, brute_force_loglikelihood(prob)) < 1e-13
+
+
+# ---- finite differences --------------------------------------------------------
+
+@pytest.mark.parametrize("S,scaling", [(4, "none"), (4, "manual"), (20, "none")])
+def test_branch_derivatives_against_finite_differences(oracle, S, scaling):
+    prob = synth.make_problem(31 + S, 7, 40 if S == 4 else 12, S=S, missing=0.05)
+    ev = driver.Evaluator(oracle, prob, scaling=scaling, transpose="none")
+    lnl, g1, g2 = ev.gradient()
+    nodes = ev.branch_nodes()
+    h = 1e-6
+    base = prob.tree.lengths.copy()
+    fd1, fd2 = [], []
+    for v in nodes:
+        vals = []
+        for sgn in (1, -1):
+            t = base.copy()
+            t[v] += sgn * h
+            vals.append(ev.gradient(t)[1][nodes.index(v)])
+            if sgn == 1:
+                lp = ev.log_likelihood(t)
+            else:
+                lm = ev.log_likelihood(t)
+        fd1.append((lp - lm) / (2 * h))
+        fd2.append((vals[0] - vals[1]) / (2 * h))
+    ev.close()
+    np.testing.assert_allclose(g1, fd1, rtol=1e-5, atol=1e-6 * abs(lnl) / 1e3)
+    np.testing.assert_allclose(g2, fd2, rtol=1e-5, atol=1e-5 * np.abs(g2).max())
+
+
+# ---- reference golden vectors ---------------------------------------------------
+
+def golden():
+    return np.load(GOLDEN, allow_pickle=False)
+
+
+@pytest.mark.parametrize("name", list(cases.CASES))
+def test_oracle_matches_reference_fixtures(oracle, name):
+    g = golden()
+    prob = synth.problem_from_arrays(g, name + "/in/")
+    _, opts, parts = cases.CASES[name]
+    out = cases.run(oracle, prob, opts, parts)
+    keys = [k[len(name) + 5:] for k in g.files if k.startswith(name + "/out/")]
+    assert set(keys) == set(out)
+    for k in keys:
+        want = g[name + "/out/" + k]
+        got = np.asarray(out[k]).reshape(want.shape)
+        scale = max(np.abs(want).max(), 1e-300)
+        np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-12 * scale, err_msg=k)
+
+
+def test_fixture_inputs_match_generators():
+    """The stored inputs are the seeded generators' output, so a change of
+    generator cannot silently change what the fixtures mean."""
+    g = golden()
+    for name in cases.CASES:
+        prob = cases.problem(name)
+        for k, v in synth.problem_arrays(prob, name + "/in/").items():
+            np.testing.assert_array_equal(g[k], v, err_msg=k)
+
+
+def test_non_generator_derivative_matrix_against_reference(oracle, ref):
+    """A derivative matrix whose rows do not sum to 0, with missing tip
+    states: the tip denominator uses pre[state & 3]
+    (BeagleCPU4StateImpl.hpp:1032), which only a reference run pins."""
+    prob = synth.make_problem(41, 6, 50, S=4, missing=0.3)
+    rng = np.random.default_rng(5)
+    D = rng.normal(size=(prob.C, 4, 4)).reshape(-1)
+    res = []
+    for lib in (oracle, ref):
+        ev = driver.Evaluator(lib, prob, transpose="none")
+        ev.log_likelihood()
+        ev.pre_order()
+        ev.inst.set_differential_matrix(ev.d1_index, D)
+        res.append(ev.edge_derivatives(False, per_site=True))
+        ev.close()
+    for a, b in zip(res[0], res[1]):
+        np.testing.assert_allclose(np.asarray(a), np.asarray(b), rtol=1e-12, atol=1e-14)
