@@ -143,6 +143,8 @@ SIGNATURES = {
     # extensions (include/beagle_amd_ext.h)
     "ExtGetStream": (_I, [_I, ctypes.POINTER(ctypes.c_void_p)]),
     "ExtUpdatePrePartialsWithDerivatives": (_I, [_I, _IP, _I, _IP, _IP, _IP, _I, _DP, _DP, _DP]),
+    "ExtCalculateLogLikelihoodAndDerivatives": (_I, [_I, _I, _I, _I, _I, _I, _IP, _I, _IP, _IP,
+                                                     _IP, _DP, _DP, _DP, _DP]),
 }
 
 
@@ -424,17 +426,40 @@ class Instance:
                 ctypes.byref(o2))
         return out.value, o1.value, o2.value
 
-    def pre_partials_with_derivatives(self, ops, post, d1, d2, weights_index=0):
+    def pre_partials_with_derivatives(self, ops, post, d1, d2, weights_index=0, outputs=True):
         """Extension: pre-order pass fused with per-branch derivative sums.
-        Returns (sum w L'/L, sum w (L'/L)^2, sum w L''/L) per operation."""
+        Returns (sum w L'/L, sum w (L'/L)^2, sum w L''/L) per operation;
+        with outputs=False all output pointers are NULL (only the pre-order
+        partials are produced, and an asynchronous instance returns at
+        once)."""
         a = _ip(np.asarray(ops).reshape(-1))
         n = len(a[0]) // 7
         b, c, d = _ip(post), _ip(d1), _ip(d2)
+        if not outputs:
+            self._c("ExtUpdatePrePartialsWithDerivatives", a[1], n, b[1], c[1], d[1],
+                    weights_index, None, None, None)
+            return None
         s1, sq, s2 = np.zeros(n), np.zeros(n), np.zeros(n)
         self._c("ExtUpdatePrePartialsWithDerivatives", a[1], n, b[1], c[1], d[1], weights_index,
                 s1.ctypes.data_as(c_double_p), sq.ctypes.data_as(c_double_p),
                 s2.ctypes.data_as(c_double_p))
         return s1, sq, s2
+
+    def log_likelihood_and_derivatives(self, root, root_pre, ops, post, d1, d2,
+                                       weights_index=0, freqs_index=0, cumulative=OP_NONE):
+        """Extension: root logL, root pre-order partials = frequencies, and
+        the fused pre-order + derivative pass with one wait.  Returns
+        (logL, sum w L'/L, sum w (L'/L)^2, sum w L''/L)."""
+        a = _ip(np.asarray(ops).reshape(-1))
+        n = len(a[0]) // 7
+        b, c, d = _ip(post), _ip(d1), _ip(d2)
+        lnl = ctypes.c_double(0.0)
+        s1, sq, s2 = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._c("ExtCalculateLogLikelihoodAndDerivatives", root, weights_index, freqs_index,
+                cumulative, root_pre, a[1], n, b[1], c[1], d[1], ctypes.byref(lnl),
+                s1.ctypes.data_as(c_double_p), sq.ctypes.data_as(c_double_p),
+                s2.ctypes.data_as(c_double_p))
+        return lnl.value, s1, sq, s2
 
     def stream(self):
         """hipStream_t of the instance (extension entry point)."""

@@ -71,6 +71,11 @@ class Evaluator:
         self.inst = lib.create(n, partials, compact, prob.S, prob.P, 1, self.n_matrices, prob.C,
                                n, resources=resources, preference=pref, requirement=req)
         self._setup()
+        # operation lists depend on the topology only: built once, as a
+        # client builds them once per tree
+        self._branch_idx = np.arange(2 * n - 2, dtype=np.int32)
+        self._post_ops = {}
+        self._pre = None
 
     # -- data ---------------------------------------------------------------
     def _setup(self):
@@ -106,7 +111,7 @@ class Evaluator:
         tree = self.prob.tree
         if lengths is None:
             lengths = tree.lengths
-        idx = np.arange(2 * self.n - 2)
+        idx = self._branch_idx
         if self.matrices == "explicit":
             # identical P(t) on every library: exp(Q r t) from scipy, clamped
             # at 0, set with pad 1 (beagleSetTransitionMatrices)
@@ -122,7 +127,9 @@ class Evaluator:
     def post_order(self):
         tree = self.prob.tree
         manual = self.scaling in ("manual", "manual_log")
-        ops = synth.postorder_ops(tree, self._scale_write if manual else None)
+        if manual not in self._post_ops:
+            self._post_ops[manual] = synth.postorder_ops(tree, self._scale_write if manual else None)
+        ops = self._post_ops[manual]
         cum = self.n - 1
         if manual:
             self.inst.reset_scale_factors(cum)
@@ -159,21 +166,34 @@ class Evaluator:
         self.inst.update_pre_partials(ops, api.OP_NONE)
         return ops
 
+    def fused_operations(self, keep_tips=False):
+        """Pre-order operations and matching post-order buffers for the
+        derivative extension; for 4 states the tips' operations have no
+        destination unless keep_tips (no later operation reads them)."""
+        key = bool(keep_tips)
+        if self._pre is None or self._pre[0] != key:
+            ops = synth.preorder_ops(self.prob.tree, self.pre_base)
+            post = (ops[:, 0] - self.pre_base).astype(np.int32)
+            if not keep_tips and self.prob.S == 4 and self.prob.C <= 8:
+                ops[post < self.n, 0] = api.OP_NONE
+            n = len(post)
+            self._pre = (key, ops, post, np.full(n, self.d1_index, np.int32),
+                         np.full(n, self.d2_index, np.int32))
+        return self._pre[1:]
+
     def fused_gradient(self, lengths=None, keep_tips=False):
         """logL, dlogL/dt and d2logL/dt2 per branch through the fused
-        pre-order + derivative extension (this package's library only).
-        For 4 states the tips' pre-order partials are not kept unless
-        keep_tips (no later operation reads them)."""
-        logl = self.log_likelihood(lengths)
-        self.set_root_pre()
-        ops = synth.preorder_ops(self.prob.tree, self.pre_base)
-        post = ops[:, 0] - self.pre_base
-        if not keep_tips and self.prob.S == 4 and self.prob.C <= 8:
-            ops = ops.copy()
-            ops[post < self.n, 0] = api.OP_NONE
-        n = len(post)
-        s1, sq, s2 = self.inst.pre_partials_with_derivatives(
-            ops, post, [self.d1_index] * n, [self.d2_index] * n)
+        pre-order + derivative extensions (this package's library only):
+        matrices and the post-order traversal, then one call for the root
+        log-likelihood, root pre-order partials and the derivative pass."""
+        ops, post, d1, d2 = self.fused_operations(keep_tips)
+        self.update_matrices(lengths)
+        self.post_order()
+        manual = self.scaling in ("manual", "manual_log")
+        cum = self.n - 1 if manual else api.OP_NONE
+        logl, s1, sq, s2 = self.inst.log_likelihood_and_derivatives(
+            self.prob.tree.root, self.pre_base + self.prob.tree.root, ops, post, d1, d2,
+            cumulative=cum)
         g1 = np.zeros(2 * self.n - 1)
         g2 = np.zeros(2 * self.n - 1)
         g1[post] = s1
@@ -182,7 +202,9 @@ class Evaluator:
         return logl, g1[nodes], g2[nodes]
 
     def branch_nodes(self):
-        return [v for v in range(2 * self.n - 1) if v != self.prob.tree.root]
+        if not hasattr(self, "_nodes"):
+            self._nodes = [v for v in range(2 * self.n - 1) if v != self.prob.tree.root]
+        return self._nodes
 
     def edge_derivatives(self, second=False, per_site=True):
         nodes = self.branch_nodes()

@@ -600,6 +600,22 @@ int beagleExtUpdatePrePartialsWithDerivatives(int instance, const BeagleOperatio
     });
 }
 
+int beagleExtCalculateLogLikelihoodAndDerivatives(
+    int instance, int rootBufferIndex, int categoryWeightsIndex, int stateFrequenciesIndex,
+    int cumulativeScaleIndex, int rootPreBufferIndex, const BeagleOperation* operations,
+    int operationCount, const int* postBufferIndices, const int* firstDerivativeIndices,
+    const int* secondDerivativeIndices, double* outLogLikelihood, double* outSumFirstDerivatives,
+    double* outSumSquaredFirstDerivatives, double* outSumSecondDerivatives) {
+    if (!outLogLikelihood) return BEAGLE_ERROR_OUT_OF_RANGE;
+    return withInstance(instance, [&](Instance* in) {
+        return in->logLikelihoodAndDerivatives(
+            rootBufferIndex, categoryWeightsIndex, stateFrequenciesIndex, cumulativeScaleIndex,
+            rootPreBufferIndex, reinterpret_cast<const int*>(operations), operationCount,
+            postBufferIndices, firstDerivativeIndices, secondDerivativeIndices, outLogLikelihood,
+            outSumFirstDerivatives, outSumSquaredFirstDerivatives, outSumSecondDerivatives);
+    });
+}
+
 int beagleExtGetStream(int instance, void** outStream) {
     return withInstance(instance, [&](Instance* in) {
         *outStream = reinterpret_cast<void*>(in->stream());

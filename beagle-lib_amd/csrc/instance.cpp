@@ -225,6 +225,7 @@ Instance::~Instance() {
     hipHostFree(hResult_);
     hipFree(dWork_);
     hipFree(dJobMats_);
+    if (hStage_) hipHostFree(hStage_);
     hipFree(dScratch_);
     for (auto& s : ring_) {
         if (s.host) hipHostFree(s.host);
@@ -943,6 +944,49 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
                                       outD2 ? outD2 : s2.data(), nullptr);
         return rc;
     }
+    const double* sums = nullptr;
+    const bool want = outD1 || outD1Sq || outD2;
+    int rc = enqueuePreDerivatives(ops, count, postIdx, d1Idx, d2Idx, wIdx, want, &sums);
+    if (rc != BEAGLE_SUCCESS || count == 0) return rc;
+    if (!want) {
+        // nothing to return: no reduction, no copy; an asynchronous
+        // instance returns without waiting
+        if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
+        return BEAGLE_SUCCESS;
+    }
+    const double* h = stageOut(sums, 3 * (size_t)count, 0);
+    sync();
+    for (int o = 0; o < count; ++o) {
+        if (outD1) outD1[o] = h[3 * o];
+        if (outD1Sq) outD1Sq[o] = h[3 * o + 1];
+        if (outD2) outD2[o] = h[3 * o + 2];
+    }
+    return BEAGLE_SUCCESS;
+}
+
+// Pinned staging for results copied back: reserveStage(n) makes room for n
+// doubles (waiting for the stream if it must grow), stageOut copies n
+// doubles from the device to element `at` on the instance stream (the
+// caller waits before reading).
+void Instance::reserveStage(size_t n) {
+    if (stageCap_ >= n) return;
+    sync();
+    if (hStage_) HIPCHECK(hipHostFree(hStage_));
+    stageCap_ = std::max<size_t>(2 * n, 1024);
+    HIPCHECK(hipHostMalloc((void**)&hStage_, stageCap_ * sizeof(double), hipHostMallocDefault));
+}
+
+const double* Instance::stageOut(const double* dsrc, size_t n, size_t at) {
+    reserveStage(at + n);
+    HIPCHECK(hipMemcpyAsync(hStage_ + at, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    return hStage_ + at;
+}
+
+// The fused 4-state derivative pass, enqueued; with `reduce` the per-
+// operation sums (3 per operation) end in *dSums on the device.
+int Instance::enqueuePreDerivatives(const int* ops, int count, const int* postIdx,
+                                    const int* d1Idx, const int* d2Idx, int wIdx, bool reduce,
+                                    const double** dSums) {
     int rc = buildPreJobs(ops, count, false, true);
     if (rc != BEAGLE_SUCCESS || count == 0) return rc;
     const size_t mstride = matrixBytes_ / realSize();
@@ -964,16 +1008,48 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
     double* sums = wk + nPart;
     preWeights_ = wIdx;
     if (!launchPreNodes(ops, count, postIdx, wk)) launchPre(ops, BEAGLE_OP_COUNT, count, wk);
-    launch_reduce_rows(wk, ntiles, 3 * count, sums, stream_);
+    if (reduce) launch_reduce_rows(wk, ntiles, 3 * count, sums, stream_);
     HIPCHECK(hipGetLastError());
-    std::vector<double> h(3 * (size_t)count);
-    copyOut(h.data(), sums, h.size() * sizeof(double));
+    *dSums = sums;
+    return BEAGLE_SUCCESS;
+}
+
+// Extension: root log-likelihood, root pre-order partials and the fused
+// derivative pass, with one wait (see beagle_amd_ext.h).
+int Instance::logLikelihoodAndDerivatives(int rootIdx, int wIdx, int fIdx, int cumIdx,
+                                          int rootPreIdx, const int* ops, int count,
+                                          const int* postIdx, const int* d1Idx, const int* d2Idx,
+                                          double* outLogL, double* outD1, double* outD1Sq,
+                                          double* outD2) {
+    const bool fused = dims_.S == 4 && dims_.C <= 8 && !preTransposed_ && d2Idx != nullptr;
+    if (!fused) {
+        int rc = calculateRootLogLikelihoods(&rootIdx, &wIdx, &fIdx, &cumIdx, 1, outLogL);
+        if (rc != BEAGLE_SUCCESS) return rc;
+        rc = setRootPrePartials(&rootPreIdx, &fIdx, 1);
+        if (rc != BEAGLE_SUCCESS) return rc;
+        return updatePrePartialsWithDerivatives(ops, count, postIdx, d1Idx, d2Idx, wIdx, outD1,
+                                                outD1Sq, outD2);
+    }
+    if (!validEigen(wIdx) || hCatWeights_[wIdx].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
+    int rc = enqueueRoot(rootIdx, wIdx, fIdx, cumIdx);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    rc = setRootPrePartials(&rootPreIdx, &fIdx, 1);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    const double* sums = nullptr;
+    rc = enqueuePreDerivatives(ops, count, postIdx, d1Idx, d2Idx, wIdx, true, &sums);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    reserveStage(1 + 3 * (size_t)count);
+    const double* hl = stageOut(dResult_, 1, 0);
+    const double* h = stageOut(sums, 3 * (size_t)count, 1);
+    sync();
+    lastLogL_ = hl[0];
+    *outLogL = lastLogL_;
     for (int o = 0; o < count; ++o) {
         if (outD1) outD1[o] = h[3 * o];
         if (outD1Sq) outD1Sq[o] = h[3 * o + 1];
         if (outD2) outD2[o] = h[3 * o + 2];
     }
-    return BEAGLE_SUCCESS;
+    return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
 }
 
 // Node-fused path of the derivative extension: the two operations of each
@@ -1138,12 +1214,22 @@ int Instance::getScaleFactors(int src, double* out) {
 int Instance::calculateRootLogLikelihoods(const int* bufIdx, const int* wIdx, const int* fIdx,
                                           const int* cumIdx, int count, double* out) {
     if (count != 1) return BEAGLE_ERROR_NO_IMPLEMENTATION;
-    const int b = bufIdx[0], w = wIdx[0], f = fIdx[0];
+    const int rc = enqueueRoot(bufIdx[0], wIdx[0], fIdx[0], cumIdx[0]);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    HIPCHECK(hipMemcpyAsync(hResult_, dResult_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    sync();
+    lastLogL_ = hResult_[0];
+    *out = lastLogL_;
+    return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+}
+
+// Root log-likelihood of buffer b into dResult_[0] (device), not waited for.
+int Instance::enqueueRoot(int b, int w, int f, int cum) {
     if (w < 0) return BEAGLE_ERROR_NO_IMPLEMENTATION;  // per-category output
     if (!validBuffer(b) || !dPartials_[b] || !validEigen(w) || !validEigen(f) ||
         hCatWeights_[w].empty() || hFreqs_[f].empty())
         return BEAGLE_ERROR_OUT_OF_RANGE;
-    int cs = always_ ? b - tipCount_ : (auto_ ? BEAGLE_OP_NONE : cumIdx[0]);
+    int cs = always_ ? b - tipCount_ : (auto_ ? BEAGLE_OP_NONE : cum);
     if (cs != BEAGLE_OP_NONE && !validScale(cs)) return BEAGLE_ERROR_OUT_OF_RANGE;
     const short* ex = auto_ ? dExp_[b] : nullptr;
     const int nb = (dims_.P + 255) / 256;
@@ -1157,11 +1243,7 @@ int Instance::calculateRootLogLikelihoods(const int* bufIdx, const int* wIdx, co
                            dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
                            dPatternWeights_, dSiteL_, work, dResult_, dims_, stream_);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(hResult_, dResult_, sizeof(double), hipMemcpyDeviceToHost, stream_));
-    sync();
-    lastLogL_ = hResult_[0];
-    *out = lastLogL_;
-    return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+    return BEAGLE_SUCCESS;
 }
 
 int Instance::calculateRootLogLikelihoodsByPartition(const int* bufIdx, const int* wIdx,

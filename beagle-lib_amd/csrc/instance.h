@@ -71,6 +71,10 @@ public:
     int updatePrePartialsWithDerivatives(const int* ops, int count, const int* postIdx,
                                          const int* d1Idx, const int* d2Idx, int wIdx,
                                          double* outD1, double* outD1Sq, double* outD2);
+    int logLikelihoodAndDerivatives(int rootIdx, int wIdx, int fIdx, int cumIdx, int rootPreIdx,
+                                    const int* ops, int count, const int* postIdx,
+                                    const int* d1Idx, const int* d2Idx, double* outLogL,
+                                    double* outD1, double* outD1Sq, double* outD2);
     int waitForPartials();
 
     int accumulateScaleFactors(const int* idx, int count, int cumIdx, double sign, int part);
@@ -131,6 +135,11 @@ private:
     void launchPre(const int* ops, int stride, int count, double* partial);
     void launchPostPlan(const std::vector<Segment>& plan);
     bool launchPreNodes(const int* ops, int count, const int* postIdx, double* partial);
+    int enqueueRoot(int b, int w, int f, int cum);
+    int enqueuePreDerivatives(const int* ops, int count, const int* postIdx, const int* d1Idx,
+                              const int* d2Idx, int wIdx, bool reduce, const double** dSums);
+    void reserveStage(size_t n);
+    const double* stageOut(const double* dsrc, size_t n, size_t at);
     static constexpr int kNarrowLevel = 8;   // levels with fewer ops run fused
     bool validBuffer(int b) const { return b >= 0 && b < bufferCount_; }
     bool validMatrix(int m) const { return m >= 0 && m < matrixCount_; }
@@ -173,6 +182,8 @@ private:
     double* hResult_ = nullptr;       // pinned mirror
     void* dWork_ = nullptr;           // grow-only reduction scratch
     size_t workCap_ = 0;
+    double* hStage_ = nullptr;        // grow-only pinned staging for results
+    size_t stageCap_ = 0;             // doubles
     void* dJobMats_ = nullptr;        // grow-only gathered matrices of node jobs
     size_t jobMatsCap_ = 0;
     void* dScratch_ = nullptr;        // C*Ppad*Sp reals (generic pre-order)

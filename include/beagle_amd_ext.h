@@ -32,7 +32,10 @@ int beagleExtGetStream(int instance, void** outStream);
  * branch dlogL/dt = outSumFirstDerivatives[o] and d2logL/dt2 =
  * outSumSecondDerivatives[o] - outSumSquaredFirstDerivatives[o].  For
  * 4 states the derivative sums are computed inside the pre-order traversal
- * (post- and pre-order partials read once).  Output pointers may be NULL.
+ * (post- and pre-order partials read once).  Output pointers may be NULL;
+ * with all three NULL only the pre-order partials are produced, and an
+ * instance created with BEAGLE_FLAG_COMPUTATION_ASYNCH returns without
+ * waiting for the device.
  * An operation whose destinationPartials is BEAGLE_OP_NONE yields only its
  * derivative sums and keeps no pre-order partials (for branches to tips,
  * whose pre-order partials no later operation reads); this needs the
@@ -47,6 +50,30 @@ int beagleExtUpdatePrePartialsWithDerivatives(int instance, const BeagleOperatio
                                               double* outSumFirstDerivatives,
                                               double* outSumSquaredFirstDerivatives,
                                               double* outSumSecondDerivatives);
+
+/* One log-likelihood + branch-gradient evaluation with a single wait for
+ * the device.  Equivalent to
+ *   beagleCalculateRootLogLikelihoods(instance, &rootBufferIndex,
+ *       &categoryWeightsIndex, &stateFrequenciesIndex,
+ *       &cumulativeScaleIndex, 1, outLogLikelihood);
+ *   beagleSetRootPrePartials(instance, &rootPreBufferIndex,
+ *       &stateFrequenciesIndex, 1);
+ *   beagleExtUpdatePrePartialsWithDerivatives(instance, operations,
+ *       operationCount, postBufferIndices, firstDerivativeIndices,
+ *       secondDerivativeIndices, categoryWeightsIndex,
+ *       outSumFirstDerivatives, outSumSquaredFirstDerivatives,
+ *       outSumSecondDerivatives);
+ * with the three steps enqueued back to back and their results copied
+ * back together (for 4 states; otherwise the three calls in turn).  Made
+ * for gradient-based samplers that need logL and the gradient at each
+ * step.  outLogLikelihood must not be NULL.  No counterpart in the
+ * reference. */
+int beagleExtCalculateLogLikelihoodAndDerivatives(
+    int instance, int rootBufferIndex, int categoryWeightsIndex, int stateFrequenciesIndex,
+    int cumulativeScaleIndex, int rootPreBufferIndex, const BeagleOperation* operations,
+    int operationCount, const int* postBufferIndices, const int* firstDerivativeIndices,
+    const int* secondDerivativeIndices, double* outLogLikelihood, double* outSumFirstDerivatives,
+    double* outSumSquaredFirstDerivatives, double* outSumSecondDerivatives);
 
 #ifdef __cplusplus
 }

@@ -40,9 +40,11 @@ def problem(name):
     return synth.make_problem(seed, n, P, **kw)
 
 
-def run(lib, prob, opts, partitions, extra_flags=0):
-    """The call sequence of one case; returns a dict of output arrays."""
-    ev = driver.Evaluator(lib, prob, transpose="none", extra_flags=extra_flags, **opts)
+def run(lib, prob, opts, partitions, extra_flags=0, transpose="none"):
+    """The call sequence of one case; returns a dict of output arrays.
+    A GPU library takes transpose="auto" (PREORDER_TRANSPOSE_AUTO) to get
+    the CPU's untransposed pre-order convention for S > 4."""
+    ev = driver.Evaluator(lib, prob, transpose=transpose, extra_flags=extra_flags, **opts)
     inst, tree, n = ev.inst, prob.tree, prob.tree.n
     out = {}
     try:

@@ -300,7 +300,7 @@ def test_hip_matches_reference_fixtures(amd, name):
     cases, g = _cases(), _golden()
     prob = synth.problem_from_arrays(g, name + "/in/")
     _, opts, parts = cases.CASES[name]
-    out = cases.run(amd, prob, opts, parts)
+    out = cases.run(amd, prob, opts, parts, transpose="auto")
     for k in out:
         want = g[name + "/out/" + k]
         got = np.asarray(out[k]).reshape(want.shape)
