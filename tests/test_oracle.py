@@ -277,3 +277,66 @@ def test_non_generator_derivative_matrix_against_reference(oracle, ref):
         ev.close()
     for a, b in zip(res[0], res[1]):
         np.testing.assert_allclose(np.asarray(a), np.asarray(b), rtol=1e-12, atol=1e-14)
+
+
+def _api_sequence(lib, prob):
+    """Calls the fixtures do not hold: getPartials with a cumulative scale
+    buffer (BeagleCPUImpl.hpp:743-754), remove / accumulate / copy of scale
+    factors (:2763-2908) followed by a root log-likelihood over the
+    rebuilt cumulative buffer, and the partitioned pre-order pass
+    (:1415-1433)."""
+    ev = driver.Evaluator(lib, prob, scaling="manual", transpose="none")
+    inst, tree, n = ev.inst, prob.tree, prob.tree.n
+    out = {}
+    try:
+        ev.log_likelihood()
+        cum = n - 1
+        out["scaled_root"] = inst.get_partials(tree.root, cum)
+        inst.remove_scale_factors([1, 2], cum)
+        inst.accumulate_scale_factors([1, 2], cum)
+        inst.copy_scale_factors(1, 0)              # buffer 1 := buffer 0
+        inst.accumulate_scale_factors([1], cum)
+        inst.remove_scale_factors([0], cum)
+        out["cumulative_logL"] = np.array([inst.root_log_likelihood(tree.root, 0, 0, cum)])
+        ev.set_root_pre()
+        parts = np.zeros(prob.P, dtype=np.int32)
+        parts[prob.P // 2:] = 1
+        inst.set_pattern_partitions(2, parts)
+        ops = synth.preorder_ops(tree, ev.pre_base)
+        pops = [list(o) + [q, -1] for q in range(2) for o in ops]
+        inst.update_pre_partials_by_partition(np.array(pops, np.int32))
+        out["pre_by_partition"] = np.stack([inst.get_partials(ev.pre_base + v)
+                                            for v in ev.branch_nodes()[:3]])
+    finally:
+        ev.close()
+    return out
+
+
+def test_remaining_calls_against_reference(oracle, ref):
+    prob = synth.make_problem(51, 9, 120, S=4, missing=0.05, mean_length=0.3)
+    a = _api_sequence(oracle, prob)
+    b = _api_sequence(ref, prob)
+    for k in a:
+        scale = np.abs(b[k]).max()
+        np.testing.assert_allclose(a[k], b[k], rtol=1e-12, atol=1e-13 * scale, err_msg=k)
+
+
+def test_multiple_model_matrices_against_reference(oracle, ref):
+    """beagleUpdateTransitionMatricesWithMultipleModels: per-branch eigen
+    buffer and category-rate buffer (BeagleCPUImpl.hpp:1240-1290)."""
+    rng = np.random.default_rng(7)
+    models = [synth.gtr(rng, 4), synth.gtr(rng, 4)]
+    rates = [np.array([0.3, 0.9, 1.8]), np.array([0.5, 1.0, 1.5])]
+    lengths = np.array([0.05, 0.2, 0.7, 0.01])
+    out = []
+    for lib in (oracle, ref):
+        inst = lib.create(2, 3, 2, 4, 8, 2, 12, 3, 1)
+        for e, m in enumerate(models):
+            inst.set_eigen(e, m.evec.reshape(-1), m.ivec.reshape(-1), m.evals)
+            inst.set_category_rates_with_index(e, rates[e])
+        idx = np.arange(4)
+        inst.update_transition_matrices_multi([0, 1, 1, 0], [1, 0, 1, 0], idx, idx + 4, idx + 8,
+                                              lengths)
+        out.append(np.stack([inst.get_transition_matrix(i) for i in range(12)]))
+        inst.finalize()
+    np.testing.assert_allclose(out[0], out[1], rtol=1e-12, atol=1e-14)
