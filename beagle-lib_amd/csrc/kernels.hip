@@ -727,115 +727,144 @@ struct Vec16<double> {
 constexpr int NODE_MATS = 6 * 20;
 constexpr int NODE_MAT_BLOCK = 64 * 16;   // bytes of one wave's block
 
+// 32-bit addressing: a wave-uniform base pointer plus a per-lane byte
+// offset below 4 GiB, so the base stays in scalar registers and an address
+// costs one VGPR (the host uses these kernels only for buffers < 4 GiB).
+template <typename T>
+__device__ __forceinline__ const GLOBAL T* at(const void* base, unsigned boff) {
+    return (const GLOBAL T*)((const GLOBAL char*)base + boff);
+}
+template <typename T>
+__device__ __forceinline__ GLOBAL T* at_w(void* base, unsigned boff) {
+    return (GLOBAL T*)((GLOBAL char*)base + boff);
+}
+
 template <typename R>
 struct NodeOps4 {
-    R u[4];             // parent pre-order partials (kPar 0)
-    Operand4<R> A, B;   // children's post-order operands
-    int eu, ea, eb;     // exponents (automatic scaling)
+    R u[4];                    // parent pre-order partials (kPar 0)
+    R xa[4], xb[4];            // children's post-order partials
+    int sa, sb;                // children's tip states
+    int eu, ea, eb;            // exponents (automatic scaling)
     typename Vec16<R>::T mv;   // this lane's share of the job's matrices
 };
 
+// Byte offsets: `off` of this lane's 4-state vector in a partials buffer,
+// `doff` of its vector in the dummy tile; tip states at 4p (dummy 4 lane),
+// exponents at 2p (dummy 2 lane).
 template <typename R, bool AUTO>
-__device__ __forceinline__ void node_issue(const NodeJob& J, const R* dx, const int* ds,
-                                           const short* dE, const R* __restrict__ jm, int c,
-                                           long off, long doff, int p, int lane,
+__device__ __forceinline__ void node_issue(const NodeJob& J, const void* dx, const void* ds,
+                                           const void* dE, const R* __restrict__ jm, int c,
+                                           unsigned off, unsigned doff, int p, int lane,
                                            NodeOps4<R>& o) {
-    const R* up = J.kPar == 0 ? reinterpret_cast<const R*>(J.parent) + off : dx + doff;
-    load4(gin<R>(up), o.u);
-    issue4(J.x[0], reinterpret_cast<const int*>(J.x[0]), J.k[0], dx, ds, off, doff, p, lane, o.A);
-    issue4(J.x[1], reinterpret_cast<const int*>(J.x[1]), J.k[1], dx, ds, off, doff, p, lane, o.B);
+    const bool lp = J.kPar == 0, pa = J.k[0] == 0, pb = J.k[1] == 0;
+    load4(at<R>(lp ? J.parent : dx, lp ? off : doff), o.u);
+    load4(at<R>(pa ? J.x[0] : dx, pa ? off : doff), o.xa);
+    load4(at<R>(pb ? J.x[1] : dx, pb ? off : doff), o.xb);
+    o.sa = *at<int>(pa ? ds : J.x[0], pa ? 4u * lane : 4u * p);
+    o.sb = *at<int>(pb ? ds : J.x[1], pb ? 4u * lane : 4u * p);
     {
         constexpr int V = Vec16<R>::N;
         const int l = min(lane, NODE_MATS / V - 1);
-        o.mv = *(const GLOBAL typename Vec16<R>::T*)gin<R>(jm + J.mbase + c * NODE_MATS + l * V);
+        o.mv = *at<typename Vec16<R>::T>(jm, (unsigned)((J.mbase + c * NODE_MATS + l * V) *
+                                                        (int)sizeof(R)));
     }
     if (AUTO) {
-        const short* a = (J.kPar == 0 && J.eParent) ? J.eParent + p : dE + lane;
-        const short* b = J.ex[0] ? J.ex[0] + p : dE + lane;
-        const short* c = J.ex[1] ? J.ex[1] + p : dE + lane;
-        o.eu = gin<short>(a)[0];
-        o.ea = gin<short>(b)[0];
-        o.eb = gin<short>(c)[0];
+        const bool a = lp && J.eParent != nullptr, b = J.ex[0] != nullptr,
+                   e = J.ex[1] != nullptr;
+        o.eu = *at<short>(a ? (const void*)J.eParent : dE, a ? 2u * p : 2u * lane);
+        o.ea = *at<short>(b ? (const void*)J.ex[0] : dE, b ? 2u * p : 2u * lane);
+        o.eb = *at<short>(e ? (const void*)J.ex[1] : dE, e ? 2u * p : 2u * lane);
     }
 }
 
-// Derivative numerators and denominator of one branch for one category:
-// den = pre . x (a tip: pre[state % 4], as the reference computes it),
-// n1 = pre . (D1 x), n2 = pre . (D2 x).
+// A child's operand as a 5-vector: its partials, or for a tip the one-hot
+// state whose fifth entry selects the matrices' pad column (missing state).
+template <typename R>
+__device__ __forceinline__ void child5(int kind, int st, const R (&x)[4], R (&o)[5]) {
+    const bool isS = kind == 1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = isS ? (st == j ? (R)1 : (R)0) : x[j];
+    o[4] = (isS && st >= 4) ? (R)1 : (R)0;
+}
+
+template <typename R>
+__device__ __forceinline__ R dot4(const R (&a)[4], const R (&b)[4]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+}
+
+// Derivative numerators and denominator of one branch for one category,
+// in the partials' precision (categories are combined in FP64):
+// den = pre . x (a tip: pre[state & 3], as the reference computes it),
+// n1 = pre . (D1 x), n2 = pre . (D2 x).  One matrix is read at a time.
 template <typename R>
 __device__ __forceinline__ void branch_sums(int kind, int st, const R (&pre)[4], const R (&x)[5],
-                                            const R* d1, const R* d2, double& den, double& n1,
-                                            double& n2) {
-    R t1[4], t2[4];
-    mat4(d1, x, t1);
-    mat4(d2, x, t2);
+                                            const R* d1, const R* d2, R& den, R& n1, R& n2) {
     if (kind == 1) {
         // one-hot weights, not an index: an indexed pick is lowered to a
         // private-memory load
         const int s = st & 3;
-        den = (double)pre[0] * (double)(s == 0) + (double)pre[1] * (double)(s == 1) +
-              (double)pre[2] * (double)(s == 2) + (double)pre[3] * (double)(s == 3);
+        den = pre[0] * (R)(s == 0) + pre[1] * (R)(s == 1) + pre[2] * (R)(s == 2) +
+              pre[3] * (R)(s == 3);
     } else {
-        den = (double)pre[0] * (double)x[0] + (double)pre[1] * (double)x[1] +
-              (double)pre[2] * (double)x[2] + (double)pre[3] * (double)x[3];
+        den = pre[0] * x[0] + pre[1] * x[1] + pre[2] * x[2] + pre[3] * x[3];
     }
-    n1 = 0.0;
-    n2 = 0.0;
+    R t[4];
+    __builtin_amdgcn_sched_barrier(0);
+    mat4(d1, x, t);
+    n1 = dot4(pre, t);
+    __builtin_amdgcn_sched_barrier(0);
+    mat4(d2, x, t);
+    n2 = dot4(pre, t);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// t_j = sum_i M[i][j] u_i: the transpose product of the pre-order update
+template <typename R>
+__device__ __forceinline__ void mat4t(const R* m, const R (&u)[4], R (&t)[4]) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        n1 += (double)pre[i] * (double)t1[i];
-        n2 += (double)pre[i] * (double)t2[i];
-    }
+    for (int j = 0; j < 4; ++j)
+        t[j] = m[0 * 5 + j] * u[0] + m[1 * 5 + j] * u[1] + m[2 * 5 + j] * u[2] +
+               m[3 * 5 + j] * u[3];
 }
 
 template <typename R, bool AUTO>
 __device__ __forceinline__ void node_compute(const NodeJob& J, const NodeOps4<R>& o, R (&pa)[4],
                                              R (&pb)[4], int& pea, int& peb, R* smat,
-                                             double* sdv, int& k,
-                                             int c, int lane, int p, int P, int C, long off,
-                                             long doff, R* dummyW, const short* dummyE,
-                                             const double* __restrict__ cw, double w,
-                                             double* partial, int tile, int ntiles) {
+                                             double* sdv, int& k, int c, int lane, int p, int P,
+                                             int C, unsigned off, unsigned doff, R* dummyW,
+                                             void* dummyEw, const double* __restrict__ cw,
+                                             double w, double* partial, int tile, int ntiles) {
     const bool in = p < P;
-    R u[4], xa[5], xb[5], ya[4], yb[4], va[4], vb[4];
+    R u[4], xa[5], xb[5], t[4], ua[4], ub[4], va[4], vb[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) u[i] = J.kPar == 2 ? pa[i] : (J.kPar == 3 ? pb[i] : o.u[i]);
     const int eu = J.kPar == 2 ? pea : (J.kPar == 3 ? peb : o.eu);
-    operand4(J.k[0], o.A, pa, xa);   // kinds 0 and 1 only: pa is never selected
-    operand4(J.k[1], o.B, pa, xb);
+    child5(J.k[0], o.sa, o.xa, xa);
+    child5(J.k[1], o.sb, o.xb, xb);
     // this wave's LDS block for the job's matrices (two alternating blocks)
     R* lm = smat + (c * 2 + (k & 1)) * (NODE_MAT_BLOCK / (int)sizeof(R));
     *(typename Vec16<R>::T*)(lm + lane * Vec16<R>::N) = o.mv;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-    const R* Pa = lm;
-    const R* Pb = lm + 20;
+    // scheduling barriers keep one matrix in registers at a time
     __builtin_amdgcn_sched_barrier(0);
-    mat4(Pa, xa, ya);
-    mat4(Pb, xb, yb);
-    R ua[4], ub[4];
+    mat4(lm, xa, t);                     // P_a x_a
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        ua[i] = u[i] * yb[i];
-        ub[i] = u[i] * ya[i];
-    }
+    for (int i = 0; i < 4; ++i) ub[i] = u[i] * t[i];
+    __builtin_amdgcn_sched_barrier(0);
+    mat4(lm + 20, xb, t);                // P_b x_b
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        va[j] = Pa[0 * 5 + j] * ua[0] + Pa[1 * 5 + j] * ua[1] + Pa[2 * 5 + j] * ua[2] +
-                Pa[3 * 5 + j] * ua[3];
-        vb[j] = Pb[0 * 5 + j] * ub[0] + Pb[1 * 5 + j] * ub[1] + Pb[2 * 5 + j] * ub[2] +
-                Pb[3 * 5 + j] * ub[3];
-    }
+    for (int i = 0; i < 4; ++i) ua[i] = u[i] * t[i];
+    __builtin_amdgcn_sched_barrier(0);
+    mat4t(lm, ua, va);                   // pre(a) = P_a^T (pre(v) * P_b x_b)
+    __builtin_amdgcn_sched_barrier(0);
+    mat4t(lm + 20, ub, vb);              // pre(b) = P_b^T (pre(v) * P_a x_a)
     // derivative sums of both branches, formed before rescaling (the
-    // ratios do not depend on it); the scheduling barriers keep one stage's
-    // matrices in registers at a time
-    double dA, n1A, n2A, dB, n1B, n2B;
-    __builtin_amdgcn_sched_barrier(0);
-    branch_sums(J.k[0], o.A.st, va, xa, lm + 40, lm + 60, dA, n1A, n2A);
-    __builtin_amdgcn_sched_barrier(0);
-    branch_sums(J.k[1], o.B.st, vb, xb, lm + 80, lm + 100, dB, n1B, n2B);
-    __builtin_amdgcn_sched_barrier(0);
+    // ratios do not depend on it)
+    R dA, n1A, n2A, dB, n1B, n2B;
+    branch_sums(J.k[0], o.sa, va, xa, lm + 40, lm + 60, dA, n1A, n2A);
+    branch_sums(J.k[1], o.sb, vb, xb, lm + 80, lm + 100, dB, n1B, n2B);
     const double wc = cw[c];
     // LDS exchange [2][C][8][TILE]: max a, max b, den/n1/n2 of a, of b
     double* buf = sdv + (k & 1) * C * 8 * TILE;
@@ -843,12 +872,12 @@ __device__ __forceinline__ void node_compute(const NodeJob& J, const NodeOps4<R>
     double* mine = buf + c * 8 * TILE + lane;
     mine[0 * TILE] = in ? (double)fmax(fmax(va[0], va[1]), fmax(va[2], va[3])) : 0.0;
     mine[1 * TILE] = in ? (double)fmax(fmax(vb[0], vb[1]), fmax(vb[2], vb[3])) : 0.0;
-    mine[2 * TILE] = wc * dA;
-    mine[3 * TILE] = wc * n1A;
-    mine[4 * TILE] = wc * n2A;
-    mine[5 * TILE] = wc * dB;
-    mine[6 * TILE] = wc * n1B;
-    mine[7 * TILE] = wc * n2B;
+    mine[2 * TILE] = wc * (double)dA;
+    mine[3 * TILE] = wc * (double)n1A;
+    mine[4 * TILE] = wc * (double)n2A;
+    mine[5 * TILE] = wc * (double)dB;
+    mine[6 * TILE] = wc * (double)n1B;
+    mine[7 * TILE] = wc * (double)n2B;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
@@ -872,10 +901,8 @@ __device__ __forceinline__ void node_compute(const NodeJob& J, const NodeOps4<R>
         // pre(v) and x_a
         outA = ea + eu + (J.ex[1] ? o.eb : 0);
         outB = eb + eu + (J.ex[0] ? o.ea : 0);
-        short* da = keepA ? J.ed[0] + p : const_cast<short*>(dummyE) + 64 + lane;
-        short* db = keepB ? J.ed[1] + p : const_cast<short*>(dummyE) + 64 + lane;
-        *gout<short>(da) = (short)outA;
-        *gout<short>(db) = (short)outB;
+        *at_w<short>(keepA ? (void*)J.ed[0] : dummyEw, keepA ? 2u * p : 2u * lane) = (short)outA;
+        *at_w<short>(keepB ? (void*)J.ed[1] : dummyEw, keepB ? 2u * p : 2u * lane) = (short)outB;
     }
     // per-tile sums of child `ch` by wave ch % C: w r1, w r1^2, w r2 with
     // r = N / D summed over categories
@@ -900,10 +927,8 @@ __device__ __forceinline__ void node_compute(const NodeJob& J, const NodeOps4<R>
             pt[(row + 2) * ntiles + tile] = s2;
         }
     }
-    R* dstA = keepA ? reinterpret_cast<R*>(J.dest[0]) + off : dummyW + doff;
-    R* dstB = keepB ? reinterpret_cast<R*>(J.dest[1]) + off : dummyW + doff;
-    store4(gout<R>(dstA), va);
-    store4(gout<R>(dstB), vb);
+    store4(at_w<R>(keepA ? J.dest[0] : (void*)dummyW, keepA ? off : doff), va);
+    store4(at_w<R>(keepB ? J.dest[1] : (void*)dummyW, keepB ? off : doff), vb);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         pa[i] = va[i];
@@ -913,24 +938,28 @@ __device__ __forceinline__ void node_compute(const NodeJob& J, const NodeOps4<R>
     peb = outB;
 }
 
+#define NODE_KERNEL_SETUP                                                                     \
+    extern __shared__ double sdv[]; /* [2][C][8][TILE] exchange, then [C][2] matrix blocks */ \
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);                          \
+    const int lane = threadIdx.x & 63;                                                        \
+    const int tile = blockIdx.x;                                                              \
+    const int p = tile * TILE + lane; /* < Ppad: pattern weights are zero-padded */           \
+    R* smat = reinterpret_cast<R*>(sdv + 2 * C * 8 * TILE);                                   \
+    const unsigned off = (unsigned)((c * cs + (long)p * 4) * (long)sizeof(R));                \
+    const unsigned doff = (unsigned)((c * TILE + lane) * 4 * (int)sizeof(R));                 \
+    const int ntiles = (P + TILE - 1) / TILE;                                                 \
+    const double w = pw[p];                                                                   \
+    void* dummyEw = const_cast<char*>(reinterpret_cast<const char*>(dummyE)) + 128;           \
+    int k = 0, pea = 0, peb = 0;                                                              \
+    R pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
+
 template <typename R, bool AUTO>
 __global__ __launch_bounds__(512) void k_pren4(const NodeJob* __restrict__ jobs, int njobs, int P,
                                                int C, long cs, const R* __restrict__ jm,
                                                const R* dummyX, const int* dummyS, R* dummyW,
                                                const short* dummyE, const double* __restrict__ cw,
                                                const double* __restrict__ pw, double* partial) {
-    extern __shared__ double sdv[];   // [2][C][8][TILE] exchange, then [C][2] matrix blocks
-    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x;
-    const int p = tile * TILE + lane;   // < Ppad: pattern weights are zero-padded
-    R* smat = reinterpret_cast<R*>(sdv + 2 * C * 8 * TILE);
-    const long off = c * cs + (long)p * 4;
-    const long doff = (long)(c * TILE + lane) * 4;
-    const int ntiles = (P + TILE - 1) / TILE;
-    const double w = pw[p];
-    int k = 0, pea = 0, peb = 0;
-    R pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
+    NODE_KERNEL_SETUP
     NodeOps4<R> S0, S1;
     NodeJob J0 = jobs[0], J1;
     node_issue<R, AUTO>(J0, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
@@ -938,12 +967,12 @@ __global__ __launch_bounds__(512) void k_pren4(const NodeJob* __restrict__ jobs,
         J1 = jobs[min(o + 1, njobs - 1)];
         node_issue<R, AUTO>(J1, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S1);
         node_compute<R, AUTO>(J0, S0, pa, pb, pea, peb, smat, sdv, k, c, lane, p, P, C, off, doff,
-                              dummyW, dummyE, cw, w, partial, tile, ntiles);
+                              dummyW, dummyEw, cw, w, partial, tile, ntiles);
         if (o + 1 >= njobs) break;
         J0 = jobs[min(o + 2, njobs - 1)];
         node_issue<R, AUTO>(J0, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
         node_compute<R, AUTO>(J1, S1, pa, pb, pea, peb, smat, sdv, k, c, lane, p, P, C, off, doff,
-                              dummyW, dummyE, cw, w, partial, tile, ntiles);
+                              dummyW, dummyEw, cw, w, partial, tile, ntiles);
         if (o + 2 >= njobs) break;
     }
 }
@@ -958,24 +987,14 @@ __global__ __launch_bounds__(512) void k_pren4_level(const NodeJob* __restrict__
                                                      const double* __restrict__ cw,
                                                      const double* __restrict__ pw,
                                                      double* partial) {
-    extern __shared__ double sdv[];   // [2][C][8][TILE] exchange, then [C][2] matrix blocks
-    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x;
-    const int p = tile * TILE + lane;
-    R* smat = reinterpret_cast<R*>(sdv + 2 * C * 8 * TILE);
-    const long off = c * cs + (long)p * 4;
-    const long doff = (long)(c * TILE + lane) * 4;
-    const int ntiles = (P + TILE - 1) / TILE;
-    const double w = pw[p];
-    int k = 0, pea = 0, peb = 0;
-    R pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
+    NODE_KERNEL_SETUP
     NodeOps4<R> S0;
     const NodeJob J = jobs[blockIdx.y];
     node_issue<R, AUTO>(J, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
     node_compute<R, AUTO>(J, S0, pa, pb, pea, peb, smat, sdv, k, c, lane, p, P, C, off, doff,
-                          dummyW, dummyE, cw, w, partial, tile, ntiles);
+                          dummyW, dummyEw, cw, w, partial, tile, ntiles);
 }
+#undef NODE_KERNEL_SETUP
 
 // ---------------------------------------------------------------------------
 // generic state count: one launch per operation, one lane per output
