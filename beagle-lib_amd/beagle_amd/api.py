@@ -145,6 +145,7 @@ SIGNATURES = {
     "ExtUpdatePrePartialsWithDerivatives": (_I, [_I, _IP, _I, _IP, _IP, _IP, _I, _DP, _DP, _DP]),
     "ExtCalculateLogLikelihoodAndDerivatives": (_I, [_I, _I, _I, _I, _I, _I, _IP, _I, _IP, _IP,
                                                      _IP, _DP, _DP, _DP, _DP]),
+    "ExtCalculateRateMatrixGradient": (_I, [_I, _I, _IP, _I, _IP, _I, _I, _I, _DP, _DP]),
 }
 
 
@@ -297,6 +298,15 @@ class Instance:
         e, r, p, a, b, t = _ip(eigens), _ip(rates), _ip(prob), _ip(d1), _ip(d2), _dp(lengths)
         return self._c("UpdateTransitionMatricesWithMultipleModels", e[1], r[1], p[1],
                        a[1] if a else None, b[1] if b else None, t[1], len(p[0]))
+
+    def update_transition_matrices_model_categories(self, eigens, prob, d1, d2, lengths):
+        e, p, a, b, t = _ip(eigens), _ip(prob), _ip(d1), _ip(d2), _dp(lengths)
+        return self._c("UpdateTransitionMatricesWithModelCategories", e[1], p[1],
+                       a[1] if a else None, b[1] if b else None, t[1], len(p[0]))
+
+    def convolve_transition_matrices(self, first, second, result):
+        a, b, c = _ip(first), _ip(second), _ip(result)
+        return self._c("ConvolveTransitionMatrices", a[1], b[1], c[1], len(c[0]))
 
     def set_transition_matrix(self, index, m, pad):
         a = _dp(m)
@@ -460,6 +470,19 @@ class Instance:
                 s1.ctypes.data_as(c_double_p), sq.ctypes.data_as(c_double_p),
                 s2.ctypes.data_as(c_double_p))
         return lnl.value, s1, sq, s2
+
+    def rate_matrix_gradient(self, eigen, ops, post, weights_index=0, root=None,
+                             freqs_index=0):
+        """Extension: (dlogL/dQ as an S x S array, dlogL/dpi_root or None)."""
+        a = _ip(np.asarray(ops).reshape(-1))
+        n = len(a[0]) // 7
+        b = _ip(post)
+        gq = np.zeros(self.S * self.S)
+        gf = np.zeros(self.S) if root is not None else None
+        self._c("ExtCalculateRateMatrixGradient", eigen, a[1], n, b[1], weights_index,
+                -1 if root is None else root, freqs_index, gq.ctypes.data_as(c_double_p),
+                gf.ctypes.data_as(c_double_p) if gf is not None else None)
+        return gq.reshape(self.S, self.S), gf
 
     def stream(self):
         """hipStream_t of the instance (extension entry point)."""

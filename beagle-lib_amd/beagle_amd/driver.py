@@ -201,6 +201,15 @@ class Evaluator:
         nodes = self.branch_nodes()
         return logl, g1[nodes], g2[nodes]
 
+    def rate_matrix_gradient(self, root_frequencies=True):
+        """dlogL/dQ (S x S) for eigen-system 0 and, optionally, dlogL/dpi
+        through the root distribution, from the current post- and
+        pre-order partials (after fused_gradient() or pre_order()); this
+        package's library only."""
+        ops, post, _, _ = self.fused_operations()
+        root = self.prob.tree.root if root_frequencies else None
+        return self.inst.rate_matrix_gradient(0, ops, post, root=root)
+
     def branch_nodes(self):
         if not hasattr(self, "_nodes"):
             self._nodes = [v for v in range(2 * self.n - 1) if v != self.prob.tree.root]

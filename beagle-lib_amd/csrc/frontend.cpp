@@ -616,6 +616,20 @@ int beagleExtCalculateLogLikelihoodAndDerivatives(
     });
 }
 
+int beagleExtCalculateRateMatrixGradient(int instance, int eigenIndex,
+                                          const BeagleOperation* operations, int operationCount,
+                                          const int* postBufferIndices, int categoryWeightsIndex,
+                                          int rootBufferIndex, int stateFrequenciesIndex,
+                                          double* outGradientRateMatrix,
+                                          double* outGradientFrequencies) {
+    return withInstance(instance, [&](Instance* in) {
+        return in->rateMatrixGradient(eigenIndex, reinterpret_cast<const int*>(operations),
+                                      operationCount, postBufferIndices, categoryWeightsIndex,
+                                      rootBufferIndex, stateFrequenciesIndex,
+                                      outGradientRateMatrix, outGradientFrequencies);
+    });
+}
+
 int beagleExtGetStream(int instance, void** outStream) {
     return withInstance(instance, [&](Instance* in) {
         *outStream = reinterpret_cast<void*>(in->stream());

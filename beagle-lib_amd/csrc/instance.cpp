@@ -37,10 +37,6 @@ int roundUp(int x, int m) { return (x + m - 1) / m * m; }
 // the latest earlier operation that writes a resource it reads or writes,
 // or reads a resource it writes.  Resources are buffer indices (partials
 // buffers, then scale buffers offset by the buffer count); -1 = unused.
-struct Access {
-    int reads[3];
-    int writes[2];
-};
 
 int dependencyLevels(const std::vector<Access>& acc, int nres, std::vector<int>& level) {
     const int count = (int)acc.size();
@@ -178,6 +174,8 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
     dCube_ = dalloc<double>(eigenCount_ * S * S * S);
     dEvals_ = dalloc<double>(eigenCount_ * S);
     eigenSet_.assign(eigenCount_, false);
+    hEigen_.resize(eigenCount_);
+    matOrigin_.resize(matrixCount_);
     dFreqs_ = dalloc<double>(eigenCount_ * S);
     dCatWeights_ = dalloc<double>(eigenCount_ * categoryCount);
     dCatRates_ = dalloc<double>(eigenCount_ * categoryCount);
@@ -428,6 +426,13 @@ int Instance::setEigenDecomposition(int e, const double* V, const double* Vi, co
     copyIn(dCube_ + (size_t)e * S * S * S, cube.data(), cube.size() * sizeof(double));
     copyIn(dEvals_ + (size_t)e * S, lam, S * sizeof(double));
     eigenSet_[e] = true;
+    // host copy for the rate-matrix gradient: V, V^-1 (row-major), lambda
+    HostEigen& h = hEigen_[e];
+    h.V.assign(V, V + (size_t)S * S);
+    h.Vi.resize((size_t)S * S);
+    for (int k = 0; k < S; ++k)
+        for (int j = 0; j < S; ++j) h.Vi[(size_t)k * S + j] = trans ? Vi[k + j * S] : Vi[k * S + j];
+    h.lam.assign(lam, lam + S);
     return BEAGLE_SUCCESS;
 }
 
@@ -516,6 +521,12 @@ int Instance::updateTransitionMatrices(int e, const int* probIdx, const int* d1I
         J.evals = dEvals_ + (size_t)e * S;
         J.rates = dCatRates_;
         J.t = t[u];
+        J.cat = -1;
+    }
+    for (int u = 0; u < count; ++u) {
+        matOrigin_[probIdx[u]] = MatrixOrigin{e, 0, t[u]};
+        if (d1Idx) matOrigin_[d1Idx[u]] = MatrixOrigin{};
+        if (d1Idx && d2Idx) matOrigin_[d2Idx[u]] = MatrixOrigin{};
     }
     return matrixUpdate(jobs);
 }
@@ -525,8 +536,32 @@ int Instance::updateTransitionMatrices(int e, const int* probIdx, const int* d1I
 int Instance::updateTransitionMatricesWithModelCategories(const int* eig, const int* probIdx,
                                                           const int* d1Idx, const int* d2Idx,
                                                           const double* t, int count) {
-    (void)eig; (void)probIdx; (void)d1Idx; (void)d2Idx; (void)t; (void)count;
-    return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    const int S = dims_.S, C = dims_.C;
+    for (int c = 0; c < C; ++c)
+        if (!validEigen(eig[c]) || !eigenSet_[eig[c]]) return BEAGLE_ERROR_OUT_OF_RANGE;
+    std::vector<MatrixJob> jobs;
+    jobs.reserve((size_t)count * C);
+    for (int u = 0; u < count; ++u) {
+        if (!validMatrix(probIdx[u]) || (d1Idx && !validMatrix(d1Idx[u])) ||
+            (d1Idx && d2Idx && !validMatrix(d2Idx[u])))
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        for (int c = 0; c < C; ++c) {
+            MatrixJob J;
+            J.P = matrix(probIdx[u]);
+            J.D1 = d1Idx ? matrix(d1Idx[u]) : nullptr;
+            J.D2 = (d1Idx && d2Idx) ? matrix(d2Idx[u]) : nullptr;
+            J.cube = dCube_ + (size_t)eig[c] * S * S * S;
+            J.evals = dEvals_ + (size_t)eig[c] * S;
+            J.rates = nullptr;
+            J.t = t[u];
+            J.cat = c;
+            jobs.push_back(J);
+        }
+        matOrigin_[probIdx[u]] = MatrixOrigin{};   // several eigen-systems
+        if (d1Idx) matOrigin_[d1Idx[u]] = MatrixOrigin{};
+        if (d1Idx && d2Idx) matOrigin_[d2Idx[u]] = MatrixOrigin{};
+    }
+    return matrixUpdate(jobs);
 }
 
 int Instance::updateTransitionMatricesWithMultipleModels(const int* eig, const int* rateIdx,
@@ -548,6 +583,12 @@ int Instance::updateTransitionMatricesWithMultipleModels(const int* eig, const i
         J.evals = dEvals_ + (size_t)e * S;
         J.rates = dCatRates_ + (size_t)r * dims_.C;
         J.t = t[u];
+        J.cat = -1;
+    }
+    for (int u = 0; u < count; ++u) {
+        matOrigin_[probIdx[u]] = MatrixOrigin{eig[u], rateIdx[u], t[u]};
+        if (jobs[u].D1) matOrigin_[d1Idx[u]] = MatrixOrigin{};
+        if (jobs[u].D2) matOrigin_[d2Idx[u]] = MatrixOrigin{};
     }
     return matrixUpdate(jobs);
 }
@@ -555,6 +596,7 @@ int Instance::updateTransitionMatricesWithMultipleModels(const int* eig, const i
 // Client matrices [C][S][S] plus the pad value (BeagleCPUImpl.hpp:1035-1091).
 int Instance::setTransitionMatrix(int m, const double* in, double pad) {
     if (!validMatrix(m)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    matOrigin_[m] = MatrixOrigin{};
     const int S = dims_.S, W = S + 1;
     std::vector<double> tmp((size_t)dims_.C * S * W);
     for (int c = 0; c < dims_.C; ++c)
@@ -597,6 +639,7 @@ int Instance::transposeTransitionMatrices(const int* in, const int* out, int cou
         if (!validMatrix(in[u]) || !validMatrix(out[u])) return BEAGLE_ERROR_OUT_OF_RANGE;
         src[u] = matrix(in[u]);
         dst[u] = matrix(out[u]);
+        matOrigin_[out[u]] = MatrixOrigin{};
     }
     if (count == 0) return BEAGLE_SUCCESS;
     auto ds = reinterpret_cast<const void* const*>(upload(src.data(), count * sizeof(void*)));
@@ -607,8 +650,24 @@ int Instance::transposeTransitionMatrices(const int* in, const int* out, int cou
     return BEAGLE_SUCCESS;
 }
 
-int Instance::convolveTransitionMatrices(const int*, const int*, const int*, int) {
-    return BEAGLE_ERROR_NO_IMPLEMENTATION;
+// result = first x second per category; in-place results are refused
+// (BeagleCPUImpl.hpp:1096-1162).
+int Instance::convolveTransitionMatrices(const int* a, const int* b, const int* out, int count) {
+    std::vector<ConvolveJob> jobs(count);
+    for (int u = 0; u < count; ++u) {
+        if (!validMatrix(a[u]) || !validMatrix(b[u]) || !validMatrix(out[u]) || a[u] == out[u] ||
+            b[u] == out[u])
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        jobs[u] = ConvolveJob{matrix(a[u]), matrix(b[u]), matrix(out[u])};
+        matOrigin_[out[u]] = MatrixOrigin{};
+    }
+    if (count == 0) return BEAGLE_SUCCESS;
+    auto d = reinterpret_cast<const ConvolveJob*>(upload(jobs.data(), count * sizeof(ConvolveJob)));
+    if (dp_) launch_convolve<double>(d, count, dims_.S, dims_.C, stream_);
+    else launch_convolve<float>(d, count, dims_.S, dims_.C, stream_);
+    HIPCHECK(hipGetLastError());
+    if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
+    return BEAGLE_SUCCESS;
 }
 
 int Instance::addTransitionMatrices(const int*, const int*, const int*, int) {
@@ -989,19 +1048,8 @@ int Instance::enqueuePreDerivatives(const int* ops, int count, const int* postId
                                     const double** dSums) {
     int rc = buildPreJobs(ops, count, false, true);
     if (rc != BEAGLE_SUCCESS || count == 0) return rc;
-    const size_t mstride = matrixBytes_ / realSize();
-    for (int o = 0; o < count; ++o) {
-        const int po = postIdx[o];
-        if (!validBuffer(po) || (!dStates_[po] && !dPartials_[po]) || !validMatrix(d1Idx[o]) ||
-            !validMatrix(d2Idx[o]))
-            return BEAGLE_ERROR_OUT_OF_RANGE;
-        PreJob& J = hPre_[o];
-        J.xself = dStates_[po] ? nullptr : dPartials_[po];
-        J.sself = dStates_[po];
-        J.kSelf = dStates_[po] ? 1 : 0;
-        J.moD1 = (int)(d1Idx[o] * mstride);
-        J.moD2 = (int)(d2Idx[o] * mstride);
-    }
+    rc = attachOwnOperands(count, postIdx, d1Idx, d2Idx);
+    if (rc != BEAGLE_SUCCESS) return rc;
     const int ntiles = (dims_.P + 63) / 64;
     const size_t nPart = (size_t)count * 3 * ntiles;
     double* wk = reinterpret_cast<double*>(grow(dWork_, workCap_, (nPart + 3 * (size_t)count) * sizeof(double)));
@@ -1052,14 +1100,13 @@ int Instance::logLikelihoodAndDerivatives(int rootIdx, int wIdx, int fIdx, int c
     return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
 }
 
-// Node-fused path of the derivative extension: the two operations of each
-// node's children (same parent, each the other's sibling, matrices
-// crossed) become one job that reads pre(v), x_a and x_b once.  Returns
-// false, launching nothing, when some operation has no such partner or
-// writes scale factors.  Jobs are planned like operations: wide
-// dependency levels batched, narrow runs fused with the parent forwarded
-// from registers.
-bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, double* partial) {
+// Pairs the two operations of each node's children (same parent, each the
+// other's sibling, matrices crossed) into node jobs, in the order the second
+// operation of each pair appears.  False when some operation has no such
+// partner or writes scale factors.  Uses hPre_ (buildPreJobs) and fills
+// acc with each job's buffer accesses.
+bool Instance::makeNodeJobs(const int* ops, int count, const int* postIdx,
+                            std::vector<NodeJob>& nodes, std::vector<Access>& acc) {
     std::unordered_map<int, int> open;   // parent buffer -> operation awaiting its sibling
     std::vector<std::array<int, 2>> pairs;
     for (int o = 0; o < count; ++o) {
@@ -1077,10 +1124,9 @@ bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, dou
         open.erase(it);
     }
     if (!open.empty() || pairs.empty()) return false;
-
     const int n = (int)pairs.size();
-    std::vector<NodeJob> nodes(n);
-    std::vector<Access> acc(n);
+    nodes.assign(n, NodeJob{});
+    acc.assign(n, Access{});
     for (int j = 0; j < n; ++j) {
         const PreJob& A = hPre_[pairs[j][0]];
         const PreJob& B = hPre_[pairs[j][1]];
@@ -1110,6 +1156,21 @@ bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, dou
         const int* qb = ops + (size_t)pairs[j][1] * BEAGLE_OP_COUNT;
         acc[j] = Access{{qa[3], -1, -1}, {qa[0], qb[0]}};
     }
+    return true;
+}
+
+// Node-fused path of the derivative extension: the two operations of each
+// node's children become one job that reads pre(v), x_a and x_b once.
+// Returns false, launching nothing, when the operations do not pair up
+// (makeNodeJobs).  Jobs are planned like operations: wide dependency
+// levels batched, narrow runs fused with the parent forwarded from
+// registers.
+bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, double* partial) {
+    if (partialsBytes_ >= (size_t(1) << 32)) return false;   // 32-bit offsets in the kernel
+    std::vector<NodeJob> nodes;
+    std::vector<Access> acc;
+    if (!makeNodeJobs(ops, count, postIdx, nodes, acc)) return false;
+    const int n = (int)nodes.size();
     std::vector<int> level;
     const int nl = dependencyLevels(acc, bufferCount_, level);
     std::vector<Segment> plan;
@@ -1148,6 +1209,126 @@ bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, dou
         else launch_pre_nodes<float>(d + L.first, L.n, L.batched, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
     }
     return true;
+}
+
+// Each pre-order job's own post-order operand (postIdx) and derivative
+// matrices (d1Idx, d2Idx; without them the job's own matrix stands in).
+int Instance::attachOwnOperands(int count, const int* postIdx, const int* d1Idx,
+                                const int* d2Idx) {
+    const size_t mstride = matrixBytes_ / realSize();
+    for (int o = 0; o < count; ++o) {
+        const int po = postIdx[o];
+        if (!validBuffer(po) || (!dStates_[po] && !dPartials_[po]) ||
+            (d1Idx && !validMatrix(d1Idx[o])) || (d2Idx && !validMatrix(d2Idx[o])))
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        PreJob& J = hPre_[o];
+        J.xself = dStates_[po] ? nullptr : dPartials_[po];
+        J.sself = dStates_[po];
+        J.kSelf = dStates_[po] ? 1 : 0;
+        J.moD1 = d1Idx ? (int)(d1Idx[o] * mstride) : J.moSelf;
+        J.moD2 = d2Idx ? (int)(d2Idx[o] * mstride) : J.moSelf;
+    }
+    return BEAGLE_SUCCESS;
+}
+
+// Extension: dlogL/dQ for one eigen-system and dlogL/dpi through the root
+// distribution (see beagle_amd_ext.h and k_qgrad4).  The pre-order
+// partials of every parent in `ops` must be current, and every branch
+// matrix must come from beagleUpdateTransitionMatrices(WithMultipleModels)
+// with eigen-system `e`.
+int Instance::rateMatrixGradient(int e, const int* ops, int count, const int* postIdx, int wIdx,
+                                 int rootIdx, int fIdx, double* outQ, double* outFreqs) {
+    const int S = dims_.S, C = dims_.C;
+    if (S != 4 || C > 8 || preTransposed_) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    if (!validEigen(e) || !eigenSet_[e] || hEigen_[e].lam.empty() || !validEigen(wIdx) ||
+        hCatWeights_[wIdx].empty())
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    if (partialsBytes_ >= (size_t(1) << 32)) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    int rc = buildPreJobs(ops, count, false, true);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    rc = attachOwnOperands(count, postIdx, nullptr, nullptr);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    std::vector<NodeJob> nodes;
+    std::vector<Access> acc;
+    if (!makeNodeJobs(ops, count, postIdx, nodes, acc)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    const int n = (int)nodes.size();
+    // F per (job, category, child) from each branch's length and rates
+    const HostEigen& E = hEigen_[e];
+    std::vector<double> F((size_t)n * C * 32);
+    for (int j = 0; j < n; ++j) {
+        for (int ch = 0; ch < 2; ++ch) {
+            const int m = ops[(size_t)nodes[j].oidx[ch] * BEAGLE_OP_COUNT + 4];
+            const MatrixOrigin& org = matOrigin_[m];
+            if (org.eigen != e || hCatRates_[org.rates].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
+            for (int c = 0; c < C; ++c) {
+                const double tau = hCatRates_[org.rates][c] * org.t;
+                double* f = &F[((size_t)j * C + c) * 32 + ch * 16];
+                for (int a = 0; a < 4; ++a)
+                    for (int b = 0; b < 4; ++b) {
+                        const double d = E.lam[a] - E.lam[b];
+                        // (e^{tau l_a} - e^{tau l_b}) / (l_a - l_b), exactly
+                        // tau e^{tau l} at d = 0, without cancellation near it
+                        f[a * 4 + b] = d == 0.0 ? tau * std::exp(tau * E.lam[b])
+                                                : std::exp(tau * E.lam[b]) * std::expm1(tau * d) / d;
+                    }
+            }
+        }
+    }
+    std::vector<double> eig(32);
+    std::copy(E.V.begin(), E.V.end(), eig.begin());
+    std::copy(E.Vi.begin(), E.Vi.end(), eig.begin() + 16);
+    // device data: jobs, F, V | V^-1 in one upload; gathered matrices
+    const size_t perJob = (size_t)C * 6 * 20;
+    for (int j = 0; j < n; ++j) nodes[j].mbase = (int)(j * perJob);
+    const size_t jobBytes = (size_t)n * sizeof(NodeJob);
+    const size_t fOff = (jobBytes + 255) / 256 * 256;
+    const size_t eOff = fOff + F.size() * sizeof(double);
+    std::vector<char> blob(eOff + eig.size() * sizeof(double));
+    std::memcpy(blob.data(), nodes.data(), jobBytes);
+    std::memcpy(blob.data() + fOff, F.data(), F.size() * sizeof(double));
+    std::memcpy(blob.data() + eOff, eig.data(), eig.size() * sizeof(double));
+    char* dblob = reinterpret_cast<char*>(upload(blob.data(), blob.size()));
+    const NodeJob* dj = reinterpret_cast<const NodeJob*>(dblob);
+    void* jm = grow(dJobMats_, jobMatsCap_, (size_t)n * perJob * realSize());
+    const int ntiles = (dims_.P + 63) / 64;
+    const int nb = (dims_.P + 255) / 256;
+    const size_t nq = (size_t)16 * ntiles * C, nf = (size_t)S * nb;
+    double* wk = reinterpret_cast<double*>(grow(dWork_, workCap_, (nq + nf + 16 + S) * sizeof(double)));
+    double* sums = wk + nq + nf;
+    const double* cw = dCatWeights_ + (size_t)wIdx * C;
+    if (dp_) launch_gather_node_matrices<double>(dj, n, dims_, dMatrices_, jm, stream_);
+    else launch_gather_node_matrices<float>(dj, n, dims_, dMatrices_, jm, stream_);
+    if (dp_)
+        launch_qgrad<double>(dj, n, dims_, jm, reinterpret_cast<const double*>(dblob + fOff),
+                             reinterpret_cast<const double*>(dblob + eOff), dDummy_, cw,
+                             dPatternWeights_, wk, stream_);
+    else
+        launch_qgrad<float>(dj, n, dims_, jm, reinterpret_cast<const double*>(dblob + fOff),
+                            reinterpret_cast<const double*>(dblob + eOff), dDummy_, cw,
+                            dPatternWeights_, wk, stream_);
+    launch_reduce_rows(wk, ntiles * C, 16, sums, stream_);
+    const bool freqs = outFreqs != nullptr;
+    if (freqs) {
+        if (!validBuffer(rootIdx) || !dPartials_[rootIdx] || !validEigen(fIdx) || hFreqs_[fIdx].empty())
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        if (dp_) launch_root_freq_grad<double>(dPartials_[rootIdx], cw, dFreqs_ + (size_t)fIdx * S, dPatternWeights_, wk + nq, dims_, stream_);
+        else launch_root_freq_grad<float>(dPartials_[rootIdx], cw, dFreqs_ + (size_t)fIdx * S, dPatternWeights_, wk + nq, dims_, stream_);
+        launch_reduce_rows(wk + nq, nb, S, sums + 16, stream_);
+    }
+    HIPCHECK(hipGetLastError());
+    const double* h = stageOut(sums, 16 + (freqs ? S : 0), 0);
+    sync();
+    // dlogL/dQ = V^-T H V^T
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) {
+            double g = 0.0;
+            for (int k = 0; k < 4; ++k)
+                for (int l = 0; l < 4; ++l) g += E.Vi[k * 4 + a] * h[k * 4 + l] * E.V[b * 4 + l];
+            if (outQ) outQ[a * 4 + b] = g;
+        }
+    if (freqs)
+        for (int s = 0; s < S; ++s) outFreqs[s] = h[16 + s];
+    return BEAGLE_SUCCESS;
 }
 
 int Instance::waitForPartials() {

@@ -15,6 +15,27 @@
 
 namespace hipbeagle {
 
+// Buffers an operation reads and writes (partials buffer indices, scale
+// buffers offset by the buffer count; -1 unused), for dependency levels.
+struct Access {
+    int reads[3];
+    int writes[2];
+};
+
+// Eigen-system kept on the host (for the rate-matrix gradient).
+struct HostEigen {
+    std::vector<double> V, Vi, lam;   // row-major V, V^-1; eigenvalues
+};
+
+// How a transition matrix was last produced: from eigen-system `eigen`
+// with category-rate buffer `rates` and length `t` (eigen < 0: set by the
+// client, transposed, convolved, or from several eigen-systems).
+struct MatrixOrigin {
+    int eigen = -1;
+    int rates = 0;
+    double t = 0.0;
+};
+
 // Operations launched together: batched (one dependency level, grid over
 // pattern tiles x operations) or fused (operations walked in list order by
 // each pattern tile).
@@ -71,6 +92,8 @@ public:
     int updatePrePartialsWithDerivatives(const int* ops, int count, const int* postIdx,
                                          const int* d1Idx, const int* d2Idx, int wIdx,
                                          double* outD1, double* outD1Sq, double* outD2);
+    int rateMatrixGradient(int eigenIdx, const int* ops, int count, const int* postIdx, int wIdx,
+                           int rootIdx, int fIdx, double* outQ, double* outFreqs);
     int logLikelihoodAndDerivatives(int rootIdx, int wIdx, int fIdx, int cumIdx, int rootPreIdx,
                                     const int* ops, int count, const int* postIdx,
                                     const int* d1Idx, const int* d2Idx, double* outLogL,
@@ -132,9 +155,12 @@ private:
     void sync();
     int matrixUpdate(const std::vector<MatrixJob>& jobs);
     int buildPreJobs(const int* ops, int count, bool byPartition, bool allowNoDest);
+    int attachOwnOperands(int count, const int* postIdx, const int* d1Idx, const int* d2Idx);
     void launchPre(const int* ops, int stride, int count, double* partial);
     void launchPostPlan(const std::vector<Segment>& plan);
     bool launchPreNodes(const int* ops, int count, const int* postIdx, double* partial);
+    bool makeNodeJobs(const int* ops, int count, const int* postIdx, std::vector<NodeJob>& nodes,
+                      std::vector<Access>& acc);
     int enqueueRoot(int b, int w, int f, int cum);
     int enqueuePreDerivatives(const int* ops, int count, const int* postIdx, const int* d1Idx,
                               const int* d2Idx, int wIdx, bool reduce, const double** dSums);
@@ -168,6 +194,8 @@ private:
     double* dCube_ = nullptr;
     double* dEvals_ = nullptr;
     std::vector<bool> eigenSet_;
+    std::vector<HostEigen> hEigen_;
+    std::vector<MatrixOrigin> matOrigin_;
     double* dFreqs_ = nullptr;
     double* dCatWeights_ = nullptr;
     double* dCatRates_ = nullptr;

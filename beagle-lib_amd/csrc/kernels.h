@@ -88,6 +88,12 @@ struct MatrixJob {
     const double* evals;              // S
     const double* rates;              // C category rates
     double t;
+    int cat;                          // >= 0: only this category, rate 1 (model categories)
+};
+
+// One matrix product per category: out = A B, pad column 1.
+struct ConvolveJob {
+    const void* A; const void* B; void* out;
 };
 
 // One node of an edge-derivative or cross-product request.
@@ -112,6 +118,8 @@ struct Dims {
 // ---- launchers (kernels.hip) ----
 template <typename R>
 void launch_matrices(const MatrixJob* jobs, int njobs, int S, int C, hipStream_t st);
+template <typename R>
+void launch_convolve(const ConvolveJob* jobs, int njobs, int S, int C, hipStream_t st);
 // Traversals take the device copy of the jobs (read by the kernels) and the
 // host copy (which decides launch shapes on the generic-S path).
 template <typename R>
@@ -143,6 +151,17 @@ template <typename R>
 void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, const Dims& d,
                       const void* jobMats, void* dummy, const double* cw, const double* pw,
                       double* partial, hipStream_t st);
+// Rate-matrix gradient over node jobs (4 states): per-tile, per-category
+// sums of H (16 rows of tiles * C); jobF holds F per (job, category, child)
+// as [job][category][2][16], eig = V | V^-1 row-major.
+template <typename R>
+void launch_qgrad(const NodeJob* djobs, int njobs, const Dims& d, const void* jobMats,
+                  const double* jobF, const double* eig, void* dummy, const double* cw,
+                  const double* pw, double* partial, hipStream_t st);
+// Root-frequency gradient block sums: rows i of [S][ceil(P / 256)].
+template <typename R>
+void launch_root_freq_grad(const void* root, const double* cw, const double* freqs,
+                           const double* pw, double* partial, const Dims& d, hipStream_t st);
 // Copies each node job's six matrices (P_a, P_b, D1_a, D2_a, D1_b, D2_b) into
 // jobMats as [job][category][6][20] reals, so a job's matrices for one
 // category are contiguous; job j's block starts at djobs[j].mbase.

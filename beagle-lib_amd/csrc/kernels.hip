@@ -128,8 +128,9 @@ __global__ __launch_bounds__(WG) void k_matrices(const MatrixJob* __restrict__ j
     double* e1 = sh + S;
     double* e2 = sh + 2 * S;
     const MatrixJob J = jobs[blockIdx.x];
-    const int c = blockIdx.y;
-    const double r = J.rates[c];
+    if (J.cat >= 0 && blockIdx.y != 0) return;   // one category per job
+    const int c = J.cat >= 0 ? J.cat : (int)blockIdx.y;
+    const double r = J.cat >= 0 ? 1.0 : J.rates[c];
     const bool deriv = J.D1 != nullptr;
     for (int k = threadIdx.x; k < S; k += WG) {
         const double lam = J.evals[k];
@@ -172,6 +173,30 @@ void launch_matrices(const MatrixJob* jobs, int njobs, int S, int C, hipStream_t
     if (njobs <= 0) return;
     dim3 grid(njobs, C);
     hipLaunchKernelGGL(k_matrices<R>, grid, dim3(WG), 3 * S * sizeof(double), st, jobs, S, C);
+}
+
+// out[c] = A[c] B[c] over S states (in the instance precision, as the
+// reference sums in its REALTYPE), pad column 1 (BeagleCPUImpl.hpp:1096-1162).
+template <typename R>
+__global__ __launch_bounds__(WG) void k_convolve(const ConvolveJob* __restrict__ jobs, int S) {
+    const ConvolveJob J = jobs[blockIdx.x];
+    const int c = blockIdx.y, W = S + 1;
+    const R* A = reinterpret_cast<const R*>(J.A) + (long)c * S * W;
+    const R* B = reinterpret_cast<const R*>(J.B) + (long)c * S * W;
+    R* O = reinterpret_cast<R*>(J.out) + (long)c * S * W;
+    for (int idx = threadIdx.x; idx < S * S; idx += WG) {
+        const int i = idx / S, j = idx - i * S;
+        R sum = 0;
+        for (int k = 0; k < S; ++k) sum += A[i * W + k] * B[k * W + j];
+        O[i * W + j] = sum;
+    }
+    for (int i = threadIdx.x; i < S; i += WG) O[i * W + S] = (R)1.0;
+}
+
+template <typename R>
+void launch_convolve(const ConvolveJob* jobs, int njobs, int S, int C, hipStream_t st) {
+    if (njobs <= 0) return;
+    hipLaunchKernelGGL(k_convolve<R>, dim3(njobs, C), dim3(WG), 0, st, jobs, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -995,6 +1020,217 @@ __global__ __launch_bounds__(512) void k_pren4_level(const NodeJob* __restrict__
                           dummyW, dummyEw, cw, w, partial, tile, ntiles);
 }
 #undef NODE_KERNEL_SETUP
+
+// ---------------------------------------------------------------------------
+// Rate-matrix gradient (4 states).  For one eigen-system Q = V diag(lam) V^-1
+// and tau = r_c t, the Frechet derivative of P = exp(tau Q) is
+//   dP = V (F o (V^-1 dQ V)) V^-1,  F_kl = (e^{tau lam_k} - e^{tau lam_l}) /
+//        (lam_k - lam_l)  (tau e^{tau lam_k} when lam_k = lam_l),
+// so dlogL/dQ = V^-T H V^T with
+//   H = sum over branches, categories c and patterns p of
+//       w_c (w_p / L_p) F o ((V^T u)(V^-1 x)^T),
+// u the branch's top vector pre(v) * (P_sib x_sib) and x its post-order
+// operand.  A node job v (children a, b) gives both branches: u_a = pre(v)
+// * P_b x_b, u_b = pre(v) * P_a x_a, and L_p = sum_c w_c sum_i pre(v)_i
+// (P_a x_a)_i (P_b x_b)_i.  Each lane accumulates H (FP64) for its pattern
+// and category over every job; scale factors cancel in u x^T / L_p.  The
+// jobs are independent (all pre-order partials are computed), so a
+// workgroup walks the list with the next job's loads in flight.
+
+template <typename R>
+struct QOps4 {
+    R u[4], xa[4], xb[4];
+    int sa, sb;
+    typename Vec16<R>::T mv;   // this lane's share of the job's matrices
+    dvec2 fv;                  // this lane's share of the job's F values
+};
+
+template <typename R>
+__device__ __forceinline__ void q_issue(const NodeJob& J, int jidx, const void* dx, const void* ds,
+                                        const R* __restrict__ jm, const double* __restrict__ jf,
+                                        int c, int C, unsigned off, unsigned doff, int p, int lane,
+                                        QOps4<R>& o) {
+    const bool pa = J.k[0] == 0, pb = J.k[1] == 0;
+    load4(at<R>(J.parent, off), o.u);
+    load4(at<R>(pa ? J.x[0] : dx, pa ? off : doff), o.xa);
+    load4(at<R>(pb ? J.x[1] : dx, pb ? off : doff), o.xb);
+    o.sa = *at<int>(pa ? ds : J.x[0], pa ? 4u * lane : 4u * p);
+    o.sb = *at<int>(pb ? ds : J.x[1], pb ? 4u * lane : 4u * p);
+    constexpr int V = Vec16<R>::N;
+    const int l = min(lane, NODE_MATS / V - 1);
+    o.mv = *at<typename Vec16<R>::T>(jm, (unsigned)((J.mbase + c * NODE_MATS + l * V) *
+                                                    (int)sizeof(R)));
+    o.fv = *at<dvec2>(jf, (unsigned)(((jidx * C + c) * 32 + 2 * min(lane, 15)) * 8));
+}
+
+template <typename R>
+__device__ __forceinline__ void q_compute(const NodeJob& J, const QOps4<R>& o, double (&H)[16],
+                                          R* smat, double* sfb, double* sx,
+                                          const double* sV, int& k, int c, int lane, int p,
+                                          int P, int C, const double* __restrict__ cw, double w) {
+    const bool in = p < P;
+    R xa[5], xb[5], ya[4], yb[4];
+    child5(J.k[0], o.sa, o.xa, xa);
+    child5(J.k[1], o.sb, o.xb, xb);
+    R* lm = smat + (c * 2 + (k & 1)) * (NODE_MAT_BLOCK / (int)sizeof(R));
+    double* lf = sfb + (c * 2 + (k & 1)) * 32;
+    *(typename Vec16<R>::T*)(lm + lane * Vec16<R>::N) = o.mv;
+    *(dvec2*)(lf + 2 * min(lane, 15)) = o.fv;   // lanes past 15 repeat lane 15's pair
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    mat4(lm, xa, ya);        // P_a x_a
+    mat4(lm + 20, xb, yb);   // P_b x_b
+    double uab[4], dc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uab[i] = (double)o.u[i] * (double)ya[i] * (double)yb[i];
+        dc += uab[i];
+    }
+    // L_p: category sums exchanged through LDS
+    double* buf = sx + (k & 1) * C * TILE;
+    ++k;
+    buf[c * TILE + lane] = cw[c] * dc;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    double D = 0.0;
+    for (int cc = 0; cc < C; ++cc) D += buf[cc * TILE + lane];
+    const double coef = in ? cw[c] * w / D : 0.0;
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+        const R(&x)[5] = ch == 0 ? xa : xb;
+        const R(&y)[4] = ch == 0 ? yb : ya;   // the sibling's P x
+        double top[4], xv[4], al[4], be[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            top[i] = (double)o.u[i] * (double)y[i];
+            // a missing tip state reads the pad column, which does not
+            // depend on Q: its one-hot part is zero, so it adds nothing
+            xv[i] = (double)x[i];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            al[q] = coef * (sV[0 * 4 + q] * top[0] + sV[1 * 4 + q] * top[1] +
+                            sV[2 * 4 + q] * top[2] + sV[3 * 4 + q] * top[3]);
+            be[q] = sV[16 + q * 4 + 0] * xv[0] + sV[16 + q * 4 + 1] * xv[1] +
+                    sV[16 + q * 4 + 2] * xv[2] + sV[16 + q * 4 + 3] * xv[3];
+        }
+        const double* F = lf + ch * 16;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) H[q * 4 + r] += F[q * 4 + r] * al[q] * be[r];
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(512) void k_qgrad4(const NodeJob* __restrict__ jobs, int njobs, int P,
+                                                int C, long cs, const R* __restrict__ jm,
+                                                const double* __restrict__ jf,
+                                                const double* __restrict__ eig,
+                                                const R* dummyX, const int* dummyS,
+                                                const double* __restrict__ cw,
+                                                const double* __restrict__ pw, double* partial) {
+    // LDS: [2][C][TILE] exchange, V | V^-1 (32), [C][2][32] F blocks, then
+    // [C][2] matrix blocks
+    extern __shared__ double sx[];
+    double* sV = sx + 2 * C * TILE;
+    double* sfb = sV + 32;
+    R* smat = reinterpret_cast<R*>(sfb + C * 2 * 32);
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x;
+    const int p = tile * TILE + lane;
+    if (threadIdx.x < 32) sV[threadIdx.x] = eig[threadIdx.x];
+    __syncthreads();
+    const unsigned off = (unsigned)((c * cs + (long)p * 4) * (long)sizeof(R));
+    const unsigned doff = (unsigned)((c * TILE + lane) * 4 * (int)sizeof(R));
+    const double w = pw[p];
+    double H[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) H[q] = 0.0;
+    int k = 0;
+    QOps4<R> S0, S1;
+    NodeJob J0 = jobs[0], J1;
+    q_issue<R>(J0, 0, dummyX, dummyS, jm, jf, c, C, off, doff, p, lane, S0);
+    for (int o = 0;; o += 2) {
+        const int o1 = min(o + 1, njobs - 1);
+        J1 = jobs[o1];
+        q_issue<R>(J1, o1, dummyX, dummyS, jm, jf, c, C, off, doff, p, lane, S1);
+        q_compute<R>(J0, S0, H, smat, sfb, sx, sV, k, c, lane, p, P, C, cw, w);
+        if (o + 1 >= njobs) break;
+        const int o2 = min(o + 2, njobs - 1);
+        J0 = jobs[o2];
+        q_issue<R>(J0, o2, dummyX, dummyS, jm, jf, c, C, off, doff, p, lane, S0);
+        q_compute<R>(J1, S1, H, smat, sfb, sx, sV, k, c, lane, p, P, C, cw, w);
+        if (o + 2 >= njobs) break;
+    }
+    // per-tile, per-category sums of H: rows kl of [16][tiles * C]
+    const int ntiles = gridDim.x;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const double s = wave_sum_dpp(H[q]);
+        if (lane == 0) partial[(long)q * ntiles * C + (long)tile * C + c] = s;
+    }
+}
+
+template <typename R>
+void launch_qgrad(const NodeJob* djobs, int njobs, const Dims& d, const void* jobMats,
+                  const double* jobF, const double* eig, void* dummy, const double* cw,
+                  const double* pw, double* partial, hipStream_t st) {
+    if (njobs <= 0) return;
+    const R* dummyX = reinterpret_cast<const R*>(dummy);
+    R* dummyW = const_cast<R*>(dummyX) + 8 * TILE * 4;
+    const double* dummyD = reinterpret_cast<const double*>(dummyW + 8 * TILE * 4);
+    const int* dummyS = reinterpret_cast<const int*>(dummyD + TILE);
+    const int tiles = (d.P + TILE - 1) / TILE;
+    const size_t lds = (size_t)(2 * d.C * TILE + 32 + d.C * 2 * 32) * sizeof(double) +
+                       (size_t)d.C * 2 * NODE_MAT_BLOCK;
+    hipLaunchKernelGGL((k_qgrad4<R>), dim3(tiles), dim3(TILE * d.C), lds, st, djobs, njobs, d.P,
+                       d.C, d.catStride, reinterpret_cast<const R*>(jobMats), jobF, eig, dummyX,
+                       dummyS, cw, pw, partial);
+}
+
+// dlogL/dpi_i through the root distribution: sum_p w_p (sum_c w_c R_cpi) /
+// (sum_i pi_i sum_c w_c R_cpi), one thread per pattern, block sums per i
+// (rows of [S][blocks]).  Scale factors of the root partials cancel.
+template <typename R>
+__global__ __launch_bounds__(WG) void k_root_freq_grad(const R* __restrict__ root,
+                                                        const double* __restrict__ cw,
+                                                        const double* __restrict__ freqs,
+                                                        const double* __restrict__ pw,
+                                                        double* __restrict__ partial, Dims d) {
+    __shared__ double sh[WG / 64];
+    const int p = blockIdx.x * WG + threadIdx.x;
+    const int nb = gridDim.x;
+    double L = 0.0;
+    if (p < d.P) {
+        for (int i = 0; i < d.S; ++i) {
+            double g = 0.0;
+            for (int c = 0; c < d.C; ++c) g += cw[c] * (double)root[c * d.catStride + (long)p * d.Sp + i];
+            L += freqs[i] * g;
+        }
+    }
+    for (int i = 0; i < d.S; ++i) {
+        double v = 0.0;
+        if (p < d.P) {
+            double g = 0.0;
+            for (int c = 0; c < d.C; ++c) g += cw[c] * (double)root[c * d.catStride + (long)p * d.Sp + i];
+            v = pw[p] * g / L;
+        }
+        const double s = block_sum(v, sh);
+        if (threadIdx.x == 0) partial[(long)i * nb + blockIdx.x] = s;
+    }
+}
+
+template <typename R>
+void launch_root_freq_grad(const void* root, const double* cw, const double* freqs,
+                           const double* pw, double* partial, const Dims& d, hipStream_t st) {
+    const int nb = (d.P + WG - 1) / WG;
+    hipLaunchKernelGGL(k_root_freq_grad<R>, dim3(nb), dim3(WG), 0, st,
+                       reinterpret_cast<const R*>(root), cw, freqs, pw, partial, d);
+}
 
 // ---------------------------------------------------------------------------
 // generic state count: one launch per operation, one lane per output
@@ -1912,6 +2148,7 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
 // explicit instantiations
 
 #define INSTANTIATE(R)                                                                          \
+    template void launch_convolve<R>(const ConvolveJob*, int, int, int, hipStream_t);           \
     template void launch_matrices<R>(const MatrixJob*, int, int, int, hipStream_t);             \
     template void launch_post<R>(const PostJob*, const PostJob*, int, const Dims&, const void*,  \
                                  void*, hipStream_t);                                            \
@@ -1922,6 +2159,11 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
                                 hipStream_t);                                                    \
     template void launch_gather_node_matrices<R>(const NodeJob*, int, const Dims&, const void*, \
                                                  void*, hipStream_t);                        \
+    template void launch_qgrad<R>(const NodeJob*, int, const Dims&, const void*, const double*,  \
+                                  const double*, void*, const double*, const double*, double*,  \
+                                  hipStream_t);                                                 \
+    template void launch_root_freq_grad<R>(const void*, const double*, const double*,          \
+                                           const double*, double*, const Dims&, hipStream_t);   \
     template void launch_pre_nodes<R>(const NodeJob*, int, bool, const Dims&, const void*,     \
                                       void*, const double*, const double*, double*,         \
                                       hipStream_t);                                         \

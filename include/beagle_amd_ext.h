@@ -75,6 +75,33 @@ int beagleExtCalculateLogLikelihoodAndDerivatives(
     const int* secondDerivativeIndices, double* outLogLikelihood, double* outSumFirstDerivatives,
     double* outSumSquaredFirstDerivatives, double* outSumSecondDerivatives);
 
+/* Gradient of the log-likelihood with respect to the rate matrix of one
+ * eigen-system and to the root frequencies.  `operations` are the tree's
+ * pre-order operations (as given to beagleUpdatePrePartials) with
+ * postBufferIndices[o] the post-order buffer of operation o's node; the
+ * pre-order partials of every parent must be current (after
+ * beagleUpdatePrePartials or one of the derivative extensions; the
+ * operations' own destinations are not read), the post-order partials
+ * too, and every P_self must have been computed by
+ * beagleUpdateTransitionMatrices(WithMultipleModels) from eigen-system
+ * eigenIndex.  With Q = V diag(lambda) V^-1 as given to
+ * beagleSetEigenDecomposition and P = exp(r_c t Q):
+ *   outGradientRateMatrix[i*S + j] = dlogL / dQ_ij   (S*S, may be NULL)
+ *   outGradientFrequencies[i] = dlogL / dpi_i through the root
+ *       distribution pi = stateFrequenciesIndex of root buffer
+ *       rootBufferIndex (S, may be NULL).
+ * Exact (Frechet derivative of the matrix exponential through the
+ * eigen-system), unlike beagleCalculateCrossProductDerivative.  Chain rules
+ * to exchangeabilities and frequencies inside Q belong to the caller.
+ * 4 states; otherwise BEAGLE_ERROR_NO_IMPLEMENTATION.  No counterpart in
+ * the reference. */
+int beagleExtCalculateRateMatrixGradient(int instance, int eigenIndex,
+                                          const BeagleOperation* operations, int operationCount,
+                                          const int* postBufferIndices, int categoryWeightsIndex,
+                                          int rootBufferIndex, int stateFrequenciesIndex,
+                                          double* outGradientRateMatrix,
+                                          double* outGradientFrequencies);
+
 #ifdef __cplusplus
 }
 #endif

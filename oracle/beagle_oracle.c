@@ -392,6 +392,85 @@ int orcUpdateTransitionMatricesWithMultipleModels(int instance, const int* eigen
     return ORC_SUCCESS;
 }
 
+/* Category l uses eigen-system eigenIdx[l] with rate 1
+ * (EigenDecompositionCube.hpp:326-470, via BeagleCPUImpl.hpp:1242-1253):
+ * the per-category form of update_matrices. */
+int orcUpdateTransitionMatricesWithModelCategories(int instance, const int* eigenIdx,
+                                                   const int* probIdx, const int* d1Idx,
+                                                   const int* d2Idx, const double* edgeLengths,
+                                                   int count) {
+    Orc* o = get(instance);
+    if (!o) return ORC_ERROR_UNINITIALIZED_INSTANCE;
+    int S = o->S;
+    for (int l = 0; l < o->C; l++)
+        if (eigenIdx[l] < 0 || eigenIdx[l] >= o->eigenCount) return ORC_ERROR_OUT_OF_RANGE;
+    double* ex = (double*)xcalloc(S, sizeof(double));
+    double* e1 = (double*)xcalloc(S, sizeof(double));
+    double* e2 = (double*)xcalloc(S, sizeof(double));
+    for (int u = 0; u < count; u++) {
+        int d1 = d1Idx ? d1Idx[u] : -1;
+        int d2 = (d1Idx && d2Idx) ? d2Idx[u] : -1;
+        for (int l = 0; l < o->C; l++) {
+            int e = eigenIdx[l];
+            for (int k = 0; k < S; k++) {
+                double lam = o->evals[e][k];
+                ex[k] = exp(lam * edgeLengths[u]);
+                e1[k] = lam * ex[k];
+                e2[k] = lam * e1[k];
+            }
+            const double* cm = o->cube[e];
+            double* Pm = o->matrices[probIdx[u]] + (size_t)l * S * (S + 1);
+            double* D1 = d1 >= 0 ? o->matrices[d1] + (size_t)l * S * (S + 1) : NULL;
+            double* D2 = d2 >= 0 ? o->matrices[d2] + (size_t)l * S * (S + 1) : NULL;
+            int m = 0;
+            for (int i = 0; i < S; i++) {
+                for (int j = 0; j < S; j++) {
+                    double s0 = 0, s1 = 0, s2 = 0;
+                    for (int k = 0; k < S; k++, m++) {
+                        s0 += cm[m] * ex[k];
+                        s1 += cm[m] * e1[k];
+                        s2 += cm[m] * e2[k];
+                    }
+                    Pm[i * (S + 1) + j] = s0 > 0 ? s0 : 0;
+                    if (D1) D1[i * (S + 1) + j] = s1;
+                    if (D2) D2[i * (S + 1) + j] = s2;
+                }
+                Pm[i * (S + 1) + S] = 1.0;
+                if (D1) D1[i * (S + 1) + S] = 0.0;
+                if (D2) D2[i * (S + 1) + S] = 0.0;
+            }
+        }
+    }
+    free(ex); free(e1); free(e2);
+    return ORC_SUCCESS;
+}
+
+/* result[l] = first[l] * second[l] (matrix product per category), pad
+ * column 1; in-place results are refused (BeagleCPUImpl.hpp:1096-1162). */
+int orcConvolveTransitionMatrices(int instance, const int* first, const int* second,
+                                  const int* result, int count) {
+    Orc* o = get(instance);
+    if (!o) return ORC_ERROR_UNINITIALIZED_INSTANCE;
+    int S = o->S, W = S + 1;
+    for (int u = 0; u < count; u++) {
+        if (first[u] == result[u] || second[u] == result[u]) return ORC_ERROR_OUT_OF_RANGE;
+        for (int l = 0; l < o->C; l++) {
+            const double* A = o->matrices[first[u]] + (size_t)l * S * W;
+            const double* B = o->matrices[second[u]] + (size_t)l * S * W;
+            double* Cm = o->matrices[result[u]] + (size_t)l * S * W;
+            for (int i = 0; i < S; i++) {
+                for (int j = 0; j < S; j++) {
+                    double sum = 0.0;
+                    for (int k = 0; k < S; k++) sum += A[i * W + k] * B[k * W + j];
+                    Cm[i * W + j] = sum;
+                }
+                Cm[i * W + S] = 1.0;
+            }
+        }
+    }
+    return ORC_SUCCESS;
+}
+
 /* Client matrices [C][S][S] plus the given pad value (BeagleCPUImpl.hpp:1035-1091). */
 int orcSetTransitionMatrix(int instance, int m, const double* in, double padValue) {
     Orc* o = get(instance);

@@ -345,3 +345,72 @@ def test_full_size_consistency(amd, shape):
     assert rel(lnl32, res["none"][0]) < 1e-6
     for a, b in zip(g32[1:], base[1:]):
         assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("S,precision", [(4, "double"), (20, "double"), (4, "single")])
+def test_matrix_utilities_match_oracle(amd, oracle, S, precision):
+    """Model-category matrices and convolution against the oracle."""
+    rng = np.random.default_rng(11 + S)
+    C = 3
+    models = [synth.gtr(rng, S) for _ in range(C)]
+    out = []
+    for lib, kw in ((amd, {"requirement": api.FLAG_PRECISION_SINGLE if precision == "single"
+                           else api.FLAG_PRECISION_DOUBLE}), (oracle, {})):
+        inst = lib.create(2, 3, 2, S, 8, C, 12, C, 1, **kw)
+        for e, m in enumerate(models):
+            inst.set_eigen(e, m.evec.reshape(-1), m.ivec.reshape(-1), m.evals)
+        idx = np.arange(3)
+        inst.update_transition_matrices_model_categories(list(range(C)), idx, idx + 3, idx + 6,
+                                                         np.array([0.03, 0.4, 1.1]))
+        inst.convolve_transition_matrices([0, 1], [1, 2], [9, 10])
+        out.append(np.stack([inst.get_transition_matrix(i) for i in list(range(9)) + [9, 10]]))
+        inst.finalize()
+    tol = 1e-12 if precision == "double" else 2e-6
+    np.testing.assert_allclose(out[0], out[1], rtol=0, atol=tol * np.abs(out[1]).max())
+
+
+def _oracle_loglik(oracle, prob):
+    ev = driver.Evaluator(oracle, prob, preorder=False, transpose="none", matrices="explicit")
+    lnl = ev.log_likelihood()
+    ev.close()
+    return lnl
+
+
+def _with_model(prob, Q=None, freqs=None):
+    import dataclasses
+    m = dataclasses.replace(prob.model, Q=prob.model.Q if Q is None else Q,
+                            freqs=prob.model.freqs if freqs is None else freqs)
+    return dataclasses.replace(prob, model=m)
+
+
+@pytest.mark.parametrize("scaling,precision", [("none", "double"), ("auto", "double"),
+                                               ("auto", "single")])
+def test_rate_matrix_gradient_against_oracle_finite_differences(amd, oracle, scaling, precision):
+    """dlogL/dQ and dlogL/dpi_root from the extension against central
+    differences of the oracle's log-likelihood with P = expm(r t (Q + h E))
+    set explicitly, for random directions E (not generators), and pi_root
+    moved one component at a time."""
+    prob = synth.make_problem(61, 9, 400, S=4, shape="yule", missing=0.05, mean_length=0.2)
+    ev = driver.Evaluator(amd, prob, scaling=scaling, precision=precision)
+    ev.fused_gradient(keep_tips=False)
+    G, gpi = ev.rate_matrix_gradient()
+    ev.close()
+    rng = np.random.default_rng(3)
+    # curvature is large (|dlogL/dQ| ~ 1e5): h = 2.5e-5 with a fourth-order
+    # stencil converges to ~1e-10 relative
+    h = 2.5e-5
+    tol = 1e-8 if precision == "double" else 2e-5
+
+    def fd4(f):
+        # fourth-order central difference
+        return (8 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12 * h)
+
+    for _ in range(3):
+        E = rng.normal(size=(4, 4))
+        fd = fd4(lambda s: _oracle_loglik(oracle, _with_model(prob, Q=prob.model.Q + s * E)))
+        got = float(np.sum(G * E))
+        assert abs(got - fd) <= tol * max(abs(fd), np.abs(G).max()), (got, fd)
+    for i in range(4):
+        e = np.eye(4)[i]
+        fd = fd4(lambda s: _oracle_loglik(oracle, _with_model(prob, freqs=prob.model.freqs + s * e)))
+        assert abs(gpi[i] - fd) <= tol * max(abs(fd), np.abs(gpi).max()), (i, gpi[i], fd)

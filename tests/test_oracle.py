@@ -340,3 +340,45 @@ def test_multiple_model_matrices_against_reference(oracle, ref):
         out.append(np.stack([inst.get_transition_matrix(i) for i in range(12)]))
         inst.finalize()
     np.testing.assert_allclose(out[0], out[1], rtol=1e-12, atol=1e-14)
+
+
+def _matrix_utilities(lib, S):
+    """updateTransitionMatricesWithModelCategories (category l uses eigen
+    buffer l, rate 1) and convolveTransitionMatrices."""
+    rng = np.random.default_rng(11 + S)
+    C = 3
+    models = [synth.gtr(rng, S) for _ in range(C)]
+    inst = lib.create(2, 3, 2, S, 8, C, 12, C, 1)
+    try:
+        for e, m in enumerate(models):
+            inst.set_eigen(e, m.evec.reshape(-1), m.ivec.reshape(-1), m.evals)
+        idx = np.arange(3)
+        inst.update_transition_matrices_model_categories(list(range(C)), idx, idx + 3, idx + 6,
+                                                         np.array([0.03, 0.4, 1.1]))
+        inst.convolve_transition_matrices([0, 1], [1, 2], [9, 10])
+        return np.stack([inst.get_transition_matrix(i) for i in list(range(9)) + [9, 10]])
+    finally:
+        inst.finalize()
+
+
+@pytest.mark.parametrize("S", [4, 20])
+def test_matrix_utilities_against_reference(oracle, ref, S):
+    a = _matrix_utilities(oracle, S)
+    b = _matrix_utilities(ref, S)
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-14)
+
+
+def test_model_categories_closed_form(oracle):
+    """Model categories: category l's P(t) = expm(Q_l t), rate 1; the
+    convolution of P(s) and P(t) from one model is P(s + t)."""
+    from scipy.linalg import expm
+    got = _matrix_utilities(oracle, 4)
+    rng = np.random.default_rng(15)
+    models = [synth.gtr(rng, 4) for _ in range(3)]
+    for u, t in enumerate([0.03, 0.4, 1.1]):
+        P = got[u].reshape(3, 4, 4)
+        for l, m in enumerate(models):
+            np.testing.assert_allclose(P[l], expm(m.Q * t), atol=1e-12)
+    conv = got[9].reshape(3, 4, 4)
+    for l, m in enumerate(models):
+        np.testing.assert_allclose(conv[l], expm(m.Q * 0.43), atol=1e-12)
