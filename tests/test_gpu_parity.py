@@ -414,3 +414,57 @@ def test_rate_matrix_gradient_against_oracle_finite_differences(amd, oracle, sca
         e = np.eye(4)[i]
         fd = fd4(lambda s: _oracle_loglik(oracle, _with_model(prob, freqs=prob.model.freqs + s * e)))
         assert abs(gpi[i] - fd) <= tol * max(abs(fd), np.abs(gpi).max()), (i, gpi[i], fd)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_torch_autograd_binding(amd, oracle, precision):
+    """beagle_amd.torch_fn: logL of (branch lengths, GTR(exch, freqs)) and
+    its autograd gradients against the oracle's log-likelihood and
+    fourth-order central differences of it."""
+    import torch
+    from beagle_amd import torch_fn
+    prob = synth.make_problem(71, 8, 300, S=4, shape="yule", missing=0.05, mean_length=0.2)
+    lik = torch_fn.PhyloLikelihood(amd, prob.tree, prob.tips, prob.pattern_weights, prob.rates,
+                                   prob.weights, precision=precision)
+    rng = np.random.default_rng(9)
+    exch0 = rng.exponential(1.0, 6)
+    freqs0 = rng.dirichlet(np.ones(4))
+    lengths0 = prob.tree.lengths[:-1].copy()
+    exch = torch.tensor(exch0, requires_grad=True)
+    freqs = torch.tensor(freqs0, requires_grad=True)
+    lengths = torch.tensor(lengths0, requires_grad=True)
+    Q = torch_fn.gtr_rate_matrix(exch, freqs)
+    logl = lik(lengths, Q, freqs)
+    logl.backward()
+    lik.close()
+
+    def oracle_logl(e, f, t):
+        import dataclasses
+        Qn = torch_fn.gtr_rate_matrix(torch.tensor(e), torch.tensor(f)).numpy()
+        m = dataclasses.replace(prob.model, Q=Qn, freqs=f)
+        tt = np.append(t, 0.0)
+        p = dataclasses.replace(prob, model=m,
+                                tree=dataclasses.replace(prob.tree, lengths=tt))
+        ev = driver.Evaluator(oracle, p, preorder=False, transpose="none", matrices="explicit")
+        v = ev.log_likelihood()
+        ev.close()
+        return v
+
+    ref = oracle_logl(exch0, freqs0, lengths0)
+    assert rel(float(logl), ref) < (1e-10 if precision == "double" else 1e-6)
+    h = 1e-4
+
+    def fd4(f):
+        return (8 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12 * h)
+
+    tol = 1e-6 if precision == "double" else 2e-4
+    for name, x0, grad, call in (
+            ("exch", exch0, exch.grad, lambda x: oracle_logl(x, freqs0, lengths0)),
+            ("freqs", freqs0, freqs.grad, lambda x: oracle_logl(exch0, x, lengths0)),
+            ("lengths", lengths0, lengths.grad, lambda x: oracle_logl(exch0, freqs0, x))):
+        g = grad.numpy()
+        for i in range(0, len(x0), max(1, len(x0) // 5)):
+            e = np.zeros(len(x0))
+            e[i] = 1.0
+            fd = fd4(lambda s: call(x0 + s * e))
+            assert abs(g[i] - fd) <= tol * max(abs(fd), np.abs(g).max()), (name, i, g[i], fd)
