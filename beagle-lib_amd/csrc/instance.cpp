@@ -196,6 +196,7 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
     HIPCHECK(hipHostMalloc((void**)&hResult_, 64 * sizeof(double), hipHostMallocDefault));
     partStart_ = {0, patternCount};
     if (const char* m = std::getenv("BEAGLE_AMD_TRAVERSAL")) traversalMode_ = std::atoi(m);
+    if (const char* m = std::getenv("BEAGLE_AMD_LEVEL_CHUNK")) levelChunk_ = std::max(1, std::atoi(m));
     ring_.resize(8);
     for (auto& s : ring_) HIPCHECK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
     sync();
@@ -744,8 +745,8 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
     const bool fourState = dims_.S == 4 && dims_.C <= 8;
     if (!fourState) {
         auto d = reinterpret_cast<const PostJob*>(upload(hPost_.data(), count * sizeof(PostJob)));
-        if (dp_) launch_post<double>(d, hPost_.data(), count, dims_, dMatrices_, dDummy_, stream_);
-        else launch_post<float>(d, hPost_.data(), count, dims_, dMatrices_, dDummy_, stream_);
+        if (dp_) launch_post<double>(d, hPost_.data(), count, 0, dims_, dMatrices_, dDummy_, stream_);
+        else launch_post<float>(d, hPost_.data(), count, 0, dims_, dMatrices_, dDummy_, stream_);
     } else {
         // A shared cumulative scale buffer is updated by every rescaling op of
         // a pattern, so such lists stay in one fused launch (ops of a pattern
@@ -818,14 +819,16 @@ void Instance::launchPostPlan(const std::vector<Segment>& plan) {
     if (all.empty()) return;
     auto d = reinterpret_cast<const PostJob*>(upload(all.data(), all.size() * sizeof(PostJob)));
     for (const Launch& L : launches) {
-        if (L.batched) {
+        PostJob* h = all.data() + L.first;
+        if (L.batched && levelChunk_ <= 1) {
             const int start[2] = {0, L.n};
             if (dp_) launch_post_levels<double>(d + L.first, start, 1, dims_, dMatrices_, stream_);
             else launch_post_levels<float>(d + L.first, start, 1, dims_, dMatrices_, stream_);
         } else {
-            PostJob* h = all.data() + L.first;
-            if (dp_) launch_post<double>(d + L.first, h, L.n, dims_, dMatrices_, dDummy_, stream_);
-            else launch_post<float>(d + L.first, h, L.n, dims_, dMatrices_, dDummy_, stream_);
+            // fused list, or a batched level walked in chunks of independent ops
+            const int chunk = L.batched ? levelChunk_ : 0;
+            if (dp_) launch_post<double>(d + L.first, h, L.n, chunk, dims_, dMatrices_, dDummy_, stream_);
+            else launch_post<float>(d + L.first, h, L.n, chunk, dims_, dMatrices_, dDummy_, stream_);
         }
     }
 }
@@ -1205,8 +1208,8 @@ bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, dou
     if (dp_) launch_gather_node_matrices<double>(d, (int)all.size(), dims_, dMatrices_, jm, stream_);
     else launch_gather_node_matrices<float>(d, (int)all.size(), dims_, dMatrices_, jm, stream_);
     for (const Launch& L : launches) {
-        if (dp_) launch_pre_nodes<double>(d + L.first, L.n, L.batched, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
-        else launch_pre_nodes<float>(d + L.first, L.n, L.batched, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
+        if (dp_) launch_pre_nodes<double>(d + L.first, L.n, L.batched, levelChunk_, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
+        else launch_pre_nodes<float>(d + L.first, L.n, L.batched, levelChunk_, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
     }
     return true;
 }

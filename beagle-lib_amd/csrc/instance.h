@@ -182,6 +182,7 @@ private:
     bool logScalers_;
     int preWeights_ = 0;      // category-weights buffer of the fused derivative pass
     int traversalMode_ = 0;   // 0 automatic, 1 always per dependency level, 2 always fused
+    int levelChunk_ = 1;      // batched level: 1 one op per workgroup, k > 1 chunks of k ops
     Dims dims_;
     size_t partialsBytes_, matrixBytes_;
     hipStream_t stream_ = nullptr;

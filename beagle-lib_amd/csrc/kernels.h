@@ -125,7 +125,7 @@ void launch_convolve(const ConvolveJob* jobs, int njobs, int S, int C, hipStream
 template <typename R>
 // dummy: the instance's scratch tile (dummy_bytes()), read and written by
 // operand slots and lanes that have no real data.
-void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Dims& d,
+void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, int chunk, const Dims& d,
                  const void* mats, void* dummy, hipStream_t st);
 inline size_t dummy_bytes() { return 2 * 8 * 64 * 4 * sizeof(double) + 64 * (8 + 4) + 128 * 2; }
 // 4-state post-order by dependency level: jobs are ordered by level and
@@ -148,7 +148,7 @@ void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, 
 // runs the njobs jobs (one dependency level) in one grid over (tiles, jobs),
 // otherwise one launch walks the list in order.
 template <typename R>
-void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, const Dims& d,
+void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, const Dims& d,
                       const void* jobMats, void* dummy, const double* cw, const double* pw,
                       double* partial, hipStream_t st);
 // Rate-matrix gradient over node jobs (4 states): per-tile, per-category

@@ -418,11 +418,15 @@ __device__ __forceinline__ void post_compute(const PostJob& J, const Operand4<R>
 }
 
 template <typename R, bool AUTO>
-__global__ __launch_bounds__(512) void k_post4(const PostJob* __restrict__ jobs, int njobs, int P,
-                                               int C, long cs, const R* __restrict__ mats,
-                                               const R* dummyX, const int* dummyS, R* dummyW,
+__global__ __launch_bounds__(512) void k_post4(const PostJob* __restrict__ jobs, int njobs,
+                                               int chunk, int P, int C, long cs,
+                                               const R* __restrict__ mats, const R* dummyX,
+                                               const int* dummyS, R* dummyW,
                                                const double* dummyD, const short* dummyE) {
     __shared__ R smax[2][8][TILE];
+    // workgroup (tile, y) walks operations [y chunk, (y + 1) chunk)
+    jobs += blockIdx.y * chunk;
+    njobs = min(chunk, njobs - (int)blockIdx.y * chunk);
     // wave index = category; readfirstlane makes it an SGPR so matrix
     // addresses are provably uniform and become scalar loads
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -979,12 +983,16 @@ __device__ __forceinline__ void node_compute(const NodeJob& J, const NodeOps4<R>
     R pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
 
 template <typename R, bool AUTO>
-__global__ __launch_bounds__(512) void k_pren4(const NodeJob* __restrict__ jobs, int njobs, int P,
-                                               int C, long cs, const R* __restrict__ jm,
-                                               const R* dummyX, const int* dummyS, R* dummyW,
+__global__ __launch_bounds__(512) void k_pren4(const NodeJob* __restrict__ jobs, int njobs,
+                                               int chunk, int P, int C, long cs,
+                                               const R* __restrict__ jm, const R* dummyX,
+                                               const int* dummyS, R* dummyW,
                                                const short* dummyE, const double* __restrict__ cw,
                                                const double* __restrict__ pw, double* partial) {
     NODE_KERNEL_SETUP
+    // workgroup (tile, y) walks jobs [y chunk, (y + 1) chunk)
+    jobs += blockIdx.y * chunk;
+    njobs = min(chunk, njobs - (int)blockIdx.y * chunk);
     NodeOps4<R> S0, S1;
     NodeJob J0 = jobs[0], J1;
     node_issue<R, AUTO>(J0, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
@@ -1369,8 +1377,8 @@ void launch_post_levels(const PostJob* djobs, const int* levelStart, int nlevels
 }
 
 template <typename R>
-void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Dims& d,
-                 const void* mats, void* dummy, hipStream_t st) {
+void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, int chunk,
+                 const Dims& d, const void* mats, void* dummy, hipStream_t st) {
     // dummy: scratch laid out as [read tile 8x64x4 R][write tile 8x64x4 R]
     //        [64 doubles][64 ints][128 shorts]
     const R* dummyX = reinterpret_cast<const R*>(dummy);
@@ -1380,15 +1388,16 @@ void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, const Di
     const short* dummyE = reinterpret_cast<const short*>(dummyS + TILE);
     if (njobs <= 0) return;
     if (d.S == 4 && d.C <= 8) {
-        dim3 grid((d.P + TILE - 1) / TILE);
+        if (chunk <= 0) chunk = njobs;
+        dim3 grid((d.P + TILE - 1) / TILE, (njobs + chunk - 1) / chunk);
         if (d.autoScale)
             hipLaunchKernelGGL((k_post4<R, true>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
-                               d.P, d.C, d.catStride, reinterpret_cast<const R*>(mats), dummyX,
-                               dummyS, dummyW, dummyD, dummyE);
+                               chunk, d.P, d.C, d.catStride, reinterpret_cast<const R*>(mats),
+                               dummyX, dummyS, dummyW, dummyD, dummyE);
         else
             hipLaunchKernelGGL((k_post4<R, false>), grid, dim3(TILE * d.C), 0, st, djobs, njobs,
-                               d.P, d.C, d.catStride, reinterpret_cast<const R*>(mats), dummyX,
-                               dummyS, dummyW, dummyD, dummyE);
+                               chunk, d.P, d.C, d.catStride, reinterpret_cast<const R*>(mats),
+                               dummyX, dummyS, dummyW, dummyD, dummyE);
         return;
     }
     for (int o = 0; o < njobs; ++o) {
@@ -1466,7 +1475,7 @@ void launch_gather_node_matrices(const NodeJob* djobs, int njobs, const Dims& d,
 }
 
 template <typename R>
-void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, const Dims& d,
+void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, const Dims& d,
                       const void* jobMats, void* dummy, const double* cw, const double* pw,
                       double* partial, hipStream_t st) {
     if (njobs <= 0) return;
@@ -1480,12 +1489,16 @@ void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, const Dims&
     const size_t lds = (size_t)2 * d.C * 8 * TILE * sizeof(double) +
                        (size_t)d.C * 2 * NODE_MAT_BLOCK;
     dim3 block(TILE * d.C);
-    if (batched) {
+    if (batched && chunk <= 1) {
         if (d.autoScale) hipLaunchKernelGGL((k_pren4_level<R, true>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
         else hipLaunchKernelGGL((k_pren4_level<R, false>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
     } else {
-        if (d.autoScale) hipLaunchKernelGGL((k_pren4<R, true>), dim3(tiles), block, lds, st, djobs, njobs, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
-        else hipLaunchKernelGGL((k_pren4<R, false>), dim3(tiles), block, lds, st, djobs, njobs, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
+        // a fused list (chunk = the whole list) or a batched level walked in
+        // chunks of independent jobs
+        if (!batched || chunk > njobs) chunk = njobs;
+        const dim3 grid(tiles, (njobs + chunk - 1) / chunk);
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4<R, true>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
+        else hipLaunchKernelGGL((k_pren4<R, false>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
     }
 }
 
@@ -2150,7 +2163,8 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
 #define INSTANTIATE(R)                                                                          \
     template void launch_convolve<R>(const ConvolveJob*, int, int, int, hipStream_t);           \
     template void launch_matrices<R>(const MatrixJob*, int, int, int, hipStream_t);             \
-    template void launch_post<R>(const PostJob*, const PostJob*, int, const Dims&, const void*,  \
+    template void launch_post<R>(const PostJob*, const PostJob*, int, int, const Dims&,         \
+                                 const void*,                                                    \
                                  void*, hipStream_t);                                            \
     template void launch_post_levels<R>(const PostJob*, const int*, int, const Dims&,            \
                                         const void*, hipStream_t);                               \
@@ -2164,7 +2178,7 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
                                   hipStream_t);                                                 \
     template void launch_root_freq_grad<R>(const void*, const double*, const double*,          \
                                            const double*, double*, const Dims&, hipStream_t);   \
-    template void launch_pre_nodes<R>(const NodeJob*, int, bool, const Dims&, const void*,     \
+    template void launch_pre_nodes<R>(const NodeJob*, int, bool, int, const Dims&, const void*, \
                                       void*, const double*, const double*, double*,         \
                                       hipStream_t);                                         \
     template void launch_pre_levels<R>(const PreJob*, const int*, int, const Dims&, const void*, \

@@ -248,19 +248,21 @@ def test_fused_pre_order_derivatives_match_oracle(amd, oracle, S, shape, scaling
         close(pre_tip, pre_tip_o, 1e-10 if precision == "double" else 1e-4, 0)
 
 
-@pytest.mark.parametrize("mode", ["1", "2"])
+@pytest.mark.parametrize("mode,chunk", [("1", "1"), ("2", "1"), ("0", "3"), ("1", "2")])
 @pytest.mark.parametrize("precision,scaling", [("double", "none"), ("double", "always"),
                                                ("single", "auto")])
-def test_traversal_plans_agree(amd, monkeypatch, mode, precision, scaling):
-    """Every dependency level batched (mode 1) and the whole list fused
-    (mode 2) give the automatic plan's logL, pre-order partials and
-    derivative sums bit for bit on a balanced tree (wide levels) and a
-    caterpillar (one op per level)."""
+def test_traversal_plans_agree(amd, monkeypatch, mode, chunk, precision, scaling):
+    """Every dependency level batched (mode 1), the whole list fused (mode
+    2), and batched levels walked in chunks of independent operations by
+    the pipelined kernels (chunk > 1) give the automatic plan's logL,
+    pre-order partials and derivative sums bit for bit on a balanced tree
+    (wide levels) and a caterpillar (one op per level)."""
     for shape in ("balanced", "caterpillar"):
         prob = synth.make_problem(5, 32, 3000, S=4, shape=shape)
         res = []
-        for m in ("0", mode):
+        for m, ch in (("0", "1"), (mode, chunk)):
             monkeypatch.setenv("BEAGLE_AMD_TRAVERSAL", m)
+            monkeypatch.setenv("BEAGLE_AMD_LEVEL_CHUNK", ch)
             ev = driver.Evaluator(amd, prob, scaling=scaling, precision=precision)
             lg, g1, g2 = ev.fused_gradient(keep_tips=True)
             ev.pre_order()
