@@ -19,7 +19,7 @@ COMMON = ["-std=c++17", "-O3", "-fPIC", "-fvisibility=hidden", "--offload-arch=g
 # packed FP32 instructions stay off (see the build notes in DESIGN.md)
 DEVICE = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["kernels.hip", "instance.cpp", "frontend.cpp"]
+SOURCES = ["kernels.hip", "mfma.hip", "instance.cpp", "frontend.cpp"]
 
 
 def run(cmd):

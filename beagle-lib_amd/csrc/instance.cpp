@@ -157,6 +157,7 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
     dims_.Ppad = roundUp(patternCount, 64);
     dims_.catStride = (long)dims_.Ppad * dims_.Sp;
     dims_.autoScale = auto_ ? 1 : 0;
+    dims_.sStride = stateCount > 4 ? PTILE : 1;
     partialsBytes_ = (size_t)categoryCount * dims_.catStride * realSize();
     matrixBytes_ = (size_t)categoryCount * stateCount * (stateCount + 1) * realSize();
     matrixBytes_ = (matrixBytes_ + 255) / 256 * 256;
@@ -224,6 +225,7 @@ Instance::~Instance() {
     hipHostFree(hResult_);
     hipFree(dWork_);
     hipFree(dJobMats_);
+    hipFree(dPack_);
     if (hStage_) hipHostFree(hStage_);
     hipFree(dScratch_);
     for (auto& s : ring_) {
@@ -351,7 +353,7 @@ int Instance::setTipPartials(int tip, const double* in) {
     for (int c = 0; c < dims_.C; ++c)
         for (int k = 0; k < dims_.P; ++k)
             for (int i = 0; i < dims_.S; ++i)
-                tmp[c * dims_.catStride + (size_t)k * dims_.Sp + i] = in[(size_t)k * dims_.S + i];
+                tmp[pbase(dims_, c, k) + (size_t)i * dims_.sStride] = in[(size_t)k * dims_.S + i];
     toDevice(dPartials_[tip], tmp.data(), n);
     clearExponent(tip);
     return BEAGLE_SUCCESS;
@@ -365,7 +367,7 @@ int Instance::setPartials(int b, const double* in) {
     for (int c = 0; c < dims_.C; ++c)
         for (int k = 0; k < dims_.P; ++k)
             for (int i = 0; i < dims_.S; ++i)
-                tmp[c * dims_.catStride + (size_t)k * dims_.Sp + i] =
+                tmp[pbase(dims_, c, k) + (size_t)i * dims_.sStride] =
                     in[((size_t)c * dims_.P + k) * dims_.S + i];
     toDevice(dPartials_[b], tmp.data(), n);
     clearExponent(b);
@@ -399,7 +401,7 @@ int Instance::getPartials(int b, int scaleIndex, double* out) {
         for (int k = 0; k < dims_.P; ++k)
             for (int i = 0; i < dims_.S; ++i)
                 out[((size_t)c * dims_.P + k) * dims_.S + i] =
-                    tmp[c * dims_.catStride + (size_t)k * dims_.Sp + i];
+                    tmp[pbase(dims_, c, k) + (size_t)i * dims_.sStride];
     if (scaleIndex != BEAGLE_OP_NONE) {
         std::vector<double> sc(dims_.P);
         copyOut(sc.data(), scale(scaleIndex), dims_.P * sizeof(double));
@@ -743,7 +745,9 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
     }
     if (count == 0) return BEAGLE_SUCCESS;
     const bool fourState = dims_.S == 4 && dims_.C <= 8;
-    if (!fourState) {
+    if (!fourState && !dp_ && mfma_post_supported(dims_.S, dims_.C)) {
+        launchPostMfma(ops, stride, count);
+    } else if (!fourState) {
         auto d = reinterpret_cast<const PostJob*>(upload(hPost_.data(), count * sizeof(PostJob)));
         if (dp_) launch_post<double>(d, hPost_.data(), count, 0, dims_, dMatrices_, dDummy_, stream_);
         else launch_post<float>(d, hPost_.data(), count, 0, dims_, dMatrices_, dDummy_, stream_);
@@ -759,17 +763,8 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
             for (int o = 0; o < count; ++o) s.ops.push_back(o);
             plan.push_back(std::move(s));
         } else {
-            std::vector<Access> acc(count);
             std::vector<int> level;
-            for (int o = 0; o < count; ++o) {
-                const int* q = ops + (size_t)o * stride;
-                const PostJob& J = hPost_[o];
-                const int sw = J.scaleWrite ? (int)((J.scaleWrite - dScale_) / dims_.Ppad) : -1;
-                const int sr = J.scaleRead ? (int)((J.scaleRead - dScale_) / dims_.Ppad) : -1;
-                acc[o] = Access{{q[3], q[5], sr >= 0 ? bufferCount_ + sr : -1},
-                                {q[0], sw >= 0 ? bufferCount_ + sw : -1}};
-            }
-            const int nl = dependencyLevels(acc, bufferCount_ + scaleCount_, level);
+            const int nl = postLevels(ops, stride, count, level);
             plan = planSegments(level, nl, traversalMode_ == 1 ? 1 : kNarrowLevel);
         }
         launchPostPlan(plan);
@@ -777,6 +772,59 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
     HIPCHECK(hipGetLastError());
     if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
     return BEAGLE_SUCCESS;
+}
+
+int Instance::postLevels(const int* ops, int stride, int count, std::vector<int>& level) {
+    std::vector<Access> acc(count);
+    for (int o = 0; o < count; ++o) {
+        const int* q = ops + (size_t)o * stride;
+        const PostJob& J = hPost_[o];
+        const int sw = J.scaleWrite ? (int)((J.scaleWrite - dScale_) / dims_.Ppad) : -1;
+        const int sr = J.scaleRead ? (int)((J.scaleRead - dScale_) / dims_.Ppad) : -1;
+        acc[o] = Access{{q[3], q[5], sr >= 0 ? bufferCount_ + sr : -1},
+                        {q[0], sw >= 0 ? bufferCount_ + sw : -1}};
+    }
+    return dependencyLevels(acc, bufferCount_ + scaleCount_, level);
+}
+
+// MFMA post-order (S > 4, FP32 partials): every child matrix of the list is
+// repacked once into the pack slab, then each dependency level is one
+// launch over (pattern tiles x ops).  A list whose ops share a cumulative
+// scale buffer runs one op per launch in list order (each rescaling op adds
+// to the same per-pattern sums).
+void Instance::launchPostMfma(const int* ops, int stride, int count) {
+    const size_t per = (size_t)mfma_pack_reals(dims_.S) * dims_.C;
+    std::vector<PackJob> pj(2 * (size_t)count);
+    for (int o = 0; o < count; ++o) {
+        PostJob& J = hPost_[o];
+        J.pk1 = (int)(2 * o * per);
+        J.pk2 = (int)((2 * o + 1) * per);
+        pj[2 * o] = PackJob{J.mo1, J.pk1, J.s1 ? PACK_ROWS : PACK_OPERAND};
+        pj[2 * o + 1] = PackJob{J.mo2, J.pk2, J.s2 ? PACK_ROWS : PACK_OPERAND};
+    }
+    float* pack = static_cast<float*>(
+        grow(reinterpret_cast<void*&>(dPack_), packCap_, pj.size() * per * sizeof(float)));
+    std::vector<int> level(count);
+    bool shared = false;
+    for (int o = 0; o < count; ++o) shared |= hPost_[o].cumulative != nullptr;
+    int nl;
+    if (shared) {
+        for (int o = 0; o < count; ++o) level[o] = o;
+        nl = count;
+    } else {
+        nl = postLevels(ops, stride, count, level);
+    }
+    std::vector<int> start(nl + 1, 0);
+    for (int o = 0; o < count; ++o) ++start[level[o] + 1];
+    for (int l = 0; l < nl; ++l) start[l + 1] += start[l];
+    std::vector<PostJob> sorted(count);
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int o = 0; o < count; ++o) sorted[fill[level[o]]++] = hPost_[o];
+    auto dpj = reinterpret_cast<const PackJob*>(upload(pj.data(), pj.size() * sizeof(PackJob)));
+    launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+    auto d = reinterpret_cast<const PostJob*>(upload(sorted.data(), count * sizeof(PostJob)));
+    for (int l = 0; l < nl; ++l)
+        launch_post_mfma(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
 }
 
 // Launch a 4-state post-order plan with one upload of all its jobs.

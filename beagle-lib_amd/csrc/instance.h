@@ -158,6 +158,11 @@ private:
     int attachOwnOperands(int count, const int* postIdx, const int* d1Idx, const int* d2Idx);
     void launchPre(const int* ops, int stride, int count, double* partial);
     void launchPostPlan(const std::vector<Segment>& plan);
+    // S > 4, FP32: matrix repacking, then one MFMA launch per dependency level
+    void launchPostMfma(const int* ops, int stride, int count);
+    // dependency levels of a post-order list (reads children and a read
+    // scale buffer; writes the destination and a write scale buffer)
+    int postLevels(const int* ops, int stride, int count, std::vector<int>& level);
     bool launchPreNodes(const int* ops, int count, const int* postIdx, double* partial);
     bool makeNodeJobs(const int* ops, int count, const int* postIdx, std::vector<NodeJob>& nodes,
                       std::vector<Access>& acc);
@@ -215,6 +220,8 @@ private:
     size_t stageCap_ = 0;             // doubles
     void* dJobMats_ = nullptr;        // grow-only gathered matrices of node jobs
     size_t jobMatsCap_ = 0;
+    float* dPack_ = nullptr;          // grow-only MFMA-packed matrices of a traversal
+    size_t packCap_ = 0;
     void* dScratch_ = nullptr;        // C*Ppad*Sp reals (generic pre-order)
     std::vector<int> partStart_;
     int partitionCount_ = 1;

@@ -37,6 +37,7 @@ struct PostJob {
     int mo1, mo2;              // m1, m2 as element offsets into the matrix slab
     int k1, k2;                // operand kinds: 0 partials, 1 tip states, 2 the
                                //   previous operation's output (same pattern range)
+    int pk1, pk2;              // MFMA path: m1, m2 repacked, offsets in the pack slab
 };
 
 // One pre-order operation: dest = P_self^T (pre_parent * (P_sib x_sib)).
@@ -105,15 +106,60 @@ struct EdgeJob {
     double scale;          // cross products: edge length
 };
 
+// Partials layout.  S <= 4: one row of Sp states per pattern,
+// [C][Ppad][Sp].  S > 4: pattern-minor tiles of PTILE patterns,
+// [C][Ppad/PTILE][S][PTILE], so that a wavefront reading one state of
+// consecutive patterns touches whole 128-byte lines (the MFMA kernels hold
+// one pattern per lane).  Element (c, p, i) sits at
+// pbase(d, c, p) + i * d.sStride.
+constexpr int PTILE = 32;
+
 struct Dims {
     int S;        // states
     int P;        // patterns
     int C;        // categories
-    int Ppad;     // padded pattern stride
-    int Sp;       // state stride in partials
+    int Ppad;     // padded pattern count (multiple of 64)
+    int Sp;       // states per pattern in storage (= S)
     long catStride;  // Ppad * Sp
     int autoScale;   // automatic power-of-two rescaling of every operation
+    int sStride;     // distance between states of one pattern: 1 or PTILE
 };
+
+__host__ __device__ inline long pbase(const Dims& d, int c, int p) {
+    return c * d.catStride + (d.sStride == 1 ? (long)p * d.Sp
+                                             : (long)(p / PTILE) * PTILE * d.Sp + p % PTILE);
+}
+
+// One child matrix of a post-order operation repacked for the MFMA kernels
+// (mfma.hip): src is the matrix's element offset in the matrix slab, dst
+// the block's offset in the pack slab (C * mfma_pack_reals(S) floats).
+struct PackJob {
+    int src, dst;
+    int kind;      // PACK_OPERAND: partials child, PACK_ROWS: tip-state child
+};
+enum PackKind : int { PACK_OPERAND = 0, PACK_ROWS = 1 };
+
+// MFMA path shapes: M blocks of 32 output states, K in groups of 8 (four
+// steps of the 32x32x2 FP32 instruction, k = 2 step + half-wave).
+__host__ __device__ inline int mfma_mblocks(int S) { return (S + 31) / 32; }
+__host__ __device__ inline int mfma_kgroups(int S) { return (S + 7) / 8; }
+// floats per category of one packed child matrix
+__host__ __device__ inline int mfma_pack_reals(int S) {
+    const int a = mfma_mblocks(S) * mfma_kgroups(S) * 256;   // operand blocks
+    const int t = (S + 1) * mfma_mblocks(S) * 32;            // rows by state
+    const int m = a > t ? a : t;
+    return (m + 63) / 64 * 64;
+}
+// S range and category count the MFMA post-order kernel covers
+__host__ __device__ inline bool mfma_post_supported(int S, int C) { return S > 4 && S <= 64 && C <= 8; }
+
+// ---- launchers (mfma.hip) ----
+template <typename R>
+void launch_pack_matrices(const PackJob* jobs, int njobs, int S, int C, const void* mats,
+                          float* pack, hipStream_t st);
+// One launch per dependency level: grid (pattern tiles, njobs).
+void launch_post_mfma(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
+                      hipStream_t st);
 
 // ---- launchers (kernels.hip) ----
 template <typename R>

@@ -1,8 +1,9 @@
 // gfx950 kernels of the MI355X BEAGLE plugin.
 //
-// Partials live in HBM as [category][pattern (padded to Ppad)][state
-// (stride Sp)], so consecutive lanes touch consecutive patterns and every
-// 4-state load/store is a full 16-byte vector access.  Transition matrices
+// Partials live in HBM as [category][pattern (padded to Ppad)][state] for
+// S <= 4, so consecutive lanes touch consecutive patterns and every 4-state
+// load/store is a full 16-byte vector access; for S > 4 as pattern-minor
+// tiles [category][pattern tile][state][32 patterns] (kernels.h, pbase).  Transition matrices
 // keep the reference CPU layout [category][row i][S+1 columns] with the pad
 // column that a missing tip state reads (1 for P(t), 0 for derivatives).
 // Scale buffers and every reduction are FP64 whatever the partials
@@ -16,25 +17,11 @@
 // Tree shape (a 127-deep caterpillar or a balanced tree) does not change
 // the launch count.
 #include "kernels.h"
+#include "device_util.h"
 
 #include <cmath>
 
 namespace hipbeagle {
-
-constexpr int WG = 256;
-
-// Pointers that arrive inside job structs are generic; casting them to the
-// global address space turns flat accesses (which wait on both the vector
-// and the scalar counters) into global loads and stores.
-#define GLOBAL __attribute__((address_space(1)))
-template <typename T>
-__device__ __forceinline__ const GLOBAL T* gin(const void* p) {
-    return (const GLOBAL T*)p;
-}
-template <typename T>
-__device__ __forceinline__ GLOBAL T* gout(void* p) {
-    return (GLOBAL T*)p;
-}
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -227,18 +214,6 @@ __device__ __forceinline__ R tile_max(R m, R (*smax)[8][TILE], int& k, int c, in
     return mx;
 }
 
-__device__ __forceinline__ int exponent_of(double x) {
-    int e;
-    frexp(x, &e);
-    return e;
-}
-__device__ __forceinline__ int exponent_of(float x) {
-    int e;
-    frexpf(x, &e);
-    return e;
-}
-__device__ __forceinline__ double pow2(double x, int e) { return ldexp(x, e); }
-__device__ __forceinline__ float pow2(float x, int e) { return ldexpf(x, e); }
 
 // Rescale one lane's four values given the tile maximum.  Manual modes
 // (reference semantics): divide by max (0 -> 1), wave 0 stores max or
@@ -1216,7 +1191,7 @@ __global__ __launch_bounds__(WG) void k_root_freq_grad(const R* __restrict__ roo
     if (p < d.P) {
         for (int i = 0; i < d.S; ++i) {
             double g = 0.0;
-            for (int c = 0; c < d.C; ++c) g += cw[c] * (double)root[c * d.catStride + (long)p * d.Sp + i];
+            for (int c = 0; c < d.C; ++c) g += cw[c] * (double)root[pbase(d, c, p) + (long)i * d.sStride];
             L += freqs[i] * g;
         }
     }
@@ -1224,7 +1199,7 @@ __global__ __launch_bounds__(WG) void k_root_freq_grad(const R* __restrict__ roo
         double v = 0.0;
         if (p < d.P) {
             double g = 0.0;
-            for (int c = 0; c < d.C; ++c) g += cw[c] * (double)root[c * d.catStride + (long)p * d.Sp + i];
+            for (int c = 0; c < d.C; ++c) g += cw[c] * (double)root[pbase(d, c, p) + (long)i * d.sStride];
             v = pw[p] * g / L;
         }
         const double s = block_sum(v, sh);
@@ -1246,12 +1221,12 @@ void launch_root_freq_grad(const void* root, const double* cw, const double* fre
 
 template <typename R>
 __device__ __forceinline__ R child_term(const R* x, const int* s, const R* m, int S, long xoff,
-                                        int p, int c, int i) {
+                                        int ss, int p, int c, int i) {
     const R* row = m + ((long)c * S + i) * (S + 1);
     if (s) return row[s[p]];
     const R* xv = x + xoff;
     R acc = 0;
-    for (int j = 0; j < S; ++j) acc += row[j] * xv[j];
+    for (int j = 0; j < S; ++j) acc += row[j] * xv[(long)j * ss];
     return acc;
 }
 
@@ -1266,14 +1241,14 @@ __global__ __launch_bounds__(WG) void k_postx(const PostJob* __restrict__ jobs, 
     const int np = J.p1 - J.p0;
     const int p = J.p0 + (int)(cp % np);
     const int c = (int)(cp / np);
-    const long off = c * d.catStride + (long)p * d.Sp;
+    const long off = pbase(d, c, p);
     const R a = child_term(reinterpret_cast<const R*>(J.x1), J.s1,
-                           reinterpret_cast<const R*>(J.m1), d.S, off, p, c, i);
+                           reinterpret_cast<const R*>(J.m1), d.S, off, d.sStride, p, c, i);
     const R b = child_term(reinterpret_cast<const R*>(J.x2), J.s2,
-                           reinterpret_cast<const R*>(J.m2), d.S, off, p, c, i);
+                           reinterpret_cast<const R*>(J.m2), d.S, off, d.sStride, p, c, i);
     R v = a * b;
     if (J.mode == SCALE_READ) v = v / (R)J.scaleRead[p];
-    reinterpret_cast<R*>(J.dest)[off + i] = v;
+    reinterpret_cast<R*>(J.dest)[off + (long)i * d.sStride] = v;
 }
 
 // pre-order, generic S, phase 1: u[c,p,i] = pre_parent[c,p,i] * term_i
@@ -1289,10 +1264,11 @@ __global__ __launch_bounds__(WG) void k_prex_u(const PreJob* __restrict__ jobs, 
     const long cp = t / d.S;
     const int p = J.p0 + (int)(cp % np);
     const int c = (int)(cp / np);
-    const long off = c * d.catStride + (long)p * d.Sp;
+    const long off = pbase(d, c, p);
     const R b = child_term(reinterpret_cast<const R*>(J.xs), J.ss,
-                           reinterpret_cast<const R*>(J.mSib), d.S, off, p, c, i);
-    u[off + i] = reinterpret_cast<const R*>(J.parent)[off + i] * b;
+                           reinterpret_cast<const R*>(J.mSib), d.S, off, d.sStride, p, c, i);
+    const long e = off + (long)i * d.sStride;
+    u[e] = reinterpret_cast<const R*>(J.parent)[e] * b;
 }
 
 // phase 2: dest[c,p,j] = sum_i M[i][j] u[c,p,i]  (M read transposed when the
@@ -1309,15 +1285,16 @@ __global__ __launch_bounds__(WG) void k_prex_out(const PreJob* __restrict__ jobs
     const long cp = t / d.S;
     const int p = J.p0 + (int)(cp % np);
     const int c = (int)(cp / np);
-    const long off = c * d.catStride + (long)p * d.Sp;
+    const long off = pbase(d, c, p);
+    const int ss = d.sStride;
     const R* M = reinterpret_cast<const R*>(J.mSelf) + (long)c * d.S * (d.S + 1);
     const int W = d.S + 1;
     R acc = 0;
     if (transposedSelf)
-        for (int i = 0; i < d.S; ++i) acc += M[j * W + i] * u[off + i];
+        for (int i = 0; i < d.S; ++i) acc += M[j * W + i] * u[off + (long)i * ss];
     else
-        for (int i = 0; i < d.S; ++i) acc += M[i * W + j] * u[off + i];
-    reinterpret_cast<R*>(J.dest)[off + j] = acc;
+        for (int i = 0; i < d.S; ++i) acc += M[i * W + j] * u[off + (long)i * ss];
+    reinterpret_cast<R*>(J.dest)[off + (long)j * ss] = acc;
 }
 
 // rescale pass for the generic path: one lane per pattern.
@@ -1331,12 +1308,12 @@ __global__ __launch_bounds__(WG) void k_rescalex(R* __restrict__ dest, int p0, i
     if (p >= p1) return;
     R mx = 0;
     for (int c = 0; c < d.C; ++c)
-        for (int i = 0; i < d.S; ++i) mx = fmax(mx, dest[c * d.catStride + (long)p * d.Sp + i]);
+        for (int i = 0; i < d.S; ++i) mx = fmax(mx, dest[pbase(d, c, p) + (long)i * d.sStride]);
     if (d.autoScale) {
         const int e = mx > (R)0 ? exponent_of(mx) : 0;
         for (int c = 0; c < d.C; ++c)
             for (int i = 0; i < d.S; ++i) {
-                R& x = dest[c * d.catStride + (long)p * d.Sp + i];
+                R& x = dest[pbase(d, c, p) + (long)i * d.sStride];
                 x = pow2(x, -e);
             }
         int t = e;
@@ -1348,7 +1325,7 @@ __global__ __launch_bounds__(WG) void k_rescalex(R* __restrict__ dest, int p0, i
     if (mx == (R)0) mx = (R)1;
     const R inv = (R)1 / mx;
     for (int c = 0; c < d.C; ++c)
-        for (int i = 0; i < d.S; ++i) dest[c * d.catStride + (long)p * d.Sp + i] *= inv;
+        for (int i = 0; i < d.S; ++i) dest[pbase(d, c, p) + (long)i * d.sStride] *= inv;
     const double lm = log((double)mx);
     double w = logScalers ? lm : (double)mx;
     if (cumulative) cumulative[p] += lm;
@@ -1577,7 +1554,7 @@ __global__ __launch_bounds__(WG) void k_root(const R* __restrict__ root,
             for (int i = 0; i < d.S; ++i) {
                 double t = 0.0;
                 for (int c = 0; c < d.C; ++c)
-                    t += (double)root[c * d.catStride + (long)p * d.Sp + i] * cw[c];
+                    t += (double)root[pbase(d, c, p) + (long)i * d.sStride] * cw[c];
                 sum += freqs[i] * t;
             }
         }
@@ -1645,7 +1622,8 @@ __global__ __launch_bounds__(WG) void k_edge_deriv(const EdgeJob* __restrict__ j
         const R* D = reinterpret_cast<const R*>(J.D);
         double num = 0.0, den = 0.0;
         for (int c = 0; c < d.C; ++c) {
-            const long off = c * d.catStride + (long)p * d.Sp;
+            const long off = pbase(d, c, p);
+            const int ss = d.sStride;
             const R* Dc = D + (long)c * S * W;
             double nc = 0.0, dc = 0.0;
             if (J.states) {
@@ -1654,18 +1632,21 @@ __global__ __launch_bounds__(WG) void k_edge_deriv(const EdgeJob* __restrict__ j
                     double dij;
                     if (transposedD) dij = s < S ? (double)Dc[s * W + i] : 0.0;
                     else dij = (double)Dc[i * W + s];
-                    nc += dij * (double)pre[off + i];
+                    nc += dij * (double)pre[off + (long)i * ss];
                 }
-                dc = (double)pre[off + (s % S)];
+                dc = (double)pre[off + (long)(s % S) * ss];
             } else {
                 for (int i = 0; i < S; ++i) {
                     double t = 0.0;
                     if (transposedD)
-                        for (int j = 0; j < S; ++j) t += (double)Dc[j * W + i] * (double)post[off + j];
+                        for (int j = 0; j < S; ++j)
+                            t += (double)Dc[j * W + i] * (double)post[off + (long)j * ss];
                     else
-                        for (int j = 0; j < S; ++j) t += (double)Dc[i * W + j] * (double)post[off + j];
-                    nc += t * (double)pre[off + i];
-                    dc += (double)post[off + i] * (double)pre[off + i];
+                        for (int j = 0; j < S; ++j)
+                            t += (double)Dc[i * W + j] * (double)post[off + (long)j * ss];
+                    const double pi = (double)pre[off + (long)i * ss];
+                    nc += t * pi;
+                    dc += (double)post[off + (long)i * ss] * pi;
                 }
             }
             num += cw[c] * nc;
@@ -1878,7 +1859,7 @@ __global__ __launch_bounds__(WG) void k_cross(const EdgeJob* __restrict__ jobs, 
     const long npairs = (long)njobs * nchunks;
     for (long pair = blockIdx.x; pair < npairs; pair += gridDim.x) {
         const int n = (int)(pair / nchunks);
-        const int pbase = (int)(pair % nchunks) * XP_CHUNK;
+        const int p0 = (int)(pair % nchunks) * XP_CHUNK;
         const EdgeJob J = jobs[n];
         const R* pre = reinterpret_cast<const R*>(J.pre);
         const R* post = reinterpret_cast<const R*>(J.post);
@@ -1886,16 +1867,16 @@ __global__ __launch_bounds__(WG) void k_cross(const EdgeJob* __restrict__ jobs, 
             const int i = t % S;
             const int c = (t / S) % C;
             const int q = t / (S * C);
-            const int p = pbase + q;
+            const int p = p0 + q;
             double a = 0.0, b = 0.0;
             if (p < d.P) {
-                const long off = c * d.catStride + (long)p * d.Sp;
-                a = (double)pre[off + i];
+                const long off = pbase(d, c, p) + (long)i * d.sStride;
+                a = (double)pre[off];
                 if (J.states) {
                     const int s = J.states[p];
                     b = (s >= S || s == i) ? 1.0 : 0.0;
                 } else {
-                    b = (double)post[off + i];
+                    b = (double)post[off];
                 }
             }
             spre[t] = a;
@@ -1904,7 +1885,7 @@ __global__ __launch_bounds__(WG) void k_cross(const EdgeJob* __restrict__ jobs, 
         __syncthreads();
         if (threadIdx.x < XP_CHUNK) {
             const int q = threadIdx.x;
-            const int p = pbase + q;
+            const int p = p0 + q;
             double den = 0.0;
             for (int c = 0; c < C; ++c) {
                 double dd = 0.0;
@@ -2004,7 +1985,8 @@ __global__ __launch_bounds__(WG) void k_edge_ll(const R* __restrict__ par, const
         for (int m = 0; m < 3; ++m) {
             if (!Ms[m]) continue;
             for (int c = 0; c < d.C; ++c) {
-                const long off = c * d.catStride + (long)p * d.Sp;
+                const long off = pbase(d, c, p);
+                const int ss = d.sStride;
                 const R* Mc = Ms[m] + (long)c * S * W;
                 double s = 0.0;
                 for (int i = 0; i < S; ++i) {
@@ -2013,9 +1995,10 @@ __global__ __launch_bounds__(WG) void k_edge_ll(const R* __restrict__ par, const
                         t = (double)Mc[i * W + states[p]];
                     } else {
                         t = 0.0;
-                        for (int j = 0; j < S; ++j) t += (double)Mc[i * W + j] * (double)child[off + j];
+                        for (int j = 0; j < S; ++j)
+                            t += (double)Mc[i * W + j] * (double)child[off + (long)j * ss];
                     }
-                    s += f[i] * (double)par[off + i] * t;
+                    s += f[i] * (double)par[off + (long)i * ss] * t;
                 }
                 acc[m] += cw[c] * s;
             }
@@ -2109,7 +2092,7 @@ __global__ __launch_bounds__(WG) void k_tile_vector(R* dst, const double* __rest
     const int i = t % d.S;
     const long cp = t / d.S;
     const long p = cp % d.Ppad, c = cp / d.Ppad;
-    dst[c * d.catStride + p * d.Sp + i] = (R)v[i];
+    dst[pbase(d, (int)c, (int)p) + (long)i * d.sStride] = (R)v[i];
 }
 
 template <typename R>

@@ -160,6 +160,78 @@ def test_single_precision_close_to_oracle(amd, oracle):
     assert rel(lg, lo) < 1e-6
 
 
+MFMA_CASES = [
+    # S, taxa, patterns, shape, tips_as, scaling, missing
+    (20, 12, 700, "yule", "states", "none", 0.05),
+    (20, 10, 513, "caterpillar", "partials", "manual", 0.05),
+    (61, 8, 300, "yule", "states", "always", 0.05),
+    (61, 10, 257, "balanced", "states", "auto", 0.05),
+    (61, 7, 130, "yule", "partials", "manual_log", 0.1),
+    (5, 9, 100, "yule", "partials", "manual", 0.1),
+    (33, 7, 97, "caterpillar", "states", "auto", 0.0),
+]
+
+
+@pytest.mark.parametrize("S,n,P,shape,tips_as,scaling,missing", MFMA_CASES)
+def test_mfma_post_order_single_matches_oracle(amd, oracle, S, n, P, shape, tips_as, scaling,
+                                               missing):
+    """S > 4 in FP32: the post-order runs on the FP32 matrix cores
+    (v_mfma_f32_32x32x2f32).  Identical P(t) on both sides; the FP32
+    result must stay within 1e-6 relative log-likelihood of the FP64
+    oracle, site log-likelihoods within 2e-6 relative, and the root
+    partials (where both sides share a scale) within 1e-5 of each row's
+    maximum."""
+    prob = synth.make_problem(17 + S + n, n, P, S=S, shape=shape, missing=missing)
+    ev = driver.Evaluator(amd, prob, precision="single", tips_as=tips_as, scaling=scaling,
+                          matrices="explicit")
+    assert ev.inst.impl_name == "MI355X-HIP-Single"
+    lg = ev.log_likelihood()
+    sg = ev.inst.site_log_likelihoods()
+    rg = ev.inst.get_partials(prob.tree.root)
+    ev.close()
+    ev = driver.Evaluator(oracle, prob, tips_as=tips_as,
+                          scaling="manual" if scaling == "auto" else scaling, matrices="explicit")
+    lo = ev.log_likelihood()
+    so = ev.inst.site_log_likelihoods()
+    ro = ev.inst.get_partials(prob.tree.root)
+    ev.close()
+    assert rel(lg, lo) < 1e-6, (lg, lo)
+    close(sg, so, 2e-6, 0)
+    if scaling != "auto":
+        # unscaled rows whose FP64 maximum lies below FP32's normal range
+        # (1.2e-38) underflow in FP32 and are left out
+        rg = np.asarray(rg).reshape(-1, S)
+        ro = np.asarray(ro).reshape(-1, S)
+        keep = ro.max(axis=1) > 1e-30
+        assert keep.mean() > 0.5
+        close_rows(rg[keep], ro[keep], S, 1e-5)
+
+
+def test_mfma_post_order_partitions(amd, oracle):
+    """Partitioned post-order (pattern ranges that start and end inside a
+    32-pattern tile) on the MFMA path, 20 states, FP32."""
+    prob = synth.make_problem(23, 10, 333, S=20)
+    parts = np.repeat(np.arange(3), [70, 101, 162]).astype(np.int32)
+    out = []
+    for lib, prec in ((amd, "single"), (oracle, "double")):
+        ev = driver.Evaluator(lib, prob, precision=prec, matrices="explicit")
+        ev.update_matrices()
+        ev.inst.set_pattern_partitions(3, parts)
+        ops = synth.postorder_ops(prob.tree)
+        pops = []
+        for q in range(3):
+            for o in ops:
+                pops.append(list(o) + [q, -1])
+        ev.inst.update_partials_by_partition(np.array(pops, np.int32))
+        r = prob.tree.root
+        by, tot = ev.inst.root_log_likelihood_by_partition([r] * 3, [0] * 3, [0] * 3, [-1] * 3,
+                                                           [0, 1, 2])
+        out.append((by, tot))
+        ev.close()
+    close(out[0][0], out[1][0], 1e-6, 0)
+    assert rel(out[0][1], out[1][1]) < 1e-6
+
+
 def test_matrices_match_oracle(amd, oracle):
     prob = synth.make_problem(3, 8, 100, S=20)
     mats = []
