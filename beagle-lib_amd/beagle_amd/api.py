@@ -142,6 +142,7 @@ SIGNATURES = {
     "GetSiteDerivatives": (_I, [_I, _DP, _DP]),
     # extensions (include/beagle_amd_ext.h)
     "ExtGetStream": (_I, [_I, ctypes.POINTER(ctypes.c_void_p)]),
+    "ExtSetSplitBF16": (_I, [_I, _I]),
     "ExtUpdatePrePartialsWithDerivatives": (_I, [_I, _IP, _I, _IP, _IP, _IP, _I, _DP, _DP, _DP]),
     "ExtCalculateLogLikelihoodAndDerivatives": (_I, [_I, _I, _I, _I, _I, _I, _IP, _I, _IP, _IP,
                                                      _IP, _DP, _DP, _DP, _DP]),
@@ -483,6 +484,11 @@ class Instance:
                 -1 if root is None else root, freqs_index, gq.ctypes.data_as(c_double_p),
                 gf.ctypes.data_as(c_double_p) if gf is not None else None)
         return gq.reshape(self.S, self.S), gf
+
+    def set_split_bf16(self, terms=3):
+        """S > 4, FP32: post-order products on split-BF16 matrix cores
+        (terms 3 or 4), or back on the FP32 matrix cores (terms 0)."""
+        self._c("ExtSetSplitBF16", int(terms))
 
     def stream(self):
         """hipStream_t of the instance (extension entry point)."""

@@ -637,5 +637,9 @@ int beagleExtGetStream(int instance, void** outStream) {
     });
 }
 
+int beagleExtSetSplitBF16(int instance, int terms) {
+    return withInstance(instance, [&](Instance* in) { return in->setSplitBf16(terms); });
+}
+
 }  // extern "C"
 #pragma GCC visibility pop

@@ -173,6 +173,7 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
     launch_fill(dScale_, 0.0, (long)std::max(scaleCount_, 1) * dims_.Ppad, stream_);
     const size_t S = stateCount;
     dCube_ = dalloc<double>(eigenCount_ * S * S * S);
+    dVVi_ = dalloc<double>(eigenCount_ * 2 * S * S);
     dEvals_ = dalloc<double>(eigenCount_ * S);
     eigenSet_.assign(eigenCount_, false);
     hEigen_.resize(eigenCount_);
@@ -198,6 +199,7 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
     partStart_ = {0, patternCount};
     if (const char* m = std::getenv("BEAGLE_AMD_TRAVERSAL")) traversalMode_ = std::atoi(m);
     if (const char* m = std::getenv("BEAGLE_AMD_LEVEL_CHUNK")) levelChunk_ = std::max(1, std::atoi(m));
+    if (const char* m = std::getenv("BEAGLE_AMD_MFMA_VARIANT")) mfmaVariant_ = std::atoi(m);
     ring_.resize(8);
     for (auto& s : ring_) HIPCHECK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
     sync();
@@ -212,6 +214,7 @@ Instance::~Instance() {
     hipFree(dMatrices_);
     hipFree(dScale_);
     hipFree(dCube_);
+    hipFree(dVVi_);
     hipFree(dEvals_);
     hipFree(dFreqs_);
     hipFree(dCatWeights_);
@@ -427,6 +430,13 @@ int Instance::setEigenDecomposition(int e, const double* V, const double* Vi, co
             for (int k = 0; k < S; ++k)
                 cube[l++] = V[i * S + k] * (trans ? Vi[k + j * S] : Vi[k * S + j]);
     copyIn(dCube_ + (size_t)e * S * S * S, cube.data(), cube.size() * sizeof(double));
+    std::vector<double> vv(2 * (size_t)S * S);
+    for (int i = 0; i < S; ++i)
+        for (int k = 0; k < S; ++k) {
+            vv[(size_t)i * S + k] = V[i * S + k];
+            vv[(size_t)S * S + (size_t)i * S + k] = trans ? Vi[i + k * S] : Vi[i * S + k];
+        }
+    copyIn(dVVi_ + (size_t)e * 2 * S * S, vv.data(), vv.size() * sizeof(double));
     copyIn(dEvals_ + (size_t)e * S, lam, S * sizeof(double));
     eigenSet_[e] = true;
     // host copy for the rate-matrix gradient: V, V^-1 (row-major), lambda
@@ -521,6 +531,7 @@ int Instance::updateTransitionMatrices(int e, const int* probIdx, const int* d1I
             }
         }
         J.cube = dCube_ + (size_t)e * S * S * S;
+        J.vv = dVVi_ + (size_t)e * 2 * S * S;
         J.evals = dEvals_ + (size_t)e * S;
         J.rates = dCatRates_;
         J.t = t[u];
@@ -554,6 +565,7 @@ int Instance::updateTransitionMatricesWithModelCategories(const int* eig, const 
             J.D1 = d1Idx ? matrix(d1Idx[u]) : nullptr;
             J.D2 = (d1Idx && d2Idx) ? matrix(d2Idx[u]) : nullptr;
             J.cube = dCube_ + (size_t)eig[c] * S * S * S;
+            J.vv = dVVi_ + (size_t)eig[c] * 2 * S * S;
             J.evals = dEvals_ + (size_t)eig[c] * S;
             J.rates = nullptr;
             J.t = t[u];
@@ -583,6 +595,7 @@ int Instance::updateTransitionMatricesWithMultipleModels(const int* eig, const i
         J.D1 = (d1Idx && validMatrix(d1Idx[u])) ? matrix(d1Idx[u]) : nullptr;
         J.D2 = (d1Idx && d2Idx && validMatrix(d2Idx[u])) ? matrix(d2Idx[u]) : nullptr;
         J.cube = dCube_ + (size_t)e * S * S * S;
+        J.vv = dVVi_ + (size_t)e * 2 * S * S;
         J.evals = dEvals_ + (size_t)e * S;
         J.rates = dCatRates_ + (size_t)r * dims_.C;
         J.t = t[u];
@@ -774,6 +787,14 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
     return BEAGLE_SUCCESS;
 }
 
+int Instance::setSplitBf16(int terms) {
+    if (terms != 0 && terms != 3 && terms != 4) return BEAGLE_ERROR_OUT_OF_RANGE;
+    if (dp_ || !mfma_post_supported(dims_.S, dims_.C)) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    splitBf16_ = terms != 0;
+    if (terms) bf16Terms_ = terms;
+    return BEAGLE_SUCCESS;
+}
+
 int Instance::postLevels(const int* ops, int stride, int count, std::vector<int>& level) {
     std::vector<Access> acc(count);
     for (int o = 0; o < count; ++o) {
@@ -787,9 +808,9 @@ int Instance::postLevels(const int* ops, int stride, int count, std::vector<int>
     return dependencyLevels(acc, bufferCount_ + scaleCount_, level);
 }
 
-// MFMA post-order (S > 4, FP32 partials): every child matrix of the list is
-// repacked once into the pack slab, then each dependency level is one
-// launch over (pattern tiles x ops).  A list whose ops share a cumulative
+// MFMA post-order (S > 4, FP32 partials; FP32 or split-BF16 matrix cores):
+// every child matrix of the list is repacked once into the pack slab, then
+// each dependency level is one launch over (pattern tiles x ops).  A list whose ops share a cumulative
 // scale buffer runs one op per launch in list order (each rescaling op adds
 // to the same per-pattern sums).
 void Instance::launchPostMfma(const int* ops, int stride, int count) {
@@ -799,8 +820,9 @@ void Instance::launchPostMfma(const int* ops, int stride, int count) {
         PostJob& J = hPost_[o];
         J.pk1 = (int)(2 * o * per);
         J.pk2 = (int)((2 * o + 1) * per);
-        pj[2 * o] = PackJob{J.mo1, J.pk1, J.s1 ? PACK_ROWS : PACK_OPERAND};
-        pj[2 * o + 1] = PackJob{J.mo2, J.pk2, J.s2 ? PACK_ROWS : PACK_OPERAND};
+        const int operand = splitBf16_ ? PACK_OPERAND_BF16 : PACK_OPERAND;
+        pj[2 * o] = PackJob{J.mo1, J.pk1, J.s1 ? PACK_ROWS : operand};
+        pj[2 * o + 1] = PackJob{J.mo2, J.pk2, J.s2 ? PACK_ROWS : operand};
     }
     float* pack = static_cast<float*>(
         grow(reinterpret_cast<void*&>(dPack_), packCap_, pj.size() * per * sizeof(float)));
@@ -824,7 +846,12 @@ void Instance::launchPostMfma(const int* ops, int stride, int count) {
     launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     auto d = reinterpret_cast<const PostJob*>(upload(sorted.data(), count * sizeof(PostJob)));
     for (int l = 0; l < nl; ++l)
-        launch_post_mfma(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
+        if (splitBf16_)
+            launch_post_bf16(d + start[l], start[l + 1] - start[l], dims_, pack, bf16Terms_,
+                             stream_);
+        else
+            launch_post_mfma(d + start[l], start[l + 1] - start[l], dims_, pack, mfmaVariant_,
+                             stream_);
 }
 
 // Launch a 4-state post-order plan with one upload of all its jobs.
@@ -1464,7 +1491,7 @@ int Instance::enqueueRoot(int b, int w, int f, int cum) {
     int cs = always_ ? b - tipCount_ : (auto_ ? BEAGLE_OP_NONE : cum);
     if (cs != BEAGLE_OP_NONE && !validScale(cs)) return BEAGLE_ERROR_OUT_OF_RANGE;
     const short* ex = auto_ ? dExp_[b] : nullptr;
-    const int nb = (dims_.P + 255) / 256;
+    const int nb = (dims_.P + 31) / 32;   // workgroup sums of the root kernels (at most P/32)
     double* work = reinterpret_cast<double*>(grow(dWork_, workCap_, (size_t)nb * sizeof(double)));
     if (dp_)
         launch_root<double>(dPartials_[b], ex, dCatWeights_ + (size_t)w * dims_.C,
@@ -1485,7 +1512,7 @@ int Instance::calculateRootLogLikelihoodsByPartition(const int* bufIdx, const in
                                                      double* outSum) {
     if (count != 1 || always_) return BEAGLE_ERROR_NO_IMPLEMENTATION;
     if (partitionCount > 64) return BEAGLE_ERROR_NO_IMPLEMENTATION;
-    const int nbMax = (dims_.P + 255) / 256;
+    const int nbMax = (dims_.P + 31) / 32;
     double* work = reinterpret_cast<double*>(grow(dWork_, workCap_, (size_t)nbMax * sizeof(double)));
     for (int q = 0; q < partitionCount; ++q) {
         const int part = partIdx[q], b = bufIdx[q], w = wIdx[q], f = fIdx[q];

@@ -160,6 +160,12 @@ private:
     void launchPostPlan(const std::vector<Segment>& plan);
     // S > 4, FP32: matrix repacking, then one MFMA launch per dependency level
     void launchPostMfma(const int* ops, int stride, int count);
+
+public:
+    // post-order matrix-core engine for S > 4 in FP32: 0 FP32, 3 or 4 split BF16
+    int setSplitBf16(int terms);
+
+private:
     // dependency levels of a post-order list (reads children and a read
     // scale buffer; writes the destination and a write scale buffer)
     int postLevels(const int* ops, int stride, int count, std::vector<int>& level);
@@ -187,6 +193,9 @@ private:
     bool logScalers_;
     int preWeights_ = 0;      // category-weights buffer of the fused derivative pass
     int traversalMode_ = 0;   // 0 automatic, 1 always per dependency level, 2 always fused
+    bool splitBf16_ = false;  // S > 4 FP32 post-order on the BF16 matrix cores (hi/lo split)
+    int bf16Terms_ = 3;       // products per split pair: 3, or 4 (adds lo x lo)
+    int mfmaVariant_ = 0;     // MFMA post-order: 0 partials pipelined, 1 whole tile up front
     int levelChunk_ = 1;      // batched level: 1 one op per workgroup, k > 1 chunks of k ops
     Dims dims_;
     size_t partialsBytes_, matrixBytes_;
@@ -198,6 +207,7 @@ private:
     void* dMatrices_ = nullptr;
     double* dScale_ = nullptr;
     double* dCube_ = nullptr;
+    double* dVVi_ = nullptr;          // per eigen-system [V | V^-1] (S x S each)
     double* dEvals_ = nullptr;
     std::vector<bool> eigenSet_;
     std::vector<HostEigen> hEigen_;

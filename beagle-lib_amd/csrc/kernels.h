@@ -86,6 +86,7 @@ struct NodeJob {
 struct MatrixJob {
     void* P; void* D1; void* D2;      // D1/D2 may be null
     const double* cube;               // S^3, C_ijk = V_ik Vinv_kj
+    const double* vv;                 // [V | Vinv], each S x S row-major
     const double* evals;              // S
     const double* rates;              // C category rates
     double t;
@@ -137,17 +138,21 @@ struct PackJob {
     int src, dst;
     int kind;      // PACK_OPERAND: partials child, PACK_ROWS: tip-state child
 };
-enum PackKind : int { PACK_OPERAND = 0, PACK_ROWS = 1 };
+enum PackKind : int { PACK_OPERAND = 0, PACK_ROWS = 1, PACK_OPERAND_BF16 = 2 };
 
 // MFMA path shapes: M blocks of 32 output states, K in groups of 8 (four
 // steps of the 32x32x2 FP32 instruction, k = 2 step + half-wave).
 __host__ __device__ inline int mfma_mblocks(int S) { return (S + 31) / 32; }
 __host__ __device__ inline int mfma_kgroups(int S) { return (S + 7) / 8; }
+// split-BF16 path: K steps of 16 (v_mfma_f32_32x32x16_bf16)
+__host__ __device__ inline int mfma_ksteps16(int S) { return (S + 15) / 16; }
 // floats per category of one packed child matrix
 __host__ __device__ inline int mfma_pack_reals(int S) {
-    const int a = mfma_mblocks(S) * mfma_kgroups(S) * 256;   // operand blocks
+    const int a = mfma_mblocks(S) * mfma_kgroups(S) * 256;   // FP32 operand blocks
+    const int b = mfma_mblocks(S) * mfma_ksteps16(S) * 512;  // hi/lo BF16 operand blocks
     const int t = (S + 1) * mfma_mblocks(S) * 32;            // rows by state
-    const int m = a > t ? a : t;
+    int m = a > t ? a : t;
+    m = m > b ? m : b;
     return (m + 63) / 64 * 64;
 }
 // S range and category count the MFMA post-order kernel covers
@@ -157,9 +162,14 @@ __host__ __device__ inline bool mfma_post_supported(int S, int C) { return S > 4
 template <typename R>
 void launch_pack_matrices(const PackJob* jobs, int njobs, int S, int C, const void* mats,
                           float* pack, hipStream_t st);
-// One launch per dependency level: grid (pattern tiles, njobs).
+// One launch per dependency level: grid (pattern tiles, njobs).  variant 0:
+// partials loaded a few k groups ahead; 1: the whole partials tile up front.
 void launch_post_mfma(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
-                      hipStream_t st);
+                      int variant, hipStream_t st);
+// The same on the BF16 matrix cores with hi/lo split operands (terms: 3 or 4
+// products per split pair); the matrices packed as PACK_OPERAND_BF16.
+void launch_post_bf16(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
+                      int terms, hipStream_t st);
 
 // ---- launchers (kernels.hip) ----
 template <typename R>

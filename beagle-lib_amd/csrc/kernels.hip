@@ -108,6 +108,9 @@ __global__ __launch_bounds__(WG) void k_reduce_rows(const double* __restrict__ i
 // (EigenDecompositionCube.hpp:105-322 semantics; computed in FP64, stored in
 // the instance precision).
 
+// state counts whose V and V^-1 fit the 64 KB of LDS a workgroup may use
+constexpr int kMatLdsMaxS = 62;
+
 template <typename R>
 __global__ __launch_bounds__(WG) void k_matrices(const MatrixJob* __restrict__ jobs, int S, int C) {
     extern __shared__ double sh[];
@@ -134,19 +137,59 @@ __global__ __launch_bounds__(WG) void k_matrices(const MatrixJob* __restrict__ j
     R* Pm = reinterpret_cast<R*>(J.P) + (long)c * S * W;
     R* D1 = deriv ? reinterpret_cast<R*>(J.D1) + (long)c * S * W : nullptr;
     R* D2 = J.D2 ? reinterpret_cast<R*>(J.D2) + (long)c * S * W : nullptr;
-    for (int idx = threadIdx.x; idx < S * S; idx += WG) {
-        const int i = idx / S, j = idx - i * S;
-        const double* cm = J.cube + (long)idx * S;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (int k = 0; k < S; ++k) {
-            const double cv = cm[k];
-            s0 += cv * ex[k];
-            s1 += cv * e1[k];
-            s2 += cv * e2[k];
+    if (S <= kMatLdsMaxS) {
+        // V and V^-1 staged in LDS; C_ijk = V_ik Vinv_kj formed on the fly
+        // (the same product the cube holds, so results are identical)
+        double* V = sh + 3 * S;
+        double* Vi = V + S * S;
+        for (int t = threadIdx.x; t < S * S; t += WG) {
+            V[t] = J.vv[t];
+            Vi[t] = J.vv[S * S + t];
         }
-        Pm[i * W + j] = (R)(s0 > 0.0 ? s0 : 0.0);
-        if (D1) D1[i * W + j] = (R)s1;
-        if (D2) D2[i * W + j] = (R)s2;
+        __syncthreads();
+        // four columns per thread: four independent sums share V_ik
+        const int JG = (S + 3) / 4;
+        for (int w = threadIdx.x; w < S * JG; w += WG) {
+            const int i = w / JG, j0 = (w - i * JG) * 4;
+            double s0[4] = {0.0, 0.0, 0.0, 0.0}, s1[4] = {0.0, 0.0, 0.0, 0.0},
+                   s2[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+            for (int k = 0; k < S; ++k) {
+                const double v = V[i * S + k];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double cv = j0 + u < S ? v * Vi[k * S + j0 + u] : 0.0;
+                    s0[u] += cv * ex[k];
+                    if (deriv) {
+                        s1[u] += cv * e1[k];
+                        s2[u] += cv * e2[k];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + u;
+                if (j >= S) break;
+                Pm[i * W + j] = (R)(s0[u] > 0.0 ? s0[u] : 0.0);
+                if (D1) D1[i * W + j] = (R)s1[u];
+                if (D2) D2[i * W + j] = (R)s2[u];
+            }
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < S * S; idx += WG) {
+            const int i = idx / S, j = idx - i * S;
+            const double* cm = J.cube + (long)idx * S;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (int k = 0; k < S; ++k) {
+                const double cv = cm[k];
+                s0 += cv * ex[k];
+                s1 += cv * e1[k];
+                s2 += cv * e2[k];
+            }
+            Pm[i * W + j] = (R)(s0 > 0.0 ? s0 : 0.0);
+            if (D1) D1[i * W + j] = (R)s1;
+            if (D2) D2[i * W + j] = (R)s2;
+        }
     }
     for (int i = threadIdx.x; i < S; i += WG) {
         Pm[i * W + S] = (R)1.0;
@@ -159,7 +202,8 @@ template <typename R>
 void launch_matrices(const MatrixJob* jobs, int njobs, int S, int C, hipStream_t st) {
     if (njobs <= 0) return;
     dim3 grid(njobs, C);
-    hipLaunchKernelGGL(k_matrices<R>, grid, dim3(WG), 3 * S * sizeof(double), st, jobs, S, C);
+    const size_t lds = (3 * S + (S <= kMatLdsMaxS ? 2 * S * S : 0)) * sizeof(double);
+    hipLaunchKernelGGL(k_matrices<R>, grid, dim3(WG), lds, st, jobs, S, C);
 }
 
 // out[c] = A[c] B[c] over S states (in the instance precision, as the
@@ -1568,15 +1612,75 @@ __global__ __launch_bounds__(WG) void k_root(const R* __restrict__ root,
     if (threadIdx.x == 0) bsum[blockIdx.x] = b;
 }
 
+// S > 4 (pattern-minor tiles): a workgroup takes 32 patterns, its eight
+// wavefront halves take every eighth state, so each load instruction reads
+// whole 128-byte rows; the eight partial sums are added in a fixed order.
+template <typename R>
+__global__ __launch_bounds__(WG) void k_root_tiled(const R* __restrict__ root,
+                                                   const double* __restrict__ cw,
+                                                   const double* __restrict__ freqs,
+                                                   const double* __restrict__ cum,
+                                                   const double* __restrict__ pw,
+                                                   const short* __restrict__ ex,
+                                                   double* __restrict__ site,
+                                                   double* __restrict__ bsum, int p0, int p1,
+                                                   Dims d) {
+    __shared__ double part[WG / 32][32];
+    __shared__ double sh[WG / 64];
+    const int n = threadIdx.x & 31, slice = threadIdx.x >> 5;
+    const int p = p0 + blockIdx.x * 32 + n;
+    double acc = 0.0;
+    if (p < p1) {
+        for (int i = slice; i < d.S; i += WG / 32) {
+            double t = 0.0;
+            for (int c = 0; c < d.C; ++c)
+                t += (double)root[pbase(d, c, p) + (long)i * PTILE] * cw[c];
+            acc += freqs[i] * t;
+        }
+    }
+    part[slice][n] = acc;
+    __syncthreads();
+    double contrib = 0.0;
+    if (slice == 0 && p < p1) {
+        double sum = 0.0;
+        for (int k = 0; k < WG / 32; ++k) sum += part[k][n];
+        double v = log(sum);
+        if (cum) v += cum[p];
+        if (ex) v += 0.69314718055994530942 * (double)ex[p];
+        site[p] = v;
+        contrib = v * pw[p];
+    }
+    const double b = block_sum(contrib, sh);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = b;
+}
+
+template <typename R>
+static int root_blocks_and_launch(const void* root, const short* rootExp, const double* cw,
+                                  const double* freqs, const double* cum, const double* pw,
+                                  double* site, double* bsum, int p0, int p1, const Dims& d,
+                                  hipStream_t st) {
+    if (d.sStride == PTILE) {
+        const int nb = (p1 - p0 + 31) / 32;
+        if (nb > 0)
+            hipLaunchKernelGGL(k_root_tiled<R>, dim3(nb), dim3(WG), 0, st,
+                               reinterpret_cast<const R*>(root), cw, freqs, cum, pw, rootExp, site,
+                               bsum, p0, p1, d);
+        return nb;
+    }
+    const int nb = (p1 - p0 + WG - 1) / WG;
+    if (nb > 0)
+        hipLaunchKernelGGL(k_root<R>, dim3(nb), dim3(WG), 0, st, reinterpret_cast<const R*>(root),
+                           cw, freqs, cum, pw, rootExp, site, bsum, p0, p1, d);
+    return nb;
+}
+
 template <typename R>
 void launch_root(const void* root, const short* rootExp, const double* catWeights,
                  const double* freqs, const double* cumulative, const double* patternWeights,
                  double* siteLogL, double* blockSums, double* result, const Dims& d,
                  hipStream_t st) {
-    const int nb = (d.P + WG - 1) / WG;
-    hipLaunchKernelGGL(k_root<R>, dim3(nb), dim3(WG), 0, st, reinterpret_cast<const R*>(root),
-                       catWeights, freqs, cumulative, patternWeights, rootExp, siteLogL, blockSums,
-                       0, d.P, d);
+    const int nb = root_blocks_and_launch<R>(root, rootExp, catWeights, freqs, cumulative,
+                                             patternWeights, siteLogL, blockSums, 0, d.P, d, st);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(WG), 0, st, blockSums, nb, 1, result, 0);
 }
 
@@ -1585,14 +1689,12 @@ void launch_root_range(const void* root, const short* rootExp, const double* cat
                        const double* freqs, const double* cumulative,
                        const double* patternWeights, double* siteLogL, double* blockSums,
                        double* result, int p0, int p1, const Dims& d, hipStream_t st) {
-    const int nb = (p1 - p0 + WG - 1) / WG;
-    if (nb <= 0) {
+    if (p1 <= p0) {
         launch_fill(result, 0.0, 1, st);
         return;
     }
-    hipLaunchKernelGGL(k_root<R>, dim3(nb), dim3(WG), 0, st, reinterpret_cast<const R*>(root),
-                       catWeights, freqs, cumulative, patternWeights, rootExp, siteLogL, blockSums,
-                       p0, p1, d);
+    const int nb = root_blocks_and_launch<R>(root, rootExp, catWeights, freqs, cumulative,
+                                             patternWeights, siteLogL, blockSums, p0, p1, d, st);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(WG), 0, st, blockSums, nb, 1, result, 0);
 }
 

@@ -31,6 +31,15 @@ namespace hipbeagle {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// x = hi + lo with hi = bf16(x) (round to nearest even) and lo = bf16(x - hi):
+// 16 significant bits of x in two BF16 numbers
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
 
 // ---------------------------------------------------------------------------
 // matrix repacking: one thread per output float
@@ -57,6 +66,25 @@ __global__ __launch_bounds__(WG) void k_pack_mats(const PackJob* __restrict__ jo
             const int i = mb * 32 + (lane & 31), k = 8 * g + 2 * q + (lane >> 5);
             if (i < S && k < S) v = (float)M[i * (S + 1) + k];
         }
+    } else if (J.kind == PACK_OPERAND_BF16) {
+        // [mb][k step t][part: hi, lo][lane][8 BF16]: row 32 mb + lane % 32,
+        // column 16 t + 8 (lane / 32) + e; one float holds elements 2e' and 2e'+1
+        const int KT = mfma_ksteps16(S);
+        if (e < MB * KT * 512) {
+            unsigned short half[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int el = 2 * e + u;
+                const int ee = el & 7, lane = (el >> 3) & 63, part = (el >> 9) & 1;
+                const int blk = el >> 10, t = blk % KT, mb = blk / KT;
+                const int i = mb * 32 + (lane & 31), k = 16 * t + 8 * (lane >> 5) + ee;
+                const float x = (i < S && k < S) ? (float)M[i * (S + 1) + k] : 0.f;
+                __bf16 hi, lo;
+                split_bf16(x, hi, lo);
+                half[u] = __builtin_bit_cast(unsigned short, part ? lo : hi);
+            }
+            v = __builtin_bit_cast(float, (unsigned)half[0] | ((unsigned)half[1] << 16));
+        }
     } else {
         // [state s (S: the pad column)][row i]
         const int W = MB * 32;
@@ -80,55 +108,171 @@ void launch_pack_matrices(const PackJob* jobs, int njobs, int S, int C, const vo
 // ---------------------------------------------------------------------------
 // post-order
 
-// y = P x of one child over the tile (layout above).  pk: the child's packed
-// matrix for this category; tbase: element offset of the tile in a
-// partials buffer; p: the lane's pattern.
+// Operands of one pipeline stage, k group g of one partials child: the
+// matrix (A) and the partials (B).
 template <int MB>
-__device__ __forceinline__ void child_mfma(const void* x, const int* states, const float* pk,
-                                           long tbase, int S, int KG, int l, int p,
-                                           floatx16 (&y)[MB]) {
-    const int n = l & 31, h = l >> 5;
-    if (states) {
-        const int s = gin<int>(states)[p];
-        const GLOBAL float* row = gin<float>(pk) + s * (MB * 32) + 4 * h;
+__device__ __forceinline__ void load_a(const GLOBAL floatx4* a, int g, int KG,
+                                       floatx4 (&am)[MB]) {
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) am[mb] = a[(mb * KG + g) * 64];
+}
+__device__ __forceinline__ void load_x(const GLOBAL float* xb, int g, int h, int S,
+                                       float (&b)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b[q] = xb[min(8 * g + 2 * q + h, S - 1) * PTILE];
+}
+
+template <int MB>
+__device__ __forceinline__ void mfma_stage(floatx16 (&y)[MB], const floatx4 (&am)[MB],
+                                           const float (&b)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
+            y[mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(am[mb][q], b[q], y[mb], 0, 0, 0);
+}
+
+// Split-BF16 stages: k step t (16 inner states) of one partials child.
+// A (v_mfma_f32_32x32x16_bf16): lane l holds P[32 mb + l % 32][16 t + 8 (l / 32) + e]
+// as packed hi and lo parts; B: X[16 t + 8 (l / 32) + e][pattern l % 32].
+template <int MB>
+__device__ __forceinline__ void load_a16(const GLOBAL u32x4* a, int t, int KT, u32x4 (&ah)[MB],
+                                         u32x4 (&al)[MB]) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const floatx4 v = *(const GLOBAL floatx4*)(row + mb * 32 + 8 * q);
-                y[mb][4 * q + 0] = v.x;
-                y[mb][4 * q + 1] = v.y;
-                y[mb][4 * q + 2] = v.z;
-                y[mb][4 * q + 3] = v.w;
-            }
-        return;
+    for (int mb = 0; mb < MB; ++mb) {
+        ah[mb] = a[((mb * KT + t) * 2 + 0) * 64];
+        al[mb] = a[((mb * KT + t) * 2 + 1) * 64];
+    }
+}
+__device__ __forceinline__ void load_x16(const GLOBAL float* xb, int t, int h, int S,
+                                         float (&b)[8]) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) b[e] = xb[min(16 * t + 8 * h + e, S - 1) * PTILE];
+}
+// NT = 3: Pl Xh + Ph Xl + Ph Xh (small terms first); NT = 4 adds Pl Xl.
+template <int MB, int NT>
+__device__ __forceinline__ void mfma16_stage(floatx16 (&y)[MB], const u32x4 (&ah)[MB],
+                                             const u32x4 (&al)[MB], const float (&b)[8]) {
+    bf16x8 xh, xl;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        __bf16 hi, lo;
+        split_bf16(b[e], hi, lo);
+        xh[e] = hi;
+        xl[e] = lo;
     }
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) y[mb] = floatx16{0};
-    const GLOBAL float* xb = gin<float>(x) + tbase + n;
-    const GLOBAL floatx4* a = (const GLOBAL floatx4*)gin<float>(pk) + l;
-    for (int g = 0; g < KG; ++g) {
-        floatx4 am[MB];
+    for (int mb = 0; mb < MB; ++mb) {
+        const bf16x8 ph = __builtin_bit_cast(bf16x8, ah[mb]);
+        const bf16x8 pl = __builtin_bit_cast(bf16x8, al[mb]);
+        if (NT == 4) y[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pl, xl, y[mb], 0, 0, 0);
+        y[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pl, xh, y[mb], 0, 0, 0);
+        y[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ph, xl, y[mb], 0, 0, 0);
+        y[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ph, xh, y[mb], 0, 0, 0);
+    }
+}
+
+// Split-BF16 stream of NA partials children, KT stages each, operands PF
+// stages ahead (as stream_mfma).
+template <int MB, int KT, int NA, int NT>
+__device__ __forceinline__ void stream_bf16(const GLOBAL float* x0, const GLOBAL u32x4* a0,
+                                            const GLOBAL float* x1, const GLOBAL u32x4* a1,
+                                            int h, int S, floatx16 (&y)[2][MB]) {
+    constexpr int PF = 2, T = NA * KT;
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) am[mb] = a[(mb * KG + g) * 64];
-        float b[4];
+    for (int k = 0; k < NA; ++k)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) y[k][mb] = floatx16{0};
+    u32x4 ah[PF][MB], al[PF][MB];
+    float b[PF][8];
+#pragma unroll
+    for (int j = 0; j < PF && j < T; ++j) {
+        load_a16<MB>(j < KT ? a0 : a1, j % KT, KT, ah[j], al[j]);
+        load_x16(j < KT ? x0 : x1, j % KT, h, S, b[j]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+        mfma16_stage<MB, NT>(y[j / KT], ah[j % PF], al[j % PF], b[j % PF]);
+        if (j + PF < T) {
+            const int t = (j + PF) % KT;
+            load_a16<MB>(j + PF < KT ? a0 : a1, t, KT, ah[j % PF], al[j % PF]);
+            load_x16(j + PF < KT ? x0 : x1, t, h, S, b[j % PF]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Column P[., s] of a tip-state child in the accumulator layout.
+template <int MB>
+__device__ __forceinline__ void state_rows(const float* pk, const int* states, int p, int h,
+                                           floatx16 (&y)[MB]) {
+    const int s = gin<int>(states)[p];
+    const GLOBAL float* row = gin<float>(pk) + s * (MB * 32) + 4 * h;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int k = min(8 * g + 2 * q + h, S - 1);
-            b[q] = xb[k * PTILE];
+            const floatx4 v = *(const GLOBAL floatx4*)(row + mb * 32 + 8 * q);
+            y[mb][4 * q + 0] = v.x;
+            y[mb][4 * q + 1] = v.y;
+            y[mb][4 * q + 2] = v.z;
+            y[mb][4 * q + 3] = v.w;
         }
+}
+
+// The MFMA stream of NA partials children (KG stages each, child k into
+// y[k]) with matrix operands loaded PF stages ahead.  XU: the partials of
+// every stage are loaded up front (the whole tile in flight, one exposed
+// latency per tile); otherwise they follow the matrix operands PF stages
+// ahead.  Straight-line code, so the wait before each stage covers only
+// what that stage reads.
+template <int MB, int KG, int NA, bool XU>
+__device__ __forceinline__ void stream_mfma(const GLOBAL float* x0, const GLOBAL floatx4* a0,
+                                            const GLOBAL float* x1, const GLOBAL floatx4* a1,
+                                            int h, int S, floatx16 (&y)[2][MB]) {
+    constexpr int PF = 3, T = NA * KG;
+    constexpr int XB = XU ? T : PF;     // partials buffers
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+    for (int k = 0; k < NA; ++k)
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb)
-                y[mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(am[mb][q], b[q], y[mb], 0, 0, 0);
+        for (int mb = 0; mb < MB; ++mb) y[k][mb] = floatx16{0};
+    floatx4 am[PF][MB];
+    float b[XB][4];
+#pragma unroll
+    for (int j = 0; j < PF && j < T; ++j) {
+        load_a<MB>(j < KG ? a0 : a1, j % KG, KG, am[j]);
+        if (!XU) load_x(j < KG ? x0 : x1, j % KG, h, S, b[j]);
+    }
+    if (XU) {
+#pragma unroll
+        for (int j = 0; j < T; ++j) load_x(j < KG ? x0 : x1, j % KG, h, S, b[j % XB]);
+    }
+    // keep the stage order: the scheduler would otherwise regroup the loads
+    // and wait for all of them before the first instruction
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+        mfma_stage<MB>(y[j / KG], am[j % PF], b[j % XB]);
+        if (j + PF < T) {
+            const int g = (j + PF) % KG;
+            load_a<MB>(j + PF < KG ? a0 : a1, g, KG, am[j % PF]);
+            if (!XU) load_x(j + PF < KG ? x0 : x1, g, h, S, b[j % XB]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
 // One post-order operation per blockIdx.y (a dependency level), one tile of
-// 32 patterns per blockIdx.x, one wave per category.  Scaling semantics as
-// the generic path (BeagleCPUImpl.hpp:1555-1577, 4008-4133).
-template <int MB, bool AUTO>
+// 32 patterns per blockIdx.x, one wave per category.  The partials children
+// (0, 1 or 2 of them) form one stream of KG-group stages whose operands are
+// loaded PF stages ahead; tip-state children are gathered afterwards.
+// Scaling semantics as the generic path (BeagleCPUImpl.hpp:1555-1577,
+// 4008-4133).
+// MODE 0: FP32 matrix cores, partials pipelined; 1: FP32, whole partials
+// tile up front; 3, 4: split BF16 with 3 or 4 products.  KX: k groups of 8
+// (FP32) or k steps of 16 (BF16).
+template <int MB, int KX, bool AUTO, int MODE>
 __global__ __launch_bounds__(512) void k_postm(const PostJob* __restrict__ jobs, Dims d,
                                                const float* __restrict__ pack) {
     __shared__ float red[8][32];
@@ -139,27 +283,52 @@ __global__ __launch_bounds__(512) void k_postm(const PostJob* __restrict__ jobs,
     if (tp0 >= J.p1 || tp0 + PTILE <= J.p0) return;    // uniform over the workgroup
     const int p = tp0 + n;
     const bool in = p >= J.p0 && p < J.p1;
-    const int S = d.S, KG = (S + 7) / 8;
+    const int S = d.S;
     const long per = mfma_pack_reals(S);
     const long tbase = pbase(d, c, tp0);
-    floatx16 y[MB], z[MB];
-    child_mfma<MB>(J.x1, J.s1, pack + J.pk1 + c * per, tbase, S, KG, l, p, y);
-    child_mfma<MB>(J.x2, J.s2, pack + J.pk2 + c * per, tbase, S, KG, l, p, z);
+    // the partials children in list order: stream 0 is child 1 unless that
+    // child is tip states, stream 1 is child 2 when both are partials
+    const GLOBAL float* x1 = gin<float>(J.x1) + tbase + n;
+    const GLOBAL float* x2 = gin<float>(J.x2) + tbase + n;
+    const GLOBAL float* p1 = gin<float>(pack + J.pk1 + c * per);
+    const GLOBAL float* p2 = gin<float>(pack + J.pk2 + c * per);
+    const GLOBAL float* xs0 = J.s1 ? x2 : x1;
+    const GLOBAL float* ps0 = J.s1 ? p2 : p1;
+    floatx16 y[2][MB];
+    const int na = (J.s1 ? 0 : 1) + (J.s2 ? 0 : 1);
+    if (MODE <= 1) {
+        const GLOBAL floatx4* a0 = (const GLOBAL floatx4*)ps0 + l;
+        const GLOBAL floatx4* a2 = (const GLOBAL floatx4*)p2 + l;
+        if (na == 2) stream_mfma<MB, KX, 2, MODE == 1>(xs0, a0, x2, a2, h, S, y);
+        else if (na == 1) stream_mfma<MB, KX, 1, MODE == 1>(xs0, a0, x2, a2, h, S, y);
+    } else {
+        const GLOBAL u32x4* a0 = (const GLOBAL u32x4*)ps0 + l;
+        const GLOBAL u32x4* a2 = (const GLOBAL u32x4*)p2 + l;
+        if (na == 2) stream_bf16<MB, KX, 2, MODE>(xs0, a0, x2, a2, h, S, y);
+        else if (na == 1) stream_bf16<MB, KX, 1, MODE>(xs0, a0, x2, a2, h, S, y);
+    }
+    // tip-state children take the accumulator slots the stream left free
+    if (J.s2) state_rows<MB>(pack + J.pk2 + c * per, J.s2, p, h, y[1]);
+    if (J.s1) {
+        if (J.s2) state_rows<MB>(pack + J.pk1 + c * per, J.s1, p, h, y[0]);
+        else state_rows<MB>(pack + J.pk1 + c * per, J.s1, p, h, y[1]);
+    }
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) y[mb] *= z[mb];
+    for (int mb = 0; mb < MB; ++mb) y[0][mb] *= y[1][mb];
+    floatx16 (&v)[MB] = y[0];
     if (J.mode == SCALE_READ) {
         const float f = (float)gin<double>(J.scaleRead)[p];
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) y[mb][r] = y[mb][r] / f;
+            for (int r = 0; r < 16; ++r) v[mb][r] = v[mb][r] / f;
     }
     if (AUTO || J.mode == SCALE_WRITE) {
         float mx = 0.f;
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) mx = fmaxf(mx, y[mb][r]);
+            for (int r = 0; r < 16; ++r) mx = fmaxf(mx, v[mb][r]);
         mx = fmaxf(mx, __shfl_xor(mx, 32));
         if (h == 0) red[c][n] = mx;
         __syncthreads();
@@ -169,7 +338,7 @@ __global__ __launch_bounds__(512) void k_postm(const PostJob* __restrict__ jobs,
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) y[mb][r] = pow2(y[mb][r], -e);
+                for (int r = 0; r < 16; ++r) v[mb][r] = pow2(v[mb][r], -e);
             if (c == 0 && h == 0 && in) {
                 int t = e;
                 if (J.e1) t += gin<short>(J.e1)[p];
@@ -180,7 +349,7 @@ __global__ __launch_bounds__(512) void k_postm(const PostJob* __restrict__ jobs,
             if (mx == 0.f) mx = 1.f;
             const float inv = 1.f / mx;
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb) y[mb] *= inv;
+            for (int mb = 0; mb < MB; ++mb) v[mb] *= inv;
             if (c == 0 && h == 0 && in) {
                 const double lm = log((double)mx);
                 double w = J.logScalers ? lm : (double)mx;
@@ -197,24 +366,53 @@ __global__ __launch_bounds__(512) void k_postm(const PostJob* __restrict__ jobs,
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int i = mb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-            if (i < S && in) out[i * PTILE] = y[mb][r];
+            if (i < S && in) out[i * PTILE] = v[mb][r];
         }
 }
 
 void launch_post_mfma(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
-                      hipStream_t st) {
+                      int variant, hipStream_t st) {
     if (njobs <= 0) return;
     const dim3 grid(d.Ppad / PTILE, njobs), block(64 * d.C);
-    const int MB = mfma_mblocks(d.S);
-#define POSTM(M, A) hipLaunchKernelGGL((k_postm<M, A>), grid, block, 0, st, djobs, d, pack)
-    if (MB == 1) {
-        if (d.autoScale) POSTM(1, true);
-        else POSTM(1, false);
-    } else {
-        if (d.autoScale) POSTM(2, true);
-        else POSTM(2, false);
+    const int KG = mfma_kgroups(d.S);
+#define POSTM_V(M, K, A)                                                                       \
+    if (variant == 1)                                                                          \
+        hipLaunchKernelGGL((k_postm<M, K, A, 1>), grid, block, 0, st, djobs, d, pack);        \
+    else                                                                                       \
+        hipLaunchKernelGGL((k_postm<M, K, A, 0>), grid, block, 0, st, djobs, d, pack);
+#define POSTM(M, K)                                                                            \
+    case K:                                                                                    \
+        if (d.autoScale) { POSTM_V(M, K, true) } else { POSTM_V(M, K, false) }                 \
+        break;
+    switch (KG) {
+        POSTM(1, 1) POSTM(1, 2) POSTM(1, 3) POSTM(1, 4)
+        POSTM(2, 5) POSTM(2, 6) POSTM(2, 7) POSTM(2, 8)
+        default: break;
     }
 #undef POSTM
+#undef POSTM_V
+}
+
+void launch_post_bf16(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
+                      int terms, hipStream_t st) {
+    if (njobs <= 0) return;
+    const dim3 grid(d.Ppad / PTILE, njobs), block(64 * d.C);
+    const int KT = mfma_ksteps16(d.S);
+#define POSTB_V(M, K, A)                                                                       \
+    if (terms == 4)                                                                            \
+        hipLaunchKernelGGL((k_postm<M, K, A, 4>), grid, block, 0, st, djobs, d, pack);        \
+    else                                                                                       \
+        hipLaunchKernelGGL((k_postm<M, K, A, 3>), grid, block, 0, st, djobs, d, pack);
+#define POSTB(M, K)                                                                            \
+    case K:                                                                                    \
+        if (d.autoScale) { POSTB_V(M, K, true) } else { POSTB_V(M, K, false) }                 \
+        break;
+    switch (KT) {
+        POSTB(1, 1) POSTB(1, 2) POSTB(2, 3) POSTB(2, 4)
+        default: break;
+    }
+#undef POSTB
+#undef POSTB_V
 }
 
 template void launch_pack_matrices<float>(const PackJob*, int, int, int, const void*, float*,

@@ -17,6 +17,16 @@ extern "C" {
  * with events.  No counterpart in the reference. */
 int beagleExtGetStream(int instance, void** outStream);
 
+/* Matrix-core engine of the post-order partials products of an FP32
+ * instance with 5 to 64 states (no counterpart in the reference).
+ * terms = 0: FP32 matrix cores (the default).  terms = 3: BF16 matrix cores
+ * with every FP32 operand split into a BF16 high and low part, x = hi + lo,
+ * and three products per pair (lo*hi + hi*lo + hi*hi, FP32 accumulation);
+ * terms = 4 adds lo*lo.  Returns BEAGLE_ERROR_NO_IMPLEMENTATION for a
+ * double-precision instance or another state count, BEAGLE_ERROR_OUT_OF_RANGE
+ * for other values of terms.  Applies to operations enqueued afterwards. */
+int beagleExtSetSplitBF16(int instance, int terms);
+
 /* Pre-order operations fused with the derivative sums of each operation's
  * branch.  Equivalent to
  *   beagleUpdatePrePartials(instance, operations, operationCount, OP_NONE);

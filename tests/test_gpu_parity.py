@@ -172,19 +172,30 @@ MFMA_CASES = [
 ]
 
 
+# engine: FP32 matrix cores, or split BF16 (hi/lo operands, three products);
+# tolerances (logL, sites, root rows) against the FP64 oracle
+MFMA_ENGINES = {"fp32": (0, 1e-6, 2e-6, 1e-5),
+                "bf16x3": (3, 1e-6, 1e-5, 1e-4)}
+
+
+@pytest.mark.parametrize("engine", sorted(MFMA_ENGINES))
 @pytest.mark.parametrize("S,n,P,shape,tips_as,scaling,missing", MFMA_CASES)
 def test_mfma_post_order_single_matches_oracle(amd, oracle, S, n, P, shape, tips_as, scaling,
-                                               missing):
+                                               missing, engine):
     """S > 4 in FP32: the post-order runs on the FP32 matrix cores
-    (v_mfma_f32_32x32x2f32).  Identical P(t) on both sides; the FP32
-    result must stay within 1e-6 relative log-likelihood of the FP64
-    oracle, site log-likelihoods within 2e-6 relative, and the root
-    partials (where both sides share a scale) within 1e-5 of each row's
-    maximum."""
+    (v_mfma_f32_32x32x2f32) or, after beagleExtSetSplitBF16(3), on the BF16
+    matrix cores with hi/lo split operands.  Identical P(t) on both
+    sides; FP32 must stay within 1e-6 relative log-likelihood of the FP64
+    oracle, site log-likelihoods within 2e-6 relative and the root partials
+    (where both sides share a scale) within 1e-5 of each row's maximum;
+    split BF16 within 1e-6, 1e-5 and 1e-4."""
+    terms, tl, ts, tr = MFMA_ENGINES[engine]
     prob = synth.make_problem(17 + S + n, n, P, S=S, shape=shape, missing=missing)
     ev = driver.Evaluator(amd, prob, precision="single", tips_as=tips_as, scaling=scaling,
                           matrices="explicit")
     assert ev.inst.impl_name == "MI355X-HIP-Single"
+    if terms:
+        ev.inst.set_split_bf16(terms)
     lg = ev.log_likelihood()
     sg = ev.inst.site_log_likelihoods()
     rg = ev.inst.get_partials(prob.tree.root)
@@ -195,8 +206,9 @@ def test_mfma_post_order_single_matches_oracle(amd, oracle, S, n, P, shape, tips
     so = ev.inst.site_log_likelihoods()
     ro = ev.inst.get_partials(prob.tree.root)
     ev.close()
-    assert rel(lg, lo) < 1e-6, (lg, lo)
-    close(sg, so, 2e-6, 0)
+    print(f"{engine} S={S}: logL rel err {rel(lg, lo):.2e}, worst site rel err {rel(sg, so):.2e}")
+    assert rel(lg, lo) < tl, (lg, lo)
+    close(sg, so, ts, 0)
     if scaling != "auto":
         # unscaled rows whose FP64 maximum lies below FP32's normal range
         # (1.2e-38) underflow in FP32 and are left out
@@ -204,7 +216,20 @@ def test_mfma_post_order_single_matches_oracle(amd, oracle, S, n, P, shape, tips
         ro = np.asarray(ro).reshape(-1, S)
         keep = ro.max(axis=1) > 1e-30
         assert keep.mean() > 0.5
-        close_rows(rg[keep], ro[keep], S, 1e-5)
+        close_rows(rg[keep], ro[keep], S, tr)
+
+
+def test_split_bf16_engine_selection(amd):
+    """beagleExtSetSplitBF16: FP32 instances with 5..64 states only, terms 0,
+    3 or 4."""
+    for S, prec, ok in ((20, "single", True), (20, "double", False), (4, "single", False)):
+        prob = synth.make_problem(3, 6, 70, S=S)
+        ev = driver.Evaluator(amd, prob, precision=prec)
+        rc = amd.call("ExtSetSplitBF16", ev.inst.h, 3)
+        assert rc == (api.SUCCESS if ok else api.ERROR_NO_IMPLEMENTATION), (S, prec, rc)
+        if ok:
+            assert amd.call("ExtSetSplitBF16", ev.inst.h, 2) == api.ERROR_OUT_OF_RANGE
+        ev.close()
 
 
 def test_mfma_post_order_partitions(amd, oracle):
