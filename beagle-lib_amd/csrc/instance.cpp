@@ -758,7 +758,7 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
     }
     if (count == 0) return BEAGLE_SUCCESS;
     const bool fourState = dims_.S == 4 && dims_.C <= 8;
-    if (!fourState && !dp_ && mfma_post_supported(dims_.S, dims_.C)) {
+    if (!fourState && mfma_post_supported(dims_.S, dims_.C)) {
         launchPostMfma(ops, stride, count);
     } else if (!fourState) {
         auto d = reinterpret_cast<const PostJob*>(upload(hPost_.data(), count * sizeof(PostJob)));
@@ -808,13 +808,15 @@ int Instance::postLevels(const int* ops, int stride, int count, std::vector<int>
     return dependencyLevels(acc, bufferCount_ + scaleCount_, level);
 }
 
-// MFMA post-order (S > 4, FP32 partials; FP32 or split-BF16 matrix cores):
-// every child matrix of the list is repacked once into the pack slab, then
-// each dependency level is one launch over (pattern tiles x ops).  A list whose ops share a cumulative
-// scale buffer runs one op per launch in list order (each rescaling op adds
-// to the same per-pattern sums).
+// MFMA post-order (S > 4; FP64, FP32 or split-BF16 matrix cores): every
+// child matrix of the list is repacked once into the pack slab, then each
+// dependency level is one launch over (pattern tiles x ops).  A list whose
+// ops share a cumulative scale buffer runs one op per launch in list order
+// (each rescaling op adds to the same per-pattern sums).
 void Instance::launchPostMfma(const int* ops, int stride, int count) {
-    const size_t per = (size_t)mfma_pack_reals(dims_.S) * dims_.C;
+    // floats per packed child matrix (all categories)
+    const size_t per = (size_t)(dp_ ? 2 * mfma_pack_doubles(dims_.S) : mfma_pack_reals(dims_.S)) *
+                       dims_.C;
     std::vector<PackJob> pj(2 * (size_t)count);
     for (int o = 0; o < count; ++o) {
         PostJob& J = hPost_[o];
@@ -843,10 +845,13 @@ void Instance::launchPostMfma(const int* ops, int stride, int count) {
     std::vector<int> fill(start.begin(), start.end() - 1);
     for (int o = 0; o < count; ++o) sorted[fill[level[o]]++] = hPost_[o];
     auto dpj = reinterpret_cast<const PackJob*>(upload(pj.data(), pj.size() * sizeof(PackJob)));
-    launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+    if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+    else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     auto d = reinterpret_cast<const PostJob*>(upload(sorted.data(), count * sizeof(PostJob)));
     for (int l = 0; l < nl; ++l)
-        if (splitBf16_)
+        if (dp_)
+            launch_post_mfma64(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
+        else if (splitBf16_)
             launch_post_bf16(d + start[l], start[l + 1] - start[l], dims_, pack, bf16Terms_,
                              stream_);
         else

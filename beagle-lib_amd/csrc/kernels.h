@@ -155,6 +155,15 @@ __host__ __device__ inline int mfma_pack_reals(int S) {
     m = m > b ? m : b;
     return (m + 63) / 64 * 64;
 }
+// FP64 path: M blocks of 16 output states (v_mfma_f64_16x16x4f64), the same k
+// groups of 8; doubles per category of one packed child matrix
+__host__ __device__ inline int mfma_mblocks16(int S) { return (S + 15) / 16; }
+__host__ __device__ inline int mfma_pack_doubles(int S) {
+    const int a = mfma_mblocks16(S) * mfma_kgroups(S) * 128;
+    const int t = (S + 1) * mfma_mblocks16(S) * 16;
+    const int m = a > t ? a : t;
+    return (m + 31) / 32 * 32;
+}
 // S range and category count the MFMA post-order kernel covers
 __host__ __device__ inline bool mfma_post_supported(int S, int C) { return S > 4 && S <= 64 && C <= 8; }
 
@@ -166,6 +175,12 @@ void launch_pack_matrices(const PackJob* jobs, int njobs, int S, int C, const vo
 // partials loaded a few k groups ahead; 1: the whole partials tile up front.
 void launch_post_mfma(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
                       int variant, hipStream_t st);
+// FP64 partials on v_mfma_f64_16x16x4f64: the matrices packed with
+// launch_pack_matrices64 (pack offsets in floats, even), grid (Ppad/16, njobs).
+void launch_pack_matrices64(const PackJob* jobs, int njobs, int S, int C, const void* mats,
+                            float* pack, hipStream_t st);
+void launch_post_mfma64(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
+                        hipStream_t st);
 // The same on the BF16 matrix cores with hi/lo split operands (terms: 3 or 4
 // products per split pair); the matrices packed as PACK_OPERAND_BF16.
 void launch_post_bf16(const PostJob* djobs, int njobs, const Dims& d, const float* pack,

@@ -415,6 +415,237 @@ void launch_post_bf16(const PostJob* djobs, int njobs, const Dims& d, const floa
 #undef POSTB_V
 }
 
+// ---------------------------------------------------------------------------
+// FP64: v_mfma_f64_16x16x4f64.  Lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; element r of its accumulator is D[4 r + l / 16][l % 16]
+// (rows interleave across the four lane groups).  A wavefront owns one
+// category and 16 patterns; M blocks of 16 output states; a stage is 8
+// inner states, k = 8 g + 4 q + l / 16 for instruction q of group g.
+
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef double doublex2 __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(WG) void k_pack_mats64(const PackJob* __restrict__ jobs, int njobs,
+                                                    int S, int C,
+                                                    const double* __restrict__ mats,
+                                                    double* __restrict__ pack) {
+    const int per = mfma_pack_doubles(S);
+    const long t = (long)blockIdx.x * WG + threadIdx.x;
+    if (t >= (long)njobs * C * per) return;
+    const int j = (int)(t / ((long)C * per));
+    const int rem = (int)(t - (long)j * C * per);
+    const int c = rem / per, e = rem - c * per;
+    const PackJob J = jobs[j];
+    const double* M = mats + J.src + (long)c * S * (S + 1);
+    const int MB = mfma_mblocks16(S), KG = mfma_kgroups(S);
+    double v = 0.0;
+    if (J.kind == PACK_OPERAND) {
+        // [mb][group g][lane][q]: row 16 mb + lane % 16, column 8 g + 4 q + lane / 16
+        if (e < MB * KG * 128) {
+            const int mb = e / (KG * 128), r = e - mb * KG * 128;
+            const int g = r / 128, lane = (r & 127) >> 1, q = r & 1;
+            const int i = mb * 16 + (lane & 15), k = 8 * g + 4 * q + (lane >> 4);
+            if (i < S && k < S) v = M[i * (S + 1) + k];
+        }
+    } else {
+        // [state s (S: the pad column)][mb][lane group kq][r]: row 16 mb + 4 r + kq
+        const int W = MB * 16;
+        if (e < (S + 1) * W) {
+            const int s = e / W, w = e - s * W;
+            const int mb = w >> 4, kq = (w & 15) >> 2, r = w & 3;
+            const int i = mb * 16 + 4 * r + kq;
+            if (i < S) v = M[i * (S + 1) + s];
+        }
+    }
+    pack[J.dst / 2 + (long)c * per + e] = v;
+}
+
+void launch_pack_matrices64(const PackJob* jobs, int njobs, int S, int C, const void* mats,
+                            float* pack, hipStream_t st) {
+    const long n = (long)njobs * C * mfma_pack_doubles(S);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pack_mats64, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, st, jobs,
+                       njobs, S, C, reinterpret_cast<const double*>(mats),
+                       reinterpret_cast<double*>(pack));
+}
+
+template <int MB>
+__device__ __forceinline__ void load_a64(const GLOBAL doublex2* a, int g, int KG,
+                                         doublex2 (&am)[MB]) {
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) am[mb] = a[(mb * KG + g) * 64];
+}
+__device__ __forceinline__ void load_x64(const GLOBAL double* xb, int g, int kq, int S,
+                                         double (&b)[2]) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) b[q] = xb[min(8 * g + 4 * q + kq, S - 1) * PTILE];
+}
+template <int MB>
+__device__ __forceinline__ void mfma64_stage(doublex4 (&y)[MB], const doublex2 (&am)[MB],
+                                             const double (&b)[2]) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+            y[mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[mb][q], b[q], y[mb], 0, 0, 0);
+}
+
+// FP64 stream of NA partials children (KG stages each, child k into y[k]),
+// operands loaded PF stages ahead; straight-line code.
+template <int MB, int KG, int NA>
+__device__ __forceinline__ void stream_mfma64(const GLOBAL double* x0, const GLOBAL doublex2* a0,
+                                              const GLOBAL double* x1, const GLOBAL doublex2* a1,
+                                              int kq, int S, doublex4 (&y)[2][MB]) {
+    constexpr int PF = 3, T = NA * KG;
+#pragma unroll
+    for (int k = 0; k < NA; ++k)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) y[k][mb] = doublex4{0.0, 0.0, 0.0, 0.0};
+    doublex2 am[PF][MB];
+    double b[PF][2];
+#pragma unroll
+    for (int j = 0; j < PF && j < T; ++j) {
+        load_a64<MB>(j < KG ? a0 : a1, j % KG, KG, am[j]);
+        load_x64(j < KG ? x0 : x1, j % KG, kq, S, b[j]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+        mfma64_stage<MB>(y[j / KG], am[j % PF], b[j % PF]);
+        if (j + PF < T) {
+            const int g = (j + PF) % KG;
+            load_a64<MB>(j + PF < KG ? a0 : a1, g, KG, am[j % PF]);
+            load_x64(j + PF < KG ? x0 : x1, g, kq, S, b[j % PF]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Column P[., s] of a tip-state child: the lane's four rows 16 mb + 4 r + kq
+// are contiguous in the packed rows.
+template <int MB>
+__device__ __forceinline__ void state_rows64(const double* pk, const int* states, int p, int kq,
+                                             doublex4 (&y)[MB]) {
+    const int s = gin<int>(states)[p];
+    const GLOBAL double* row = gin<double>(pk) + s * (MB * 16) + 4 * kq;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+        const doublex2 u = *(const GLOBAL doublex2*)(row + mb * 16);
+        const doublex2 w = *(const GLOBAL doublex2*)(row + mb * 16 + 2);
+        y[mb] = doublex4{u.x, u.y, w.x, w.y};
+    }
+}
+
+// FP64 post-order: one operation per blockIdx.y, 16 patterns per
+// blockIdx.x, one wave per category; scaling as k_postm.
+template <int MB, int KG, bool AUTO>
+__global__ __launch_bounds__(512) void k_postd(const PostJob* __restrict__ jobs, Dims d,
+                                               const float* __restrict__ pack) {
+    __shared__ double red[8][16];
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int l = threadIdx.x & 63, n = l & 15, kq = l >> 4;
+    const PostJob& J = jobs[blockIdx.y];
+    const int tp0 = blockIdx.x * 16;
+    if (tp0 >= J.p1 || tp0 + 16 <= J.p0) return;    // uniform over the workgroup
+    const int p = tp0 + n;
+    const bool in = p >= J.p0 && p < J.p1;
+    const int S = d.S;
+    const long per = mfma_pack_doubles(S);
+    const long tbase = pbase(d, c, tp0);
+    const GLOBAL double* x1 = gin<double>(J.x1) + tbase + n;
+    const GLOBAL double* x2 = gin<double>(J.x2) + tbase + n;
+    const double* p1 = reinterpret_cast<const double*>(pack + J.pk1) + c * per;
+    const double* p2 = reinterpret_cast<const double*>(pack + J.pk2) + c * per;
+    const GLOBAL doublex2* a1 = (const GLOBAL doublex2*)gin<double>(p1) + l;
+    const GLOBAL doublex2* a2 = (const GLOBAL doublex2*)gin<double>(p2) + l;
+    doublex4 y[2][MB];
+    const int na = (J.s1 ? 0 : 1) + (J.s2 ? 0 : 1);
+    if (na == 2) stream_mfma64<MB, KG, 2>(x1, a1, x2, a2, kq, S, y);
+    else if (na == 1)
+        stream_mfma64<MB, KG, 1>(J.s1 ? x2 : x1, J.s1 ? a2 : a1, x2, a2, kq, S, y);
+    if (J.s2) state_rows64<MB>(p2, J.s2, p, kq, y[1]);
+    if (J.s1) {
+        if (J.s2) state_rows64<MB>(p1, J.s1, p, kq, y[0]);
+        else state_rows64<MB>(p1, J.s1, p, kq, y[1]);
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) y[0][mb] *= y[1][mb];
+    doublex4 (&v)[MB] = y[0];
+    if (J.mode == SCALE_READ) {
+        const double f = gin<double>(J.scaleRead)[p];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[mb][r] = v[mb][r] / f;
+    }
+    if (AUTO || J.mode == SCALE_WRITE) {
+        double mx = 0.0;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) mx = fmax(mx, v[mb][r]);
+        mx = fmax(mx, __shfl_xor(mx, 16));
+        mx = fmax(mx, __shfl_xor(mx, 32));
+        if (kq == 0) red[c][n] = mx;
+        __syncthreads();
+        for (int cc = 0; cc < d.C; ++cc) mx = fmax(mx, red[cc][n]);
+        if (AUTO) {
+            const int e = mx > 0.0 ? exponent_of(mx) : 0;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[mb][r] = pow2(v[mb][r], -e);
+            if (c == 0 && kq == 0 && in) {
+                int t = e;
+                if (J.e1) t += gin<short>(J.e1)[p];
+                if (J.e2) t += gin<short>(J.e2)[p];
+                gout<short>(J.ed)[p] = (short)t;
+            }
+        } else {
+            if (mx == 0.0) mx = 1.0;
+            const double inv = 1.0 / mx;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) v[mb] *= inv;
+            if (c == 0 && kq == 0 && in) {
+                const double lm = log(mx);
+                double w = J.logScalers ? lm : mx;
+                if (J.cumulative) gout<double>(J.cumulative)[p] += lm;
+                if (J.accA) w += gin<double>(J.accA)[p];
+                if (J.accB) w += gin<double>(J.accB)[p];
+                gout<double>(J.scaleWrite)[p] = w;
+            }
+        }
+    }
+    GLOBAL double* out = gout<double>(J.dest) + tbase + n;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = mb * 16 + 4 * r + kq;
+            if (i < S && in) out[i * PTILE] = v[mb][r];
+        }
+}
+
+void launch_post_mfma64(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
+                        hipStream_t st) {
+    if (njobs <= 0) return;
+    const dim3 grid(d.Ppad / 16, njobs), block(64 * d.C);
+    const int KG = mfma_kgroups(d.S);
+#define POSTD(M, K)                                                                            \
+    case K:                                                                                    \
+        if (d.autoScale)                                                                       \
+            hipLaunchKernelGGL((k_postd<M, K, true>), grid, block, 0, st, djobs, d, pack);    \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_postd<M, K, false>), grid, block, 0, st, djobs, d, pack);   \
+        break;
+    switch (KG) {
+        POSTD(1, 1) POSTD(1, 2) POSTD(2, 3) POSTD(2, 4)
+        POSTD(3, 5) POSTD(3, 6) POSTD(4, 7) POSTD(4, 8)
+        default: break;
+    }
+#undef POSTD
+}
+
 template void launch_pack_matrices<float>(const PackJob*, int, int, int, const void*, float*,
                                           hipStream_t);
 
