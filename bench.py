@@ -20,6 +20,12 @@ Also reported: the plain FP64 path through the beagle.h calls
 fused path, the FP64 post-order traversal's HBM bandwidth, and the
 reference's threaded CPU plugin on this host's cores.
 
+codon: the 61-state GY94 codon workload (64 taxa, 20,000 patterns, Gamma(4);
+transition matrices + post-order + root logL per evaluation) on the FP64,
+FP32 and split-BF16 matrix-core post-order kernels: evaluations/s, error
+against FP64, partials GB/s and algorithmic TFLOP/s against each MFMA
+peak, and the reference CPU plugin on a pattern sample.
+
 roofline: the dominant kernel of the value path, the node-fused FP32
 pre-order + derivative traversal of the caterpillar (one launch of 127
 node jobs, i.e. 254 pre-order operations), timed with HIP events on the
@@ -161,6 +167,119 @@ def cpu_baseline(budget_s):
                       f"derivative calls run on one), rate scaled by {sample}/100000"}
 
 
+CODON = dict(S=61, taxa=64, patterns=20_000, shape="yule")
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
+
+
+def codon_problem(patterns=None):
+    from beagle_amd import synth
+    return synth.make_problem(SEED, CODON["taxa"], patterns or CODON["patterns"], S=CODON["S"],
+                              shape=CODON["shape"])
+
+
+def postorder_flops(prob):
+    """Algorithmic flops of one post-order traversal (synthetictest's
+    convention, examples/synthetictest/synthetictest.cpp:2567-2580): per
+    operation C*P*S*f with f = 4S-1 (two partials children), 2S (one tip),
+    1 (two tips)."""
+    C, P, S = prob.C, prob.P, prob.S
+    n = prob.tree.n
+    total = 0
+    for v in prob.tree.postorder():
+        tips = sum(1 for x in prob.tree.children[v] if x < n)
+        total += C * P * S * (4 * S - 1 if tips == 0 else 2 * S if tips == 1 else 1)
+    return total
+
+
+def codon_bench(lib, steps):
+    """61-state GY94 codon model, 64 taxa, 20,000 patterns, Gamma(4): one
+    evaluation = transition matrices + post-order + root log-likelihood.
+    FP64 (v_mfma_f64_16x16x4f64), FP32 (v_mfma_f32_32x32x2f32) and split BF16
+    (v_mfma_f32_32x32x16_bf16, three products per hi/lo pair) post-order
+    kernels, all with exact power-of-two rescaling; errors against FP64."""
+    import numpy as np
+    import torch
+    from beagle_amd import api, driver
+    prob = codon_problem()
+    flops = postorder_flops(prob)
+    out, ref = {}, None
+    for name, prec, terms, peak in (("fp64", "double", 0, "f64"), ("fp32_mfma", "single", 0, "f32"),
+                                    ("split_bf16x3", "single", 3, "bf16")):
+        ev = driver.Evaluator(lib, prob, precision=prec, preorder=False, scaling="auto",
+                              extra_flags=api.FLAG_COMPUTATION_ASYNCH)
+        if terms:
+            ev.inst.set_split_bf16(terms)
+        st = torch.cuda.ExternalStream(ev.inst.stream())
+        for _ in range(3):
+            ev.log_likelihood()
+        n = max(steps, 10)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            logl = ev.log_likelihood()
+        torch.cuda.synchronize()
+        rate = n / (time.perf_counter() - t0)
+        pairs = []
+        for _ in range(n):
+            ev.update_matrices()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            ev.post_order()
+            e1.record(st)
+            ev.root()
+            pairs.append((e0, e1))
+        torch.cuda.synchronize()
+        ms = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
+        ev.close()
+        w = 8 if prec == "double" else 4
+        gbs = postorder_bytes(prob, w, True) / (ms * 1e-3) / 1e9
+        tf = flops / (ms * 1e-3) / 1e12
+        if ref is None:
+            ref = logl
+        out[name] = {"evals_per_s": rate, "logL": logl, "logL_rel_err_vs_fp64": rel(logl, ref),
+                     "postorder_ms": ms, "partials_GBps": gbs,
+                     "partials_frac_of_hbm_peak": gbs / HBM_PEAK_GBS,
+                     "algorithmic_TFLOPs": tf, "mfma_peak_TFLOPs": PEAK_TFLOPS[peak],
+                     "frac_of_mfma_peak": tf / PEAK_TFLOPS[peak]}
+    for name in ("fp32_mfma", "split_bf16x3"):
+        out[name]["speedup_vs_fp64"] = out[name]["evals_per_s"] / out["fp64"]["evals_per_s"]
+    out["workload"] = ("61-state GY94 codon, 64 taxa (Yule), 20,000 patterns, Gamma(4); "
+                       "matrices + post-order + root logL per evaluation; algorithmic flops "
+                       f"{flops:.3e} per traversal")
+    return out
+
+
+def cpu_baseline_codon(budget_s):
+    """The reference CPU plugin (threaded SSE, double) on the codon workload:
+    log-likelihood evaluations (matrices, post-order, root) of a pattern
+    sample, scaled to 20,000 patterns."""
+    from beagle_amd import api, driver
+    ref_path = os.path.join(REPO, "oracle", "_ref", "libhmsbeagle.so")
+    if not os.path.exists(ref_path):
+        return None
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    sample = 1000
+    lib = api.Library(ref_path, "beagle")
+    flags = api.FLAG_PROCESSOR_CPU | api.FLAG_VECTOR_SSE | api.FLAG_THREADING_CPP
+    ev = driver.Evaluator(lib, codon_problem(sample), preorder=False, scaling="manual",
+                          extra_flags=flags, transpose="none")
+    lib.call("SetCPUThreadCount", ev.inst.h, cores)
+    ev.log_likelihood()
+    t0, n = time.perf_counter(), 0
+    while n < 2 or time.perf_counter() - t0 < budget_s:
+        ev.log_likelihood()
+        n += 1
+    dt = time.perf_counter() - t0
+    name = ev.inst.impl_name
+    ev.close()
+    return {"value": n / dt * sample / CODON["patterns"], "unit": "evals/s", "cores": cores,
+            "kind": "reference",
+            "sample": f"{n} FP64 logL evaluations of the 64-taxon codon tree on a {sample}-pattern "
+                      f"alignment from the same generators, {name} ({dt:.1f} s), rate scaled by "
+                      f"{sample}/20000"}
+
+
 def rel(a, b):
     return abs(a - b) / abs(b)
 
@@ -184,7 +303,9 @@ def main():
     args = ap.parse_args()
 
     if args.cpu_baseline_only:
-        print(json.dumps(cpu_baseline(args.cpu_budget)))
+        res = cpu_baseline(args.cpu_budget)
+        res["codon"] = cpu_baseline_codon(min(5.0, args.cpu_budget / 3))
+        print(json.dumps(res))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -295,6 +416,8 @@ def main():
     post_gbs = postorder_bytes(probs[0], 8) / (post_ms * 1e-3) / 1e9
     ev.close()
 
+    codon = codon_bench(lib, args.steps)
+
     if args.calibrate_copy:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
         dst = torch.empty_like(src)
@@ -358,7 +481,10 @@ def main():
             "postorder_kernel": "k_post4<double,false>: caterpillar, 127 operations, one launch"},
         "logL": [r[0] for r in api64],
         "cpu_baseline": cpu,
+        "codon": codon,
     }
+    if cpu and cpu.get("codon"):
+        codon["cpu_reference"] = cpu.pop("codon")
     print(json.dumps(out))
 
 
