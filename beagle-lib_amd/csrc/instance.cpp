@@ -478,22 +478,67 @@ int Instance::setPatternWeights(const double* w) {
     return BEAGLE_SUCCESS;
 }
 
-// Contiguous partitions only: pattern k of partition q lies in
-// [partStart[q], partStart[q+1]).  Non-contiguous assignments would need
-// the reference's pattern reordering (BeagleCPUImpl.hpp:4140-4219).
+// Pattern k of partition q lies in [partStart[q], partStart[q+1]).  When the
+// partition indices are not already 0, 1, 2, ... in runs, the patterns are
+// reordered so that each partition is contiguous, keeping their order within
+// a partition: pattern i moves to partStart[part[i]] + its rank in its
+// partition.  The tips' states or partials and the pattern weights are
+// permuted (other buffers are not), site log-likelihoods are returned in the
+// original order, and a second reorder is refused (BeagleCPUImpl.hpp:803-920,
+// 4140-4219, 1009-1021).
 int Instance::setPatternPartitions(int count, const int* parts) {
     if (count < 1) return BEAGLE_ERROR_OUT_OF_RANGE;
-    if (parts[0] != 0) return BEAGLE_ERROR_NO_IMPLEMENTATION;
-    int run = 0;
-    for (int k = 1; k < dims_.P; ++k) {
-        if (parts[k] != parts[k - 1]) ++run;
-        if (parts[k] != run) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    const int P = dims_.P;
+    for (int k = 0; k < P; ++k)
+        if (parts[k] < 0 || parts[k] >= count) return BEAGLE_ERROR_OUT_OF_RANGE;
+    bool reorder = false;
+    for (int k = 0, run = 0; k < P; ++k) {
+        if (k > 0 && parts[k] != parts[k - 1]) ++run;
+        if (parts[k] != run) {
+            reorder = true;
+            break;
+        }
     }
-    if (run + 1 != count) return BEAGLE_ERROR_OUT_OF_RANGE;
+    if (!reorder) {
+        partStart_.assign(count + 1, P);
+        partStart_[0] = 0;
+        for (int k = 1; k < P; ++k)
+            if (parts[k] != parts[k - 1]) partStart_[parts[k]] = k;
+        partitionCount_ = count;
+        return BEAGLE_SUCCESS;
+    }
+    if (!newOrder_.empty()) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    std::vector<int> size(count, 0), order(P);
+    for (int k = 0; k < P; ++k) order[k] = size[parts[k]]++;
     partStart_.assign(count + 1, 0);
-    for (int k = 1; k < dims_.P; ++k)
-        if (parts[k] != parts[k - 1]) partStart_[parts[k]] = k;
-    partStart_[count] = dims_.P;
+    for (int q = 0; q < count; ++q) partStart_[q + 1] = partStart_[q] + size[q];
+    for (int k = 0; k < P; ++k) order[k] += partStart_[parts[k]];
+    // pattern weights
+    std::vector<double> w(P);
+    for (int k = 0; k < P; ++k) w[order[k]] = hPatternWeights_.empty() ? 0.0 : hPatternWeights_[k];
+    if (!hPatternWeights_.empty()) setPatternWeights(w.data());
+    // tips: states, or partials in the device layout
+    for (int t = 0; t < tipCount_; ++t) {
+        if (dStates_[t]) {
+            std::vector<int> a(dims_.Ppad), b;
+            copyOut(a.data(), dStates_[t], a.size() * sizeof(int));
+            b = a;
+            for (int k = 0; k < P; ++k) b[order[k]] = a[k];
+            copyIn(dStates_[t], b.data(), b.size() * sizeof(int));
+        } else if (dPartials_[t]) {
+            const size_t rs = realSize();
+            std::vector<char> a(partialsBytes_), b;
+            copyOut(a.data(), dPartials_[t], partialsBytes_);
+            b = a;
+            for (int c = 0; c < dims_.C; ++c)
+                for (int k = 0; k < P; ++k)
+                    for (int i = 0; i < dims_.S; ++i)
+                        std::memcpy(&b[(pbase(dims_, c, order[k]) + (long)i * dims_.sStride) * rs],
+                                    &a[(pbase(dims_, c, k) + (long)i * dims_.sStride) * rs], rs);
+            copyIn(dPartials_[t], b.data(), partialsBytes_);
+        }
+    }
+    newOrder_ = std::move(order);
     partitionCount_ = count;
     return BEAGLE_SUCCESS;
 }
@@ -1712,7 +1757,14 @@ int Instance::getDerivatives(double* d1, double* d2) {
 }
 
 int Instance::getSiteLogLikelihoods(double* out) {
-    copyOut(out, dSiteL_, dims_.P * sizeof(double));
+    if (newOrder_.empty()) {
+        copyOut(out, dSiteL_, dims_.P * sizeof(double));
+    } else {
+        // reordered patterns: back to the client's order (BeagleCPUImpl.hpp:1009-1021)
+        std::vector<double> tmp(dims_.P);
+        copyOut(tmp.data(), dSiteL_, dims_.P * sizeof(double));
+        for (int k = 0; k < dims_.P; ++k) out[k] = tmp[newOrder_[k]];
+    }
     return BEAGLE_SUCCESS;
 }
 

@@ -234,6 +234,7 @@ private:
     size_t packCap_ = 0;
     void* dScratch_ = nullptr;        // C*Ppad*Sp reals (generic pre-order)
     std::vector<int> partStart_;
+    std::vector<int> newOrder_;       // after a pattern reorder: internal index of pattern i
     int partitionCount_ = 1;
     std::vector<Slot> ring_;
     int ringPos_ = 0;

@@ -75,6 +75,7 @@ typedef struct {
     double* denTmp;      /* P */
     int* partStart;      /* partition boundaries, partitionCount+1 */
     int partitionCount;
+    int* newOrder;       /* after a pattern reorder: internal index of pattern i */
 } Orc;
 
 #define MAX_INSTANCES 4096
@@ -169,7 +170,7 @@ int orcFinalizeInstance(int instance) {
     free(o->partials); free(o->tipStates); free(o->evals); free(o->cube); free(o->freqs);
     free(o->catWeights); free(o->catRates); free(o->patternWeights); free(o->matrices);
     free(o->scale); free(o->siteLogL); free(o->siteD1); free(o->siteD2); free(o->numTmp);
-    free(o->denTmp); free(o->partStart);
+    free(o->denTmp); free(o->partStart); free(o->newOrder);
     free(o);
     g_inst[instance] = NULL;
     return ORC_SUCCESS;
@@ -299,23 +300,64 @@ int orcSetPatternWeights(int instance, const double* in) {
     return ORC_SUCCESS;
 }
 
-/* Contiguous partitions only: boundaries of runs of equal partition index
- * (BeagleCPUImpl.hpp:803-920 without the non-contiguous reorder). */
+/* Partition boundaries (BeagleCPUImpl.hpp:803-920).  When the partition
+ * indices are not already 0, 1, 2, ... in runs, the patterns are reordered
+ * so that each partition is contiguous, keeping their order within a
+ * partition (BeagleCPUImpl.hpp:4140-4219): pattern i moves to
+ * start[part[i]] + (its rank in its partition); the tips' states or
+ * partials and the pattern weights are permuted, other buffers are not, and
+ * site log-likelihoods are returned in the original order
+ * (BeagleCPUImpl.hpp:1009-1021).  A second reorder is refused. */
 int orcSetPatternPartitions(int instance, int partitionCount, const int* parts) {
     Orc* o = get(instance);
     if (!o) return ORC_ERROR_UNINITIALIZED_INSTANCE;
-    int run = 0;
-    for (int k = 1; k < o->P; k++) {
-        if (parts[k] != parts[k - 1]) run++;
-        if (parts[k] != run) return ORC_ERROR_NO_IMPLEMENTATION;
+    if (partitionCount < 1) return ORC_ERROR_OUT_OF_RANGE;
+    for (int k = 0; k < o->P; k++)
+        if (parts[k] < 0 || parts[k] >= partitionCount) return ORC_ERROR_OUT_OF_RANGE;
+    int reorder = 0, run = 0;
+    for (int k = 0; k < o->P; k++) {
+        if (k > 0 && parts[k] != parts[k - 1]) run++;
+        if (parts[k] != run) { reorder = 1; break; }
     }
-    if (parts[0] != 0) return ORC_ERROR_NO_IMPLEMENTATION;
     free(o->partStart);
     o->partitionCount = partitionCount;
     o->partStart = (int*)xcalloc(partitionCount + 1, sizeof(int));
-    for (int k = 1; k < o->P; k++)
-        if (parts[k] != parts[k - 1]) o->partStart[parts[k]] = k;
-    o->partStart[parts[o->P - 1] + 1] = o->P;
+    if (!reorder) {
+        for (int k = 1; k < o->P; k++)
+            if (parts[k] != parts[k - 1]) o->partStart[parts[k]] = k;
+        for (int q = parts[o->P - 1] + 1; q <= partitionCount; q++) o->partStart[q] = o->P;
+        return ORC_SUCCESS;
+    }
+    if (o->newOrder) return ORC_ERROR_NO_IMPLEMENTATION;
+    int* size = (int*)xcalloc(partitionCount, sizeof(int));
+    o->newOrder = (int*)xcalloc(o->P, sizeof(int));
+    for (int k = 0; k < o->P; k++) o->newOrder[k] = size[parts[k]]++;
+    for (int q = 0; q < partitionCount; q++) o->partStart[q + 1] = o->partStart[q] + size[q];
+    for (int k = 0; k < o->P; k++) o->newOrder[k] += o->partStart[parts[k]];
+    free(size);
+    double* w = (double*)xcalloc(o->P, sizeof(double));
+    for (int k = 0; k < o->P; k++) w[o->newOrder[k]] = o->patternWeights[k];
+    memcpy(o->patternWeights, w, o->P * sizeof(double));
+    free(w);
+    int S = o->S;
+    for (int t = 0; t < o->tipCount; t++) {
+        if (o->tipStates[t]) {
+            int* st = (int*)xcalloc(o->P, sizeof(int));
+            for (int k = 0; k < o->P; k++) st[o->newOrder[k]] = o->tipStates[t][k];
+            memcpy(o->tipStates[t], st, o->P * sizeof(int));
+            free(st);
+        } else if (o->partials[t]) {
+            size_t n = (size_t)o->C * o->P * S;
+            double* x = (double*)xcalloc(n, sizeof(double));
+            for (int l = 0; l < o->C; l++)
+                for (int k = 0; k < o->P; k++)
+                    for (int j = 0; j < S; j++)
+                        x[((size_t)l * o->P + o->newOrder[k]) * S + j] =
+                            o->partials[t][((size_t)l * o->P + k) * S + j];
+            memcpy(o->partials[t], x, n * sizeof(double));
+            free(x);
+        }
+    }
     return ORC_SUCCESS;
 }
 
@@ -859,7 +901,10 @@ int orcCalculateRootLogLikelihoodsByPartition(int instance, const int* bufIdx, c
 int orcGetSiteLogLikelihoods(int instance, double* out) {
     Orc* o = get(instance);
     if (!o) return ORC_ERROR_UNINITIALIZED_INSTANCE;
-    memcpy(out, o->siteLogL, o->P * sizeof(double));
+    if (o->newOrder)
+        for (int k = 0; k < o->P; k++) out[k] = o->siteLogL[o->newOrder[k]];
+    else
+        memcpy(out, o->siteLogL, o->P * sizeof(double));
     return ORC_SUCCESS;
 }
 int orcGetLogLikelihood(int instance, double* out) {

@@ -219,6 +219,32 @@ def test_mfma_post_order_single_matches_oracle(amd, oracle, S, n, P, shape, tips
         close_rows(rg[keep], ro[keep], S, tr)
 
 
+@pytest.mark.parametrize("S,tips_as,precision", [(4, "states", "double"),
+                                                  (4, "partials", "double"),
+                                                  (20, "states", "double"),
+                                                  (61, "partials", "double"),
+                                                  (4, "states", "single"),
+                                                  (20, "partials", "single")])
+def test_partition_reorder_matches_oracle(amd, oracle, S, tips_as, precision):
+    """Non-contiguous partitions reorder the patterns (tips and weights
+    permuted on the device, site log-likelihoods returned in the client's
+    order); the oracle's reorder is pinned to the reference CPU library in
+    test_oracle.py."""
+    from test_oracle import reordered_partition_run
+    prob = synth.make_problem(61 + S, 7, 150 if S > 4 else 700, S=S, missing=0.05)
+    parts = np.random.default_rng(3).integers(0, 3, prob.P).astype(np.int32)
+    g = reordered_partition_run(amd, prob, parts, tips_as, precision=precision,
+                                matrices="explicit")
+    o = reordered_partition_run(oracle, prob, parts, tips_as, matrices="explicit")
+    assert g["second_reorder_rc"][0] == api.ERROR_NO_IMPLEMENTATION
+    tol = 1e-12 if precision == "double" else 1e-6
+    close(g["by_partition"], o["by_partition"], tol, 0)
+    close(g["total"], o["total"], tol, 0)
+    close(g["sites"], o["sites"], tol, 0)
+    if precision == "double":
+        close_rows(g["root"], o["root"], S, 1e-12)
+
+
 def test_split_bf16_engine_selection(amd):
     """beagleExtSetSplitBF16: FP32 instances with 5..64 states only, terms 0,
     3 or 4."""

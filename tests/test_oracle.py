@@ -20,6 +20,7 @@ mistake in the oracle (a dropped term, a wrong sign, index or transpose):
     fixtures do not hold (a non-generator derivative matrix, whose tip
     denominator follows BeagleCPU4StateImpl.hpp:1032).
 """
+import ctypes
 import itertools
 import json
 import os
@@ -318,6 +319,49 @@ def test_remaining_calls_against_reference(oracle, ref):
     b = _api_sequence(ref, prob)
     for k in a:
         scale = np.abs(b[k]).max()
+        np.testing.assert_allclose(a[k], b[k], rtol=1e-12, atol=1e-13 * scale, err_msg=k)
+
+
+def reordered_partition_run(lib, prob, parts, tips_as, **kw):
+    """Non-contiguous pattern partitions (BeagleCPUImpl.hpp:803-920,
+    4140-4219): set after the tip data, so the library reorders the
+    patterns; then a partitioned post-order and per-partition root
+    log-likelihoods, the site log-likelihoods (original order,
+    :1009-1021), and a second reorder, which is refused."""
+    ev = driver.Evaluator(lib, prob, tips_as=tips_as, transpose="none", preorder=False, **kw)
+    inst, tree = ev.inst, prob.tree
+    out = {}
+    try:
+        nparts = int(parts.max()) + 1
+        ev.update_matrices()
+        inst.set_pattern_partitions(nparts, parts)
+        ops = synth.postorder_ops(tree)
+        pops = [list(o) + [q, -1] for q in range(nparts) for o in ops]
+        inst.update_partials_by_partition(np.array(pops, np.int32))
+        r = tree.root
+        by, tot = inst.root_log_likelihood_by_partition([r] * nparts, [0] * nparts, [0] * nparts,
+                                                        [-1] * nparts, list(range(nparts)))
+        out["by_partition"] = np.asarray(by)
+        out["total"] = np.array([tot])
+        out["sites"] = np.asarray(inst.site_log_likelihoods())
+        out["root"] = np.asarray(inst.get_partials(r))
+        out["second_reorder_rc"] = np.array([lib.call("SetPatternPartitions", inst.h, nparts,
+                                                      np.ascontiguousarray(parts[::-1]).ctypes
+                                                      .data_as(ctypes.POINTER(ctypes.c_int)))])
+    finally:
+        ev.close()
+    return out
+
+
+@pytest.mark.parametrize("S,tips_as", [(4, "states"), (4, "partials"), (20, "states")])
+def test_partition_reorder_against_reference(oracle, ref, S, tips_as):
+    prob = synth.make_problem(61 + S, 7, 150, S=S, missing=0.05)
+    parts = np.random.default_rng(3).integers(0, 3, prob.P).astype(np.int32)
+    a = reordered_partition_run(oracle, prob, parts, tips_as)
+    b = reordered_partition_run(ref, prob, parts, tips_as)
+    assert b["second_reorder_rc"][0] == api.ERROR_NO_IMPLEMENTATION
+    for k in a:
+        scale = max(np.abs(b[k]).max(), 1e-300)
         np.testing.assert_allclose(a[k], b[k], rtol=1e-12, atol=1e-13 * scale, err_msg=k)
 
 
