@@ -143,6 +143,12 @@ SIGNATURES = {
     # extensions (include/beagle_amd_ext.h)
     "ExtGetStream": (_I, [_I, ctypes.POINTER(ctypes.c_void_p)]),
     "ExtSetSplitBF16": (_I, [_I, _I]),
+    "ExtBeginCapture": (_I, [_I]),
+    "ExtEndCapture": (_I, [_I, ctypes.POINTER(ctypes.c_int)]),
+    "ExtLaunchGraph": (_I, [_I, _I]),
+    "ExtWaitGraph": (_I, [_I, _I]),
+    "ExtSetGraphBranchLengths": (_I, [_I, _I, _DP, _I]),
+    "ExtDestroyGraph": (_I, [_I, _I]),
     "ExtUpdatePrePartialsWithDerivatives": (_I, [_I, _IP, _I, _IP, _IP, _IP, _I, _DP, _DP, _DP]),
     "ExtCalculateLogLikelihoodAndDerivatives": (_I, [_I, _I, _I, _I, _I, _I, _IP, _I, _IP, _IP,
                                                      _IP, _DP, _DP, _DP, _DP]),
@@ -377,12 +383,14 @@ class Instance:
         return out
 
     def root_log_likelihood(self, buf, weights_index=0, freqs_index=0, cumulative=OP_NONE,
-                            allow=()):
-        out = ctypes.c_double(0.0)
+                            allow=(), out=None):
+        """out: a caller-owned float64 array of one element (needed while
+        capturing a graph, whose replays write to it)."""
+        res = np.zeros(1) if out is None else out
         a, b, c, d = _ip([buf]), _ip([weights_index]), _ip([freqs_index]), _ip([cumulative])
         rc = self._c("CalculateRootLogLikelihoods", a[1], b[1], c[1], d[1], 1,
-                     ctypes.byref(out), allow=allow)
-        return out.value if rc >= 0 else rc
+                     res.ctypes.data_as(c_double_p), allow=allow)
+        return float(res[0]) if rc >= 0 else rc
 
     def root_log_likelihood_by_partition(self, bufs, weights, freqs, cumulative, parts):
         n = len(parts)
@@ -457,20 +465,23 @@ class Instance:
         return s1, sq, s2
 
     def log_likelihood_and_derivatives(self, root, root_pre, ops, post, d1, d2,
-                                       weights_index=0, freqs_index=0, cumulative=OP_NONE):
+                                       weights_index=0, freqs_index=0, cumulative=OP_NONE,
+                                       out=None):
         """Extension: root logL, root pre-order partials = frequencies, and
         the fused pre-order + derivative pass with one wait.  Returns
-        (logL, sum w L'/L, sum w (L'/L)^2, sum w L''/L)."""
+        (logL, sum w L'/L, sum w (L'/L)^2, sum w L''/L).  out: caller-owned
+        float64 arrays (logL[1], s1[n], sq[n], s2[n]), needed while
+        capturing a graph."""
         a = _ip(np.asarray(ops).reshape(-1))
         n = len(a[0]) // 7
         b, c, d = _ip(post), _ip(d1), _ip(d2)
-        lnl = ctypes.c_double(0.0)
-        s1, sq, s2 = np.zeros(n), np.zeros(n), np.zeros(n)
+        lnl, s1, sq, s2 = out if out is not None else (np.zeros(1), np.zeros(n), np.zeros(n),
+                                                       np.zeros(n))
         self._c("ExtCalculateLogLikelihoodAndDerivatives", root, weights_index, freqs_index,
-                cumulative, root_pre, a[1], n, b[1], c[1], d[1], ctypes.byref(lnl),
+                cumulative, root_pre, a[1], n, b[1], c[1], d[1], lnl.ctypes.data_as(c_double_p),
                 s1.ctypes.data_as(c_double_p), sq.ctypes.data_as(c_double_p),
                 s2.ctypes.data_as(c_double_p))
-        return lnl.value, s1, sq, s2
+        return float(lnl[0]), s1, sq, s2
 
     def rate_matrix_gradient(self, eigen, ops, post, weights_index=0, root=None,
                              freqs_index=0):
@@ -484,6 +495,30 @@ class Instance:
                 -1 if root is None else root, freqs_index, gq.ctypes.data_as(c_double_p),
                 gf.ctypes.data_as(c_double_p) if gf is not None else None)
         return gq.reshape(self.S, self.S), gf
+
+    # HIP-graph capture (extension entry points): calls between
+    # begin_capture() and end_capture() are recorded; their output arrays
+    # (which must stay alive) receive each replay's results at wait_graph().
+    def begin_capture(self):
+        self._c("ExtBeginCapture")
+
+    def end_capture(self):
+        g = ctypes.c_int(-1)
+        self._c("ExtEndCapture", ctypes.byref(g))
+        return g.value
+
+    def launch_graph(self, graph):
+        self._c("ExtLaunchGraph", int(graph))
+
+    def wait_graph(self, graph):
+        self._c("ExtWaitGraph", int(graph))
+
+    def set_graph_branch_lengths(self, graph, lengths):
+        a = _dp(lengths)
+        self._c("ExtSetGraphBranchLengths", int(graph), a[1], len(a[0]))
+
+    def destroy_graph(self, graph):
+        self._c("ExtDestroyGraph", int(graph))
 
     def set_split_bf16(self, terms=3):
         """S > 4, FP32: post-order products on split-BF16 matrix cores

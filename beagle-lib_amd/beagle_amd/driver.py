@@ -201,6 +201,51 @@ class Evaluator:
         nodes = self.branch_nodes()
         return logl, g1[nodes], g2[nodes]
 
+    def capture_fused_gradient(self, keep_tips=False):
+        """Record one fused_gradient() evaluation (matrices from the
+        eigen-system, post-order, root logL, fused derivative pass) as a HIP
+        graph of this package's library; replay_fused_gradient(lengths)
+        reruns it with new branch lengths.  The calls run once first so
+        that the library's work buffers have their size."""
+        if self.matrices != "eigen":
+            raise ValueError("graphs record matrices computed from the eigen-system")
+        self.fused_gradient(keep_tips=keep_tips)
+        ops, post, d1, d2 = self.fused_operations(keep_tips)
+        n = len(post)
+        self._gout = (np.zeros(1), np.zeros(n), np.zeros(n), np.zeros(n))
+        self._gpost = post
+        manual = self.scaling in ("manual", "manual_log")
+        cum = self.n - 1 if manual else api.OP_NONE
+        self.inst.begin_capture()
+        try:
+            self.update_matrices()
+            self.post_order()
+            self.inst.log_likelihood_and_derivatives(
+                self.prob.tree.root, self.pre_base + self.prob.tree.root, ops, post, d1, d2,
+                cumulative=cum, out=self._gout)
+        finally:
+            self._graph = self.inst.end_capture()
+        return self._graph
+
+    def replay_fused_gradient(self, lengths=None, wait=True):
+        """One replay of the captured evaluation; returns what
+        fused_gradient() returns (wait=False: launch only, collect with
+        collect_fused_gradient())."""
+        if lengths is not None:
+            self.inst.set_graph_branch_lengths(self._graph, np.asarray(lengths)[:2 * self.n - 2])
+        self.inst.launch_graph(self._graph)
+        return self.collect_fused_gradient() if wait else None
+
+    def collect_fused_gradient(self):
+        self.inst.wait_graph(self._graph)
+        lnl, s1, sq, s2 = self._gout
+        g1 = np.zeros(2 * self.n - 1)
+        g2 = np.zeros(2 * self.n - 1)
+        g1[self._gpost] = s1
+        g2[self._gpost] = s2 - sq
+        nodes = self.branch_nodes()
+        return float(lnl[0]), g1[nodes], g2[nodes]
+
     def rate_matrix_gradient(self, root_frequencies=True):
         """dlogL/dQ (S x S) for eigen-system 0 and, optionally, dlogL/dpi
         through the root distribution, from the current post- and

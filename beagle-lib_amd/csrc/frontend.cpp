@@ -641,5 +641,33 @@ int beagleExtSetSplitBF16(int instance, int terms) {
     return withInstance(instance, [&](Instance* in) { return in->setSplitBf16(terms); });
 }
 
+int beagleExtBeginCapture(int instance) {
+    return withInstance(instance, [&](Instance* in) { return in->beginCapture(); });
+}
+
+int beagleExtEndCapture(int instance, int* outGraph) {
+    if (!outGraph) return BEAGLE_ERROR_OUT_OF_RANGE;
+    return withInstance(instance, [&](Instance* in) { return in->endCapture(outGraph); });
+}
+
+int beagleExtLaunchGraph(int instance, int graph) {
+    return withInstance(instance, [&](Instance* in) { return in->launchGraph(graph); });
+}
+
+int beagleExtWaitGraph(int instance, int graph) {
+    return withInstance(instance, [&](Instance* in) { return in->waitGraph(graph); });
+}
+
+int beagleExtSetGraphBranchLengths(int instance, int graph, const double* lengths, int count) {
+    if (!lengths && count) return BEAGLE_ERROR_OUT_OF_RANGE;
+    return withInstance(instance, [&](Instance* in) {
+        return in->setGraphBranchLengths(graph, lengths, count);
+    });
+}
+
+int beagleExtDestroyGraph(int instance, int graph) {
+    return withInstance(instance, [&](Instance* in) { return in->destroyGraph(graph); });
+}
+
 }  // extern "C"
 #pragma GCC visibility pop

@@ -207,7 +207,22 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
 
 Instance::~Instance() {
     hipSetDevice(device_);
+    if (capturing_) {
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(stream_, &g) == hipSuccess && g) hipGraphDestroy(g);
+        (void)hipGetLastError();
+        hipHostFree(cap_.host);
+        hipFree(cap_.dev);
+        capturing_ = false;
+    }
     if (stream_) hipStreamSynchronize(stream_);
+    for (Graph& G : graphs_)
+        if (G.live) {
+            hipGraphExecDestroy(G.exec);
+            hipHostFree(G.host);
+            hipFree(G.dev);
+        }
+    for (void* p : retired_) hipFree(p);
     for (void* p : dPartials_) hipFree(p);
     for (short* p : dExp_) hipFree(p);
     for (int* p : dStates_) hipFree(p);
@@ -245,14 +260,23 @@ const char* Instance::implName() const {
 
 // ---- transfers --------------------------------------------------------------
 
-void Instance::sync() { HIPCHECK(hipStreamSynchronize(stream_)); }
+void Instance::sync() {
+    if (capturing_) captureBreak("a call that waits for the device cannot be captured");
+    HIPCHECK(hipStreamSynchronize(stream_));
+}
+
+void Instance::finishCall() {
+    if (!capturing_ && !(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
+}
 
 void Instance::copyIn(void* dst, const void* src, size_t bytes) {
+    if (capturing_) captureBreak("a host copy cannot be captured");
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
     sync();
 }
 
 void Instance::copyOut(void* dst, const void* src, size_t bytes) {
+    if (capturing_) captureBreak("a host copy cannot be captured");
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
     sync();
 }
@@ -280,6 +304,15 @@ void Instance::toHost(double* dst, const void* src, size_t n) {
 // slots so the host never waits on the device unless a slot is reused
 // before its previous copy finished.
 void* Instance::upload(const void* data, size_t bytes) {
+    if (capturing_) {
+        // the graph's own staging: replays copy from it again
+        const size_t off = captureAlloc(bytes);
+        std::memcpy(cap_.host + off, data, bytes);
+        HIPCHECK(hipMemcpyAsync(cap_.dev + off, cap_.host + off, bytes, hipMemcpyHostToDevice,
+                                stream_));
+        lastUploadOff_ = off;
+        return cap_.dev + off;
+    }
     Slot& s = ring_[ringPos_];
     ringPos_ = (ringPos_ + 1) % (int)ring_.size();
     if (s.used) HIPCHECK(hipEventSynchronize(s.ev));
@@ -304,8 +337,13 @@ void* Instance::upload(const void* data, size_t bytes) {
 
 void* Instance::grow(void*& buf, size_t& cap, size_t bytes) {
     if (cap < bytes) {
+        if (capturing_) captureBreak("a work buffer must grow: run the calls once before capturing");
         sync();
-        if (buf) HIPCHECK(hipFree(buf));
+        // a live graph may hold the old buffer's address: keep it until the end
+        bool held = false;
+        for (const Graph& G : graphs_) held |= G.live;
+        if (buf && held) retired_.push_back(buf);
+        else if (buf) HIPCHECK(hipFree(buf));
         size_t c = std::max(bytes, cap * 2);
         HIPCHECK(hipMalloc(&buf, c));
         cap = c;
@@ -327,8 +365,9 @@ void Instance::ensurePartials(int b) {
     }
 }
 
+// a fill kernel, not a memset: the call may be captured into a graph
 void Instance::clearExponent(int b) {
-    if (auto_ && dExp_[b]) HIPCHECK(hipMemsetAsync(dExp_[b], 0, dims_.Ppad * sizeof(short), stream_));
+    if (auto_ && dExp_[b]) launch_fill_short(dExp_[b], 0, dims_.Ppad, stream_);
 }
 
 // ---- data in and out ---------------------------------------------------------
@@ -549,6 +588,7 @@ int Instance::matrixUpdate(const std::vector<MatrixJob>& jobs) {
     if (jobs.empty()) return BEAGLE_SUCCESS;
     const MatrixJob* d = reinterpret_cast<const MatrixJob*>(
         upload(jobs.data(), jobs.size() * sizeof(MatrixJob)));
+    if (capturing_) cap_.matJobs.push_back({lastUploadOff_, (int)jobs.size()});
     if (dp_) launch_matrices<double>(d, (int)jobs.size(), dims_.S, dims_.C, stream_);
     else launch_matrices<float>(d, (int)jobs.size(), dims_.S, dims_.C, stream_);
     HIPCHECK(hipGetLastError());
@@ -727,7 +767,7 @@ int Instance::convolveTransitionMatrices(const int* a, const int* b, const int* 
     if (dp_) launch_convolve<double>(d, count, dims_.S, dims_.C, stream_);
     else launch_convolve<float>(d, count, dims_.S, dims_.C, stream_);
     HIPCHECK(hipGetLastError());
-    if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
+    finishCall();
     return BEAGLE_SUCCESS;
 }
 
@@ -828,7 +868,7 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
         launchPostPlan(plan);
     }
     HIPCHECK(hipGetLastError());
-    if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
+    finishCall();
     return BEAGLE_SUCCESS;
 }
 
@@ -1104,7 +1144,7 @@ int Instance::updatePrePartials(const int* ops, int count, int cumulativeIndex, 
     if (rc != BEAGLE_SUCCESS || count == 0) return rc;
     launchPre(ops, byPartition ? BEAGLE_PARTITION_OP_COUNT : BEAGLE_OP_COUNT, count, nullptr);
     HIPCHECK(hipGetLastError());
-    if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
+    finishCall();
     return BEAGLE_SUCCESS;
 }
 
@@ -1138,7 +1178,14 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
     if (!want) {
         // nothing to return: no reduction, no copy; an asynchronous
         // instance returns without waiting
-        if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) sync();
+        finishCall();
+        return BEAGLE_SUCCESS;
+    }
+    if (capturing_) {
+        const size_t off = captureResult(sums, 3 * (size_t)count);
+        double* dst[3] = {outD1, outD1Sq, outD2};
+        for (int k = 0; k < 3; ++k)
+            if (dst[k]) cap_.outs.push_back({dst[k], off + k * sizeof(double), count, 3, false});
         return BEAGLE_SUCCESS;
     }
     const double* h = stageOut(sums, 3 * (size_t)count, 0);
@@ -1157,6 +1204,7 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
 // caller waits before reading).
 void Instance::reserveStage(size_t n) {
     if (stageCap_ >= n) return;
+    if (capturing_) captureBreak("results cannot be staged while capturing");
     sync();
     if (hStage_) HIPCHECK(hipHostFree(hStage_));
     stageCap_ = std::max<size_t>(2 * n, 1024);
@@ -1214,6 +1262,14 @@ int Instance::logLikelihoodAndDerivatives(int rootIdx, int wIdx, int fIdx, int c
     const double* sums = nullptr;
     rc = enqueuePreDerivatives(ops, count, postIdx, d1Idx, d2Idx, wIdx, true, &sums);
     if (rc != BEAGLE_SUCCESS) return rc;
+    if (capturing_) {
+        cap_.outs.push_back({outLogL, captureResult(dResult_, 1), 1, 1, true});
+        const size_t off = captureResult(sums, 3 * (size_t)count);
+        double* dst[3] = {outD1, outD1Sq, outD2};
+        for (int k = 0; k < 3; ++k)
+            if (dst[k]) cap_.outs.push_back({dst[k], off + k * sizeof(double), count, 3, false});
+        return BEAGLE_SUCCESS;
+    }
     reserveStage(1 + 3 * (size_t)count);
     const double* hl = stageOut(dResult_, 1, 0);
     const double* h = stageOut(sums, 3 * (size_t)count, 1);
@@ -1525,6 +1581,10 @@ int Instance::calculateRootLogLikelihoods(const int* bufIdx, const int* wIdx, co
     if (count != 1) return BEAGLE_ERROR_NO_IMPLEMENTATION;
     const int rc = enqueueRoot(bufIdx[0], wIdx[0], fIdx[0], cumIdx[0]);
     if (rc != BEAGLE_SUCCESS) return rc;
+    if (capturing_) {
+        cap_.outs.push_back({out, captureResult(dResult_, 1), 1, 1, true});
+        return BEAGLE_SUCCESS;
+    }
     HIPCHECK(hipMemcpyAsync(hResult_, dResult_, sizeof(double), hipMemcpyDeviceToHost, stream_));
     sync();
     lastLogL_ = hResult_[0];
@@ -1771,6 +1831,134 @@ int Instance::getSiteLogLikelihoods(double* out) {
 int Instance::getSiteDerivatives(double* d1, double* d2) {
     copyOut(d1, dSiteD1_, dims_.P * sizeof(double));
     if (d2) copyOut(d2, dSiteD2_, dims_.P * sizeof(double));
+    return BEAGLE_SUCCESS;
+}
+
+// ---- HIP-graph capture ---------------------------------------------------------
+
+void Instance::captureBreak(const char* why) {
+    if (captureError_.empty()) captureError_ = why;
+    throw std::runtime_error(why);
+}
+
+size_t Instance::captureAlloc(size_t bytes) {
+    const size_t off = (cap_.used + 255) / 256 * 256;
+    if (off + bytes > kCaptureArena) captureBreak("capture arena full");
+    cap_.used = off + bytes;
+    return off;
+}
+
+size_t Instance::captureResult(const double* dsrc, size_t n) {
+    const size_t off = captureAlloc(n * sizeof(double));
+    HIPCHECK(hipMemcpyAsync(cap_.host + off, dsrc, n * sizeof(double), hipMemcpyDeviceToHost,
+                            stream_));
+    return off;
+}
+
+// Calls between beginCapture and endCapture are recorded, not run (their
+// outputs are written at each replay's wait); calls that must wait for the
+// device (getters, copies, the generic-state derivative path) fail and
+// spoil the capture.  Work buffers must already have their size, i.e. the
+// same calls run once before capturing.
+int Instance::beginCapture() {
+    if (capturing_) return BEAGLE_ERROR_GENERAL;
+    sync();
+    cap_ = Graph{};
+    HIPCHECK(hipHostMalloc((void**)&cap_.host, kCaptureArena, hipHostMallocDefault));
+    HIPCHECK(hipMalloc((void**)&cap_.dev, kCaptureArena));
+    captureError_.clear();
+    HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+    capturing_ = true;
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::endCapture(int* outGraph) {
+    if (!capturing_) return BEAGLE_ERROR_GENERAL;
+    capturing_ = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream_, &g);
+    if (e != hipSuccess || !captureError_.empty() || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        (void)hipHostFree(cap_.host);
+        (void)hipFree(cap_.dev);
+        cap_ = Graph{};
+        return BEAGLE_ERROR_GENERAL;
+    }
+    const hipError_t ie = hipGraphInstantiate(&cap_.exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipHostFree(cap_.host);
+        (void)hipFree(cap_.dev);
+        cap_ = Graph{};
+        return BEAGLE_ERROR_GENERAL;
+    }
+    cap_.live = true;
+    graphs_.push_back(cap_);
+    cap_ = Graph{};
+    *outGraph = (int)graphs_.size() - 1;
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::launchGraph(int graph) {
+    if (capturing_) return BEAGLE_ERROR_GENERAL;
+    if (graph < 0 || graph >= (int)graphs_.size() || !graphs_[graph].live)
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    HIPCHECK(hipGraphLaunch(graphs_[graph].exec, stream_));
+    if (!(flags_ & BEAGLE_FLAG_COMPUTATION_ASYNCH)) return waitGraph(graph);
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::waitGraph(int graph) {
+    if (graph < 0 || graph >= (int)graphs_.size() || !graphs_[graph].live)
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    sync();
+    const Graph& G = graphs_[graph];
+    bool nan = false;
+    for (const CapturedOut& o : G.outs) {
+        const double* v = reinterpret_cast<const double*>(G.host + o.src);
+        for (int k = 0; k < o.n; ++k) o.dst[k] = v[(size_t)k * o.stride];
+        if (o.logL) {
+            lastLogL_ = v[0];
+            nan |= v[0] != v[0];
+        }
+    }
+    return nan ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+}
+
+// New branch lengths for the captured transition-matrix updates, in the
+// order their matrix jobs were captured (one per matrix index and call).
+int Instance::setGraphBranchLengths(int graph, const double* lengths, int count) {
+    if (graph < 0 || graph >= (int)graphs_.size() || !graphs_[graph].live)
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    Graph& G = graphs_[graph];
+    int total = 0;
+    for (const auto& m : G.matJobs) total += m.second;
+    if (count != total) return BEAGLE_ERROR_OUT_OF_RANGE;
+    sync();   // a replay in flight still reads the staged jobs
+    int k = 0;
+    for (const auto& m : G.matJobs) {
+        MatrixJob* J = reinterpret_cast<MatrixJob*>(G.host + m.first);
+        for (int j = 0; j < m.second; ++j, ++k) {
+            J[j].t = lengths[k];
+            const long idx = (static_cast<char*>(J[j].P) - static_cast<char*>(dMatrices_)) /
+                             (long)matrixBytes_;
+            if (idx >= 0 && idx < matrixCount_ && J[j].cat < 0) matOrigin_[idx].t = lengths[k];
+        }
+    }
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::destroyGraph(int graph) {
+    if (graph < 0 || graph >= (int)graphs_.size() || !graphs_[graph].live)
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    sync();
+    Graph& G = graphs_[graph];
+    (void)hipGraphExecDestroy(G.exec);
+    (void)hipHostFree(G.host);
+    (void)hipFree(G.dev);
+    G = Graph{};
     return BEAGLE_SUCCESS;
 }
 

@@ -127,6 +127,14 @@ public:
 
     // Device pointers for the extension entry points (Q-parameter
     // gradients, fused evaluations).
+    // HIP-graph capture of this instance's calls (beagle_amd_ext.h)
+    int beginCapture();
+    int endCapture(int* outGraph);
+    int launchGraph(int graph);
+    int waitGraph(int graph);
+    int setGraphBranchLengths(int graph, const double* lengths, int count);
+    int destroyGraph(int graph);
+
     const Dims& dims() const { return dims_; }
     hipStream_t stream() const { return stream_; }
     bool doublePrecision() const { return dp_; }
@@ -141,6 +149,37 @@ private:
     };
 
     void* upload(const void* data, size_t bytes);
+    // SYNCH instances wait at the end of an update call (not while capturing)
+    void finishCall();
+
+    // A captured graph: its uploads and results live in its own pinned host
+    // and device arena (memcpy nodes read and write them at every replay);
+    // `outs` are copied to the client's arrays after each replay.
+    struct CapturedOut {
+        double* dst;
+        size_t src;      // byte offset of the first value in the host arena
+        int n, stride;   // dst[k] = value k * stride
+        bool logL;       // a log-likelihood: NaN gives BEAGLE_ERROR_FLOATING_POINT
+    };
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        char* host = nullptr;
+        char* dev = nullptr;
+        size_t used = 0;
+        std::vector<CapturedOut> outs;
+        std::vector<std::pair<size_t, int>> matJobs;   // (host offset, count) of matrix jobs
+        bool live = false;
+    };
+    static constexpr size_t kCaptureArena = size_t(4) << 20;
+    size_t captureAlloc(size_t bytes);
+    size_t captureResult(const double* dsrc, size_t n);
+    void captureBreak(const char* why);
+    std::vector<Graph> graphs_;
+    std::vector<void*> retired_;      // grown-out buffers a live graph may still use
+    Graph cap_;
+    bool capturing_ = false;
+    std::string captureError_;
+    size_t lastUploadOff_ = 0;        // arena offset of the last upload while capturing
     void* grow(void*& buf, size_t& cap, size_t bytes);
     void ensurePartials(int b);
     void clearExponent(int b);

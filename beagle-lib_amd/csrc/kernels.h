@@ -273,6 +273,7 @@ int cross_products_rows(const Dims& d);
 void launch_accumulate(double* cum, const double* const* bufs, int nbufs, double sign,
                        int logScalers, int p0, int p1, hipStream_t st);
 void launch_fill(double* dst, double v, long n, hipStream_t st);
+void launch_fill_short(short* dst, short v, long n, hipStream_t st);
 template <typename R>
 void launch_transpose(const void* const* src, void* const* dst, int count, int S, int C,
                       hipStream_t st);

@@ -2179,6 +2179,17 @@ __global__ __launch_bounds__(WG) void k_fill(double* dst, double v, long n) {
     if (i < n) dst[i] = v;
 }
 
+__global__ __launch_bounds__(WG) void k_fill_short(short* dst, short v, long n) {
+    const long i = (long)blockIdx.x * WG + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+void launch_fill_short(short* dst, short v, long n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_short, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, st, dst, v,
+                       n);
+}
+
 void launch_fill(double* dst, double v, long n, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill, dim3((n + WG - 1) / WG), dim3(WG), 0, st, dst, v, n);

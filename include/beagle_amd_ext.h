@@ -27,6 +27,35 @@ int beagleExtGetStream(int instance, void** outStream);
  * for other values of terms.  Applies to operations enqueued afterwards. */
 int beagleExtSetSplitBF16(int instance, int terms);
 
+/* HIP-graph capture of an instance's calls (no counterpart in the
+ * reference).  Between beagleExtBeginCapture and beagleExtEndCapture the
+ * instance's calls are recorded into a graph instead of run:
+ * transition-matrix updates, post- and pre-order traversals, root
+ * log-likelihoods, beagleExtUpdatePrePartialsWithDerivatives and
+ * beagleExtCalculateLogLikelihoodAndDerivatives (4 states).  A call that
+ * must wait for the device (getters, set* copies, the derivative path for
+ * other state counts) fails and spoils the capture: EndCapture then returns
+ * BEAGLE_ERROR_GENERAL and records nothing.  Work buffers must already have
+ * their size, so run the same calls once before capturing.  The output
+ * arrays passed to captured calls receive each replay's results when
+ * beagleExtWaitGraph returns; they must stay valid while the graph lives.
+ * Uploads are staged in pinned memory owned by the graph; no memset node is
+ * recorded. */
+int beagleExtBeginCapture(int instance);
+int beagleExtEndCapture(int instance, int* outGraph);
+/* Enqueues one replay on the instance's stream; an instance without
+ * BEAGLE_FLAG_COMPUTATION_ASYNCH also waits, as beagleExtWaitGraph. */
+int beagleExtLaunchGraph(int instance, int graph);
+/* Waits for the instance's stream and writes the last replay's results to
+ * the captured output arrays; BEAGLE_ERROR_FLOATING_POINT on a NaN
+ * log-likelihood. */
+int beagleExtWaitGraph(int instance, int graph);
+/* New branch lengths for the graph's transition-matrix updates, in the
+ * order the matrix indices were captured (count = their total number);
+ * waits for a replay in flight first. */
+int beagleExtSetGraphBranchLengths(int instance, int graph, const double* lengths, int count);
+int beagleExtDestroyGraph(int instance, int graph);
+
 /* Pre-order operations fused with the derivative sums of each operation's
  * branch.  Equivalent to
  *   beagleUpdatePrePartials(instance, operations, operationCount, OP_NONE);

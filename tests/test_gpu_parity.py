@@ -245,6 +245,65 @@ def test_partition_reorder_matches_oracle(amd, oracle, S, tips_as, precision):
         close_rows(g["root"], o["root"], S, 1e-12)
 
 
+@pytest.mark.parametrize("precision,scaling,shape,asynch", [("single", "auto", "caterpillar", True),
+                                                            ("double", "none", "yule", False),
+                                                            ("double", "manual", "balanced", True)])
+def test_graph_replays_match_eager(amd, precision, scaling, shape, asynch):
+    """HIP-graph capture of a fused logL + gradient evaluation: replays with
+    new branch lengths, with eager evaluations and waits between them,
+    reproduce the eager results exactly, and a second replay of the same
+    lengths gives the same values again."""
+    prob = synth.make_problem(91, 24, 3000, S=4, shape=shape, missing=0.03)
+    rng = np.random.default_rng(2)
+    lens = [prob.tree.lengths * rng.uniform(0.5, 1.5, prob.tree.lengths.shape) for _ in range(4)]
+    ev = driver.Evaluator(amd, prob, precision=precision, scaling=scaling,
+                          extra_flags=api.FLAG_COMPUTATION_ASYNCH if asynch else 0)
+    ref = [ev.fused_gradient(x) for x in lens]
+    ev.capture_fused_gradient()
+    for k, x in enumerate(lens):
+        got = ev.replay_fused_gradient(x, wait=not asynch)
+        if asynch:
+            got = ev.collect_fused_gradient()
+        eager = ev.fused_gradient(lens[(k + 1) % len(lens)])     # eager work in between
+        for a, b in zip(got, ref[k]):
+            np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+        for a, b in zip(eager, ref[(k + 1) % len(lens)]):
+            np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+    again = ev.replay_fused_gradient(lens[-1])
+    for a, b in zip(again, ref[-1]):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+    ev.close()
+
+
+def test_graph_codon_root_and_spoiled_capture(amd):
+    """A 61-state FP32 (MFMA) matrices + post-order + root logL graph
+    replays to the eager value; a capture that contains a waiting call
+    (a getter) is refused and leaves the instance usable."""
+    prob = synth.make_problem(93, 12, 500, S=61)
+    ev = driver.Evaluator(amd, prob, precision="single", scaling="auto", preorder=False)
+    want = ev.log_likelihood()
+    out = np.zeros(1)
+    ev.inst.begin_capture()
+    ev.update_matrices()
+    ev.post_order()
+    ev.inst.root_log_likelihood(prob.tree.root, 0, 0, api.OP_NONE, out=out)
+    g = ev.inst.end_capture()
+    for _ in range(3):
+        out[0] = 0.0
+        ev.inst.launch_graph(g)
+        ev.inst.wait_graph(g)
+        assert out[0] == want
+        assert ev.log_likelihood() == want
+    ev.inst.begin_capture()
+    with pytest.raises(api.BeagleError):
+        ev.inst.site_log_likelihoods()
+    with pytest.raises(api.BeagleError):
+        ev.inst.end_capture()
+    assert ev.log_likelihood() == want
+    ev.inst.destroy_graph(g)
+    ev.close()
+
+
 def test_split_bf16_engine_selection(amd):
     """beagleExtSetSplitBF16: FP32 instances with 5..64 states only, terms 0,
     3 or 4."""
