@@ -50,7 +50,12 @@ class Evaluator:
             compact = 0
             partials = nodes + pre
         self.tips_as = tips_as
-        self.n_matrices = nodes + 4
+        # transpose="manual" (S > 4): the reference GPU convention, pre-order
+        # self and derivative matrices passed transposed; the branch
+        # matrices' transposes live at t_offset + index
+        self.manual_transpose = transpose == "manual" and prob.S > 4
+        self.t_offset = nodes + 4
+        self.n_matrices = nodes + 4 + (nodes if self.manual_transpose else 0)
         self.d1_index = nodes
         self.d2_index = nodes + 1
         pref = api.FLAG_EIGEN_REAL
@@ -98,7 +103,6 @@ class Evaluator:
         d2 = np.concatenate([(r * r * (Q @ Q)).reshape(-1) for r in p.rates])
         inst.set_differential_matrix(self.d1_index, d1)
         inst.set_differential_matrix(self.d2_index, d2)
-        self.manual_transpose = self.transpose == "manual" and p.S > 4
         if self.manual_transpose:
             inst.transpose_transition_matrices([self.d1_index, self.d2_index],
                                                [self.d1_index + 2, self.d2_index + 2])
@@ -122,7 +126,7 @@ class Evaluator:
         if self.manual_transpose:
             # pre-order "self" matrices are passed transposed (the reference
             # GPU convention for S > 4, examples/hmctest/hmctest.cpp:495)
-            raise NotImplementedError("manual transpose driver path")
+            self.inst.transpose_transition_matrices(idx, np.asarray(idx) + self.t_offset)
 
     def post_order(self):
         tree = self.prob.tree
@@ -162,9 +166,19 @@ class Evaluator:
     def pre_order(self):
         tree = self.prob.tree
         self.set_root_pre()
-        ops = synth.preorder_ops(tree, self.pre_base)
+        ops = self._transposed_self(synth.preorder_ops(tree, self.pre_base))
         self.inst.update_pre_partials(ops, api.OP_NONE)
         return ops
+
+    def _transposed_self(self, ops):
+        if self.manual_transpose:
+            ops = np.array(ops, copy=True)
+            ops[:, 4] += self.t_offset
+        return ops
+
+    def _deriv_index(self, second):
+        d = self.d2_index if second else self.d1_index
+        return d + 2 if self.manual_transpose else d
 
     def fused_operations(self, keep_tips=False):
         """Pre-order operations and matching post-order buffers for the
@@ -172,13 +186,13 @@ class Evaluator:
         destination unless keep_tips (no later operation reads them)."""
         key = bool(keep_tips)
         if self._pre is None or self._pre[0] != key:
-            ops = synth.preorder_ops(self.prob.tree, self.pre_base)
+            ops = self._transposed_self(synth.preorder_ops(self.prob.tree, self.pre_base))
             post = (ops[:, 0] - self.pre_base).astype(np.int32)
             if not keep_tips and self.prob.S == 4 and self.prob.C <= 8:
                 ops[post < self.n, 0] = api.OP_NONE
             n = len(post)
-            self._pre = (key, ops, post, np.full(n, self.d1_index, np.int32),
-                         np.full(n, self.d2_index, np.int32))
+            self._pre = (key, ops, post, np.full(n, self._deriv_index(False), np.int32),
+                         np.full(n, self._deriv_index(True), np.int32))
         return self._pre[1:]
 
     def fused_gradient(self, lengths=None, keep_tips=False):
@@ -262,7 +276,7 @@ class Evaluator:
 
     def edge_derivatives(self, second=False, per_site=True):
         nodes = self.branch_nodes()
-        d = self.d2_index if second else self.d1_index
+        d = self._deriv_index(second)
         return self.inst.edge_derivatives(nodes, [self.pre_base + v for v in nodes],
                                           [d] * len(nodes), per_site=per_site)
 

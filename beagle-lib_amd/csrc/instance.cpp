@@ -244,6 +244,7 @@ Instance::~Instance() {
     hipFree(dWork_);
     hipFree(dJobMats_);
     hipFree(dPack_);
+    hipFree(dSites_);
     if (hStage_) hipHostFree(hStage_);
     hipFree(dScratch_);
     for (auto& s : ring_) {
@@ -1058,9 +1059,104 @@ int Instance::buildPreJobs(const int* ops, int count, bool byPartition, bool all
 // Launch the pre-order jobs in hPre_ (built from `ops`): the generic-S
 // path one launch per op; for 4 states a level plan of batched and fused
 // segments (see planSegments).  partial != null adds the derivative sums.
+// S > 4 in FP32 (any engine setting): pre-order and edge derivatives on the
+// FP32 matrix cores.
+bool Instance::mfmaAdjoint() const { return !dp_ && mfma_post_supported(dims_.S, dims_.C); }
+
+// MFMA pre-order: P_self^T packed for the register-resident second product
+// (from the stored transpose when the client passes transposed self
+// matrices), P_sib as a post-order child; one launch per dependency level.
+void Instance::launchPreMfma(const int* ops, int stride, int count) {
+    const size_t per = (size_t)mfma_pack_reals(dims_.S) * dims_.C;
+    std::vector<PackJob> pj(2 * (size_t)count);
+    const int tin = preTransposed_ ? PACK_TRANSPOSED_INPUT : 0;
+    for (int o = 0; o < count; ++o) {
+        PreJob& J = hPre_[o];
+        J.pkSelf = (int)(2 * o * per);
+        J.pkSib = (int)((2 * o + 1) * per);
+        pj[2 * o] = PackJob{J.moSelf, J.pkSelf, PACK_PRE_SELF | tin};
+        pj[2 * o + 1] = PackJob{J.moSib, J.pkSib, J.ss ? PACK_ROWS : PACK_OPERAND};
+    }
+    float* pack = static_cast<float*>(
+        grow(reinterpret_cast<void*&>(dPack_), packCap_, pj.size() * per * sizeof(float)));
+    std::vector<Access> acc(count);
+    std::vector<int> level;
+    for (int o = 0; o < count; ++o) {
+        const int* q = ops + (size_t)o * stride;
+        const PreJob& J = hPre_[o];
+        const int sw = J.scaleWrite ? (int)((J.scaleWrite - dScale_) / dims_.Ppad) : -1;
+        acc[o] = Access{{q[3], q[5], -1}, {q[0], sw >= 0 ? bufferCount_ + sw : -1}};
+    }
+    const int nl = dependencyLevels(acc, bufferCount_ + scaleCount_, level);
+    std::vector<int> start(nl + 1, 0);
+    for (int o = 0; o < count; ++o) ++start[level[o] + 1];
+    for (int l = 0; l < nl; ++l) start[l + 1] += start[l];
+    std::vector<PreJob> sorted(count);
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int o = 0; o < count; ++o) sorted[fill[level[o]]++] = hPre_[o];
+    auto dpj = reinterpret_cast<const PackJob*>(upload(pj.data(), pj.size() * sizeof(PackJob)));
+    launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+    auto d = reinterpret_cast<const PreJob*>(upload(sorted.data(), count * sizeof(PreJob)));
+    for (int l = 0; l < nl; ++l)
+        launch_pre_mfma(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
+}
+
+// MFMA edge derivatives of `count` nodes with one (d2 null) or two
+// derivative matrices, enqueued: *dSums receives per node NR = 2 or 3 sums
+// (sum w r1, sum w r1^2, sum w r2), interleaved; sites (one matrix only, may
+// be null) the per-pattern ratios.
+int Instance::enqueueDerivMfma(const int* postIdx, const int* preIdx, const int* d1,
+                               const int* d2, int wIdx, int count, double* sites,
+                               double** dSums) {
+    const size_t per = (size_t)mfma_pack_reals(dims_.S) * dims_.C;
+    const int nd = d2 ? 2 : 1, NR = d2 ? 3 : 2;
+    const size_t mstride = matrixBytes_ / realSize();
+    const int tin = preTransposed_ ? PACK_TRANSPOSED_INPUT : 0;
+    std::vector<DerivJob> jobs(count);
+    std::vector<PackJob> pj((size_t)nd * count);
+    for (int n = 0; n < count; ++n) {
+        const int po = postIdx[n], pr = preIdx[n];
+        if (!validBuffer(po) || !validBuffer(pr) || !dPartials_[pr] || !validMatrix(d1[n]) ||
+            (d2 && !validMatrix(d2[n])) || (!dStates_[po] && !dPartials_[po]))
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        DerivJob& J = jobs[n];
+        J.pre = dPartials_[pr];
+        J.states = dStates_[po];
+        J.post = dStates_[po] ? nullptr : dPartials_[po];
+        J.row = n;
+        const int kind = (J.states ? PACK_ROWS : PACK_OPERAND) | tin;
+        J.pk1 = (int)((size_t)nd * n * per);
+        pj[(size_t)nd * n] = PackJob{(int)(d1[n] * mstride), J.pk1, kind};
+        J.pk2 = J.pk1;
+        if (d2) {
+            J.pk2 = (int)(((size_t)nd * n + 1) * per);
+            pj[(size_t)nd * n + 1] = PackJob{(int)(d2[n] * mstride), J.pk2, kind};
+        }
+    }
+    float* pack = static_cast<float*>(
+        grow(reinterpret_cast<void*&>(dPack_), packCap_, pj.size() * per * sizeof(float)));
+    const int ntiles = (dims_.P + PTILE - 1) / PTILE;
+    const size_t nPart = (size_t)NR * count * ntiles;
+    double* wk = reinterpret_cast<double*>(
+        grow(dWork_, workCap_, (nPart + (size_t)NR * count) * sizeof(double)));
+    auto dpj = reinterpret_cast<const PackJob*>(upload(pj.data(), pj.size() * sizeof(PackJob)));
+    launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+    auto dj = reinterpret_cast<const DerivJob*>(upload(jobs.data(), count * sizeof(DerivJob)));
+    launch_deriv_mfma(dj, count, nd, dims_, pack, dCatWeights_ + (size_t)wIdx * dims_.C,
+                      dPatternWeights_, sites, wk, stream_);
+    launch_reduce_rows(wk, ntiles, NR * count, wk + nPart, stream_);
+    HIPCHECK(hipGetLastError());
+    *dSums = wk + nPart;
+    return BEAGLE_SUCCESS;
+}
+
 void Instance::launchPre(const int* ops, int stride, int count, double* partial) {
     const bool fourState = dims_.S == 4 && dims_.C <= 8 && !preTransposed_;
     const double* cw = partial ? dCatWeights_ + (size_t)preWeights_ * dims_.C : nullptr;
+    if (!fourState && !partial && mfmaAdjoint()) {
+        launchPreMfma(ops, stride, count);
+        return;
+    }
     if (!fourState) {
         if (!dScratch_) HIPCHECK(hipMalloc(&dScratch_, partialsBytes_));
         auto d = reinterpret_cast<const PreJob*>(upload(hPre_.data(), count * sizeof(PreJob)));
@@ -1156,6 +1252,20 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
                                                double* outD1, double* outD1Sq, double* outD2) {
     if (!validEigen(wIdx) || hCatWeights_[wIdx].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
     const bool fused = dims_.S == 4 && dims_.C <= 8 && !preTransposed_ && d2Idx != nullptr;
+    if (!fused && mfmaAdjoint() && d2Idx != nullptr) {
+        // S > 4: MFMA pre-order, then both derivative matrices in one pass
+        for (int o = 0; o < count; ++o)
+            if (ops[(size_t)o * BEAGLE_OP_COUNT] == BEAGLE_OP_NONE) return BEAGLE_ERROR_OUT_OF_RANGE;
+        int rc = updatePrePartials(ops, count, BEAGLE_OP_NONE, false);
+        if (rc != BEAGLE_SUCCESS || count == 0) return rc;
+        if (!(outD1 || outD1Sq || outD2)) return BEAGLE_SUCCESS;
+        std::vector<int> pre(count);
+        for (int o = 0; o < count; ++o) pre[o] = ops[(size_t)o * BEAGLE_OP_COUNT];
+        double* sums = nullptr;
+        rc = enqueueDerivMfma(postIdx, pre.data(), d1Idx, d2Idx, wIdx, count, nullptr, &sums);
+        if (rc != BEAGLE_SUCCESS) return rc;
+        return deliverDerivativeSums(sums, count, outD1, outD1Sq, outD2);
+    }
     if (!fused) {
         for (int o = 0; o < count; ++o)
             if (ops[(size_t)o * BEAGLE_OP_COUNT] == BEAGLE_OP_NONE) return BEAGLE_ERROR_OUT_OF_RANGE;
@@ -1181,6 +1291,13 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
         finishCall();
         return BEAGLE_SUCCESS;
     }
+    return deliverDerivativeSums(sums, count, outD1, outD1Sq, outD2);
+}
+
+// Per-operation sums (3 per operation on the device) to the caller: copied
+// and waited for, or, while capturing, written at each replay's wait.
+int Instance::deliverDerivativeSums(const double* sums, int count, double* outD1,
+                                    double* outD1Sq, double* outD2) {
     if (capturing_) {
         const size_t off = captureResult(sums, 3 * (size_t)count);
         double* dst[3] = {outD1, outD1Sq, outD2};
@@ -1676,6 +1793,25 @@ int Instance::calculateEdgeDerivatives(const int* postIdx, const int* preIdx, co
         jobs[n].scale = 0.0;
     }
     if (count == 0) return BEAGLE_SUCCESS;
+    if (mfmaAdjoint()) {
+        // sites go to a buffer of their own after the work area
+        double* sites = nullptr;
+        if (outD) {
+            sites = reinterpret_cast<double*>(
+                grow(dSites_, sitesCap_, (size_t)count * dims_.P * sizeof(double)));
+        }
+        double* dSums = nullptr;
+        const int rc = enqueueDerivMfma(postIdx, preIdx, dIdx, nullptr, w, count, sites, &dSums);
+        if (rc != BEAGLE_SUCCESS) return rc;
+        std::vector<double> sums(2 * (size_t)count);
+        copyOut(sums.data(), dSums, sums.size() * sizeof(double));
+        for (int n = 0; n < count; ++n) {
+            if (outSum) outSum[n] = sums[2 * n];
+            if (outSumSq) outSumSq[n] = sums[2 * n + 1];
+        }
+        if (outD) copyOut(outD, sites, (size_t)count * dims_.P * sizeof(double));
+        return BEAGLE_SUCCESS;
+    }
     const int nb = (dims_.P + 255) / 256;
     const size_t work = (size_t)2 * count * nb + 2 * (size_t)count;
     size_t extra = outD ? (size_t)count * dims_.P : 0;

@@ -199,6 +199,12 @@ private:
     void launchPostPlan(const std::vector<Segment>& plan);
     // S > 4, FP32: matrix repacking, then one MFMA launch per dependency level
     void launchPostMfma(const int* ops, int stride, int count);
+    bool mfmaAdjoint() const;
+    void launchPreMfma(const int* ops, int stride, int count);
+    int enqueueDerivMfma(const int* postIdx, const int* preIdx, const int* d1, const int* d2,
+                         int wIdx, int count, double* sites, double** dSums);
+    int deliverDerivativeSums(const double* sums, int count, double* outD1, double* outD1Sq,
+                              double* outD2);
 
 public:
     // post-order matrix-core engine for S > 4 in FP32: 0 FP32, 3 or 4 split BF16
@@ -269,6 +275,8 @@ private:
     size_t stageCap_ = 0;             // doubles
     void* dJobMats_ = nullptr;        // grow-only gathered matrices of node jobs
     size_t jobMatsCap_ = 0;
+    void* dSites_ = nullptr;          // grow-only per-site derivative ratios (MFMA path)
+    size_t sitesCap_ = 0;
     float* dPack_ = nullptr;          // grow-only MFMA-packed matrices of a traversal
     size_t packCap_ = 0;
     void* dScratch_ = nullptr;        // C*Ppad*Sp reals (generic pre-order)

@@ -61,6 +61,7 @@ struct PreJob {
     const void* xself; const int* sself;
     int kSelf, moD1, moD2;
     int oidx;                  // row of this operation in the derivative output
+    int pkSelf, pkSib;         // MFMA path (S > 4): packed P_self^T and P_sib
 };
 
 // The two pre-order operations of one node's children a (slot 0) and b
@@ -138,7 +139,15 @@ struct PackJob {
     int src, dst;
     int kind;      // PACK_OPERAND: partials child, PACK_ROWS: tip-state child
 };
-enum PackKind : int { PACK_OPERAND = 0, PACK_ROWS = 1, PACK_OPERAND_BF16 = 2 };
+enum PackKind : int {
+    PACK_OPERAND = 0,       // A operand of M x (partials): rows i, inner states k
+    PACK_ROWS = 1,          // M[., s] by state s (tip-state operand)
+    PACK_OPERAND_BF16 = 2,  // PACK_OPERAND as hi/lo BF16 parts
+    PACK_PRE_SELF = 3,      // A operand of M^T u with u held in accumulator registers
+    // added to a kind: the stored matrix is the transpose (the reference GPU
+    // convention for pre-order self and derivative matrices, S > 4)
+    PACK_TRANSPOSED_INPUT = 8,
+};
 
 // MFMA path shapes: M blocks of 32 output states, K in groups of 8 (four
 // steps of the 32x32x2 FP32 instruction, k = 2 step + half-wave).
@@ -149,10 +158,12 @@ __host__ __device__ inline int mfma_ksteps16(int S) { return (S + 15) / 16; }
 // floats per category of one packed child matrix
 __host__ __device__ inline int mfma_pack_reals(int S) {
     const int a = mfma_mblocks(S) * mfma_kgroups(S) * 256;   // FP32 operand blocks
+    const int q = mfma_mblocks(S) * mfma_mblocks(S) * 1024;   // pre-order self blocks
     const int b = mfma_mblocks(S) * mfma_ksteps16(S) * 512;  // hi/lo BF16 operand blocks
     const int t = (S + 1) * mfma_mblocks(S) * 32;            // rows by state
     int m = a > t ? a : t;
     m = m > b ? m : b;
+    m = m > q ? m : q;
     return (m + 63) / 64 * 64;
 }
 // FP64 path: M blocks of 16 output states (v_mfma_f64_16x16x4f64), the same k
@@ -175,6 +186,30 @@ void launch_pack_matrices(const PackJob* jobs, int njobs, int S, int C, const vo
 // partials loaded a few k groups ahead; 1: the whole partials tile up front.
 void launch_post_mfma(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
                       int variant, hipStream_t st);
+// One node of an MFMA edge-derivative request (S > 4, FP32): its pre- and
+// post-order partials (or tip states) and one or two packed derivative
+// matrices; `row` is its index in the request (output rows row * NR + k).
+struct DerivJob {
+    const void* pre;
+    const void* post;
+    const int* states;
+    int pk1, pk2;
+    int row;
+};
+
+// Pre-order (S > 4, FP32) per dependency level, grid (Ppad/32, njobs): the
+// jobs' pkSelf (PACK_PRE_SELF) and pkSib (PACK_OPERAND / PACK_ROWS) offsets
+// in the pack slab.
+void launch_pre_mfma(const PreJob* djobs, int njobs, const Dims& d, const float* pack,
+                     hipStream_t st);
+// Edge derivatives (S > 4, FP32): nd = 1 or 2 derivative matrices; per job
+// and 32-pattern tile, partial sums rows [row * NR + k][tile] (NR = 2: sum w
+// r1, sum w r1^2; NR = 3: also sum w r2) with r = N / D over categories;
+// sites (nd = 1, may be null): r1 per job and pattern.
+void launch_deriv_mfma(const DerivJob* djobs, int njobs, int nd, const Dims& d,
+                       const float* pack, const double* cw, const double* pw, double* sites,
+                       double* partial, hipStream_t st);
+
 // FP64 partials on v_mfma_f64_16x16x4f64: the matrices packed with
 // launch_pack_matrices64 (pack offsets in floats, even), grid (Ppad/16, njobs).
 void launch_pack_matrices64(const PackJob* jobs, int njobs, int S, int C, const void* mats,

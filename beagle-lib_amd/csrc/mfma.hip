@@ -58,15 +58,33 @@ __global__ __launch_bounds__(WG) void k_pack_mats(const PackJob* __restrict__ jo
     const R* M = mats + J.src + (long)c * S * (S + 1);
     const int MB = mfma_mblocks(S), KG = mfma_kgroups(S);
     float v = 0.f;
-    if (J.kind == PACK_OPERAND) {
+    // element (i, k) of the matrix as used, or of its transpose as stored
+    const bool tin = (J.kind & PACK_TRANSPOSED_INPUT) != 0;
+    const int kind = J.kind & ~PACK_TRANSPOSED_INPUT;
+    auto at = [&](int i, int k) { return (float)(tin ? M[k * (S + 1) + i] : M[i * (S + 1) + k]); };
+    if (kind == PACK_OPERAND) {
         // [mb][k group g][lane][q]: row 32 mb + lane % 32, column 8 g + 2 q + lane / 32
         if (e < MB * KG * 256) {
             const int mb = e / (KG * 256), r = e - mb * KG * 256;
             const int g = r / 256, lane = (r & 255) >> 2, q = r & 3;
             const int i = mb * 32 + (lane & 31), k = 8 * g + 2 * q + (lane >> 5);
-            if (i < S && k < S) v = (float)M[i * (S + 1) + k];
+            if (i < S && k < S) v = at(i, k);
         }
-    } else if (J.kind == PACK_OPERAND_BF16) {
+    } else if (kind == PACK_PRE_SELF) {
+        // M^T u with u in the accumulator layout: [mo][group g][lane][e];
+        // step s = 4 g + e holds accumulator element r = s % 16 of block
+        // s / 16, i.e. inner state 32 (s / 16) + 8 (r / 4) + 4 (lane / 32) + r % 4;
+        // output row j = 32 mo + lane % 32; value M[k][j]
+        const int NG = MB * 4;
+        if (e < MB * NG * 256) {
+            const int mo = e / (NG * 256), r0 = e - mo * NG * 256;
+            const int g = r0 / 256, lane = (r0 & 255) >> 2, q = r0 & 3;
+            const int s4 = 4 * g + q, mb = s4 >> 4, r = s4 & 15;
+            const int k = mb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int j = mo * 32 + (lane & 31);
+            if (j < S && k < S) v = at(k, j);
+        }
+    } else if (kind == PACK_OPERAND_BF16) {
         // [mb][k step t][part: hi, lo][lane][8 BF16]: row 32 mb + lane % 32,
         // column 16 t + 8 (lane / 32) + e; one float holds elements 2e' and 2e'+1
         const int KT = mfma_ksteps16(S);
@@ -86,11 +104,13 @@ __global__ __launch_bounds__(WG) void k_pack_mats(const PackJob* __restrict__ jo
             v = __builtin_bit_cast(float, (unsigned)half[0] | ((unsigned)half[1] << 16));
         }
     } else {
-        // [state s (S: the pad column)][row i]
+        // [state s (S: the pad column)][row i]; a transposed input has no pad
+        // row: a missing state reads 0 there (the derivative matrices' pad)
         const int W = MB * 32;
         if (e < (S + 1) * W) {
             const int s = e / W, i = e - s * W;
-            if (i < S) v = (float)M[i * (S + 1) + s];
+            if (i < S) v = tin ? (s < S ? (float)M[s * (S + 1) + i] : 0.f)
+                               : (float)M[i * (S + 1) + s];
         }
     }
     pack[J.dst + (long)c * per + e] = v;
@@ -269,6 +289,84 @@ __device__ __forceinline__ void stream_mfma(const GLOBAL float* x0, const GLOBAL
 // loaded PF stages ahead; tip-state children are gathered afterwards.
 // Scaling semantics as the generic path (BeagleCPUImpl.hpp:1555-1577,
 // 4008-4133).
+// Scaling and store of one operation's output tile (post- or pre-order).
+struct ScaleOut {
+    int mode, logScalers;
+    const double* scaleRead;
+    double* scaleWrite;
+    double* cumulative;
+    const double* accA;
+    const double* accB;
+    const short* e1;
+    const short* e2;
+    short* ed;
+    void* dest;
+};
+
+// Semantics of the generic path (BeagleCPUImpl.hpp:1555-1577, 4008-4133):
+// SCALE_READ divides by the stored raw factor; SCALE_WRITE (manual) divides
+// by the per-pattern maximum over states and categories (0 -> 1), stores it
+// (or its log) plus the children's factors, and adds log(max) to the
+// cumulative buffer; AUTO multiplies by 2^-e (max = f 2^e) and stores the
+// exponent sum.  The maximum over categories goes through LDS (`red`).
+template <int MB, bool AUTO>
+__device__ __forceinline__ void rescale_store(floatx16 (&v)[MB], const ScaleOut& J, long tbase,
+                                              int S, int C, int c, int n, int h, int p, bool in,
+                                              float (*red)[32]) {
+    if (J.mode == SCALE_READ) {
+        const float f = (float)gin<double>(J.scaleRead)[p];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[mb][r] = v[mb][r] / f;
+    }
+    if (AUTO || J.mode == SCALE_WRITE) {
+        float mx = 0.f;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) mx = fmaxf(mx, v[mb][r]);
+        mx = fmaxf(mx, __shfl_xor(mx, 32));
+        if (h == 0) red[c][n] = mx;
+        __syncthreads();
+        for (int cc = 0; cc < C; ++cc) mx = fmaxf(mx, red[cc][n]);
+        if (AUTO) {
+            const int e = mx > 0.f ? exponent_of(mx) : 0;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[mb][r] = pow2(v[mb][r], -e);
+            if (c == 0 && h == 0 && in) {
+                int t = e;
+                if (J.e1) t += gin<short>(J.e1)[p];
+                if (J.e2) t += gin<short>(J.e2)[p];
+                gout<short>(J.ed)[p] = (short)t;
+            }
+        } else {
+            if (mx == 0.f) mx = 1.f;
+            const float inv = 1.f / mx;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) v[mb] *= inv;
+            if (c == 0 && h == 0 && in) {
+                const double lm = log((double)mx);
+                double w = J.logScalers ? lm : (double)mx;
+                if (J.cumulative) gout<double>(J.cumulative)[p] += lm;
+                if (J.accA) w += gin<double>(J.accA)[p];
+                if (J.accB) w += gin<double>(J.accB)[p];
+                gout<double>(J.scaleWrite)[p] = w;
+            }
+        }
+    }
+    GLOBAL float* out = gout<float>(J.dest) + tbase + n;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = mb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            if (i < S && in) out[i * PTILE] = v[mb][r];
+        }
+}
+
 // MODE 0: FP32 matrix cores, partials pipelined; 1: FP32, whole partials
 // tile up front; 3, 4: split BF16 with 3 or 4 products.  KX: k groups of 8
 // (FP32) or k steps of 16 (BF16).
@@ -315,60 +413,11 @@ __global__ __launch_bounds__(512) void k_postm(const PostJob* __restrict__ jobs,
     }
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) y[0][mb] *= y[1][mb];
-    floatx16 (&v)[MB] = y[0];
-    if (J.mode == SCALE_READ) {
-        const float f = (float)gin<double>(J.scaleRead)[p];
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) v[mb][r] = v[mb][r] / f;
-    }
-    if (AUTO || J.mode == SCALE_WRITE) {
-        float mx = 0.f;
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) mx = fmaxf(mx, v[mb][r]);
-        mx = fmaxf(mx, __shfl_xor(mx, 32));
-        if (h == 0) red[c][n] = mx;
-        __syncthreads();
-        for (int cc = 0; cc < d.C; ++cc) mx = fmaxf(mx, red[cc][n]);
-        if (AUTO) {
-            const int e = mx > 0.f ? exponent_of(mx) : 0;
-#pragma unroll
-            for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) v[mb][r] = pow2(v[mb][r], -e);
-            if (c == 0 && h == 0 && in) {
-                int t = e;
-                if (J.e1) t += gin<short>(J.e1)[p];
-                if (J.e2) t += gin<short>(J.e2)[p];
-                gout<short>(J.ed)[p] = (short)t;
-            }
-        } else {
-            if (mx == 0.f) mx = 1.f;
-            const float inv = 1.f / mx;
-#pragma unroll
-            for (int mb = 0; mb < MB; ++mb) v[mb] *= inv;
-            if (c == 0 && h == 0 && in) {
-                const double lm = log((double)mx);
-                double w = J.logScalers ? lm : (double)mx;
-                if (J.cumulative) gout<double>(J.cumulative)[p] += lm;
-                if (J.accA) w += gin<double>(J.accA)[p];
-                if (J.accB) w += gin<double>(J.accB)[p];
-                gout<double>(J.scaleWrite)[p] = w;
-            }
-        }
-    }
-    GLOBAL float* out = gout<float>(J.dest) + tbase + n;
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int i = mb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-            if (i < S && in) out[i * PTILE] = v[mb][r];
-        }
+    const ScaleOut so{J.mode, J.logScalers, J.scaleRead, J.scaleWrite, J.cumulative, J.accA,
+                      J.accB, J.e1, J.e2, J.ed, J.dest};
+    rescale_store<MB, AUTO>(y[0], so, tbase, S, d.C, c, n, h, p, in, red);
 }
+
 
 void launch_post_mfma(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
                       int variant, hipStream_t st) {
@@ -413,6 +462,238 @@ void launch_post_bf16(const PostJob* djobs, int njobs, const Dims& d, const floa
     }
 #undef POSTB
 #undef POSTB_V
+}
+
+// ---------------------------------------------------------------------------
+// Pre-order (S > 4, FP32): dest = P_self^T (pre_parent * (P_sib x_sib))
+// (BeagleCPUImpl.hpp:4530-4662; the reference GPU computes the same sums in
+// libhmsbeagle/GPU/kernels/kernelsXDerivatives.cu:1-82).  The first product
+// runs as a post-order child stream; its result, multiplied by the parent's
+// pre-order partials, stays in the accumulator registers and is the B
+// operand of the second product: inner state k of instruction s = 4 g + e is
+// the accumulator element the lane already holds (PACK_PRE_SELF orders
+// P_self^T the same way), so no data moves between lanes.
+
+// z = M^T u, u in the accumulator layout; A operands loaded PF groups ahead.
+template <int MB>
+__device__ __forceinline__ void gemm_from_acc(const float* pk, int l, const floatx16 (&u)[MB],
+                                              floatx16 (&z)[MB]) {
+    constexpr int NG = MB * 4, PF = 3;
+    const GLOBAL floatx4* a = (const GLOBAL floatx4*)gin<float>(pk) + l;
+#pragma unroll
+    for (int mo = 0; mo < MB; ++mo) z[mo] = floatx16{0};
+    floatx4 am[PF][MB];
+#pragma unroll
+    for (int j = 0; j < PF && j < NG; ++j)
+#pragma unroll
+        for (int mo = 0; mo < MB; ++mo) am[j][mo] = a[(mo * NG + j) * 64];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int s4 = 4 * g + e;
+#pragma unroll
+            for (int mo = 0; mo < MB; ++mo)
+                z[mo] = __builtin_amdgcn_mfma_f32_32x32x2f32(am[g % PF][mo][e], u[s4 >> 4][s4 & 15],
+                                                            z[mo], 0, 0, 0);
+        }
+        if (g + PF < NG) {
+#pragma unroll
+            for (int mo = 0; mo < MB; ++mo) am[g % PF][mo] = a[(mo * NG + g + PF) * 64];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// values of a partials buffer in the accumulator layout (rows >= S read 0)
+template <int MB>
+__device__ __forceinline__ void load_acc(const void* x, long tbase, int n, int h, int S,
+                                         floatx16 (&v)[MB]) {
+    const GLOBAL float* xb = gin<float>(x) + tbase + n;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = mb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            v[mb][r] = i < S ? xb[i * PTILE] : 0.f;
+        }
+}
+
+template <int MB, int KG, bool AUTO>
+__global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, Dims d,
+                                              const float* __restrict__ pack) {
+    __shared__ float red[8][32];
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int l = threadIdx.x & 63, n = l & 31, h = l >> 5;
+    const PreJob& J = jobs[blockIdx.y];
+    const int tp0 = blockIdx.x * PTILE;
+    if (tp0 >= J.p1 || tp0 + PTILE <= J.p0) return;    // uniform over the workgroup
+    const int p = tp0 + n;
+    const bool in = p >= J.p0 && p < J.p1;
+    const int S = d.S;
+    const long per = mfma_pack_reals(S);
+    const long tbase = pbase(d, c, tp0);
+    floatx16 par[MB];
+    load_acc<MB>(J.parent, tbase, n, h, S, par);
+    floatx16 y[2][MB];
+    const float* psib = pack + J.pkSib + c * per;
+    if (J.ss) {
+        state_rows<MB>(psib, J.ss, p, h, y[0]);
+    } else {
+        const GLOBAL float* xs = gin<float>(J.xs) + tbase + n;
+        const GLOBAL floatx4* as = (const GLOBAL floatx4*)gin<float>(psib) + l;
+        stream_mfma<MB, KG, 1, false>(xs, as, xs, as, h, S, y);
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) y[0][mb] *= par[mb];
+    gemm_from_acc<MB>(pack + J.pkSelf + c * per, l, y[0], y[1]);
+    const ScaleOut so{J.mode, J.logScalers, nullptr, J.scaleWrite, nullptr, J.accA,
+                      J.accB, J.e1, J.e2, J.ed, J.dest};
+    rescale_store<MB, AUTO>(y[1], so, tbase, S, d.C, c, n, h, p, in, red);
+}
+
+void launch_pre_mfma(const PreJob* djobs, int njobs, const Dims& d, const float* pack,
+                     hipStream_t st) {
+    if (njobs <= 0) return;
+    const dim3 grid(d.Ppad / PTILE, njobs), block(64 * d.C);
+    const int KG = mfma_kgroups(d.S);
+#define PREM(M, K)                                                                             \
+    case K:                                                                                    \
+        if (d.autoScale)                                                                       \
+            hipLaunchKernelGGL((k_prem<M, K, true>), grid, block, 0, st, djobs, d, pack);     \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_prem<M, K, false>), grid, block, 0, st, djobs, d, pack);    \
+        break;
+    switch (KG) {
+        PREM(1, 1) PREM(1, 2) PREM(1, 3) PREM(1, 4)
+        PREM(2, 5) PREM(2, 6) PREM(2, 7) PREM(2, 8)
+        default: break;
+    }
+#undef PREM
+}
+
+// ---------------------------------------------------------------------------
+// Edge derivatives (S > 4, FP32), BeagleCPUImpl.hpp:2086-2119 semantics: per
+// node and pattern N_k = sum_c w_c pre . (D_k x), D = sum_c w_c pre . x (a tip:
+// (D_k x)_i = D_k[i][state], D = sum_c w_c pre[state % S]); r_k = N_k / D.
+// D_k x runs as a child stream; the dot products with pre use the
+// accumulator layout, per category in FP32, combined over categories in
+// FP64 through LDS.  Tile sums of w r1, w r1^2 (and w r2) per node.
+
+__device__ __forceinline__ double half_wave_sum(double v) {
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+template <int MB, int KG, int ND>
+__global__ __launch_bounds__(512) void k_derivm(const DerivJob* __restrict__ jobs, Dims d,
+                                                const float* __restrict__ pack,
+                                                const double* __restrict__ cw,
+                                                const double* __restrict__ pw, double* sites,
+                                                double* __restrict__ partial) {
+    __shared__ double red[8][3][32];
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int l = threadIdx.x & 63, n = l & 31, h = l >> 5;
+    const DerivJob& J = jobs[blockIdx.y];
+    const int tile = blockIdx.x, ntiles = gridDim.x;
+    const int p = tile * PTILE + n;
+    const bool in = p < d.P;
+    const int S = d.S;
+    const long per = mfma_pack_reals(S);
+    const long tbase = pbase(d, c, tile * PTILE);
+    const float* p1 = pack + J.pk1 + c * per;
+    const float* p2 = pack + (ND == 2 ? J.pk2 : J.pk1) + c * per;
+    floatx16 y[2][MB], z[MB];
+    int st = 0;
+    if (J.states) {
+        st = gin<int>(J.states)[p];
+        state_rows<MB>(p1, J.states, p, h, y[0]);
+        if (ND == 2) state_rows<MB>(p2, J.states, p, h, y[1]);
+    } else {
+        const GLOBAL float* x = gin<float>(J.post) + tbase + n;
+        stream_mfma<MB, KG, ND, false>(x, (const GLOBAL floatx4*)gin<float>(p1) + l, x,
+                                       (const GLOBAL floatx4*)gin<float>(p2) + l, h, S, y);
+    }
+    load_acc<MB>(J.pre, tbase, n, h, S, z);
+    float den = 0.f, n1 = 0.f, n2 = 0.f;
+    if (J.states) {
+        const int sd = st % S;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = mb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                den += i == sd ? z[mb][r] : 0.f;
+                n1 += z[mb][r] * y[0][mb][r];
+                if (ND == 2) n2 += z[mb][r] * y[1][mb][r];
+            }
+    } else {
+        floatx16 x[MB];
+        load_acc<MB>(J.post, tbase, n, h, S, x);
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                den += z[mb][r] * x[mb][r];
+                n1 += z[mb][r] * y[0][mb][r];
+                if (ND == 2) n2 += z[mb][r] * y[1][mb][r];
+            }
+    }
+    den += __shfl_xor(den, 32);
+    n1 += __shfl_xor(n1, 32);
+    if (ND == 2) n2 += __shfl_xor(n2, 32);
+    const double wc = cw[c];
+    if (h == 0) {
+        red[c][0][n] = wc * (double)den;
+        red[c][1][n] = wc * (double)n1;
+        red[c][2][n] = ND == 2 ? wc * (double)n2 : 0.0;
+    }
+    __syncthreads();
+    if (c != 0) return;
+    double D = 0.0, N1 = 0.0, N2 = 0.0;
+    for (int cc = 0; cc < d.C; ++cc) {
+        D += red[cc][0][n];
+        N1 += red[cc][1][n];
+        N2 += red[cc][2][n];
+    }
+    const bool use = in && h == 0;
+    const double r1 = use ? N1 / D : 0.0, r2 = use ? N2 / D : 0.0, w = use ? pw[p] : 0.0;
+    if (ND == 1 && sites && use) sites[(long)J.row * d.P + p] = r1;
+    constexpr int NR = ND == 2 ? 3 : 2;
+    const double s1 = half_wave_sum(w * r1), sq = half_wave_sum(w * r1 * r1);
+    const double s2 = ND == 2 ? half_wave_sum(w * r2) : 0.0;
+    if (l == 0) {
+        GLOBAL double* pt = gout<double>(partial);
+        const long row = (long)J.row * NR;
+        pt[(row + 0) * ntiles + tile] = s1;
+        pt[(row + 1) * ntiles + tile] = sq;
+        if (ND == 2) pt[(row + 2) * ntiles + tile] = s2;
+    }
+}
+
+void launch_deriv_mfma(const DerivJob* djobs, int njobs, int nd, const Dims& d,
+                       const float* pack, const double* cw, const double* pw, double* sites,
+                       double* partial, hipStream_t st) {
+    if (njobs <= 0) return;
+    const dim3 grid((d.P + PTILE - 1) / PTILE, njobs), block(64 * d.C);
+    const int KG = mfma_kgroups(d.S);
+#define DERIVM(M, K)                                                                           \
+    case K:                                                                                    \
+        if (nd == 2)                                                                           \
+            hipLaunchKernelGGL((k_derivm<M, K, 2>), grid, block, 0, st, djobs, d, pack, cw, pw, \
+                               sites, partial);                                                \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_derivm<M, K, 1>), grid, block, 0, st, djobs, d, pack, cw, pw, \
+                               sites, partial);                                                \
+        break;
+    switch (KG) {
+        DERIVM(1, 1) DERIVM(1, 2) DERIVM(1, 3) DERIVM(1, 4)
+        DERIVM(2, 5) DERIVM(2, 6) DERIVM(2, 7) DERIVM(2, 8)
+        default: break;
+    }
+#undef DERIVM
 }
 
 // ---------------------------------------------------------------------------

@@ -304,6 +304,75 @@ def test_graph_codon_root_and_spoiled_capture(amd):
     ev.close()
 
 
+ADJ_CASES = [
+    # S, taxa, patterns, shape, tips_as, scaling, transpose
+    (20, 10, 400, "yule", "states", "none", "auto"),
+    (20, 9, 333, "caterpillar", "partials", "manual", "auto"),
+    (61, 8, 200, "balanced", "states", "auto", "auto"),
+    (61, 7, 150, "yule", "partials", "always", "auto"),
+    (33, 6, 97, "yule", "states", "auto", "auto"),
+    (20, 8, 300, "yule", "states", "none", "manual"),
+    (61, 6, 130, "balanced", "partials", "auto", "manual"),
+]
+
+
+def _adjoint_run(lib, prob, precision, tips_as, scaling, transpose):
+    ev = driver.Evaluator(lib, prob, precision=precision, tips_as=tips_as, scaling=scaling,
+                          matrices="explicit", transpose=transpose)
+    out = {"logl": ev.log_likelihood()}
+    ev.pre_order()
+    out["d1"], out["s1"], out["q1"] = ev.edge_derivatives(False)
+    _, out["s2"], _ = ev.edge_derivatives(True, per_site=False)
+    out["pre"] = ev.inst.get_partials(ev.pre_base + ev.branch_nodes()[1])
+    ev.close()
+    return out
+
+
+@pytest.mark.parametrize("S,n,P,shape,tips_as,scaling,transpose", ADJ_CASES)
+def test_mfma_adjoint_single_matches_oracle(amd, oracle, S, n, P, shape, tips_as, scaling,
+                                            transpose):
+    """S > 4 in FP32: pre-order partials (P_self^T (pre_parent * P_sib x_sib)
+    with the second product fed from accumulator registers) and edge
+    derivatives (D x with D = r_c Q, r_c^2 Q^2) on the FP32 matrix cores,
+    against the FP64 oracle: logL within 1e-6 relative, per-site
+    derivative ratios and per-branch sums within 1e-4 of their largest
+    magnitude, pre-order partials (unscaled runs) within 1e-5 of each row's
+    maximum.  transpose="manual": the client passes the pre-order self and
+    derivative matrices transposed (the reference GPU convention)."""
+    prob = synth.make_problem(29 + S + n, n, P, S=S, shape=shape, missing=0.05, mean_length=0.2)
+    g = _adjoint_run(amd, prob, "single", tips_as, scaling, transpose)
+    o = _adjoint_run(oracle, prob, "double", tips_as,
+                     "manual" if scaling == "auto" else scaling, "auto")
+    assert rel(g["logl"], o["logl"]) < 1e-6
+    for k in ("d1", "s1", "q1", "s2"):
+        a, b = np.asarray(g[k]), np.asarray(o[k])
+        assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max(), (k, np.abs(a - b).max())
+    if scaling == "none":
+        pg = np.asarray(g["pre"]).reshape(-1, S)
+        po = np.asarray(o["pre"]).reshape(-1, S)
+        keep = po.max(axis=1) > 1e-30
+        close_rows(pg[keep], po[keep], S, 1e-5)
+
+
+@pytest.mark.parametrize("S,transpose", [(20, "auto"), (61, "auto"), (61, "manual")])
+def test_mfma_fused_gradient_single(amd, oracle, S, transpose):
+    """beagleExtCalculateLogLikelihoodAndDerivatives for S > 4 in FP32
+    (matrix-core pre-order, then both derivative matrices in one pass)
+    against the oracle's beagleUpdatePrePartials + two
+    beagleCalculateEdgeDerivatives."""
+    prob = synth.make_problem(37 + S, 9, 300, S=S, missing=0.05, mean_length=0.2)
+    ev = driver.Evaluator(amd, prob, precision="single", scaling="auto", matrices="explicit",
+                          transpose=transpose)
+    lg, g1, g2 = ev.fused_gradient()
+    ev.close()
+    ev = driver.Evaluator(oracle, prob, scaling="manual", matrices="explicit")
+    lo, o1, o2 = ev.gradient()
+    ev.close()
+    assert rel(lg, lo) < 1e-6
+    assert np.abs(g1 - o1).max() <= 1e-4 * np.abs(o1).max()
+    assert np.abs(g2 - o2).max() <= 1e-4 * np.abs(o2).max()
+
+
 def test_split_bf16_engine_selection(amd):
     """beagleExtSetSplitBF16: FP32 instances with 5..64 states only, terms 0,
     3 or 4."""
