@@ -900,9 +900,7 @@ int Instance::postLevels(const int* ops, int stride, int count, std::vector<int>
 // ops share a cumulative scale buffer runs one op per launch in list order
 // (each rescaling op adds to the same per-pattern sums).
 void Instance::launchPostMfma(const int* ops, int stride, int count) {
-    // floats per packed child matrix (all categories)
-    const size_t per = (size_t)(dp_ ? 2 * mfma_pack_doubles(dims_.S) : mfma_pack_reals(dims_.S)) *
-                       dims_.C;
+    const size_t per = packFloats();
     std::vector<PackJob> pj(2 * (size_t)count);
     for (int o = 0; o < count; ++o) {
         PostJob& J = hPost_[o];
@@ -1059,15 +1057,20 @@ int Instance::buildPreJobs(const int* ops, int count, bool byPartition, bool all
 // Launch the pre-order jobs in hPre_ (built from `ops`): the generic-S
 // path one launch per op; for 4 states a level plan of batched and fused
 // segments (see planSegments).  partial != null adds the derivative sums.
-// S > 4 in FP32 (any engine setting): pre-order and edge derivatives on the
-// FP32 matrix cores.
-bool Instance::mfmaAdjoint() const { return !dp_ && mfma_post_supported(dims_.S, dims_.C); }
+// S > 4: pre-order and edge derivatives on the FP32 matrix cores (FP32
+// partials, whatever the post-order engine) or the FP64 ones.
+bool Instance::mfmaAdjoint() const { return mfma_post_supported(dims_.S, dims_.C); }
+
+// floats per packed child matrix (all categories) of the MFMA kernels
+size_t Instance::packFloats() const {
+    return (size_t)(dp_ ? 2 * mfma_pack_doubles(dims_.S) : mfma_pack_reals(dims_.S)) * dims_.C;
+}
 
 // MFMA pre-order: P_self^T packed for the register-resident second product
 // (from the stored transpose when the client passes transposed self
 // matrices), P_sib as a post-order child; one launch per dependency level.
 void Instance::launchPreMfma(const int* ops, int stride, int count) {
-    const size_t per = (size_t)mfma_pack_reals(dims_.S) * dims_.C;
+    const size_t per = packFloats();
     std::vector<PackJob> pj(2 * (size_t)count);
     const int tin = preTransposed_ ? PACK_TRANSPOSED_INPUT : 0;
     for (int o = 0; o < count; ++o) {
@@ -1095,10 +1098,13 @@ void Instance::launchPreMfma(const int* ops, int stride, int count) {
     std::vector<int> fill(start.begin(), start.end() - 1);
     for (int o = 0; o < count; ++o) sorted[fill[level[o]]++] = hPre_[o];
     auto dpj = reinterpret_cast<const PackJob*>(upload(pj.data(), pj.size() * sizeof(PackJob)));
-    launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+    if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+    else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     auto d = reinterpret_cast<const PreJob*>(upload(sorted.data(), count * sizeof(PreJob)));
-    for (int l = 0; l < nl; ++l)
-        launch_pre_mfma(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
+    for (int l = 0; l < nl; ++l) {
+        if (dp_) launch_pre_mfma64(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
+        else launch_pre_mfma(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
+    }
 }
 
 // MFMA edge derivatives of `count` nodes with one (d2 null) or two
@@ -1108,7 +1114,7 @@ void Instance::launchPreMfma(const int* ops, int stride, int count) {
 int Instance::enqueueDerivMfma(const int* postIdx, const int* preIdx, const int* d1,
                                const int* d2, int wIdx, int count, double* sites,
                                double** dSums) {
-    const size_t per = (size_t)mfma_pack_reals(dims_.S) * dims_.C;
+    const size_t per = packFloats();
     const int nd = d2 ? 2 : 1, NR = d2 ? 3 : 2;
     const size_t mstride = matrixBytes_ / realSize();
     const int tin = preTransposed_ ? PACK_TRANSPOSED_INPUT : 0;
@@ -1135,15 +1141,21 @@ int Instance::enqueueDerivMfma(const int* postIdx, const int* preIdx, const int*
     }
     float* pack = static_cast<float*>(
         grow(reinterpret_cast<void*&>(dPack_), packCap_, pj.size() * per * sizeof(float)));
-    const int ntiles = (dims_.P + PTILE - 1) / PTILE;
+    const int tileP = dp_ ? 16 : PTILE;          // patterns per derivative tile
+    const int ntiles = (dims_.P + tileP - 1) / tileP;
     const size_t nPart = (size_t)NR * count * ntiles;
     double* wk = reinterpret_cast<double*>(
         grow(dWork_, workCap_, (nPart + (size_t)NR * count) * sizeof(double)));
     auto dpj = reinterpret_cast<const PackJob*>(upload(pj.data(), pj.size() * sizeof(PackJob)));
-    launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+    if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+    else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     auto dj = reinterpret_cast<const DerivJob*>(upload(jobs.data(), count * sizeof(DerivJob)));
-    launch_deriv_mfma(dj, count, nd, dims_, pack, dCatWeights_ + (size_t)wIdx * dims_.C,
-                      dPatternWeights_, sites, wk, stream_);
+    if (dp_)
+        launch_deriv_mfma64(dj, count, nd, dims_, pack, dCatWeights_ + (size_t)wIdx * dims_.C,
+                            dPatternWeights_, sites, wk, stream_);
+    else
+        launch_deriv_mfma(dj, count, nd, dims_, pack, dCatWeights_ + (size_t)wIdx * dims_.C,
+                          dPatternWeights_, sites, wk, stream_);
     launch_reduce_rows(wk, ntiles, NR * count, wk + nPart, stream_);
     HIPCHECK(hipGetLastError());
     *dSums = wk + nPart;

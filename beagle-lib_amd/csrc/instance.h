@@ -200,6 +200,7 @@ private:
     // S > 4, FP32: matrix repacking, then one MFMA launch per dependency level
     void launchPostMfma(const int* ops, int stride, int count);
     bool mfmaAdjoint() const;
+    size_t packFloats() const;
     void launchPreMfma(const int* ops, int stride, int count);
     int enqueueDerivMfma(const int* postIdx, const int* preIdx, const int* d1, const int* d2,
                          int wIdx, int count, double* sites, double** dSums);

@@ -172,7 +172,9 @@ __host__ __device__ inline int mfma_mblocks16(int S) { return (S + 15) / 16; }
 __host__ __device__ inline int mfma_pack_doubles(int S) {
     const int a = mfma_mblocks16(S) * mfma_kgroups(S) * 128;
     const int t = (S + 1) * mfma_mblocks16(S) * 16;
-    const int m = a > t ? a : t;
+    const int q = mfma_mblocks16(S) * mfma_mblocks16(S) * 256;   // pre-order self blocks
+    int m = a > t ? a : t;
+    m = m > q ? m : q;
     return (m + 31) / 32 * 32;
 }
 // S range and category count the MFMA post-order kernel covers
@@ -216,6 +218,13 @@ void launch_pack_matrices64(const PackJob* jobs, int njobs, int S, int C, const 
                             float* pack, hipStream_t st);
 void launch_post_mfma64(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
                         hipStream_t st);
+// FP64 pre-order and edge derivatives (as launch_pre_mfma / launch_deriv_mfma;
+// grids over 16-pattern tiles, partial-sum rows [row * NR + k][P/16 tiles]).
+void launch_pre_mfma64(const PreJob* djobs, int njobs, const Dims& d, const float* pack,
+                       hipStream_t st);
+void launch_deriv_mfma64(const DerivJob* djobs, int njobs, int nd, const Dims& d,
+                         const float* pack, const double* cw, const double* pw, double* sites,
+                         double* partial, hipStream_t st);
 // The same on the BF16 matrix cores with hi/lo split operands (terms: 3 or 4
 // products per split pair); the matrices packed as PACK_OPERAND_BF16.
 void launch_post_bf16(const PostJob* djobs, int njobs, const Dims& d, const float* pack,

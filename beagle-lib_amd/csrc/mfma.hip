@@ -720,13 +720,28 @@ __global__ __launch_bounds__(WG) void k_pack_mats64(const PackJob* __restrict__ 
     const double* M = mats + J.src + (long)c * S * (S + 1);
     const int MB = mfma_mblocks16(S), KG = mfma_kgroups(S);
     double v = 0.0;
-    if (J.kind == PACK_OPERAND) {
+    const bool tin = (J.kind & PACK_TRANSPOSED_INPUT) != 0;
+    const int kind = J.kind & ~PACK_TRANSPOSED_INPUT;
+    auto at = [&](int i, int k) { return tin ? M[k * (S + 1) + i] : M[i * (S + 1) + k]; };
+    if (kind == PACK_OPERAND) {
         // [mb][group g][lane][q]: row 16 mb + lane % 16, column 8 g + 4 q + lane / 16
         if (e < MB * KG * 128) {
             const int mb = e / (KG * 128), r = e - mb * KG * 128;
             const int g = r / 128, lane = (r & 127) >> 1, q = r & 1;
             const int i = mb * 16 + (lane & 15), k = 8 * g + 4 * q + (lane >> 4);
-            if (i < S && k < S) v = M[i * (S + 1) + k];
+            if (i < S && k < S) v = at(i, k);
+        }
+    } else if (kind == PACK_PRE_SELF) {
+        // M^T u with u in the accumulator layout: [mo][group g][lane][q];
+        // step s = 2 g + q holds element r = s % 4 of block s / 4, i.e.
+        // inner state 16 (s / 4) + 4 r + lane / 16; output row j = 16 mo + lane % 16
+        const int NG = MB * 2;
+        if (e < MB * NG * 128) {
+            const int mo = e / (NG * 128), r0 = e - mo * NG * 128;
+            const int g = r0 / 128, lane = (r0 & 127) >> 1, q = r0 & 1;
+            const int s4 = 2 * g + q, mb = s4 >> 2, r = s4 & 3;
+            const int k = mb * 16 + 4 * r + (lane >> 4), j = mo * 16 + (lane & 15);
+            if (j < S && k < S) v = at(k, j);
         }
     } else {
         // [state s (S: the pad column)][mb][lane group kq][r]: row 16 mb + 4 r + kq
@@ -735,7 +750,7 @@ __global__ __launch_bounds__(WG) void k_pack_mats64(const PackJob* __restrict__ 
             const int s = e / W, w = e - s * W;
             const int mb = w >> 4, kq = (w & 15) >> 2, r = w & 3;
             const int i = mb * 16 + 4 * r + kq;
-            if (i < S) v = M[i * (S + 1) + s];
+            if (i < S) v = tin ? (s < S ? M[s * (S + 1) + i] : 0.0) : M[i * (S + 1) + s];
         }
     }
     pack[J.dst / 2 + (long)c * per + e] = v;
@@ -817,41 +832,12 @@ __device__ __forceinline__ void state_rows64(const double* pk, const int* states
     }
 }
 
-// FP64 post-order: one operation per blockIdx.y, 16 patterns per
-// blockIdx.x, one wave per category; scaling as k_postm.
-template <int MB, int KG, bool AUTO>
-__global__ __launch_bounds__(512) void k_postd(const PostJob* __restrict__ jobs, Dims d,
-                                               const float* __restrict__ pack) {
-    __shared__ double red[8][16];
-    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int l = threadIdx.x & 63, n = l & 15, kq = l >> 4;
-    const PostJob& J = jobs[blockIdx.y];
-    const int tp0 = blockIdx.x * 16;
-    if (tp0 >= J.p1 || tp0 + 16 <= J.p0) return;    // uniform over the workgroup
-    const int p = tp0 + n;
-    const bool in = p >= J.p0 && p < J.p1;
-    const int S = d.S;
-    const long per = mfma_pack_doubles(S);
-    const long tbase = pbase(d, c, tp0);
-    const GLOBAL double* x1 = gin<double>(J.x1) + tbase + n;
-    const GLOBAL double* x2 = gin<double>(J.x2) + tbase + n;
-    const double* p1 = reinterpret_cast<const double*>(pack + J.pk1) + c * per;
-    const double* p2 = reinterpret_cast<const double*>(pack + J.pk2) + c * per;
-    const GLOBAL doublex2* a1 = (const GLOBAL doublex2*)gin<double>(p1) + l;
-    const GLOBAL doublex2* a2 = (const GLOBAL doublex2*)gin<double>(p2) + l;
-    doublex4 y[2][MB];
-    const int na = (J.s1 ? 0 : 1) + (J.s2 ? 0 : 1);
-    if (na == 2) stream_mfma64<MB, KG, 2>(x1, a1, x2, a2, kq, S, y);
-    else if (na == 1)
-        stream_mfma64<MB, KG, 1>(J.s1 ? x2 : x1, J.s1 ? a2 : a1, x2, a2, kq, S, y);
-    if (J.s2) state_rows64<MB>(p2, J.s2, p, kq, y[1]);
-    if (J.s1) {
-        if (J.s2) state_rows64<MB>(p1, J.s1, p, kq, y[0]);
-        else state_rows64<MB>(p1, J.s1, p, kq, y[1]);
-    }
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) y[0][mb] *= y[1][mb];
-    doublex4 (&v)[MB] = y[0];
+// Scaling and store of an FP64 output tile (accumulator layout of
+// v_mfma_f64_16x16x4f64); semantics as rescale_store.
+template <int MB, bool AUTO>
+__device__ __forceinline__ void rescale_store64(doublex4 (&v)[MB], const ScaleOut& J, long tbase,
+                                                int S, int C, int c, int n, int kq, int p,
+                                                bool in, double (*red)[16]) {
     if (J.mode == SCALE_READ) {
         const double f = gin<double>(J.scaleRead)[p];
 #pragma unroll
@@ -869,7 +855,7 @@ __global__ __launch_bounds__(512) void k_postd(const PostJob* __restrict__ jobs,
         mx = fmax(mx, __shfl_xor(mx, 32));
         if (kq == 0) red[c][n] = mx;
         __syncthreads();
-        for (int cc = 0; cc < d.C; ++cc) mx = fmax(mx, red[cc][n]);
+        for (int cc = 0; cc < C; ++cc) mx = fmax(mx, red[cc][n]);
         if (AUTO) {
             const int e = mx > 0.0 ? exponent_of(mx) : 0;
 #pragma unroll
@@ -907,6 +893,45 @@ __global__ __launch_bounds__(512) void k_postd(const PostJob* __restrict__ jobs,
         }
 }
 
+// FP64 post-order: one operation per blockIdx.y, 16 patterns per
+// blockIdx.x, one wave per category; scaling as k_postm.
+template <int MB, int KG, bool AUTO>
+__global__ __launch_bounds__(512) void k_postd(const PostJob* __restrict__ jobs, Dims d,
+                                               const float* __restrict__ pack) {
+    __shared__ double red[8][16];
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int l = threadIdx.x & 63, n = l & 15, kq = l >> 4;
+    const PostJob& J = jobs[blockIdx.y];
+    const int tp0 = blockIdx.x * 16;
+    if (tp0 >= J.p1 || tp0 + 16 <= J.p0) return;    // uniform over the workgroup
+    const int p = tp0 + n;
+    const bool in = p >= J.p0 && p < J.p1;
+    const int S = d.S;
+    const long per = mfma_pack_doubles(S);
+    const long tbase = pbase(d, c, tp0);
+    const GLOBAL double* x1 = gin<double>(J.x1) + tbase + n;
+    const GLOBAL double* x2 = gin<double>(J.x2) + tbase + n;
+    const double* p1 = reinterpret_cast<const double*>(pack + J.pk1) + c * per;
+    const double* p2 = reinterpret_cast<const double*>(pack + J.pk2) + c * per;
+    const GLOBAL doublex2* a1 = (const GLOBAL doublex2*)gin<double>(p1) + l;
+    const GLOBAL doublex2* a2 = (const GLOBAL doublex2*)gin<double>(p2) + l;
+    doublex4 y[2][MB];
+    const int na = (J.s1 ? 0 : 1) + (J.s2 ? 0 : 1);
+    if (na == 2) stream_mfma64<MB, KG, 2>(x1, a1, x2, a2, kq, S, y);
+    else if (na == 1)
+        stream_mfma64<MB, KG, 1>(J.s1 ? x2 : x1, J.s1 ? a2 : a1, x2, a2, kq, S, y);
+    if (J.s2) state_rows64<MB>(p2, J.s2, p, kq, y[1]);
+    if (J.s1) {
+        if (J.s2) state_rows64<MB>(p1, J.s1, p, kq, y[0]);
+        else state_rows64<MB>(p1, J.s1, p, kq, y[1]);
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) y[0][mb] *= y[1][mb];
+    const ScaleOut so{J.mode, J.logScalers, J.scaleRead, J.scaleWrite, J.cumulative, J.accA,
+                      J.accB, J.e1, J.e2, J.ed, J.dest};
+    rescale_store64<MB, AUTO>(y[0], so, tbase, S, d.C, c, n, kq, p, in, red);
+}
+
 void launch_post_mfma64(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
                         hipStream_t st) {
     if (njobs <= 0) return;
@@ -925,6 +950,226 @@ void launch_post_mfma64(const PostJob* djobs, int njobs, const Dims& d, const fl
         default: break;
     }
 #undef POSTD
+}
+
+// FP64 pre-order (as k_prem): z = M^T u with u in the accumulator layout;
+// instruction s = 2 g + q uses inner states 16 (s / 4) + 4 (s % 4) + l / 16,
+// the rows the lane's element s % 4 of block s / 4 already holds.
+template <int MB>
+__device__ __forceinline__ void gemm_from_acc64(const double* pk, int l, const doublex4 (&u)[MB],
+                                                doublex4 (&z)[MB]) {
+    constexpr int NG = MB * 2, PF = 3;
+    const GLOBAL doublex2* a = (const GLOBAL doublex2*)gin<double>(pk) + l;
+#pragma unroll
+    for (int mo = 0; mo < MB; ++mo) z[mo] = doublex4{0.0, 0.0, 0.0, 0.0};
+    doublex2 am[PF][MB];
+#pragma unroll
+    for (int j = 0; j < PF && j < NG; ++j)
+#pragma unroll
+        for (int mo = 0; mo < MB; ++mo) am[j][mo] = a[(mo * NG + j) * 64];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int s4 = 2 * g + q;
+#pragma unroll
+            for (int mo = 0; mo < MB; ++mo)
+                z[mo] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[g % PF][mo][q], u[s4 >> 2][s4 & 3],
+                                                            z[mo], 0, 0, 0);
+        }
+        if (g + PF < NG) {
+#pragma unroll
+            for (int mo = 0; mo < MB; ++mo) am[g % PF][mo] = a[(mo * NG + g + PF) * 64];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <int MB>
+__device__ __forceinline__ void load_acc64(const void* x, long tbase, int n, int kq, int S,
+                                           doublex4 (&v)[MB]) {
+    const GLOBAL double* xb = gin<double>(x) + tbase + n;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = mb * 16 + 4 * r + kq;
+            v[mb][r] = i < S ? xb[i * PTILE] : 0.0;
+        }
+}
+
+template <int MB, int KG, bool AUTO>
+__global__ __launch_bounds__(512) void k_prem64(const PreJob* __restrict__ jobs, Dims d,
+                                                const float* __restrict__ pack) {
+    __shared__ double red[8][16];
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int l = threadIdx.x & 63, n = l & 15, kq = l >> 4;
+    const PreJob& J = jobs[blockIdx.y];
+    const int tp0 = blockIdx.x * 16;
+    if (tp0 >= J.p1 || tp0 + 16 <= J.p0) return;    // uniform over the workgroup
+    const int p = tp0 + n;
+    const bool in = p >= J.p0 && p < J.p1;
+    const int S = d.S;
+    const long per = mfma_pack_doubles(S);
+    const long tbase = pbase(d, c, tp0);
+    doublex4 par[MB];
+    load_acc64<MB>(J.parent, tbase, n, kq, S, par);
+    doublex4 y[2][MB];
+    const double* psib = reinterpret_cast<const double*>(pack + J.pkSib) + c * per;
+    if (J.ss) {
+        state_rows64<MB>(psib, J.ss, p, kq, y[0]);
+    } else {
+        const GLOBAL double* xs = gin<double>(J.xs) + tbase + n;
+        const GLOBAL doublex2* as = (const GLOBAL doublex2*)gin<double>(psib) + l;
+        stream_mfma64<MB, KG, 1>(xs, as, xs, as, kq, S, y);
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) y[0][mb] *= par[mb];
+    gemm_from_acc64<MB>(reinterpret_cast<const double*>(pack + J.pkSelf) + c * per, l, y[0],
+                        y[1]);
+    const ScaleOut so{J.mode, J.logScalers, nullptr, J.scaleWrite, nullptr, J.accA,
+                      J.accB, J.e1, J.e2, J.ed, J.dest};
+    rescale_store64<MB, AUTO>(y[1], so, tbase, S, d.C, c, n, kq, p, in, red);
+}
+
+void launch_pre_mfma64(const PreJob* djobs, int njobs, const Dims& d, const float* pack,
+                       hipStream_t st) {
+    if (njobs <= 0) return;
+    const dim3 grid(d.Ppad / 16, njobs), block(64 * d.C);
+    const int KG = mfma_kgroups(d.S);
+#define PREM64(M, K)                                                                           \
+    case K:                                                                                    \
+        if (d.autoScale)                                                                       \
+            hipLaunchKernelGGL((k_prem64<M, K, true>), grid, block, 0, st, djobs, d, pack);   \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_prem64<M, K, false>), grid, block, 0, st, djobs, d, pack);  \
+        break;
+    switch (KG) {
+        PREM64(1, 1) PREM64(1, 2) PREM64(2, 3) PREM64(2, 4)
+        PREM64(3, 5) PREM64(3, 6) PREM64(4, 7) PREM64(4, 8)
+        default: break;
+    }
+#undef PREM64
+}
+
+// FP64 edge derivatives (as k_derivm; 16-pattern tiles, FP64 throughout)
+__device__ __forceinline__ double quarter_wave_sum(double v) {
+#pragma unroll
+    for (int o = 8; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+template <int MB, int KG, int ND>
+__global__ __launch_bounds__(512) void k_derivm64(const DerivJob* __restrict__ jobs, Dims d,
+                                                  const float* __restrict__ pack,
+                                                  const double* __restrict__ cw,
+                                                  const double* __restrict__ pw, double* sites,
+                                                  double* __restrict__ partial) {
+    __shared__ double red[8][3][16];
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int l = threadIdx.x & 63, n = l & 15, kq = l >> 4;
+    const DerivJob& J = jobs[blockIdx.y];
+    const int tile = blockIdx.x, ntiles = gridDim.x;
+    const int p = tile * 16 + n;
+    const bool in = p < d.P;
+    const int S = d.S;
+    const long per = mfma_pack_doubles(S);
+    const long tbase = pbase(d, c, tile * 16);
+    const double* p1 = reinterpret_cast<const double*>(pack + J.pk1) + c * per;
+    const double* p2 = reinterpret_cast<const double*>(pack + (ND == 2 ? J.pk2 : J.pk1)) + c * per;
+    doublex4 y[2][MB], z[MB];
+    int st = 0;
+    if (J.states) {
+        st = gin<int>(J.states)[p];
+        state_rows64<MB>(p1, J.states, p, kq, y[0]);
+        if (ND == 2) state_rows64<MB>(p2, J.states, p, kq, y[1]);
+    } else {
+        const GLOBAL double* x = gin<double>(J.post) + tbase + n;
+        stream_mfma64<MB, KG, ND>(x, (const GLOBAL doublex2*)gin<double>(p1) + l, x,
+                                  (const GLOBAL doublex2*)gin<double>(p2) + l, kq, S, y);
+    }
+    load_acc64<MB>(J.pre, tbase, n, kq, S, z);
+    double den = 0.0, n1 = 0.0, n2 = 0.0;
+    if (J.states) {
+        const int sd = st % S;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = mb * 16 + 4 * r + kq;
+                den += i == sd ? z[mb][r] : 0.0;
+                n1 += z[mb][r] * y[0][mb][r];
+                if (ND == 2) n2 += z[mb][r] * y[1][mb][r];
+            }
+    } else {
+        doublex4 x[MB];
+        load_acc64<MB>(J.post, tbase, n, kq, S, x);
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                den += z[mb][r] * x[mb][r];
+                n1 += z[mb][r] * y[0][mb][r];
+                if (ND == 2) n2 += z[mb][r] * y[1][mb][r];
+            }
+    }
+#pragma unroll
+    for (int o = 16; o <= 32; o <<= 1) {
+        den += __shfl_xor(den, o);
+        n1 += __shfl_xor(n1, o);
+        if (ND == 2) n2 += __shfl_xor(n2, o);
+    }
+    const double wc = cw[c];
+    if (kq == 0) {
+        red[c][0][n] = wc * den;
+        red[c][1][n] = wc * n1;
+        red[c][2][n] = ND == 2 ? wc * n2 : 0.0;
+    }
+    __syncthreads();
+    if (c != 0) return;
+    double D = 0.0, N1 = 0.0, N2 = 0.0;
+    for (int cc = 0; cc < d.C; ++cc) {
+        D += red[cc][0][n];
+        N1 += red[cc][1][n];
+        N2 += red[cc][2][n];
+    }
+    const bool use = in && kq == 0;
+    const double r1 = use ? N1 / D : 0.0, r2 = use ? N2 / D : 0.0, w = use ? pw[p] : 0.0;
+    if (ND == 1 && sites && use) sites[(long)J.row * d.P + p] = r1;
+    constexpr int NR = ND == 2 ? 3 : 2;
+    const double s1 = quarter_wave_sum(w * r1), sq = quarter_wave_sum(w * r1 * r1);
+    const double s2 = ND == 2 ? quarter_wave_sum(w * r2) : 0.0;
+    if (l == 0) {
+        GLOBAL double* pt = gout<double>(partial);
+        const long row = (long)J.row * NR;
+        pt[(row + 0) * ntiles + tile] = s1;
+        pt[(row + 1) * ntiles + tile] = sq;
+        if (ND == 2) pt[(row + 2) * ntiles + tile] = s2;
+    }
+}
+
+void launch_deriv_mfma64(const DerivJob* djobs, int njobs, int nd, const Dims& d,
+                         const float* pack, const double* cw, const double* pw, double* sites,
+                         double* partial, hipStream_t st) {
+    if (njobs <= 0) return;
+    const dim3 grid((d.P + 15) / 16, njobs), block(64 * d.C);
+    const int KG = mfma_kgroups(d.S);
+#define DERIVM64(M, K)                                                                         \
+    case K:                                                                                    \
+        if (nd == 2)                                                                           \
+            hipLaunchKernelGGL((k_derivm64<M, K, 2>), grid, block, 0, st, djobs, d, pack, cw,  \
+                               pw, sites, partial);                                            \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_derivm64<M, K, 1>), grid, block, 0, st, djobs, d, pack, cw,  \
+                               pw, sites, partial);                                            \
+        break;
+    switch (KG) {
+        DERIVM64(1, 1) DERIVM64(1, 2) DERIVM64(2, 3) DERIVM64(2, 4)
+        DERIVM64(3, 5) DERIVM64(3, 6) DERIVM64(4, 7) DERIVM64(4, 8)
+        default: break;
+    }
+#undef DERIVM64
 }
 
 template void launch_pack_matrices<float>(const PackJob*, int, int, int, const void*, float*,

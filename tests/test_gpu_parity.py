@@ -354,6 +354,27 @@ def test_mfma_adjoint_single_matches_oracle(amd, oracle, S, n, P, shape, tips_as
         close_rows(pg[keep], po[keep], S, 1e-5)
 
 
+@pytest.mark.parametrize("S,tips_as,scaling,transpose", [(20, "states", "manual", "auto"),
+                                                        (61, "partials", "none", "manual"),
+                                                        (33, "states", "auto", "auto")])
+def test_mfma_adjoint_double_matches_oracle(amd, oracle, S, tips_as, scaling, transpose):
+    """S > 4 in FP64 (v_mfma_f64_16x16x4f64 pre-order and edge
+    derivatives) against the oracle on identical matrices: logL within
+    1e-12, derivative ratios and sums within 1e-10 of their largest
+    magnitude, pre-order partials (unscaled) within 1e-12 of each row's
+    maximum."""
+    prob = synth.make_problem(31 + S, 8, 200, S=S, missing=0.05, mean_length=0.2)
+    g = _adjoint_run(amd, prob, "double", tips_as, scaling, transpose)
+    o = _adjoint_run(oracle, prob, "double", tips_as, "manual" if scaling == "auto" else scaling,
+                     "auto")
+    assert rel(g["logl"], o["logl"]) < 1e-12
+    for k in ("d1", "s1", "q1", "s2"):
+        a, b = np.asarray(g[k]), np.asarray(o[k])
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max(), (k, np.abs(a - b).max())
+    if scaling == "none":
+        close_rows(g["pre"], o["pre"], S, 1e-12)
+
+
 @pytest.mark.parametrize("S,transpose", [(20, "auto"), (61, "auto"), (61, "manual")])
 def test_mfma_fused_gradient_single(amd, oracle, S, transpose):
     """beagleExtCalculateLogLikelihoodAndDerivatives for S > 4 in FP32
