@@ -24,7 +24,12 @@ codon: the 61-state GY94 codon workload (64 taxa, 20,000 patterns, Gamma(4);
 transition matrices + post-order + root logL per evaluation) on the FP64,
 FP32 and split-BF16 matrix-core post-order kernels: evaluations/s, error
 against FP64, partials GB/s and algorithmic TFLOP/s against each MFMA
-peak, and the reference CPU plugin on a pattern sample.
+peak; logL + full branch-gradient evaluations/s (matrix-core pre-order and
+derivative passes); and the reference CPU plugin on a pattern sample.
+
+partitions: 48 partitions with 4-, 20- and 61-state models and their own
+trees, FP32 logL + gradient per partition, eager calls against HIP-graph
+replays launched across the instances' streams.
 
 roofline: the dominant kernel of the value path, the node-fused FP32
 pre-order + derivative traversal of the caterpillar (one launch of 127
@@ -232,22 +237,114 @@ def codon_bench(lib, steps):
         torch.cuda.synchronize()
         ms = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
         ev.close()
+        # logL + dlogL/dt and d2logL/dt2 of every branch (matrix-core
+        # pre-order and derivative passes; the post-order on the path's engine)
+        ev = driver.Evaluator(lib, prob, precision=prec, scaling="auto",
+                              extra_flags=api.FLAG_COMPUTATION_ASYNCH)
+        if terms:
+            ev.inst.set_split_bf16(terms)
+        grad = ev.fused_gradient()
+        ng = max(steps // 3, 5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(ng):
+            grad = ev.fused_gradient()
+        torch.cuda.synchronize()
+        grad_rate = ng / (time.perf_counter() - t0)
+        ev.close()
+        if name == "fp64":
+            grad_ref = grad
         w = 8 if prec == "double" else 4
         gbs = postorder_bytes(prob, w, True) / (ms * 1e-3) / 1e9
         tf = flops / (ms * 1e-3) / 1e12
         if ref is None:
             ref = logl
         out[name] = {"evals_per_s": rate, "logL": logl, "logL_rel_err_vs_fp64": rel(logl, ref),
+                     "gradient_evals_per_s": grad_rate,
+                     "gradient_err_vs_fp64": max(grad_err(grad[1], grad_ref[1]),
+                                                 grad_err(grad[2], grad_ref[2])),
                      "postorder_ms": ms, "partials_GBps": gbs,
                      "partials_frac_of_hbm_peak": gbs / HBM_PEAK_GBS,
                      "algorithmic_TFLOPs": tf, "mfma_peak_TFLOPs": PEAK_TFLOPS[peak],
                      "frac_of_mfma_peak": tf / PEAK_TFLOPS[peak]}
     for name in ("fp32_mfma", "split_bf16x3"):
         out[name]["speedup_vs_fp64"] = out[name]["evals_per_s"] / out["fp64"]["evals_per_s"]
+        out[name]["gradient_speedup_vs_fp64"] = (out[name]["gradient_evals_per_s"] /
+                                                 out["fp64"]["gradient_evals_per_s"])
     out["workload"] = ("61-state GY94 codon, 64 taxa (Yule), 20,000 patterns, Gamma(4); "
                        "matrices + post-order + root logL per evaluation; algorithmic flops "
                        f"{flops:.3e} per traversal")
     return out
+
+
+PARTS = dict(count=48, pmin=200, pmax=50_000, taxa=(32, 256), states=(4, 20, 61))
+
+
+def partition_problems():
+    """48 partitions: state counts cycling 4 (GTR), 20 (exchangeabilities
+    ~Exp(1), pi ~Dirichlet(1)) and 61 (GY94 codon); patterns ~LogUniform(200,
+    50,000); Yule trees of 32-256 taxa; Gamma(4)."""
+    import numpy as np
+    from beagle_amd import synth
+    rng = np.random.default_rng(SEED + 100)
+    out = []
+    for k in range(PARTS["count"]):
+        S = PARTS["states"][k % 3]
+        P = int(round(np.exp(rng.uniform(np.log(PARTS["pmin"]), np.log(PARTS["pmax"])))))
+        taxa = int(rng.integers(PARTS["taxa"][0], PARTS["taxa"][1] + 1))
+        out.append(synth.make_problem(SEED + 1000 + k, taxa, P, S=S, shape="yule"))
+    return out
+
+
+def partitions_bench(lib, rounds):
+    """One round = FP32 logL + every branch's dlogL/dt and d2logL/dt2 for all
+    48 partitions (one instance each, matrix-core kernels for 20 and 61
+    states).  Eager: the extension calls instance after instance (each
+    waits for its results).  Graphs: each instance's evaluation captured
+    once as a HIP graph; a round launches all 48 replays (one stream per
+    instance, so partitions overlap) and then collects them."""
+    import numpy as np
+    import torch
+    from beagle_amd import api, driver
+    probs = partition_problems()
+    evs = [driver.Evaluator(lib, p, precision="single", scaling="auto",
+                            extra_flags=api.FLAG_COMPUTATION_ASYNCH) for p in probs]
+    for ev in evs:
+        ev.fused_gradient()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        eager = [ev.fused_gradient() for ev in evs]
+    torch.cuda.synchronize()
+    t_eager = (time.perf_counter() - t0) / rounds
+    for ev in evs:
+        ev.capture_fused_gradient()
+    for ev in evs:
+        ev.replay_fused_gradient(wait=False)
+    for ev in evs:
+        ev.collect_fused_gradient()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        for ev in evs:
+            ev.replay_fused_gradient(wait=False)
+        graph = [ev.collect_fused_gradient() for ev in evs]
+    t_graph = (time.perf_counter() - t0) / rounds
+    same = all(a[0] == b[0] and np.array_equal(a[1], b[1]) for a, b in zip(eager, graph))
+    for ev in evs:
+        ev.close()
+    n = len(probs)
+    return {"workload": "48 partitions: 4-, 20- and 61-state models in turn, patterns "
+                        "~LogUniform(200, 50000), Yule trees of 32-256 taxa, Gamma(4); FP32 "
+                        "logL + per-branch first and second derivatives per partition",
+            "total_patterns": int(sum(p.P for p in probs)),
+            "patterns_by_states": {str(S): int(sum(p.P for p in probs if p.S == S))
+                                   for S in PARTS["states"]},
+            "eager_ms_per_round": t_eager * 1e3, "graph_ms_per_round": t_graph * 1e3,
+            "eager_partition_evals_per_s": n / t_eager,
+            "graph_partition_evals_per_s": n / t_graph,
+            "graph_speedup": t_eager / t_graph,
+            "graph_results_identical_to_eager": bool(same), "rounds": rounds}
 
 
 def cpu_baseline_codon(budget_s):
@@ -417,6 +514,7 @@ def main():
     ev.close()
 
     codon = codon_bench(lib, args.steps)
+    partitions = partitions_bench(lib, max(args.steps // 10, 3))
 
     if args.calibrate_copy:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
@@ -482,6 +580,7 @@ def main():
         "logL": [r[0] for r in api64],
         "cpu_baseline": cpu,
         "codon": codon,
+        "partitions": partitions,
     }
     if cpu and cpu.get("codon"):
         codon["cpu_reference"] = cpu.pop("codon")
