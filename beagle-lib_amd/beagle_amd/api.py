@@ -143,6 +143,7 @@ SIGNATURES = {
     # extensions (include/beagle_amd_ext.h)
     "ExtGetStream": (_I, [_I, ctypes.POINTER(ctypes.c_void_p)]),
     "ExtSetSplitBF16": (_I, [_I, _I]),
+    "ExtSetAutotune": (_I, [_I, _I]),
     "ExtBeginCapture": (_I, [_I]),
     "ExtEndCapture": (_I, [_I, ctypes.POINTER(ctypes.c_int)]),
     "ExtLaunchGraph": (_I, [_I, _I]),
@@ -499,6 +500,10 @@ class Instance:
     # HIP-graph capture (extension entry points): calls between
     # begin_capture() and end_capture() are recorded; their output arrays
     # (which must stay alive) receive each replay's results at wait_graph().
+    def set_autotune(self, on=True):
+        """Extension: per-operation-list choice of launch plans by timing."""
+        self._c("ExtSetAutotune", int(bool(on)))
+
     def begin_capture(self):
         self._c("ExtBeginCapture")
 

@@ -641,6 +641,10 @@ int beagleExtSetSplitBF16(int instance, int terms) {
     return withInstance(instance, [&](Instance* in) { return in->setSplitBf16(terms); });
 }
 
+int beagleExtSetAutotune(int instance, int on) {
+    return withInstance(instance, [&](Instance* in) { return in->setAutotune(on); });
+}
+
 int beagleExtBeginCapture(int instance) {
     return withInstance(instance, [&](Instance* in) { return in->beginCapture(); });
 }

@@ -199,7 +199,11 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
     partStart_ = {0, patternCount};
     if (const char* m = std::getenv("BEAGLE_AMD_TRAVERSAL")) traversalMode_ = std::atoi(m);
     if (const char* m = std::getenv("BEAGLE_AMD_LEVEL_CHUNK")) levelChunk_ = std::max(1, std::atoi(m));
-    if (const char* m = std::getenv("BEAGLE_AMD_MFMA_VARIANT")) mfmaVariant_ = std::atoi(m);
+    if (const char* m = std::getenv("BEAGLE_AMD_MFMA_VARIANT")) {
+        mfmaVariant_ = std::atoi(m);
+        mfmaVariantForced_ = true;
+    }
+    if (const char* m = std::getenv("BEAGLE_AMD_AUTOTUNE")) autotune_ = std::atoi(m) != 0;
     ring_.resize(8);
     for (auto& s : ring_) HIPCHECK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
     sync();
@@ -856,17 +860,28 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
         // in order); otherwise wide levels are batched.
         bool shared = false;
         for (int o = 0; o < count; ++o) shared |= hPost_[o].cumulative != nullptr;
-        std::vector<Segment> plan;
-        if (shared || traversalMode_ == 2) {
-            Segment s{false, {}};
-            for (int o = 0; o < count; ++o) s.ops.push_back(o);
-            plan.push_back(std::move(s));
+        std::vector<int> level;
+        int nl = 0;
+        if (!shared) nl = postLevels(ops, stride, count, level);
+        // plan candidate k: kPlans[k] = {fused list, narrow-level threshold}
+        auto plan = [&](int k) {
+            std::vector<Segment> pl;
+            if (shared || kPlans[k].fused) {
+                Segment sg{false, {}};
+                for (int o = 0; o < count; ++o) sg.ops.push_back(o);
+                pl.push_back(std::move(sg));
+            } else {
+                pl = planSegments(level, nl, kPlans[k].narrow);
+            }
+            return pl;
+        };
+        if (shared) {
+            launchPostPlan(plan(0));
         } else {
-            std::vector<int> level;
-            const int nl = postLevels(ops, stride, count, level);
-            plan = planSegments(level, nl, traversalMode_ == 1 ? 1 : kNarrowLevel);
+            const int k = tunedChoice(hashOps(ops, (size_t)count * stride, 1), kNumPlans,
+                                      forcedPlan(), [&](int c) { launchPostPlan(plan(c)); });
+            if (k >= 0) launchPostPlan(plan(k));
         }
-        launchPostPlan(plan);
     }
     HIPCHECK(hipGetLastError());
     finishCall();
@@ -878,6 +893,69 @@ int Instance::setSplitBf16(int terms) {
     if (dp_ || !mfma_post_supported(dims_.S, dims_.C)) return BEAGLE_ERROR_NO_IMPLEMENTATION;
     splitBf16_ = terms != 0;
     if (terms) bf16Terms_ = terms;
+    return BEAGLE_SUCCESS;
+}
+
+// ---- autotuning ----------------------------------------------------------------
+// A traversal's launch plan (4-state: one fused launch, or levels batched
+// above a width threshold; MFMA: partials pipelined or a whole tile up
+// front) does not change its results, only its speed.  On the first call
+// with a given operation list the instance runs every candidate twice,
+// times the second run with events on its stream, keeps the fastest for
+// that list, and leaves the last run's outputs in place (the operations are
+// idempotent: lists that add to a cumulative scale buffer are not tuned).
+
+const Instance::PlanChoice Instance::kPlans[Instance::kNumPlans] = {
+    {false, kNarrowLevel}, {true, 0}, {false, 4}, {false, 16}, {false, 1}};
+
+uint64_t Instance::hashOps(const int* ops, size_t n, uint64_t tag) {
+    uint64_t h = 1469598103934665603ull ^ tag;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= (uint32_t)ops[i];
+        h *= 1099511628211ull;
+    }
+    return h ^ (n << 1);
+}
+
+int Instance::forcedPlan() const {
+    if (traversalMode_ == 1) return 4;   // every level batched
+    if (traversalMode_ == 2) return 1;   // the whole list fused
+    return -1;
+}
+
+template <typename F>
+int Instance::tunedChoice(uint64_t key, int ncand, int forced, F&& run) {
+    if (forced >= 0) return forced;
+    const auto it = tuned_.find(key);
+    if (it != tuned_.end()) return it->second;
+    if (!autotune_ || capturing_) return 0;
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    int best = 0;
+    float bestMs = 0.f;
+    for (int k = 0; k < ncand; ++k) {
+        run(k);
+        HIPCHECK(hipEventRecord(a, stream_));
+        run(k);
+        HIPCHECK(hipEventRecord(b, stream_));
+        HIPCHECK(hipEventSynchronize(b));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, a, b));
+        if (k == 0 || ms < bestMs) {
+            best = k;
+            bestMs = ms;
+        }
+    }
+    HIPCHECK(hipEventDestroy(a));
+    HIPCHECK(hipEventDestroy(b));
+    tuned_[key] = best;
+    return -1;
+}
+
+int Instance::setAutotune(int on) {
+    autotune_ = on != 0;
+    if (!autotune_) tuned_.clear();
     return BEAGLE_SUCCESS;
 }
 
@@ -932,15 +1010,22 @@ void Instance::launchPostMfma(const int* ops, int stride, int count) {
     if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     auto d = reinterpret_cast<const PostJob*>(upload(sorted.data(), count * sizeof(PostJob)));
-    for (int l = 0; l < nl; ++l)
-        if (dp_)
-            launch_post_mfma64(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
-        else if (splitBf16_)
-            launch_post_bf16(d + start[l], start[l + 1] - start[l], dims_, pack, bf16Terms_,
-                             stream_);
-        else
-            launch_post_mfma(d + start[l], start[l + 1] - start[l], dims_, pack, mfmaVariant_,
-                             stream_);
+    auto run = [&](int variant) {
+        for (int l = 0; l < nl; ++l) {
+            const int nj = start[l + 1] - start[l];
+            if (dp_) launch_post_mfma64(d + start[l], nj, dims_, pack, stream_);
+            else if (splitBf16_) launch_post_bf16(d + start[l], nj, dims_, pack, bf16Terms_, stream_);
+            else launch_post_mfma(d + start[l], nj, dims_, pack, variant, stream_);
+        }
+    };
+    if (dp_ || splitBf16_ || shared) {
+        run(mfmaVariant_);
+    } else {
+        // candidates: partials pipelined (0) or the whole tile up front (1)
+        const int k = tunedChoice(hashOps(ops, (size_t)count * stride, 3), 2,
+                                  mfmaVariantForced_ ? mfmaVariant_ : -1, run);
+        if (k >= 0) run(k);
+    }
 }
 
 // Launch a 4-state post-order plan with one upload of all its jobs.
@@ -1177,24 +1262,34 @@ void Instance::launchPre(const int* ops, int stride, int count, double* partial)
         else launch_pre<float>(d, hPre_.data(), count, dims_, t, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
         return;
     }
-    std::vector<Segment> plan;
-    if (traversalMode_ == 2) {
-        Segment s{false, {}};
-        for (int o = 0; o < count; ++o) s.ops.push_back(o);
-        plan.push_back(std::move(s));
-    } else {
-        std::vector<Access> acc(count);
-        std::vector<int> level;
-        for (int o = 0; o < count; ++o) {
-            const int* q = ops + (size_t)o * stride;
-            const PreJob& J = hPre_[o];
-            const int sw = J.scaleWrite ? (int)((J.scaleWrite - dScale_) / dims_.Ppad) : -1;
-            acc[o] = Access{{q[3], q[5], -1}, {q[0], sw >= 0 ? bufferCount_ + sw : -1}};
-        }
-        const int nl = dependencyLevels(acc, bufferCount_ + scaleCount_, level);
-        plan = planSegments(level, nl, traversalMode_ == 1 ? 1 : kNarrowLevel);
+    std::vector<Access> acc(count);
+    std::vector<int> level;
+    for (int o = 0; o < count; ++o) {
+        const int* q = ops + (size_t)o * stride;
+        const PreJob& J = hPre_[o];
+        const int sw = J.scaleWrite ? (int)((J.scaleWrite - dScale_) / dims_.Ppad) : -1;
+        acc[o] = Access{{q[3], q[5], -1}, {q[0], sw >= 0 ? bufferCount_ + sw : -1}};
     }
-    // one upload of all segments' jobs, then the launches
+    const int nl = dependencyLevels(acc, bufferCount_ + scaleCount_, level);
+    auto run = [&](int k) {
+        std::vector<Segment> plan;
+        if (kPlans[k].fused) {
+            Segment sg{false, {}};
+            for (int o = 0; o < count; ++o) sg.ops.push_back(o);
+            plan.push_back(std::move(sg));
+        } else {
+            plan = planSegments(level, nl, kPlans[k].narrow);
+        }
+        launchPrePlan(plan, cw, partial);
+    };
+    const int k = tunedChoice(hashOps(ops, (size_t)count * stride, partial ? 5 : 4), kNumPlans,
+                              forcedPlan(), run);
+    if (k >= 0) run(k);
+}
+
+// Launch a 4-state pre-order plan with one upload of all its jobs (the
+// batched / fused rules of launchPostPlan).
+void Instance::launchPrePlan(const std::vector<Segment>& plan, const double* cw, double* partial) {
     std::vector<PreJob> all;
     struct Launch { int first, n; bool batched; };
     std::vector<Launch> launches;
@@ -1486,14 +1581,27 @@ bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, dou
     const int n = (int)nodes.size();
     std::vector<int> level;
     const int nl = dependencyLevels(acc, bufferCount_, level);
-    std::vector<Segment> plan;
-    if (traversalMode_ == 2) {
-        Segment s{false, {}};
-        for (int j = 0; j < n; ++j) s.ops.push_back(j);
-        plan.push_back(std::move(s));
-    } else {
-        plan = planSegments(level, nl, traversalMode_ == 1 ? 1 : kNarrowLevel);
-    }
+    auto run = [&](int k) {
+        std::vector<Segment> plan;
+        if (kPlans[k].fused) {
+            Segment sg{false, {}};
+            for (int j = 0; j < n; ++j) sg.ops.push_back(j);
+            plan.push_back(std::move(sg));
+        } else {
+            plan = planSegments(level, nl, kPlans[k].narrow);
+        }
+        launchNodePlan(nodes, plan, partial);
+    };
+    const int k = tunedChoice(hashOps(ops, (size_t)count * BEAGLE_OP_COUNT, 2), kNumPlans,
+                              forcedPlan(), run);
+    if (k >= 0) run(k);
+    return true;
+}
+
+// Launch node jobs as planned: fused segments forward each job's parent from
+// the previous job's registers where it is that job's output.
+void Instance::launchNodePlan(const std::vector<NodeJob>& nodes, const std::vector<Segment>& plan,
+                              double* partial) {
     std::vector<NodeJob> all;
     struct Launch { int first, n; bool batched; };
     std::vector<Launch> launches;
@@ -1521,7 +1629,6 @@ bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, dou
         if (dp_) launch_pre_nodes<double>(d + L.first, L.n, L.batched, levelChunk_, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
         else launch_pre_nodes<float>(d + L.first, L.n, L.batched, levelChunk_, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
     }
-    return true;
 }
 
 // Each pre-order job's own post-order operand (postIdx) and derivative

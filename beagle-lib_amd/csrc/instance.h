@@ -8,7 +8,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "kernels.h"
@@ -210,6 +212,8 @@ private:
 public:
     // post-order matrix-core engine for S > 4 in FP32: 0 FP32, 3 or 4 split BF16
     int setSplitBf16(int terms);
+    // measure-and-keep choice of launch plans per operation list (on by default)
+    int setAutotune(int on);
 
 private:
     // dependency levels of a post-order list (reads children and a read
@@ -241,7 +245,24 @@ private:
     int traversalMode_ = 0;   // 0 automatic, 1 always per dependency level, 2 always fused
     bool splitBf16_ = false;  // S > 4 FP32 post-order on the BF16 matrix cores (hi/lo split)
     int bf16Terms_ = 3;       // products per split pair: 3, or 4 (adds lo x lo)
-    int mfmaVariant_ = 0;     // MFMA post-order: 0 partials pipelined, 1 whole tile up front
+    int mfmaVariant_ = 0;
+    bool mfmaVariantForced_ = false;
+    // autotuning: chosen plan candidate per operation list (hash)
+    struct PlanChoice {
+        bool fused;
+        int narrow;
+    };
+    static constexpr int kNumPlans = 5;
+    static const PlanChoice kPlans[kNumPlans];
+    bool autotune_ = true;
+    std::unordered_map<uint64_t, int> tuned_;
+    static uint64_t hashOps(const int* ops, size_t n, uint64_t tag);
+    int forcedPlan() const;
+    template <typename F>
+    int tunedChoice(uint64_t key, int ncand, int forced, F&& run);
+    void launchNodePlan(const std::vector<NodeJob>& nodes, const std::vector<Segment>& plan,
+                        double* partial);
+    void launchPrePlan(const std::vector<Segment>& plan, const double* cw, double* partial);     // MFMA post-order: 0 partials pipelined, 1 whole tile up front
     int levelChunk_ = 1;      // batched level: 1 one op per workgroup, k > 1 chunks of k ops
     Dims dims_;
     size_t partialsBytes_, matrixBytes_;

@@ -27,6 +27,17 @@ int beagleExtGetStream(int instance, void** outStream);
  * for other values of terms.  Applies to operations enqueued afterwards. */
 int beagleExtSetSplitBF16(int instance, int terms);
 
+/* Autotuning of launch plans (no counterpart in the reference; on by
+ * default, BEAGLE_AMD_AUTOTUNE=0 in the environment turns it off).  The
+ * first traversal call with a given operation list times each candidate
+ * plan (4 states: one fused launch over the list, or dependency levels
+ * batched above a width of 4, 8, 16 or 1 operations; S > 4 in FP32: partial
+ * loads pipelined or a whole tile issued at once) and keeps the fastest for
+ * that list; plans give identical results.  Lists that add to a cumulative
+ * scale buffer, and calls being captured into a graph, use the default
+ * plan.  on = 0 turns it off and forgets the choices. */
+int beagleExtSetAutotune(int instance, int on);
+
 /* HIP-graph capture of an instance's calls (no counterpart in the
  * reference).  Between beagleExtBeginCapture and beagleExtEndCapture the
  * instance's calls are recorded into a graph instead of run:
