@@ -394,6 +394,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
+    ap.add_argument("--only", choices=("codon", "partitions"),
+                    help="run one auxiliary workload alone (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB -> 1 GiB elementwise copy kernel, the "
                          "reference size for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -417,6 +419,12 @@ def main():
 
     assert torch.cuda.is_available(), "needs an MI355X"
     lib = beagle_amd.load()
+    if args.only == "codon":
+        print(json.dumps({"codon": codon_bench(lib, args.steps)}))
+        return
+    if args.only == "partitions":
+        print(json.dumps({"partitions": partitions_bench(lib, max(args.steps // 10, 3))}))
+        return
     probs = problems()
     asyn = api.FLAG_COMPUTATION_ASYNCH
 
