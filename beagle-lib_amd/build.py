@@ -16,7 +16,8 @@ LIB = os.path.join(HERE, "libhmsbeagle_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-fvisibility=hidden", "--offload-arch=gfx950",
           "-I" + os.path.join(REPO, "include"), "-I" + CSRC]
-# packed FP32 instructions stay off (see the build notes in DESIGN.md)
+# packed FP32 instructions (v_pk_*_f32) stay off: the compiler would form them
+# from ordinary float code, and this library's kernels do not rely on them
 DEVICE = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 SOURCES = ["kernels.hip", "mfma.hip", "instance.cpp", "frontend.cpp"]

@@ -409,7 +409,7 @@ def main():
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(patterns shard across GPUs with one all-reduce of the per-pattern sums "
-              "per evaluation; see DESIGN.md)", file=sys.stderr)
+              "per evaluation)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -15,7 +15,7 @@ Two matrix sources:
   device exp() and the host libm may differ in the last ulp, which is
   ~1e-16 absolute in P but up to ~1e-10 relative in P entries near zero
   (and more for 61-state codon entries of order 1e-15), so here only the
-  quantities the north star bounds are compared: log-likelihood within
+  quantities the accuracy targets bound are compared: log-likelihood within
   1e-10 relative, site log-likelihoods, derivative sums.
 """
 import numpy as np
