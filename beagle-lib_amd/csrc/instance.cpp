@@ -1609,11 +1609,16 @@ void Instance::launchNodePlan(const std::vector<NodeJob>& nodes, const std::vect
         const int first = (int)all.size();
         for (int j : s.ops) all.push_back(nodes[j]);
         if (!s.batched) {
+            const bool lookahead2 = !dp_ && node_f32_supported(dims_.C);
             for (size_t j = first + 1; j < all.size(); ++j) {
                 NodeJob& J = all[j];
                 const NodeJob& Pv = all[j - 1];
                 if (Pv.dest[0] && J.parent == Pv.dest[0]) J.kPar = 2;
                 else if (Pv.dest[1] && J.parent == Pv.dest[1]) J.kPar = 3;
+                else if (lookahead2 && j >= (size_t)first + 2 &&
+                         ((all[j - 2].dest[0] && J.parent == all[j - 2].dest[0]) ||
+                          (all[j - 2].dest[1] && J.parent == all[j - 2].dest[1])))
+                    J.kPar = 4;
             }
         }
         launches.push_back({first, (int)s.ops.size(), s.batched});
@@ -1624,7 +1629,8 @@ void Instance::launchNodePlan(const std::vector<NodeJob>& nodes, const std::vect
     void* jm = grow(dJobMats_, jobMatsCap_, all.size() * perJob * realSize());
     auto d = reinterpret_cast<const NodeJob*>(upload(all.data(), all.size() * sizeof(NodeJob)));
     if (dp_) launch_gather_node_matrices<double>(d, (int)all.size(), dims_, dMatrices_, jm, stream_);
-    else launch_gather_node_matrices<float>(d, (int)all.size(), dims_, dMatrices_, jm, stream_);
+    else launch_gather_node_matrices<float>(d, (int)all.size(), dims_, dMatrices_, jm, stream_,
+                                            node_f32_supported(dims_.C));
     for (const Launch& L : launches) {
         if (dp_) launch_pre_nodes<double>(d + L.first, L.n, L.batched, levelChunk_, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
         else launch_pre_nodes<float>(d + L.first, L.n, L.batched, levelChunk_, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);

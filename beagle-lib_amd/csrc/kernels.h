@@ -77,7 +77,9 @@ struct NodeJob {
     const short* ex[2];        //   partials, the children's post-order partials
     short* ed[2];              //   (null for tips), the children's pre-order partials
     int k[2];                  // 0 partials, 1 tip states
-    int kPar;                  // 0 load; 2, 3: previous job's dest[0], dest[1] (registers)
+    int kPar;                  // 0 load; 2, 3: previous job's dest[0], dest[1] (registers);
+                               // 4: load when computed (the job two back wrote it; FP32
+                               // node kernels, which prefetch two jobs ahead)
     int moP[2], moD1[2], moD2[2];  // matrix element offsets in the matrix slab
     int oidx[2];               // operation index of each child in the derivative output
     int mbase;                 // this job's block in the gathered matrices (reals)
@@ -279,10 +281,16 @@ void launch_root_freq_grad(const void* root, const double* cw, const double* fre
                            const double* pw, double* partial, const Dims& d, hipStream_t st);
 // Copies each node job's six matrices (P_a, P_b, D1_a, D2_a, D1_b, D2_b) into
 // jobMats as [job][category][6][20] reals, so a job's matrices for one
-// category are contiguous; job j's block starts at djobs[j].mbase.
+// category are contiguous; job j's block starts at djobs[j].mbase.  Each
+// matrix keeps its [4 rows][5 columns] order, or with colMajor becomes
+// [5 columns][4 rows] (the FP32 node kernels, node_f32_supported).
 template <typename R>
 void launch_gather_node_matrices(const NodeJob* djobs, int njobs, const Dims& d,
-                                 const void* mats, void* jobMats, hipStream_t st);
+                                 const void* mats, void* jobMats, hipStream_t st,
+                                 bool colMajor = false);
+// 4-state FP32 node jobs with these category counts run on the kernels that
+// read column-major job matrices
+inline bool node_f32_supported(int C) { return C == 1 || C == 2 || C == 4; }
 template <typename R>
 void launch_root_range(const void* root, const short* rootExp, const double* catWeights,
                        const double* freqs, const double* cumulative,

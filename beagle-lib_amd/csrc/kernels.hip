@@ -7,7 +7,8 @@
 // keep the reference CPU layout [category][row i][S+1 columns] with the pad
 // column that a missing tip state reads (1 for P(t), 0 for derivatives).
 // Scale buffers and every reduction are FP64 whatever the partials
-// precision.
+// precision, except the FP32 node kernels' per-tile derivative sums (FP32
+// within a 64-pattern tile, FP64 across tiles).
 //
 // The 4-state traversals are fused: one launch walks the whole operation
 // list, each lane owning one pattern (all categories) from the first
@@ -20,6 +21,8 @@
 #include "device_util.h"
 
 #include <cmath>
+#include <cstdlib>
+#include <type_traits>
 
 namespace hipbeagle {
 
@@ -773,6 +776,7 @@ struct Vec16<double> {
 // them; lanes past the matrices repeat the last one) so no branch
 // separates the lanes.
 constexpr int NODE_MATS = 6 * 20;
+constexpr int NODE_F32_BLOCK = 128;       // floats of a wave's block in the FP32 node kernels
 constexpr int NODE_MAT_BLOCK = 64 * 16;   // bytes of one wave's block
 
 // 32-bit addressing: a wave-uniform base pointer plus a per-lane byte
@@ -1047,6 +1051,330 @@ __global__ __launch_bounds__(512) void k_pren4_level(const NodeJob* __restrict__
                           dummyW, dummyEw, cw, w, partial, tile, ntiles);
 }
 #undef NODE_KERNEL_SETUP
+
+// ---------------------------------------------------------------------------
+// FP32 node jobs with the category count C as a template parameter (C = 1,
+// 2, 4).  Same jobs and results as node_compute up to FP32 rounding; the
+// work per job is cut down to the 4x4 products:
+//  - a tip child's products P x and D x are its state's matrix column,
+//    read from the wave's LDS block with the state as the lane's index (the
+//    pad column for a missing state), not 20 products with a one-hot vector;
+//  - categories exchange FP32 values, [2][8 quantities][C][TILE] floats;
+//  - the derivative ratios are formed and summed over the tile's patterns
+//    in FP32 (the tile sums are added in FP64 across tiles), by DPP steps;
+//  - the two waves that form the sums alternate from job to job, so every
+//    SIMD does the same work.
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_movf(float v) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+
+// Sum over the 64 lanes, the steps of wave_sum_dpp in FP32.
+__device__ __forceinline__ float wave_sum_f32(float v) {
+    v += dpp_movf<0xB1, 0xf>(v);
+    v += dpp_movf<0x4E, 0xf>(v);
+    v += dpp_movf<0x114, 0xf>(v);
+    v += dpp_movf<0x118, 0xf>(v);
+    v += dpp_movf<0x142, 0xa>(v);
+    v += dpp_movf<0x143, 0xc>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// The FP32 node kernels read each matrix of a job column by column from the
+// wave's LDS block: [5 columns][4 rows] (column 4 is the pad column), so a
+// column is one aligned 16-byte read.
+__device__ __forceinline__ fvec4 column(const float* m, int j) {
+    return *(const fvec4*)(m + 4 * j);
+}
+
+// t = M x for a child: its partials (a combination of the four columns),
+// or for a tip the column of its state (s in [0, 4], a per-lane read;
+// column 4 is the pad column of a missing state).
+__device__ __forceinline__ void child_product(bool tip, int s, const float (&x)[4],
+                                              const float* m, float (&t)[4]) {
+    fvec4 r;
+    if (tip) {
+        r = column(m, s);
+    } else {
+        r = column(m, 0) * x[0];
+        r += column(m, 1) * x[1];
+        r += column(m, 2) * x[2];
+        r += column(m, 3) * x[3];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t[i] = r[i];
+}
+
+// t_j = sum_i M[i][j] u_i = column j . u
+__device__ __forceinline__ void mat4t_cols(const float* m, const float (&u)[4], float (&t)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const fvec4 c = column(m, j);
+        t[j] = c[0] * u[0] + c[1] * u[1] + c[2] * u[2] + c[3] * u[3];
+    }
+}
+
+// Derivative numerators and denominator of one branch for one category:
+// den = pre . x (a tip: pre[state & 3], as the reference computes it),
+// n1 = pre . (D1 x), n2 = pre . (D2 x).
+__device__ __forceinline__ void branch_sums_f32(bool tip, int s, const float (&pre)[4],
+                                                const float (&x)[4], const float* d1,
+                                                const float* d2, float& den, float& n1,
+                                                float& n2) {
+    float t[4];
+    if (tip) {
+        // bit tests, not an index: an indexed pick is lowered to a
+        // private-memory load
+        const float lo = (s & 1) ? pre[1] : pre[0], hi = (s & 1) ? pre[3] : pre[2];
+        den = (s & 2) ? hi : lo;
+    } else {
+        den = dot4(pre, x);
+    }
+    child_product(tip, s, x, d1, t);
+    n1 = dot4(pre, t);
+    child_product(tip, s, x, d2, t);
+    n2 = dot4(pre, t);
+}
+
+template <int C, bool AUTO>
+__device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps4<float>& o,
+                                                 float (&pa)[4], float (&pb)[4], int& pea,
+                                                 int& peb, float* smat, float* sx, int& k, int c,
+                                                 int lane, int p, int P, unsigned off,
+                                                 unsigned doff, float* dummyW, void* dummyEw,
+                                                 const float (&wc)[C], float w, double* partial,
+                                                 int tile, int ntiles) {
+    const bool in = p < P;
+    float u[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) u[i] = J.kPar == 2 ? pa[i] : (J.kPar == 3 ? pb[i] : o.u[i]);
+    int eu = J.kPar == 2 ? pea : (J.kPar == 3 ? peb : o.eu);
+    if (J.kPar == 4) {
+        // the parent is the output of the job two back, stored after this
+        // job's operands were prefetched: load it now
+        load4(at<float>(J.parent, off), u);
+        if (AUTO) eu = J.eParent ? *at<short>(J.eParent, 2u * p) : 0;
+    }
+    const bool tipA = J.k[0] == 1, tipB = J.k[1] == 1;
+    // tip states are stored in [0, 4]; the clamp keeps the column index in
+    // the matrix whatever a lane without a tip read
+    const int sa = min((unsigned)o.sa, 4u), sb = min((unsigned)o.sb, 4u);
+    const int par = k & 1;
+    float* lm = smat + (c * 2 + par) * NODE_F32_BLOCK;
+    *(fvec4*)(lm + min(lane, NODE_MATS / 4 - 1) * 4) = o.mv;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    float t[4], ua[4], ub[4], va[4], vb[4];
+    child_product(tipA, sa, o.xa, lm, t);          // P_a x_a
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ub[i] = u[i] * t[i];
+    child_product(tipB, sb, o.xb, lm + 20, t);     // P_b x_b
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ua[i] = u[i] * t[i];
+    mat4t_cols(lm, ua, va);                        // pre(a) = P_a^T (pre(v) * P_b x_b)
+    mat4t_cols(lm + 20, ub, vb);                   // pre(b) = P_b^T (pre(v) * P_a x_a)
+    float dA, n1A, n2A, dB, n1B, n2B;
+    branch_sums_f32(tipA, sa, va, o.xa, lm + 40, lm + 60, dA, n1A, n2A);
+    branch_sums_f32(tipB, sb, vb, o.xb, lm + 80, lm + 100, dB, n1B, n2B);
+    // exchange: max a, max b, den/n1/n2 of a, of b
+    float* buf = sx + par * 8 * C * TILE;
+    ++k;
+    float* mine = buf + c * TILE + lane;
+    mine[0 * C * TILE] = in ? fmaxf(fmaxf(va[0], va[1]), fmaxf(va[2], va[3])) : 0.0f;
+    mine[1 * C * TILE] = in ? fmaxf(fmaxf(vb[0], vb[1]), fmaxf(vb[2], vb[3])) : 0.0f;
+    mine[2 * C * TILE] = dA;
+    mine[3 * C * TILE] = n1A;
+    mine[4 * C * TILE] = n2A;
+    mine[5 * C * TILE] = dB;
+    mine[6 * C * TILE] = n1B;
+    mine[7 * C * TILE] = n2B;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    const bool keepA = in && J.dest[0] != nullptr;
+    const bool keepB = in && J.dest[1] != nullptr;
+    int outA = 0, outB = 0;
+    if (AUTO) {
+        float ma = buf[lane], mb = buf[C * TILE + lane];
+#pragma unroll
+        for (int cc = 1; cc < C; ++cc) {
+            ma = fmaxf(ma, buf[cc * TILE + lane]);
+            mb = fmaxf(mb, buf[(C + cc) * TILE + lane]);
+        }
+        const int ea = ma > 0.0f ? exponent_of(ma) : 0;
+        const int eb = mb > 0.0f ? exponent_of(mb) : 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            va[i] = pow2(va[i], -ea);
+            vb[i] = pow2(vb[i], -eb);
+        }
+        outA = ea + eu + (J.ex[1] ? o.eb : 0);
+        outB = eb + eu + (J.ex[0] ? o.ea : 0);
+        *at_w<short>(keepA ? (void*)J.ed[0] : dummyEw, keepA ? 2u * p : 2u * lane) = (short)outA;
+        *at_w<short>(keepB ? (void*)J.ed[1] : dummyEw, keepB ? 2u * p : 2u * lane) = (short)outB;
+    }
+    // tile sums of child ch (w r1, w r1^2, w r2, r = N / D over categories)
+    // by wave (ch + 2 par) % C
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+        if (c != (ch + 2 * par) % C) continue;
+        const float* q = buf + (2 + 3 * ch) * C * TILE + lane;
+        float D = wc[0] * q[0], N1 = wc[0] * q[C * TILE], N2 = wc[0] * q[2 * C * TILE];
+#pragma unroll
+        for (int cc = 1; cc < C; ++cc) {
+            D += wc[cc] * q[cc * TILE];
+            N1 += wc[cc] * q[(C + cc) * TILE];
+            N2 += wc[cc] * q[(2 * C + cc) * TILE];
+        }
+        const float inv = 1.0f / D;
+        const float r1 = in ? N1 * inv : 0.0f, r2 = in ? N2 * inv : 0.0f;
+        const float a = w * r1;
+        const float s1 = wave_sum_f32(a), sq = wave_sum_f32(a * r1), s2 = wave_sum_f32(w * r2);
+        if (lane == 0) {
+            GLOBAL double* pt = gout<double>(partial);
+            const long row = (long)J.oidx[ch] * 3;
+            pt[(row + 0) * ntiles + tile] = (double)s1;
+            pt[(row + 1) * ntiles + tile] = (double)sq;
+            pt[(row + 2) * ntiles + tile] = (double)s2;
+        }
+    }
+    store4(at_w<float>(keepA ? J.dest[0] : (void*)dummyW, keepA ? off : doff), va);
+    store4(at_w<float>(keepB ? J.dest[1] : (void*)dummyW, keepB ? off : doff), vb);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        pa[i] = va[i];
+        pb[i] = vb[i];
+    }
+    pea = outA;
+    peb = outB;
+}
+
+#define NODE_F32_SETUP                                                                        \
+    extern __shared__ double sdv[]; /* [2][8][C][TILE] floats, then [C][2] matrix blocks */   \
+    static_assert(NODE_MATS <= NODE_F32_BLOCK, "matrix block");                               \
+    float* sx = reinterpret_cast<float*>(sdv);                                                \
+    float* smat = sx + 2 * 8 * C * TILE;                                                      \
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);                          \
+    const int lane = threadIdx.x & 63;                                                        \
+    const int tile = blockIdx.x;                                                              \
+    const int p = tile * TILE + lane; /* < Ppad: pattern weights are zero-padded */           \
+    const unsigned off = (unsigned)((c * cs + (long)p * 4) * (long)sizeof(float));            \
+    const unsigned doff = (unsigned)((c * TILE + lane) * 4 * (int)sizeof(float));             \
+    const int ntiles = (P + TILE - 1) / TILE;                                                 \
+    const float w = (float)pw[p];                                                             \
+    float wc[C];                                                                              \
+    _Pragma("unroll") for (int cc = 0; cc < C; ++cc) wc[cc] = __int_as_float(                 \
+        __builtin_amdgcn_readfirstlane(__float_as_int((float)cw[cc])));                       \
+    void* dummyEw = const_cast<char*>(reinterpret_cast<const char*>(dummyE)) + 128;           \
+    int k = 0, pea = 0, peb = 0;                                                              \
+    float pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
+
+// PF = operand prefetch distance in jobs: 0 (loads issued when the job
+// starts; at most 72 VGPRs, so seven waves per SIMD and a 100,000-pattern
+// grid resident at once), 1 or 2 (rotating register sets; jobs past the
+// end re-read the last job and are not computed).  0 is the default;
+// BEAGLE_AMD_NODE_PREFETCH selects another.
+template <int C, bool AUTO, int PF>
+__global__ __launch_bounds__(64 * C, PF == 0 ? 7 : 1) void k_pren4f(const NodeJob* __restrict__ jobs, int njobs,
+                                                   int chunk, int P, long cs,
+                                                   const float* __restrict__ jm,
+                                                   const float* dummyX, const int* dummyS,
+                                                   float* dummyW, const short* dummyE,
+                                                   const double* __restrict__ cw,
+                                                   const double* __restrict__ pw,
+                                                   double* partial) {
+    NODE_F32_SETUP
+    jobs += blockIdx.y * chunk;
+    njobs = min(chunk, njobs - (int)blockIdx.y * chunk);
+#define NODE_F32_STEP(JC, SC, JN, SN, I)                                                         \
+    JN = jobs[min(o + (I) + PF, njobs - 1)];                                                      \
+    node_issue<float, AUTO>(JN, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, SN);          \
+    node_compute_f32<C, AUTO>(JC, SC, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, doff,   \
+                              dummyW, dummyEw, wc, w, partial, tile, ntiles);                    \
+    if (o + (I) + 1 >= njobs) break;
+    if (PF == 0) {
+        for (int o = 0; o < njobs; ++o) {
+            const NodeJob J = jobs[o];
+            NodeOps4<float> S0;
+            node_issue<float, AUTO>(J, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
+            node_compute_f32<C, AUTO>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
+                                      doff, dummyW, dummyEw, wc, w, partial, tile, ntiles);
+        }
+    } else if (PF == 1) {
+        NodeOps4<float> S0, S1;
+        NodeJob J0 = jobs[0], J1;
+        node_issue<float, AUTO>(J0, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
+        for (int o = 0;; o += 2) {
+            NODE_F32_STEP(J0, S0, J1, S1, 0)
+            NODE_F32_STEP(J1, S1, J0, S0, 1)
+        }
+    } else {
+        NodeOps4<float> S0, S1, S2;
+        NodeJob J0 = jobs[0], J1 = jobs[min(1, njobs - 1)], J2;
+        node_issue<float, AUTO>(J0, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
+        node_issue<float, AUTO>(J1, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S1);
+        for (int o = 0;; o += 3) {
+            NODE_F32_STEP(J0, S0, J2, S2, 0)
+            NODE_F32_STEP(J1, S1, J0, S0, 1)
+            NODE_F32_STEP(J2, S2, J1, S1, 2)
+        }
+    }
+#undef NODE_F32_STEP
+}
+
+template <int C, bool AUTO>
+__global__ __launch_bounds__(64 * C) void k_pren4f_level(const NodeJob* __restrict__ jobs,
+                                                         int P, long cs,
+                                                         const float* __restrict__ jm,
+                                                         const float* dummyX, const int* dummyS,
+                                                         float* dummyW, const short* dummyE,
+                                                         const double* __restrict__ cw,
+                                                         const double* __restrict__ pw,
+                                                         double* partial) {
+    NODE_F32_SETUP
+    NodeOps4<float> S0;
+    const NodeJob J = jobs[blockIdx.y];
+    node_issue<float, AUTO>(J, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
+    node_compute_f32<C, AUTO>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, doff,
+                              dummyW, dummyEw, wc, w, partial, tile, ntiles);
+}
+#undef NODE_F32_SETUP
+
+template <int C>
+static void launch_pre_nodes_f32(const NodeJob* djobs, int njobs, bool batched, int chunk,
+                                 const Dims& d, const float* m, const float* dummyX,
+                                 const int* dummyS, float* dummyW, const short* dummyE,
+                                 const double* cw, const double* pw, double* partial,
+                                 hipStream_t st) {
+    const int tiles = (d.P + TILE - 1) / TILE;
+    const size_t lds = (size_t)(2 * 8 * C * TILE + C * 2 * NODE_F32_BLOCK) * sizeof(float);
+    const dim3 block(TILE * C);
+    if (batched && chunk <= 1) {
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4f_level<C, true>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
+        else hipLaunchKernelGGL((k_pren4f_level<C, false>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
+    } else {
+        if (!batched || chunk > njobs) chunk = njobs;
+        const dim3 grid(tiles, (njobs + chunk - 1) / chunk);
+        static const int pf = [] {
+            const char* e = std::getenv("BEAGLE_AMD_NODE_PREFETCH");
+            return e ? std::min(2, std::max(0, std::atoi(e))) : 0;
+        }();
+#define PREN4F(A, F) hipLaunchKernelGGL((k_pren4f<C, A, F>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial)
+        if (d.autoScale) {
+            if (pf == 0) PREN4F(true, 0);
+            else if (pf == 1) PREN4F(true, 1);
+            else PREN4F(true, 2);
+        } else {
+            if (pf == 0) PREN4F(false, 0);
+            else if (pf == 1) PREN4F(false, 1);
+            else PREN4F(false, 2);
+        }
+#undef PREN4F
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Rate-matrix gradient (4 states).  For one eigen-system Q = V diag(lam) V^-1
@@ -1471,7 +1799,7 @@ template <typename R>
 __global__ __launch_bounds__(WG) void k_gather_node_mats(const NodeJob* __restrict__ jobs,
                                                          int njobs, int C,
                                                          const R* __restrict__ mats,
-                                                         R* __restrict__ out) {
+                                                         R* __restrict__ out, bool colMajor) {
     const long t = (long)blockIdx.x * WG + threadIdx.x;
     const long per = (long)C * NODE_MATS;
     if (t >= per * njobs) return;
@@ -1479,20 +1807,22 @@ __global__ __launch_bounds__(WG) void k_gather_node_mats(const NodeJob* __restri
     const int rem = (int)(t - (long)j * per);
     const int c = rem / NODE_MATS, r = rem - c * NODE_MATS;
     const int m = r / 20, e = r - m * 20;
+    // row-major [4][5] as stored, or column-major [5][4]
+    const int d = colMajor ? m * 20 + (e % 5) * 4 + e / 5 : r;
     const NodeJob& J = jobs[j];
     const int mo = m == 0 ? J.moP[0] : m == 1 ? J.moP[1] : m == 2 ? J.moD1[0]
                  : m == 3 ? J.moD2[0] : m == 4 ? J.moD1[1] : J.moD2[1];
-    out[(long)J.mbase + c * NODE_MATS + r] = mats[(long)mo + c * 20 + e];
+    out[(long)J.mbase + c * NODE_MATS + d] = mats[(long)mo + c * 20 + e];
 }
 
 template <typename R>
 void launch_gather_node_matrices(const NodeJob* djobs, int njobs, const Dims& d,
-                                 const void* mats, void* jobMats, hipStream_t st) {
+                                 const void* mats, void* jobMats, hipStream_t st, bool colMajor) {
     const long n = (long)njobs * d.C * NODE_MATS;
     if (n <= 0) return;
     hipLaunchKernelGGL(k_gather_node_mats<R>, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, st,
                        djobs, njobs, d.C, reinterpret_cast<const R*>(mats),
-                       reinterpret_cast<R*>(jobMats));
+                       reinterpret_cast<R*>(jobMats), colMajor);
 }
 
 template <typename R>
@@ -1506,6 +1836,15 @@ void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, 
     const int* dummyS = reinterpret_cast<const int*>(dummyD + TILE);
     const short* dummyE = reinterpret_cast<const short*>(dummyS + TILE);
     const R* m = reinterpret_cast<const R*>(jobMats);
+    if (std::is_same<R, float>::value && node_f32_supported(d.C)) {
+        const float* mf = reinterpret_cast<const float*>(m);
+        const float* xf = reinterpret_cast<const float*>(dummyX);
+        float* wf = reinterpret_cast<float*>(dummyW);
+        if (d.C == 4) launch_pre_nodes_f32<4>(djobs, njobs, batched, chunk, d, mf, xf, dummyS, wf, dummyE, cw, pw, partial, st);
+        else if (d.C == 2) launch_pre_nodes_f32<2>(djobs, njobs, batched, chunk, d, mf, xf, dummyS, wf, dummyE, cw, pw, partial, st);
+        else launch_pre_nodes_f32<1>(djobs, njobs, batched, chunk, d, mf, xf, dummyS, wf, dummyE, cw, pw, partial, st);
+        return;
+    }
     const int tiles = (d.P + TILE - 1) / TILE;
     const size_t lds = (size_t)2 * d.C * 8 * TILE * sizeof(double) +
                        (size_t)d.C * 2 * NODE_MAT_BLOCK;
@@ -2268,7 +2607,7 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
                                 const void*, void*, const double*, const double*, double*,       \
                                 hipStream_t);                                                    \
     template void launch_gather_node_matrices<R>(const NodeJob*, int, const Dims&, const void*, \
-                                                 void*, hipStream_t);                        \
+                                                 void*, hipStream_t, bool);                  \
     template void launch_qgrad<R>(const NodeJob*, int, const Dims&, const void*, const double*,  \
                                   const double*, void*, const double*, const double*, double*,  \
                                   hipStream_t);                                                 \

@@ -12,9 +12,13 @@ pre-order traversal and the two derivative sums per branch (derivative
 matrices r_c Q and r_c^2 Q^2).  One step = one evaluation of each tree.
 
 value: evaluations per second of the FP32 path (partials in FP32 with
-exact power-of-two rescaling; derivative sums, scale exponents and the
-log-likelihood in FP64; pre-order fused with the derivative sums), with
-its logL and gradient error against the FP64 path in the same line.
+exact power-of-two rescaling; derivative ratios summed in FP32 within
+64-pattern tiles and in FP64 across tiles; scale exponents and the
+log-likelihood in FP64; pre-order fused with the derivative sums), each
+tree's evaluation replayed as a HIP graph with its branch lengths set per
+step and both trees launched before either is collected, with its logL and
+gradient error against the FP64 path in the same line (and the eager
+calls' rate beside it).
 Also reported: the plain FP64 path through the beagle.h calls
 (beagleUpdatePrePartials + two beagleCalculateEdgeDerivatives), the FP64
 fused path, the FP64 post-order traversal's HBM bandwidth, and the
@@ -439,13 +443,35 @@ def main():
             fused_eval(ev)
         w += 1
     torch.cuda.synchronize()
+    # eager calls, one evaluation after the other
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res32 = [fused_eval(evs[0]), fused_eval(evs[1])]
+        eager32 = [fused_eval(evs[0]), fused_eval(evs[1])]
+    torch.cuda.synchronize()
+    eager_rate = 2 * args.steps / (time.perf_counter() - t0)
+    # the batched path: each tree's evaluation captured once as a HIP graph;
+    # a step sets the branch lengths of both graphs, launches both on their
+    # instances' streams and then collects both results
+    lengths = [ev.prob.tree.lengths for ev in evs]
+    for ev in evs:
+        ev.capture_fused_gradient()
+    for _ in range(max(args.warmup, 3)):
+        for ev, x in zip(evs, lengths):
+            ev.replay_fused_gradient(x, wait=False)
+        for ev in evs:
+            ev.collect_fused_gradient()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for ev, x in zip(evs, lengths):
+            ev.replay_fused_gradient(x, wait=False)
+        res32 = [ev.collect_fused_gradient() for ev in evs]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms_per_step = dt * 1e3 / args.steps
     value = 2 * args.steps / dt
+    graph_identical = all(a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+                          for a, b in zip(res32, eager32))
     cat_ops, cat_post, _, _ = evs[0].fused_operations()
     post_idx = dict(zip(map(tuple, cat_ops.tolist()), cat_post.tolist()))
     pre_bytes = preorder_bytes(probs[0], cat_ops, lambda o: post_idx[tuple(o.tolist())], 4, True)
@@ -566,10 +592,13 @@ def main():
                    "taxa": N_TAXA, "patterns": N_PATTERNS, "categories": 4,
                    "evals_per_step": 2,
                    "path": "FP32 partials, power-of-two rescaling, fused pre-order + "
-                           "derivative extension; FP64 sums"},
+                           "derivative extension; each tree's evaluation replayed as a HIP "
+                           "graph, both trees launched before either is collected"},
+        "eager_evals_per_s": eager_rate,
+        "graph_results_identical_to_eager": graph_identical,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_pren4<float,true>: node-fused FP32 pre-order + derivative "
+                     "kernel": "k_pren4f<4,true,PF>: node-fused FP32 pre-order + derivative "
                                "traversal of the caterpillar (127 node jobs = 254 operations, "
                                "one launch)",
                      "kernel_ms": kernel_ms, "algorithmic_bytes": pre_bytes},
