@@ -307,6 +307,8 @@ template <typename R>
 struct Operand4 {
     R x[4];
     int st;
+    int e;        // pipelined post-order: the operand's scale exponent
+    double sr;    // pipelined post-order (first operand): the read-scale factor
 };
 
 template <typename R>
@@ -382,8 +384,29 @@ __device__ __forceinline__ void mat4(const R* m, const R (&x)[5], R (&t)[4]) {
                m[i * 5 + 3] * x[3] + m[i * 5 + 4] * x[4];
 }
 
-// Automatic (power-of-two) rescale inside the pipelined kernel: every wave
-// of the tile loads both children's exponents and stores the node's (the
+// One pipelined step's loads: both operands and, with automatic scaling,
+// their exponents (a partials operand's exponent array, else the dummy),
+// without it the read-scale factor (the dummy unless SCALE_READ); issued a
+// step ahead like the operands themselves.
+template <typename R, bool AUTO>
+__device__ __forceinline__ void post_issue(const PostJob& J, const void* dx, const int* ds,
+                                           const double* dD, const short* dE, long off,
+                                           long doff, int p, int lane, Operand4<R>& A,
+                                           Operand4<R>& B) {
+    issue4(J.x1, J.s1, J.k1, dx, ds, off, doff, p, lane, A);
+    issue4(J.x2, J.s2, J.k2, dx, ds, off, doff, p, lane, B);
+    if (AUTO) {
+        const bool l1 = J.k1 == 0 && J.e1 != nullptr, l2 = J.k2 == 0 && J.e2 != nullptr;
+        A.e = gin<short>(l1 ? J.e1 : dE)[l1 ? p : lane];
+        B.e = gin<short>(l2 ? J.e2 : dE)[l2 ? p : lane];
+    } else {
+        const bool r = J.mode == SCALE_READ;
+        A.sr = gin<double>(r ? J.scaleRead : dD)[r ? p : lane];
+    }
+}
+
+// Automatic (power-of-two) rescale inside the pipelined pre-order kernel:
+// every wave of the tile loads both exponents and stores the result's (the
 // same value from each wave), so the step's memory operations do not
 // depend on the wave or on which operands are tips.
 template <typename R>
@@ -399,13 +422,33 @@ __device__ __forceinline__ void auto_rescale4(R (&v)[4], R mx, const short* e1, 
     *gout<short>(dst) = (short)(e + (e1 ? x1 : 0) + (e2 ? x2 : 0));
 }
 
+// Automatic (power-of-two) rescale inside the pipelined post-order kernel:
+// every wave of the tile stores the node's exponent (the same value from
+// each wave).  A child's exponent is the prefetched one, the previous
+// step's output exponent (forwarded operand) or 0 (tip).
+template <typename R>
+__device__ __forceinline__ void post_auto_rescale4(R (&v)[4], R mx, const PostJob& J,
+                                              const Operand4<R>& A, const Operand4<R>& B,
+                                              int& prevE, const short* dummyE, int p, int lane,
+                                              bool in) {
+    const int e = mx > (R)0 ? exponent_of(mx) : 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = pow2(v[i], -e);
+    const int x1 = J.k1 == 2 ? prevE : (J.e1 ? A.e : 0);
+    const int x2 = J.k2 == 2 ? prevE : (J.e2 ? B.e : 0);
+    const int out = e + x1 + x2;
+    short* dst = in ? J.ed + p : const_cast<short*>(dummyE) + 64 + lane;
+    *gout<short>(dst) = (short)out;
+    prevE = out;
+}
+
 template <typename R, bool AUTO>
 __device__ __forceinline__ void post_compute(const PostJob& J, const Operand4<R>& A,
-                                             const Operand4<R>& B, R (&prev)[4],
+                                             const Operand4<R>& B, R (&prev)[4], int& prevE,
                                              const R* __restrict__ mats, R (*smax)[8][TILE],
                                              int& k, int c, int lane, int p, int P, int C,
                                              long off, long doff, R* dummyW,
-                                             const double* dummyD, const short* dummyE) {
+                                             const short* dummyE) {
     const bool in = p < P && p >= J.p0 && p < J.p1;
     R xa[5], xb[5], a[4], b[4], v[4];
     operand4(J.k1, A, prev, xa);
@@ -414,17 +457,15 @@ __device__ __forceinline__ void post_compute(const PostJob& J, const Operand4<R>
     mat4(mats + J.mo2 + c * 20, xb, b);
 #pragma unroll
     for (int i = 0; i < 4; ++i) v[i] = a[i] * b[i];
-    // the read-scale factor is loaded whether or not it is used
-    const double* srp = J.mode == SCALE_READ ? J.scaleRead + p : dummyD + lane;
-    const R sr = (R)*gin<double>(srp);
-    if (J.mode == SCALE_READ) {
+    if (!AUTO && J.mode == SCALE_READ) {
+        const R sr = (R)A.sr;
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[i] = v[i] / sr;
     }
     if (AUTO) {
         const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
         const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
-        auto_rescale4(v, mx, J.e1, J.e2, J.ed, dummyE, p, lane, in);
+        post_auto_rescale4(v, mx, J, A, B, prevE, dummyE, p, lane, in);
     } else if (J.mode == SCALE_WRITE) {
         const R m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
         const R mx = tile_max<R>(in ? m : (R)0, smax, k, c, lane, C);
@@ -456,26 +497,23 @@ __global__ __launch_bounds__(512) void k_post4(const PostJob* __restrict__ jobs,
     const int p = blockIdx.x * TILE + lane;
     const long off = c * cs + (long)p * 4;
     const long doff = (long)(c * TILE + lane) * 4;
-    int k = 0;
+    int k = 0, prevE = 0;
     R prev[4] = {0, 0, 0, 0};
     Operand4<R> A0, B0, A1, B1;
     PostJob J0 = jobs[0], J1;
-    issue4(J0.x1, J0.s1, J0.k1, dummyX, dummyS, off, doff, p, lane, A0);
-    issue4(J0.x2, J0.s2, J0.k2, dummyX, dummyS, off, doff, p, lane, B0);
+    post_issue<R, AUTO>(J0, dummyX, dummyS, dummyD, dummyE, off, doff, p, lane, A0, B0);
     for (int o = 0;; o += 2) {
         // operands of the next step; past the end the last operation's are
         // reloaded (in bounds, unused) so every step issues the same loads
         J1 = jobs[min(o + 1, njobs - 1)];
-        issue4(J1.x1, J1.s1, J1.k1, dummyX, dummyS, off, doff, p, lane, A1);
-        issue4(J1.x2, J1.s2, J1.k2, dummyX, dummyS, off, doff, p, lane, B1);
-        post_compute<R, AUTO>(J0, A0, B0, prev, mats, smax, k, c, lane, p, P, C, off, doff, dummyW,
-                              dummyD, dummyE);
+        post_issue<R, AUTO>(J1, dummyX, dummyS, dummyD, dummyE, off, doff, p, lane, A1, B1);
+        post_compute<R, AUTO>(J0, A0, B0, prev, prevE, mats, smax, k, c, lane, p, P, C, off, doff,
+                              dummyW, dummyE);
         if (o + 1 >= njobs) break;
         J0 = jobs[min(o + 2, njobs - 1)];
-        issue4(J0.x1, J0.s1, J0.k1, dummyX, dummyS, off, doff, p, lane, A0);
-        issue4(J0.x2, J0.s2, J0.k2, dummyX, dummyS, off, doff, p, lane, B0);
-        post_compute<R, AUTO>(J1, A1, B1, prev, mats, smax, k, c, lane, p, P, C, off, doff, dummyW,
-                              dummyD, dummyE);
+        post_issue<R, AUTO>(J0, dummyX, dummyS, dummyD, dummyE, off, doff, p, lane, A0, B0);
+        post_compute<R, AUTO>(J1, A1, B1, prev, prevE, mats, smax, k, c, lane, p, P, C, off, doff,
+                              dummyW, dummyE);
         if (o + 2 >= njobs) break;
     }
 }
