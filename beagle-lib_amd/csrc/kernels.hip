@@ -1127,37 +1127,53 @@ __device__ __forceinline__ fvec4 column(const float* m, int j) {
     return *(const fvec4*)(m + 4 * j);
 }
 
-// t = M x for a child: its partials (a combination of the four columns),
-// or for a tip the column of its state (s in [0, 4], a per-lane read;
-// column 4 is the pad column of a missing state).
-__device__ __forceinline__ void child_product(bool tip, int s, const float (&x)[4],
-                                              const float* m, float (&t)[4]) {
-    fvec4 r;
-    if (tip) {
-        r = column(m, s);
-    } else {
-        r = column(m, 0) * x[0];
-        r += column(m, 1) * x[1];
-        r += column(m, 2) * x[2];
-        r += column(m, 3) * x[3];
-    }
+// 4x4 products on the matrix cores, lane = pattern: v_mfma_f32_4x4x1f32
+// treats the wave as 16 blocks of 4 lanes; in instruction k lane l supplies
+// A[l % 4][k] and B[k][l % 4] of block l / 4, and element r of lane l's
+// accumulator is D[r][l % 4] of that block.  With B[k][.] = x[k] of the
+// lane's own pattern and A[r][k] = M[r][k] (the same matrix in every block),
+// lane l ends with (M x)[0..3] for its pattern after four instructions.
+// r = lane % 4.
+
+// t = M x: A from row r of M, M[r][k] = column k's entry r
+__device__ __forceinline__ void mfma_rows(const float* m, int r, const float (&x)[4],
+                                          float (&t)[4]) {
+    fvec4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int i = 0; i < 4; ++i) t[i] = r[i];
+    for (int k = 0; k < 4; ++k) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m[4 * k + r], x[k], acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t[i] = acc[i];
 }
 
-// t_j = sum_i M[i][j] u_i = column j . u
-__device__ __forceinline__ void mat4t_cols(const float* m, const float (&u)[4], float (&t)[4]) {
+// t = M^T u: A[r][k] = M[k][r], column r of M
+__device__ __forceinline__ void mfma_cols(const float* m, int r, const float (&u)[4],
+                                          float (&t)[4]) {
+    const fvec4 col = column(m, r);
+    fvec4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const fvec4 c = column(m, j);
-        t[j] = c[0] * u[0] + c[1] * u[1] + c[2] * u[2] + c[3] * u[3];
+    for (int k = 0; k < 4; ++k) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(col[k], u[k], acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t[i] = acc[i];
+}
+
+// t = M x for a child: its partials (matrix cores), or for a tip the
+// column of its state (s in [0, 4], a per-lane read; column 4 is the pad
+// column of a missing state).
+__device__ __forceinline__ void child_product(bool tip, int s, int r, const float (&x)[4],
+                                              const float* m, float (&t)[4]) {
+    if (tip) {
+        const fvec4 c = column(m, s);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) t[i] = c[i];
+    } else {
+        mfma_rows(m, r, x, t);
     }
 }
 
 // Derivative numerators and denominator of one branch for one category:
 // den = pre . x (a tip: pre[state & 3], as the reference computes it),
 // n1 = pre . (D1 x), n2 = pre . (D2 x).
-__device__ __forceinline__ void branch_sums_f32(bool tip, int s, const float (&pre)[4],
+__device__ __forceinline__ void branch_sums_f32(bool tip, int s, int r, const float (&pre)[4],
                                                 const float (&x)[4], const float* d1,
                                                 const float* d2, float& den, float& n1,
                                                 float& n2) {
@@ -1170,9 +1186,9 @@ __device__ __forceinline__ void branch_sums_f32(bool tip, int s, const float (&p
     } else {
         den = dot4(pre, x);
     }
-    child_product(tip, s, x, d1, t);
+    child_product(tip, s, r, x, d1, t);
     n1 = dot4(pre, t);
-    child_product(tip, s, x, d2, t);
+    child_product(tip, s, r, x, d2, t);
     n2 = dot4(pre, t);
 }
 
@@ -1205,18 +1221,19 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    const int r = lane & 3;
     float t[4], ua[4], ub[4], va[4], vb[4];
-    child_product(tipA, sa, o.xa, lm, t);          // P_a x_a
+    child_product(tipA, sa, r, o.xa, lm, t);       // P_a x_a
 #pragma unroll
     for (int i = 0; i < 4; ++i) ub[i] = u[i] * t[i];
-    child_product(tipB, sb, o.xb, lm + 20, t);     // P_b x_b
+    child_product(tipB, sb, r, o.xb, lm + 20, t);  // P_b x_b
 #pragma unroll
     for (int i = 0; i < 4; ++i) ua[i] = u[i] * t[i];
-    mat4t_cols(lm, ua, va);                        // pre(a) = P_a^T (pre(v) * P_b x_b)
-    mat4t_cols(lm + 20, ub, vb);                   // pre(b) = P_b^T (pre(v) * P_a x_a)
+    mfma_cols(lm, r, ua, va);                      // pre(a) = P_a^T (pre(v) * P_b x_b)
+    mfma_cols(lm + 20, r, ub, vb);                 // pre(b) = P_b^T (pre(v) * P_a x_a)
     float dA, n1A, n2A, dB, n1B, n2B;
-    branch_sums_f32(tipA, sa, va, o.xa, lm + 40, lm + 60, dA, n1A, n2A);
-    branch_sums_f32(tipB, sb, vb, o.xb, lm + 80, lm + 100, dB, n1B, n2B);
+    branch_sums_f32(tipA, sa, r, va, o.xa, lm + 40, lm + 60, dA, n1A, n2A);
+    branch_sums_f32(tipB, sb, r, vb, o.xb, lm + 80, lm + 100, dB, n1B, n2B);
     // exchange: max a, max b, den/n1/n2 of a, of b
     float* buf = sx + par * 8 * C * TILE;
     ++k;
