@@ -1192,7 +1192,35 @@ __device__ __forceinline__ void branch_sums_f32(bool tip, int s, int r, const fl
     n2 = dot4(pre, t);
 }
 
-template <int C, bool AUTO>
+// Operand loads of an FP32 node job without the dummy tile: uniform
+// branches skip what the job does not read (a forwarded parent, the
+// partials of a tip child, the states of a partials child).  Used where no
+// later job's loads are in flight, so the branches cost no extra waits.
+template <bool AUTO>
+__device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float* __restrict__ jm,
+                                                  int c, unsigned off, int p, int lane,
+                                                  NodeOps4<float>& o) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o.u[i] = o.xa[i] = o.xb[i] = 0.0f;
+    o.sa = o.sb = 0;
+    o.eu = o.ea = o.eb = 0;
+    if (J.kPar == 0) load4(at<float>(J.parent, off), o.u);
+    if (J.k[0] == 0) load4(at<float>(J.x[0], off), o.xa);
+    else o.sa = *at<int>(J.x[0], 4u * p);
+    if (J.k[1] == 0) load4(at<float>(J.x[1], off), o.xb);
+    else o.sb = *at<int>(J.x[1], 4u * p);
+    {
+        const int l = min(lane, NODE_MATS / 4 - 1);
+        o.mv = *at<fvec4>(jm, (unsigned)((J.mbase + c * NODE_MATS + l * 4) * (int)sizeof(float)));
+    }
+    if (AUTO) {
+        if (J.kPar == 0 && J.eParent != nullptr) o.eu = *at<short>(J.eParent, 2u * p);
+        if (J.ex[0] != nullptr) o.ea = *at<short>(J.ex[0], 2u * p);
+        if (J.ex[1] != nullptr) o.eb = *at<short>(J.ex[1], 2u * p);
+    }
+}
+
+template <int C, bool AUTO, bool DIRECT>
 __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps4<float>& o,
                                                  float (&pa)[4], float (&pb)[4], int& pea,
                                                  int& peb, float* smat, float* sx, int& k, int c,
@@ -1268,8 +1296,13 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
         }
         outA = ea + eu + (J.ex[1] ? o.eb : 0);
         outB = eb + eu + (J.ex[0] ? o.ea : 0);
-        *at_w<short>(keepA ? (void*)J.ed[0] : dummyEw, keepA ? 2u * p : 2u * lane) = (short)outA;
-        *at_w<short>(keepB ? (void*)J.ed[1] : dummyEw, keepB ? 2u * p : 2u * lane) = (short)outB;
+        if (DIRECT) {
+            if (keepA) *at_w<short>(J.ed[0], 2u * p) = (short)outA;
+            if (keepB) *at_w<short>(J.ed[1], 2u * p) = (short)outB;
+        } else {
+            *at_w<short>(keepA ? (void*)J.ed[0] : dummyEw, keepA ? 2u * p : 2u * lane) = (short)outA;
+            *at_w<short>(keepB ? (void*)J.ed[1] : dummyEw, keepB ? 2u * p : 2u * lane) = (short)outB;
+        }
     }
     // tile sums of child ch (w r1, w r1^2, w r2, r = N / D over categories)
     // by wave (ch + 2 par) % C
@@ -1296,8 +1329,13 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
             pt[(row + 2) * ntiles + tile] = (double)s2;
         }
     }
-    store4(at_w<float>(keepA ? J.dest[0] : (void*)dummyW, keepA ? off : doff), va);
-    store4(at_w<float>(keepB ? J.dest[1] : (void*)dummyW, keepB ? off : doff), vb);
+    if (DIRECT) {
+        if (keepA) store4(at_w<float>(J.dest[0], off), va);
+        if (keepB) store4(at_w<float>(J.dest[1], off), vb);
+    } else {
+        store4(at_w<float>(keepA ? J.dest[0] : (void*)dummyW, keepA ? off : doff), va);
+        store4(at_w<float>(keepB ? J.dest[1] : (void*)dummyW, keepB ? off : doff), vb);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         pa[i] = va[i];
@@ -1347,16 +1385,17 @@ __global__ __launch_bounds__(64 * C, PF == 0 ? 7 : 1) void k_pren4f(const NodeJo
 #define NODE_F32_STEP(JC, SC, JN, SN, I)                                                         \
     JN = jobs[min(o + (I) + PF, njobs - 1)];                                                      \
     node_issue<float, AUTO>(JN, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, SN);          \
-    node_compute_f32<C, AUTO>(JC, SC, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, doff,   \
-                              dummyW, dummyEw, wc, w, partial, tile, ntiles);                    \
+    node_compute_f32<C, AUTO, false>(JC, SC, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,  \
+                                     doff, dummyW, dummyEw, wc, w, partial, tile, ntiles);       \
     if (o + (I) + 1 >= njobs) break;
     if (PF == 0) {
         for (int o = 0; o < njobs; ++o) {
             const NodeJob J = jobs[o];
             NodeOps4<float> S0;
-            node_issue<float, AUTO>(J, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
-            node_compute_f32<C, AUTO>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
-                                      doff, dummyW, dummyEw, wc, w, partial, tile, ntiles);
+            node_issue_direct<AUTO>(J, jm, c, off, p, lane, S0);
+            node_compute_f32<C, AUTO, true>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P,
+                                            off, doff, dummyW, dummyEw, wc, w, partial, tile,
+                                            ntiles);
         }
     } else if (PF == 1) {
         NodeOps4<float> S0, S1;
@@ -1392,9 +1431,9 @@ __global__ __launch_bounds__(64 * C) void k_pren4f_level(const NodeJob* __restri
     NODE_F32_SETUP
     NodeOps4<float> S0;
     const NodeJob J = jobs[blockIdx.y];
-    node_issue<float, AUTO>(J, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
-    node_compute_f32<C, AUTO>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, doff,
-                              dummyW, dummyEw, wc, w, partial, tile, ntiles);
+    node_issue_direct<AUTO>(J, jm, c, off, p, lane, S0);
+    node_compute_f32<C, AUTO, true>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, doff,
+                                    dummyW, dummyEw, wc, w, partial, tile, ntiles);
 }
 #undef NODE_F32_SETUP
 
