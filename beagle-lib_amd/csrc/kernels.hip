@@ -299,6 +299,20 @@ __device__ __forceinline__ void rescale4(R (&v)[4], R mx, int c, int p, double* 
     }
 }
 
+// One 16-byte vector of reals.
+template <typename R>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef fvec4 T;
+    static constexpr int N = 4;
+};
+template <>
+struct Vec16<double> {
+    typedef dvec2 T;
+    static constexpr int N = 2;
+};
+
 // A 4-state operand as held in registers between its load and its use:
 // partials values, or a tip state (expanded at use to a one-hot vector whose
 // fifth entry selects the matrix pad column, so a missing state reads the
@@ -309,6 +323,8 @@ struct Operand4 {
     int st;
     int e;        // pipelined post-order: the operand's scale exponent
     double sr;    // pipelined post-order (first operand): the read-scale factor
+    typename Vec16<R>::T mv;   // pipelined post-order (first operand): this
+                               // lane's 16 bytes of the step's two matrices
 };
 
 template <typename R>
@@ -388,6 +404,19 @@ __device__ __forceinline__ void mat4(const R* m, const R (&x)[5], R (&t)[4]) {
 // their exponents (a partials operand's exponent array, else the dummy),
 // without it the read-scale factor (the dummy unless SCALE_READ); issued a
 // step ahead like the operands themselves.
+// The step's two 4 x 5 matrices for category c (20 reals each) arrive as
+// one 16-byte vector per lane: lanes [0, n) the first matrix, [n, 2n) the
+// second, the rest repeat the last vector (n = 20 reals / 16 bytes).
+template <typename R>
+__device__ __forceinline__ void post_issue_mats(const PostJob& J, const R* __restrict__ mats,
+                                                int c, int lane, Operand4<R>& A) {
+    constexpr int V = Vec16<R>::N, n = 20 / V;
+    const int l = min(lane, 2 * n - 1);
+    const int mo = l < n ? J.mo1 : J.mo2;
+    A.mv = *reinterpret_cast<const GLOBAL typename Vec16<R>::T*>(
+        gin<R>(mats) + mo + c * 20 + (l < n ? l : l - n) * V);
+}
+
 template <typename R, bool AUTO>
 __device__ __forceinline__ void post_issue(const PostJob& J, const void* dx, const int* ds,
                                            const double* dD, const short* dE, long off,
@@ -445,7 +474,7 @@ __device__ __forceinline__ void post_auto_rescale4(R (&v)[4], R mx, const PostJo
 template <typename R, bool AUTO>
 __device__ __forceinline__ void post_compute(const PostJob& J, const Operand4<R>& A,
                                              const Operand4<R>& B, R (&prev)[4], int& prevE,
-                                             const R* __restrict__ mats, R (*smax)[8][TILE],
+                                             R* smat, R (*smax)[8][TILE],
                                              int& k, int c, int lane, int p, int P, int C,
                                              long off, long doff, R* dummyW,
                                              const short* dummyE) {
@@ -453,8 +482,18 @@ __device__ __forceinline__ void post_compute(const PostJob& J, const Operand4<R>
     R xa[5], xb[5], a[4], b[4], v[4];
     operand4(J.k1, A, prev, xa);
     operand4(J.k2, B, prev, xb);
-    mat4(mats + J.mo1 + c * 20, xa, a);
-    mat4(mats + J.mo2 + c * 20, xb, b);
+    // the wave's own block of the matrices: written by the lanes that
+    // carry them, read back as broadcasts
+    {
+        constexpr int V = Vec16<R>::N;
+        R* lm = smat + c * 40;
+        *reinterpret_cast<typename Vec16<R>::T*>(lm + min(lane, 40 / V - 1) * V) = A.mv;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        mat4(lm, xa, a);
+        mat4(lm + 20, xb, b);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) v[i] = a[i] * b[i];
     if (!AUTO && J.mode == SCALE_READ) {
@@ -487,6 +526,7 @@ __global__ __launch_bounds__(512) void k_post4(const PostJob* __restrict__ jobs,
                                                const int* dummyS, R* dummyW,
                                                const double* dummyD, const short* dummyE) {
     __shared__ R smax[2][8][TILE];
+    __shared__ __attribute__((aligned(16))) R smat[8 * 40];   // [category][2 matrices][20]
     // workgroup (tile, y) walks operations [y chunk, (y + 1) chunk)
     jobs += blockIdx.y * chunk;
     njobs = min(chunk, njobs - (int)blockIdx.y * chunk);
@@ -502,17 +542,20 @@ __global__ __launch_bounds__(512) void k_post4(const PostJob* __restrict__ jobs,
     Operand4<R> A0, B0, A1, B1;
     PostJob J0 = jobs[0], J1;
     post_issue<R, AUTO>(J0, dummyX, dummyS, dummyD, dummyE, off, doff, p, lane, A0, B0);
+    post_issue_mats(J0, mats, c, lane, A0);
     for (int o = 0;; o += 2) {
         // operands of the next step; past the end the last operation's are
         // reloaded (in bounds, unused) so every step issues the same loads
         J1 = jobs[min(o + 1, njobs - 1)];
         post_issue<R, AUTO>(J1, dummyX, dummyS, dummyD, dummyE, off, doff, p, lane, A1, B1);
-        post_compute<R, AUTO>(J0, A0, B0, prev, prevE, mats, smax, k, c, lane, p, P, C, off, doff,
+        post_issue_mats(J1, mats, c, lane, A1);
+        post_compute<R, AUTO>(J0, A0, B0, prev, prevE, smat, smax, k, c, lane, p, P, C, off, doff,
                               dummyW, dummyE);
         if (o + 1 >= njobs) break;
         J0 = jobs[min(o + 2, njobs - 1)];
         post_issue<R, AUTO>(J0, dummyX, dummyS, dummyD, dummyE, off, doff, p, lane, A0, B0);
-        post_compute<R, AUTO>(J1, A1, B1, prev, prevE, mats, smax, k, c, lane, p, P, C, off, doff,
+        post_issue_mats(J0, mats, c, lane, A0);
+        post_compute<R, AUTO>(J1, A1, B1, prev, prevE, smat, smax, k, c, lane, p, P, C, off, doff,
                               dummyW, dummyE);
         if (o + 2 >= njobs) break;
     }
@@ -791,20 +834,6 @@ __global__ __launch_bounds__(512) void k_pre4_level(const PreJob* __restrict__ j
 // the dummy tile, stores of lanes past P or of outputs that are not kept go
 // to the dummy tile.  With automatic scaling the exponents are loaded with
 // the operands, one job ahead, and the pattern weight is loaded once.
-
-// One 16-byte vector of reals.
-template <typename R>
-struct Vec16;
-template <>
-struct Vec16<float> {
-    typedef fvec4 T;
-    static constexpr int N = 4;
-};
-template <>
-struct Vec16<double> {
-    typedef dvec2 T;
-    static constexpr int N = 2;
-};
 
 // A node job's six matrices for one category (P_a, P_b, D1_a, D2_a, D1_b,
 // D2_b; 20 reals each, contiguous after launch_gather_node_matrices) are
