@@ -391,6 +391,49 @@ def grad_err(a, b):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
 
 
+def calibrate_copy():
+    """A plain 1 GiB -> 1 GiB elementwise copy kernel, the reference size
+    for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings."""
+    import torch
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def roofline_timing(ev, stream, prob, steps):
+    """The dominant kernel alone: the caterpillar's fused pre-order launch
+    bracketed by HIP events on the library's stream, through the extension
+    with no outputs (no reduction, copy-back or host wait inside the
+    bracket; the job upload and the matrix gather, a few microseconds, are).
+    The post-order traversal is enqueued just before, without waiting for
+    it, so the host's preparation of the launch overlaps device work and the
+    first event is not stamped ahead of it.  Returns (mean ms per launch,
+    algorithmic GB/s, algorithmic bytes per launch)."""
+    import numpy as np
+    import torch
+    ops, post, _, _ = ev.fused_operations()
+    post_idx = dict(zip(map(tuple, ops.tolist()), post.tolist()))
+    pre_bytes = preorder_bytes(prob, ops, lambda o: post_idx[tuple(o.tolist())], 4, True)
+    pairs = []
+    for _ in range(max(steps, 10)):
+        ev.update_matrices()
+        ev.post_order()
+        ev.set_root_pre()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ev.inst.pre_partials_with_derivatives(ops, post, [ev.d1_index] * len(post),
+                                              [ev.d2_index] * len(post), outputs=False)
+        e1.record(stream)
+        pairs.append((e0, e1))
+    torch.cuda.synchronize()
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
+    return kernel_ms, pre_bytes / (kernel_ms * 1e-3) / 1e9, pre_bytes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -399,8 +442,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
-    ap.add_argument("--only", choices=("codon", "partitions"),
-                    help="run one auxiliary workload alone (for profiling)")
+    ap.add_argument("--only", choices=("codon", "partitions", "roofline"),
+                    help="run one workload alone (for profiling): the codon or partition "
+                         "workload, or the roofline timing of the dominant kernel")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB -> 1 GiB elementwise copy kernel, the "
                          "reference size for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -432,6 +476,18 @@ def main():
         return
     probs = problems()
     asyn = api.FLAG_COMPUTATION_ASYNCH
+    if args.only == "roofline":
+        ev = driver.Evaluator(lib, probs[0], precision="single", scaling="auto", extra_flags=asyn)
+        for _ in range(3):
+            fused_eval(ev)
+        kernel_ms, achieved, pre_bytes = roofline_timing(
+            ev, torch.cuda.ExternalStream(ev.inst.stream()), probs[0], args.steps)
+        ev.close()
+        if args.calibrate_copy:
+            calibrate_copy()
+        print(json.dumps({"roofline": {"kernel_ms": kernel_ms, "achieved": achieved,
+                                       "algorithmic_bytes": pre_bytes}}))
+        return
 
     # ---- value: FP32 fused logL + gradient, both trees ----
     evs = [driver.Evaluator(lib, p, precision="single", scaling="auto", extra_flags=asyn)
@@ -481,32 +537,7 @@ def main():
     value = 2 * args.steps / dt
     graph_identical = all(a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
                           for a, b in zip(res32, eager32))
-    cat_ops, cat_post, _, _ = evs[0].fused_operations()
-    post_idx = dict(zip(map(tuple, cat_ops.tolist()), cat_post.tolist()))
-    pre_bytes = preorder_bytes(probs[0], cat_ops, lambda o: post_idx[tuple(o.tolist())], 4, True)
-    # the dominant kernel alone: the caterpillar's fused pre-order launch
-    # bracketed by events, through the extension with no outputs (no
-    # reduction, copy-back or host wait inside the bracket; the job upload
-    # and the matrix gather, a few microseconds, are).  The post-order
-    # traversal is enqueued just before, without waiting for it, so the
-    # host's preparation of the launch overlaps device work and the first
-    # event is not stamped ahead of it.
-    ev = evs[0]
-    pairs = []
-    for _ in range(max(args.steps, 10)):
-        ev.update_matrices()
-        ev.post_order()
-        ev.set_root_pre()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(streams[0])
-        ev.inst.pre_partials_with_derivatives(cat_ops, cat_post, [ev.d1_index] * len(cat_post),
-                                              [ev.d2_index] * len(cat_post), outputs=False)
-        e1.record(streams[0])
-        pairs.append((e0, e1))
-    torch.cuda.synchronize()
-    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
-    achieved = pre_bytes / (kernel_ms * 1e-3) / 1e9
+    kernel_ms, achieved, pre_bytes = roofline_timing(evs[0], streams[0], probs[0], args.steps)
     for ev in evs:
         ev.close()
 
@@ -560,12 +591,7 @@ def main():
     partitions = partitions_bench(lib, max(args.steps // 10, 3))
 
     if args.calibrate_copy:
-        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.mul(src, 1.0, out=dst)
-        torch.cuda.synchronize()
-        del src, dst
+        calibrate_copy()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -578,7 +604,7 @@ def main():
             print(r.stderr[-2000:], file=sys.stderr)
 
     traffic = None
-    tpath = os.path.join(REPO, "profiles", "pren4_fp32_caterpillar_traffic.json")
+    tpath = os.path.join(REPO, "profiles", "pren4f_fp32_caterpillar_traffic.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
 
@@ -608,7 +634,7 @@ def main():
         "graph_results_identical_to_eager": graph_identical,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_pren4f<4,true,PF>: node-fused FP32 pre-order + derivative "
+                     "kernel": "k_pren4f<4,true,0>: node-fused FP32 pre-order + derivative "
                                "traversal of the caterpillar (127 node jobs = 254 operations, "
                                "one launch)",
                      "kernel_ms": kernel_ms, "algorithmic_bytes": pre_bytes},
