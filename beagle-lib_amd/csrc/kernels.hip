@@ -61,6 +61,22 @@ __device__ __forceinline__ void store4(GLOBAL double* p, const double (&v)[4]) {
 __device__ __forceinline__ void store4(GLOBAL float* p, const float (&v)[4]) {
     *(GLOBAL fvec4*)p = fvec4{v[0], v[1], v[2], v[3]};
 }
+// The 64 lanes' 4-state FP64 vectors of 64 consecutive patterns (lane l
+// holds pattern l; t points at pattern 0) stored as two fully contiguous
+// 1 KB writes instead of two writes of 16-byte halves 32 bytes apart:
+// write h covers patterns [32 h, 32 h + 32), lane l writing half (l & 1)
+// of pattern 32 h + l / 2, fetched from that pattern's lane.  All 64 lanes
+// must be active.
+__device__ __forceinline__ void store4_tile(GLOBAL double* t, const double (&v)[4], int lane) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int src = 32 * h + (lane >> 1);
+        const double a = __shfl(v[0], src, 64), b = __shfl(v[1], src, 64);
+        const double c = __shfl(v[2], src, 64), d = __shfl(v[3], src, 64);
+        *(GLOBAL dvec2*)(t + 128 * h + 2 * lane) = (lane & 1) ? dvec2{c, d} : dvec2{a, b};
+    }
+}
+
 template <typename R>
 __device__ __forceinline__ void store4(R* p, const R (&v)[4]);
 template <>
@@ -512,9 +528,16 @@ __device__ __forceinline__ void post_compute(const PostJob& J, const Operand4<R>
             rescale4<R, false>(v, mx, c, p, J.scaleWrite, J.logScalers, J.cumulative, J.accA,
                                J.accB, J.e1, J.e2, J.ed);
     }
-    // lanes outside the operation's pattern range store into a scratch tile
-    R* dst = in ? reinterpret_cast<R*>(J.dest) + off : dummyW + doff;
-    store4(gout<R>(dst), v);
+    // FP64 tiles wholly inside the operation's pattern range: contiguous
+    // stores; otherwise lanes outside the range store into a scratch tile
+    const int t0 = p - lane;
+    if (sizeof(R) == 8 && t0 >= J.p0 && t0 + TILE <= min(J.p1, P)) {
+        store4_tile(gout<double>(reinterpret_cast<double*>(J.dest) + (off - 4 * lane)),
+                    reinterpret_cast<const double(&)[4]>(v), lane);
+    } else {
+        R* dst = in ? reinterpret_cast<R*>(J.dest) + off : dummyW + doff;
+        store4(gout<R>(dst), v);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) prev[i] = v[i];
 }
