@@ -144,6 +144,8 @@ SIGNATURES = {
     "ExtGetStream": (_I, [_I, ctypes.POINTER(ctypes.c_void_p)]),
     "ExtSetSplitBF16": (_I, [_I, _I]),
     "ExtSetAutotune": (_I, [_I, _I]),
+    "ExtCheckpointedGradient": (_I, [_I, _IP, _I, _IP, _IP, _I, _I, _IP, _I, _I, _IP, _IP, _DP,
+                                     _DP, _DP, _DP]),
     "ExtBeginCapture": (_I, [_I]),
     "ExtEndCapture": (_I, [_I, ctypes.POINTER(ctypes.c_int)]),
     "ExtLaunchGraph": (_I, [_I, _I]),
@@ -483,6 +485,31 @@ class Instance:
                 s1.ctypes.data_as(c_double_p), sq.ctypes.data_as(c_double_p),
                 s2.ctypes.data_as(c_double_p))
         return float(lnl[0]), s1, sq, s2
+
+    def checkpointed_gradient(self, ops, d1, d2, buffers, interval=0, weights_index=0,
+                              freqs_index=0, plan_only=False):
+        """Extension: logL and per-branch derivative sums of one tree using
+        only the partials buffers in `buffers` (post-order partials kept at
+        checkpoint heights, the rest recomputed).  ops: post-order operations
+        over node ids; d1, d2: derivative matrix index per node id.  Returns
+        (logL, s1, sq, s2, interval, peak buffers), the sums indexed by node
+        id; plan_only: (None, None, None, None, interval, peak)."""
+        a = _ip(np.asarray(ops).reshape(-1))
+        n = len(a[0]) // 7
+        size = int(np.asarray(ops).reshape(-1, 7)[:, [0, 3, 5]].max()) + 1
+        b, c, p = _ip(d1), _ip(d2), _ip(buffers)
+        k, peak = ctypes.c_int(0), ctypes.c_int(0)
+        if plan_only:
+            self._c("ExtCheckpointedGradient", a[1], n, b[1], c[1], weights_index, freqs_index,
+                    p[1], len(p[0]), int(interval), ctypes.byref(k), ctypes.byref(peak),
+                    None, None, None, None)
+            return None, None, None, None, k.value, peak.value
+        lnl, s1, sq, s2 = np.zeros(1), np.zeros(size), np.zeros(size), np.zeros(size)
+        self._c("ExtCheckpointedGradient", a[1], n, b[1], c[1], weights_index, freqs_index,
+                p[1], len(p[0]), int(interval), ctypes.byref(k), ctypes.byref(peak),
+                lnl.ctypes.data_as(c_double_p), s1.ctypes.data_as(c_double_p),
+                sq.ctypes.data_as(c_double_p), s2.ctypes.data_as(c_double_p))
+        return float(lnl[0]), s1, sq, s2, k.value, peak.value
 
     def rate_matrix_gradient(self, eigen, ops, post, weights_index=0, root=None,
                              freqs_index=0):

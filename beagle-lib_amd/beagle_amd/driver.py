@@ -215,6 +215,29 @@ class Evaluator:
         nodes = self.branch_nodes()
         return logl, g1[nodes], g2[nodes]
 
+    def checkpointed_gradient(self, buffers, interval=0, lengths=None, plan_only=False):
+        """logL, dlogL/dt and d2logL/dt2 per branch (as fused_gradient())
+        through beagleExtCheckpointedGradient, using only the partials
+        buffers in `buffers`: post-order partials kept every `interval`
+        heights (0: the smallest interval that fits) and the rest recomputed.
+        Returns (logL, d1, d2, interval, peak buffers in use); plan_only:
+        (None, None, None, interval, peak)."""
+        if self.scaling in ("manual", "manual_log", "always"):
+            raise ValueError("the checkpointed gradient takes automatic scaling or none")
+        tree = self.prob.tree
+        ops = synth.postorder_ops(tree, None)
+        size = 2 * self.n - 1
+        d1 = np.full(size, self.d1_index, dtype=np.int32)
+        d2 = np.full(size, self.d2_index, dtype=np.int32)
+        if not plan_only:
+            self.update_matrices(lengths)
+        lnl, s1, sq, s2, k, peak = self.inst.checkpointed_gradient(
+            ops, d1, d2, buffers, interval, plan_only=plan_only)
+        if plan_only:
+            return None, None, None, k, peak
+        nodes = self.branch_nodes()
+        return lnl, s1[nodes], (s2 - sq)[nodes], k, peak
+
     def capture_fused_gradient(self, keep_tips=False):
         """Record one fused_gradient() evaluation (matrices from the
         eigen-system, post-order, root logL, fused derivative pass) as a HIP

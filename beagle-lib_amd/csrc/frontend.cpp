@@ -645,6 +645,25 @@ int beagleExtSetAutotune(int instance, int on) {
     return withInstance(instance, [&](Instance* in) { return in->setAutotune(on); });
 }
 
+int beagleExtCheckpointedGradient(int instance, const BeagleOperation* operations, int count,
+                                  const int* firstDerivativeIndices,
+                                  const int* secondDerivativeIndices,
+                                  int categoryWeightsIndex, int stateFrequenciesIndex,
+                                  const int* bufferIndices, int bufferCount, int interval,
+                                  int* outInterval, int* outPeakBuffers, double* outLogL,
+                                  double* outSumFirstDerivatives,
+                                  double* outSumSquaredFirstDerivatives,
+                                  double* outSumSecondDerivatives) {
+    return withInstance(instance, [&](Instance* in) {
+        return in->checkpointedGradient(reinterpret_cast<const int*>(operations), count,
+                                        firstDerivativeIndices, secondDerivativeIndices,
+                                        categoryWeightsIndex, stateFrequenciesIndex,
+                                        bufferIndices, bufferCount, interval, outInterval,
+                                        outPeakBuffers, outLogL, outSumFirstDerivatives,
+                                        outSumSquaredFirstDerivatives, outSumSecondDerivatives);
+    });
+}
+
 int beagleExtBeginCapture(int instance) {
     return withInstance(instance, [&](Instance* in) { return in->beginCapture(); });
 }

@@ -100,6 +100,10 @@ public:
                                     const int* ops, int count, const int* postIdx,
                                     const int* d1Idx, const int* d2Idx, double* outLogL,
                                     double* outD1, double* outD1Sq, double* outD2);
+    int checkpointedGradient(const int* ops, int count, const int* d1Idx, const int* d2Idx,
+                             int wIdx, int fIdx, const int* pool, int poolCount, int interval,
+                             int* outInterval, int* outPeak, double* outLogL, double* outD1,
+                             double* outD1Sq, double* outD2);
     int waitForPartials();
 
     int accumulateScaleFactors(const int* idx, int count, int cumIdx, double sign, int part);

@@ -38,6 +38,39 @@ int beagleExtSetSplitBF16(int instance, int terms);
  * plan.  on = 0 turns it off and forgets the choices. */
 int beagleExtSetAutotune(int instance, int on);
 
+/* Log-likelihood and per-branch derivative sums of one tree under a
+ * partials budget (no counterpart in the reference; the reference keeps
+ * every post-order and pre-order partials buffer).  `operations` is the
+ * tree's post-order traversal as for beagleUpdatePartials, with node ids in
+ * place of destination buffers: a tip is its tip buffer, every internal
+ * node an id >= the tip count that need not be a buffer; scale indices must
+ * be BEAGLE_OP_NONE and the last operation's destination is the root.
+ * firstDerivativeIndices / secondDerivativeIndices give, per node id, the
+ * derivative matrices of the branch above the node (r_c Q and r_c^2 Q^2).
+ * The call uses only the bufferCount partials buffers in bufferIndices:
+ * it keeps the post-order partials of internal nodes whose height (edges to
+ * the deepest tip below) is a multiple of `interval`, and of the root,
+ * recomputes the others from those below them when the pre-order pass
+ * reaches them, and holds each node's pre-order partials only between the
+ * jobs that write and read them.  interval = 0 picks the smallest interval
+ * whose schedule fits the buffers.  Outputs, per node id (the root's
+ * entries untouched): sum_p w_p L'/L, sum_p w_p (L'/L)^2 and sum_p w_p L''/L
+ * of the branch above the node; the log-likelihood; the interval used and
+ * the most buffers in use at once.  With all four result pointers null the
+ * call only plans.  BEAGLE_ERROR_GENERAL when no schedule fits the buffers;
+ * BEAGLE_ERROR_NO_IMPLEMENTATION with partitions, SCALING_ALWAYS or the
+ * transposed pre-order convention (S > 4 without PREORDER_TRANSPOSE_AUTO).
+ * The results equal those of a full evaluation of the same tree. */
+int beagleExtCheckpointedGradient(int instance, const BeagleOperation* operations, int count,
+                                  const int* firstDerivativeIndices,
+                                  const int* secondDerivativeIndices,
+                                  int categoryWeightsIndex, int stateFrequenciesIndex,
+                                  const int* bufferIndices, int bufferCount, int interval,
+                                  int* outInterval, int* outPeakBuffers, double* outLogL,
+                                  double* outSumFirstDerivatives,
+                                  double* outSumSquaredFirstDerivatives,
+                                  double* outSumSecondDerivatives);
+
 /* HIP-graph capture of an instance's calls (no counterpart in the
  * reference).  Between beagleExtBeginCapture and beagleExtEndCapture the
  * instance's calls are recorded into a graph instead of run:

@@ -742,3 +742,51 @@ def test_torch_autograd_binding(amd, oracle, precision):
             e[i] = 1.0
             fd = fd4(lambda s: call(x0 + s * e))
             assert abs(g[i] - fd) <= tol * max(abs(fd), np.abs(g).max()), (name, i, g[i], fd)
+
+
+# ---- gradient under a partials budget ----------------------------------------
+
+@pytest.mark.parametrize("precision,scaling,shape,tips_as",
+                         [("single", "auto", "caterpillar", "states"),
+                          ("single", "auto", "yule", "states"),
+                          ("double", "none", "balanced", "partials"),
+                          ("double", "none", "yule", "states")])
+def test_checkpointed_gradient_matches_full(amd, precision, scaling, shape, tips_as):
+    """beagleExtCheckpointedGradient keeps post-order partials every k
+    heights and recomputes the rest in the pre-order pass: for several k,
+    and for the k it picks for a pool far smaller than a full evaluation
+    uses, logL and every branch's dlogL/dt and d2logL/dt2 equal the fused
+    full evaluation's bit for bit; the peak buffer count it reports stays
+    within the pool, and a pool no schedule fits is refused."""
+    prob = synth.make_problem(71, 40, 2000, S=4, shape=shape, missing=0.03)
+    ev = driver.Evaluator(amd, prob, precision=precision, scaling=scaling, tips_as=tips_as)
+    full = ev.fused_gradient()
+    n = prob.tree.n
+    pool = list(range(ev.pre_base, ev.pre_base + 2 * n - 1))
+    for k in (1, 2, 5):
+        lnl, g1, g2, used, peak = ev.checkpointed_gradient(pool, k)
+        assert used == k and peak <= len(pool)
+        assert lnl == full[0]
+        np.testing.assert_array_equal(g1, full[1])
+        np.testing.assert_array_equal(g2, full[2])
+    # the smallest pool any interval fits, found by planning
+    best = None
+    for m in range(4, len(pool) + 1):
+        try:
+            _, _, _, k, peak = ev.checkpointed_gradient(pool[:m], 0, plan_only=True)
+        except api.BeagleError as e:
+            assert e.code == api.ERROR_GENERAL
+            continue
+        best = (m, k, peak)
+        break
+    assert best is not None and best[2] <= best[0] and best[0] < n - 1
+    m, k, _ = best
+    lnl, g1, g2, used, peak = ev.checkpointed_gradient(pool[:m], 0)
+    assert used == k and peak <= m
+    assert lnl == full[0]
+    np.testing.assert_array_equal(g1, full[1])
+    np.testing.assert_array_equal(g2, full[2])
+    with pytest.raises(api.BeagleError) as e:
+        ev.checkpointed_gradient(pool[:m - 1], 0)
+    assert e.value.code == api.ERROR_GENERAL
+    ev.close()
