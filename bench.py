@@ -391,6 +391,53 @@ def grad_err(a, b):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
 
 
+def checkpoint_bench(lib, probs, steps):
+    """FP32 logL + gradient of each benchmark tree through
+    beagleExtCheckpointedGradient with the smallest pool of partials
+    buffers any checkpoint interval fits (found with its planning call),
+    against the full fused evaluation (every post- and pre-order partials
+    buffer of the internal nodes held): buffers, bytes, evaluations/s and
+    whether the results are identical."""
+    import numpy as np
+    import torch
+    from beagle_amd import api, driver
+    out = {}
+    for prob, shape in zip(probs, SHAPES):
+        ev = driver.Evaluator(lib, prob, precision="single", scaling="auto")
+        n = prob.tree.n
+        pool = list(range(ev.pre_base, ev.pre_base + 2 * n - 1))
+        m = None
+        for size in range(4, len(pool) + 1):
+            try:
+                ev.checkpointed_gradient(pool[:size], 0, plan_only=True)
+            except api.BeagleError:
+                continue
+            m = size
+            break
+        full = ev.fused_gradient()
+        ck = ev.checkpointed_gradient(pool[:m], 0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ck = ev.checkpointed_gradient(pool[:m], 0)
+        rate = steps / (time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            full = ev.fused_gradient()
+        full_rate = steps / (time.perf_counter() - t0)
+        buf = prob.C * prob.P * prob.S * 4 + 2 * prob.P
+        full_bufs = 2 * (n - 1)    # internal post-order + internal pre-order partials
+        out[shape] = {
+            "pool_buffers": m, "interval": ck[3], "peak_buffers": ck[4],
+            "peak_partials_MB": ck[4] * buf / 1e6,
+            "full_evaluation_buffers": full_bufs, "full_partials_MB": full_bufs * buf / 1e6,
+            "evals_per_s": rate, "full_evals_per_s": full_rate,
+            "identical_to_full": bool(ck[0] == full[0] and np.array_equal(ck[1], full[1])
+                                      and np.array_equal(ck[2], full[2]))}
+        ev.close()
+    return out
+
+
 def calibrate_copy():
     """A plain 1 GiB -> 1 GiB elementwise copy kernel, the reference size
     for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings."""
@@ -587,6 +634,7 @@ def main():
     post_gbs = postorder_bytes(probs[0], 8) / (post_ms * 1e-3) / 1e9
     ev.close()
 
+    checkpointed = checkpoint_bench(lib, probs, max(args.steps // 3, 3))
     codon = codon_bench(lib, args.steps)
     partitions = partitions_bench(lib, max(args.steps // 10, 3))
 
@@ -652,6 +700,7 @@ def main():
             "postorder_kernel": "k_post4<double,false>: caterpillar, 127 operations, one launch"},
         "logL": [r[0] for r in api64],
         "cpu_baseline": cpu,
+        "checkpointed": checkpointed,
         "codon": codon,
         "partitions": partitions,
     }
