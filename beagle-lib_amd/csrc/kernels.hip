@@ -450,6 +450,29 @@ __device__ __forceinline__ void post_issue(const PostJob& J, const void* dx, con
     }
 }
 
+// t = M x for one operand of a pipelined post-order step, M the 4 x 5
+// block in LDS: a tip's state column (column 4, the pad column, for a
+// missing state), read with the state as the lane's index; otherwise M
+// times the loaded or forwarded partials, the pad column left out (it
+// multiplies 0).  The same values as the one-hot product.
+template <typename R>
+__device__ __forceinline__ void step_product(int kind, const Operand4<R>& o, const R (&prev)[4],
+                                             const R* m, R (&t)[4]) {
+    if (kind == 1) {
+        const int s = min((unsigned)o.st, 4u);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) t[i] = m[i * 5 + s];
+    } else {
+        R x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[j] = kind == 2 ? prev[j] : o.x[j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            t[i] = m[i * 5 + 0] * x[0] + m[i * 5 + 1] * x[1] + m[i * 5 + 2] * x[2] +
+                   m[i * 5 + 3] * x[3];
+    }
+}
+
 // Automatic (power-of-two) rescale inside the pipelined pre-order kernel:
 // every wave of the tile loads both exponents and stores the result's (the
 // same value from each wave), so the step's memory operations do not
@@ -495,11 +518,9 @@ __device__ __forceinline__ void post_compute(const PostJob& J, const Operand4<R>
                                              long off, long doff, R* dummyW,
                                              const short* dummyE) {
     const bool in = p < P && p >= J.p0 && p < J.p1;
-    R xa[5], xb[5], a[4], b[4], v[4];
-    operand4(J.k1, A, prev, xa);
-    operand4(J.k2, B, prev, xb);
+    R a[4], b[4], v[4];
     // the wave's own block of the matrices: written by the lanes that
-    // carry them, read back as broadcasts
+    // carry them, read back as broadcasts (a tip's state column per lane)
     {
         constexpr int V = Vec16<R>::N;
         R* lm = smat + c * 40;
@@ -507,8 +528,8 @@ __device__ __forceinline__ void post_compute(const PostJob& J, const Operand4<R>
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-        mat4(lm, xa, a);
-        mat4(lm + 20, xb, b);
+        step_product(J.k1, A, prev, lm, a);
+        step_product(J.k2, B, prev, lm + 20, b);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) v[i] = a[i] * b[i];
