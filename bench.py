@@ -438,6 +438,33 @@ def checkpoint_bench(lib, probs, steps):
     return out
 
 
+def plain_post_timing(lib, prob, steps):
+    """The plain path's FP64 post-order traversal of the caterpillar (one
+    fused launch of 127 operations, no scaling) bracketed by HIP events on
+    the library's stream.  Returns (mean ms, algorithmic GB/s)."""
+    import numpy as np
+    import torch
+    from beagle_amd import api, driver
+    ev = driver.Evaluator(lib, prob, preorder=False, extra_flags=api.FLAG_COMPUTATION_ASYNCH)
+    st = torch.cuda.ExternalStream(ev.inst.stream())
+    for _ in range(3):
+        ev.log_likelihood()
+    pairs = []
+    for _ in range(max(steps // 2, 5)):
+        ev.update_matrices()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        ev.post_order()
+        e1.record(st)
+        ev.root()
+        pairs.append((e0, e1))
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
+    ev.close()
+    return ms, postorder_bytes(prob, 8) / (ms * 1e-3) / 1e9
+
+
 def calibrate_copy():
     """A plain 1 GiB -> 1 GiB elementwise copy kernel, the reference size
     for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings."""
@@ -489,9 +516,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
-    ap.add_argument("--only", choices=("codon", "partitions", "roofline"),
+    ap.add_argument("--only", choices=("codon", "partitions", "roofline", "plain-post"),
                     help="run one workload alone (for profiling): the codon or partition "
-                         "workload, or the roofline timing of the dominant kernel")
+                         "workload, the roofline timing of the dominant kernel, or the "
+                         "plain path's FP64 post-order traversal")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB -> 1 GiB elementwise copy kernel, the "
                          "reference size for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -523,6 +551,13 @@ def main():
         return
     probs = problems()
     asyn = api.FLAG_COMPUTATION_ASYNCH
+    if args.only == "plain-post":
+        ms, gbs = plain_post_timing(lib, probs[0], args.steps)
+        if args.calibrate_copy:
+            calibrate_copy()
+        print(json.dumps({"plain_post": {"ms": ms, "GBps": gbs,
+                                         "algorithmic_bytes": postorder_bytes(probs[0], 8)}}))
+        return
     if args.only == "roofline":
         ev = driver.Evaluator(lib, probs[0], precision="single", scaling="auto", extra_flags=asyn)
         for _ in range(3):
@@ -613,26 +648,7 @@ def main():
     fused64_diff = max(max(grad_err(a[1], b[1]), grad_err(a[2], b[2]), rel(a[0], b[0]))
                        for a, b in zip(res64["fused"], api64))
 
-    # ---- plain FP64 post-order traversal bandwidth (caterpillar: one
-    # fused launch of 127 operations) ----
-    ev = driver.Evaluator(lib, probs[0], preorder=False, extra_flags=asyn)
-    st = torch.cuda.ExternalStream(ev.inst.stream())
-    for _ in range(3):
-        ev.log_likelihood()
-    pairs = []
-    for _ in range(max(args.steps // 2, 5)):
-        ev.update_matrices()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        ev.post_order()
-        e1.record(st)
-        ev.root()
-        pairs.append((e0, e1))
-    torch.cuda.synchronize()
-    post_ms = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
-    post_gbs = postorder_bytes(probs[0], 8) / (post_ms * 1e-3) / 1e9
-    ev.close()
+    post_ms, post_gbs = plain_post_timing(lib, probs[0], args.steps)
 
     checkpointed = checkpoint_bench(lib, probs, max(args.steps // 3, 3))
     codon = codon_bench(lib, args.steps)
