@@ -1609,16 +1609,11 @@ void Instance::launchNodePlan(const std::vector<NodeJob>& nodes, const std::vect
         const int first = (int)all.size();
         for (int j : s.ops) all.push_back(nodes[j]);
         if (!s.batched) {
-            const bool lookahead2 = !dp_ && node_f32_supported(dims_.C);
             for (size_t j = first + 1; j < all.size(); ++j) {
                 NodeJob& J = all[j];
                 const NodeJob& Pv = all[j - 1];
                 if (Pv.dest[0] && J.parent == Pv.dest[0]) J.kPar = 2;
                 else if (Pv.dest[1] && J.parent == Pv.dest[1]) J.kPar = 3;
-                else if (lookahead2 && j >= (size_t)first + 2 &&
-                         ((all[j - 2].dest[0] && J.parent == all[j - 2].dest[0]) ||
-                          (all[j - 2].dest[1] && J.parent == all[j - 2].dest[1])))
-                    J.kPar = 4;
             }
         }
         launches.push_back({first, (int)s.ops.size(), s.batched});

@@ -77,9 +77,7 @@ struct NodeJob {
     const short* ex[2];        //   partials, the children's post-order partials
     short* ed[2];              //   (null for tips), the children's pre-order partials
     int k[2];                  // 0 partials, 1 tip states
-    int kPar;                  // 0 load; 2, 3: previous job's dest[0], dest[1] (registers);
-                               // 4: load when computed (the job two back wrote it; FP32
-                               // node kernels, which prefetch two jobs ahead)
+    int kPar;                  // 0 load; 2, 3: previous job's dest[0], dest[1] (registers)
     int moP[2], moD1[2], moD2[2];  // matrix element offsets in the matrix slab
     int oidx[2];               // operation index of each child in the derivative output
     int mbase;                 // this job's block in the gathered matrices (reals)

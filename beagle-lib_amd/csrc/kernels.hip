@@ -1293,25 +1293,18 @@ __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float*
     }
 }
 
-template <int C, bool AUTO, bool DIRECT>
+template <int C, bool AUTO>
 __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps4<float>& o,
                                                  float (&pa)[4], float (&pb)[4], int& pea,
                                                  int& peb, float* smat, float* sx, int& k, int c,
                                                  int lane, int p, int P, unsigned off,
-                                                 unsigned doff, float* dummyW, void* dummyEw,
                                                  const float (&wc)[C], float w, double* partial,
                                                  int tile, int ntiles) {
     const bool in = p < P;
     float u[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) u[i] = J.kPar == 2 ? pa[i] : (J.kPar == 3 ? pb[i] : o.u[i]);
-    int eu = J.kPar == 2 ? pea : (J.kPar == 3 ? peb : o.eu);
-    if (J.kPar == 4) {
-        // the parent is the output of the job two back, stored after this
-        // job's operands were prefetched: load it now
-        load4(at<float>(J.parent, off), u);
-        if (AUTO) eu = J.eParent ? *at<short>(J.eParent, 2u * p) : 0;
-    }
+    const int eu = J.kPar == 2 ? pea : (J.kPar == 3 ? peb : o.eu);
     const bool tipA = J.k[0] == 1, tipB = J.k[1] == 1;
     // tip states are stored in [0, 4]; the clamp keeps the column index in
     // the matrix whatever a lane without a tip read
@@ -1369,13 +1362,8 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
         }
         outA = ea + eu + (J.ex[1] ? o.eb : 0);
         outB = eb + eu + (J.ex[0] ? o.ea : 0);
-        if (DIRECT) {
-            if (keepA) *at_w<short>(J.ed[0], 2u * p) = (short)outA;
-            if (keepB) *at_w<short>(J.ed[1], 2u * p) = (short)outB;
-        } else {
-            *at_w<short>(keepA ? (void*)J.ed[0] : dummyEw, keepA ? 2u * p : 2u * lane) = (short)outA;
-            *at_w<short>(keepB ? (void*)J.ed[1] : dummyEw, keepB ? 2u * p : 2u * lane) = (short)outB;
-        }
+        if (keepA) *at_w<short>(J.ed[0], 2u * p) = (short)outA;
+        if (keepB) *at_w<short>(J.ed[1], 2u * p) = (short)outB;
     }
     // tile sums of child ch (w r1, w r1^2, w r2, r = N / D over categories)
     // by wave (ch + 2 par) % C
@@ -1402,13 +1390,8 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
             pt[(row + 2) * ntiles + tile] = (double)s2;
         }
     }
-    if (DIRECT) {
-        if (keepA) store4(at_w<float>(J.dest[0], off), va);
-        if (keepB) store4(at_w<float>(J.dest[1], off), vb);
-    } else {
-        store4(at_w<float>(keepA ? J.dest[0] : (void*)dummyW, keepA ? off : doff), va);
-        store4(at_w<float>(keepB ? J.dest[1] : (void*)dummyW, keepB ? off : doff), vb);
-    }
+    if (keepA) store4(at_w<float>(J.dest[0], off), va);
+    if (keepB) store4(at_w<float>(J.dest[1], off), vb);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         pa[i] = va[i];
@@ -1428,76 +1411,41 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     const int tile = blockIdx.x;                                                              \
     const int p = tile * TILE + lane; /* < Ppad: pattern weights are zero-padded */           \
     const unsigned off = (unsigned)((c * cs + (long)p * 4) * (long)sizeof(float));            \
-    const unsigned doff = (unsigned)((c * TILE + lane) * 4 * (int)sizeof(float));             \
     const int ntiles = (P + TILE - 1) / TILE;                                                 \
     const float w = (float)pw[p];                                                             \
     float wc[C];                                                                              \
     _Pragma("unroll") for (int cc = 0; cc < C; ++cc) wc[cc] = __int_as_float(                 \
         __builtin_amdgcn_readfirstlane(__float_as_int((float)cw[cc])));                       \
-    void* dummyEw = const_cast<char*>(reinterpret_cast<const char*>(dummyE)) + 128;           \
     int k = 0, pea = 0, peb = 0;                                                              \
     float pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
 
-// PF = operand prefetch distance in jobs: 0 (loads issued when the job
-// starts; at most 72 VGPRs, so seven waves per SIMD and a 100,000-pattern
-// grid resident at once), 1 or 2 (rotating register sets; jobs past the
-// end re-read the last job and are not computed).  0 is the default;
-// BEAGLE_AMD_NODE_PREFETCH selects another.
-template <int C, bool AUTO, int PF>
-__global__ __launch_bounds__(64 * C, PF == 0 ? 7 : 1) void k_pren4f(const NodeJob* __restrict__ jobs, int njobs,
-                                                   int chunk, int P, long cs,
-                                                   const float* __restrict__ jm,
-                                                   const float* dummyX, const int* dummyS,
-                                                   float* dummyW, const short* dummyE,
-                                                   const double* __restrict__ cw,
-                                                   const double* __restrict__ pw,
-                                                   double* partial) {
+// Each job's operands are loaded when the job starts (node_issue_direct):
+// at most 64 VGPRs, so eight waves per SIMD and a 100,000-pattern grid
+// resident at once, which covers the loads' latency better than loading
+// the next job ahead in more registers does.
+template <int C, bool AUTO>
+__global__ __launch_bounds__(64 * C, 7) void k_pren4f(const NodeJob* __restrict__ jobs, int njobs,
+                                                      int chunk, int P, long cs,
+                                                      const float* __restrict__ jm,
+                                                      const double* __restrict__ cw,
+                                                      const double* __restrict__ pw,
+                                                      double* partial) {
     NODE_F32_SETUP
     jobs += blockIdx.y * chunk;
     njobs = min(chunk, njobs - (int)blockIdx.y * chunk);
-#define NODE_F32_STEP(JC, SC, JN, SN, I)                                                         \
-    JN = jobs[min(o + (I) + PF, njobs - 1)];                                                      \
-    node_issue<float, AUTO>(JN, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, SN);          \
-    node_compute_f32<C, AUTO, false>(JC, SC, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,  \
-                                     doff, dummyW, dummyEw, wc, w, partial, tile, ntiles);       \
-    if (o + (I) + 1 >= njobs) break;
-    if (PF == 0) {
-        for (int o = 0; o < njobs; ++o) {
-            const NodeJob J = jobs[o];
-            NodeOps4<float> S0;
-            node_issue_direct<AUTO>(J, jm, c, off, p, lane, S0);
-            node_compute_f32<C, AUTO, true>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P,
-                                            off, doff, dummyW, dummyEw, wc, w, partial, tile,
-                                            ntiles);
-        }
-    } else if (PF == 1) {
-        NodeOps4<float> S0, S1;
-        NodeJob J0 = jobs[0], J1;
-        node_issue<float, AUTO>(J0, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
-        for (int o = 0;; o += 2) {
-            NODE_F32_STEP(J0, S0, J1, S1, 0)
-            NODE_F32_STEP(J1, S1, J0, S0, 1)
-        }
-    } else {
-        NodeOps4<float> S0, S1, S2;
-        NodeJob J0 = jobs[0], J1 = jobs[min(1, njobs - 1)], J2;
-        node_issue<float, AUTO>(J0, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
-        node_issue<float, AUTO>(J1, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S1);
-        for (int o = 0;; o += 3) {
-            NODE_F32_STEP(J0, S0, J2, S2, 0)
-            NODE_F32_STEP(J1, S1, J0, S0, 1)
-            NODE_F32_STEP(J2, S2, J1, S1, 2)
-        }
+    for (int o = 0; o < njobs; ++o) {
+        const NodeJob J = jobs[o];
+        NodeOps4<float> S0;
+        node_issue_direct<AUTO>(J, jm, c, off, p, lane, S0);
+        node_compute_f32<C, AUTO>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, wc, w,
+                                  partial, tile, ntiles);
     }
-#undef NODE_F32_STEP
 }
 
 template <int C, bool AUTO>
 __global__ __launch_bounds__(64 * C) void k_pren4f_level(const NodeJob* __restrict__ jobs,
                                                          int P, long cs,
                                                          const float* __restrict__ jm,
-                                                         const float* dummyX, const int* dummyS,
-                                                         float* dummyW, const short* dummyE,
                                                          const double* __restrict__ cw,
                                                          const double* __restrict__ pw,
                                                          double* partial) {
@@ -1505,41 +1453,26 @@ __global__ __launch_bounds__(64 * C) void k_pren4f_level(const NodeJob* __restri
     NodeOps4<float> S0;
     const NodeJob J = jobs[blockIdx.y];
     node_issue_direct<AUTO>(J, jm, c, off, p, lane, S0);
-    node_compute_f32<C, AUTO, true>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, doff,
-                                    dummyW, dummyEw, wc, w, partial, tile, ntiles);
+    node_compute_f32<C, AUTO>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, wc, w,
+                              partial, tile, ntiles);
 }
 #undef NODE_F32_SETUP
 
 template <int C>
 static void launch_pre_nodes_f32(const NodeJob* djobs, int njobs, bool batched, int chunk,
-                                 const Dims& d, const float* m, const float* dummyX,
-                                 const int* dummyS, float* dummyW, const short* dummyE,
-                                 const double* cw, const double* pw, double* partial,
-                                 hipStream_t st) {
+                                 const Dims& d, const float* m, const double* cw,
+                                 const double* pw, double* partial, hipStream_t st) {
     const int tiles = (d.P + TILE - 1) / TILE;
     const size_t lds = (size_t)(2 * 8 * C * TILE + C * 2 * NODE_F32_BLOCK) * sizeof(float);
     const dim3 block(TILE * C);
     if (batched && chunk <= 1) {
-        if (d.autoScale) hipLaunchKernelGGL((k_pren4f_level<C, true>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
-        else hipLaunchKernelGGL((k_pren4f_level<C, false>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4f_level<C, true>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial);
+        else hipLaunchKernelGGL((k_pren4f_level<C, false>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial);
     } else {
         if (!batched || chunk > njobs) chunk = njobs;
         const dim3 grid(tiles, (njobs + chunk - 1) / chunk);
-        static const int pf = [] {
-            const char* e = std::getenv("BEAGLE_AMD_NODE_PREFETCH");
-            return e ? std::min(2, std::max(0, std::atoi(e))) : 0;
-        }();
-#define PREN4F(A, F) hipLaunchKernelGGL((k_pren4f<C, A, F>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial)
-        if (d.autoScale) {
-            if (pf == 0) PREN4F(true, 0);
-            else if (pf == 1) PREN4F(true, 1);
-            else PREN4F(true, 2);
-        } else {
-            if (pf == 0) PREN4F(false, 0);
-            else if (pf == 1) PREN4F(false, 1);
-            else PREN4F(false, 2);
-        }
-#undef PREN4F
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4f<C, true>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial);
+        else hipLaunchKernelGGL((k_pren4f<C, false>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial);
     }
 }
 
@@ -2005,11 +1938,9 @@ void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, 
     const R* m = reinterpret_cast<const R*>(jobMats);
     if (std::is_same<R, float>::value && node_f32_supported(d.C)) {
         const float* mf = reinterpret_cast<const float*>(m);
-        const float* xf = reinterpret_cast<const float*>(dummyX);
-        float* wf = reinterpret_cast<float*>(dummyW);
-        if (d.C == 4) launch_pre_nodes_f32<4>(djobs, njobs, batched, chunk, d, mf, xf, dummyS, wf, dummyE, cw, pw, partial, st);
-        else if (d.C == 2) launch_pre_nodes_f32<2>(djobs, njobs, batched, chunk, d, mf, xf, dummyS, wf, dummyE, cw, pw, partial, st);
-        else launch_pre_nodes_f32<1>(djobs, njobs, batched, chunk, d, mf, xf, dummyS, wf, dummyE, cw, pw, partial, st);
+        if (d.C == 4) launch_pre_nodes_f32<4>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st);
+        else if (d.C == 2) launch_pre_nodes_f32<2>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st);
+        else launch_pre_nodes_f32<1>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st);
         return;
     }
     const int tiles = (d.P + TILE - 1) / TILE;

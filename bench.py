@@ -698,7 +698,7 @@ def main():
         "graph_results_identical_to_eager": graph_identical,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_pren4f<4,true,0>: node-fused FP32 pre-order + derivative "
+                     "kernel": "k_pren4f<4,true>: node-fused FP32 pre-order + derivative "
                                "traversal of the caterpillar (127 node jobs = 254 operations, "
                                "one launch)",
                      "kernel_ms": kernel_ms, "algorithmic_bytes": pre_bytes},
