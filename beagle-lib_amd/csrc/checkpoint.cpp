@@ -257,6 +257,7 @@ int Instance::checkpointedGradient(const int* ops, int count, const int* d1Idx, 
     reserveStage(1 + 3 * total);
     std::vector<std::pair<size_t, const Step*>> staged;
     std::vector<std::array<double, 4>> direct;   // other paths: node, sums
+    double logL = 0.0;
     size_t at = 1;
     for (const Step& s : S.steps) {
         const int m = (int)s.out.size();
@@ -267,6 +268,12 @@ int Instance::checkpointedGradient(const int* ops, int count, const int* d1Idx, 
             rc = enqueueRoot(s.root, wIdx, fIdx, BEAGLE_OP_NONE);
             if (rc == BEAGLE_SUCCESS) {
                 stageOut(dResult_, 1, 0);
+                if (!fused) {
+                    // the other paths deliver each batch's sums through the
+                    // staging area: read the log-likelihood now
+                    sync();
+                    logL = hStage_[0];
+                }
                 rc = setRootPrePartials(&s.rootPre, &fIdx, 1);
             }
         } else if (fused) {
@@ -288,7 +295,7 @@ int Instance::checkpointedGradient(const int* ops, int count, const int* d1Idx, 
         if (rc != BEAGLE_SUCCESS) return rc;
     }
     sync();
-    lastLogL_ = hStage_[0];
+    lastLogL_ = fused ? hStage_[0] : logL;
     if (outLogL) *outLogL = lastLogL_;
     for (const auto& st : staged) {
         const double* h = hStage_ + st.first;

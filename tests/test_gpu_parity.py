@@ -816,3 +816,33 @@ def test_checkpointed_gradient_matches_full(amd, precision, scaling, shape, tips
         ev.checkpointed_gradient(pool[:m - 1], 0)
     assert e.value.code == api.ERROR_GENERAL
     ev.close()
+
+
+@pytest.mark.parametrize("S,precision,scaling", [(20, "single", "auto"), (61, "single", "none"),
+                                                 (20, "double", "none")])
+def test_checkpointed_gradient_states(amd, S, precision, scaling):
+    """The checkpointed gradient for S > 4 (matrix-core post-order,
+    pre-order and derivative passes; tips' pre-order partials taken from
+    the pool): at interval 3 and at the interval picked for a small pool,
+    logL and every branch's derivatives equal the full evaluation's."""
+    prob = synth.make_problem(40 + S, 18, 300, S=S, shape="yule", missing=0.03)
+    ev = driver.Evaluator(amd, prob, precision=precision, scaling=scaling)
+    full = ev.fused_gradient()
+    n = prob.tree.n
+    pool = list(range(ev.pre_base, ev.pre_base + 2 * n - 1))
+    _, _, _, k, peak = ev.checkpointed_gradient(pool, 3, plan_only=True)
+    assert k == 3 and peak <= len(pool)
+    runs = [ev.checkpointed_gradient(pool, 3)]
+    for m in range(4, len(pool) + 1):
+        try:
+            ev.checkpointed_gradient(pool[:m], 0, plan_only=True)
+        except api.BeagleError:
+            continue
+        runs.append(ev.checkpointed_gradient(pool[:m], 0))
+        break
+    assert len(runs) == 2
+    for lnl, g1, g2, used, peak in runs:
+        assert lnl == full[0]
+        np.testing.assert_array_equal(g1, full[1])
+        np.testing.assert_array_equal(g2, full[2])
+    ev.close()
