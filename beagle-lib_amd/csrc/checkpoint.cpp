@@ -226,6 +226,12 @@ int Instance::checkpointedGradient(const int* ops, int count, const int* d1Idx, 
     }
     for (int i = 0; i < poolCount; ++i)
         if (!validBuffer(pool[i]) || pool[i] < tipCount_) return BEAGLE_ERROR_OUT_OF_RANGE;
+    {
+        std::vector<int> sorted(pool, pool + poolCount);   // each buffer once
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+    }
     // the schedule: the given interval, or the smallest that fits the pool
     const bool fused = dims_.S == 4 && dims_.C <= 8;
     Schedule S;

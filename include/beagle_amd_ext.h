@@ -47,7 +47,8 @@ int beagleExtSetAutotune(int instance, int on);
  * be BEAGLE_OP_NONE and the last operation's destination is the root.
  * firstDerivativeIndices / secondDerivativeIndices give, per node id, the
  * derivative matrices of the branch above the node (r_c Q and r_c^2 Q^2).
- * The call uses only the bufferCount partials buffers in bufferIndices:
+ * The call uses only the bufferCount partials buffers in bufferIndices
+ * (distinct, not tip buffers):
  * it keeps the post-order partials of internal nodes whose height (edges to
  * the deepest tip below) is a multiple of `interval`, and of the root,
  * recomputes the others from those below them when the pre-order pass

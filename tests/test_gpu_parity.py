@@ -815,6 +815,9 @@ def test_checkpointed_gradient_matches_full(amd, precision, scaling, shape, tips
     with pytest.raises(api.BeagleError) as e:
         ev.checkpointed_gradient(pool[:m - 1], 0)
     assert e.value.code == api.ERROR_GENERAL
+    with pytest.raises(api.BeagleError) as e:
+        ev.checkpointed_gradient(pool[:m] + pool[:1], 0)     # a buffer twice
+    assert e.value.code == api.ERROR_OUT_OF_RANGE
     ev.close()
 
 
