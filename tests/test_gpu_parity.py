@@ -546,6 +546,24 @@ def test_fused_gradient_category_counts(amd, oracle, monkeypatch, C, shape):
         np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
 
 
+@pytest.mark.parametrize("P,C", [(1, 4), (37, 1), (64, 2), (65, 4), (130, 4)])
+def test_fused_gradient_small_pattern_counts(amd, oracle, P, C):
+    """The FP32 node-fused derivative pass with fewer patterns than one
+    64-pattern tile, exactly one tile and one pattern past it (lanes past
+    the last pattern neither store nor add to the sums): logL and each
+    branch's derivatives against the oracle."""
+    prob = synth.make_problem(500 + P, 12, P, S=4, shape="yule", C=C, missing=0.1)
+    ev = driver.Evaluator(amd, prob, precision="single", scaling="auto", matrices="explicit")
+    lg, g1, g2 = ev.fused_gradient(keep_tips=True)
+    ev.close()
+    ev = driver.Evaluator(oracle, prob, scaling="manual", matrices="explicit")
+    lo, o1, o2 = ev.gradient()
+    ev.close()
+    assert rel(lg, lo) < 1e-6
+    assert np.max(np.abs(g1 - o1)) <= 1e-5 * max(np.max(np.abs(o1)), 1e-30)
+    assert np.max(np.abs(g2 - o2)) <= 1e-5 * max(np.max(np.abs(o2)), 1e-30)
+
+
 @pytest.mark.parametrize("mode,chunk", [("1", "1"), ("2", "1"), ("0", "3"), ("1", "2")])
 @pytest.mark.parametrize("precision,scaling", [("double", "none"), ("double", "always"),
                                                ("single", "auto")])
