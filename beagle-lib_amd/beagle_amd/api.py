@@ -146,6 +146,7 @@ SIGNATURES = {
     "ExtSetAutotune": (_I, [_I, _I]),
     "ExtCheckpointedGradient": (_I, [_I, _IP, _I, _IP, _IP, _I, _I, _IP, _I, _I, _IP, _IP, _DP,
                                      _DP, _DP, _DP]),
+    "ExtPlanCheckpoints": (_I, [_I, _IP, _I, _I, _I, _I, _IP, _IP, _IP, _IP]),
     "ExtBeginCapture": (_I, [_I]),
     "ExtEndCapture": (_I, [_I, ctypes.POINTER(ctypes.c_int)]),
     "ExtLaunchGraph": (_I, [_I, _I]),
@@ -191,6 +192,18 @@ class Library:
 
     def has(self, name):
         return name in self.fn
+
+    def plan_checkpoints(self, tip_count, ops, buffers, interval=0, tip_preorder=False):
+        """Extension (no instance): the checkpointed gradient's schedule for
+        a tree with `buffers` partials buffers.  Returns (interval, peak
+        buffers, checkpoints, recomputed operations)."""
+        a = _ip(np.asarray(ops).reshape(-1))
+        out = [ctypes.c_int(0) for _ in range(4)]
+        rc = self.fn["ExtPlanCheckpoints"](tip_count, a[1], len(a[0]) // 7, buffers, interval,
+                                           int(bool(tip_preorder)),
+                                           *[ctypes.byref(x) for x in out])
+        _check(self.prefix + "ExtPlanCheckpoints", rc)
+        return tuple(x.value for x in out)
 
     def call(self, name, *args):
         return self.fn[name](*args)

@@ -178,8 +178,8 @@ bool makeSchedule(const std::vector<TreeOp>& ops, int tipCount, const int* d1, c
             row(batch.rows, internal(c) || tipPre ? pre[c] : BEAGLE_OP_NONE, pre[v], above[c],
                 loc[z], above[z]);
             batch.post.push_back(loc[c]);
-            batch.d1.push_back(d1[c]);
-            batch.d2.push_back(d2[c]);
+            batch.d1.push_back(d1 ? d1[c] : 0);
+            batch.d2.push_back(d2 ? d2[c] : 0);
             batch.out.push_back(c);
         }
         // after this job: v's pre-order partials and its children's
@@ -199,7 +199,63 @@ bool makeSchedule(const std::vector<TreeOp>& ops, int tipCount, const int* d1, c
     return true;
 }
 
+// The schedule for `interval`, or for interval 0 the smallest interval
+// whose schedule fits the pool; *used is the interval taken.
+int chooseSchedule(const std::vector<TreeOp>& tree, int tipCount, const int* d1, const int* d2,
+                   const int* pool, int poolCount, int interval, bool tipPre, Schedule& S,
+                   int* used) {
+    int rc = BEAGLE_ERROR_GENERAL;
+    if (interval > 0) {
+        if (!makeSchedule(tree, tipCount, d1, d2, pool, poolCount, interval, tipPre, S, rc))
+            return rc;
+        *used = interval;
+        return BEAGLE_SUCCESS;
+    }
+    for (int k = 1; k <= (int)tree.size() + 1; ++k) {
+        if (makeSchedule(tree, tipCount, d1, d2, pool, poolCount, k, tipPre, S, rc)) {
+            *used = k;
+            return BEAGLE_SUCCESS;
+        }
+        if (rc != BEAGLE_ERROR_GENERAL) return rc;
+    }
+    return BEAGLE_ERROR_GENERAL;
+}
+
+// Post-order operations (7 ints each) as tree operations; scale indices
+// must be BEAGLE_OP_NONE.
+int readTree(const int* ops, int count, std::vector<TreeOp>& tree) {
+    if (count <= 0 || !ops) return BEAGLE_ERROR_OUT_OF_RANGE;
+    tree.resize(count);
+    for (int i = 0; i < count; ++i) {
+        const int* q = ops + (size_t)i * BEAGLE_OP_COUNT;
+        if (q[1] != BEAGLE_OP_NONE || q[2] != BEAGLE_OP_NONE) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+        tree[i] = TreeOp{q[0], q[3], q[4], q[5], q[6]};
+    }
+    return BEAGLE_SUCCESS;
+}
+
 }  // namespace
+
+int planCheckpoints(int tipCount, const int* ops, int count, int bufferCount, int interval,
+                    int tipPreorder, int* outInterval, int* outPeak, int* outCheckpoints,
+                    int* outRecomputed) {
+    if (tipCount < 1 || bufferCount <= 0 || interval < 0) return BEAGLE_ERROR_OUT_OF_RANGE;
+    std::vector<TreeOp> tree;
+    int rc = readTree(ops, count, tree);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    std::vector<int> pool(bufferCount);
+    for (int i = 0; i < bufferCount; ++i) pool[i] = i;
+    Schedule S;
+    int used = 0;
+    rc = chooseSchedule(tree, tipCount, nullptr, nullptr, pool.data(), bufferCount, interval,
+                        tipPreorder != 0, S, &used);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    if (outInterval) *outInterval = used;
+    if (outPeak) *outPeak = S.peak;
+    if (outCheckpoints) *outCheckpoints = S.checkpoints;
+    if (outRecomputed) *outRecomputed = S.recomputed;
+    return BEAGLE_SUCCESS;
+}
 
 int Instance::checkpointedGradient(const int* ops, int count, const int* d1Idx, const int* d2Idx,
                                    int wIdx, int fIdx, const int* pool, int poolCount,
@@ -215,14 +271,13 @@ int Instance::checkpointedGradient(const int* ops, int count, const int* d1Idx, 
         return BEAGLE_ERROR_NO_IMPLEMENTATION;
     if (!validEigen(wIdx) || hCatWeights_[wIdx].empty() || !validEigen(fIdx))
         return BEAGLE_ERROR_OUT_OF_RANGE;
-    std::vector<TreeOp> tree(count);
-    for (int i = 0; i < count; ++i) {
-        const int* q = ops + (size_t)i * BEAGLE_OP_COUNT;
-        tree[i] = TreeOp{q[0], q[3], q[4], q[5], q[6]};
-        if (q[1] != BEAGLE_OP_NONE || q[2] != BEAGLE_OP_NONE) return BEAGLE_ERROR_NO_IMPLEMENTATION;
-        if (!validMatrix(q[4]) || !validMatrix(q[6])) return BEAGLE_ERROR_OUT_OF_RANGE;
-        if (q[3] < tipCount_ && !validBuffer(q[3])) return BEAGLE_ERROR_OUT_OF_RANGE;
-        if (q[5] < tipCount_ && !validBuffer(q[5])) return BEAGLE_ERROR_OUT_OF_RANGE;
+    std::vector<TreeOp> tree;
+    int rc = readTree(ops, count, tree);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    for (const TreeOp& o : tree) {
+        if (!validMatrix(o.m1) || !validMatrix(o.m2)) return BEAGLE_ERROR_OUT_OF_RANGE;
+        if (o.c1 < tipCount_ && !validBuffer(o.c1)) return BEAGLE_ERROR_OUT_OF_RANGE;
+        if (o.c2 < tipCount_ && !validBuffer(o.c2)) return BEAGLE_ERROR_OUT_OF_RANGE;
     }
     for (int i = 0; i < poolCount; ++i)
         if (!validBuffer(pool[i]) || pool[i] < tipCount_) return BEAGLE_ERROR_OUT_OF_RANGE;
@@ -235,19 +290,9 @@ int Instance::checkpointedGradient(const int* ops, int count, const int* d1Idx, 
     // the schedule: the given interval, or the smallest that fits the pool
     const bool fused = dims_.S == 4 && dims_.C <= 8;
     Schedule S;
-    int rc = BEAGLE_ERROR_GENERAL;
-    if (interval > 0) {
-        if (!makeSchedule(tree, tipCount_, d1Idx, d2Idx, pool, poolCount, interval, !fused, S, rc))
-            return rc;
-    } else {
-        bool ok = false;
-        for (int k = 1; k <= count + 1 && !ok; ++k) {
-            ok = makeSchedule(tree, tipCount_, d1Idx, d2Idx, pool, poolCount, k, !fused, S, rc);
-            if (!ok && rc != BEAGLE_ERROR_GENERAL) return rc;
-            interval = k;
-        }
-        if (!ok) return BEAGLE_ERROR_GENERAL;
-    }
+    rc = chooseSchedule(tree, tipCount_, d1Idx, d2Idx, pool, poolCount, interval, !fused, S,
+                        &interval);
+    if (rc != BEAGLE_SUCCESS) return rc;
     for (int i = 0; i < count; ++i)
         for (int c : {tree[i].c1, tree[i].c2})
             if (c < S.outSize && (!validMatrix(d1Idx[c]) || !validMatrix(d2Idx[c])))

@@ -645,6 +645,17 @@ int beagleExtSetAutotune(int instance, int on) {
     return withInstance(instance, [&](Instance* in) { return in->setAutotune(on); });
 }
 
+int beagleExtPlanCheckpoints(int tipCount, const BeagleOperation* operations, int count,
+                             int bufferCount, int interval, int tipPreorderBuffers,
+                             int* outInterval, int* outPeakBuffers, int* outCheckpoints,
+                             int* outRecomputed) {
+    return guarded([&]() {
+        return hipbeagle::planCheckpoints(tipCount, reinterpret_cast<const int*>(operations), count,
+                               bufferCount, interval, tipPreorderBuffers, outInterval,
+                               outPeakBuffers, outCheckpoints, outRecomputed);
+    });
+}
+
 int beagleExtCheckpointedGradient(int instance, const BeagleOperation* operations, int count,
                                   const int* firstDerivativeIndices,
                                   const int* secondDerivativeIndices,

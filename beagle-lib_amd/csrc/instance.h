@@ -46,6 +46,12 @@ struct Segment {
     std::vector<int> ops;
 };
 
+// The checkpoint schedule of beagleExtCheckpointedGradient for a tree,
+// without an instance (beagleExtPlanCheckpoints).
+int planCheckpoints(int tipCount, const int* ops, int count, int bufferCount, int interval,
+                    int tipPreorder, int* outInterval, int* outPeak, int* outCheckpoints,
+                    int* outRecomputed);
+
 class Instance {
 public:
     Instance(int tipCount, int partialsBufferCount, int compactBufferCount, int stateCount,

@@ -72,6 +72,19 @@ int beagleExtCheckpointedGradient(int instance, const BeagleOperation* operation
                                   double* outSumSquaredFirstDerivatives,
                                   double* outSumSecondDerivatives);
 
+/* The schedule beagleExtCheckpointedGradient would run for a tree with
+ * bufferCount buffers, without an instance or a device (no counterpart in
+ * the reference): the interval (interval = 0: the smallest that fits), the
+ * most buffers in use at once, the internal nodes kept as checkpoints and
+ * the post-order operations run again.  tipPreorderBuffers = 1 when the
+ * instance stores tips' pre-order partials too (every state count other
+ * than 4, and more than 8 categories).  Return codes as
+ * beagleExtCheckpointedGradient. */
+int beagleExtPlanCheckpoints(int tipCount, const BeagleOperation* operations, int count,
+                             int bufferCount, int interval, int tipPreorderBuffers,
+                             int* outInterval, int* outPeakBuffers, int* outCheckpoints,
+                             int* outRecomputed);
+
 /* HIP-graph capture of an instance's calls (no counterpart in the
  * reference).  Between beagleExtBeginCapture and beagleExtEndCapture the
  * instance's calls are recorded into a graph instead of run:
