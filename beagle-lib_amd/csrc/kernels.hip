@@ -690,15 +690,26 @@ __device__ __forceinline__ double dpp_move(double v) {
     return __hiloint2double(hi, lo);
 }
 
+// v moved across lanes by a DPP control into the rows ROWS selects; the
+// other rows read an unspecified value (no instruction clears them).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move_rows(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
 // Sum over the 64 lanes of a wave with DPP steps (quad swaps, row shifts,
 // row broadcasts); the total ends in lane 63 and is returned to all lanes.
+// The broadcasts leave rows 0 and 2 unspecified: lane 63 reads row 2's
+// sum before the first broadcast and row 1's (complete) after it.
 __device__ __forceinline__ double wave_sum_dpp(double v) {
     v += dpp_move<0xB1, 0xf>(v);    // quad_perm [1,0,3,2]
     v += dpp_move<0x4E, 0xf>(v);    // quad_perm [2,3,0,1]: every lane holds its quad's sum
     v += dpp_move<0x114, 0xf>(v);   // row_shr:4
     v += dpp_move<0x118, 0xf>(v);   // row_shr:8: lanes 12-15 of each row hold the row sum
-    v += dpp_move<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
-    v += dpp_move<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3: lane 63 holds the total
+    v += dpp_move_rows<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_move_rows<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3: lane 63 holds the total
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
     return __hiloint2double(hi, lo);
@@ -1182,14 +1193,19 @@ __device__ __forceinline__ float dpp_movf(float v) {
         __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
 }
 
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_movf_rows(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+
 // Sum over the 64 lanes, the steps of wave_sum_dpp in FP32.
 __device__ __forceinline__ float wave_sum_f32(float v) {
     v += dpp_movf<0xB1, 0xf>(v);
     v += dpp_movf<0x4E, 0xf>(v);
     v += dpp_movf<0x114, 0xf>(v);
     v += dpp_movf<0x118, 0xf>(v);
-    v += dpp_movf<0x142, 0xa>(v);
-    v += dpp_movf<0x143, 0xc>(v);
+    v += dpp_movf_rows<0x142, 0xa>(v);
+    v += dpp_movf_rows<0x143, 0xc>(v);
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
@@ -1269,14 +1285,31 @@ __device__ __forceinline__ void branch_sums_f32(bool tip, int s, int r, const fl
 // branches skip what the job does not read (a forwarded parent, the
 // partials of a tip child, the states of a partials child).  Used where no
 // later job's loads are in flight, so the branches cost no extra waits.
+// A register with no particular value and no instruction to set it, for
+// operand slots a job does not load (its uses are selected or branched
+// away by the same job fields that skip the load).
+template <typename T>
+__device__ __forceinline__ T unset() {
+    T x;
+    asm volatile("" : "=v"(x));
+    return x;
+}
+
 template <bool AUTO>
 __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float* __restrict__ jm,
                                                   int c, unsigned off, int p, int lane,
                                                   NodeOps4<float>& o) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) o.u[i] = o.xa[i] = o.xb[i] = 0.0f;
-    o.sa = o.sb = 0;
-    o.eu = o.ea = o.eb = 0;
+    for (int i = 0; i < 4; ++i) {
+        o.u[i] = unset<float>();
+        o.xa[i] = unset<float>();
+        o.xb[i] = unset<float>();
+    }
+    o.sa = unset<int>();
+    o.sb = unset<int>();
+    o.eu = unset<int>();
+    o.ea = unset<int>();
+    o.eb = unset<int>();
     if (J.kPar == 0) load4(at<float>(J.parent, off), o.u);
     if (J.k[0] == 0) load4(at<float>(J.x[0], off), o.xa);
     else o.sa = *at<int>(J.x[0], 4u * p);
