@@ -50,11 +50,14 @@ def _eigen(Q):
 
 class PhyloLikelihood:
     def __init__(self, lib, tree, tips, pattern_weights, rates, weights, *, precision="double",
-                 scaling="auto"):
-        """tree: synth.Tree; tips: (taxa, patterns) int states 0..3 (>= 4
-        missing); rates, weights: the rate categories.  4 states (the
-        rate-matrix gradient is a 4-state kernel)."""
-        self.prob = synth.Problem(tree=tree, model=synth.jc69(), rates=np.asarray(rates, float),
+                 scaling="auto", states=4):
+        """tree: synth.Tree; tips: (taxa, patterns) int states 0..states-1
+        (>= states missing); rates, weights: the rate categories.  Any state
+        count up to 64 (the rate-matrix gradient covers them); the model is
+        a placeholder until the first evaluation sets Q and pi."""
+        S = int(states)
+        model = synth.model_from(np.ones(S * (S - 1) // 2), np.full(S, 1.0 / S))
+        self.prob = synth.Problem(tree=tree, model=model, rates=np.asarray(rates, float),
                                   weights=np.asarray(weights, float), tips=np.asarray(tips),
                                   pattern_weights=np.asarray(pattern_weights, float))
         self.ev = driver.Evaluator(lib, self.prob, precision=precision, scaling=scaling)

@@ -1660,10 +1660,13 @@ int Instance::attachOwnOperands(int count, const int* postIdx, const int* d1Idx,
 int Instance::rateMatrixGradient(int e, const int* ops, int count, const int* postIdx, int wIdx,
                                  int rootIdx, int fIdx, double* outQ, double* outFreqs) {
     const int S = dims_.S, C = dims_.C;
-    if (S != 4 || C > 8 || preTransposed_) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    if (preTransposed_) return BEAGLE_ERROR_NO_IMPLEMENTATION;
     if (!validEigen(e) || !eigenSet_[e] || hEigen_[e].lam.empty() || !validEigen(wIdx) ||
         hCatWeights_[wIdx].empty())
         return BEAGLE_ERROR_OUT_OF_RANGE;
+    if (S != 4 || C > 8)
+        return rateMatrixGradientGeneric(e, ops, count, postIdx, wIdx, rootIdx, fIdx, outQ,
+                                         outFreqs);
     if (partialsBytes_ >= (size_t(1) << 32)) return BEAGLE_ERROR_NO_IMPLEMENTATION;
     int rc = buildPreJobs(ops, count, false, true);
     if (rc != BEAGLE_SUCCESS) return rc;
@@ -1749,6 +1752,96 @@ int Instance::rateMatrixGradient(int e, const int* ops, int count, const int* po
         }
     if (freqs)
         for (int s = 0; s < S; ++s) outFreqs[s] = h[16 + s];
+    return BEAGLE_SUCCESS;
+}
+
+// Any S up to 64 (and 4 states with more than 8 categories): one branch
+// per pre-order operation, see launch_qgrad_generic.  Same contract as the
+// 4-state path.
+int Instance::rateMatrixGradientGeneric(int e, const int* ops, int count, const int* postIdx,
+                                        int wIdx, int rootIdx, int fIdx, double* outQ,
+                                        double* outFreqs) {
+    const int S = dims_.S, C = dims_.C;
+    if (S > 64) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    int rc = buildPreJobs(ops, count, false, true);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    rc = attachOwnOperands(count, postIdx, nullptr, nullptr);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    const HostEigen& E = hEigen_[e];
+    std::vector<double> tau((size_t)count * C);
+    for (int o = 0; o < count; ++o) {
+        const MatrixOrigin& org = matOrigin_[ops[(size_t)o * BEAGLE_OP_COUNT + 4]];
+        if (org.eigen != e || hCatRates_[org.rates].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
+        for (int c = 0; c < C; ++c) tau[(size_t)o * C + c] = hCatRates_[org.rates][c] * org.t;
+    }
+    const bool freqs = outFreqs != nullptr;
+    if (freqs && (!validBuffer(rootIdx) || !dPartials_[rootIdx] || !validEigen(fIdx) ||
+                  hFreqs_[fIdx].empty()))
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    // one upload: jobs | V, V^-1, eigenvalues | tau
+    const size_t S2 = (size_t)S * S;
+    const size_t jobBytes = (size_t)count * sizeof(PreJob);
+    const size_t eOff = (jobBytes + 255) / 256 * 256;
+    const size_t tOff = eOff + (2 * S2 + S) * sizeof(double);
+    std::vector<char> blob(tOff + tau.size() * sizeof(double));
+    std::memcpy(blob.data(), hPre_.data(), jobBytes);
+    double* he = reinterpret_cast<double*>(blob.data() + eOff);
+    std::copy(E.V.begin(), E.V.end(), he);
+    std::copy(E.Vi.begin(), E.Vi.end(), he + S2);
+    std::copy(E.lam.begin(), E.lam.end(), he + 2 * S2);
+    std::memcpy(blob.data() + tOff, tau.data(), tau.size() * sizeof(double));
+    char* dblob = reinterpret_cast<char*>(upload(blob.data(), blob.size()));
+    // work: coef [count][Ppad] | X chunks [count][C][chunks][S^2] |
+    // z [S^2][count C] | root-frequency block sums [S][nb] | sums [S^2 + S]
+    const int ntiles = dims_.Ppad / 32;
+    const int chunks = std::max(1, std::min(ntiles, (1024 + count * C - 1) / (count * C)));
+    const int nb = (dims_.P + 255) / 256;
+    const size_t nCoef = (size_t)count * dims_.Ppad, nX = (size_t)count * C * chunks * S2,
+                 nZ = S2 * count * C, nF = (size_t)S * nb;
+    double* wk = reinterpret_cast<double*>(
+        grow(dWork_, workCap_, (nCoef + nX + nZ + nF + S2 + S) * sizeof(double)));
+    double* coef = wk;
+    double* xs = coef + nCoef;
+    double* z = xs + nX;
+    double* fb = z + nZ;
+    double* sums = fb + nF;
+    const double* cw = dCatWeights_ + (size_t)wIdx * C;
+    const PreJob* dj = reinterpret_cast<const PreJob*>(dblob);
+    const double* deig = reinterpret_cast<const double*>(dblob + eOff);
+    const double* dtau = reinterpret_cast<const double*>(dblob + tOff);
+    if (dp_)
+        launch_qgrad_generic<double>(dj, count, dims_, cw, dPatternWeights_, deig, dtau, coef, xs,
+                                     chunks, z, stream_);
+    else
+        launch_qgrad_generic<float>(dj, count, dims_, cw, dPatternWeights_, deig, dtau, coef, xs,
+                                    chunks, z, stream_);
+    launch_reduce_rows(z, count * C, (int)S2, sums, stream_);
+    if (freqs) {
+        if (dp_) launch_root_freq_grad<double>(dPartials_[rootIdx], cw, dFreqs_ + (size_t)fIdx * S, dPatternWeights_, fb, dims_, stream_);
+        else launch_root_freq_grad<float>(dPartials_[rootIdx], cw, dFreqs_ + (size_t)fIdx * S, dPatternWeights_, fb, dims_, stream_);
+        launch_reduce_rows(fb, nb, S, sums + S2, stream_);
+    }
+    HIPCHECK(hipGetLastError());
+    const double* h = stageOut(sums, S2 + (freqs ? S : 0), 0);
+    sync();
+    // dlogL/dQ = V^-T H V^T
+    if (outQ) {
+        std::vector<double> t(S2, 0.0);   // t = H V^T
+        for (int k = 0; k < S; ++k)
+            for (int b = 0; b < S; ++b) {
+                double v = 0.0;
+                for (int l = 0; l < S; ++l) v += h[k * S + l] * E.V[b * S + l];
+                t[k * S + b] = v;
+            }
+        for (int a = 0; a < S; ++a)
+            for (int b = 0; b < S; ++b) {
+                double g = 0.0;
+                for (int k = 0; k < S; ++k) g += E.Vi[k * S + a] * t[k * S + b];
+                outQ[a * S + b] = g;
+            }
+    }
+    if (freqs)
+        for (int s = 0; s < S; ++s) outFreqs[s] = h[S2 + s];
     return BEAGLE_SUCCESS;
 }
 

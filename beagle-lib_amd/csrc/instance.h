@@ -102,6 +102,9 @@ public:
                                          double* outD1, double* outD1Sq, double* outD2);
     int rateMatrixGradient(int eigenIdx, const int* ops, int count, const int* postIdx, int wIdx,
                            int rootIdx, int fIdx, double* outQ, double* outFreqs);
+    int rateMatrixGradientGeneric(int eigenIdx, const int* ops, int count, const int* postIdx,
+                                  int wIdx, int rootIdx, int fIdx, double* outQ,
+                                  double* outFreqs);
     int logLikelihoodAndDerivatives(int rootIdx, int wIdx, int fIdx, int cumIdx, int rootPreIdx,
                                     const int* ops, int count, const int* postIdx,
                                     const int* d1Idx, const int* d2Idx, double* outLogL,

@@ -273,6 +273,14 @@ template <typename R>
 void launch_qgrad(const NodeJob* djobs, int njobs, const Dims& d, const void* jobMats,
                   const double* jobF, const double* eig, void* dummy, const double* cw,
                   const double* pw, double* partial, hipStream_t st);
+// Rate-matrix gradient for any S <= 64 over pre-order jobs (one branch
+// each, own operands attached): coef [nops][Ppad], xs [nops][C][chunks][S*S]
+// work; z receives (V^T X V^-T) o F as rows of [S*S][nops * C], summed
+// with launch_reduce_rows.  eig = V | V^-1 | eigenvalues, tau [nops][C].
+template <typename R>
+void launch_qgrad_generic(const PreJob* djobs, int nops, const Dims& d, const double* cw,
+                          const double* pw, const double* eig, const double* tau, double* coef,
+                          double* xs, int chunks, double* z, hipStream_t st);
 // Root-frequency gradient block sums: rows i of [S][ceil(P / 256)].
 template <typename R>
 void launch_root_freq_grad(const void* root, const double* cw, const double* freqs,

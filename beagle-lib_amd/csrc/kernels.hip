@@ -1721,6 +1721,292 @@ void launch_root_freq_grad(const void* root, const double* cw, const double* fre
 }
 
 // ---------------------------------------------------------------------------
+// dlogL/dQ for any state count up to 64, one pre-order operation (branch)
+// at a time.  With top = pre(parent) * (P_sib x_sib), x the branch's own
+// post-order operand and P its matrix:
+//   coef_p = w_p / sum_c w_c sum_i top_i (P x)_i                   (k_qgx_coef)
+//   X_c    = sum_p coef_p w_c top_c(p) x_c(p)^T                      (k_qgx_outer)
+//   H     += (V^T X_c V^-T) o F(r_c t),                              (k_qgx_eigen)
+//   F_ab   = (e^(tau l_a) - e^(tau l_b)) / (l_a - l_b)
+// the sums k_qgrad4 forms for 4 states.  The likelihood in coef is formed
+// from the same operands as top and x, so scale factors and exponents
+// cancel.  Patterns go in tiles of QGX_T, staged in LDS as [S][QGX_LD]
+// doubles (row stride QGX_LD: column reads of consecutive states do not
+// share a bank); matrices are read from memory (cached: every pattern of a
+// workgroup reads the same rows).
+
+constexpr int QGX_T = 32;
+constexpr int QGX_LD = QGX_T + 1;
+static_assert(WG == 256, "the Q-gradient register blocks map 256 threads to 16 x 16");
+constexpr int QGX_NE = 64 * 64 / WG;   // S x S entries per thread, S <= 64 (k_qgx_eigen)
+
+// dynamic LDS of the two tile kernels: three tiles, then (k_qgx_coef) the
+// [16][QGX_T] partial sums, L and the two state rows
+static size_t qgx_lds_bytes(int S) {
+    return ((size_t)3 * S * QGX_LD + 16 * QGX_T + 2 * QGX_T) * sizeof(double);
+}
+
+// Stage category c of the tile at p0: A = pre(parent), B = the sibling's
+// post-order partials (not for a tip sibling), X = own operand (one-hot of
+// a tip state < S; zero for a missing state, whose pad column does not
+// depend on Q).  Tip states into sts (sibling) and sto (own).
+template <typename R>
+__device__ void qgx_stage(const PreJob& J, const Dims& d, int c, int p0, double* A, double* B,
+                          double* X, int* sts, int* sto) {
+    const R* par = reinterpret_cast<const R*>(J.parent);
+    const R* xs = reinterpret_cast<const R*>(J.xs);
+    const R* xo = reinterpret_cast<const R*>(J.xself);
+    for (int t = threadIdx.x; t < d.S * QGX_T; t += WG) {
+        const int i = t / QGX_T, q = t % QGX_T, p = p0 + q;
+        const long e = pbase(d, c, p) + (long)i * d.sStride;
+        A[i * QGX_LD + q] = (double)par[e];
+        if (xs) B[i * QGX_LD + q] = (double)xs[e];
+        X[i * QGX_LD + q] = xo ? (double)xo[e] : (J.sself[p] == i ? 1.0 : 0.0);
+    }
+    if (threadIdx.x < QGX_T) {
+        const int p = p0 + threadIdx.x;
+        sts[threadIdx.x] = J.ss ? J.ss[p] : 0;
+        sto[threadIdx.x] = J.sself ? J.sself[p] : 0;
+    }
+}
+
+// Register blocks: the 256 threads of a workgroup as a 16 x 16 grid (bi,
+// bj); products cover states 4 bi + u (u < 4) at patterns 2 bj + w (w < 2),
+// the outer product entries (4 bi + u, 4 bj + v).  Rows and columns past S
+// are clamped to S - 1 (their results are discarded), so every read stays
+// inside the staged tiles.
+
+// (M v)_i at pattern q for the thread's block: M the category's rows
+// ([S][S + 1] reals, pad column S), v a staged tile, or for a tip the
+// rows' entries at the pattern's state (the pad for a missing one).
+template <typename R>
+__device__ __forceinline__ void qgx_mv_block(const R* M, int S, const double* v, bool tip,
+                                             const int* st, int i0, int q0,
+                                             double (&out)[4][2]) {
+    const R* r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) r[u] = M + (long)min(i0 + u, S - 1) * (S + 1);
+    if (tip) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int w = 0; w < 2; ++w) out[u][w] = (double)r[u][st[q0 + w]];
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) out[u][0] = out[u][1] = 0.0;
+    for (int k = 0; k < S; ++k) {
+        const double v0 = v[k * QGX_LD + q0], v1 = v[k * QGX_LD + q0 + 1];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double m = (double)r[u][k];
+            out[u][0] += m * v0;
+            out[u][1] += m * v1;
+        }
+    }
+}
+
+// grid (Ppad / QGX_T, 1, operations): coef[o][p]
+template <typename R>
+__global__ __launch_bounds__(WG) void k_qgx_coef(const PreJob* __restrict__ jobs, Dims d,
+                                                 const double* __restrict__ cw,
+                                                 const double* __restrict__ pw,
+                                                 double* __restrict__ coef) {
+    extern __shared__ double qs[];
+    const int S = d.S;
+    double* A = qs;
+    double* B = A + S * QGX_LD;
+    double* X = B + S * QGX_LD;
+    double* red = X + S * QGX_LD;                     // [16][QGX_T]
+    double* L = red + 16 * QGX_T;                     // [QGX_T]
+    int* sts = reinterpret_cast<int*>(L + QGX_T);
+    int* sto = sts + QGX_T;
+    const int o = blockIdx.z;
+    const PreJob J = jobs[o];
+    const int p0 = blockIdx.x * QGX_T;
+    const int bi = threadIdx.x / 16, i0 = 4 * bi, q0 = 2 * (threadIdx.x % 16);
+    if (threadIdx.x < QGX_T) L[threadIdx.x] = 0.0;
+    for (int c = 0; c < d.C; ++c) {
+        __syncthreads();
+        qgx_stage<R>(J, d, c, p0, A, B, X, sts, sto);
+        __syncthreads();
+        const R* Ms = reinterpret_cast<const R*>(J.mSib) + (long)c * S * (S + 1);
+        const R* Mo = reinterpret_cast<const R*>(J.mSelf) + (long)c * S * (S + 1);
+        double tb[4][2], yb[4][2];
+        qgx_mv_block(Ms, S, B, J.ss != nullptr, sts, i0, q0, tb);   // P_sib x_sib
+        qgx_mv_block(Mo, S, X, J.sself != nullptr, sto, i0, q0, yb);  // P x
+        double part[2] = {0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u < S)
+#pragma unroll
+                for (int w = 0; w < 2; ++w) part[w] += A[(i0 + u) * QGX_LD + q0 + w] * tb[u][w] * yb[u][w];
+        red[bi * QGX_T + q0] = part[0];
+        red[bi * QGX_T + q0 + 1] = part[1];
+        __syncthreads();
+        if (threadIdx.x < QGX_T) {
+            double t = 0.0;
+            for (int k = 0; k < 16; ++k) t += red[k * QGX_T + threadIdx.x];
+            L[threadIdx.x] += cw[c] * t;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < QGX_T) {
+        const int p = p0 + threadIdx.x;
+        const double l = L[threadIdx.x];
+        coef[(long)o * d.Ppad + p] = (p < d.P && l != 0.0) ? pw[p] / l : 0.0;
+    }
+}
+
+// grid (chunks, C, operations): this chunk's X_c, out[o][c][chunk][S * S]
+template <typename R>
+__global__ __launch_bounds__(WG) void k_qgx_outer(const PreJob* __restrict__ jobs, Dims d,
+                                                  const double* __restrict__ cw,
+                                                  const double* __restrict__ coef,
+                                                  int tilesPerChunk, double* __restrict__ out) {
+    extern __shared__ double qs[];
+    const int S = d.S;
+    double* A = qs;
+    double* B = A + S * QGX_LD;
+    double* X = B + S * QGX_LD;
+    double* cf = X + S * QGX_LD;
+    int* sts = reinterpret_cast<int*>(cf + QGX_T);
+    int* sto = sts + QGX_T;
+    const int o = blockIdx.z, c = blockIdx.y;
+    const PreJob J = jobs[o];
+    const R* Ms = reinterpret_cast<const R*>(J.mSib) + (long)c * S * (S + 1);
+    const double wc = cw[c];
+    const int bi = threadIdx.x / 16, bj = threadIdx.x % 16;
+    const int i0 = 4 * bi, q0 = 2 * bj, j0 = 4 * bj;
+    const bool live = i0 < S && j0 < S;   // outer-product block inside S x S
+    int ri[4], rj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        ri[u] = min(i0 + u, S - 1) * QGX_LD;
+        rj[u] = min(j0 + u, S - 1) * QGX_LD;
+    }
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+    const int ntiles = d.Ppad / QGX_T;
+    const int t0 = blockIdx.x * tilesPerChunk, t1 = min(ntiles, t0 + tilesPerChunk);
+    for (int tile = t0; tile < t1; ++tile) {
+        const int p0 = tile * QGX_T;
+        __syncthreads();
+        qgx_stage<R>(J, d, c, p0, A, B, X, sts, sto);
+        if (threadIdx.x < QGX_T) cf[threadIdx.x] = wc * coef[(long)o * d.Ppad + p0 + threadIdx.x];
+        __syncthreads();
+        // top scaled by coef, written over the parent's tile once every
+        // thread has read it
+        double tb[4][2];
+        qgx_mv_block(Ms, S, B, J.ss != nullptr, sts, i0, q0, tb);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int w = 0; w < 2; ++w) tb[u][w] *= cf[q0 + w] * A[ri[u] + q0 + w];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u < S) {
+                A[(i0 + u) * QGX_LD + q0] = tb[u][0];
+                A[(i0 + u) * QGX_LD + q0 + 1] = tb[u][1];
+            }
+        __syncthreads();
+        if (live)
+            for (int k = 0; k < QGX_T; ++k) {
+                double a[4], x[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    a[u] = A[ri[u] + k];
+                    x[u] = X[rj[u] + k];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * x[v];
+            }
+    }
+    double* dst = out + (((long)o * d.C + c) * gridDim.x + blockIdx.x) * S * S;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (i0 + u < S && j0 + v < S) dst[(i0 + u) * S + j0 + v] = acc[u][v];
+}
+
+// grid (C, operations): sums the chunks of X_c, forms (V^T X_c V^-T) o F
+// into z[(a S + b)][o C + c].  eig = V | V^-1 (row-major) | eigenvalues;
+// tau[o C + c] = r_c t of the operation's branch.
+__global__ __launch_bounds__(WG) void k_qgx_eigen(const double* __restrict__ xs, int chunks,
+                                                  int nops, int S, int C,
+                                                  const double* __restrict__ eig,
+                                                  const double* __restrict__ tau,
+                                                  double* __restrict__ z) {
+    extern __shared__ double qs[];
+    double* Xs = qs;   // S x S
+    const int c = blockIdx.x, o = blockIdx.y;
+    const double* V = eig;
+    const double* Vi = eig + S * S;
+    const double* lam = eig + 2 * S * S;
+    const double* src = xs + ((long)o * C + c) * chunks * S * S;
+    for (int e = threadIdx.x; e < S * S; e += WG) {
+        double v = 0.0;
+        for (int k = 0; k < chunks; ++k) v += src[(long)k * S * S + e];
+        Xs[e] = v;
+    }
+    __syncthreads();
+    double t[QGX_NE];
+#pragma unroll
+    for (int m = 0; m < QGX_NE; ++m) {   // T = V^T X: T[a][j] = sum_i V[i][a] X[i][j]
+        const int e = threadIdx.x + m * WG;
+        t[m] = 0.0;
+        if (e < S * S) {
+            const int a = e / S, j = e % S;
+            for (int i = 0; i < S; ++i) t[m] += V[i * S + a] * Xs[i * S + j];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < QGX_NE; ++m) {
+        const int e = threadIdx.x + m * WG;
+        if (e < S * S) Xs[e] = t[m];
+    }
+    __syncthreads();
+    const double tv = tau[o * C + c];
+    const long cols = (long)nops * C;
+#pragma unroll
+    for (int m = 0; m < QGX_NE; ++m) {   // Y = T V^-T: Y[a][b] = sum_j T[a][j] Vi[b][j]
+        const int e = threadIdx.x + m * WG;
+        if (e < S * S) {
+            const int a = e / S, b = e % S;
+            double y = 0.0;
+            for (int j = 0; j < S; ++j) y += Xs[a * S + j] * Vi[b * S + j];
+            const double dl = lam[a] - lam[b];
+            // tau e^(tau l) at dl = 0, expm1 near it (no cancellation)
+            const double f = dl == 0.0 ? tv * exp(tv * lam[b]) : exp(tv * lam[b]) * expm1(tv * dl) / dl;
+            z[(long)e * cols + o * C + c] = y * f;
+        }
+    }
+}
+
+template <typename R>
+void launch_qgrad_generic(const PreJob* djobs, int nops, const Dims& d, const double* cw,
+                          const double* pw, const double* eig, const double* tau, double* coef,
+                          double* xs, int chunks, double* z, hipStream_t st) {
+    if (nops <= 0) return;
+    const int ntiles = d.Ppad / QGX_T;
+    hipLaunchKernelGGL(k_qgx_coef<R>, dim3(ntiles, 1, nops), dim3(WG), qgx_lds_bytes(d.S), st,
+                       djobs, d, cw, pw, coef);
+    const int per = (ntiles + chunks - 1) / chunks;
+    hipLaunchKernelGGL(k_qgx_outer<R>, dim3(chunks, d.C, nops), dim3(WG), qgx_lds_bytes(d.S), st,
+                       djobs, d, cw, coef, per, xs);
+    hipLaunchKernelGGL(k_qgx_eigen, dim3(d.C, nops), dim3(WG), (size_t)d.S * d.S * sizeof(double),
+                       st, xs, chunks, nops, d.S, d.C, eig, tau, z);
+}
+
+// ---------------------------------------------------------------------------
 // generic state count: one launch per operation, one lane per output
 // element (category, pattern, state); a second pass rescales.
 
@@ -2744,6 +3030,9 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
                                   hipStream_t);                                                 \
     template void launch_root_freq_grad<R>(const void*, const double*, const double*,          \
                                            const double*, double*, const Dims&, hipStream_t);   \
+    template void launch_qgrad_generic<R>(const PreJob*, int, const Dims&, const double*,       \
+                                          const double*, const double*, const double*, double*,  \
+                                          double*, int, double*, hipStream_t);                  \
     template void launch_pre_nodes<R>(const NodeJob*, int, bool, int, const Dims&, const void*, \
                                       void*, const double*, const double*, double*,         \
                                       hipStream_t);                                         \

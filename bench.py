@@ -256,9 +256,17 @@ def codon_bench(lib, steps):
             grad = ev.fused_gradient()
         torch.cuda.synchronize()
         grad_rate = ng / (time.perf_counter() - t0)
+        # exact dlogL/dQ and dlogL/dpi_root from the same partials (one
+        # branch per pre-order operation for S > 4; synchronous call)
+        G, _ = ev.rate_matrix_gradient()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            G, _ = ev.rate_matrix_gradient()
+        qgrad_ms = (time.perf_counter() - t0) / 3 * 1e3
         ev.close()
         if name == "fp64":
             grad_ref = grad
+            G_ref = G
         w = 8 if prec == "double" else 4
         gbs = postorder_bytes(prob, w, True) / (ms * 1e-3) / 1e9
         tf = flops / (ms * 1e-3) / 1e12
@@ -268,6 +276,9 @@ def codon_bench(lib, steps):
                      "gradient_evals_per_s": grad_rate,
                      "gradient_err_vs_fp64": max(grad_err(grad[1], grad_ref[1]),
                                                  grad_err(grad[2], grad_ref[2])),
+                     "rate_matrix_gradient_ms": qgrad_ms,
+                     "rate_matrix_gradient_err_vs_fp64":
+                         float(np.abs(G - G_ref).max() / np.abs(G_ref).max()),
                      "postorder_ms": ms, "partials_GBps": gbs,
                      "partials_frac_of_hbm_peak": gbs / HBM_PEAK_GBS,
                      "algorithmic_TFLOPs": tf, "mfma_peak_TFLOPs": PEAK_TFLOPS[peak],

@@ -190,8 +190,11 @@ int beagleExtCalculateLogLikelihoodAndDerivatives(
  * Exact (Frechet derivative of the matrix exponential through the
  * eigen-system), unlike beagleCalculateCrossProductDerivative.  Chain rules
  * to exchangeabilities and frequencies inside Q belong to the caller.
- * 4 states; otherwise BEAGLE_ERROR_NO_IMPLEMENTATION.  No counterpart in
- * the reference. */
+ * Any state count up to 64: 4 states with at most 8 categories go through
+ * the node-fused kernel, every other case one branch per operation.  For
+ * S > 4 the instance needs BEAGLE_FLAG_PREORDER_TRANSPOSE_AUTO (the
+ * operations name untransposed matrices); otherwise, and above 64 states,
+ * BEAGLE_ERROR_NO_IMPLEMENTATION.  No counterpart in the reference. */
 int beagleExtCalculateRateMatrixGradient(int instance, int eigenIndex,
                                           const BeagleOperation* operations, int operationCount,
                                           const int* postBufferIndices, int categoryWeightsIndex,

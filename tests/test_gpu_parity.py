@@ -734,19 +734,63 @@ def test_rate_matrix_gradient_against_oracle_finite_differences(amd, oracle, sca
         assert abs(gpi[i] - fd) <= tol * max(abs(fd), np.abs(gpi).max()), (i, gpi[i], fd)
 
 
-@pytest.mark.parametrize("precision", ["double", "single"])
-def test_torch_autograd_binding(amd, oracle, precision):
+@pytest.mark.parametrize("S,C,scaling,precision", [(20, 4, "none", "double"),
+                                                   (20, 4, "auto", "single"),
+                                                   (20, 3, "auto", "double"),
+                                                   (61, 4, "none", "single"),
+                                                   (4, 9, "none", "double")])
+def test_rate_matrix_gradient_generic_against_oracle_finite_differences(amd, oracle, S, C,
+                                                                        scaling, precision):
+    """The rate-matrix gradient for state counts other than 4 (and 4 states
+    with more than 8 categories), one branch per pre-order operation:
+    dlogL/dQ along random generator directions E (E_ij = Q_ij z_ij, rows
+    summing to 0, so Q + h E stays a rate matrix and P stays >= 0: with 20
+    Dirichlet(1) frequencies P has entries near 1e-9 that an arbitrary
+    direction drives below the clamp at 0) and dlogL/dpi_root for a few
+    components, against fourth-order central differences of the oracle's
+    log-likelihood with P = expm(r t (Q + h E)) set explicitly."""
+    prob = synth.make_problem(83 + S + C, 9, 240, S=S, C=C, shape="yule", missing=0.05,
+                              mean_length=0.2)
+    ev = driver.Evaluator(amd, prob, scaling=scaling, precision=precision)
+    ev.fused_gradient(keep_tips=False)
+    G, gpi = ev.rate_matrix_gradient()
+    ev.close()
+    assert G.shape == (S, S) and gpi.shape == (S,)
+    rng = np.random.default_rng(5)
+    h = 2.5e-5
+    tol = 1e-8 if precision == "double" else 3e-5
+
+    def fd4(f):
+        return (8 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12 * h)
+
+    for _ in range(3):
+        E = prob.model.Q * rng.normal(size=(S, S))
+        np.fill_diagonal(E, 0.0)
+        np.fill_diagonal(E, -E.sum(axis=1))
+        fd = fd4(lambda s: _oracle_loglik(oracle, _with_model(prob, Q=prob.model.Q + s * E)))
+        got = float(np.sum(G * E))
+        assert abs(got - fd) <= tol * max(abs(fd), np.abs(G * E).sum()), (got, fd)
+    for i in rng.choice(S, size=4, replace=False):
+        e = np.eye(S)[i]
+        fd = fd4(lambda s: _oracle_loglik(oracle, _with_model(prob, freqs=prob.model.freqs + s * e)))
+        assert abs(gpi[i] - fd) <= tol * max(abs(fd), np.abs(gpi).max()), (i, gpi[i], fd)
+
+
+@pytest.mark.parametrize("precision,S", [("double", 4), ("single", 4), ("double", 20)])
+def test_torch_autograd_binding(amd, oracle, precision, S):
     """beagle_amd.torch_fn: logL of (branch lengths, GTR(exch, freqs)) and
     its autograd gradients against the oracle's log-likelihood and
-    fourth-order central differences of it."""
+    fourth-order central differences of it (20 states: 190
+    exchangeabilities, frequencies ~ Dirichlet(5), none near the stencil's
+    reach)."""
     import torch
     from beagle_amd import torch_fn
-    prob = synth.make_problem(71, 8, 300, S=4, shape="yule", missing=0.05, mean_length=0.2)
+    prob = synth.make_problem(71, 8, 300, S=S, shape="yule", missing=0.05, mean_length=0.2)
     lik = torch_fn.PhyloLikelihood(amd, prob.tree, prob.tips, prob.pattern_weights, prob.rates,
-                                   prob.weights, precision=precision)
+                                   prob.weights, precision=precision, states=S)
     rng = np.random.default_rng(9)
-    exch0 = rng.exponential(1.0, 6)
-    freqs0 = rng.dirichlet(np.ones(4))
+    exch0 = rng.exponential(1.0, S * (S - 1) // 2)
+    freqs0 = rng.dirichlet(np.full(S, 1.0 if S == 4 else 5.0))
     lengths0 = prob.tree.lengths[:-1].copy()
     exch = torch.tensor(exch0, requires_grad=True)
     freqs = torch.tensor(freqs0, requires_grad=True)
