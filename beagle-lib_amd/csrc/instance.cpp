@@ -206,6 +206,8 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
     if (const char* m = std::getenv("BEAGLE_AMD_AUTOTUNE")) autotune_ = std::atoi(m) != 0;
     ring_.resize(8);
     for (auto& s : ring_) HIPCHECK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+    jobs_.resize(8);
+    for (auto& e : jobs_) HIPCHECK(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming));
     sync();
 }
 
@@ -255,6 +257,11 @@ Instance::~Instance() {
         if (s.host) hipHostFree(s.host);
         if (s.dev) hipFree(s.dev);
         if (s.ev) hipEventDestroy(s.ev);
+    }
+    for (auto& e : jobs_) {
+        if (e.host) hipHostFree(e.host);
+        if (e.dev) hipFree(e.dev);
+        if (e.ev) hipEventDestroy(e.ev);
     }
     if (stream_) hipStreamDestroy(stream_);
 }
@@ -338,6 +345,48 @@ void* Instance::upload(const void* data, size_t bytes) {
     HIPCHECK(hipEventRecord(s.ev, stream_));
     s.used = true;
     return s.dev;
+}
+
+// Repeated traversals of one tree upload the same job lists: the list is
+// compared with the recent ones (bytes, so the same operations on the same
+// buffers) and a match is launched from its device copy.  Under graph
+// capture the graph keeps its own copy (upload).
+void* Instance::uploadJobs(const void* data, size_t bytes) {
+    if (capturing_) return upload(data, bytes);
+    if (jobsLast_ >= 0) {
+        HIPCHECK(hipEventRecord(jobs_[jobsLast_].ev, stream_));
+        jobs_[jobsLast_].pending = true;
+    }
+    ++jobsTick_;
+    int pick = 0;
+    for (int i = 0; i < (int)jobs_.size(); ++i) {
+        JobEntry& e = jobs_[i];
+        if (e.bytes == bytes && e.host && std::memcmp(e.host, data, bytes) == 0) {
+            e.last = jobsTick_;
+            jobsLast_ = i;
+            return e.dev;
+        }
+        if (e.last < jobs_[pick].last) pick = i;
+    }
+    JobEntry& e = jobs_[pick];
+    // its earlier kernels and its last transfer read these buffers
+    if (e.pending) HIPCHECK(hipEventSynchronize(e.ev));
+    e.pending = false;
+    if (e.cap < bytes) {
+        size_t cap = (std::max<size_t>(bytes, 4096) + 4095) / 4096 * 4096;
+        if (e.host) HIPCHECK(hipHostFree(e.host));
+        if (e.dev) HIPCHECK(hipFree(e.dev));
+        e.host = e.dev = nullptr;
+        HIPCHECK(hipHostMalloc(&e.host, cap, hipHostMallocDefault));
+        HIPCHECK(hipMalloc(&e.dev, cap));
+        e.cap = cap;
+    }
+    std::memcpy(e.host, data, bytes);
+    e.bytes = bytes;
+    HIPCHECK(hipMemcpyAsync(e.dev, e.host, bytes, hipMemcpyHostToDevice, stream_));
+    e.last = jobsTick_;
+    jobsLast_ = pick;
+    return e.dev;
 }
 
 void* Instance::grow(void*& buf, size_t& cap, size_t bytes) {
@@ -851,7 +900,7 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
     if (!fourState && mfma_post_supported(dims_.S, dims_.C)) {
         launchPostMfma(ops, stride, count);
     } else if (!fourState) {
-        auto d = reinterpret_cast<const PostJob*>(upload(hPost_.data(), count * sizeof(PostJob)));
+        auto d = reinterpret_cast<const PostJob*>(uploadJobs(hPost_.data(), count * sizeof(PostJob)));
         if (dp_) launch_post<double>(d, hPost_.data(), count, 0, dims_, dMatrices_, dDummy_, stream_);
         else launch_post<float>(d, hPost_.data(), count, 0, dims_, dMatrices_, dDummy_, stream_);
     } else {
@@ -1006,10 +1055,10 @@ void Instance::launchPostMfma(const int* ops, int stride, int count) {
     std::vector<PostJob> sorted(count);
     std::vector<int> fill(start.begin(), start.end() - 1);
     for (int o = 0; o < count; ++o) sorted[fill[level[o]]++] = hPost_[o];
-    auto dpj = reinterpret_cast<const PackJob*>(upload(pj.data(), pj.size() * sizeof(PackJob)));
+    auto dpj = reinterpret_cast<const PackJob*>(uploadJobs(pj.data(), pj.size() * sizeof(PackJob)));
     if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
-    auto d = reinterpret_cast<const PostJob*>(upload(sorted.data(), count * sizeof(PostJob)));
+    auto d = reinterpret_cast<const PostJob*>(uploadJobs(sorted.data(), count * sizeof(PostJob)));
     auto run = [&](int variant) {
         for (int l = 0; l < nl; ++l) {
             const int nj = start[l + 1] - start[l];
@@ -1066,7 +1115,7 @@ void Instance::launchPostPlan(const std::vector<Segment>& plan) {
         }
     }
     if (all.empty()) return;
-    auto d = reinterpret_cast<const PostJob*>(upload(all.data(), all.size() * sizeof(PostJob)));
+    auto d = reinterpret_cast<const PostJob*>(uploadJobs(all.data(), all.size() * sizeof(PostJob)));
     for (const Launch& L : launches) {
         PostJob* h = all.data() + L.first;
         if (L.batched && levelChunk_ <= 1) {
@@ -1182,10 +1231,10 @@ void Instance::launchPreMfma(const int* ops, int stride, int count) {
     std::vector<PreJob> sorted(count);
     std::vector<int> fill(start.begin(), start.end() - 1);
     for (int o = 0; o < count; ++o) sorted[fill[level[o]]++] = hPre_[o];
-    auto dpj = reinterpret_cast<const PackJob*>(upload(pj.data(), pj.size() * sizeof(PackJob)));
+    auto dpj = reinterpret_cast<const PackJob*>(uploadJobs(pj.data(), pj.size() * sizeof(PackJob)));
     if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
-    auto d = reinterpret_cast<const PreJob*>(upload(sorted.data(), count * sizeof(PreJob)));
+    auto d = reinterpret_cast<const PreJob*>(uploadJobs(sorted.data(), count * sizeof(PreJob)));
     for (int l = 0; l < nl; ++l) {
         if (dp_) launch_pre_mfma64(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
         else launch_pre_mfma(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
@@ -1231,10 +1280,10 @@ int Instance::enqueueDerivMfma(const int* postIdx, const int* preIdx, const int*
     const size_t nPart = (size_t)NR * count * ntiles;
     double* wk = reinterpret_cast<double*>(
         grow(dWork_, workCap_, (nPart + (size_t)NR * count) * sizeof(double)));
-    auto dpj = reinterpret_cast<const PackJob*>(upload(pj.data(), pj.size() * sizeof(PackJob)));
+    auto dpj = reinterpret_cast<const PackJob*>(uploadJobs(pj.data(), pj.size() * sizeof(PackJob)));
     if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
-    auto dj = reinterpret_cast<const DerivJob*>(upload(jobs.data(), count * sizeof(DerivJob)));
+    auto dj = reinterpret_cast<const DerivJob*>(uploadJobs(jobs.data(), count * sizeof(DerivJob)));
     if (dp_)
         launch_deriv_mfma64(dj, count, nd, dims_, pack, dCatWeights_ + (size_t)wIdx * dims_.C,
                             dPatternWeights_, sites, wk, stream_);
@@ -1256,7 +1305,7 @@ void Instance::launchPre(const int* ops, int stride, int count, double* partial)
     }
     if (!fourState) {
         if (!dScratch_) HIPCHECK(hipMalloc(&dScratch_, partialsBytes_));
-        auto d = reinterpret_cast<const PreJob*>(upload(hPre_.data(), count * sizeof(PreJob)));
+        auto d = reinterpret_cast<const PreJob*>(uploadJobs(hPre_.data(), count * sizeof(PreJob)));
         const int t = preTransposed_ ? 1 : 0;
         if (dp_) launch_pre<double>(d, hPre_.data(), count, dims_, t, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
         else launch_pre<float>(d, hPre_.data(), count, dims_, t, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
@@ -1327,7 +1376,7 @@ void Instance::launchPrePlan(const std::vector<Segment>& plan, const double* cw,
         }
     }
     if (all.empty()) return;
-    auto d = reinterpret_cast<const PreJob*>(upload(all.data(), all.size() * sizeof(PreJob)));
+    auto d = reinterpret_cast<const PreJob*>(uploadJobs(all.data(), all.size() * sizeof(PreJob)));
     for (const Launch& L : launches) {
         if (L.batched) {
             const int start[2] = {0, L.n};
@@ -1622,7 +1671,7 @@ void Instance::launchNodePlan(const std::vector<NodeJob>& nodes, const std::vect
     const size_t perJob = (size_t)dims_.C * 6 * 20;
     for (size_t j = 0; j < all.size(); ++j) all[j].mbase = (int)(j * perJob);
     void* jm = grow(dJobMats_, jobMatsCap_, all.size() * perJob * realSize());
-    auto d = reinterpret_cast<const NodeJob*>(upload(all.data(), all.size() * sizeof(NodeJob)));
+    auto d = reinterpret_cast<const NodeJob*>(uploadJobs(all.data(), all.size() * sizeof(NodeJob)));
     if (dp_) launch_gather_node_matrices<double>(d, (int)all.size(), dims_, dMatrices_, jm, stream_);
     else launch_gather_node_matrices<float>(d, (int)all.size(), dims_, dMatrices_, jm, stream_,
                                             node_f32_supported(dims_.C));
@@ -2031,7 +2080,7 @@ int Instance::calculateEdgeDerivatives(const int* postIdx, const int* preIdx, co
     double* wk = reinterpret_cast<double*>(grow(dWork_, workCap_, (work + 2 * count + extra) * sizeof(double)));
     double* dSums = wk + work;
     double* dSites = outD ? dSums + 2 * count : nullptr;
-    auto dj = reinterpret_cast<const EdgeJob*>(upload(jobs.data(), count * sizeof(EdgeJob)));
+    auto dj = reinterpret_cast<const EdgeJob*>(uploadJobs(jobs.data(), count * sizeof(EdgeJob)));
     if (dp_)
         launch_edge_derivatives<double>(dj, count, dCatWeights_ + (size_t)w * dims_.C,
                                         dPatternWeights_, dSites, wk, dSums, dSums + count,
@@ -2078,7 +2127,7 @@ int Instance::calculateCrossProducts(const int* postIdx, const int* preIdx, cons
         grow(dWork_, workCap_, (rows * S2 + S2) * sizeof(double)));
     double* dOut = wk + rows * S2;
     copyIn(dOut, out, S2 * sizeof(double));
-    auto dj = reinterpret_cast<const EdgeJob*>(upload(jobs.data(), count * sizeof(EdgeJob)));
+    auto dj = reinterpret_cast<const EdgeJob*>(uploadJobs(jobs.data(), count * sizeof(EdgeJob)));
     if (dp_)
         launch_cross_products<double>(dj, count, dCatWeights_ + (size_t)w * dims_.C,
                                       dCatRates_ + (size_t)r * dims_.C, dPatternWeights_, wk, dOut,

@@ -164,6 +164,9 @@ private:
     };
 
     void* upload(const void* data, size_t bytes);
+    // Job lists that depend only on the operations (and the buffers they
+    // name): an identical list reuses its device copy, no transfer.
+    void* uploadJobs(const void* data, size_t bytes);
     // SYNCH instances wait at the end of an update call (not while capturing)
     void finishCall();
 
@@ -320,6 +323,20 @@ private:
     int partitionCount_ = 1;
     std::vector<Slot> ring_;
     int ringPos_ = 0;
+    // device copies of recent job lists (least recently used replaced);
+    // an entry's event, recorded when the next list is asked for, covers
+    // the kernels launched with it
+    struct JobEntry {
+        void* host = nullptr;   // pinned copy of the bytes (the transfer's source)
+        void* dev = nullptr;
+        size_t cap = 0, bytes = 0;
+        hipEvent_t ev = nullptr;
+        unsigned long long last = 0;
+        bool pending = false;
+    };
+    std::vector<JobEntry> jobs_;
+    int jobsLast_ = -1;
+    unsigned long long jobsTick_ = 0;
     std::vector<PostJob> hPost_;
     std::vector<PreJob> hPre_;
     double lastLogL_ = 0.0, lastD1_ = 0.0, lastD2_ = 0.0;
