@@ -74,6 +74,51 @@ def gtr(rng, S=4):
     return model_from(exch, freqs)
 
 
+def nonreversible(rng, S=4):
+    """Non-reversible model with complex eigenvalues: random rates R_ij ~
+    Exp(1) (i != j), Q = R - diag(R 1), normalised to one expected
+    substitution at its stationary distribution.  The eigen-system in the
+    real form BEAGLE_FLAG_EIGEN_COMPLEX takes (EigenDecompositionSquare.hpp:
+    108-185): a conjugate pair a +- ib with eigenvector x + iy becomes the
+    columns x, y of V with the block [[a, b], [-b, a]], evals = S real parts
+    then S imaginary parts (b on the pair's first row, -b on its second)."""
+    for _ in range(100):
+        R = rng.exponential(1.0, (S, S))
+        np.fill_diagonal(R, 0.0)
+        Q = R - np.diag(R.sum(axis=1))
+        w, vl = np.linalg.eig(Q.T)
+        pi = np.real(vl[:, np.argmin(np.abs(w))])
+        pi = pi / pi.sum()
+        Q = Q / -(pi * np.diag(Q)).sum()
+        lam, vec = np.linalg.eig(Q)
+        if np.any(np.abs(lam.imag) > 1e-6):
+            break
+    V = np.zeros((S, S))
+    re = np.zeros(S)
+    im = np.zeros(S)
+    order = np.argsort(-lam.real, kind="stable")
+    done = np.zeros(S, bool)
+    k = 0
+    for i in order:
+        if done[i]:
+            continue
+        if abs(lam[i].imag) < 1e-12:
+            V[:, k], re[k] = np.real(vec[:, i]), lam[i].real
+            done[i] = True
+            k += 1
+            continue
+        j = min((m for m in range(S) if not done[m] and m != i),
+                key=lambda m: abs(lam[m] - np.conj(lam[i])))
+        a, b = (i, j) if lam[i].imag > 0 else (j, i)   # a: positive imaginary part
+        V[:, k], V[:, k + 1] = np.real(vec[:, a]), np.imag(vec[:, a])
+        re[k] = re[k + 1] = lam[a].real
+        im[k], im[k + 1] = lam[a].imag, -lam[a].imag
+        done[a] = done[b] = True
+        k += 2
+    Vi = np.linalg.inv(V)
+    return Model(Q=Q, freqs=pi, evec=V, ivec=Vi, evals=np.concatenate([re, im]))
+
+
 def jc69():
     return model_from(np.ones(6), np.full(4, 0.25))
 

@@ -47,6 +47,7 @@ const long kGpuSupport =
     BEAGLE_FLAG_SCALING_AUTO |
     BEAGLE_FLAG_THREADING_NONE | BEAGLE_FLAG_VECTOR_NONE | BEAGLE_FLAG_PROCESSOR_GPU |
     BEAGLE_FLAG_SCALERS_LOG | BEAGLE_FLAG_SCALERS_RAW | BEAGLE_FLAG_EIGEN_REAL |
+    BEAGLE_FLAG_EIGEN_COMPLEX |
     BEAGLE_FLAG_INVEVEC_STANDARD | BEAGLE_FLAG_INVEVEC_TRANSPOSED |
     BEAGLE_FLAG_PREORDER_TRANSPOSE_MANUAL | (long)BEAGLE_FLAG_PREORDER_TRANSPOSE_AUTO |
     BEAGLE_FLAG_PARALLELOPS_GRID | BEAGLE_FLAG_PARALLELOPS_STREAMS | BEAGLE_FLAG_FRAMEWORK_HIP;
@@ -203,7 +204,9 @@ int beagleCreateInstance(int tipCount, int partialsBufferCount, int compactBuffe
         int err = BEAGLE_ERROR_NO_RESOURCE;
         for (const Cand& c : cands) {
             const long both = preferenceFlags | requirementFlags;
-            if (both & (BEAGLE_FLAG_SCALING_DYNAMIC | BEAGLE_FLAG_EIGEN_COMPLEX)) {
+            // complex eigen-systems stage V and V^-1 in LDS: up to 62 states
+            if ((both & BEAGLE_FLAG_SCALING_DYNAMIC) ||
+                ((both & BEAGLE_FLAG_EIGEN_COMPLEX) && stateCount > 62)) {
                 err = BEAGLE_ERROR_NO_IMPLEMENTATION;
                 continue;
             }

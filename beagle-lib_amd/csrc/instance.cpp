@@ -119,8 +119,10 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
         throw std::out_of_range("instance dimensions");
     HIPCHECK(hipSetDevice(device_));
     const long both = preferenceFlags | requirementFlags;
+    complex_ = (both & BEAGLE_FLAG_EIGEN_COMPLEX) != 0;
     flags_ = BEAGLE_FLAG_PROCESSOR_GPU | BEAGLE_FLAG_VECTOR_NONE | BEAGLE_FLAG_THREADING_NONE |
-             BEAGLE_FLAG_EIGEN_REAL | BEAGLE_FLAG_FRAMEWORK_HIP | BEAGLE_FLAG_PARALLELOPS_GRID;
+             (complex_ ? BEAGLE_FLAG_EIGEN_COMPLEX : BEAGLE_FLAG_EIGEN_REAL) |
+             BEAGLE_FLAG_FRAMEWORK_HIP | BEAGLE_FLAG_PARALLELOPS_GRID;
     flags_ |= dp_ ? BEAGLE_FLAG_PRECISION_DOUBLE : BEAGLE_FLAG_PRECISION_SINGLE;
     flags_ |= (both & BEAGLE_FLAG_COMPUTATION_ASYNCH) ? BEAGLE_FLAG_COMPUTATION_ASYNCH
                                                       : BEAGLE_FLAG_COMPUTATION_SYNCH;
@@ -174,7 +176,7 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
     const size_t S = stateCount;
     dCube_ = dalloc<double>(eigenCount_ * S * S * S);
     dVVi_ = dalloc<double>(eigenCount_ * 2 * S * S);
-    dEvals_ = dalloc<double>(eigenCount_ * S);
+    dEvals_ = dalloc<double>(eigenCount_ * 2 * S);   // real parts, then imaginary parts
     eigenSet_.assign(eigenCount_, false);
     hEigen_.resize(eigenCount_);
     matOrigin_.resize(matrixCount_);
@@ -530,7 +532,9 @@ int Instance::setEigenDecomposition(int e, const double* V, const double* Vi, co
             vv[(size_t)S * S + (size_t)i * S + k] = trans ? Vi[i + k * S] : Vi[i * S + k];
         }
     copyIn(dVVi_ + (size_t)e * 2 * S * S, vv.data(), vv.size() * sizeof(double));
-    copyIn(dEvals_ + (size_t)e * S, lam, S * sizeof(double));
+    // a complex system passes S real then S imaginary parts
+    // (EigenDecompositionSquare.hpp:100, 121)
+    copyIn(dEvals_ + (size_t)e * 2 * S, lam, (complex_ ? 2 : 1) * S * sizeof(double));
     eigenSet_[e] = true;
     // host copy for the rate-matrix gradient: V, V^-1 (row-major), lambda
     HostEigen& h = hEigen_[e];
@@ -671,7 +675,8 @@ int Instance::updateTransitionMatrices(int e, const int* probIdx, const int* d1I
         }
         J.cube = dCube_ + (size_t)e * S * S * S;
         J.vv = dVVi_ + (size_t)e * 2 * S * S;
-        J.evals = dEvals_ + (size_t)e * S;
+        J.evals = dEvals_ + (size_t)e * 2 * S;
+        J.cplx = complex_ ? 1 : 0;
         J.rates = dCatRates_;
         J.t = t[u];
         J.cat = -1;
@@ -705,7 +710,8 @@ int Instance::updateTransitionMatricesWithModelCategories(const int* eig, const 
             J.D2 = (d1Idx && d2Idx) ? matrix(d2Idx[u]) : nullptr;
             J.cube = dCube_ + (size_t)eig[c] * S * S * S;
             J.vv = dVVi_ + (size_t)eig[c] * 2 * S * S;
-            J.evals = dEvals_ + (size_t)eig[c] * S;
+            J.evals = dEvals_ + (size_t)eig[c] * 2 * S;
+            J.cplx = complex_ ? 1 : 0;
             J.rates = nullptr;
             J.t = t[u];
             J.cat = c;
@@ -735,7 +741,8 @@ int Instance::updateTransitionMatricesWithMultipleModels(const int* eig, const i
         J.D2 = (d1Idx && d2Idx && validMatrix(d2Idx[u])) ? matrix(d2Idx[u]) : nullptr;
         J.cube = dCube_ + (size_t)e * S * S * S;
         J.vv = dVVi_ + (size_t)e * 2 * S * S;
-        J.evals = dEvals_ + (size_t)e * S;
+        J.evals = dEvals_ + (size_t)e * 2 * S;
+        J.cplx = complex_ ? 1 : 0;
         J.rates = dCatRates_ + (size_t)r * dims_.C;
         J.t = t[u];
         J.cat = -1;
@@ -1709,7 +1716,7 @@ int Instance::attachOwnOperands(int count, const int* postIdx, const int* d1Idx,
 int Instance::rateMatrixGradient(int e, const int* ops, int count, const int* postIdx, int wIdx,
                                  int rootIdx, int fIdx, double* outQ, double* outFreqs) {
     const int S = dims_.S, C = dims_.C;
-    if (preTransposed_) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    if (preTransposed_ || complex_) return BEAGLE_ERROR_NO_IMPLEMENTATION;
     if (!validEigen(e) || !eigenSet_[e] || hEigen_[e].lam.empty() || !validEigen(wIdx) ||
         hCatWeights_[wIdx].empty())
         return BEAGLE_ERROR_OUT_OF_RANGE;

@@ -254,6 +254,7 @@ private:
     long flags_;
     bool dp_;
     bool preTransposed_;   // S > 4 without PREORDER_TRANSPOSE_AUTO
+    bool complex_ = false; // EIGEN_COMPLEX: 2 S eigenvalues, conjugate pairs as 2 x 2 blocks
     bool always_;          // SCALING_ALWAYS
     bool auto_;            // SCALING_AUTO: power-of-two exponents per buffer
     bool logScalers_;

@@ -88,10 +88,11 @@ struct MatrixJob {
     void* P; void* D1; void* D2;      // D1/D2 may be null
     const double* cube;               // S^3, C_ijk = V_ik Vinv_kj
     const double* vv;                 // [V | Vinv], each S x S row-major
-    const double* evals;              // S
+    const double* evals;              // S real parts, then S imaginary parts
     const double* rates;              // C category rates
     double t;
     int cat;                          // >= 0: only this category, rate 1 (model categories)
+    int cplx;                         // complex eigen-system (conjugate pairs as 2 x 2 blocks)
 };
 
 // One matrix product per category: out = A B, pad column 1.

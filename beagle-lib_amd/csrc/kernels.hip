@@ -130,6 +130,68 @@ __global__ __launch_bounds__(WG) void k_reduce_rows(const double* __restrict__ i
 // state counts whose V and V^-1 fit the 64 KB of LDS a workgroup may use
 constexpr int kMatLdsMaxS = 62;
 
+// Complex eigen-system (EigenDecompositionSquare.hpp:108-185): the rows of
+// V^-1 are scaled by e^(a d) for a real eigenvalue a, or taken in conjugate
+// pairs (a +- ib, b the imaginary part of the pair's first row) and rotated,
+//   row_i' = e^(a d) (cos(b d) row_i + sin(b d) row_i+1),
+//   row_i+1' = e^(a d) (cos(b d) row_i+1 - sin(b d) row_i),
+// then P = V T, clamped at 0, pad column 1.  Like the reference, no
+// derivative matrices.  S <= kMatLdsMaxS (V and T staged in LDS).
+template <typename R>
+__device__ void matrices_complex(const MatrixJob& J, int S, double r, int c, double* sh) {
+    double* ex = sh;
+    double* ec = sh + S;
+    double* es = sh + 2 * S;
+    double* V = sh + 3 * S;
+    double* T = V + S * S;
+    int* kind = reinterpret_cast<int*>(T + S * S);   // 0 real, 1 first of a pair, 2 second
+    const double d = J.t * r;
+    for (int k = threadIdx.x; k < S; k += WG) {
+        const double e = exp(J.evals[k] * d), b = J.evals[S + k] * d;
+        ex[k] = e;
+        ec[k] = e * cos(b);
+        es[k] = e * sin(b);
+    }
+    for (int t = threadIdx.x; t < S * S; t += WG) {
+        V[t] = J.vv[t];
+        T[t] = J.vv[S * S + t];
+    }
+    if (threadIdx.x == 0) {
+        // pairs are found in row order, as the reference walks them (a
+        // nonzero imaginary part on the last row has no partner: real)
+        for (int i = 0; i < S;) {
+            if (J.evals[S + i] != 0.0 && i + 1 < S) {
+                kind[i] = 1;
+                kind[i + 1] = 2;
+                i += 2;
+            } else {
+                kind[i++] = 0;
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < S * S; t += WG) {
+        const int i = t / S, j = t - i * S;
+        if (kind[i] == 0) {
+            T[t] *= ex[i];
+        } else if (kind[i] == 1) {   // this thread owns (i, j) and (i + 1, j)
+            const double a = T[t], b = T[t + S];
+            T[t] = ec[i] * a + es[i] * b;
+            T[t + S] = ec[i] * b - es[i] * a;
+        }
+    }
+    __syncthreads();
+    const int W = S + 1;
+    R* Pm = reinterpret_cast<R*>(J.P) + (long)c * S * W;
+    for (int t = threadIdx.x; t < S * S; t += WG) {
+        const int i = t / S, j = t - i * S;
+        double s0 = 0.0;
+        for (int k = 0; k < S; ++k) s0 += V[i * S + k] * T[k * S + j];
+        Pm[i * W + j] = (R)(s0 > 0.0 ? s0 : 0.0);
+    }
+    for (int i = threadIdx.x; i < S; i += WG) Pm[i * W + S] = (R)1.0;
+}
+
 template <typename R>
 __global__ __launch_bounds__(WG) void k_matrices(const MatrixJob* __restrict__ jobs, int S, int C) {
     extern __shared__ double sh[];
@@ -140,6 +202,10 @@ __global__ __launch_bounds__(WG) void k_matrices(const MatrixJob* __restrict__ j
     if (J.cat >= 0 && blockIdx.y != 0) return;   // one category per job
     const int c = J.cat >= 0 ? J.cat : (int)blockIdx.y;
     const double r = J.cat >= 0 ? 1.0 : J.rates[c];
+    if (J.cplx) {
+        matrices_complex<R>(J, S, r, c, sh);
+        return;
+    }
     const bool deriv = J.D1 != nullptr;
     for (int k = threadIdx.x; k < S; k += WG) {
         const double lam = J.evals[k];
@@ -221,7 +287,8 @@ template <typename R>
 void launch_matrices(const MatrixJob* jobs, int njobs, int S, int C, hipStream_t st) {
     if (njobs <= 0) return;
     dim3 grid(njobs, C);
-    const size_t lds = (3 * S + (S <= kMatLdsMaxS ? 2 * S * S : 0)) * sizeof(double);
+    const size_t lds = (3 * S + (S <= kMatLdsMaxS ? 2 * S * S : 0)) * sizeof(double) +
+                       (S <= kMatLdsMaxS ? S * sizeof(int) : 0);   // complex: row kinds
     hipLaunchKernelGGL(k_matrices<R>, grid, dim3(WG), lds, st, jobs, S, C);
 }
 

@@ -911,3 +911,80 @@ def test_checkpointed_gradient_states(amd, S, precision, scaling):
         np.testing.assert_array_equal(g1, full[1])
         np.testing.assert_array_equal(g2, full[2])
     ev.close()
+
+
+@pytest.mark.parametrize("S,precision", [(4, "double"), (5, "double"), (20, "double"),
+                                         (61, "double"), (20, "single")])
+def test_complex_eigen_matrices(amd, S, precision):
+    """BEAGLE_FLAG_EIGEN_COMPLEX: conjugate pairs as 2 x 2 rotation blocks
+    (EigenDecompositionSquare.hpp:108-185); P(r t) against scipy expm,
+    clamped at 0, for the category-rate and the model-category updates; the
+    derivative matrices are left alone, as in the reference."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(40 + S)
+    m = synth.nonreversible(rng, S)
+    assert np.any(m.evals[S:] != 0)
+    rates = np.array([0.4, 1.0, 1.6])
+    lengths = np.array([0.02, 0.3, 1.1])
+    prefs = api.FLAG_PRECISION_DOUBLE if precision == "double" else api.FLAG_PRECISION_SINGLE
+    inst = amd.create(2, 3, 2, S, 8, 3, 12, 3, 1, preference=prefs,
+                      requirement=api.FLAG_EIGEN_COMPLEX)
+    tol = 1e-12 if precision == "double" else 1e-6
+    try:
+        assert inst.flags & api.FLAG_EIGEN_COMPLEX
+        for e in range(3):
+            inst.set_eigen(e, m.evec.reshape(-1), m.ivec.reshape(-1), m.evals)
+        inst.set_category_rates(rates)
+        marker = np.full(3 * S * S, 7.0)
+        inst.set_transition_matrix(3, marker, 0.0)
+        inst.update_transition_matrices(0, [0, 1, 2], [3, 4, 5], None, lengths)
+        for u, t in enumerate(lengths):
+            got = inst.get_transition_matrix(u).reshape(3, S, S)
+            for c, r in enumerate(rates):
+                np.testing.assert_allclose(got[c], np.maximum(expm(m.Q * (r * t)), 0.0),
+                                           rtol=0, atol=tol)
+        np.testing.assert_array_equal(inst.get_transition_matrix(3), marker)
+        inst.update_transition_matrices_model_categories([0, 1, 2], [6, 7, 8], None, None,
+                                                         lengths)
+        for u, t in enumerate(lengths):
+            got = inst.get_transition_matrix(6 + u).reshape(3, S, S)
+            for c in range(3):
+                np.testing.assert_allclose(got[c], np.maximum(expm(m.Q * t), 0.0), rtol=0,
+                                           atol=tol)
+    finally:
+        inst.finalize()
+
+
+@pytest.mark.parametrize("S,scaling", [(4, "none"), (20, "auto")])
+def test_complex_eigen_likelihood_matches_oracle(amd, oracle, S, scaling):
+    """A non-reversible model through the complex eigen path: logL, site
+    logL and branch derivative sums against the oracle fed expm(Q r t)
+    explicitly (derivative matrices r Q, r^2 Q^2 on both sides); the
+    rate-matrix gradient is refused for complex systems, as is a complex
+    instance of more than 62 states."""
+    rng = np.random.default_rng(60 + S)
+    model = synth.nonreversible(rng, S)
+    prob = synth.make_problem(61 + S, 12, 400, S=S, shape="yule", missing=0.05, model=model)
+    ev = driver.Evaluator(amd, prob, scaling=scaling, extra_flags=api.FLAG_EIGEN_COMPLEX)
+    assert ev.inst.flags & api.FLAG_EIGEN_COMPLEX
+    lg = ev.log_likelihood()
+    sites = ev.inst.site_log_likelihoods()
+    ev.pre_order()
+    _, s1, _ = ev.edge_derivatives(False, per_site=False)
+    with pytest.raises(api.BeagleError) as e:
+        ev.rate_matrix_gradient()
+    assert e.value.code == api.ERROR_NO_IMPLEMENTATION
+    ev.close()
+    ev = driver.Evaluator(oracle, prob, scaling="manual" if scaling == "auto" else scaling,
+                          matrices="explicit", transpose="none")
+    lo = ev.log_likelihood()
+    so = ev.inst.site_log_likelihoods()
+    ev.pre_order()
+    _, o1, _ = ev.edge_derivatives(False, per_site=False)
+    ev.close()
+    assert rel(lg, lo) < 1e-10, (lg, lo)
+    close(sites, so, 1e-10, 1e-10)
+    close(s1, o1, 1e-8, 1e-8)
+    with pytest.raises(api.BeagleError) as e:
+        amd.create(2, 3, 2, 64, 8, 1, 4, 1, 1, requirement=api.FLAG_EIGEN_COMPLEX)
+    assert e.value.code == api.ERROR_NO_IMPLEMENTATION

@@ -426,3 +426,29 @@ def test_model_categories_closed_form(oracle):
     conv = got[9].reshape(3, 4, 4)
     for l, m in enumerate(models):
         np.testing.assert_allclose(conv[l], expm(m.Q * 0.43), atol=1e-12)
+
+
+@pytest.mark.parametrize("S", [4, 20])
+def test_complex_eigen_reference_matches_expm(ref, S):
+    """BEAGLE_FLAG_EIGEN_COMPLEX in the reference CPU library
+    (EigenDecompositionSquare.hpp:108-185) with the real block form that
+    synth.nonreversible builds: P(r t) = expm(Q r t), clamped at 0 — pins
+    the convention the HIP library's complex path follows."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(20 + S)
+    m = synth.nonreversible(rng, S)
+    assert np.any(m.evals[S:] != 0)
+    rates = np.array([0.4, 1.0, 1.6])
+    lengths = np.array([0.02, 0.3, 1.1])
+    inst = ref.create(2, 3, 2, S, 8, 1, 4, 3, 1, requirement=api.FLAG_EIGEN_COMPLEX)
+    try:
+        inst.set_eigen(0, m.evec.reshape(-1), m.ivec.reshape(-1), m.evals)
+        inst.set_category_rates(rates)
+        inst.update_transition_matrices(0, [0, 1, 2], None, None, lengths)
+        for u, t in enumerate(lengths):
+            got = inst.get_transition_matrix(u).reshape(3, S, S)
+            for c, r in enumerate(rates):
+                np.testing.assert_allclose(got[c], np.maximum(expm(m.Q * (r * t)), 0.0),
+                                           rtol=0, atol=1e-12)
+    finally:
+        inst.finalize()
