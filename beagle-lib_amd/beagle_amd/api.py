@@ -408,6 +408,24 @@ class Instance:
                      res.ctypes.data_as(c_double_p), allow=allow)
         return float(res[0]) if rc >= 0 else rc
 
+    def root_log_likelihoods(self, bufs, weights, freqs, cumulative):
+        """Several roots combined into one likelihood per pattern
+        (count = len(bufs) > 1); returns the summed log-likelihood."""
+        res = np.zeros(1)
+        a, b, c, d = _ip(bufs), _ip(weights), _ip(freqs), _ip(cumulative)
+        self._c("CalculateRootLogLikelihoods", a[1], b[1], c[1], d[1], len(a[0]),
+                res.ctypes.data_as(c_double_p))
+        return float(res[0])
+
+    def root_log_likelihood_per_category(self, buf, freqs_index=0, cumulative=OP_NONE):
+        """Category weights index -1: per-category site log-likelihoods as a
+        (categories, patterns) array."""
+        res = np.zeros(self.C * self.P)
+        a, b, c, d = _ip([buf]), _ip([-1]), _ip([freqs_index]), _ip([cumulative])
+        self._c("CalculateRootLogLikelihoods", a[1], b[1], c[1], d[1], 1,
+                res.ctypes.data_as(c_double_p))
+        return res.reshape(self.C, self.P)
+
     def root_log_likelihood_by_partition(self, bufs, weights, freqs, cumulative, parts):
         n = len(parts)
         by = np.zeros(n)

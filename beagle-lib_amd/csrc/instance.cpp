@@ -1964,7 +1964,11 @@ int Instance::getScaleFactors(int src, double* out) {
 
 int Instance::calculateRootLogLikelihoods(const int* bufIdx, const int* wIdx, const int* fIdx,
                                           const int* cumIdx, int count, double* out) {
-    if (count != 1) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    if (count < 1) return BEAGLE_ERROR_OUT_OF_RANGE;
+    if (count > 1) return rootLogLikelihoodsMulti(bufIdx, wIdx, fIdx, cumIdx, count, out);
+    if (wIdx[0] < 0)
+        return rootLogLikelihoodsPerCategory(bufIdx[0], fIdx[0],
+                                             cumIdx ? cumIdx[0] : BEAGLE_OP_NONE, out);
     const int rc = enqueueRoot(bufIdx[0], wIdx[0], fIdx[0], cumIdx[0]);
     if (rc != BEAGLE_SUCCESS) return rc;
     if (capturing_) {
@@ -1978,9 +1982,65 @@ int Instance::calculateRootLogLikelihoods(const int* bufIdx, const int* wIdx, co
     return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
 }
 
+// Several roots, one likelihood (BeagleCPUImpl.hpp:2474-2586): scale
+// factors apply when the first subset names a cumulative buffer (or with
+// SCALING_ALWAYS, each root's own), and then every subset must name one;
+// automatic scaling uses each root's exponents.  Site values are kept for
+// beagleGetSiteLogLikelihoods.
+int Instance::rootLogLikelihoodsMulti(const int* bufIdx, const int* wIdx, const int* fIdx,
+                                      const int* cumIdx, int count, double* out) {
+    const bool manual = !auto_ && (always_ || (cumIdx && cumIdx[0] != BEAGLE_OP_NONE));
+    std::vector<RootSubset> subs(count);
+    for (int s = 0; s < count; ++s) {
+        const int b = bufIdx[s], w = wIdx[s], f = fIdx[s];
+        if (!validBuffer(b) || !dPartials_[b] || !validEigen(w) || !validEigen(f) ||
+            hCatWeights_[w].empty() || hFreqs_[f].empty())
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+        const int cs = manual ? (always_ ? b - tipCount_ : cumIdx[s]) : BEAGLE_OP_NONE;
+        if (manual && !validScale(cs)) return BEAGLE_ERROR_OUT_OF_RANGE;
+        subs[s] = RootSubset{dPartials_[b], dCatWeights_ + (size_t)w * dims_.C,
+                             dFreqs_ + (size_t)f * dims_.S, manual ? scale(cs) : nullptr,
+                             auto_ ? dExp_[b] : nullptr};
+    }
+    const int nb = (dims_.P + 255) / 256;   // the kernel's workgroup sums
+    double* work = reinterpret_cast<double*>(grow(dWork_, workCap_, (size_t)nb * sizeof(double)));
+    auto d = reinterpret_cast<const RootSubset*>(upload(subs.data(), count * sizeof(RootSubset)));
+    if (dp_) launch_root_multi<double>(d, count, dPatternWeights_, dSiteL_, work, dims_, stream_);
+    else launch_root_multi<float>(d, count, dPatternWeights_, dSiteL_, work, dims_, stream_);
+    launch_reduce_rows(work, nb, 1, dResult_, stream_);
+    HIPCHECK(hipGetLastError());
+    copyOut(hResult_, dResult_, sizeof(double));
+    lastLogL_ = hResult_[0];
+    *out = lastLogL_;
+    return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+}
+
+// categoryWeightsIndices[0] < 0: out[c P + p] = per-category site
+// log-likelihoods, C x P values, in the instance's pattern order, as
+// calcRootLogLikelihoodsPerCategory (BeagleCPUImpl.hpp:2589-2625) writes.
+int Instance::rootLogLikelihoodsPerCategory(int b, int f, int cum, double* out) {
+    if (!validBuffer(b) || !dPartials_[b] || !validEigen(f) || hFreqs_[f].empty())
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    const int cs = always_ ? b - tipCount_ : (auto_ ? BEAGLE_OP_NONE : cum);
+    if (cs != BEAGLE_OP_NONE && !validScale(cs)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    const size_t n = (size_t)dims_.C * dims_.P;
+    double* dst = reinterpret_cast<double*>(grow(dWork_, workCap_, n * sizeof(double)));
+    if (dp_)
+        launch_root_per_category<double>(dPartials_[b], dFreqs_ + (size_t)f * dims_.S,
+                                         cs >= 0 ? scale(cs) : nullptr,
+                                         auto_ ? dExp_[b] : nullptr, dst, dims_, stream_);
+    else
+        launch_root_per_category<float>(dPartials_[b], dFreqs_ + (size_t)f * dims_.S,
+                                        cs >= 0 ? scale(cs) : nullptr,
+                                        auto_ ? dExp_[b] : nullptr, dst, dims_, stream_);
+    HIPCHECK(hipGetLastError());
+    copyOut(out, dst, n * sizeof(double));
+    return BEAGLE_SUCCESS;
+}
+
 // Root log-likelihood of buffer b into dResult_[0] (device), not waited for.
 int Instance::enqueueRoot(int b, int w, int f, int cum) {
-    if (w < 0) return BEAGLE_ERROR_NO_IMPLEMENTATION;  // per-category output
+    if (w < 0) return BEAGLE_ERROR_NO_IMPLEMENTATION;  // per-category output: not here
     if (!validBuffer(b) || !dPartials_[b] || !validEigen(w) || !validEigen(f) ||
         hCatWeights_[w].empty() || hFreqs_[f].empty())
         return BEAGLE_ERROR_OUT_OF_RANGE;

@@ -102,6 +102,9 @@ public:
                                          double* outD1, double* outD1Sq, double* outD2);
     int rateMatrixGradient(int eigenIdx, const int* ops, int count, const int* postIdx, int wIdx,
                            int rootIdx, int fIdx, double* outQ, double* outFreqs);
+    int rootLogLikelihoodsMulti(const int* bufIdx, const int* wIdx, const int* fIdx,
+                                const int* cumIdx, int count, double* out);
+    int rootLogLikelihoodsPerCategory(int b, int f, int cum, double* out);
     int rateMatrixGradientGeneric(int eigenIdx, const int* ops, int count, const int* postIdx,
                                   int wIdx, int rootIdx, int fIdx, double* outQ,
                                   double* outFreqs);

@@ -95,6 +95,15 @@ struct MatrixJob {
     int cplx;                         // complex eigen-system (conjugate pairs as 2 x 2 blocks)
 };
 
+// One root of a several-root likelihood (calcRootLogLikelihoodsMulti).
+struct RootSubset {
+    const void* root;
+    const double* cw;      // category weights
+    const double* freqs;   // state frequencies
+    const double* cum;     // cumulative scale factors (logs) or null
+    const short* ex;       // automatic-scaling exponents or null
+};
+
 // One matrix product per category: out = A B, pad column 1.
 struct ConvolveJob {
     const void* A; const void* B; void* out;
@@ -282,6 +291,15 @@ template <typename R>
 void launch_qgrad_generic(const PreJob* djobs, int nops, const Dims& d, const double* cw,
                           const double* pw, const double* eig, const double* tau, double* coef,
                           double* xs, int chunks, double* z, hipStream_t st);
+// Several roots, one likelihood per pattern: site[p], block sums of
+// w_p site[p] in bsum (ceil(P / WG) of them).
+template <typename R>
+void launch_root_multi(const RootSubset* subs, int count, const double* pw, double* site,
+                       double* bsum, const Dims& d, hipStream_t st);
+// Per-category root log-likelihoods out[c * P + p] (no category weights).
+template <typename R>
+void launch_root_per_category(const void* root, const double* freqs, const double* cum,
+                              const short* ex, double* out, const Dims& d, hipStream_t st);
 // Root-frequency gradient block sums: rows i of [S][ceil(P / 256)].
 template <typename R>
 void launch_root_freq_grad(const void* root, const double* cw, const double* freqs,

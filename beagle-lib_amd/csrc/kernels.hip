@@ -2435,6 +2435,89 @@ __global__ __launch_bounds__(WG) void k_root(const R* __restrict__ root,
     if (threadIdx.x == 0) bsum[blockIdx.x] = b;
 }
 
+// Several roots combined into one likelihood per pattern
+// (calcRootLogLikelihoodsMulti, BeagleCPUImpl.hpp:2474-2586): subset s
+// contributes sum_i pi_s[i] sum_c w_s[c] root_s[c,p,i]; where scale factors
+// apply, each is multiplied by exp(scale_s[p] - max_s scale_s[p]) and the
+// maximum is added back after the log.  A subset's scale is its cumulative
+// factors (logs) or its root's automatic-scaling exponents.
+template <typename R>
+__global__ __launch_bounds__(WG) void k_root_multi(const RootSubset* __restrict__ subs, int count,
+                                                   const double* __restrict__ pw,
+                                                   double* __restrict__ site,
+                                                   double* __restrict__ bsum, Dims d) {
+    __shared__ double sh[WG / 64];
+    const int p = blockIdx.x * WG + threadIdx.x;
+    double contrib = 0.0;
+    if (p < d.P) {
+        const double ln2 = 0.69314718055994530942;
+        bool scaled = false;
+        double mx = 0.0;
+        for (int s = 0; s < count; ++s) {
+            const RootSubset& U = subs[s];
+            if (!U.cum && !U.ex) continue;
+            const double sc = (U.cum ? U.cum[p] : 0.0) + (U.ex ? ln2 * (double)U.ex[p] : 0.0);
+            mx = scaled ? fmax(mx, sc) : sc;
+            scaled = true;
+        }
+        double total = 0.0;
+        for (int s = 0; s < count; ++s) {
+            const RootSubset& U = subs[s];
+            const R* root = reinterpret_cast<const R*>(U.root);
+            double sum = 0.0;
+            for (int i = 0; i < d.S; ++i) {
+                double t = 0.0;
+                for (int c = 0; c < d.C; ++c)
+                    t += (double)root[pbase(d, c, p) + (long)i * d.sStride] * U.cw[c];
+                sum += U.freqs[i] * t;
+            }
+            if (scaled) {
+                const double sc = (U.cum ? U.cum[p] : 0.0) + (U.ex ? ln2 * (double)U.ex[p] : 0.0);
+                sum *= exp(sc - mx);
+            }
+            total += sum;
+        }
+        const double v = log(total) + mx;
+        site[p] = v;
+        contrib = v * pw[p];
+    }
+    const double b = block_sum(contrib, sh);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = b;
+}
+
+template <typename R>
+void launch_root_multi(const RootSubset* subs, int count, const double* pw, double* site,
+                       double* bsum, const Dims& d, hipStream_t st) {
+    const int nb = (d.P + WG - 1) / WG;
+    hipLaunchKernelGGL(k_root_multi<R>, dim3(nb), dim3(WG), 0, st, subs, count, pw, site, bsum, d);
+}
+
+// Per-category root log-likelihoods, no category weights
+// (calcRootLogLikelihoodsPerCategory, BeagleCPUImpl.hpp:2589-2625):
+// out[c P + p] = log(sum_i root[c,p,i] pi_i) + the pattern's scale.
+template <typename R>
+__global__ __launch_bounds__(WG) void k_root_per_category(const R* __restrict__ root,
+                                                          const double* __restrict__ freqs,
+                                                          const double* __restrict__ cum,
+                                                          const short* __restrict__ ex,
+                                                          double* __restrict__ out, Dims d) {
+    const int p = blockIdx.x * WG + threadIdx.x, c = blockIdx.y;
+    if (p >= d.P) return;
+    double sum = 0.0;
+    for (int i = 0; i < d.S; ++i) sum += (double)root[pbase(d, c, p) + (long)i * d.sStride] * freqs[i];
+    double v = log(sum);
+    if (cum) v += cum[p];
+    if (ex) v += 0.69314718055994530942 * (double)ex[p];
+    out[(long)c * d.P + p] = v;
+}
+
+template <typename R>
+void launch_root_per_category(const void* root, const double* freqs, const double* cum,
+                              const short* ex, double* out, const Dims& d, hipStream_t st) {
+    hipLaunchKernelGGL(k_root_per_category<R>, dim3((d.P + WG - 1) / WG, d.C), dim3(WG), 0, st,
+                       reinterpret_cast<const R*>(root), freqs, cum, ex, out, d);
+}
+
 // S > 4 (pattern-minor tiles): a workgroup takes 32 patterns, its eight
 // wavefront halves take every eighth state, so each load instruction reads
 // whole 128-byte rows; the eight partial sums are added in a fixed order.
@@ -3097,6 +3180,10 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
                                   hipStream_t);                                                 \
     template void launch_root_freq_grad<R>(const void*, const double*, const double*,          \
                                            const double*, double*, const Dims&, hipStream_t);   \
+    template void launch_root_multi<R>(const RootSubset*, int, const double*, double*, double*,  \
+                                       const Dims&, hipStream_t);                               \
+    template void launch_root_per_category<R>(const void*, const double*, const double*,        \
+                                              const short*, double*, const Dims&, hipStream_t);  \
     template void launch_qgrad_generic<R>(const PreJob*, int, const Dims&, const double*,       \
                                           const double*, const double*, const double*, double*,  \
                                           double*, int, double*, hipStream_t);                  \

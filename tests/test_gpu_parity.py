@@ -988,3 +988,83 @@ def test_complex_eigen_likelihood_matches_oracle(amd, oracle, S, scaling):
     with pytest.raises(api.BeagleError) as e:
         amd.create(2, 3, 2, 64, 8, 1, 4, 1, 1, requirement=api.FLAG_EIGEN_COMPLEX)
     assert e.value.code == api.ERROR_NO_IMPLEMENTATION
+
+
+def _two_model_roots(lib, prob, model2, scaling, precision):
+    """One tree under two models in two buffer sets, then the
+    several-root likelihood (count 2, weights/frequencies 0 and 1, each
+    subset's cumulative scale buffer), its site values and the first
+    subset's per-category output."""
+    S, n, C, P = prob.S, prob.tree.n, prob.C, prob.tips.shape[1]
+    nodes = 2 * n - 1
+    manual = scaling == "manual"
+    pref = api.FLAG_PRECISION_DOUBLE if precision == "double" else api.FLAG_PRECISION_SINGLE
+    pref |= {"none": 0, "manual": api.FLAG_SCALING_MANUAL, "auto": api.FLAG_SCALING_AUTO}[scaling]
+    inst = lib.create(n, 2 * nodes, n, S, P, 2, 2 * nodes, C, 2 * n, preference=pref)
+    try:
+        for t in range(n):
+            inst.set_tip_states(t, prob.tips[t])
+        for e, m in enumerate((prob.model, model2)):
+            inst.set_eigen(e, m.evec.reshape(-1), m.ivec.reshape(-1), m.evals)
+            inst.set_state_frequencies(e, m.freqs)
+        inst.set_category_weights(0, prob.weights)
+        inst.set_category_weights(1, prob.weights[::-1].copy())
+        inst.set_category_rates(prob.rates)
+        inst.set_pattern_weights(prob.pattern_weights)
+        idx = np.arange(nodes - 1)
+        lengths = prob.tree.lengths[:nodes - 1]
+        inst.update_transition_matrices(0, idx, None, None, lengths)
+        inst.update_transition_matrices(1, idx + nodes, None, None, lengths)
+        ops1 = synth.postorder_ops(prob.tree, (lambda v: v - n) if manual else None)
+        ops2 = ops1.copy()
+        ops2[:, 0] += nodes
+        for col in (3, 5):
+            ops2[:, col] = np.where(ops2[:, col] >= n, ops2[:, col] + nodes, ops2[:, col])
+        ops2[:, 4] += nodes
+        ops2[:, 6] += nodes
+        if manual:
+            ops2[:, 1] += n - 1
+        cum = [2 * n - 2, 2 * n - 1] if manual else [api.OP_NONE, api.OP_NONE]
+        inst.update_partials(ops1, cum[0] if manual else api.OP_NONE)
+        inst.update_partials(ops2, cum[1] if manual else api.OP_NONE)
+        if manual:
+            for k, ops in enumerate((ops1, ops2)):
+                inst.reset_scale_factors(cum[k])
+                inst.accumulate_scale_factors(ops[:, 1], cum[k])
+        root = prob.tree.root
+        lnl = inst.root_log_likelihoods([root, root + nodes], [0, 1], [0, 1], cum)
+        sites = inst.site_log_likelihoods()
+        per_cat = inst.root_log_likelihood_per_category(root, 0, cum[0])
+        single = inst.root_log_likelihood(root, 0, 0, cum[0])
+        single_sites = inst.site_log_likelihoods()
+        return lnl, sites, per_cat, single, single_sites
+    finally:
+        inst.finalize()
+
+
+@pytest.mark.parametrize("S,scaling,precision", [(4, "none", "double"), (4, "manual", "double"),
+                                                 (20, "manual", "double"),
+                                                 (4, "auto", "single")])
+def test_several_roots_and_per_category_root(amd, ref, S, scaling, precision):
+    """calculateRootLogLikelihoods with count > 1 (one likelihood from
+    several roots, per-pattern maximum of the subsets' scale factors,
+    BeagleCPUImpl.hpp:2474-2586) and with category weights index -1
+    (per-category site log-likelihoods, :2589-2625) against the reference
+    CPU library on the same calls; automatic scaling against the reference
+    with manual scaling.  Closed form: with category weights w, the
+    per-category values give the single-root site values,
+    log sum_c w_c exp(per_cat[c]) (no scale factors)."""
+    rng = np.random.default_rng(90 + S)
+    prob = synth.make_problem(91 + S, 14 if S == 4 else 9, 700 if S == 4 else 150, S=S,
+                              shape="yule", missing=0.05)
+    model2 = synth.gtr(rng, S)
+    g = _two_model_roots(amd, prob, model2, scaling, precision)
+    r = _two_model_roots(ref, prob, model2, "manual" if scaling == "auto" else scaling, "double")
+    tol = 1e-10 if precision == "double" else 1e-6
+    assert rel(g[0], r[0]) < tol, (g[0], r[0])
+    close(g[1], r[1], tol, tol)
+    close(g[2], r[2], 1e-10 if precision == "double" else 1e-5, tol)
+    assert rel(g[3], r[3]) < tol
+    if scaling == "none":
+        mix = np.log(np.exp(g[2]).T @ prob.weights)
+        close(mix, g[4], 1e-12, 1e-12)
