@@ -447,16 +447,49 @@ __device__ __forceinline__ void apply4(const R* m, bool isState, const Operand4<
 // sets so no loaded register is ever copied.  Operand kinds are decided on
 // the host: partials (0), tip states (1), or the previous operation's
 // output forwarded from registers (2).
+// Buffer resource over `base` when `on`, else an empty range over `valid`:
+// an access to an empty range reads 0 and touches no memory, so a load a
+// step does not need costs one instruction, no branch and no traffic.  The
+// base and range are wave-uniform (job fields); readfirstlane states it,
+// so the resource stays in scalar registers.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t range_if(const void* base, bool on,
+                                                           const void* valid) {
+    const unsigned long long a = reinterpret_cast<unsigned long long>(on ? base : valid);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    const int n = __builtin_amdgcn_readfirstlane(on ? -1 : 0);
+    void* b = reinterpret_cast<void*>(((unsigned long long)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, n, 0x00020000);
+}
+
+// four reals at byte offset `at` of a buffer resource
 template <typename R>
-__device__ __forceinline__ void issue4(const void* x, const int* s, int kind, const void* dx,
-                                       const int* ds, long off, long doff, int p, int lane,
-                                       Operand4<R>& o) {
-    const void* xa = kind == 0 ? x : dx;
-    const long xo = kind == 0 ? off : doff;
-    const int* sa = kind == 1 ? s : ds;
-    const int so = kind == 1 ? p : lane;
-    load4(gin<R>(xa) + xo, o.x);
-    o.st = gin<int>(sa)[so];
+__device__ __forceinline__ void load4_range(__amdgpu_buffer_rsrc_t r, unsigned at, R (&v)[4]) {
+    if (sizeof(R) == 4) {
+        const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, at, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = (R)__uint_as_float(q[i]);
+    } else {
+        const auto q0 = __builtin_amdgcn_raw_buffer_load_b128(r, at, 0, 0);
+        const auto q1 = __builtin_amdgcn_raw_buffer_load_b128(r, at + 16, 0, 0);
+        v[0] = (R)__hiloint2double((int)q0[1], (int)q0[0]);
+        v[1] = (R)__hiloint2double((int)q0[3], (int)q0[2]);
+        v[2] = (R)__hiloint2double((int)q1[1], (int)q1[0]);
+        v[3] = (R)__hiloint2double((int)q1[3], (int)q1[2]);
+    }
+}
+
+// One operand's loads, issued whatever its kind so that every step of a
+// pipelined kernel issues the same loads: partials (kind 0) at element
+// offset `off`, tip states (kind 1) at 4p bytes; the one the kind does not
+// read comes from an empty range (0).  Partials buffers are under 4 GiB
+// (byte offsets in 32 bits).
+template <typename R>
+__device__ __forceinline__ void issue4(const void* x, const int* s, int kind, const void* valid,
+                                       long off, int p, Operand4<R>& o) {
+    load4_range<R>(range_if(x, kind == 0, valid), (unsigned)(off * (long)sizeof(R)), o.x);
+    o.st = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(s, kind == 1, valid),
+                                                     4u * (unsigned)p, 0, 0);
 }
 
 // Operand value by kind, formed with selects (no branches, so no register
@@ -505,15 +538,17 @@ __device__ __forceinline__ void post_issue(const PostJob& J, const void* dx, con
                                            const double* dD, const short* dE, long off,
                                            long doff, int p, int lane, Operand4<R>& A,
                                            Operand4<R>& B) {
-    issue4(J.x1, J.s1, J.k1, dx, ds, off, doff, p, lane, A);
-    issue4(J.x2, J.s2, J.k2, dx, ds, off, doff, p, lane, B);
+    issue4(J.x1, J.s1, J.k1, dx, off, p, A);
+    issue4(J.x2, J.s2, J.k2, dx, off, p, B);
     if (AUTO) {
         const bool l1 = J.k1 == 0 && J.e1 != nullptr, l2 = J.k2 == 0 && J.e2 != nullptr;
-        A.e = gin<short>(l1 ? J.e1 : dE)[l1 ? p : lane];
-        B.e = gin<short>(l2 ? J.e2 : dE)[l2 ? p : lane];
+        A.e = (short)__builtin_amdgcn_raw_buffer_load_b16(range_if(J.e1, l1, dx), 2u * p, 0, 0);
+        B.e = (short)__builtin_amdgcn_raw_buffer_load_b16(range_if(J.e2, l2, dx), 2u * p, 0, 0);
     } else {
         const bool r = J.mode == SCALE_READ;
-        A.sr = gin<double>(r ? J.scaleRead : dD)[r ? p : lane];
+        const auto q = __builtin_amdgcn_raw_buffer_load_b64(range_if(J.scaleRead, r, dx), 8u * p,
+                                                            0, 0);
+        A.sr = __hiloint2double((int)q[1], (int)q[0]);
     }
 }
 
@@ -739,9 +774,9 @@ struct PreOps4 {
 template <typename R, bool DERIV>
 __device__ __forceinline__ void pre_issue(const PreJob& J, const void* dx, const int* ds, long off,
                                           long doff, int p, int lane, PreOps4<R>& o) {
-    issue4(J.parent, nullptr, J.kPar == 0 ? 0 : 2, dx, ds, off, doff, p, lane, o.U);
-    issue4(J.xs, J.ss, J.kSib, dx, ds, off, doff, p, lane, o.B);
-    if (DERIV) issue4(J.xself, J.sself, J.kSelf, dx, ds, off, doff, p, lane, o.Y);
+    issue4(J.parent, nullptr, J.kPar == 0 ? 0 : 2, dx, off, p, o.U);
+    issue4(J.xs, J.ss, J.kSib, dx, off, p, o.B);
+    if (DERIV) issue4(J.xself, J.sself, J.kSelf, dx, off, p, o.Y);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
