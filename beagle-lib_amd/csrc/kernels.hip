@@ -1032,6 +1032,8 @@ __device__ __forceinline__ void node_issue(const NodeJob& J, const void* dx, con
                                            const void* dE, const R* __restrict__ jm, int c,
                                            unsigned off, unsigned doff, int p, int lane,
                                            NodeOps4<R>& o) {
+    // (scratch-tile reads here, not empty buffer ranges: those take more
+    // registers and this kernel is at its budget)
     const bool lp = J.kPar == 0, pa = J.k[0] == 0, pb = J.k[1] == 0;
     load4(at<R>(lp ? J.parent : dx, lp ? off : doff), o.u);
     load4(at<R>(pa ? J.x[0] : dx, pa ? off : doff), o.xa);
@@ -1642,10 +1644,10 @@ __device__ __forceinline__ void q_issue(const NodeJob& J, int jidx, const void* 
                                         QOps4<R>& o) {
     const bool pa = J.k[0] == 0, pb = J.k[1] == 0;
     load4(at<R>(J.parent, off), o.u);
-    load4(at<R>(pa ? J.x[0] : dx, pa ? off : doff), o.xa);
-    load4(at<R>(pb ? J.x[1] : dx, pb ? off : doff), o.xb);
-    o.sa = *at<int>(pa ? ds : J.x[0], pa ? 4u * lane : 4u * p);
-    o.sb = *at<int>(pb ? ds : J.x[1], pb ? 4u * lane : 4u * p);
+    load4_range<R>(range_if(J.x[0], pa, jm), off, o.xa);
+    load4_range<R>(range_if(J.x[1], pb, jm), off, o.xb);
+    o.sa = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[0], !pa, jm), 4u * p, 0, 0);
+    o.sb = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[1], !pb, jm), 4u * p, 0, 0);
     constexpr int V = Vec16<R>::N;
     const int l = min(lane, NODE_MATS / V - 1);
     o.mv = *at<typename Vec16<R>::T>(jm, (unsigned)((J.mbase + c * NODE_MATS + l * V) *
