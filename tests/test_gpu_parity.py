@@ -1068,3 +1068,46 @@ def test_several_roots_and_per_category_root(amd, ref, S, scaling, precision):
     if scaling == "none":
         mix = np.log(np.exp(g[2]).T @ prob.weights)
         close(mix, g[4], 1e-12, 1e-12)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_job_list_copies_evicted_and_reused(amd, oracle, precision):
+    """More distinct operation lists than an instance keeps on the device
+    (eight): twelve Yule topologies over the same tips, evaluated in turn,
+    three passes, on one instance (each list is reused from its device copy
+    or replaces the least recently used one); every logL equals the
+    oracle's for that topology, and repeats of a topology are
+    bit-identical."""
+    rng = np.random.default_rng(123)
+    base = synth.make_problem(124, 10, 500, S=4, missing=0.05)
+    n, S, C, P = 10, 4, base.C, base.tips.shape[1]
+    trees = [synth.make_tree(rng, n, "yule", 0.1) for _ in range(12)]
+
+    def run(lib, pref, passes):
+        inst = lib.create(n, 2 * n, n, S, P, 1, 2 * n, C, 0, preference=pref)
+        try:
+            for t in range(n):
+                inst.set_tip_states(t, base.tips[t])
+            m = base.model
+            inst.set_eigen(0, m.evec.reshape(-1), m.ivec.reshape(-1), m.evals)
+            inst.set_state_frequencies(0, m.freqs)
+            inst.set_category_weights(0, base.weights)
+            inst.set_category_rates(base.rates)
+            inst.set_pattern_weights(base.pattern_weights)
+            out = []
+            for _ in range(passes):
+                for tr in trees:
+                    inst.update_transition_matrices(0, np.arange(2 * n - 2), None, None,
+                                                    tr.lengths[:2 * n - 2])
+                    inst.update_partials(synth.postorder_ops(tr))
+                    out.append(inst.root_log_likelihood(tr.root))
+            return np.array(out)
+        finally:
+            inst.finalize()
+
+    pref = api.FLAG_PRECISION_DOUBLE if precision == "double" else api.FLAG_PRECISION_SINGLE
+    g = run(amd, pref, 3).reshape(3, -1)
+    o = run(oracle, 0, 1)
+    np.testing.assert_array_equal(g[0], g[1])
+    np.testing.assert_array_equal(g[0], g[2])
+    close(g[0], o, 1e-10 if precision == "double" else 1e-6)
