@@ -1844,10 +1844,23 @@ constexpr int QGX_LD = QGX_T + 1;
 static_assert(WG == 256, "the Q-gradient register blocks map 256 threads to 16 x 16");
 constexpr int QGX_NE = 64 * 64 / WG;   // S x S entries per thread, S <= 64 (k_qgx_eigen)
 
-// dynamic LDS of the two tile kernels: three tiles, then (k_qgx_coef) the
-// [16][QGX_T] partial sums, L and the two state rows
-static size_t qgx_lds_bytes(int S) {
-    return ((size_t)3 * S * QGX_LD + 16 * QGX_T + 2 * QGX_T) * sizeof(double);
+// Tiles are staged in the partials' own precision T = R (the products are
+// formed in FP64 from them).  With FP32 partials k_qgx_outer, which walks
+// many tiles of one category, also stages that category's sibling matrix
+// in LDS (qgx_mstage); k_qgx_coef takes one tile per workgroup, where
+// staging two matrices per category costs more than reading their rows.
+// LDS, [doubles][T tiles][matrix][ints]: k_qgx_coef: [16][QGX_T] partial
+// sums and L; k_qgx_outer: the tile's coef, the staged matrix.  At most
+// 55,296 bytes (S = 64).
+template <typename R>
+constexpr bool qgx_mstage() { return sizeof(R) == 4; }
+
+template <typename R>
+static size_t qgx_lds_bytes(int S, bool coef) {
+    const size_t d = coef ? 16 * QGX_T + QGX_T : QGX_T;
+    const size_t mats = (!coef && qgx_mstage<R>()) ? (size_t)S * (S + 1) * sizeof(R) : 0;
+    return d * sizeof(double) + (size_t)3 * S * QGX_LD * sizeof(R) + mats +
+           2 * QGX_T * sizeof(int);
 }
 
 // Stage category c of the tile at p0: A = pre(parent), B = the sibling's
@@ -1855,23 +1868,30 @@ static size_t qgx_lds_bytes(int S) {
 // a tip state < S; zero for a missing state, whose pad column does not
 // depend on Q).  Tip states into sts (sibling) and sto (own).
 template <typename R>
-__device__ void qgx_stage(const PreJob& J, const Dims& d, int c, int p0, double* A, double* B,
-                          double* X, int* sts, int* sto) {
+__device__ void qgx_stage(const PreJob& J, const Dims& d, int c, int p0, R* A, R* B, R* X,
+                          int* sts, int* sto) {
     const R* par = reinterpret_cast<const R*>(J.parent);
     const R* xs = reinterpret_cast<const R*>(J.xs);
     const R* xo = reinterpret_cast<const R*>(J.xself);
     for (int t = threadIdx.x; t < d.S * QGX_T; t += WG) {
         const int i = t / QGX_T, q = t % QGX_T, p = p0 + q;
         const long e = pbase(d, c, p) + (long)i * d.sStride;
-        A[i * QGX_LD + q] = (double)par[e];
-        if (xs) B[i * QGX_LD + q] = (double)xs[e];
-        X[i * QGX_LD + q] = xo ? (double)xo[e] : (J.sself[p] == i ? 1.0 : 0.0);
+        A[i * QGX_LD + q] = par[e];
+        if (xs) B[i * QGX_LD + q] = xs[e];
+        X[i * QGX_LD + q] = xo ? xo[e] : (J.sself[p] == i ? (R)1 : (R)0);
     }
     if (threadIdx.x < QGX_T) {
         const int p = p0 + threadIdx.x;
         sts[threadIdx.x] = J.ss ? J.ss[p] : 0;
         sto[threadIdx.x] = J.sself ? J.sself[p] : 0;
     }
+}
+
+// Category c's matrix rows ([S][S + 1] reals) into LDS when staged.
+template <typename R>
+__device__ void qgx_stage_matrix(const R* M, int S, int c, R* dst) {
+    const R* src = M + (long)c * S * (S + 1);
+    for (int t = threadIdx.x; t < S * (S + 1); t += WG) dst[t] = src[t];
 }
 
 // Register blocks: the 256 threads of a workgroup as a 16 x 16 grid (bi,
@@ -1881,10 +1901,11 @@ __device__ void qgx_stage(const PreJob& J, const Dims& d, int c, int p0, double*
 // inside the staged tiles.
 
 // (M v)_i at pattern q for the thread's block: M the category's rows
-// ([S][S + 1] reals, pad column S), v a staged tile, or for a tip the
-// rows' entries at the pattern's state (the pad for a missing one).
-template <typename R>
-__device__ __forceinline__ void qgx_mv_block(const R* M, int S, const double* v, bool tip,
+// ([S][S + 1] reals, pad column S; in LDS or memory), v a staged tile, or
+// for a tip the rows' entries at the pattern's state (the pad for a
+// missing one).
+template <typename R, typename T>
+__device__ __forceinline__ void qgx_mv_block(const R* M, int S, const T* v, bool tip,
                                              const int* st, int i0, int q0,
                                              double (&out)[4][2]) {
     const R* r[4];
@@ -1900,7 +1921,7 @@ __device__ __forceinline__ void qgx_mv_block(const R* M, int S, const double* v,
 #pragma unroll
     for (int u = 0; u < 4; ++u) out[u][0] = out[u][1] = 0.0;
     for (int k = 0; k < S; ++k) {
-        const double v0 = v[k * QGX_LD + q0], v1 = v[k * QGX_LD + q0 + 1];
+        const double v0 = (double)v[k * QGX_LD + q0], v1 = (double)v[k * QGX_LD + q0 + 1];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const double m = (double)r[u][k];
@@ -1918,12 +1939,12 @@ __global__ __launch_bounds__(WG) void k_qgx_coef(const PreJob* __restrict__ jobs
                                                  double* __restrict__ coef) {
     extern __shared__ double qs[];
     const int S = d.S;
-    double* A = qs;
-    double* B = A + S * QGX_LD;
-    double* X = B + S * QGX_LD;
-    double* red = X + S * QGX_LD;                     // [16][QGX_T]
+    double* red = qs;                                 // [16][QGX_T]
     double* L = red + 16 * QGX_T;                     // [QGX_T]
-    int* sts = reinterpret_cast<int*>(L + QGX_T);
+    R* A = reinterpret_cast<R*>(L + QGX_T);
+    R* B = A + S * QGX_LD;
+    R* X = B + S * QGX_LD;
+    int* sts = reinterpret_cast<int*>(X + S * QGX_LD);
     int* sto = sts + QGX_T;
     const int o = blockIdx.z;
     const PreJob J = jobs[o];
@@ -1944,7 +1965,8 @@ __global__ __launch_bounds__(WG) void k_qgx_coef(const PreJob* __restrict__ jobs
         for (int u = 0; u < 4; ++u)
             if (i0 + u < S)
 #pragma unroll
-                for (int w = 0; w < 2; ++w) part[w] += A[(i0 + u) * QGX_LD + q0 + w] * tb[u][w] * yb[u][w];
+                for (int w = 0; w < 2; ++w)
+                    part[w] += (double)A[(i0 + u) * QGX_LD + q0 + w] * tb[u][w] * yb[u][w];
         red[bi * QGX_T + q0] = part[0];
         red[bi * QGX_T + q0 + 1] = part[1];
         __syncthreads();
@@ -1962,23 +1984,30 @@ __global__ __launch_bounds__(WG) void k_qgx_coef(const PreJob* __restrict__ jobs
     }
 }
 
-// grid (chunks, C, operations): this chunk's X_c, out[o][c][chunk][S * S]
+// grid (chunks, C, operations): this chunk's X_c, out[o][c][chunk][S * S].
+// The coef-scaled top vectors are kept in FP64 registers and written over
+// the parent's tile in T (FP32 partials: the top values at FP32 precision,
+// as the likelihood path holds them).
 template <typename R>
 __global__ __launch_bounds__(WG) void k_qgx_outer(const PreJob* __restrict__ jobs, Dims d,
                                                   const double* __restrict__ cw,
                                                   const double* __restrict__ coef,
                                                   int tilesPerChunk, double* __restrict__ out) {
     extern __shared__ double qs[];
+    constexpr bool MS = qgx_mstage<R>();
     const int S = d.S;
-    double* A = qs;
-    double* B = A + S * QGX_LD;
-    double* X = B + S * QGX_LD;
-    double* cf = X + S * QGX_LD;
-    int* sts = reinterpret_cast<int*>(cf + QGX_T);
+    double* cf = qs;
+    R* A = reinterpret_cast<R*>(cf + QGX_T);
+    R* B = A + S * QGX_LD;
+    R* X = B + S * QGX_LD;
+    R* lMs = X + S * QGX_LD;
+    int* sts = reinterpret_cast<int*>(lMs + (MS ? S * (S + 1) : 0));
     int* sto = sts + QGX_T;
     const int o = blockIdx.z, c = blockIdx.y;
     const PreJob J = jobs[o];
-    const R* Ms = reinterpret_cast<const R*>(J.mSib) + (long)c * S * (S + 1);
+    const R* gMs = reinterpret_cast<const R*>(J.mSib);
+    if (MS) qgx_stage_matrix(gMs, S, c, lMs);   // visible after the first tile's barrier
+    const R* Ms = MS ? lMs : gMs + (long)c * S * (S + 1);
     const double wc = cw[c];
     const int bi = threadIdx.x / 16, bj = threadIdx.x % 16;
     const int i0 = 4 * bi, q0 = 2 * bj, j0 = 4 * bj;
@@ -2002,20 +2031,18 @@ __global__ __launch_bounds__(WG) void k_qgx_outer(const PreJob* __restrict__ job
         qgx_stage<R>(J, d, c, p0, A, B, X, sts, sto);
         if (threadIdx.x < QGX_T) cf[threadIdx.x] = wc * coef[(long)o * d.Ppad + p0 + threadIdx.x];
         __syncthreads();
-        // top scaled by coef, written over the parent's tile once every
-        // thread has read it
         double tb[4][2];
         qgx_mv_block(Ms, S, B, J.ss != nullptr, sts, i0, q0, tb);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int w = 0; w < 2; ++w) tb[u][w] *= cf[q0 + w] * A[ri[u] + q0 + w];
+            for (int w = 0; w < 2; ++w) tb[u][w] *= cf[q0 + w] * (double)A[ri[u] + q0 + w];
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (i0 + u < S) {
-                A[(i0 + u) * QGX_LD + q0] = tb[u][0];
-                A[(i0 + u) * QGX_LD + q0 + 1] = tb[u][1];
+                A[(i0 + u) * QGX_LD + q0] = (R)tb[u][0];
+                A[(i0 + u) * QGX_LD + q0 + 1] = (R)tb[u][1];
             }
         __syncthreads();
         if (live)
@@ -2023,8 +2050,8 @@ __global__ __launch_bounds__(WG) void k_qgx_outer(const PreJob* __restrict__ job
                 double a[4], x[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    a[u] = A[ri[u] + k];
-                    x[u] = X[rj[u] + k];
+                    a[u] = (double)A[ri[u] + k];
+                    x[u] = (double)X[rj[u] + k];
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
@@ -2101,10 +2128,10 @@ void launch_qgrad_generic(const PreJob* djobs, int nops, const Dims& d, const do
                           double* xs, int chunks, double* z, hipStream_t st) {
     if (nops <= 0) return;
     const int ntiles = d.Ppad / QGX_T;
-    hipLaunchKernelGGL(k_qgx_coef<R>, dim3(ntiles, 1, nops), dim3(WG), qgx_lds_bytes(d.S), st,
+    hipLaunchKernelGGL(k_qgx_coef<R>, dim3(ntiles, 1, nops), dim3(WG), qgx_lds_bytes<R>(d.S, true), st,
                        djobs, d, cw, pw, coef);
     const int per = (ntiles + chunks - 1) / chunks;
-    hipLaunchKernelGGL(k_qgx_outer<R>, dim3(chunks, d.C, nops), dim3(WG), qgx_lds_bytes(d.S), st,
+    hipLaunchKernelGGL(k_qgx_outer<R>, dim3(chunks, d.C, nops), dim3(WG), qgx_lds_bytes<R>(d.S, false), st,
                        djobs, d, cw, coef, per, xs);
     hipLaunchKernelGGL(k_qgx_eigen, dim3(d.C, nops), dim3(WG), (size_t)d.S * d.S * sizeof(double),
                        st, xs, chunks, nops, d.S, d.C, eig, tau, z);
