@@ -1111,3 +1111,26 @@ def test_job_list_copies_evicted_and_reused(amd, oracle, precision):
     np.testing.assert_array_equal(g[0], g[1])
     np.testing.assert_array_equal(g[0], g[2])
     close(g[0], o, 1e-10 if precision == "double" else 1e-6)
+
+
+def test_complex_eigen_transposed_inverse(amd):
+    """BEAGLE_FLAG_INVEVEC_TRANSPOSED with a complex eigen-system: V^-1
+    passed transposed gives the same P(r t) = expm(Q r t) (clamped)."""
+    from scipy.linalg import expm
+    S = 20
+    m = synth.nonreversible(np.random.default_rng(77), S)
+    rates = np.array([0.5, 1.5])
+    lengths = np.array([0.05, 0.4])
+    inst = amd.create(2, 3, 2, S, 8, 1, 4, 2, 1, preference=api.FLAG_PRECISION_DOUBLE,
+                      requirement=api.FLAG_EIGEN_COMPLEX | api.FLAG_INVEVEC_TRANSPOSED)
+    try:
+        inst.set_eigen(0, m.evec.reshape(-1), m.ivec.T.copy().reshape(-1), m.evals)
+        inst.set_category_rates(rates)
+        inst.update_transition_matrices(0, [0, 1], None, None, lengths)
+        for u, t in enumerate(lengths):
+            got = inst.get_transition_matrix(u).reshape(2, S, S)
+            for c, r in enumerate(rates):
+                np.testing.assert_allclose(got[c], np.maximum(expm(m.Q * (r * t)), 0.0),
+                                           rtol=0, atol=1e-12)
+    finally:
+        inst.finalize()

@@ -452,3 +452,33 @@ def test_complex_eigen_reference_matches_expm(ref, S):
                                            rtol=0, atol=1e-12)
     finally:
         inst.finalize()
+
+
+@pytest.mark.parametrize("S", [4, 7, 20, 61])
+def test_nonreversible_generator(S):
+    """synth.nonreversible: Q is a rate matrix (rows sum to 0, off-diagonal
+    >= 0) normalised to one substitution at its stationary pi (pi Q = 0),
+    has complex eigenvalues, and V B V^-1 = Q with B block-diagonal:
+    a on the diagonal, [[a, b], [-b, a]] for each pair (b on the first
+    row); expm through the blocks equals scipy's expm."""
+    from scipy.linalg import expm
+    m = synth.nonreversible(np.random.default_rng(S), S)
+    Q = m.Q
+    assert np.allclose(Q.sum(axis=1), 0.0, atol=1e-13)
+    assert (Q - np.diag(np.diag(Q)) >= 0).all()
+    assert abs(m.freqs.sum() - 1.0) < 1e-13 and np.abs(m.freqs @ Q).max() < 1e-12
+    assert abs(-(m.freqs * np.diag(Q)).sum() - 1.0) < 1e-12
+    re, im = m.evals[:S], m.evals[S:]
+    assert np.any(im != 0)
+    B = np.diag(re)
+    k = 0
+    while k < S:
+        if im[k] != 0:
+            assert im[k + 1] == -im[k] and re[k + 1] == re[k]
+            B[k, k + 1], B[k + 1, k] = im[k], -im[k]
+            k += 2
+        else:
+            k += 1
+    np.testing.assert_allclose(m.evec @ B @ m.ivec, Q, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(m.evec @ expm(0.7 * B) @ m.ivec, expm(0.7 * Q), rtol=0,
+                               atol=1e-12)
