@@ -384,6 +384,10 @@ class Instance:
         a = _ip(idx)
         return self._c("RemoveScaleFactors", a[1], len(a[0]), cumulative)
 
+    def remove_scale_factors_by_partition(self, idx, cumulative, part):
+        a = _ip(idx)
+        return self._c("RemoveScaleFactorsByPartition", a[1], len(a[0]), cumulative, part)
+
     def reset_scale_factors(self, cumulative):
         return self._c("ResetScaleFactors", cumulative)
 

@@ -13,6 +13,8 @@ v (n..2n-2), pre-order partials of node v at 2n-1+v.  Matrix of the
 branch above v is v; the two derivative matrices follow at 2n-1 and 2n
 (their transposes, for the S>4 manual-transpose convention, at 2n+1, 2n+2).
 The scale buffer of internal node v is v-n; the cumulative buffer is n-1.
+With pre_scaling, pre-order operation k rescales its output into scale
+buffer n+k (the pattern of examples/hmctest/hmctest.cpp:503-506).
 """
 import numpy as np
 
@@ -31,13 +33,14 @@ def explicit_matrices(prob, lengths):
 class Evaluator:
     def __init__(self, lib, prob, *, tips_as="states", scaling="none", preorder=True,
                  precision="double", extra_flags=0, resources=None, transpose="auto",
-                 matrices="eigen"):
+                 matrices="eigen", pre_scaling=False):
         self.lib = lib
         self.prob = prob
         self.scaling = scaling
         self.preorder = preorder
         self.transpose = transpose
         self.matrices = matrices
+        self.pre_scaling = pre_scaling
         n = prob.tree.n
         self.n = n
         nodes = 2 * n - 1
@@ -73,8 +76,10 @@ class Evaluator:
         if transpose == "auto":
             pref |= api.FLAG_PREORDER_TRANSPOSE_AUTO
         pref |= extra_flags
+        scale_buffers = n + (2 * n - 2 if pre_scaling else 0)
         self.inst = lib.create(n, partials, compact, prob.S, prob.P, 1, self.n_matrices, prob.C,
-                               n, resources=resources, preference=pref, requirement=req)
+                               scale_buffers, resources=resources, preference=pref,
+                               requirement=req)
         self._setup()
         # operation lists depend on the topology only: built once, as a
         # client builds them once per tree
@@ -163,10 +168,16 @@ class Evaluator:
         else:
             self.inst.set_partials(root, np.tile(self.prob.model.freqs, self.prob.C * self.prob.P))
 
+    def preorder_operations(self):
+        ops = self._transposed_self(synth.preorder_ops(self.prob.tree, self.pre_base))
+        if self.pre_scaling:
+            ops = np.array(ops, copy=True)
+            ops[:, 1] = self.n + np.arange(len(ops))
+        return ops
+
     def pre_order(self):
-        tree = self.prob.tree
         self.set_root_pre()
-        ops = self._transposed_self(synth.preorder_ops(tree, self.pre_base))
+        ops = self.preorder_operations()
         self.inst.update_pre_partials(ops, api.OP_NONE)
         return ops
 

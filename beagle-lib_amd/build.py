@@ -6,6 +6,7 @@ points (everything else has hidden visibility and binds locally).
 """
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
@@ -32,14 +33,17 @@ def run(cmd):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC] + COMMON + DEVICE + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        run(cmd)
+        cmds.append([HIPCC] + COMMON + DEVICE + ["-x", "hip", "-c", os.path.join(CSRC, src),
+                                                 "-o", obj])
         objs.append(obj)
+        if verbose:
+            print(" ".join(cmds[-1]), flush=True)
+    # one compiler process per source (the two kernel files dominate)
+    with ThreadPoolExecutor(max_workers=min(len(cmds), 8)) as pool:
+        list(pool.map(run, cmds))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + [
         "-Wl,-Bsymbolic", "-Wl,--no-undefined"]

@@ -23,6 +23,8 @@ import pytest
 
 from beagle_amd import api, driver, synth
 
+import sequences
+
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-10
@@ -1134,3 +1136,230 @@ def test_complex_eigen_transposed_inverse(amd):
                                            rtol=0, atol=1e-12)
     finally:
         inst.finalize()
+
+
+# ---- calls no fixture holds: scale-factor bookkeeping, pre-order write
+# scales, partitions, matrix derivatives (the same sequences the CPU tests
+# pin between the oracle and the reference CPU library) -------------------
+
+
+@pytest.mark.parametrize("S,precision", [(4, "double"), (4, "single"), (20, "double"),
+                                         (20, "single"), (61, "double")])
+def test_api_sequence_matches_oracle(amd, oracle, S, precision):
+    """sequences.api_sequence on the HIP library against the oracle:
+    getPartials with a cumulative buffer, remove / accumulate / copy of
+    scale factors and the root log-likelihood over them, pre-order
+    operations with write-scale buffers (rescaled partials, their factors,
+    the derivatives after them), reset / accumulate / remove of scale
+    factors by partition with root log-likelihoods by partition, and the
+    partitioned pre-order pass with write scales.  Identical explicit
+    matrices on both sides.  FP64: 1e-11 of each array's largest entry;
+    FP32: 1e-5 (log-likelihoods 1e-6 relative)."""
+    P = {4: 300, 20: 120, 61: 64}[S]
+    prob = synth.make_problem(151 + S, 9, P, S=S, missing=0.05, mean_length=0.3)
+    got = sequences.api_sequence(amd, prob, precision, transpose="auto", matrices="explicit")
+    want = sequences.api_sequence(oracle, prob, "double", transpose="auto", matrices="explicit")
+    assert set(got) == set(want)
+    for k in want:
+        a, b = np.asarray(got[k]), np.asarray(want[k])
+        assert a.shape == b.shape, k
+        if k.endswith("logL"):
+            assert rel(a, b) < (1e-11 if precision == "double" else 1e-6), k
+            continue
+        scale = max(np.abs(b).max(), 1e-300)
+        tol = 1e-11 if precision == "double" else 1e-5
+        err = np.abs(a - b).max() / scale
+        assert err < tol, f"{k}: {err:.2e}"
+
+
+@pytest.mark.parametrize("S,precision", [(4, "double"), (4, "single"), (20, "double"),
+                                         (20, "single"), (61, "double"), (61, "single")])
+def test_transition_derivatives_match_oracle(amd, oracle, S, precision):
+    """beagleUpdateTransitionMatrices with first- and second-derivative
+    index lists: P, dP/dt and d2P/dt2 on the device against the oracle
+    (which the CPU tests pin against the reference and the closed forms
+    dP/dt = rQP, d2P/dt2 = r^2 Q^2 P), t = 0 included.  Entries compared to
+    the largest magnitude of each matrix kind: FP64 1e-13, FP32 2e-6."""
+    prob = synth.make_problem(160 + S, 6, 32, S=S)
+    lengths = [0.0, 0.003, 0.05, 0.4, 1.7]
+    got = sequences.transition_derivatives(amd, prob, lengths, precision)
+    want = sequences.transition_derivatives(oracle, prob, lengths)
+    tol = 1e-13 if precision == "double" else 2e-6
+    for kind, a, b in zip(("P", "dP", "d2P"), got, want):
+        scale = np.abs(b).max(axis=(2, 3), keepdims=True)
+        err = (np.abs(a - b) / np.maximum(scale, 1e-300)).max()
+        assert err < tol, f"{kind}: {err:.2e}"
+
+
+def _oracle_gradient(oracle, prob, lengths, scaling, matrices="eigen"):
+    ev = driver.Evaluator(oracle, prob, scaling="manual" if scaling == "auto" else scaling,
+                          matrices=matrices)
+    res = ev.gradient(lengths)
+    ev.close()
+    return res
+
+
+def _assert_gradient_close(got, want, precision):
+    lg, g1, g2 = got
+    lo, o1, o2 = want
+    if precision == "double":
+        assert rel(lg, lo) < 1e-10
+        assert np.max(np.abs(g1 - o1)) <= 1e-9 * np.max(np.abs(o1))
+        assert np.max(np.abs(g2 - o2)) <= 1e-9 * np.max(np.abs(o2))
+    else:
+        assert rel(lg, lo) < 1e-6
+        assert np.max(np.abs(g1 - o1)) <= 1e-5 * np.max(np.abs(o1))
+        assert np.max(np.abs(g2 - o2)) <= 1e-5 * np.max(np.abs(o2))
+
+
+@pytest.mark.parametrize("precision,scaling,shape", [("single", "auto", "caterpillar"),
+                                                     ("double", "none", "yule"),
+                                                     ("double", "manual", "balanced")])
+def test_graph_replays_match_oracle(amd, oracle, precision, scaling, shape):
+    """Each replay of a captured logL + gradient evaluation, at branch
+    lengths set after the capture, against the oracle's beagle.h call
+    sequence at those lengths: logL 1e-10 (FP64) / 1e-6 (FP32) relative,
+    every branch's dlogL/dt and d2logL/dt2 within 1e-9 / 1e-5 of the
+    largest."""
+    prob = synth.make_problem(191, 24, 3000, S=4, shape=shape, missing=0.03)
+    rng = np.random.default_rng(12)
+    lens = [prob.tree.lengths * rng.uniform(0.5, 1.5, prob.tree.lengths.shape) for _ in range(3)]
+    ev = driver.Evaluator(amd, prob, precision=precision, scaling=scaling)
+    ev.capture_fused_gradient()
+    for x in lens:
+        got = ev.replay_fused_gradient(x)
+        _assert_gradient_close(got, _oracle_gradient(oracle, prob, x, scaling), precision)
+    ev.close()
+
+
+@pytest.mark.parametrize("S,precision,scaling,shape", [(4, "single", "auto", "caterpillar"),
+                                                       (4, "double", "none", "yule"),
+                                                       (20, "single", "auto", "yule"),
+                                                       (61, "double", "none", "balanced")])
+def test_checkpointed_gradient_matches_oracle(amd, oracle, S, precision, scaling, shape):
+    """beagleExtCheckpointedGradient at interval 2 and with the smallest pool
+    that fits, against the oracle's full beagle.h evaluation on identical
+    explicit matrices (tolerances as the graph replays)."""
+    P = 1500 if S == 4 else 200
+    prob = synth.make_problem(230 + S, 30 if S == 4 else 14, P, S=S, shape=shape, missing=0.03)
+    want = _oracle_gradient(oracle, prob, None, scaling, matrices="explicit")
+    ev = driver.Evaluator(amd, prob, precision=precision, scaling=scaling, matrices="explicit")
+    n = prob.tree.n
+    pool = list(range(ev.pre_base, ev.pre_base + 2 * n - 1))
+    lnl, g1, g2, used, _ = ev.checkpointed_gradient(pool, 2)
+    assert used == 2
+    _assert_gradient_close((lnl, g1, g2), want, precision)
+    for m in range(4, len(pool) + 1):
+        try:
+            ev.checkpointed_gradient(pool[:m], 0, plan_only=True)
+        except api.BeagleError:
+            continue
+        lnl, g1, g2, _, peak = ev.checkpointed_gradient(pool[:m], 0)
+        assert peak <= m
+        _assert_gradient_close((lnl, g1, g2), want, precision)
+        break
+    else:
+        raise AssertionError("no pool fits")
+    ev.close()
+
+
+def _small_partitions(seed, count):
+    """The 48-partition workload's generators at test size: state counts
+    cycling 4, 20, 61; patterns ~LogUniform over a per-state-count range;
+    Yule trees of 32-96 taxa (32-48 for 61 states); Gamma(4)."""
+    rng = np.random.default_rng(seed)
+    ranges = {4: (100, 3000), 20: (50, 400), 61: (40, 160)}
+    out = []
+    for k in range(count):
+        S = (4, 20, 61)[k % 3]
+        lo, hi = ranges[S]
+        P = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        taxa = int(rng.integers(32, 49 if S == 61 else 97))
+        out.append(synth.make_problem(seed + k, taxa, P, S=S, shape="yule", missing=0.02))
+    return out
+
+
+def test_partitions_workload_matches_oracle(amd, oracle):
+    """The 48-partition workload scaled down to 9 partitions (4-, 20- and
+    61-state models in turn, each with its own tree and pattern count): one
+    ASYNCH FP32 instance per partition, each logL + gradient evaluation
+    captured as a HIP graph; all 9 replays, at new branch lengths, are
+    launched across the instances' streams before any is collected, twice.
+    Every partition's logL (1e-6 relative) and every branch's dlogL/dt and
+    d2logL/dt2 (1e-4 of the largest; 1e-5 for 4 states) against the
+    oracle at the same lengths."""
+    probs = _small_partitions(4100, 9)
+    rng = np.random.default_rng(3)
+    evs = [driver.Evaluator(amd, p, precision="single", scaling="auto",
+                            extra_flags=api.FLAG_COMPUTATION_ASYNCH) for p in probs]
+    try:
+        for ev in evs:
+            ev.capture_fused_gradient()
+        for _ in range(2):
+            lens = [p.tree.lengths * rng.uniform(0.5, 1.5, p.tree.lengths.shape) for p in probs]
+            for ev, x in zip(evs, lens):
+                ev.replay_fused_gradient(x, wait=False)
+            got = [ev.collect_fused_gradient() for ev in evs]
+            for p, x, (lg, g1, g2) in zip(probs, lens, got):
+                lo, o1, o2 = _oracle_gradient(oracle, p, x, "auto")
+                tol = 1e-5 if p.S == 4 else 1e-4
+                assert rel(lg, lo) < 1e-6, (p.S, p.P)
+                assert np.abs(g1 - o1).max() <= tol * np.abs(o1).max(), (p.S, p.P)
+                assert np.abs(g2 - o2).max() <= tol * np.abs(o2).max(), (p.S, p.P)
+    finally:
+        for ev in evs:
+            ev.close()
+
+
+def test_codon_full_size_engines(amd):
+    """The 61-state GY94 codon workload at full size (64 taxa, 20,000
+    patterns, Gamma(4)), where the oracle would take minutes: the FP32 and
+    split-BF16 (3 and 4 products) matrix-core post-order give logL within
+    1e-6 relative of the FP64 matrix-core path, and the FP32 logL + branch
+    gradient is within 1e-6 of FP64 (relative to the largest component)."""
+    prob = synth.make_problem(3, 64, 20_000, S=61, shape="yule")
+    ev = driver.Evaluator(amd, prob)
+    l64 = ev.log_likelihood()
+    assert rel(l64, float(np.dot(ev.inst.site_log_likelihoods(), prob.pattern_weights))) < 1e-12
+    g64 = ev.fused_gradient()
+    ev.close()
+    ev = driver.Evaluator(amd, prob, precision="single", scaling="auto")
+    for terms in (0, 3, 4):
+        ev.inst.set_split_bf16(terms)
+        assert rel(ev.log_likelihood(), l64) < 1e-6, terms
+    ev.inst.set_split_bf16(0)
+    g32 = ev.fused_gradient()
+    ev.close()
+    assert rel(g32[0], g64[0]) < 1e-6
+    for a, b in zip(g32[1:], g64[1:]):
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
+
+
+def test_adjoint_full_size_512_taxa(amd):
+    """The adjoint workload at full size (512-taxon Yule tree, 50,000
+    patterns, 4-state GTR + Gamma(4); unscaled FP64 partials underflow at
+    this depth): the fused FP64 gradient extension with power-of-two
+    scaling equals the beagle.h call sequence with manual scale buffers
+    (1e-10); FP32 logL, branch gradient and rate-matrix / root-frequency
+    gradient within 1e-6 of FP64 (relative to the largest component)."""
+    prob = synth.make_problem(4, 512, 50_000, S=4, shape="yule")
+    ev = driver.Evaluator(amd, prob, scaling="manual")
+    api64 = ev.gradient()
+    ev.close()
+    ev = driver.Evaluator(amd, prob, scaling="auto")
+    f64 = ev.fused_gradient()
+    q64, pi64 = ev.rate_matrix_gradient()
+    ev.close()
+    assert np.isfinite(api64[0])
+    assert rel(f64[0], api64[0]) < 1e-12
+    for a, b in zip(f64[1:], api64[1:]):
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+    ev = driver.Evaluator(amd, prob, precision="single", scaling="auto")
+    f32 = ev.fused_gradient()
+    q32, pi32 = ev.rate_matrix_gradient()
+    ev.close()
+    assert rel(f32[0], f64[0]) < 1e-6
+    for a, b in zip(f32[1:], f64[1:]):
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
+    assert np.abs(q32 - q64).max() <= 1e-6 * np.abs(q64).max()
+    assert np.abs(pi32 - pi64).max() <= 1e-6 * np.abs(pi64).max()

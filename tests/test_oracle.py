@@ -34,6 +34,7 @@ from beagle_amd import api, driver, synth
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
 import cases  # noqa: E402
+import sequences  # noqa: E402
 
 GOLDEN = os.path.join(HERE, "golden", "reference_outputs.npz")
 KNOWN = os.path.join(HERE, "golden", "known_answers.json")
@@ -280,46 +281,97 @@ def test_non_generator_derivative_matrix_against_reference(oracle, ref):
         np.testing.assert_allclose(np.asarray(a), np.asarray(b), rtol=1e-12, atol=1e-14)
 
 
-def _api_sequence(lib, prob):
-    """Calls the fixtures do not hold: getPartials with a cumulative scale
-    buffer (BeagleCPUImpl.hpp:743-754), remove / accumulate / copy of scale
-    factors (:2763-2908) followed by a root log-likelihood over the
-    rebuilt cumulative buffer, and the partitioned pre-order pass
-    (:1415-1433)."""
-    ev = driver.Evaluator(lib, prob, scaling="manual", transpose="none")
-    inst, tree, n = ev.inst, prob.tree, prob.tree.n
-    out = {}
-    try:
-        ev.log_likelihood()
-        cum = n - 1
-        out["scaled_root"] = inst.get_partials(tree.root, cum)
-        inst.remove_scale_factors([1, 2], cum)
-        inst.accumulate_scale_factors([1, 2], cum)
-        inst.copy_scale_factors(1, 0)              # buffer 1 := buffer 0
-        inst.accumulate_scale_factors([1], cum)
-        inst.remove_scale_factors([0], cum)
-        out["cumulative_logL"] = np.array([inst.root_log_likelihood(tree.root, 0, 0, cum)])
-        ev.set_root_pre()
-        parts = np.zeros(prob.P, dtype=np.int32)
-        parts[prob.P // 2:] = 1
-        inst.set_pattern_partitions(2, parts)
-        ops = synth.preorder_ops(tree, ev.pre_base)
-        pops = [list(o) + [q, -1] for q in range(2) for o in ops]
-        inst.update_pre_partials_by_partition(np.array(pops, np.int32))
-        out["pre_by_partition"] = np.stack([inst.get_partials(ev.pre_base + v)
-                                            for v in ev.branch_nodes()[:3]])
-    finally:
-        ev.close()
-    return out
-
-
-def test_remaining_calls_against_reference(oracle, ref):
-    prob = synth.make_problem(51, 9, 120, S=4, missing=0.05, mean_length=0.3)
-    a = _api_sequence(oracle, prob)
-    b = _api_sequence(ref, prob)
+@pytest.mark.parametrize("S", [4, 20])
+def test_remaining_calls_against_reference(oracle, ref, S):
+    """sequences.api_sequence on the oracle and on the reference CPU
+    library: scaled getPartials, scale-factor bookkeeping (also by
+    partition), root log-likelihoods by partition, pre-order operations
+    with write-scale buffers (whole and by partition); 4 states go through
+    the reference's 4-state kernels, 20 through its generic ones."""
+    prob = synth.make_problem(51 + S, 9, 120 if S == 4 else 60, S=S, missing=0.05,
+                              mean_length=0.3)
+    a = sequences.api_sequence(oracle, prob)
+    b = sequences.api_sequence(ref, prob)
+    assert set(a) == set(b)
+    if S != 4:
+        # The reference's generic pre-order kernel clears the whole
+        # destination buffer before it computes one partition
+        # (BeagleCPUImpl.hpp:4545), so after the partition-1 operations only
+        # partition 1 is left; its 4-state kernel and the oracle keep the
+        # other partitions.  Compare partition 1 here; partition 0 is pinned
+        # against the whole-pattern pass below.
+        part1 = (np.arange(prob.C * prob.P * S) // S) % prob.P >= prob.P // 3
+        for k in ("pre_by_partition", "pre_by_partition_unscaled"):
+            a[k], b[k] = a[k][..., part1], b[k][..., part1]
     for k in a:
         scale = np.abs(b[k]).max()
         np.testing.assert_allclose(a[k], b[k], rtol=1e-12, atol=1e-13 * scale, err_msg=k)
+
+
+@pytest.mark.parametrize("S", [4, 20])
+def test_pre_order_by_partition_equals_whole_pass(oracle, S):
+    """The partitioned pre-order pass (every operation once per partition)
+    gives the whole-pattern pass's pre-order partials in every partition."""
+    prob = synth.make_problem(71 + S, 9, 90, S=S, missing=0.05)
+    ev = driver.Evaluator(oracle, prob, transpose="none")
+    ev.log_likelihood()
+    ops = ev.pre_order()
+    nodes = ev.branch_nodes()
+    whole = np.stack([ev.inst.get_partials(ev.pre_base + v) for v in nodes])
+    parts = np.zeros(prob.P, dtype=np.int32)
+    parts[prob.P // 3:] = 1
+    ev.inst.set_pattern_partitions(2, parts)
+    ev.set_root_pre()
+    ev.inst.update_pre_partials_by_partition(
+        np.array([list(o) + [q, -1] for q in range(2) for o in ops], np.int32))
+    split = np.stack([ev.inst.get_partials(ev.pre_base + v) for v in nodes])
+    ev.close()
+    np.testing.assert_array_equal(split, whole)
+
+
+def test_prescaled_derivatives_equal_unscaled(oracle):
+    """Per-pattern factors cancel from the derivative ratios: the edge
+    derivatives after a pre-order pass that rescales every operation equal
+    those of the unscaled pass, and the rescaled pre-order partials have a
+    per-pattern maximum of 1 (BeagleCPUImpl.hpp:4008-4056)."""
+    prob = synth.make_problem(58, 9, 120, S=4, missing=0.05, mean_length=0.3)
+    seq = sequences.api_sequence(oracle, prob)
+    ev = driver.Evaluator(oracle, prob, scaling="manual", transpose="none")
+    ev.log_likelihood()
+    ev.pre_order()
+    d1 = ev.edge_derivatives(False, per_site=True)
+    d2 = ev.edge_derivatives(True, per_site=False)
+    ev.close()
+    np.testing.assert_allclose(seq["pre_scaled_d1_sites"], np.asarray(d1[0]).reshape(-1),
+                               rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(seq["pre_scaled_d1_sums"],
+                               np.concatenate([np.asarray(d1[1]), np.asarray(d1[2])]),
+                               rtol=1e-11)
+    np.testing.assert_allclose(seq["pre_scaled_d2_sums"], np.asarray(d2[1]), rtol=1e-11)
+    x = seq["pre_rescaled"].reshape(4, prob.C, prob.P, prob.S)
+    np.testing.assert_allclose(x.max(axis=(1, 3)), 1.0, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("S", [4, 20, 61])
+def test_transition_derivatives_against_reference(oracle, ref, S):
+    """beagleUpdateTransitionMatrices with first- and second-derivative
+    index lists (EigenDecompositionCube.hpp:105-322): P, dP/dt and d2P/dt2
+    of the oracle equal the reference CPU library's, and dP/dt = r Q P,
+    d2P/dt2 = r^2 Q^2 P (closed forms of the exponential)."""
+    prob = synth.make_problem(60 + S, 6, 20, S=S)
+    lengths = [0.0, 0.01, 0.1, 0.7]
+    a = sequences.transition_derivatives(oracle, prob, lengths)
+    b = sequences.transition_derivatives(ref, prob, lengths)
+    for x, y in zip(a, b):
+        np.testing.assert_allclose(x, y, rtol=0, atol=1e-14 * max(1.0, np.abs(y).max()))
+    Q = prob.model.Q
+    for u, t in enumerate(lengths):
+        for c, r in enumerate(prob.rates):
+            P = a[0][u, c]
+            np.testing.assert_allclose(a[1][u, c], r * Q @ P, rtol=0,
+                                       atol=1e-11 * max(1.0, r * np.abs(Q).max()))
+            np.testing.assert_allclose(a[2][u, c], r * r * (Q @ Q) @ P, rtol=0,
+                                       atol=1e-10 * max(1.0, (r * np.abs(Q).max()) ** 2))
 
 
 def reordered_partition_run(lib, prob, parts, tips_as, **kw):
