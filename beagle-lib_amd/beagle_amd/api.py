@@ -157,6 +157,7 @@ SIGNATURES = {
     "ExtCalculateLogLikelihoodAndDerivatives": (_I, [_I, _I, _I, _I, _I, _I, _IP, _I, _IP, _IP,
                                                      _IP, _DP, _DP, _DP, _DP]),
     "ExtCalculateRateMatrixGradient": (_I, [_I, _I, _IP, _I, _IP, _I, _I, _I, _DP, _DP]),
+    "ExtCalculateRootLogLikelihoodsNoAlloc": (_I, [_I, _IP, _IP, _IP, _IP, _I, _DP]),
 }
 
 
@@ -409,6 +410,16 @@ class Instance:
         res = np.zeros(1) if out is None else out
         a, b, c, d = _ip([buf]), _ip([weights_index]), _ip([freqs_index]), _ip([cumulative])
         rc = self._c("CalculateRootLogLikelihoods", a[1], b[1], c[1], d[1], 1,
+                     res.ctypes.data_as(c_double_p), allow=allow)
+        return float(res[0]) if rc >= 0 else rc
+
+    def root_log_likelihood_no_alloc(self, buf, weights_index=0, freqs_index=0,
+                                     cumulative=OP_NONE, allow=()):
+        """Extension: the single-root log-likelihood through the entry point
+        that allocates no device memory."""
+        res = np.zeros(1)
+        a, b, c, d = _ip([buf]), _ip([weights_index]), _ip([freqs_index]), _ip([cumulative])
+        rc = self._c("ExtCalculateRootLogLikelihoodsNoAlloc", a[1], b[1], c[1], d[1], 1,
                      res.ctypes.data_as(c_double_p), allow=allow)
         return float(res[0]) if rc >= 0 else rc
 

@@ -487,6 +487,19 @@ int beagleGetScaleFactors(int instance, int src, double* out) {
     return withInstance(instance, [&](Instance* in) { return in->getScaleFactors(src, out); });
 }
 
+int beagleExtCalculateRootLogLikelihoodsNoAlloc(int instance, const int* bufferIndices,
+                                                const int* categoryWeightsIndices,
+                                                const int* stateFrequenciesIndices,
+                                                const int* cumulativeScaleIndices, int count,
+                                                double* outSumLogLikelihood) {
+    return withInstance(instance, [&](Instance* in) {
+        return in->calculateRootLogLikelihoodsNoAlloc(bufferIndices, categoryWeightsIndices,
+                                                      stateFrequenciesIndices,
+                                                      cumulativeScaleIndices, count,
+                                                      outSumLogLikelihood);
+    });
+}
+
 int beagleCalculateRootLogLikelihoods(int instance, const int* bufferIndices,
                                       const int* categoryWeightsIndices,
                                       const int* stateFrequenciesIndices,

@@ -198,6 +198,10 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
     dDummy_ = dalloc<char>(dummy_bytes());
     HIPCHECK(hipMemsetAsync(dDummy_, 0, dummy_bytes(), stream_));
     HIPCHECK(hipHostMalloc((void**)&hResult_, 64 * sizeof(double), hipHostMallocDefault));
+    // device address of the allocation-free entry point's scratch array,
+    // looked up once per device with that device current (null: the entry
+    // point reports BEAGLE_ERROR_GENERAL)
+    noAllocScratch_ = noalloc_scratch(device_);
     partStart_ = {0, patternCount};
     if (const char* m = std::getenv("BEAGLE_AMD_TRAVERSAL")) traversalMode_ = std::atoi(m);
     if (const char* m = std::getenv("BEAGLE_AMD_LEVEL_CHUNK")) levelChunk_ = std::max(1, std::atoi(m));
@@ -1976,6 +1980,48 @@ int Instance::calculateRootLogLikelihoods(const int* bufIdx, const int* wIdx, co
         return BEAGLE_SUCCESS;
     }
     HIPCHECK(hipMemcpyAsync(hResult_, dResult_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    sync();
+    lastLogL_ = hResult_[0];
+    *out = lastLogL_;
+    return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+}
+
+// The single-root case of calculateRootLogLikelihoods with the root
+// kernels' workgroup sums and the result slot in this GPU's fixed scratch
+// array instead of dWork_ (which grows on demand): no device allocation and
+// no host staging beyond the pinned result mirror made at creation.  The
+// kernels write every workgroup sum before the reduction reads it and
+// overwrite the result slot, so nothing needs clearing between calls; calls
+// on one GPU that overlap in time share the array and race.  Not capturable
+// into this library's graphs.
+int Instance::calculateRootLogLikelihoodsNoAlloc(const int* bufIdx, const int* wIdx,
+                                                 const int* fIdx, const int* cumIdx, int count,
+                                                 double* out) {
+    if (capturing_) captureBreak("the allocation-free root log-likelihood cannot be captured");
+    if (count != 1 || wIdx[0] < 0) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    if (!noAllocScratch_) return BEAGLE_ERROR_GENERAL;
+    const int b = bufIdx[0], w = wIdx[0], f = fIdx[0];
+    const int cum = cumIdx ? cumIdx[0] : BEAGLE_OP_NONE;
+    if (!validBuffer(b) || !dPartials_[b] || !validEigen(w) || !validEigen(f) ||
+        hCatWeights_[w].empty() || hFreqs_[f].empty())
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    const int cs = always_ ? b - tipCount_ : (auto_ ? BEAGLE_OP_NONE : cum);
+    if (cs != BEAGLE_OP_NONE && !validScale(cs)) return BEAGLE_ERROR_OUT_OF_RANGE;
+    if (root_block_count(dims_, 0, dims_.P) > kNoAllocScratchDoubles - 1)
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    double* sums = noAllocScratch_;
+    double* result = noAllocScratch_ + (kNoAllocScratchDoubles - 1);
+    const short* ex = auto_ ? dExp_[b] : nullptr;
+    if (dp_)
+        launch_root<double>(dPartials_[b], ex, dCatWeights_ + (size_t)w * dims_.C,
+                            dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
+                            dPatternWeights_, dSiteL_, sums, result, dims_, stream_);
+    else
+        launch_root<float>(dPartials_[b], ex, dCatWeights_ + (size_t)w * dims_.C,
+                           dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
+                           dPatternWeights_, dSiteL_, sums, result, dims_, stream_);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(hResult_, result, sizeof(double), hipMemcpyDeviceToHost, stream_));
     sync();
     lastLogL_ = hResult_[0];
     *out = lastLogL_;

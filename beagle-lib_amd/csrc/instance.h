@@ -125,6 +125,10 @@ public:
 
     int calculateRootLogLikelihoods(const int* bufIdx, const int* wIdx, const int* fIdx,
                                     const int* cumIdx, int count, double* out);
+    // calculateRootLogLikelihoods (one root) with its workgroup sums and
+    // result in the GPU's fixed scratch array (noalloc.hip): no allocation
+    int calculateRootLogLikelihoodsNoAlloc(const int* bufIdx, const int* wIdx, const int* fIdx,
+                                           const int* cumIdx, int count, double* out);
     int calculateRootLogLikelihoodsByPartition(const int* bufIdx, const int* wIdx,
                                                const int* fIdx, const int* cumIdx,
                                                const int* partIdx, int partitionCount, int count,
@@ -309,6 +313,7 @@ private:
     double* dSiteD1_ = nullptr;
     double* dSiteD2_ = nullptr;
     double* dResult_ = nullptr;       // small device result vector
+    double* noAllocScratch_ = nullptr;  // this device's scratch array (noalloc.hip)
     char* dDummy_ = nullptr;          // scratch tile for the pipelined traversal
     double* hResult_ = nullptr;       // pinned mirror
     void* dWork_ = nullptr;           // grow-only reduction scratch

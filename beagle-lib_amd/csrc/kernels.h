@@ -346,6 +346,17 @@ void launch_edge_loglik(const void* parent, const void* child, const int* childS
 
 // Rows of S*S partial sums the cross-product launch writes into its workspace.
 int cross_products_rows(const Dims& d);
+// Workgroup sums launch_root / launch_root_range write for patterns [p0, p1).
+int root_block_count(const Dims& d, int p0, int p1);
+
+// Fixed device scratch of the allocation-free root log-likelihood
+// (noalloc.hip): 256 KiB per GPU, the last double the result slot, the rest
+// workgroup sums (root_block_count must not exceed kNoAllocScratchDoubles - 1:
+// at most 1,048,544 patterns for S > 4, 8,388,352 for S <= 4).
+constexpr int kNoAllocScratchDoubles = 32768;
+// Device address of this device's scratch array (looked up once per device,
+// with that device current); null if the lookup failed.
+double* noalloc_scratch(int device);
 
 void launch_accumulate(double* cum, const double* const* bufs, int nbufs, double sign,
                        int logScalers, int p0, int p1, hipStream_t st);

@@ -2644,6 +2644,11 @@ static int root_blocks_and_launch(const void* root, const short* rootExp, const 
     return nb;
 }
 
+int root_block_count(const Dims& d, int p0, int p1) {
+    if (p1 <= p0) return 0;
+    return d.sStride == PTILE ? (p1 - p0 + 31) / 32 : (p1 - p0 + WG - 1) / WG;
+}
+
 template <typename R>
 void launch_root(const void* root, const short* rootExp, const double* catWeights,
                  const double* freqs, const double* cumulative, const double* patternWeights,

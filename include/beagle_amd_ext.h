@@ -202,6 +202,27 @@ int beagleExtCalculateRateMatrixGradient(int instance, int eigenIndex,
                                           double* outGradientRateMatrix,
                                           double* outGradientFrequencies);
 
+/* beagleCalculateRootLogLikelihoods for one root (count = 1, a category
+ * weights index >= 0) that allocates no device memory.  No counterpart in
+ * the reference: for callers that cannot call hipMalloc (serving loops,
+ * harnesses that forbid allocation, code running while a stream is being
+ * captured).  Arguments and results as beagleCalculateRootLogLikelihoods;
+ * the site log-likelihoods are kept for beagleGetSiteLogLikelihoods.  The
+ * root kernels' workgroup sums and the result go to a fixed 256 KiB device
+ * array per GPU (held until the process ends) instead of instance buffers
+ * grown on demand; the result is returned through pinned memory made when
+ * the instance was created.  Limits: at most 1,048,544 patterns with more
+ * than 4 states, 8,388,352 with 4 or fewer (BEAGLE_ERROR_OUT_OF_RANGE
+ * beyond); count != 1 or a negative weights index gives
+ * BEAGLE_ERROR_NO_IMPLEMENTATION.  Every instance on one GPU shares that
+ * GPU's array: calls that overlap in time on one GPU (instances driven from
+ * different threads) race.  Not recorded by beagleExtBeginCapture. */
+int beagleExtCalculateRootLogLikelihoodsNoAlloc(int instance, const int* bufferIndices,
+                                                const int* categoryWeightsIndices,
+                                                const int* stateFrequenciesIndices,
+                                                const int* cumulativeScaleIndices, int count,
+                                                double* outSumLogLikelihood);
+
 #ifdef __cplusplus
 }
 #endif
