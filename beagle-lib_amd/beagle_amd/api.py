@@ -144,6 +144,7 @@ SIGNATURES = {
     "ExtGetStream": (_I, [_I, ctypes.POINTER(ctypes.c_void_p)]),
     "ExtSetSplitBF16": (_I, [_I, _I]),
     "ExtSetAutotune": (_I, [_I, _I]),
+    "ExtGetAutotuneStats": (_I, [_I, _IP, _IP]),
     "ExtCheckpointedGradient": (_I, [_I, _IP, _I, _IP, _IP, _I, _I, _IP, _I, _I, _IP, _IP, _DP,
                                      _DP, _DP, _DP]),
     "ExtPlanCheckpoints": (_I, [_I, _IP, _I, _I, _I, _I, _IP, _IP, _IP, _IP]),
@@ -494,6 +495,25 @@ class Instance:
                 ctypes.byref(o2))
         return out.value, o1.value, o2.value
 
+    def edge_log_likelihood_by_partition(self, parents, children, pmats, d1, d2, weights,
+                                         freqs, cumulative, parts):
+        """Per-partition edge log-likelihoods (and, with d1 and d2 lists,
+        first and second derivatives): returns (logL by partition, total,
+        d1 by partition, total, d2 by partition, total)."""
+        n = len(parts)
+        by = [np.zeros(n) for _ in range(3)]
+        tot = [ctypes.c_double(0.0) for _ in range(3)]
+        a, b, c = _ip(parents), _ip(children), _ip(pmats)
+        e1 = _ip(d1) if d1 is not None else None
+        e2 = _ip(d2) if d2 is not None else None
+        w, f, s, q = _ip(weights), _ip(freqs), _ip(cumulative), _ip(parts)
+        self._c("CalculateEdgeLogLikelihoodsByPartition", a[1], b[1], c[1],
+                e1[1] if e1 else None, e2[1] if e2 else None, w[1], f[1], s[1], q[1], n, 1,
+                by[0].ctypes.data_as(c_double_p), ctypes.byref(tot[0]),
+                by[1].ctypes.data_as(c_double_p), ctypes.byref(tot[1]),
+                by[2].ctypes.data_as(c_double_p), ctypes.byref(tot[2]))
+        return by[0], tot[0].value, by[1], tot[1].value, by[2], tot[2].value
+
     def pre_partials_with_derivatives(self, ops, post, d1, d2, weights_index=0, outputs=True):
         """Extension: pre-order pass fused with per-branch derivative sums.
         Returns (sum w L'/L, sum w (L'/L)^2, sum w L''/L) per operation;
@@ -573,8 +593,14 @@ class Instance:
     # HIP-graph capture (extension entry points): calls between
     # begin_capture() and end_capture() are recorded; their output arrays
     # (which must stay alive) receive each replay's results at wait_graph().
+    def autotune_stats(self):
+        """Extension: (shape classes tuned so far, of which pending)."""
+        tuned, pending = ctypes.c_int(0), ctypes.c_int(0)
+        self._c("ExtGetAutotuneStats", ctypes.byref(tuned), ctypes.byref(pending))
+        return tuned.value, pending.value
+
     def set_autotune(self, on=True):
-        """Extension: per-operation-list choice of launch plans by timing."""
+        """Extension: per-shape-class choice of launch plans by timing."""
         self._c("ExtSetAutotune", int(bool(on)))
 
     def begin_capture(self):

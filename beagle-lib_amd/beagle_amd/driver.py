@@ -16,6 +16,8 @@ The scale buffer of internal node v is v-n; the cumulative buffer is n-1.
 With pre_scaling, pre-order operation k rescales its output into scale
 buffer n+k (the pattern of examples/hmctest/hmctest.cpp:503-506).
 """
+import dataclasses
+
 import numpy as np
 
 from . import api, synth
@@ -86,6 +88,16 @@ class Evaluator:
         self._branch_idx = np.arange(2 * n - 2, dtype=np.int32)
         self._post_ops = {}
         self._pre = None
+
+    def set_tree(self, tree):
+        """Another topology over the same tips (a sampler's proposal): the
+        operation lists are rebuilt on next use."""
+        if tree.n != self.n:
+            raise ValueError("the new tree must have the same tips")
+        self.prob = dataclasses.replace(self.prob, tree=tree)
+        self._post_ops = {}
+        self._pre = None
+        self.__dict__.pop("_nodes", None)
 
     # -- data ---------------------------------------------------------------
     def _setup(self):

@@ -577,11 +577,23 @@ int beagleCalculateEdgeLogLikelihoods(int instance, const int* parentBufferIndic
     });
 }
 
-int beagleCalculateEdgeLogLikelihoodsByPartition(int instance, const int*, const int*, const int*,
-                                                 const int*, const int*, const int*, const int*,
-                                                 const int*, const int*, int, int, double*,
-                                                 double*, double*, double*, double*, double*) {
-    return withInstance(instance, [&](Instance*) { return BEAGLE_ERROR_NO_IMPLEMENTATION; });
+int beagleCalculateEdgeLogLikelihoodsByPartition(
+    int instance, const int* parentBufferIndices, const int* childBufferIndices,
+    const int* probabilityIndices, const int* firstDerivativeIndices,
+    const int* secondDerivativeIndices, const int* categoryWeightsIndices,
+    const int* stateFrequenciesIndices, const int* cumulativeScaleIndices,
+    const int* partitionIndices, int partitionCount, int count,
+    double* outSumLogLikelihoodByPartition, double* outSumLogLikelihood,
+    double* outSumFirstDerivativeByPartition, double* outSumFirstDerivative,
+    double* outSumSecondDerivativeByPartition, double* outSumSecondDerivative) {
+    return withInstance(instance, [&](Instance* in) {
+        return in->calculateEdgeLogLikelihoodsByPartition(
+            parentBufferIndices, childBufferIndices, probabilityIndices, firstDerivativeIndices,
+            secondDerivativeIndices, categoryWeightsIndices, stateFrequenciesIndices,
+            cumulativeScaleIndices, partitionIndices, partitionCount, count,
+            outSumLogLikelihoodByPartition, outSumLogLikelihood, outSumFirstDerivativeByPartition,
+            outSumFirstDerivative, outSumSecondDerivativeByPartition, outSumSecondDerivative);
+    });
 }
 
 int beagleGetLogLikelihood(int instance, double* out) {
@@ -659,6 +671,10 @@ int beagleExtSetSplitBF16(int instance, int terms) {
 
 int beagleExtSetAutotune(int instance, int on) {
     return withInstance(instance, [&](Instance* in) { return in->setAutotune(on); });
+}
+
+int beagleExtGetAutotuneStats(int instance, int* outTuned, int* outPending) {
+    return withInstance(instance, [&](Instance* in) { return in->autotuneStats(outTuned, outPending); });
 }
 
 int beagleExtPlanCheckpoints(int tipCount, const BeagleOperation* operations, int count,

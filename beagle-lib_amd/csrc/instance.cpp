@@ -117,107 +117,118 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
         bufferCount_ <= tipCount || eigenBufferCount < 1 || matrixBufferCount < 1 ||
         scaleBufferCount < 0)
         throw std::out_of_range("instance dimensions");
-    HIPCHECK(hipSetDevice(device_));
-    const long both = preferenceFlags | requirementFlags;
-    complex_ = (both & BEAGLE_FLAG_EIGEN_COMPLEX) != 0;
-    flags_ = BEAGLE_FLAG_PROCESSOR_GPU | BEAGLE_FLAG_VECTOR_NONE | BEAGLE_FLAG_THREADING_NONE |
-             (complex_ ? BEAGLE_FLAG_EIGEN_COMPLEX : BEAGLE_FLAG_EIGEN_REAL) |
-             BEAGLE_FLAG_FRAMEWORK_HIP | BEAGLE_FLAG_PARALLELOPS_GRID;
-    flags_ |= dp_ ? BEAGLE_FLAG_PRECISION_DOUBLE : BEAGLE_FLAG_PRECISION_SINGLE;
-    flags_ |= (both & BEAGLE_FLAG_COMPUTATION_ASYNCH) ? BEAGLE_FLAG_COMPUTATION_ASYNCH
-                                                      : BEAGLE_FLAG_COMPUTATION_SYNCH;
-    flags_ |= (both & BEAGLE_FLAG_INVEVEC_TRANSPOSED) ? BEAGLE_FLAG_INVEVEC_TRANSPOSED
-                                                      : BEAGLE_FLAG_INVEVEC_STANDARD;
-    const int internal = bufferCount_ - tipCount_;
-    always_ = false;
-    auto_ = false;
-    if (both & BEAGLE_FLAG_SCALING_AUTO) {
-        auto_ = true;
-        flags_ |= BEAGLE_FLAG_SCALING_AUTO | BEAGLE_FLAG_SCALERS_LOG;
-    } else if (both & BEAGLE_FLAG_SCALING_ALWAYS) {
-        always_ = true;
-        flags_ |= BEAGLE_FLAG_SCALING_ALWAYS | BEAGLE_FLAG_SCALERS_LOG;
-        scaleCount_ = internal + 1;
-    } else if (both & BEAGLE_FLAG_SCALERS_LOG) {
-        flags_ |= BEAGLE_FLAG_SCALING_MANUAL | BEAGLE_FLAG_SCALERS_LOG;
-    } else {
-        flags_ |= BEAGLE_FLAG_SCALING_MANUAL | BEAGLE_FLAG_SCALERS_RAW;
-    }
-    logScalers_ = (flags_ & BEAGLE_FLAG_SCALERS_LOG) != 0;
-    // Pre-order "self" matrices and derivative matrices arrive transposed
-    // for S > 4 unless the client asked the library to transpose
-    // (the reference GPU convention, GPU/BeagleGPUImpl.hpp:483-486, 787-788).
-    const bool autoT = (both & (long)BEAGLE_FLAG_PREORDER_TRANSPOSE_AUTO) != 0;
-    flags_ |= autoT ? (long)BEAGLE_FLAG_PREORDER_TRANSPOSE_AUTO
-                    : (long)BEAGLE_FLAG_PREORDER_TRANSPOSE_MANUAL;
-    preTransposed_ = stateCount > 4 && !autoT;
+    // a failed allocation (or any other error) part way through releases
+    // what was made so far before the error reaches the caller
+    try {
+        HIPCHECK(hipSetDevice(device_));
+        const long both = preferenceFlags | requirementFlags;
+        complex_ = (both & BEAGLE_FLAG_EIGEN_COMPLEX) != 0;
+        flags_ = BEAGLE_FLAG_PROCESSOR_GPU | BEAGLE_FLAG_VECTOR_NONE | BEAGLE_FLAG_THREADING_NONE |
+                 (complex_ ? BEAGLE_FLAG_EIGEN_COMPLEX : BEAGLE_FLAG_EIGEN_REAL) |
+                 BEAGLE_FLAG_FRAMEWORK_HIP | BEAGLE_FLAG_PARALLELOPS_GRID;
+        flags_ |= dp_ ? BEAGLE_FLAG_PRECISION_DOUBLE : BEAGLE_FLAG_PRECISION_SINGLE;
+        flags_ |= (both & BEAGLE_FLAG_COMPUTATION_ASYNCH) ? BEAGLE_FLAG_COMPUTATION_ASYNCH
+                                                          : BEAGLE_FLAG_COMPUTATION_SYNCH;
+        flags_ |= (both & BEAGLE_FLAG_INVEVEC_TRANSPOSED) ? BEAGLE_FLAG_INVEVEC_TRANSPOSED
+                                                          : BEAGLE_FLAG_INVEVEC_STANDARD;
+        const int internal = bufferCount_ - tipCount_;
+        always_ = false;
+        auto_ = false;
+        if (both & BEAGLE_FLAG_SCALING_AUTO) {
+            auto_ = true;
+            flags_ |= BEAGLE_FLAG_SCALING_AUTO | BEAGLE_FLAG_SCALERS_LOG;
+        } else if (both & BEAGLE_FLAG_SCALING_ALWAYS) {
+            always_ = true;
+            flags_ |= BEAGLE_FLAG_SCALING_ALWAYS | BEAGLE_FLAG_SCALERS_LOG;
+            scaleCount_ = internal + 1;
+        } else if (both & BEAGLE_FLAG_SCALERS_LOG) {
+            flags_ |= BEAGLE_FLAG_SCALING_MANUAL | BEAGLE_FLAG_SCALERS_LOG;
+        } else {
+            flags_ |= BEAGLE_FLAG_SCALING_MANUAL | BEAGLE_FLAG_SCALERS_RAW;
+        }
+        logScalers_ = (flags_ & BEAGLE_FLAG_SCALERS_LOG) != 0;
+        // Pre-order "self" matrices and derivative matrices arrive transposed
+        // for S > 4 unless the client asked the library to transpose
+        // (the reference GPU convention, GPU/BeagleGPUImpl.hpp:483-486, 787-788).
+        const bool autoT = (both & (long)BEAGLE_FLAG_PREORDER_TRANSPOSE_AUTO) != 0;
+        flags_ |= autoT ? (long)BEAGLE_FLAG_PREORDER_TRANSPOSE_AUTO
+                        : (long)BEAGLE_FLAG_PREORDER_TRANSPOSE_MANUAL;
+        preTransposed_ = stateCount > 4 && !autoT;
 
-    dims_.S = stateCount;
-    dims_.P = patternCount;
-    dims_.C = categoryCount;
-    dims_.Sp = stateCount;
-    dims_.Ppad = roundUp(patternCount, 64);
-    dims_.catStride = (long)dims_.Ppad * dims_.Sp;
-    dims_.autoScale = auto_ ? 1 : 0;
-    dims_.sStride = stateCount > 4 ? PTILE : 1;
-    partialsBytes_ = (size_t)categoryCount * dims_.catStride * realSize();
-    matrixBytes_ = (size_t)categoryCount * stateCount * (stateCount + 1) * realSize();
-    matrixBytes_ = (matrixBytes_ + 255) / 256 * 256;
+        dims_.S = stateCount;
+        dims_.P = patternCount;
+        dims_.C = categoryCount;
+        dims_.Sp = stateCount;
+        dims_.Ppad = roundUp(patternCount, 64);
+        dims_.catStride = (long)dims_.Ppad * dims_.Sp;
+        dims_.autoScale = auto_ ? 1 : 0;
+        dims_.sStride = stateCount > 4 ? PTILE : 1;
+        partialsBytes_ = (size_t)categoryCount * dims_.catStride * realSize();
+        matrixBytes_ = (size_t)categoryCount * stateCount * (stateCount + 1) * realSize();
+        matrixBytes_ = (matrixBytes_ + 255) / 256 * 256;
 
-    HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    dPartials_.assign(bufferCount_, nullptr);
-    dExp_.assign(bufferCount_, nullptr);
-    dStates_.assign(bufferCount_, nullptr);
-    for (int b = tipCount_; b < bufferCount_; ++b) ensurePartials(b);
-    dMatrices_ = dalloc<char>(matrixBytes_ * matrixCount_);
-    HIPCHECK(hipMemsetAsync(dMatrices_, 0, matrixBytes_ * matrixCount_, stream_));
-    dScale_ = dalloc<double>((size_t)std::max(scaleCount_, 1) * dims_.Ppad);
-    launch_fill(dScale_, 0.0, (long)std::max(scaleCount_, 1) * dims_.Ppad, stream_);
-    const size_t S = stateCount;
-    dCube_ = dalloc<double>(eigenCount_ * S * S * S);
-    dVVi_ = dalloc<double>(eigenCount_ * 2 * S * S);
-    dEvals_ = dalloc<double>(eigenCount_ * 2 * S);   // real parts, then imaginary parts
-    eigenSet_.assign(eigenCount_, false);
-    hEigen_.resize(eigenCount_);
-    matOrigin_.resize(matrixCount_);
-    dFreqs_ = dalloc<double>(eigenCount_ * S);
-    dCatWeights_ = dalloc<double>(eigenCount_ * categoryCount);
-    dCatRates_ = dalloc<double>(eigenCount_ * categoryCount);
-    hFreqs_.assign(eigenCount_, {});
-    hCatWeights_.assign(eigenCount_, {});
-    hCatRates_.assign(eigenCount_, {});
-    dPatternWeights_ = dalloc<double>(dims_.Ppad);
-    launch_fill(dPatternWeights_, 0.0, dims_.Ppad, stream_);
-    hPatternWeights_.assign(patternCount, 0.0);
-    dSiteL_ = dalloc<double>(dims_.Ppad);
-    dSiteD1_ = dalloc<double>(dims_.Ppad);
-    dSiteD2_ = dalloc<double>(dims_.Ppad);
-    dResult_ = dalloc<double>(64);
-    // one-tile stand-ins read by unused operand slots of the pipelined
-    // traversal (in bounds for every lane and category, never stored to)
-    dDummy_ = dalloc<char>(dummy_bytes());
-    HIPCHECK(hipMemsetAsync(dDummy_, 0, dummy_bytes(), stream_));
-    HIPCHECK(hipHostMalloc((void**)&hResult_, 64 * sizeof(double), hipHostMallocDefault));
-    // device address of the allocation-free entry point's scratch array,
-    // looked up once per device with that device current (null: the entry
-    // point reports BEAGLE_ERROR_GENERAL)
-    noAllocScratch_ = noalloc_scratch(device_);
-    partStart_ = {0, patternCount};
-    if (const char* m = std::getenv("BEAGLE_AMD_TRAVERSAL")) traversalMode_ = std::atoi(m);
-    if (const char* m = std::getenv("BEAGLE_AMD_LEVEL_CHUNK")) levelChunk_ = std::max(1, std::atoi(m));
-    if (const char* m = std::getenv("BEAGLE_AMD_MFMA_VARIANT")) {
-        mfmaVariant_ = std::atoi(m);
-        mfmaVariantForced_ = true;
+        HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        dPartials_.assign(bufferCount_, nullptr);
+        dExp_.assign(bufferCount_, nullptr);
+        dStates_.assign(bufferCount_, nullptr);
+        for (int b = tipCount_; b < bufferCount_; ++b) ensurePartials(b);
+        dMatrices_ = dalloc<char>(matrixBytes_ * matrixCount_);
+        HIPCHECK(hipMemsetAsync(dMatrices_, 0, matrixBytes_ * matrixCount_, stream_));
+        dScale_ = dalloc<double>((size_t)std::max(scaleCount_, 1) * dims_.Ppad);
+        launch_fill(dScale_, 0.0, (long)std::max(scaleCount_, 1) * dims_.Ppad, stream_);
+        const size_t S = stateCount;
+        dCube_ = dalloc<double>(eigenCount_ * S * S * S);
+        dVVi_ = dalloc<double>(eigenCount_ * 2 * S * S);
+        dEvals_ = dalloc<double>(eigenCount_ * 2 * S);   // real parts, then imaginary parts
+        eigenSet_.assign(eigenCount_, false);
+        hEigen_.resize(eigenCount_);
+        matOrigin_.resize(matrixCount_);
+        dFreqs_ = dalloc<double>(eigenCount_ * S);
+        dCatWeights_ = dalloc<double>(eigenCount_ * categoryCount);
+        dCatRates_ = dalloc<double>(eigenCount_ * categoryCount);
+        hFreqs_.assign(eigenCount_, {});
+        hCatWeights_.assign(eigenCount_, {});
+        hCatRates_.assign(eigenCount_, {});
+        dPatternWeights_ = dalloc<double>(dims_.Ppad);
+        launch_fill(dPatternWeights_, 0.0, dims_.Ppad, stream_);
+        hPatternWeights_.assign(patternCount, 0.0);
+        dSiteL_ = dalloc<double>(dims_.Ppad);
+        dSiteD1_ = dalloc<double>(dims_.Ppad);
+        dSiteD2_ = dalloc<double>(dims_.Ppad);
+        dResult_ = dalloc<double>(64);
+        // one-tile stand-ins read by unused operand slots of the pipelined
+        // traversal (in bounds for every lane and category, never stored to)
+        dDummy_ = dalloc<char>(dummy_bytes());
+        HIPCHECK(hipMemsetAsync(dDummy_, 0, dummy_bytes(), stream_));
+        HIPCHECK(hipHostMalloc((void**)&hResult_, 64 * sizeof(double), hipHostMallocDefault));
+        // device address of the allocation-free entry point's scratch array,
+        // looked up once per device with that device current (null: the entry
+        // point reports BEAGLE_ERROR_GENERAL)
+        noAllocScratch_ = noalloc_scratch(device_);
+        partStart_ = {0, patternCount};
+        if (const char* m = std::getenv("BEAGLE_AMD_TRAVERSAL")) traversalMode_ = std::atoi(m);
+        if (const char* m = std::getenv("BEAGLE_AMD_LEVEL_CHUNK")) levelChunk_ = std::max(1, std::atoi(m));
+        if (const char* m = std::getenv("BEAGLE_AMD_MFMA_VARIANT")) {
+            mfmaVariant_ = std::atoi(m);
+            mfmaVariantForced_ = true;
+        }
+        if (const char* m = std::getenv("BEAGLE_AMD_AUTOTUNE")) autotune_ = std::atoi(m) != 0;
+        ring_.resize(8);
+        for (auto& s : ring_) HIPCHECK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+        jobs_.resize(8);
+        for (auto& e : jobs_) HIPCHECK(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming));
+        sync();
+    } catch (...) {
+        release();
+        throw;
     }
-    if (const char* m = std::getenv("BEAGLE_AMD_AUTOTUNE")) autotune_ = std::atoi(m) != 0;
-    ring_.resize(8);
-    for (auto& s : ring_) HIPCHECK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
-    jobs_.resize(8);
-    for (auto& e : jobs_) HIPCHECK(hipEventCreateWithFlags(&e.ev, hipEventDisableTiming));
-    sync();
 }
 
-Instance::~Instance() {
+Instance::~Instance() { release(); }
+
+// Frees everything the instance holds (also a partly built one: every
+// handle starts null).
+void Instance::release() {
     hipSetDevice(device_);
     if (capturing_) {
         hipGraph_t g = nullptr;
@@ -269,7 +280,12 @@ Instance::~Instance() {
         if (e.dev) hipFree(e.dev);
         if (e.ev) hipEventDestroy(e.ev);
     }
+    clearTuning();
     if (stream_) hipStreamDestroy(stream_);
+    stream_ = nullptr;
+    // a failed allocation leaves its error for hipGetLastError: clear it so
+    // it is not reported by a later, unrelated call
+    (void)hipGetLastError();
 }
 
 const char* Instance::implName() const {
@@ -938,8 +954,8 @@ int Instance::updatePartials(const int* ops, int count, int cumulativeIndex, boo
         if (shared) {
             launchPostPlan(plan(0));
         } else {
-            const int k = tunedChoice(hashOps(ops, (size_t)count * stride, 1), kNumPlans,
-                                      forcedPlan(), [&](int c) { launchPostPlan(plan(c)); });
+            const int k = tunedChoice(shapeKey(1, count, nl), kNumPlans, forcedPlan(),
+                                      [&](int c) { launchPostPlan(plan(c)); });
             if (k >= 0) launchPostPlan(plan(k));
         }
     }
@@ -959,22 +975,28 @@ int Instance::setSplitBf16(int terms) {
 // ---- autotuning ----------------------------------------------------------------
 // A traversal's launch plan (4-state: one fused launch, or levels batched
 // above a width threshold; MFMA: partials pipelined or a whole tile up
-// front) does not change its results, only its speed.  On the first call
-// with a given operation list the instance runs every candidate twice,
-// times the second run with events on its stream, keeps the fastest for
-// that list, and leaves the last run's outputs in place (the operations are
-// idempotent: lists that add to a cumulative scale buffer are not tuned).
+// front) does not change its results, only its speed, and which plan is
+// fastest depends on the instance's sizes (fixed), the call kind, the
+// number of operations and how they spread over dependency levels.  Calls
+// are keyed by that shape class (shapeKey), not by the exact operation
+// list, so a sampler proposing new topologies of one tree size tunes once.
+// The first call of a class runs every candidate twice and records events
+// around the second run; the timings are read on a later call of the class
+// once the events have completed (hipEventQuery, no wait: an ASYNCH caller
+// is never blocked), and until then the default plan runs.  The last
+// candidate's outputs are left in place (the operations are idempotent:
+// lists that add to a cumulative scale buffer are not tuned).  At most
+// kMaxTuned classes are kept; beyond that the default plan runs untuned.
 
 const Instance::PlanChoice Instance::kPlans[Instance::kNumPlans] = {
     {false, kNarrowLevel}, {true, 0}, {false, 4}, {false, 16}, {false, 1}};
 
-uint64_t Instance::hashOps(const int* ops, size_t n, uint64_t tag) {
-    uint64_t h = 1469598103934665603ull ^ tag;
-    for (size_t i = 0; i < n; ++i) {
-        h ^= (uint32_t)ops[i];
-        h *= 1099511628211ull;
-    }
-    return h ^ (n << 1);
+// Shape class of a call: kind (tag), operation count and the mean width of
+// its dependency levels in three classes (chains, below 16, 16 or more).
+uint64_t Instance::shapeKey(uint64_t tag, int count, int levels) {
+    const int width = count / std::max(levels, 1);
+    const uint64_t cls = width < 2 ? 0 : (width < 16 ? 1 : 2);
+    return (tag << 48) ^ ((uint64_t)(uint32_t)count << 4) ^ cls;
 }
 
 int Instance::forcedPlan() const {
@@ -983,39 +1005,79 @@ int Instance::forcedPlan() const {
     return -1;
 }
 
+// True once every candidate's timed run has completed; then picks the
+// fastest and frees the events.
+bool Instance::resolveTuning(Tuning& t) {
+    if (t.choice >= 0) return true;
+    for (auto& e : t.ev) {
+        const hipError_t q = hipEventQuery(e.second);
+        if (q == hipErrorNotReady) return false;
+        HIPCHECK(q);
+    }
+    int best = 0;
+    float bestMs = 0.f;
+    for (size_t k = 0; k < t.ev.size(); ++k) {
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, t.ev[k].first, t.ev[k].second));
+        if (k == 0 || ms < bestMs) {
+            best = (int)k;
+            bestMs = ms;
+        }
+    }
+    for (auto& e : t.ev) {
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    t.ev.clear();
+    t.choice = best;
+    return true;
+}
+
+void Instance::clearTuning() {
+    for (auto& kv : tuned_)
+        for (auto& e : kv.second.ev) {
+            hipEventDestroy(e.first);
+            hipEventDestroy(e.second);
+        }
+    tuned_.clear();
+}
+
 template <typename F>
 int Instance::tunedChoice(uint64_t key, int ncand, int forced, F&& run) {
     if (forced >= 0) return forced;
     const auto it = tuned_.find(key);
-    if (it != tuned_.end()) return it->second;
-    if (!autotune_ || capturing_) return 0;
-    hipEvent_t a, b;
-    HIPCHECK(hipEventCreate(&a));
-    HIPCHECK(hipEventCreate(&b));
-    int best = 0;
-    float bestMs = 0.f;
+    if (it != tuned_.end()) {
+        if (it->second.choice >= 0) return it->second.choice;
+        return !capturing_ && resolveTuning(it->second) ? it->second.choice : 0;
+    }
+    if (!autotune_ || capturing_ || tuned_.size() >= kMaxTuned) return 0;
+    Tuning t;
     for (int k = 0; k < ncand; ++k) {
+        hipEvent_t a, b;
+        HIPCHECK(hipEventCreate(&a));
+        HIPCHECK(hipEventCreate(&b));
+        t.ev.emplace_back(a, b);
         run(k);
         HIPCHECK(hipEventRecord(a, stream_));
         run(k);
         HIPCHECK(hipEventRecord(b, stream_));
-        HIPCHECK(hipEventSynchronize(b));
-        float ms = 0.f;
-        HIPCHECK(hipEventElapsedTime(&ms, a, b));
-        if (k == 0 || ms < bestMs) {
-            best = k;
-            bestMs = ms;
-        }
     }
-    HIPCHECK(hipEventDestroy(a));
-    HIPCHECK(hipEventDestroy(b));
-    tuned_[key] = best;
+    tuned_.emplace(key, std::move(t));
+    ++tunings_;
     return -1;
 }
 
 int Instance::setAutotune(int on) {
     autotune_ = on != 0;
-    if (!autotune_) tuned_.clear();
+    if (!autotune_) clearTuning();
+    return BEAGLE_SUCCESS;
+}
+
+int Instance::autotuneStats(int* outTuned, int* outPending) {
+    int pending = 0;
+    for (auto& kv : tuned_) pending += resolveTuning(kv.second) ? 0 : 1;
+    if (outTuned) *outTuned = tunings_;
+    if (outPending) *outPending = pending;
     return BEAGLE_SUCCESS;
 }
 
@@ -1082,7 +1144,7 @@ void Instance::launchPostMfma(const int* ops, int stride, int count) {
         run(mfmaVariant_);
     } else {
         // candidates: partials pipelined (0) or the whole tile up front (1)
-        const int k = tunedChoice(hashOps(ops, (size_t)count * stride, 3), 2,
+        const int k = tunedChoice(shapeKey(3, count, nl), 2,
                                   mfmaVariantForced_ ? mfmaVariant_ : -1, run);
         if (k >= 0) run(k);
     }
@@ -1342,8 +1404,8 @@ void Instance::launchPre(const int* ops, int stride, int count, double* partial)
         }
         launchPrePlan(plan, cw, partial);
     };
-    const int k = tunedChoice(hashOps(ops, (size_t)count * stride, partial ? 5 : 4), kNumPlans,
-                              forcedPlan(), run);
+    const int k = tunedChoice(shapeKey(partial ? 5 : 4, count, nl), kNumPlans, forcedPlan(),
+                              run);
     if (k >= 0) run(k);
 }
 
@@ -1652,8 +1714,7 @@ bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, dou
         }
         launchNodePlan(nodes, plan, partial);
     };
-    const int k = tunedChoice(hashOps(ops, (size_t)count * BEAGLE_OP_COUNT, 2), kNumPlans,
-                              forcedPlan(), run);
+    const int k = tunedChoice(shapeKey(2, n, nl), kNumPlans, forcedPlan(), run);
     if (k >= 0) run(k);
     return true;
 }
@@ -2113,35 +2174,44 @@ int Instance::calculateRootLogLikelihoodsByPartition(const int* bufIdx, const in
                                                      int count, double* outByPart,
                                                      double* outSum) {
     if (count != 1 || always_) return BEAGLE_ERROR_NO_IMPLEMENTATION;
-    if (partitionCount > 64) return BEAGLE_ERROR_NO_IMPLEMENTATION;
-    const int nbMax = (dims_.P + 31) / 32;
-    double* work = reinterpret_cast<double*>(grow(dWork_, workCap_, (size_t)nbMax * sizeof(double)));
+    if (partitionCount < 1) return BEAGLE_ERROR_OUT_OF_RANGE;
     for (int q = 0; q < partitionCount; ++q) {
         const int part = partIdx[q], b = bufIdx[q], w = wIdx[q], f = fIdx[q];
         const int cs = auto_ ? BEAGLE_OP_NONE : cumIdx[q];
         if (part < 0 || part >= partitionCount_ || !validBuffer(b) || !dPartials_[b] ||
-            !validEigen(w) || !validEigen(f) || (cs != BEAGLE_OP_NONE && !validScale(cs)))
+            !validEigen(w) || !validEigen(f) || hCatWeights_[w].empty() || hFreqs_[f].empty() ||
+            (cs != BEAGLE_OP_NONE && !validScale(cs)))
             return BEAGLE_ERROR_OUT_OF_RANGE;
+    }
+    // work: one result per entry, then the workgroup sums (reused per entry)
+    const size_t nbMax = (size_t)root_block_count(dims_, 0, dims_.P);
+    double* res = reinterpret_cast<double*>(
+        grow(dWork_, workCap_, ((size_t)partitionCount + nbMax) * sizeof(double)));
+    double* bs = res + partitionCount;
+    for (int q = 0; q < partitionCount; ++q) {
+        const int part = partIdx[q], b = bufIdx[q], w = wIdx[q], f = fIdx[q];
+        const int cs = auto_ ? BEAGLE_OP_NONE : cumIdx[q];
         const int p0 = partStart_[part], p1 = partStart_[part + 1];
         if (dp_)
-            launch_root_range<double>(dPartials_[b], auto_ ? dExp_[b] : nullptr, dCatWeights_ + (size_t)w * dims_.C,
+            launch_root_range<double>(dPartials_[b], auto_ ? dExp_[b] : nullptr,
+                                      dCatWeights_ + (size_t)w * dims_.C,
                                       dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
-                                      dPatternWeights_, dSiteL_, work, dResult_ + q, p0, p1, dims_,
+                                      dPatternWeights_, dSiteL_, bs, res + q, p0, p1, dims_,
                                       stream_);
         else
-            launch_root_range<float>(dPartials_[b], auto_ ? dExp_[b] : nullptr, dCatWeights_ + (size_t)w * dims_.C,
+            launch_root_range<float>(dPartials_[b], auto_ ? dExp_[b] : nullptr,
+                                     dCatWeights_ + (size_t)w * dims_.C,
                                      dFreqs_ + (size_t)f * dims_.S, cs >= 0 ? scale(cs) : nullptr,
-                                     dPatternWeights_, dSiteL_, work, dResult_ + q, p0, p1, dims_,
+                                     dPatternWeights_, dSiteL_, bs, res + q, p0, p1, dims_,
                                      stream_);
         HIPCHECK(hipGetLastError());
     }
-    HIPCHECK(hipMemcpyAsync(hResult_, dResult_, partitionCount * sizeof(double),
-                            hipMemcpyDeviceToHost, stream_));
+    const double* h = stageOut(res, partitionCount, 0);
     sync();
     double total = 0.0;
     for (int q = 0; q < partitionCount; ++q) {
-        outByPart[q] = hResult_[q];
-        total += hResult_[q];
+        outByPart[q] = h[q];
+        total += h[q];
     }
     *outSum = total;
     lastLogL_ = total;
@@ -2294,13 +2364,13 @@ int Instance::calculateEdgeLogLikelihoods(const int* parIdx, const int* childIdx
                                    D2, dCatWeights_ + (size_t)w * dims_.C,
                                    dFreqs_ + (size_t)f * dims_.S, cumA, cumB, exA, exB,
                                    dPatternWeights_, dSiteL_, dSiteD1_, dSiteD2_, wk, dResult_,
-                                   dims_, stream_);
+                                   0, dims_.P, dims_, stream_);
     else
         launch_edge_loglik<float>(dPartials_[par], dPartials_[ch], dStates_[ch], matrix(pm), D1,
                                   D2, dCatWeights_ + (size_t)w * dims_.C,
                                   dFreqs_ + (size_t)f * dims_.S, cumA, cumB, exA, exB,
                                   dPatternWeights_, dSiteL_, dSiteD1_, dSiteD2_, wk, dResult_,
-                                  dims_, stream_);
+                                  0, dims_.P, dims_, stream_);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(hResult_, dResult_, 3 * sizeof(double), hipMemcpyDeviceToHost, stream_));
     sync();
@@ -2312,6 +2382,83 @@ int Instance::calculateEdgeLogLikelihoods(const int* parIdx, const int* childIdx
     if (outD1 && D1) *outD1 = lastD1_;
     if (outD2 && D2) *outD2 = lastD2_;
     return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+}
+
+// Edge log-likelihoods (and derivatives) per pattern partition
+// (BeagleCPUImpl.hpp:3008-3110, 3452-3590): entry q is the edge
+// parIdx[q] - childIdx[q] over partition partIdx[q] with its own matrices,
+// weights, frequencies and cumulative buffer.  As the reference: one set
+// (count = 1), no automatic or always-on scaling, derivatives both or none.
+int Instance::calculateEdgeLogLikelihoodsByPartition(
+    const int* parIdx, const int* childIdx, const int* pIdx, const int* d1Idx, const int* d2Idx,
+    const int* wIdx, const int* fIdx, const int* cumIdx, const int* partIdx, int partitionCount,
+    int count, double* outLByPart, double* outL, double* outD1ByPart, double* outD1,
+    double* outD2ByPart, double* outD2) {
+    if (count != 1 || auto_ || always_) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    if ((d1Idx != nullptr) != (d2Idx != nullptr)) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    if (partitionCount < 1) return BEAGLE_ERROR_OUT_OF_RANGE;
+    const bool derivs = d1Idx != nullptr;
+    for (int q = 0; q < partitionCount; ++q) {
+        const int par = parIdx[q], ch = childIdx[q], part = partIdx[q];
+        if (part < 0 || part >= partitionCount_ || !validBuffer(par) || !dPartials_[par] ||
+            !validBuffer(ch) || (!dStates_[ch] && !dPartials_[ch]) || !validMatrix(pIdx[q]) ||
+            !validEigen(wIdx[q]) || !validEigen(fIdx[q]) || hCatWeights_[wIdx[q]].empty() ||
+            hFreqs_[fIdx[q]].empty() ||
+            (derivs && (!validMatrix(d1Idx[q]) || !validMatrix(d2Idx[q]))) ||
+            (cumIdx[q] != BEAGLE_OP_NONE && !validScale(cumIdx[q])))
+            return BEAGLE_ERROR_OUT_OF_RANGE;
+    }
+    // work: three results per entry, then the workgroup sums (reused)
+    const size_t nbMax = (size_t)(dims_.P + 255) / 256;
+    double* res = reinterpret_cast<double*>(
+        grow(dWork_, workCap_, (3 * (size_t)partitionCount + 3 * nbMax) * sizeof(double)));
+    double* bs = res + 3 * (size_t)partitionCount;
+    for (int q = 0; q < partitionCount; ++q) {
+        const int par = parIdx[q], ch = childIdx[q], part = partIdx[q];
+        const int p0 = partStart_[part], p1 = partStart_[part + 1];
+        const void* D1 = derivs ? matrix(d1Idx[q]) : nullptr;
+        const void* D2 = derivs ? matrix(d2Idx[q]) : nullptr;
+        const double* cum = cumIdx[q] != BEAGLE_OP_NONE ? scale(cumIdx[q]) : nullptr;
+        const double* cw = dCatWeights_ + (size_t)wIdx[q] * dims_.C;
+        const double* fr = dFreqs_ + (size_t)fIdx[q] * dims_.S;
+        if (dp_)
+            launch_edge_loglik<double>(dPartials_[par], dPartials_[ch], dStates_[ch],
+                                       matrix(pIdx[q]), D1, D2, cw, fr, cum, nullptr, nullptr,
+                                       nullptr, dPatternWeights_, dSiteL_, dSiteD1_, dSiteD2_, bs,
+                                       res + 3 * q, p0, p1, dims_, stream_);
+        else
+            launch_edge_loglik<float>(dPartials_[par], dPartials_[ch], dStates_[ch],
+                                      matrix(pIdx[q]), D1, D2, cw, fr, cum, nullptr, nullptr,
+                                      nullptr, dPatternWeights_, dSiteL_, dSiteD1_, dSiteD2_, bs,
+                                      res + 3 * q, p0, p1, dims_, stream_);
+        HIPCHECK(hipGetLastError());
+    }
+    const double* h = stageOut(res, 3 * (size_t)partitionCount, 0);
+    sync();
+    double tL = 0.0, t1 = 0.0, t2 = 0.0;
+    for (int q = 0; q < partitionCount; ++q) {
+        outLByPart[q] = h[3 * q];
+        tL += h[3 * q];
+        if (derivs) {
+            outD1ByPart[q] = h[3 * q + 1];
+            outD2ByPart[q] = h[3 * q + 2];
+            t1 += h[3 * q + 1];
+            t2 += h[3 * q + 2];
+        }
+    }
+    int rc = BEAGLE_SUCCESS;
+    if (derivs) {
+        *outD1 = t1;
+        *outD2 = t2;
+        if (t1 != t1 || t2 != t2) rc = BEAGLE_ERROR_FLOATING_POINT;
+    }
+    *outL = tL;
+    lastLogL_ = tL;
+    haveDerivs_ = derivs;
+    lastD1_ = t1;
+    lastD2_ = t2;
+    if (tL != tL) rc = BEAGLE_ERROR_FLOATING_POINT;
+    return rc;
 }
 
 int Instance::getLogLikelihood(double* out) {

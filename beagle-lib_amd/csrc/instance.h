@@ -142,6 +142,11 @@ public:
                                     const int* d1Idx, const int* d2Idx, const int* wIdx,
                                     const int* fIdx, const int* cumIdx, int count, double* outL,
                                     double* outD1, double* outD2);
+    int calculateEdgeLogLikelihoodsByPartition(
+        const int* parIdx, const int* childIdx, const int* pIdx, const int* d1Idx,
+        const int* d2Idx, const int* wIdx, const int* fIdx, const int* cumIdx,
+        const int* partIdx, int partitionCount, int count, double* outLByPart, double* outL,
+        double* outD1ByPart, double* outD1, double* outD2ByPart, double* outD2);
     int getLogLikelihood(double* out);
     int getDerivatives(double* d1, double* d2);
     int getSiteLogLikelihoods(double* out);
@@ -170,6 +175,7 @@ private:
         bool used = false;
     };
 
+    void release();
     void* upload(const void* data, size_t bytes);
     // Job lists that depend only on the operations (and the buffers they
     // name): an identical list reuses its device copy, no transfer.
@@ -235,8 +241,10 @@ private:
 public:
     // post-order matrix-core engine for S > 4 in FP32: 0 FP32, 3 or 4 split BF16
     int setSplitBf16(int terms);
-    // measure-and-keep choice of launch plans per operation list (on by default)
+    // measure-and-keep choice of launch plans per shape class (on by default)
     int setAutotune(int on);
+    // shape classes tuned so far, and those whose timings are still pending
+    int autotuneStats(int* outTuned, int* outPending);
 
 private:
     // dependency levels of a post-order list (reads children and a read
@@ -271,16 +279,26 @@ private:
     int bf16Terms_ = 3;       // products per split pair: 3, or 4 (adds lo x lo)
     int mfmaVariant_ = 0;
     bool mfmaVariantForced_ = false;
-    // autotuning: chosen plan candidate per operation list (hash)
+    // autotuning: chosen plan candidate per shape class (shapeKey); a
+    // class being tuned holds the events around each candidate's timed run
+    // until they have completed (read on a later call, never waited for)
     struct PlanChoice {
         bool fused;
         int narrow;
     };
     static constexpr int kNumPlans = 5;
     static const PlanChoice kPlans[kNumPlans];
+    static constexpr size_t kMaxTuned = 64;   // shape classes kept per instance
+    struct Tuning {
+        int choice = -1;                                   // -1: timings pending
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;  // per candidate
+    };
     bool autotune_ = true;
-    std::unordered_map<uint64_t, int> tuned_;
-    static uint64_t hashOps(const int* ops, size_t n, uint64_t tag);
+    std::unordered_map<uint64_t, Tuning> tuned_;
+    int tunings_ = 0;   // shape classes tuned so far
+    static uint64_t shapeKey(uint64_t tag, int count, int levels);
+    bool resolveTuning(Tuning& t);
+    void clearTuning();
     int forcedPlan() const;
     template <typename F>
     int tunedChoice(uint64_t key, int ncand, int forced, F&& run);

@@ -342,7 +342,7 @@ void launch_edge_loglik(const void* parent, const void* child, const int* childS
                         const double* cumA, const double* cumB, const short* exA,
                         const short* exB, const double* patternWeights, double* siteL,
                         double* siteD1, double* siteD2, double* blockSums, double* result,
-                        const Dims& d, hipStream_t st);
+                        int p0, int p1, const Dims& d, hipStream_t st);
 
 // Rows of S*S partial sums the cross-product launch writes into its workspace.
 int cross_products_rows(const Dims& d);

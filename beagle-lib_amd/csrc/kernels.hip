@@ -3051,12 +3051,13 @@ __global__ __launch_bounds__(WG) void k_edge_ll(const R* __restrict__ par, const
                                                 const short* exA, const short* exB,
                                                 const double* __restrict__ pw, double* siteL,
                                                 double* siteD1, double* siteD2,
-                                                double* __restrict__ bsum, Dims d) {
+                                                double* __restrict__ bsum, int p0, int p1,
+                                                Dims d) {
     __shared__ double sh[WG / 64];
-    const int p = blockIdx.x * WG + threadIdx.x;
+    const int p = p0 + blockIdx.x * WG + threadIdx.x;
     const int S = d.S, W = S + 1;
     double cl = 0.0, c1 = 0.0, c2 = 0.0;
-    if (p < d.P) {
+    if (p < p1) {
         double acc[3] = {0.0, 0.0, 0.0};
         const R* Ms[3] = {Pm, D1, D2};
         for (int m = 0; m < 3; ++m) {
@@ -3116,13 +3117,17 @@ void launch_edge_loglik(const void* parent, const void* child, const int* childS
                         const double* cumA, const double* cumB, const short* exA,
                         const short* exB, const double* patternWeights, double* siteL,
                         double* siteD1, double* siteD2, double* blockSums, double* result,
-                        const Dims& d, hipStream_t st) {
-    const int nb = (d.P + WG - 1) / WG;
+                        int p0, int p1, const Dims& d, hipStream_t st) {
+    if (p1 <= p0) {
+        launch_fill(result, 0.0, 3, st);
+        return;
+    }
+    const int nb = (p1 - p0 + WG - 1) / WG;
     hipLaunchKernelGGL(k_edge_ll<R>, dim3(nb), dim3(WG), 0, st, reinterpret_cast<const R*>(parent),
                        reinterpret_cast<const R*>(child), childStates,
                        reinterpret_cast<const R*>(P), reinterpret_cast<const R*>(D1),
                        reinterpret_cast<const R*>(D2), catWeights, freqs, cumA, cumB, exA, exB,
-                       patternWeights, siteL, siteD1, siteD2, blockSums, d);
+                       patternWeights, siteL, siteD1, siteD2, blockSums, p0, p1, d);
     hipLaunchKernelGGL(k_reduce_rows, dim3(3), dim3(WG), 0, st, blockSums, nb, 3, result, 0);
 }
 
@@ -3278,7 +3283,7 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
                                         const void*, const void*, const double*, const double*,  \
                                         const double*, const double*, const short*, const short*, \
                                         const double*, double*, double*, double*, double*,       \
-                                        double*, const Dims&, hipStream_t);                      \
+                                        double*, int, int, const Dims&, hipStream_t);            \
     template void launch_transpose<R>(const void* const*, void* const*, int, int, int,           \
                                       hipStream_t);                                              \
     template void launch_pad_set<R>(void*, long, R, hipStream_t);                               \

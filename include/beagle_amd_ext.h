@@ -28,15 +28,27 @@ int beagleExtGetStream(int instance, void** outStream);
 int beagleExtSetSplitBF16(int instance, int terms);
 
 /* Autotuning of launch plans (no counterpart in the reference; on by
- * default, BEAGLE_AMD_AUTOTUNE=0 in the environment turns it off).  The
- * first traversal call with a given operation list times each candidate
- * plan (4 states: one fused launch over the list, or dependency levels
- * batched above a width of 4, 8, 16 or 1 operations; S > 4 in FP32: partial
- * loads pipelined or a whole tile issued at once) and keeps the fastest for
- * that list; plans give identical results.  Lists that add to a cumulative
- * scale buffer, and calls being captured into a graph, use the default
- * plan.  on = 0 turns it off and forgets the choices. */
+ * default, BEAGLE_AMD_AUTOTUNE=0 in the environment turns it off).  Calls
+ * are grouped into shape classes: the call kind (post-order, pre-order,
+ * fused pre-order + derivatives, matrix-core post-order), the number of
+ * operations and the mean width of their dependency levels (below 2, below
+ * 16, 16 or more); the instance's sizes are fixed.  The first call of a
+ * class times each candidate plan (4 states: one fused launch over the
+ * list, or dependency levels batched above a width of 4, 8, 16 or 1
+ * operations; S > 4 in FP32: partial loads pipelined or a whole tile issued
+ * at once) with events on the instance's stream, without waiting; a later
+ * call of the class reads the timings once they are complete and keeps the
+ * fastest plan, and runs the default plan until then.  New topologies of
+ * one tree size therefore tune once.  Plans give identical results.  Lists
+ * that add to a cumulative scale buffer, calls being captured into a graph
+ * and classes beyond the 64th use the default plan.  on = 0 turns it off
+ * and forgets the choices. */
 int beagleExtSetAutotune(int instance, int on);
+
+/* The number of shape classes the instance has tuned so far and of those
+ * whose timings have not completed yet (no counterpart in the reference;
+ * either pointer may be NULL). */
+int beagleExtGetAutotuneStats(int instance, int* outTuned, int* outPending);
 
 /* Log-likelihood and per-branch derivative sums of one tree under a
  * partials budget (no counterpart in the reference; the reference keeps

@@ -642,6 +642,10 @@ static int up_partials(Orc* o, const int* ops, int count, int cumIdx, int stride
         const double* x1 = o->partials[c1];
         const double* x2 = o->partials[c2];
         if ((!s1 && !x1) || (!s2 && !x2)) return ORC_ERROR_OUT_OF_RANGE;
+        /* scale indices in range (the reference does not check them) */
+        if (wS >= o->scaleCount || rS >= o->scaleCount || cum >= o->scaleCount ||
+            ((o->flags & F_SCALING_ALWAYS) && dest - o->tipCount >= o->scaleCount))
+            return ORC_ERROR_OUT_OF_RANGE;
         double* d = o->partials[dest];
         int mode = OP_NONE;
         double* sf = NULL;
@@ -722,8 +726,13 @@ static int up_pre_partials(Orc* o, const int* ops, int count, int stride) {
             mode = 1;
             sf = o->scale[dest - o->tipCount];
         } else if (wS >= 0) {
+            if (wS >= o->scaleCount) { free(tmp); return ORC_ERROR_OUT_OF_RANGE; }
             mode = 1;
             sf = o->scale[wS];
+        }
+        if ((o->flags & F_SCALING_ALWAYS) && dest - o->tipCount >= o->scaleCount) {
+            free(tmp);
+            return ORC_ERROR_OUT_OF_RANGE;
         }
         for (int l = 0; l < o->C; l++) {
             const double* Ms = o->matrices[mSelf] + (size_t)l * S * (S + 1);
@@ -765,6 +774,8 @@ int orcWaitForPartials(int instance, const int* d, int n) {
  * (BeagleCPUImpl.hpp:2763-2860). */
 static int accumulate(Orc* o, const int* idx, int count, int cumIdx, int sign, int k0, int k1) {
     if (cumIdx < 0 || cumIdx >= o->scaleCount) return ORC_ERROR_OUT_OF_RANGE;
+    for (int i = 0; i < count; i++)
+        if (idx[i] < 0 || idx[i] >= o->scaleCount) return ORC_ERROR_OUT_OF_RANGE;
     double* cum = o->scale[cumIdx];
     for (int i = 0; i < count; i++) {
         if (idx[i] < 0 || idx[i] >= o->scaleCount) return ORC_ERROR_OUT_OF_RANGE;
@@ -1029,6 +1040,51 @@ int orcCalculateCrossProductDerivative(int instance, const int* postIdx, const i
     return ORC_SUCCESS;
 }
 
+/* Site values and pattern-weighted sums [logL, d1, d2] of the edge
+ * parent - child over patterns [k0, k1) (the per-pattern body of
+ * BeagleCPUImpl.hpp:3209-3322 and :3452-3590); Ms = {P, D1 or NULL, D2 or
+ * NULL}, cum a cumulative scale buffer or NULL, alw the two node scale
+ * buffers under SCALING_ALWAYS. */
+static void edge_range(Orc* o, const double* par, const double* ch, const int* st,
+                       const double* const Ms[3], const double* w, const double* f,
+                       const double* cum, const double* const alw[2], int k0, int k1,
+                       double sums[3]) {
+    const int S = o->S, P = o->P;
+    double sL = 0, s1 = 0, s2 = 0;
+    for (int k = k0; k < k1; k++) {
+        double acc[3] = {0, 0, 0};
+        for (int m = 0; m < 3; m++) {
+            if (!Ms[m]) continue;
+            for (int l = 0; l < o->C; l++) {
+                const double* pv = par + ((size_t)l * P + k) * S;
+                double s = 0;
+                for (int i = 0; i < S; i++) {
+                    double t = child_term(o, Ms[m], ch, st, l, k, i);
+                    s += f[i] * pv[i] * t;
+                }
+                acc[m] += w[l] * s;
+            }
+        }
+        o->siteLogL[k] = log(acc[0]) + (cum ? cum[k] : 0.0);
+        if (alw[0]) o->siteLogL[k] += alw[0][k];
+        if (alw[1]) o->siteLogL[k] += alw[1][k];
+        sL += o->siteLogL[k] * o->patternWeights[k];
+        if (Ms[1]) {
+            double d1 = acc[1] / acc[0];
+            o->siteD1[k] = d1;
+            s1 += d1 * o->patternWeights[k];
+            if (Ms[2]) {
+                double d2 = acc[2] / acc[0] - d1 * d1;
+                o->siteD2[k] = d2;
+                s2 += d2 * o->patternWeights[k];
+            }
+        }
+    }
+    sums[0] = sL;
+    sums[1] = s1;
+    sums[2] = s2;
+}
+
 /* Edge log-likelihood between a parent (post-order above the edge) and a
  * child, with optional first and second derivatives along the edge
  * (BeagleCPUImpl.hpp:2918-3005, 3209-3322):
@@ -1042,7 +1098,7 @@ int orcCalculateEdgeLogLikelihoods(int instance, const int* parIdx, const int* c
     Orc* o = get(instance);
     if (!o) return ORC_ERROR_UNINITIALIZED_INSTANCE;
     if (count != 1) return ORC_ERROR_NO_IMPLEMENTATION;
-    int S = o->S, P = o->P;
+    int P = o->P;
     const double* par = o->partials[parIdx[0]];
     const int* st = o->tipStates[childIdx[0]];
     const double* ch = o->partials[childIdx[0]];
@@ -1059,40 +1115,66 @@ int orcCalculateEdgeLogLikelihoods(int instance, const int* parIdx, const int* c
         if (parIdx[0] - o->tipCount >= 0) alw[0] = o->scale[parIdx[0] - o->tipCount];
         if (childIdx[0] - o->tipCount >= 0) alw[1] = o->scale[childIdx[0] - o->tipCount];
     }
-    double sL = 0, s1 = 0, s2 = 0;
-    for (int k = 0; k < P; k++) {
-        double acc[3] = {0, 0, 0};
-        for (int m = 0; m < 3; m++) {
-            if (!Ms[m]) continue;
-            for (int l = 0; l < o->C; l++) {
-                const double* pv = par + ((size_t)l * P + k) * S;
-                double s = 0;
-                for (int i = 0; i < S; i++) {
-                    double t = child_term(o, Ms[m], ch, st, l, k, i);
-                    s += f[i] * pv[i] * t;
-                }
-                acc[m] += w[l] * s;
-            }
-        }
-        o->siteLogL[k] = log(acc[0]) + (cs >= 0 ? o->scale[cs][k] : 0.0);
-        if (alw[0]) o->siteLogL[k] += alw[0][k];
-        if (alw[1]) o->siteLogL[k] += alw[1][k];
-        sL += o->siteLogL[k] * o->patternWeights[k];
-        if (Ms[1]) {
-            double d1 = acc[1] / acc[0];
-            o->siteD1[k] = d1;
-            s1 += d1 * o->patternWeights[k];
-            if (Ms[2]) {
-                double d2 = acc[2] / acc[0] - d1 * d1;
-                o->siteD2[k] = d2;
-                s2 += d2 * o->patternWeights[k];
-            }
+    double sums[3];
+    edge_range(o, par, ch, st, Ms, w, f, cs >= 0 ? o->scale[cs] : NULL, alw, 0, P, sums);
+    *outL = sums[0];
+    if (outD1 && Ms[1]) *outD1 = sums[1];
+    if (outD2 && Ms[2]) *outD2 = sums[2];
+    return sums[0] != sums[0] ? ORC_ERROR_FLOATING_POINT : ORC_SUCCESS;
+}
+
+/* Edge log-likelihoods per pattern partition
+ * (BeagleCPUImpl.hpp:3008-3110, calcEdgeLogLikelihoodsSecondDerivByPartition
+ * :3452-3590): for partition entry q the edge parentBufferIndices[q] -
+ * childBufferIndices[q] over the patterns of partitionIndices[q], with its
+ * own matrices, weights, frequencies and cumulative scale buffer; sums per
+ * entry and over all entries.  As the reference: count must be 1, automatic
+ * and always-on scaling are not implemented, and derivatives come both or
+ * not at all (a first derivative alone is not implemented). */
+int orcCalculateEdgeLogLikelihoodsByPartition(
+    int instance, const int* parIdx, const int* childIdx, const int* pIdx, const int* d1Idx,
+    const int* d2Idx, const int* wIdx, const int* fIdx, const int* cumIdx, const int* partIdx,
+    int partitionCount, int count, double* outLByPart, double* outL, double* outD1ByPart,
+    double* outD1, double* outD2ByPart, double* outD2) {
+    Orc* o = get(instance);
+    if (!o) return ORC_ERROR_UNINITIALIZED_INSTANCE;
+    if (count != 1 || (o->flags & (F_SCALING_AUTO | F_SCALING_ALWAYS)))
+        return ORC_ERROR_NO_IMPLEMENTATION;
+    const int derivs = d1Idx != NULL;
+    if ((d1Idx != NULL) != (d2Idx != NULL)) return ORC_ERROR_NO_IMPLEMENTATION;
+    double tL = 0, t1 = 0, t2 = 0;
+    for (int q = 0; q < partitionCount; q++) {
+        const int part = partIdx[q];
+        if (part < 0 || part >= o->partitionCount) return ORC_ERROR_OUT_OF_RANGE;
+        const double* par = o->partials[parIdx[q]];
+        const int* st = o->tipStates[childIdx[q]];
+        const double* ch = o->partials[childIdx[q]];
+        if (!par || (!st && !ch)) return ORC_ERROR_OUT_OF_RANGE;
+        const double* Ms[3] = {o->matrices[pIdx[q]], derivs ? o->matrices[d1Idx[q]] : NULL,
+                               derivs ? o->matrices[d2Idx[q]] : NULL};
+        const double* alw[2] = {NULL, NULL};
+        double sums[3];
+        edge_range(o, par, ch, st, Ms, o->catWeights[wIdx[q]], o->freqs[fIdx[q]],
+                   cumIdx[q] >= 0 ? o->scale[cumIdx[q]] : NULL, alw, o->partStart[part],
+                   o->partStart[part + 1], sums);
+        outLByPart[q] = sums[0];
+        tL += sums[0];
+        if (derivs) {
+            outD1ByPart[q] = sums[1];
+            outD2ByPart[q] = sums[2];
+            t1 += sums[1];
+            t2 += sums[2];
         }
     }
-    *outL = sL;
-    if (outD1 && Ms[1]) *outD1 = s1;
-    if (outD2 && Ms[2]) *outD2 = s2;
-    return sL != sL ? ORC_ERROR_FLOATING_POINT : ORC_SUCCESS;
+    int rc = ORC_SUCCESS;
+    if (derivs) {
+        *outD1 = t1;
+        *outD2 = t2;
+        if (t1 != t1 || t2 != t2) rc = ORC_ERROR_FLOATING_POINT;
+    }
+    *outL = tL;
+    if (tL != tL) rc = ORC_ERROR_FLOATING_POINT;
+    return rc;
 }
 
 int orcGetSiteDerivatives(int instance, double* d1, double* d2) {

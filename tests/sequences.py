@@ -112,3 +112,43 @@ def transition_derivatives(lib, prob, lengths, precision="double"):
     finally:
         ev.close()
     return res
+
+
+def edge_by_partition(lib, prob, precision="double", matrices="eigen"):
+    """beagleCalculateEdgeLogLikelihoodsByPartition
+    (BeagleCPUImpl.hpp:3008-3110, 3452-3590) over three uneven contiguous
+    partitions, after a post-order pass with manual scaling: the root's
+    post-order partials as the parent and, per partition, one of the
+    root's children (its own branch matrix, the derivative matrices r_c Q
+    and r_c^2 Q^2) as the child, with the cumulative scale buffer; with and
+    without derivatives; the site log-likelihoods after it; and the same
+    edge over all patterns through beagleCalculateEdgeLogLikelihoods."""
+    ev = driver.Evaluator(lib, prob, scaling="manual", transpose="none", precision=precision,
+                          matrices=matrices, preorder=False)
+    inst, tree, n = ev.inst, prob.tree, prob.tree.n
+    out = {}
+    try:
+        ev.log_likelihood()
+        cum = n - 1
+        root = tree.root
+        a, b = tree.children[root]
+        parts = np.zeros(prob.P, dtype=np.int32)
+        parts[prob.P // 5:] = 1
+        parts[prob.P // 2:] = 2
+        inst.set_pattern_partitions(3, parts)
+        ch = [a, b, a]
+        d1 = [ev.d1_index] * 3
+        d2 = [ev.d2_index] * 3
+        res = inst.edge_log_likelihood_by_partition([root] * 3, ch, ch, d1, d2, [0] * 3, [0] * 3,
+                                                    [cum] * 3, [0, 1, 2])
+        out["with_derivatives"] = np.concatenate([res[0], [res[1]], res[2], [res[3]], res[4],
+                                                  [res[5]]])
+        out["sites_after"] = inst.site_log_likelihoods()
+        res = inst.edge_log_likelihood_by_partition([root] * 3, [a] * 3, [a] * 3, None, None,
+                                                    [0] * 3, [0] * 3, [cum] * 3, [2, 0, 1])
+        out["without_derivatives"] = np.concatenate([res[0], [res[1]]])
+        whole = inst.edge_log_likelihood(root, a, a, ev.d1_index, ev.d2_index, cumulative=cum)
+        out["whole_edge"] = np.array(whole)
+    finally:
+        ev.close()
+    return out

@@ -1363,3 +1363,89 @@ def test_adjoint_full_size_512_taxa(amd):
         assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
     assert np.abs(q32 - q64).max() <= 1e-6 * np.abs(q64).max()
     assert np.abs(pi32 - pi64).max() <= 1e-6 * np.abs(pi64).max()
+
+
+@pytest.mark.parametrize("S,precision", [(4, "single"), (20, "single")])
+def test_autotuner_tunes_once_per_tree_size(amd, S, precision):
+    """A sampler proposing topologies: 20 random Yule trees of one size over
+    the same tips, each a new operation list, evaluated (logL + branch
+    gradient) on one instance with autotuning and one without.  Each call
+    kind is tuned at most once (shape classes, not operation lists), no
+    timing is left pending after the calls, and every result equals the
+    untuned instance's bit for bit."""
+    rng = np.random.default_rng(8 + S)
+    n = 64
+    prob = synth.make_problem(88 + S, n, 3000 if S == 4 else 300, S=S, shape="yule")
+    trees = [synth.make_tree(rng, n, "yule", 0.1) for _ in range(20)]
+    on = driver.Evaluator(amd, prob, precision=precision, scaling="auto")
+    off = driver.Evaluator(amd, prob, precision=precision, scaling="auto")
+    off.inst.set_autotune(False)
+    try:
+        for t in trees:
+            on.set_tree(t)
+            off.set_tree(t)
+            a = on.fused_gradient()
+            b = off.fused_gradient()
+            assert a[0] == b[0]
+            np.testing.assert_array_equal(a[1], b[1])
+            np.testing.assert_array_equal(a[2], b[2])
+        tuned, pending = on.inst.autotune_stats()
+        # 4 states: post-order and node-fused pre-order; S > 4: the FP32
+        # matrix-core post-order (its pre-order has a single plan)
+        assert 1 <= tuned <= (2 if S == 4 else 1), tuned
+        assert pending == 0
+        assert off.inst.autotune_stats() == (0, 0)
+    finally:
+        on.close()
+        off.close()
+
+
+@pytest.mark.parametrize("S,precision", [(4, "double"), (4, "single"), (20, "double"),
+                                         (20, "single")])
+def test_edge_log_likelihood_by_partition_matches_oracle(amd, oracle, S, precision):
+    """beagleCalculateEdgeLogLikelihoodsByPartition against the oracle
+    (which the CPU tests pin against the reference CPU library) on identical
+    explicit matrices: per-partition and total log-likelihoods, first and
+    second derivatives, site log-likelihoods, and the whole-pattern edge
+    call; FP64 1e-11, FP32 1e-6 of each array's largest magnitude."""
+    prob = synth.make_problem(181 + S, 9, 700 if S == 4 else 150, S=S, missing=0.05,
+                              mean_length=0.3)
+    got = sequences.edge_by_partition(amd, prob, precision, matrices="explicit")
+    want = sequences.edge_by_partition(oracle, prob, "double", matrices="explicit")
+    tol = 1e-11 if precision == "double" else 1e-6
+    for k in want:
+        a, b = np.asarray(got[k]), np.asarray(want[k])
+        err = np.abs(a - b).max() / np.abs(b).max()
+        assert err < tol, f"{k}: {err:.2e}"
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_root_by_partition_beyond_64_partitions(amd, oracle, precision):
+    """Root log-likelihoods by partition for 100 partitions of uneven sizes
+    (one with no pattern): each against the oracle (FP64 1e-11, FP32 1e-6
+    relative), and their sum against the whole-pattern root call."""
+    P = 3000
+    prob = synth.make_problem(321, 12, P, S=4, missing=0.05)
+    rng = np.random.default_rng(5)
+    cuts = np.sort(rng.choice(np.arange(1, P), 98, replace=False))
+    parts = np.zeros(P, dtype=np.int32)
+    for c in cuts:
+        parts[c:] += 1
+    parts[parts >= 50] += 1          # partition 50 holds no pattern
+    res = []
+    for lib, prec in ((amd, precision), (oracle, "double")):
+        ev = driver.Evaluator(lib, prob, scaling="manual", precision=prec, matrices="explicit",
+                              preorder=False)
+        whole = ev.log_likelihood()
+        ev.inst.set_pattern_partitions(100, parts)
+        cum = ev.n - 1
+        by, tot = ev.inst.root_log_likelihood_by_partition([prob.tree.root] * 100, [0] * 100,
+                                                           [0] * 100, [cum] * 100,
+                                                           list(range(100)))
+        res.append((whole, by, tot))
+        ev.close()
+    (wg, bg, tg), (wo, bo, to) = res
+    tol = 1e-11 if precision == "double" else 1e-6
+    assert bg[50] == 0.0
+    assert np.abs(bg - bo).max() <= tol * np.abs(bo).max()
+    assert rel(tg, to) < tol and rel(tg, wg) < tol

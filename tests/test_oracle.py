@@ -534,3 +534,20 @@ def test_nonreversible_generator(S):
     np.testing.assert_allclose(m.evec @ B @ m.ivec, Q, rtol=0, atol=1e-12)
     np.testing.assert_allclose(m.evec @ expm(0.7 * B) @ m.ivec, expm(0.7 * Q), rtol=0,
                                atol=1e-12)
+
+
+@pytest.mark.parametrize("S,tips_state_child", [(4, True), (20, False)])
+def test_edge_log_likelihood_by_partition_against_reference(oracle, ref, S, tips_state_child):
+    """Edge log-likelihoods by partition on the oracle and the reference CPU
+    library (values per partition and totals, with and without
+    derivatives, site log-likelihoods), and the partitions' sums of one edge
+    equal the whole-pattern edge call."""
+    n = 3 if tips_state_child else 9      # 3 taxa: a root child is a tip
+    prob = synth.make_problem(81 + S, n, 150 if S == 4 else 70, S=S, missing=0.05,
+                              mean_length=0.3)
+    a = sequences.edge_by_partition(oracle, prob)
+    b = sequences.edge_by_partition(ref, prob)
+    for k in a:
+        np.testing.assert_allclose(a[k], b[k], rtol=1e-12, atol=1e-13 * np.abs(b[k]).max(),
+                                   err_msg=k)
+    np.testing.assert_allclose(a["without_derivatives"][-1], a["whole_edge"][0], rtol=1e-12)
