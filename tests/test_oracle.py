@@ -44,34 +44,8 @@ def rel(a, b):
     return abs(a - b) / abs(b)
 
 
-def seq_states(seq):
-    return np.array(["ACGT".index(ch) if ch in "ACGT" else 4 for ch in seq], dtype=np.int32)
-
-
-def small_problem(spec):
-    """A Problem from a known-answer entry: three tips, ((0,1)3,2)4."""
-    seqs = spec["sequences"]
-    n = len(seqs)
-    children = {n + i: tuple(c) for i, c in enumerate(spec["tree"]["children"])}
-    parent = np.full(2 * n - 1, -1)
-    for v, (a, b) in children.items():
-        parent[a] = v
-        parent[b] = v
-    lengths = np.zeros(2 * n - 1)
-    lengths[:2 * n - 2] = spec["tree"]["lengths"]
-    tree = synth.Tree(n=n, children=children, parent=parent, lengths=lengths)
-    if spec["model"] == "JC69":
-        model = synth.jc69()
-    else:  # an explicit eigen-system, Q = evec diag(evals) ivec
-        e = spec["eigen"]
-        V, Vi, lam = np.array(e["evec"]), np.array(e["ivec"]), np.array(e["evals"])
-        model = synth.Model(Q=V @ np.diag(lam) @ Vi, freqs=np.array(spec["freqs"]), evec=V,
-                            ivec=Vi, evals=lam)
-    rates = np.array(spec["rates"])
-    return synth.Problem(tree=tree, model=model, rates=rates,
-                         weights=np.full(len(rates), 1.0 / len(rates)),
-                         tips=np.stack([seq_states(s) for s in seqs]),
-                         pattern_weights=np.ones(len(seqs[0])))
+seq_states = sequences.seq_states
+small_problem = sequences.small_problem
 
 
 # ---- known answers -------------------------------------------------------------
