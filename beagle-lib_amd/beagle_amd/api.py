@@ -578,13 +578,18 @@ class Instance:
         return float(lnl[0]), s1, sq, s2, k.value, peak.value
 
     def rate_matrix_gradient(self, eigen, ops, post, weights_index=0, root=None,
-                             freqs_index=0):
-        """Extension: (dlogL/dQ as an S x S array, dlogL/dpi_root or None)."""
+                             freqs_index=0, out=None):
+        """Extension: (dlogL/dQ as an S x S array, dlogL/dpi_root or None).
+        out: caller-owned float64 arrays (S*S, S or None) that stay alive
+        while a graph that captured the call replays."""
         a = _ip(np.asarray(ops).reshape(-1))
         n = len(a[0]) // 7
         b = _ip(post)
-        gq = np.zeros(self.S * self.S)
-        gf = np.zeros(self.S) if root is not None else None
+        if out is not None:
+            gq, gf = out
+        else:
+            gq = np.zeros(self.S * self.S)
+            gf = np.zeros(self.S) if root is not None else None
         self._c("ExtCalculateRateMatrixGradient", eigen, a[1], n, b[1], weights_index,
                 -1 if root is None else root, freqs_index, gq.ctypes.data_as(c_double_p),
                 gf.ctypes.data_as(c_double_p) if gf is not None else None)

@@ -218,11 +218,12 @@ class Evaluator:
                          np.full(n, self._deriv_index(True), np.int32))
         return self._pre[1:]
 
-    def fused_gradient(self, lengths=None, keep_tips=False):
+    def fused_gradient(self, lengths=None, keep_tips=False, rate_matrix=False):
         """logL, dlogL/dt and d2logL/dt2 per branch through the fused
         pre-order + derivative extensions (this package's library only):
         matrices and the post-order traversal, then one call for the root
-        log-likelihood, root pre-order partials and the derivative pass."""
+        log-likelihood, root pre-order partials and the derivative pass;
+        with rate_matrix also dlogL/dQ and dlogL/dpi_root."""
         ops, post, d1, d2 = self.fused_operations(keep_tips)
         self.update_matrices(lengths)
         self.post_order()
@@ -236,6 +237,9 @@ class Evaluator:
         g1[post] = s1
         g2[post] = s2 - sq
         nodes = self.branch_nodes()
+        if rate_matrix:
+            q, pi = self.rate_matrix_gradient()
+            return logl, g1[nodes], g2[nodes], q, pi
         return logl, g1[nodes], g2[nodes]
 
     def checkpointed_gradient(self, buffers, interval=0, lengths=None, plan_only=False):
@@ -261,18 +265,23 @@ class Evaluator:
         nodes = self.branch_nodes()
         return lnl, s1[nodes], (s2 - sq)[nodes], k, peak
 
-    def capture_fused_gradient(self, keep_tips=False):
+    def capture_fused_gradient(self, keep_tips=False, rate_matrix=False):
         """Record one fused_gradient() evaluation (matrices from the
-        eigen-system, post-order, root logL, fused derivative pass) as a HIP
-        graph of this package's library; replay_fused_gradient(lengths)
-        reruns it with new branch lengths.  The calls run once first so
-        that the library's work buffers have their size."""
+        eigen-system, post-order, root logL, fused derivative pass and, with
+        rate_matrix, dlogL/dQ and dlogL/dpi_root) as a HIP graph of this
+        package's library; replay_fused_gradient(lengths) reruns it with new
+        branch lengths.  The calls run once first so that the library's work
+        buffers have their size."""
         if self.matrices != "eigen":
             raise ValueError("graphs record matrices computed from the eigen-system")
         self.fused_gradient(keep_tips=keep_tips)
         ops, post, d1, d2 = self.fused_operations(keep_tips)
+        S = self.prob.S
+        if rate_matrix:
+            self.rate_matrix_gradient()
         n = len(post)
         self._gout = (np.zeros(1), np.zeros(n), np.zeros(n), np.zeros(n))
+        self._qout = (np.zeros(S * S), np.zeros(S)) if rate_matrix else None
         self._gpost = post
         manual = self.scaling in ("manual", "manual_log")
         cum = self.n - 1 if manual else api.OP_NONE
@@ -283,6 +292,9 @@ class Evaluator:
             self.inst.log_likelihood_and_derivatives(
                 self.prob.tree.root, self.pre_base + self.prob.tree.root, ops, post, d1, d2,
                 cumulative=cum, out=self._gout)
+            if rate_matrix:
+                self.inst.rate_matrix_gradient(0, ops, post, root=self.prob.tree.root,
+                                               out=self._qout)
         finally:
             self._graph = self.inst.end_capture()
         return self._graph
@@ -297,6 +309,8 @@ class Evaluator:
         return self.collect_fused_gradient() if wait else None
 
     def collect_fused_gradient(self):
+        """(logL, dlogL/dt, d2logL/dt2) of the last replay, and with a
+        captured rate-matrix gradient also (dlogL/dQ, dlogL/dpi_root)."""
         self.inst.wait_graph(self._graph)
         lnl, s1, sq, s2 = self._gout
         g1 = np.zeros(2 * self.n - 1)
@@ -304,7 +318,11 @@ class Evaluator:
         g1[self._gpost] = s1
         g2[self._gpost] = s2 - sq
         nodes = self.branch_nodes()
-        return float(lnl[0]), g1[nodes], g2[nodes]
+        res = (float(lnl[0]), g1[nodes], g2[nodes])
+        if self._qout is not None:
+            S = self.prob.S
+            res += (self._qout[0].reshape(S, S).copy(), self._qout[1].copy())
+        return res
 
     def rate_matrix_gradient(self, root_frequencies=True):
         """dlogL/dQ (S x S) for eigen-system 0 and, optionally, dlogL/dpi

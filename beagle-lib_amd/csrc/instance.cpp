@@ -196,6 +196,13 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
         dSiteD1_ = dalloc<double>(dims_.Ppad);
         dSiteD2_ = dalloc<double>(dims_.Ppad);
         dResult_ = dalloc<double>(64);
+        // 4-state real eigen-systems: every matrix computed from one also
+        // gets the Frechet factors of its exponential (rate-matrix gradient)
+        if (stateCount == 4 && !complex_ && categoryCount <= 8) {
+            const long nF = (long)matrixCount_ * categoryCount * 16;
+            dF_ = dalloc<double>(nF);
+            launch_fill(dF_, 0.0, nF, stream_);
+        }
         // one-tile stand-ins read by unused operand slots of the pipelined
         // traversal (in bounds for every lane and category, never stored to)
         dDummy_ = dalloc<char>(dummy_bytes());
@@ -262,6 +269,7 @@ void Instance::release() {
     hipFree(dSiteD1_);
     hipFree(dSiteD2_);
     hipFree(dResult_);
+    hipFree(dF_);
     hipFree(dDummy_);
     hipHostFree(hResult_);
     hipFree(dWork_);
@@ -700,6 +708,7 @@ int Instance::updateTransitionMatrices(int e, const int* probIdx, const int* d1I
         J.rates = dCatRates_;
         J.t = t[u];
         J.cat = -1;
+        J.F = dF_ ? dF_ + (size_t)probIdx[u] * dims_.C * 16 : nullptr;
     }
     for (int u = 0; u < count; ++u) {
         matOrigin_[probIdx[u]] = MatrixOrigin{e, 0, t[u]};
@@ -735,6 +744,7 @@ int Instance::updateTransitionMatricesWithModelCategories(const int* eig, const 
             J.rates = nullptr;
             J.t = t[u];
             J.cat = c;
+            J.F = nullptr;
             jobs.push_back(J);
         }
         matOrigin_[probIdx[u]] = MatrixOrigin{};   // several eigen-systems
@@ -766,6 +776,7 @@ int Instance::updateTransitionMatricesWithMultipleModels(const int* eig, const i
         J.rates = dCatRates_ + (size_t)r * dims_.C;
         J.t = t[u];
         J.cat = -1;
+        J.F = dF_ ? dF_ + (size_t)probIdx[u] * dims_.C * 16 : nullptr;
     }
     for (int u = 0; u < count; ++u) {
         matOrigin_[probIdx[u]] = MatrixOrigin{eig[u], rateIdx[u], t[u]};
@@ -1788,7 +1799,7 @@ int Instance::rateMatrixGradient(int e, const int* ops, int count, const int* po
     if (S != 4 || C > 8)
         return rateMatrixGradientGeneric(e, ops, count, postIdx, wIdx, rootIdx, fIdx, outQ,
                                          outFreqs);
-    if (partialsBytes_ >= (size_t(1) << 32)) return BEAGLE_ERROR_NO_IMPLEMENTATION;
+    if (partialsBytes_ >= (size_t(1) << 32) || !dF_) return BEAGLE_ERROR_NO_IMPLEMENTATION;
     int rc = buildPreJobs(ops, count, false, true);
     if (rc != BEAGLE_SUCCESS) return rc;
     rc = attachOwnOperands(count, postIdx, nullptr, nullptr);
@@ -1797,60 +1808,39 @@ int Instance::rateMatrixGradient(int e, const int* ops, int count, const int* po
     std::vector<Access> acc;
     if (!makeNodeJobs(ops, count, postIdx, nodes, acc)) return BEAGLE_ERROR_OUT_OF_RANGE;
     const int n = (int)nodes.size();
-    // F per (job, category, child) from each branch's length and rates
-    const HostEigen& E = hEigen_[e];
-    std::vector<double> F((size_t)n * C * 32);
+    // every branch's matrix must come from eigen-system e: the matrix kernel
+    // then wrote its Frechet factors F (r_c t) next to it, on the device, so
+    // a graph replay with new branch lengths uses the new factors
+    const size_t perJob = (size_t)C * 6 * 20;
     for (int j = 0; j < n; ++j) {
         for (int ch = 0; ch < 2; ++ch) {
             const int m = ops[(size_t)nodes[j].oidx[ch] * BEAGLE_OP_COUNT + 4];
             const MatrixOrigin& org = matOrigin_[m];
             if (org.eigen != e || hCatRates_[org.rates].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
-            for (int c = 0; c < C; ++c) {
-                const double tau = hCatRates_[org.rates][c] * org.t;
-                double* f = &F[((size_t)j * C + c) * 32 + ch * 16];
-                for (int a = 0; a < 4; ++a)
-                    for (int b = 0; b < 4; ++b) {
-                        const double d = E.lam[a] - E.lam[b];
-                        // (e^{tau l_a} - e^{tau l_b}) / (l_a - l_b), exactly
-                        // tau e^{tau l} at d = 0, without cancellation near it
-                        f[a * 4 + b] = d == 0.0 ? tau * std::exp(tau * E.lam[b])
-                                                : std::exp(tau * E.lam[b]) * std::expm1(tau * d) / d;
-                    }
-            }
+            nodes[j].mF[ch] = m * C * 16;
         }
+        nodes[j].mbase = (int)(j * perJob);
     }
-    std::vector<double> eig(32);
-    std::copy(E.V.begin(), E.V.end(), eig.begin());
-    std::copy(E.Vi.begin(), E.Vi.end(), eig.begin() + 16);
-    // device data: jobs, F, V | V^-1 in one upload; gathered matrices
-    const size_t perJob = (size_t)C * 6 * 20;
-    for (int j = 0; j < n; ++j) nodes[j].mbase = (int)(j * perJob);
-    const size_t jobBytes = (size_t)n * sizeof(NodeJob);
-    const size_t fOff = (jobBytes + 255) / 256 * 256;
-    const size_t eOff = fOff + F.size() * sizeof(double);
-    std::vector<char> blob(eOff + eig.size() * sizeof(double));
-    std::memcpy(blob.data(), nodes.data(), jobBytes);
-    std::memcpy(blob.data() + fOff, F.data(), F.size() * sizeof(double));
-    std::memcpy(blob.data() + eOff, eig.data(), eig.size() * sizeof(double));
-    char* dblob = reinterpret_cast<char*>(upload(blob.data(), blob.size()));
-    const NodeJob* dj = reinterpret_cast<const NodeJob*>(dblob);
+    const NodeJob* dj =
+        reinterpret_cast<const NodeJob*>(upload(nodes.data(), (size_t)n * sizeof(NodeJob)));
+    const double* eig = dVVi_ + (size_t)e * 2 * S * S;
     void* jm = grow(dJobMats_, jobMatsCap_, (size_t)n * perJob * realSize());
     const int ntiles = (dims_.P + 63) / 64;
     const int nb = (dims_.P + 255) / 256;
     const size_t nq = (size_t)16 * ntiles * C, nf = (size_t)S * nb;
-    double* wk = reinterpret_cast<double*>(grow(dWork_, workCap_, (nq + nf + 16 + S) * sizeof(double)));
+    double* wk = reinterpret_cast<double*>(
+        grow(dWork_, workCap_, (nq + nf + 2 * (16 + S)) * sizeof(double)));
     double* sums = wk + nq + nf;
+    double* out = sums + 16 + S;
     const double* cw = dCatWeights_ + (size_t)wIdx * C;
     if (dp_) launch_gather_node_matrices<double>(dj, n, dims_, dMatrices_, jm, stream_);
     else launch_gather_node_matrices<float>(dj, n, dims_, dMatrices_, jm, stream_);
     if (dp_)
-        launch_qgrad<double>(dj, n, dims_, jm, reinterpret_cast<const double*>(dblob + fOff),
-                             reinterpret_cast<const double*>(dblob + eOff), dDummy_, cw,
-                             dPatternWeights_, wk, stream_);
+        launch_qgrad<double>(dj, n, dims_, jm, dF_, eig, dDummy_, cw, dPatternWeights_, wk,
+                             stream_);
     else
-        launch_qgrad<float>(dj, n, dims_, jm, reinterpret_cast<const double*>(dblob + fOff),
-                            reinterpret_cast<const double*>(dblob + eOff), dDummy_, cw,
-                            dPatternWeights_, wk, stream_);
+        launch_qgrad<float>(dj, n, dims_, jm, dF_, eig, dDummy_, cw, dPatternWeights_, wk,
+                            stream_);
     launch_reduce_rows(wk, ntiles * C, 16, sums, stream_);
     const bool freqs = outFreqs != nullptr;
     if (freqs) {
@@ -1860,17 +1850,18 @@ int Instance::rateMatrixGradient(int e, const int* ops, int count, const int* po
         else launch_root_freq_grad<float>(dPartials_[rootIdx], cw, dFreqs_ + (size_t)fIdx * S, dPatternWeights_, wk + nq, dims_, stream_);
         launch_reduce_rows(wk + nq, nb, S, sums + 16, stream_);
     }
+    // dlogL/dQ = V^-T H V^T on the device, then the frequency sums
+    launch_qgrad_finish(sums, eig, freqs ? S : 0, out, stream_);
     HIPCHECK(hipGetLastError());
-    const double* h = stageOut(sums, 16 + (freqs ? S : 0), 0);
+    if (capturing_) {
+        if (outQ) cap_.outs.push_back({outQ, captureResult(out, 16), 16, 1, false});
+        if (freqs) cap_.outs.push_back({outFreqs, captureResult(out + 16, S), S, 1, false});
+        return BEAGLE_SUCCESS;
+    }
+    const double* h = stageOut(out, 16 + (freqs ? S : 0), 0);
     sync();
-    // dlogL/dQ = V^-T H V^T
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) {
-            double g = 0.0;
-            for (int k = 0; k < 4; ++k)
-                for (int l = 0; l < 4; ++l) g += E.Vi[k * 4 + a] * h[k * 4 + l] * E.V[b * 4 + l];
-            if (outQ) outQ[a * 4 + b] = g;
-        }
+    if (outQ)
+        for (int q = 0; q < 16; ++q) outQ[q] = h[q];
     if (freqs)
         for (int s = 0; s < S; ++s) outFreqs[s] = h[16 + s];
     return BEAGLE_SUCCESS;

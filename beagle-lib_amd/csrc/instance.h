@@ -331,6 +331,7 @@ private:
     double* dSiteD1_ = nullptr;
     double* dSiteD2_ = nullptr;
     double* dResult_ = nullptr;       // small device result vector
+    double* dF_ = nullptr;            // 4 real states: Frechet factors per matrix [C][4][4]
     double* noAllocScratch_ = nullptr;  // this device's scratch array (noalloc.hip)
     char* dDummy_ = nullptr;          // scratch tile for the pipelined traversal
     double* hResult_ = nullptr;       // pinned mirror

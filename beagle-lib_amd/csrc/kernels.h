@@ -81,6 +81,7 @@ struct NodeJob {
     int moP[2], moD1[2], moD2[2];  // matrix element offsets in the matrix slab
     int oidx[2];               // operation index of each child in the derivative output
     int mbase;                 // this job's block in the gathered matrices (reals)
+    int mF[2];                 // the children's branch factors in the factor slab (doubles)
 };
 
 // One transition-matrix update: P, dP/dt, d2P/dt2 for all categories.
@@ -93,6 +94,8 @@ struct MatrixJob {
     double t;
     int cat;                          // >= 0: only this category, rate 1 (model categories)
     int cplx;                         // complex eigen-system (conjugate pairs as 2 x 2 blocks)
+    double* F;                        // 4 real states: the branch's Frechet factors
+                                      //   [C][4][4] for the rate-matrix gradient, or null
 };
 
 // One root of a several-root likelihood (calcRootLogLikelihoodsMulti).
@@ -280,9 +283,14 @@ void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, 
 // sums of H (16 rows of tiles * C); jobF holds F per (job, category, child)
 // as [job][category][2][16], eig = V | V^-1 row-major.
 template <typename R>
+// factors: the factor slab (NodeJob::mF offsets); eig: [V | V^-1] (FP64)
 void launch_qgrad(const NodeJob* djobs, int njobs, const Dims& d, const void* jobMats,
-                  const double* jobF, const double* eig, void* dummy, const double* cw,
+                  const double* factors, const double* eig, void* dummy, const double* cw,
                   const double* pw, double* partial, hipStream_t st);
+// out[0..15] = V^-T H V^T from the summed H (4 states); out[16..16+nf) =
+// sums[16..16+nf) (the root-frequency gradient, passed through)
+void launch_qgrad_finish(const double* sums, const double* eig, int nf, double* out,
+                         hipStream_t st);
 // Rate-matrix gradient for any S <= 64 over pre-order jobs (one branch
 // each, own operands attached): coef [nops][Ppad], xs [nops][C][chunks][S*S]
 // work; z receives (V^T X V^-T) o F as rows of [S*S][nops * C], summed

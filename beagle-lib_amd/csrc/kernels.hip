@@ -217,6 +217,16 @@ __global__ __launch_bounds__(WG) void k_matrices(const MatrixJob* __restrict__ j
         e1[k] = sl * e;
         e2[k] = sl * e1[k];
     }
+    if (J.F && threadIdx.x < 16) {
+        // Frechet factors of exp(tau Q), tau = r t, for the rate-matrix
+        // gradient: F_ab = (e^{tau l_a} - e^{tau l_b}) / (l_a - l_b), as
+        // e^{tau l_b} expm1(tau d) / d (no cancellation near d = 0), and
+        // tau e^{tau l} at d = 0
+        const int a = threadIdx.x >> 2, b = threadIdx.x & 3;
+        const double tau = r * J.t, la = J.evals[a], lb = J.evals[b], dl = la - lb;
+        const double eb = exp(tau * lb);
+        J.F[c * 16 + threadIdx.x] = dl == 0.0 ? tau * eb : eb * expm1(tau * dl) / dl;
+    }
     __syncthreads();
     const int W = S + 1;
     R* Pm = reinterpret_cast<R*>(J.P) + (long)c * S * W;
@@ -1652,7 +1662,9 @@ __device__ __forceinline__ void q_issue(const NodeJob& J, int jidx, const void* 
     const int l = min(lane, NODE_MATS / V - 1);
     o.mv = *at<typename Vec16<R>::T>(jm, (unsigned)((J.mbase + c * NODE_MATS + l * V) *
                                                     (int)sizeof(R)));
-    o.fv = *at<dvec2>(jf, (unsigned)(((jidx * C + c) * 32 + 2 * min(lane, 15)) * 8));
+    // lanes 0-7: child a's 16 factors of category c, lanes 8-15: child b's
+    const int fl = min(lane, 15);
+    o.fv = *at<dvec2>(jf, (unsigned)((J.mF[fl >> 3] + c * 16 + 2 * (fl & 7)) * 8));
 }
 
 template <typename R>
@@ -1782,6 +1794,29 @@ void launch_qgrad(const NodeJob* djobs, int njobs, const Dims& d, const void* jo
     hipLaunchKernelGGL((k_qgrad4<R>), dim3(tiles), dim3(TILE * d.C), lds, st, djobs, njobs, d.P,
                        d.C, d.catStride, reinterpret_cast<const R*>(jobMats), jobF, eig, dummyX,
                        dummyS, cw, pw, partial);
+}
+
+// dlogL/dQ = V^-T H V^T (one wave), the root-frequency sums copied after it.
+__global__ __launch_bounds__(64) void k_qgrad_finish(const double* __restrict__ sums,
+                                                     const double* __restrict__ eig, int nf,
+                                                     double* __restrict__ out) {
+    const int t = threadIdx.x;
+    if (t < 16) {
+        const int a = t >> 2, b = t & 3;
+        const double* V = eig;
+        const double* Vi = eig + 16;
+        double g = 0.0;
+        for (int k = 0; k < 4; ++k)
+            for (int l = 0; l < 4; ++l) g += Vi[k * 4 + a] * sums[k * 4 + l] * V[b * 4 + l];
+        out[t] = g;
+    } else if (t < 16 + nf) {
+        out[t] = sums[t];
+    }
+}
+
+void launch_qgrad_finish(const double* sums, const double* eig, int nf, double* out,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_qgrad_finish, dim3(1), dim3(64), 0, st, sums, eig, nf, out);
 }
 
 // dlogL/dpi_i through the root distribution: sum_p w_p (sum_c w_c R_cpi) /

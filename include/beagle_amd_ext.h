@@ -101,10 +101,13 @@ int beagleExtPlanCheckpoints(int tipCount, const BeagleOperation* operations, in
  * reference).  Between beagleExtBeginCapture and beagleExtEndCapture the
  * instance's calls are recorded into a graph instead of run:
  * transition-matrix updates, post- and pre-order traversals, root
- * log-likelihoods, beagleExtUpdatePrePartialsWithDerivatives and
- * beagleExtCalculateLogLikelihoodAndDerivatives (4 states).  A call that
- * must wait for the device (getters, set* copies, the derivative path for
- * other state counts) fails and spoils the capture: EndCapture then returns
+ * log-likelihoods, beagleExtUpdatePrePartialsWithDerivatives,
+ * beagleExtCalculateLogLikelihoodAndDerivatives and, for 4 states with at
+ * most 8 categories, beagleExtCalculateRateMatrixGradient (its Frechet
+ * factors come from the captured transition-matrix updates, so replays at
+ * new branch lengths give the new gradient).  A call that must wait for the
+ * device (getters, set* copies, the rate-matrix gradient for other state
+ * counts) fails and spoils the capture: EndCapture then returns
  * BEAGLE_ERROR_GENERAL and records nothing.  Work buffers must already have
  * their size, so run the same calls once before capturing.  The output
  * arrays passed to captured calls receive each replay's results when

@@ -1449,3 +1449,39 @@ def test_root_by_partition_beyond_64_partitions(amd, oracle, precision):
     assert bg[50] == 0.0
     assert np.abs(bg - bo).max() <= tol * np.abs(bo).max()
     assert rel(tg, to) < tol and rel(tg, wg) < tol
+
+
+@pytest.mark.parametrize("precision,scaling", [("single", "auto"), ("double", "none")])
+def test_graph_replays_rate_matrix_gradient(amd, oracle, precision, scaling):
+    """logL, branch derivatives, dlogL/dQ and dlogL/dpi_root captured in one
+    HIP graph (4 states): each replay at new branch lengths equals the eager
+    calls at those lengths bit for bit (the Frechet factors come from the
+    replayed matrix updates), and dlogL/dQ matches fourth-order differences
+    of the oracle's log-likelihood along a random direction at the last
+    lengths."""
+    prob = synth.make_problem(261, 16, 800, S=4, shape="yule", missing=0.05, mean_length=0.2)
+    rng = np.random.default_rng(21)
+    lens = [prob.tree.lengths * rng.uniform(0.5, 1.5, prob.tree.lengths.shape) for _ in range(3)]
+    ev = driver.Evaluator(amd, prob, precision=precision, scaling=scaling)
+    want = [ev.fused_gradient(x, rate_matrix=True) for x in lens]
+    ev.capture_fused_gradient(rate_matrix=True)
+    for x, w in zip(lens, want):
+        got = ev.replay_fused_gradient(x)
+        assert len(got) == 5
+        for a, b in zip(got, w):
+            np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+    ev.close()
+    G = want[-1][3]
+    last = _with_lengths(prob, lens[-1])
+    h = 2.5e-5
+    E = rng.normal(size=(4, 4))
+    f = [_oracle_loglik(oracle, _with_model(last, Q=last.model.Q + s * E))
+         for s in (-2 * h, -h, h, 2 * h)]
+    fd = (8 * (f[2] - f[1]) - (f[3] - f[0])) / (12 * h)
+    tol = 1e-8 if precision == "double" else 2e-5
+    assert abs(float(np.sum(G * E)) - fd) <= tol * max(abs(fd), np.abs(G).max())
+
+
+def _with_lengths(prob, lengths):
+    import dataclasses
+    return dataclasses.replace(prob, tree=dataclasses.replace(prob.tree, lengths=lengths))
