@@ -21,7 +21,7 @@ COMMON = ["-std=c++17", "-O3", "-fPIC", "-fvisibility=hidden", "--offload-arch=g
 # from ordinary float code, and this library's kernels do not rely on them
 DEVICE = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["kernels.hip", "mfma.hip", "noalloc.hip", "instance.cpp", "checkpoint.cpp",
+SOURCES = ["kernels.hip", "nodes.hip", "mfma.hip", "noalloc.hip", "instance.cpp", "checkpoint.cpp",
            "frontend.cpp"]
 
 

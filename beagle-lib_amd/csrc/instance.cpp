@@ -1827,7 +1827,8 @@ int Instance::rateMatrixGradient(int e, const int* ops, int count, const int* po
     void* jm = grow(dJobMats_, jobMatsCap_, (size_t)n * perJob * realSize());
     const int ntiles = (dims_.P + 63) / 64;
     const int nb = (dims_.P + 255) / 256;
-    const size_t nq = (size_t)16 * ntiles * C, nf = (size_t)S * nb;
+    const int chunks = qgrad_chunks(dims_, n);
+    const size_t nq = (size_t)16 * ntiles * chunks * C, nf = (size_t)S * nb;
     double* wk = reinterpret_cast<double*>(
         grow(dWork_, workCap_, (nq + nf + 2 * (16 + S)) * sizeof(double)));
     double* sums = wk + nq + nf;
@@ -1841,7 +1842,7 @@ int Instance::rateMatrixGradient(int e, const int* ops, int count, const int* po
     else
         launch_qgrad<float>(dj, n, dims_, jm, dF_, eig, dDummy_, cw, dPatternWeights_, wk,
                             stream_);
-    launch_reduce_rows(wk, ntiles * C, 16, sums, stream_);
+    launch_reduce_rows(wk, ntiles * chunks * C, 16, sums, stream_);
     const bool freqs = outFreqs != nullptr;
     if (freqs) {
         if (!validBuffer(rootIdx) || !dPartials_[rootIdx] || !validEigen(fIdx) || hFreqs_[fIdx].empty())

@@ -282,8 +282,11 @@ void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, 
 // Rate-matrix gradient over node jobs (4 states): per-tile, per-category
 // sums of H (16 rows of tiles * C); jobF holds F per (job, category, child)
 // as [job][category][2][16], eig = V | V^-1 row-major.
-template <typename R>
+// Chunks of the job list k_qgrad4 spreads over its grid's second dimension;
+// partial holds [16][tiles * chunks * C] sums.
+int qgrad_chunks(const Dims& d, int njobs);
 // factors: the factor slab (NodeJob::mF offsets); eig: [V | V^-1] (FP64)
+template <typename R>
 void launch_qgrad(const NodeJob* djobs, int njobs, const Dims& d, const void* jobMats,
                   const double* factors, const double* eig, void* dummy, const double* cw,
                   const double* pw, double* partial, hipStream_t st);

@@ -18,93 +18,14 @@
 // Tree shape (a 127-deep caterpillar or a balanced tree) does not change
 // the launch count.
 #include "kernels.h"
-#include "device_util.h"
+#include "device_common.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <type_traits>
 
 namespace hipbeagle {
-
-// ---------------------------------------------------------------------------
-// small helpers
-
-template <typename R>
-__device__ __forceinline__ void load4(const R* p, R (&v)[4]);
-template <>
-__device__ __forceinline__ void load4<double>(const double* p, double (&v)[4]) {
-    const double2* q = reinterpret_cast<const double2*>(p);
-    double2 a = q[0], b = q[1];
-    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
-}
-template <>
-__device__ __forceinline__ void load4<float>(const float* p, float (&v)[4]) {
-    float4 a = *reinterpret_cast<const float4*>(p);
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-}
-typedef double dvec2 __attribute__((ext_vector_type(2)));
-typedef float fvec4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void load4(const GLOBAL double* p, double (&v)[4]) {
-    const GLOBAL dvec2* q = (const GLOBAL dvec2*)p;
-    const dvec2 a = q[0], b = q[1];
-    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
-}
-__device__ __forceinline__ void load4(const GLOBAL float* p, float (&v)[4]) {
-    const fvec4 a = *(const GLOBAL fvec4*)p;
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-}
-__device__ __forceinline__ void store4(GLOBAL double* p, const double (&v)[4]) {
-    GLOBAL dvec2* q = (GLOBAL dvec2*)p;
-    q[0] = dvec2{v[0], v[1]};
-    q[1] = dvec2{v[2], v[3]};
-}
-__device__ __forceinline__ void store4(GLOBAL float* p, const float (&v)[4]) {
-    *(GLOBAL fvec4*)p = fvec4{v[0], v[1], v[2], v[3]};
-}
-// The 64 lanes' 4-state FP64 vectors of 64 consecutive patterns (lane l
-// holds pattern l; t points at pattern 0) stored as two fully contiguous
-// 1 KB writes instead of two writes of 16-byte halves 32 bytes apart:
-// write h covers patterns [32 h, 32 h + 32), lane l writing half (l & 1)
-// of pattern 32 h + l / 2, fetched from that pattern's lane.  All 64 lanes
-// must be active.
-__device__ __forceinline__ void store4_tile(GLOBAL double* t, const double (&v)[4], int lane) {
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int src = 32 * h + (lane >> 1);
-        const double a = __shfl(v[0], src, 64), b = __shfl(v[1], src, 64);
-        const double c = __shfl(v[2], src, 64), d = __shfl(v[3], src, 64);
-        *(GLOBAL dvec2*)(t + 128 * h + 2 * lane) = (lane & 1) ? dvec2{c, d} : dvec2{a, b};
-    }
-}
-
-template <typename R>
-__device__ __forceinline__ void store4(R* p, const R (&v)[4]);
-template <>
-__device__ __forceinline__ void store4<double>(double* p, const double (&v)[4]) {
-    double2* q = reinterpret_cast<double2*>(p);
-    q[0] = make_double2(v[0], v[1]);
-    q[1] = make_double2(v[2], v[3]);
-}
-template <>
-__device__ __forceinline__ void store4<float>(float* p, const float (&v)[4]) {
-    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
-}
-
-// Sum over a 256-lane workgroup; the result is valid in thread 0.  The tree
-// is fixed for a given launch shape, so sums are reproducible run to run.
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) sh[wave] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0) {
-        const int nw = (blockDim.x + 63) >> 6;
-        for (int w = 0; w < nw; ++w) r += sh[w];
-    }
-    __syncthreads();
-    return r;
-}
 
 // out[j] = sum over n of in[j*stride + n]-style partials: one workgroup per
 // output, strided lane sums then a fixed tree.
@@ -335,7 +256,6 @@ void launch_convolve(const ConvolveJob* jobs, int njobs, int S, int C, hipStream
 // stores are 64 consecutive 16/32-byte vectors.  The per-pattern maximum
 // over categories that rescaling needs is exchanged through LDS; only
 // rescaled operations pay that barrier.
-constexpr int TILE = 64;
 
 // Cross-category per-pattern maximum; `k` counts rescaled operations so
 // consecutive exchanges alternate LDS buffers and one barrier suffices.
@@ -392,19 +312,6 @@ __device__ __forceinline__ void rescale4(R (&v)[4], R mx, int c, int p, double* 
     }
 }
 
-// One 16-byte vector of reals.
-template <typename R>
-struct Vec16;
-template <>
-struct Vec16<float> {
-    typedef fvec4 T;
-    static constexpr int N = 4;
-};
-template <>
-struct Vec16<double> {
-    typedef dvec2 T;
-    static constexpr int N = 2;
-};
 
 // A 4-state operand as held in registers between its load and its use:
 // partials values, or a tip state (expanded at use to a one-hot vector whose
@@ -457,37 +364,6 @@ __device__ __forceinline__ void apply4(const R* m, bool isState, const Operand4<
 // sets so no loaded register is ever copied.  Operand kinds are decided on
 // the host: partials (0), tip states (1), or the previous operation's
 // output forwarded from registers (2).
-// Buffer resource over `base` when `on`, else an empty range over `valid`:
-// an access to an empty range reads 0 and touches no memory, so a load a
-// step does not need costs one instruction, no branch and no traffic.  The
-// base and range are wave-uniform (job fields); readfirstlane states it,
-// so the resource stays in scalar registers.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t range_if(const void* base, bool on,
-                                                           const void* valid) {
-    const unsigned long long a = reinterpret_cast<unsigned long long>(on ? base : valid);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
-    const int n = __builtin_amdgcn_readfirstlane(on ? -1 : 0);
-    void* b = reinterpret_cast<void*>(((unsigned long long)hi << 32) | lo);
-    return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, n, 0x00020000);
-}
-
-// four reals at byte offset `at` of a buffer resource
-template <typename R>
-__device__ __forceinline__ void load4_range(__amdgpu_buffer_rsrc_t r, unsigned at, R (&v)[4]) {
-    if (sizeof(R) == 4) {
-        const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, at, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = (R)__uint_as_float(q[i]);
-    } else {
-        const auto q0 = __builtin_amdgcn_raw_buffer_load_b128(r, at, 0, 0);
-        const auto q1 = __builtin_amdgcn_raw_buffer_load_b128(r, at + 16, 0, 0);
-        v[0] = (R)__hiloint2double((int)q0[1], (int)q0[0]);
-        v[1] = (R)__hiloint2double((int)q0[3], (int)q0[2]);
-        v[2] = (R)__hiloint2double((int)q1[1], (int)q1[0]);
-        v[3] = (R)__hiloint2double((int)q1[3], (int)q1[2]);
-    }
-}
 
 // One operand's loads, issued whatever its kind so that every step of a
 // pipelined kernel issues the same loads: partials (kind 0) at element
@@ -517,14 +393,6 @@ __device__ __forceinline__ void operand4(int kind, const Operand4<R>& o, const R
     x[4] = (isS && st >= 4) ? (R)1 : (R)0;
 }
 
-// t_i = sum_j M[i][j] x_j with x_4 selecting the pad column
-template <typename R>
-__device__ __forceinline__ void mat4(const R* m, const R (&x)[5], R (&t)[4]) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        t[i] = m[i * 5 + 0] * x[0] + m[i * 5 + 1] * x[1] + m[i * 5 + 2] * x[2] +
-               m[i * 5 + 3] * x[3] + m[i * 5 + 4] * x[4];
-}
 
 // One pipelined step's loads: both operands and, with automatic scaling,
 // their exponents (a partials operand's exponent array, else the dummy),
@@ -789,44 +657,6 @@ __device__ __forceinline__ void pre_issue(const PreJob& J, const void* dx, const
     if (DERIV) issue4(J.xself, J.sself, J.kSelf, dx, off, p, o.Y);
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
-
-// v moved across lanes by a DPP control; lanes without a source read 0.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_move(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-
-// v moved across lanes by a DPP control into the rows ROWS selects; the
-// other rows read an unspecified value (no instruction clears them).
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_move_rows(double v) {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, ROWS, 0xf, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, ROWS, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-
-// Sum over the 64 lanes of a wave with DPP steps (quad swaps, row shifts,
-// row broadcasts); the total ends in lane 63 and is returned to all lanes.
-// The broadcasts leave rows 0 and 2 unspecified: lane 63 reads row 2's
-// sum before the first broadcast and row 1's (complete) after it.
-__device__ __forceinline__ double wave_sum_dpp(double v) {
-    v += dpp_move<0xB1, 0xf>(v);    // quad_perm [1,0,3,2]
-    v += dpp_move<0x4E, 0xf>(v);    // quad_perm [2,3,0,1]: every lane holds its quad's sum
-    v += dpp_move<0x114, 0xf>(v);   // row_shr:4
-    v += dpp_move<0x118, 0xf>(v);   // row_shr:8: lanes 12-15 of each row hold the row sum
-    v += dpp_move_rows<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
-    v += dpp_move_rows<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3: lane 63 holds the total
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
-    return __hiloint2double(hi, lo);
-}
-
 template <typename R, bool AUTO, bool DERIV>
 __device__ __forceinline__ void pre_compute(const PreJob& J, const PreOps4<R>& o, R (&prev)[4],
                                             R (&uprev)[4], const R* __restrict__ mats,
@@ -992,872 +822,6 @@ __global__ __launch_bounds__(512) void k_pre4_level(const PreJob* __restrict__ j
                                 dummyW, dummyE, cw, pw, partial, tile, ntiles);
 }
 
-// ---------------------------------------------------------------------------
-// Node-fused pre-order + derivatives (4 states).  One job per internal node
-// v with children a, b: pre(v), x_a and x_b are loaded once (pre(v) comes
-// from registers when the previous job produced it) and both children's
-// pre-order partials and branch derivative sums are formed.  Pipelining as
-// in k_post4: fixed loads into alternating register sets, unused slots read
-// the dummy tile, stores of lanes past P or of outputs that are not kept go
-// to the dummy tile.  With automatic scaling the exponents are loaded with
-// the operands, one job ahead, and the pattern weight is loaded once.
-
-// A node job's six matrices for one category (P_a, P_b, D1_a, D2_a, D1_b,
-// D2_b; 20 reals each, contiguous after launch_gather_node_matrices) are
-// loaded one job ahead as one 16-byte vector per lane, staged in the
-// wave's own LDS block and read back as broadcasts, which keeps them out
-// of scalar registers.  Every lane stores its vector (the block holds 64 of
-// them; lanes past the matrices repeat the last one) so no branch
-// separates the lanes.
-constexpr int NODE_MATS = 6 * 20;
-constexpr int NODE_F32_BLOCK = 128;       // floats of a wave's block in the FP32 node kernels
-constexpr int NODE_MAT_BLOCK = 64 * 16;   // bytes of one wave's block
-
-// 32-bit addressing: a wave-uniform base pointer plus a per-lane byte
-// offset below 4 GiB, so the base stays in scalar registers and an address
-// costs one VGPR (the host uses these kernels only for buffers < 4 GiB).
-template <typename T>
-__device__ __forceinline__ const GLOBAL T* at(const void* base, unsigned boff) {
-    return (const GLOBAL T*)((const GLOBAL char*)base + boff);
-}
-template <typename T>
-__device__ __forceinline__ GLOBAL T* at_w(void* base, unsigned boff) {
-    return (GLOBAL T*)((GLOBAL char*)base + boff);
-}
-
-template <typename R>
-struct NodeOps4 {
-    R u[4];                    // parent pre-order partials (kPar 0)
-    R xa[4], xb[4];            // children's post-order partials
-    int sa, sb;                // children's tip states
-    int eu, ea, eb;            // exponents (automatic scaling)
-    typename Vec16<R>::T mv;   // this lane's share of the job's matrices
-};
-
-// Byte offsets: `off` of this lane's 4-state vector in a partials buffer,
-// `doff` of its vector in the dummy tile; tip states at 4p (dummy 4 lane),
-// exponents at 2p (dummy 2 lane).
-template <typename R, bool AUTO>
-__device__ __forceinline__ void node_issue(const NodeJob& J, const void* dx, const void* ds,
-                                           const void* dE, const R* __restrict__ jm, int c,
-                                           unsigned off, unsigned doff, int p, int lane,
-                                           NodeOps4<R>& o) {
-    // (scratch-tile reads here, not empty buffer ranges: those take more
-    // registers and this kernel is at its budget)
-    const bool lp = J.kPar == 0, pa = J.k[0] == 0, pb = J.k[1] == 0;
-    load4(at<R>(lp ? J.parent : dx, lp ? off : doff), o.u);
-    load4(at<R>(pa ? J.x[0] : dx, pa ? off : doff), o.xa);
-    load4(at<R>(pb ? J.x[1] : dx, pb ? off : doff), o.xb);
-    o.sa = *at<int>(pa ? ds : J.x[0], pa ? 4u * lane : 4u * p);
-    o.sb = *at<int>(pb ? ds : J.x[1], pb ? 4u * lane : 4u * p);
-    {
-        constexpr int V = Vec16<R>::N;
-        const int l = min(lane, NODE_MATS / V - 1);
-        o.mv = *at<typename Vec16<R>::T>(jm, (unsigned)((J.mbase + c * NODE_MATS + l * V) *
-                                                        (int)sizeof(R)));
-    }
-    if (AUTO) {
-        const bool a = lp && J.eParent != nullptr, b = J.ex[0] != nullptr,
-                   e = J.ex[1] != nullptr;
-        o.eu = *at<short>(a ? (const void*)J.eParent : dE, a ? 2u * p : 2u * lane);
-        o.ea = *at<short>(b ? (const void*)J.ex[0] : dE, b ? 2u * p : 2u * lane);
-        o.eb = *at<short>(e ? (const void*)J.ex[1] : dE, e ? 2u * p : 2u * lane);
-    }
-}
-
-// A child's operand as a 5-vector: its partials, or for a tip the one-hot
-// state whose fifth entry selects the matrices' pad column (missing state).
-template <typename R>
-__device__ __forceinline__ void child5(int kind, int st, const R (&x)[4], R (&o)[5]) {
-    const bool isS = kind == 1;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) o[j] = isS ? (st == j ? (R)1 : (R)0) : x[j];
-    o[4] = (isS && st >= 4) ? (R)1 : (R)0;
-}
-
-template <typename R>
-__device__ __forceinline__ R dot4(const R (&a)[4], const R (&b)[4]) {
-    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
-}
-
-// Derivative numerators and denominator of one branch for one category,
-// in the partials' precision (categories are combined in FP64):
-// den = pre . x (a tip: pre[state & 3], as the reference computes it),
-// n1 = pre . (D1 x), n2 = pre . (D2 x).  One matrix is read at a time.
-template <typename R>
-__device__ __forceinline__ void branch_sums(int kind, int st, const R (&pre)[4], const R (&x)[5],
-                                            const R* d1, const R* d2, R& den, R& n1, R& n2) {
-    if (kind == 1) {
-        // one-hot weights, not an index: an indexed pick is lowered to a
-        // private-memory load
-        const int s = st & 3;
-        den = pre[0] * (R)(s == 0) + pre[1] * (R)(s == 1) + pre[2] * (R)(s == 2) +
-              pre[3] * (R)(s == 3);
-    } else {
-        den = pre[0] * x[0] + pre[1] * x[1] + pre[2] * x[2] + pre[3] * x[3];
-    }
-    R t[4];
-    __builtin_amdgcn_sched_barrier(0);
-    mat4(d1, x, t);
-    n1 = dot4(pre, t);
-    __builtin_amdgcn_sched_barrier(0);
-    mat4(d2, x, t);
-    n2 = dot4(pre, t);
-    __builtin_amdgcn_sched_barrier(0);
-}
-
-// t_j = sum_i M[i][j] u_i: the transpose product of the pre-order update
-template <typename R>
-__device__ __forceinline__ void mat4t(const R* m, const R (&u)[4], R (&t)[4]) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-        t[j] = m[0 * 5 + j] * u[0] + m[1 * 5 + j] * u[1] + m[2 * 5 + j] * u[2] +
-               m[3 * 5 + j] * u[3];
-}
-
-template <typename R, bool AUTO>
-__device__ __forceinline__ void node_compute(const NodeJob& J, const NodeOps4<R>& o, R (&pa)[4],
-                                             R (&pb)[4], int& pea, int& peb, R* smat,
-                                             double* sdv, int& k, int c, int lane, int p, int P,
-                                             int C, unsigned off, unsigned doff, R* dummyW,
-                                             void* dummyEw, const double* __restrict__ cw,
-                                             double w, double* partial, int tile, int ntiles) {
-    const bool in = p < P;
-    R u[4], xa[5], xb[5], t[4], ua[4], ub[4], va[4], vb[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) u[i] = J.kPar == 2 ? pa[i] : (J.kPar == 3 ? pb[i] : o.u[i]);
-    const int eu = J.kPar == 2 ? pea : (J.kPar == 3 ? peb : o.eu);
-    child5(J.k[0], o.sa, o.xa, xa);
-    child5(J.k[1], o.sb, o.xb, xb);
-    // this wave's LDS block for the job's matrices (two alternating blocks)
-    R* lm = smat + (c * 2 + (k & 1)) * (NODE_MAT_BLOCK / (int)sizeof(R));
-    *(typename Vec16<R>::T*)(lm + lane * Vec16<R>::N) = o.mv;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-    // scheduling barriers keep one matrix in registers at a time
-    __builtin_amdgcn_sched_barrier(0);
-    mat4(lm, xa, t);                     // P_a x_a
-#pragma unroll
-    for (int i = 0; i < 4; ++i) ub[i] = u[i] * t[i];
-    __builtin_amdgcn_sched_barrier(0);
-    mat4(lm + 20, xb, t);                // P_b x_b
-#pragma unroll
-    for (int i = 0; i < 4; ++i) ua[i] = u[i] * t[i];
-    __builtin_amdgcn_sched_barrier(0);
-    mat4t(lm, ua, va);                   // pre(a) = P_a^T (pre(v) * P_b x_b)
-    __builtin_amdgcn_sched_barrier(0);
-    mat4t(lm + 20, ub, vb);              // pre(b) = P_b^T (pre(v) * P_a x_a)
-    // derivative sums of both branches, formed before rescaling (the
-    // ratios do not depend on it)
-    R dA, n1A, n2A, dB, n1B, n2B;
-    branch_sums(J.k[0], o.sa, va, xa, lm + 40, lm + 60, dA, n1A, n2A);
-    branch_sums(J.k[1], o.sb, vb, xb, lm + 80, lm + 100, dB, n1B, n2B);
-    const double wc = cw[c];
-    // LDS exchange [2][C][8][TILE]: max a, max b, den/n1/n2 of a, of b
-    double* buf = sdv + (k & 1) * C * 8 * TILE;
-    ++k;
-    double* mine = buf + c * 8 * TILE + lane;
-    mine[0 * TILE] = in ? (double)fmax(fmax(va[0], va[1]), fmax(va[2], va[3])) : 0.0;
-    mine[1 * TILE] = in ? (double)fmax(fmax(vb[0], vb[1]), fmax(vb[2], vb[3])) : 0.0;
-    mine[2 * TILE] = wc * (double)dA;
-    mine[3 * TILE] = wc * (double)n1A;
-    mine[4 * TILE] = wc * (double)n2A;
-    mine[5 * TILE] = wc * (double)dB;
-    mine[6 * TILE] = wc * (double)n1B;
-    mine[7 * TILE] = wc * (double)n2B;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    const bool keepA = in && J.dest[0] != nullptr;
-    const bool keepB = in && J.dest[1] != nullptr;
-    int outA = 0, outB = 0;
-    if (AUTO) {
-        double ma = buf[lane], mb = buf[TILE + lane];
-        for (int cc = 1; cc < C; ++cc) {
-            ma = fmax(ma, buf[cc * 8 * TILE + lane]);
-            mb = fmax(mb, buf[cc * 8 * TILE + TILE + lane]);
-        }
-        const int ea = ma > 0.0 ? exponent_of((R)ma) : 0;
-        const int eb = mb > 0.0 ? exponent_of((R)mb) : 0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            va[i] = pow2(va[i], -ea);
-            vb[i] = pow2(vb[i], -eb);
-        }
-        // pre(a) carries the exponents of pre(v) and of x_b; pre(b) those of
-        // pre(v) and x_a
-        outA = ea + eu + (J.ex[1] ? o.eb : 0);
-        outB = eb + eu + (J.ex[0] ? o.ea : 0);
-        *at_w<short>(keepA ? (void*)J.ed[0] : dummyEw, keepA ? 2u * p : 2u * lane) = (short)outA;
-        *at_w<short>(keepB ? (void*)J.ed[1] : dummyEw, keepB ? 2u * p : 2u * lane) = (short)outB;
-    }
-    // per-tile sums of child `ch` by wave ch % C: w r1, w r1^2, w r2 with
-    // r = N / D summed over categories
-#pragma unroll
-    for (int ch = 0; ch < 2; ++ch) {
-        if (c != ch % C) continue;
-        double D = 0, N1 = 0, N2 = 0;
-        for (int cc = 0; cc < C; ++cc) {
-            const double* q = buf + cc * 8 * TILE + (2 + 3 * ch) * TILE + lane;
-            D += q[0];
-            N1 += q[TILE];
-            N2 += q[2 * TILE];
-        }
-        const double r1 = in ? N1 / D : 0.0, r2 = in ? N2 / D : 0.0;
-        const double s1 = wave_sum_dpp(w * r1), sq = wave_sum_dpp(w * r1 * r1),
-                     s2 = wave_sum_dpp(w * r2);
-        if (lane == 0) {
-            GLOBAL double* pt = gout<double>(partial);
-            const long row = (long)J.oidx[ch] * 3;
-            pt[(row + 0) * ntiles + tile] = s1;
-            pt[(row + 1) * ntiles + tile] = sq;
-            pt[(row + 2) * ntiles + tile] = s2;
-        }
-    }
-    store4(at_w<R>(keepA ? J.dest[0] : (void*)dummyW, keepA ? off : doff), va);
-    store4(at_w<R>(keepB ? J.dest[1] : (void*)dummyW, keepB ? off : doff), vb);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        pa[i] = va[i];
-        pb[i] = vb[i];
-    }
-    pea = outA;
-    peb = outB;
-}
-
-#define NODE_KERNEL_SETUP                                                                     \
-    extern __shared__ double sdv[]; /* [2][C][8][TILE] exchange, then [C][2] matrix blocks */ \
-    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);                          \
-    const int lane = threadIdx.x & 63;                                                        \
-    const int tile = blockIdx.x;                                                              \
-    const int p = tile * TILE + lane; /* < Ppad: pattern weights are zero-padded */           \
-    R* smat = reinterpret_cast<R*>(sdv + 2 * C * 8 * TILE);                                   \
-    const unsigned off = (unsigned)((c * cs + (long)p * 4) * (long)sizeof(R));                \
-    const unsigned doff = (unsigned)((c * TILE + lane) * 4 * (int)sizeof(R));                 \
-    const int ntiles = (P + TILE - 1) / TILE;                                                 \
-    const double w = pw[p];                                                                   \
-    void* dummyEw = const_cast<char*>(reinterpret_cast<const char*>(dummyE)) + 128;           \
-    int k = 0, pea = 0, peb = 0;                                                              \
-    R pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
-
-template <typename R, bool AUTO>
-__global__ __launch_bounds__(512) void k_pren4(const NodeJob* __restrict__ jobs, int njobs,
-                                               int chunk, int P, int C, long cs,
-                                               const R* __restrict__ jm, const R* dummyX,
-                                               const int* dummyS, R* dummyW,
-                                               const short* dummyE, const double* __restrict__ cw,
-                                               const double* __restrict__ pw, double* partial) {
-    NODE_KERNEL_SETUP
-    // workgroup (tile, y) walks jobs [y chunk, (y + 1) chunk)
-    jobs += blockIdx.y * chunk;
-    njobs = min(chunk, njobs - (int)blockIdx.y * chunk);
-    NodeOps4<R> S0, S1;
-    NodeJob J0 = jobs[0], J1;
-    node_issue<R, AUTO>(J0, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
-    for (int o = 0;; o += 2) {
-        J1 = jobs[min(o + 1, njobs - 1)];
-        node_issue<R, AUTO>(J1, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S1);
-        node_compute<R, AUTO>(J0, S0, pa, pb, pea, peb, smat, sdv, k, c, lane, p, P, C, off, doff,
-                              dummyW, dummyEw, cw, w, partial, tile, ntiles);
-        if (o + 1 >= njobs) break;
-        J0 = jobs[min(o + 2, njobs - 1)];
-        node_issue<R, AUTO>(J0, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
-        node_compute<R, AUTO>(J1, S1, pa, pb, pea, peb, smat, sdv, k, c, lane, p, P, C, off, doff,
-                              dummyW, dummyEw, cw, w, partial, tile, ntiles);
-        if (o + 2 >= njobs) break;
-    }
-}
-
-// One dependency level of node jobs: one job per workgroup, grid (tiles,
-// jobs); every parent is read from memory.
-template <typename R, bool AUTO>
-__global__ __launch_bounds__(512) void k_pren4_level(const NodeJob* __restrict__ jobs, int P,
-                                                     int C, long cs, const R* __restrict__ jm,
-                                                     const R* dummyX, const int* dummyS,
-                                                     R* dummyW, const short* dummyE,
-                                                     const double* __restrict__ cw,
-                                                     const double* __restrict__ pw,
-                                                     double* partial) {
-    NODE_KERNEL_SETUP
-    NodeOps4<R> S0;
-    const NodeJob J = jobs[blockIdx.y];
-    node_issue<R, AUTO>(J, dummyX, dummyS, dummyE, jm, c, off, doff, p, lane, S0);
-    node_compute<R, AUTO>(J, S0, pa, pb, pea, peb, smat, sdv, k, c, lane, p, P, C, off, doff,
-                          dummyW, dummyEw, cw, w, partial, tile, ntiles);
-}
-#undef NODE_KERNEL_SETUP
-
-// ---------------------------------------------------------------------------
-// FP32 node jobs with the category count C as a template parameter (C = 1,
-// 2, 4).  Same jobs and results as node_compute up to FP32 rounding; the
-// work per job is cut down to the 4x4 products:
-//  - a tip child's products P x and D x are its state's matrix column,
-//    read from the wave's LDS block with the state as the lane's index (the
-//    pad column for a missing state), not 20 products with a one-hot vector;
-//  - categories exchange FP32 values, [2][8 quantities][C][TILE] floats;
-//  - the derivative ratios are formed and summed over the tile's patterns
-//    in FP32 (the tile sums are added in FP64 across tiles), by DPP steps;
-//  - the two waves that form the sums alternate from job to job, so every
-//    SIMD does the same work.
-
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_movf(float v) {
-    return __int_as_float(
-        __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
-}
-
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_movf_rows(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, ROWS, 0xf, false));
-}
-
-// Sum over the 64 lanes, the steps of wave_sum_dpp in FP32.
-__device__ __forceinline__ float wave_sum_f32(float v) {
-    v += dpp_movf<0xB1, 0xf>(v);
-    v += dpp_movf<0x4E, 0xf>(v);
-    v += dpp_movf<0x114, 0xf>(v);
-    v += dpp_movf<0x118, 0xf>(v);
-    v += dpp_movf_rows<0x142, 0xa>(v);
-    v += dpp_movf_rows<0x143, 0xc>(v);
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
-}
-
-// The FP32 node kernels read each matrix of a job column by column from the
-// wave's LDS block: [5 columns][4 rows] (column 4 is the pad column), so a
-// column is one aligned 16-byte read.
-__device__ __forceinline__ fvec4 column(const float* m, int j) {
-    return *(const fvec4*)(m + 4 * j);
-}
-
-// 4x4 products on the matrix cores, lane = pattern: v_mfma_f32_4x4x1f32
-// treats the wave as 16 blocks of 4 lanes; in instruction k lane l supplies
-// A[l % 4][k] and B[k][l % 4] of block l / 4, and element r of lane l's
-// accumulator is D[r][l % 4] of that block.  With B[k][.] = x[k] of the
-// lane's own pattern and A[r][k] = M[r][k] (the same matrix in every block),
-// lane l ends with (M x)[0..3] for its pattern after four instructions.
-// r = lane % 4.
-
-// t = M x: A from row r of M, M[r][k] = column k's entry r
-__device__ __forceinline__ void mfma_rows(const float* m, int r, const float (&x)[4],
-                                          float (&t)[4]) {
-    fvec4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m[4 * k + r], x[k], acc, 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) t[i] = acc[i];
-}
-
-// t = M^T u: A[r][k] = M[k][r], column r of M
-__device__ __forceinline__ void mfma_cols(const float* m, int r, const float (&u)[4],
-                                          float (&t)[4]) {
-    const fvec4 col = column(m, r);
-    fvec4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(col[k], u[k], acc, 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) t[i] = acc[i];
-}
-
-// t = M x for a child: its partials (matrix cores), or for a tip the
-// column of its state (s in [0, 4], a per-lane read; column 4 is the pad
-// column of a missing state).
-__device__ __forceinline__ void child_product(bool tip, int s, int r, const float (&x)[4],
-                                              const float* m, float (&t)[4]) {
-    if (tip) {
-        const fvec4 c = column(m, s);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) t[i] = c[i];
-    } else {
-        mfma_rows(m, r, x, t);
-    }
-}
-
-// Derivative numerators and denominator of one branch for one category:
-// den = pre . x (a tip: pre[state & 3], as the reference computes it),
-// n1 = pre . (D1 x), n2 = pre . (D2 x).
-__device__ __forceinline__ void branch_sums_f32(bool tip, int s, int r, const float (&pre)[4],
-                                                const float (&x)[4], const float* d1,
-                                                const float* d2, float& den, float& n1,
-                                                float& n2) {
-    float t[4];
-    if (tip) {
-        // bit tests, not an index: an indexed pick is lowered to a
-        // private-memory load
-        const float lo = (s & 1) ? pre[1] : pre[0], hi = (s & 1) ? pre[3] : pre[2];
-        den = (s & 2) ? hi : lo;
-    } else {
-        den = dot4(pre, x);
-    }
-    child_product(tip, s, r, x, d1, t);
-    n1 = dot4(pre, t);
-    child_product(tip, s, r, x, d2, t);
-    n2 = dot4(pre, t);
-}
-
-// Operand loads of an FP32 node job without the dummy tile: uniform
-// branches skip what the job does not read (a forwarded parent, the
-// partials of a tip child, the states of a partials child).  Used where no
-// later job's loads are in flight, so the branches cost no extra waits.
-// A register with no particular value and no instruction to set it, for
-// operand slots a job does not load (its uses are selected or branched
-// away by the same job fields that skip the load).
-template <typename T>
-__device__ __forceinline__ T unset() {
-    T x;
-    asm volatile("" : "=v"(x));
-    return x;
-}
-
-template <bool AUTO>
-__device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float* __restrict__ jm,
-                                                  int c, unsigned off, int p, int lane,
-                                                  NodeOps4<float>& o) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        o.u[i] = unset<float>();
-        o.xa[i] = unset<float>();
-        o.xb[i] = unset<float>();
-    }
-    o.sa = unset<int>();
-    o.sb = unset<int>();
-    o.eu = unset<int>();
-    o.ea = unset<int>();
-    o.eb = unset<int>();
-    if (J.kPar == 0) load4(at<float>(J.parent, off), o.u);
-    if (J.k[0] == 0) load4(at<float>(J.x[0], off), o.xa);
-    else o.sa = *at<int>(J.x[0], 4u * p);
-    if (J.k[1] == 0) load4(at<float>(J.x[1], off), o.xb);
-    else o.sb = *at<int>(J.x[1], 4u * p);
-    {
-        const int l = min(lane, NODE_MATS / 4 - 1);
-        o.mv = *at<fvec4>(jm, (unsigned)((J.mbase + c * NODE_MATS + l * 4) * (int)sizeof(float)));
-    }
-    if (AUTO) {
-        if (J.kPar == 0 && J.eParent != nullptr) o.eu = *at<short>(J.eParent, 2u * p);
-        if (J.ex[0] != nullptr) o.ea = *at<short>(J.ex[0], 2u * p);
-        if (J.ex[1] != nullptr) o.eb = *at<short>(J.ex[1], 2u * p);
-    }
-}
-
-template <int C, bool AUTO>
-__device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps4<float>& o,
-                                                 float (&pa)[4], float (&pb)[4], int& pea,
-                                                 int& peb, float* smat, float* sx, int& k, int c,
-                                                 int lane, int p, int P, unsigned off,
-                                                 const float (&wc)[C], float w, double* partial,
-                                                 int tile, int ntiles) {
-    const bool in = p < P;
-    float u[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) u[i] = J.kPar == 2 ? pa[i] : (J.kPar == 3 ? pb[i] : o.u[i]);
-    const int eu = J.kPar == 2 ? pea : (J.kPar == 3 ? peb : o.eu);
-    const bool tipA = J.k[0] == 1, tipB = J.k[1] == 1;
-    // tip states are stored in [0, 4]; the clamp keeps the column index in
-    // the matrix whatever a lane without a tip read
-    const int sa = min((unsigned)o.sa, 4u), sb = min((unsigned)o.sb, 4u);
-    const int par = k & 1;
-    float* lm = smat + (c * 2 + par) * NODE_F32_BLOCK;
-    *(fvec4*)(lm + min(lane, NODE_MATS / 4 - 1) * 4) = o.mv;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-    const int r = lane & 3;
-    float t[4], ua[4], ub[4], va[4], vb[4];
-    child_product(tipA, sa, r, o.xa, lm, t);       // P_a x_a
-#pragma unroll
-    for (int i = 0; i < 4; ++i) ub[i] = u[i] * t[i];
-    child_product(tipB, sb, r, o.xb, lm + 20, t);  // P_b x_b
-#pragma unroll
-    for (int i = 0; i < 4; ++i) ua[i] = u[i] * t[i];
-    mfma_cols(lm, r, ua, va);                      // pre(a) = P_a^T (pre(v) * P_b x_b)
-    mfma_cols(lm + 20, r, ub, vb);                 // pre(b) = P_b^T (pre(v) * P_a x_a)
-    float dA, n1A, n2A, dB, n1B, n2B;
-    branch_sums_f32(tipA, sa, r, va, o.xa, lm + 40, lm + 60, dA, n1A, n2A);
-    branch_sums_f32(tipB, sb, r, vb, o.xb, lm + 80, lm + 100, dB, n1B, n2B);
-    // exchange: max a, max b, den/n1/n2 of a, of b
-    float* buf = sx + par * 8 * C * TILE;
-    ++k;
-    float* mine = buf + c * TILE + lane;
-    mine[0 * C * TILE] = in ? fmaxf(fmaxf(va[0], va[1]), fmaxf(va[2], va[3])) : 0.0f;
-    mine[1 * C * TILE] = in ? fmaxf(fmaxf(vb[0], vb[1]), fmaxf(vb[2], vb[3])) : 0.0f;
-    mine[2 * C * TILE] = dA;
-    mine[3 * C * TILE] = n1A;
-    mine[4 * C * TILE] = n2A;
-    mine[5 * C * TILE] = dB;
-    mine[6 * C * TILE] = n1B;
-    mine[7 * C * TILE] = n2B;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    const bool keepA = in && J.dest[0] != nullptr;
-    const bool keepB = in && J.dest[1] != nullptr;
-    int outA = 0, outB = 0;
-    if (AUTO) {
-        float ma = buf[lane], mb = buf[C * TILE + lane];
-#pragma unroll
-        for (int cc = 1; cc < C; ++cc) {
-            ma = fmaxf(ma, buf[cc * TILE + lane]);
-            mb = fmaxf(mb, buf[(C + cc) * TILE + lane]);
-        }
-        const int ea = ma > 0.0f ? exponent_of(ma) : 0;
-        const int eb = mb > 0.0f ? exponent_of(mb) : 0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            va[i] = pow2(va[i], -ea);
-            vb[i] = pow2(vb[i], -eb);
-        }
-        outA = ea + eu + (J.ex[1] ? o.eb : 0);
-        outB = eb + eu + (J.ex[0] ? o.ea : 0);
-        if (keepA) *at_w<short>(J.ed[0], 2u * p) = (short)outA;
-        if (keepB) *at_w<short>(J.ed[1], 2u * p) = (short)outB;
-    }
-    // tile sums of child ch (w r1, w r1^2, w r2, r = N / D over categories)
-    // by wave (ch + 2 par) % C
-#pragma unroll
-    for (int ch = 0; ch < 2; ++ch) {
-        if (c != (ch + 2 * par) % C) continue;
-        const float* q = buf + (2 + 3 * ch) * C * TILE + lane;
-        float D = wc[0] * q[0], N1 = wc[0] * q[C * TILE], N2 = wc[0] * q[2 * C * TILE];
-#pragma unroll
-        for (int cc = 1; cc < C; ++cc) {
-            D += wc[cc] * q[cc * TILE];
-            N1 += wc[cc] * q[(C + cc) * TILE];
-            N2 += wc[cc] * q[(2 * C + cc) * TILE];
-        }
-        const float inv = 1.0f / D;
-        const float r1 = in ? N1 * inv : 0.0f, r2 = in ? N2 * inv : 0.0f;
-        const float a = w * r1;
-        const float s1 = wave_sum_f32(a), sq = wave_sum_f32(a * r1), s2 = wave_sum_f32(w * r2);
-        if (lane == 0) {
-            GLOBAL double* pt = gout<double>(partial);
-            const long row = (long)J.oidx[ch] * 3;
-            pt[(row + 0) * ntiles + tile] = (double)s1;
-            pt[(row + 1) * ntiles + tile] = (double)sq;
-            pt[(row + 2) * ntiles + tile] = (double)s2;
-        }
-    }
-    if (keepA) store4(at_w<float>(J.dest[0], off), va);
-    if (keepB) store4(at_w<float>(J.dest[1], off), vb);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        pa[i] = va[i];
-        pb[i] = vb[i];
-    }
-    pea = outA;
-    peb = outB;
-}
-
-#define NODE_F32_SETUP                                                                        \
-    extern __shared__ double sdv[]; /* [2][8][C][TILE] floats, then [C][2] matrix blocks */   \
-    static_assert(NODE_MATS <= NODE_F32_BLOCK, "matrix block");                               \
-    float* sx = reinterpret_cast<float*>(sdv);                                                \
-    float* smat = sx + 2 * 8 * C * TILE;                                                      \
-    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);                          \
-    const int lane = threadIdx.x & 63;                                                        \
-    const int tile = blockIdx.x;                                                              \
-    const int p = tile * TILE + lane; /* < Ppad: pattern weights are zero-padded */           \
-    const unsigned off = (unsigned)((c * cs + (long)p * 4) * (long)sizeof(float));            \
-    const int ntiles = (P + TILE - 1) / TILE;                                                 \
-    const float w = (float)pw[p];                                                             \
-    float wc[C];                                                                              \
-    _Pragma("unroll") for (int cc = 0; cc < C; ++cc) wc[cc] = __int_as_float(                 \
-        __builtin_amdgcn_readfirstlane(__float_as_int((float)cw[cc])));                       \
-    int k = 0, pea = 0, peb = 0;                                                              \
-    float pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
-
-// Each job's operands are loaded when the job starts (node_issue_direct):
-// at most 64 VGPRs, so eight waves per SIMD and a 100,000-pattern grid
-// resident at once, which covers the loads' latency better than loading
-// the next job ahead in more registers does.
-template <int C, bool AUTO>
-__global__ __launch_bounds__(64 * C, 7) void k_pren4f(const NodeJob* __restrict__ jobs, int njobs,
-                                                      int chunk, int P, long cs,
-                                                      const float* __restrict__ jm,
-                                                      const double* __restrict__ cw,
-                                                      const double* __restrict__ pw,
-                                                      double* partial) {
-    NODE_F32_SETUP
-    jobs += blockIdx.y * chunk;
-    njobs = min(chunk, njobs - (int)blockIdx.y * chunk);
-    for (int o = 0; o < njobs; ++o) {
-        const NodeJob J = jobs[o];
-        NodeOps4<float> S0;
-        node_issue_direct<AUTO>(J, jm, c, off, p, lane, S0);
-        node_compute_f32<C, AUTO>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, wc, w,
-                                  partial, tile, ntiles);
-    }
-}
-
-template <int C, bool AUTO>
-__global__ __launch_bounds__(64 * C) void k_pren4f_level(const NodeJob* __restrict__ jobs,
-                                                         int P, long cs,
-                                                         const float* __restrict__ jm,
-                                                         const double* __restrict__ cw,
-                                                         const double* __restrict__ pw,
-                                                         double* partial) {
-    NODE_F32_SETUP
-    NodeOps4<float> S0;
-    const NodeJob J = jobs[blockIdx.y];
-    node_issue_direct<AUTO>(J, jm, c, off, p, lane, S0);
-    node_compute_f32<C, AUTO>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, wc, w,
-                              partial, tile, ntiles);
-}
-#undef NODE_F32_SETUP
-
-template <int C>
-static void launch_pre_nodes_f32(const NodeJob* djobs, int njobs, bool batched, int chunk,
-                                 const Dims& d, const float* m, const double* cw,
-                                 const double* pw, double* partial, hipStream_t st) {
-    const int tiles = (d.P + TILE - 1) / TILE;
-    const size_t lds = (size_t)(2 * 8 * C * TILE + C * 2 * NODE_F32_BLOCK) * sizeof(float);
-    const dim3 block(TILE * C);
-    if (batched && chunk <= 1) {
-        if (d.autoScale) hipLaunchKernelGGL((k_pren4f_level<C, true>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial);
-        else hipLaunchKernelGGL((k_pren4f_level<C, false>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial);
-    } else {
-        if (!batched || chunk > njobs) chunk = njobs;
-        const dim3 grid(tiles, (njobs + chunk - 1) / chunk);
-        if (d.autoScale) hipLaunchKernelGGL((k_pren4f<C, true>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial);
-        else hipLaunchKernelGGL((k_pren4f<C, false>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Rate-matrix gradient (4 states).  For one eigen-system Q = V diag(lam) V^-1
-// and tau = r_c t, the Frechet derivative of P = exp(tau Q) is
-//   dP = V (F o (V^-1 dQ V)) V^-1,  F_kl = (e^{tau lam_k} - e^{tau lam_l}) /
-//        (lam_k - lam_l)  (tau e^{tau lam_k} when lam_k = lam_l),
-// so dlogL/dQ = V^-T H V^T with
-//   H = sum over branches, categories c and patterns p of
-//       w_c (w_p / L_p) F o ((V^T u)(V^-1 x)^T),
-// u the branch's top vector pre(v) * (P_sib x_sib) and x its post-order
-// operand.  A node job v (children a, b) gives both branches: u_a = pre(v)
-// * P_b x_b, u_b = pre(v) * P_a x_a, and L_p = sum_c w_c sum_i pre(v)_i
-// (P_a x_a)_i (P_b x_b)_i.  Each lane accumulates H (FP64) for its pattern
-// and category over every job; scale factors cancel in u x^T / L_p.  The
-// jobs are independent (all pre-order partials are computed), so a
-// workgroup walks the list with the next job's loads in flight.
-
-template <typename R>
-struct QOps4 {
-    R u[4], xa[4], xb[4];
-    int sa, sb;
-    typename Vec16<R>::T mv;   // this lane's share of the job's matrices
-    dvec2 fv;                  // this lane's share of the job's F values
-};
-
-template <typename R>
-__device__ __forceinline__ void q_issue(const NodeJob& J, int jidx, const void* dx, const void* ds,
-                                        const R* __restrict__ jm, const double* __restrict__ jf,
-                                        int c, int C, unsigned off, unsigned doff, int p, int lane,
-                                        QOps4<R>& o) {
-    const bool pa = J.k[0] == 0, pb = J.k[1] == 0;
-    load4(at<R>(J.parent, off), o.u);
-    load4_range<R>(range_if(J.x[0], pa, jm), off, o.xa);
-    load4_range<R>(range_if(J.x[1], pb, jm), off, o.xb);
-    o.sa = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[0], !pa, jm), 4u * p, 0, 0);
-    o.sb = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[1], !pb, jm), 4u * p, 0, 0);
-    constexpr int V = Vec16<R>::N;
-    const int l = min(lane, NODE_MATS / V - 1);
-    o.mv = *at<typename Vec16<R>::T>(jm, (unsigned)((J.mbase + c * NODE_MATS + l * V) *
-                                                    (int)sizeof(R)));
-    // lanes 0-7: child a's 16 factors of category c, lanes 8-15: child b's
-    const int fl = min(lane, 15);
-    o.fv = *at<dvec2>(jf, (unsigned)((J.mF[fl >> 3] + c * 16 + 2 * (fl & 7)) * 8));
-}
-
-template <typename R>
-__device__ __forceinline__ void q_compute(const NodeJob& J, const QOps4<R>& o, double (&H)[16],
-                                          R* smat, double* sfb, double* sx,
-                                          const double* sV, int& k, int c, int lane, int p,
-                                          int P, int C, const double* __restrict__ cw, double w) {
-    const bool in = p < P;
-    R xa[5], xb[5], ya[4], yb[4];
-    child5(J.k[0], o.sa, o.xa, xa);
-    child5(J.k[1], o.sb, o.xb, xb);
-    R* lm = smat + (c * 2 + (k & 1)) * (NODE_MAT_BLOCK / (int)sizeof(R));
-    double* lf = sfb + (c * 2 + (k & 1)) * 32;
-    *(typename Vec16<R>::T*)(lm + lane * Vec16<R>::N) = o.mv;
-    *(dvec2*)(lf + 2 * min(lane, 15)) = o.fv;   // lanes past 15 repeat lane 15's pair
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-    mat4(lm, xa, ya);        // P_a x_a
-    mat4(lm + 20, xb, yb);   // P_b x_b
-    double uab[4], dc = 0.0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        uab[i] = (double)o.u[i] * (double)ya[i] * (double)yb[i];
-        dc += uab[i];
-    }
-    // L_p: category sums exchanged through LDS
-    double* buf = sx + (k & 1) * C * TILE;
-    ++k;
-    buf[c * TILE + lane] = cw[c] * dc;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    double D = 0.0;
-    for (int cc = 0; cc < C; ++cc) D += buf[cc * TILE + lane];
-    const double coef = in ? cw[c] * w / D : 0.0;
-#pragma unroll
-    for (int ch = 0; ch < 2; ++ch) {
-        const R(&x)[5] = ch == 0 ? xa : xb;
-        const R(&y)[4] = ch == 0 ? yb : ya;   // the sibling's P x
-        double top[4], xv[4], al[4], be[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            top[i] = (double)o.u[i] * (double)y[i];
-            // a missing tip state reads the pad column, which does not
-            // depend on Q: its one-hot part is zero, so it adds nothing
-            xv[i] = (double)x[i];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            al[q] = coef * (sV[0 * 4 + q] * top[0] + sV[1 * 4 + q] * top[1] +
-                            sV[2 * 4 + q] * top[2] + sV[3 * 4 + q] * top[3]);
-            be[q] = sV[16 + q * 4 + 0] * xv[0] + sV[16 + q * 4 + 1] * xv[1] +
-                    sV[16 + q * 4 + 2] * xv[2] + sV[16 + q * 4 + 3] * xv[3];
-        }
-        const double* F = lf + ch * 16;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) H[q * 4 + r] += F[q * 4 + r] * al[q] * be[r];
-    }
-}
-
-template <typename R>
-__global__ __launch_bounds__(512) void k_qgrad4(const NodeJob* __restrict__ jobs, int njobs, int P,
-                                                int C, long cs, const R* __restrict__ jm,
-                                                const double* __restrict__ jf,
-                                                const double* __restrict__ eig,
-                                                const R* dummyX, const int* dummyS,
-                                                const double* __restrict__ cw,
-                                                const double* __restrict__ pw, double* partial) {
-    // LDS: [2][C][TILE] exchange, V | V^-1 (32), [C][2][32] F blocks, then
-    // [C][2] matrix blocks
-    extern __shared__ double sx[];
-    double* sV = sx + 2 * C * TILE;
-    double* sfb = sV + 32;
-    R* smat = reinterpret_cast<R*>(sfb + C * 2 * 32);
-    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x;
-    const int p = tile * TILE + lane;
-    if (threadIdx.x < 32) sV[threadIdx.x] = eig[threadIdx.x];
-    __syncthreads();
-    const unsigned off = (unsigned)((c * cs + (long)p * 4) * (long)sizeof(R));
-    const unsigned doff = (unsigned)((c * TILE + lane) * 4 * (int)sizeof(R));
-    const double w = pw[p];
-    double H[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) H[q] = 0.0;
-    int k = 0;
-    QOps4<R> S0, S1;
-    NodeJob J0 = jobs[0], J1;
-    q_issue<R>(J0, 0, dummyX, dummyS, jm, jf, c, C, off, doff, p, lane, S0);
-    for (int o = 0;; o += 2) {
-        const int o1 = min(o + 1, njobs - 1);
-        J1 = jobs[o1];
-        q_issue<R>(J1, o1, dummyX, dummyS, jm, jf, c, C, off, doff, p, lane, S1);
-        q_compute<R>(J0, S0, H, smat, sfb, sx, sV, k, c, lane, p, P, C, cw, w);
-        if (o + 1 >= njobs) break;
-        const int o2 = min(o + 2, njobs - 1);
-        J0 = jobs[o2];
-        q_issue<R>(J0, o2, dummyX, dummyS, jm, jf, c, C, off, doff, p, lane, S0);
-        q_compute<R>(J1, S1, H, smat, sfb, sx, sV, k, c, lane, p, P, C, cw, w);
-        if (o + 2 >= njobs) break;
-    }
-    // per-tile, per-category sums of H: rows kl of [16][tiles * C]
-    const int ntiles = gridDim.x;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const double s = wave_sum_dpp(H[q]);
-        if (lane == 0) partial[(long)q * ntiles * C + (long)tile * C + c] = s;
-    }
-}
-
-template <typename R>
-void launch_qgrad(const NodeJob* djobs, int njobs, const Dims& d, const void* jobMats,
-                  const double* jobF, const double* eig, void* dummy, const double* cw,
-                  const double* pw, double* partial, hipStream_t st) {
-    if (njobs <= 0) return;
-    const R* dummyX = reinterpret_cast<const R*>(dummy);
-    R* dummyW = const_cast<R*>(dummyX) + 8 * TILE * 4;
-    const double* dummyD = reinterpret_cast<const double*>(dummyW + 8 * TILE * 4);
-    const int* dummyS = reinterpret_cast<const int*>(dummyD + TILE);
-    const int tiles = (d.P + TILE - 1) / TILE;
-    const size_t lds = (size_t)(2 * d.C * TILE + 32 + d.C * 2 * 32) * sizeof(double) +
-                       (size_t)d.C * 2 * NODE_MAT_BLOCK;
-    hipLaunchKernelGGL((k_qgrad4<R>), dim3(tiles), dim3(TILE * d.C), lds, st, djobs, njobs, d.P,
-                       d.C, d.catStride, reinterpret_cast<const R*>(jobMats), jobF, eig, dummyX,
-                       dummyS, cw, pw, partial);
-}
-
-// dlogL/dQ = V^-T H V^T (one wave), the root-frequency sums copied after it.
-__global__ __launch_bounds__(64) void k_qgrad_finish(const double* __restrict__ sums,
-                                                     const double* __restrict__ eig, int nf,
-                                                     double* __restrict__ out) {
-    const int t = threadIdx.x;
-    if (t < 16) {
-        const int a = t >> 2, b = t & 3;
-        const double* V = eig;
-        const double* Vi = eig + 16;
-        double g = 0.0;
-        for (int k = 0; k < 4; ++k)
-            for (int l = 0; l < 4; ++l) g += Vi[k * 4 + a] * sums[k * 4 + l] * V[b * 4 + l];
-        out[t] = g;
-    } else if (t < 16 + nf) {
-        out[t] = sums[t];
-    }
-}
-
-void launch_qgrad_finish(const double* sums, const double* eig, int nf, double* out,
-                         hipStream_t st) {
-    hipLaunchKernelGGL(k_qgrad_finish, dim3(1), dim3(64), 0, st, sums, eig, nf, out);
-}
-
-// dlogL/dpi_i through the root distribution: sum_p w_p (sum_c w_c R_cpi) /
-// (sum_i pi_i sum_c w_c R_cpi), one thread per pattern, block sums per i
-// (rows of [S][blocks]).  Scale factors of the root partials cancel.
-template <typename R>
-__global__ __launch_bounds__(WG) void k_root_freq_grad(const R* __restrict__ root,
-                                                        const double* __restrict__ cw,
-                                                        const double* __restrict__ freqs,
-                                                        const double* __restrict__ pw,
-                                                        double* __restrict__ partial, Dims d) {
-    __shared__ double sh[WG / 64];
-    const int p = blockIdx.x * WG + threadIdx.x;
-    const int nb = gridDim.x;
-    double L = 0.0;
-    if (p < d.P) {
-        for (int i = 0; i < d.S; ++i) {
-            double g = 0.0;
-            for (int c = 0; c < d.C; ++c) g += cw[c] * (double)root[pbase(d, c, p) + (long)i * d.sStride];
-            L += freqs[i] * g;
-        }
-    }
-    for (int i = 0; i < d.S; ++i) {
-        double v = 0.0;
-        if (p < d.P) {
-            double g = 0.0;
-            for (int c = 0; c < d.C; ++c) g += cw[c] * (double)root[pbase(d, c, p) + (long)i * d.sStride];
-            v = pw[p] * g / L;
-        }
-        const double s = block_sum(v, sh);
-        if (threadIdx.x == 0) partial[(long)i * nb + blockIdx.x] = s;
-    }
-}
-
-template <typename R>
-void launch_root_freq_grad(const void* root, const double* cw, const double* freqs,
-                           const double* pw, double* partial, const Dims& d, hipStream_t st) {
-    const int nb = (d.P + WG - 1) / WG;
-    hipLaunchKernelGGL(k_root_freq_grad<R>, dim3(nb), dim3(WG), 0, st,
-                       reinterpret_cast<const R*>(root), cw, freqs, pw, partial, d);
-}
 
 // ---------------------------------------------------------------------------
 // dlogL/dQ for any state count up to 64, one pre-order operation (branch)
@@ -2377,71 +1341,6 @@ void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, 
             else PREL(false, false);
         }
 #undef PREL
-    }
-}
-
-template <typename R>
-__global__ __launch_bounds__(WG) void k_gather_node_mats(const NodeJob* __restrict__ jobs,
-                                                         int njobs, int C,
-                                                         const R* __restrict__ mats,
-                                                         R* __restrict__ out, bool colMajor) {
-    const long t = (long)blockIdx.x * WG + threadIdx.x;
-    const long per = (long)C * NODE_MATS;
-    if (t >= per * njobs) return;
-    const int j = (int)(t / per);
-    const int rem = (int)(t - (long)j * per);
-    const int c = rem / NODE_MATS, r = rem - c * NODE_MATS;
-    const int m = r / 20, e = r - m * 20;
-    // row-major [4][5] as stored, or column-major [5][4]
-    const int d = colMajor ? m * 20 + (e % 5) * 4 + e / 5 : r;
-    const NodeJob& J = jobs[j];
-    const int mo = m == 0 ? J.moP[0] : m == 1 ? J.moP[1] : m == 2 ? J.moD1[0]
-                 : m == 3 ? J.moD2[0] : m == 4 ? J.moD1[1] : J.moD2[1];
-    out[(long)J.mbase + c * NODE_MATS + d] = mats[(long)mo + c * 20 + e];
-}
-
-template <typename R>
-void launch_gather_node_matrices(const NodeJob* djobs, int njobs, const Dims& d,
-                                 const void* mats, void* jobMats, hipStream_t st, bool colMajor) {
-    const long n = (long)njobs * d.C * NODE_MATS;
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_gather_node_mats<R>, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, st,
-                       djobs, njobs, d.C, reinterpret_cast<const R*>(mats),
-                       reinterpret_cast<R*>(jobMats), colMajor);
-}
-
-template <typename R>
-void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, const Dims& d,
-                      const void* jobMats, void* dummy, const double* cw, const double* pw,
-                      double* partial, hipStream_t st) {
-    if (njobs <= 0) return;
-    const R* dummyX = reinterpret_cast<const R*>(dummy);
-    R* dummyW = const_cast<R*>(dummyX) + 8 * TILE * 4;
-    const double* dummyD = reinterpret_cast<const double*>(dummyW + 8 * TILE * 4);
-    const int* dummyS = reinterpret_cast<const int*>(dummyD + TILE);
-    const short* dummyE = reinterpret_cast<const short*>(dummyS + TILE);
-    const R* m = reinterpret_cast<const R*>(jobMats);
-    if (std::is_same<R, float>::value && node_f32_supported(d.C)) {
-        const float* mf = reinterpret_cast<const float*>(m);
-        if (d.C == 4) launch_pre_nodes_f32<4>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st);
-        else if (d.C == 2) launch_pre_nodes_f32<2>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st);
-        else launch_pre_nodes_f32<1>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st);
-        return;
-    }
-    const int tiles = (d.P + TILE - 1) / TILE;
-    const size_t lds = (size_t)2 * d.C * 8 * TILE * sizeof(double) +
-                       (size_t)d.C * 2 * NODE_MAT_BLOCK;
-    dim3 block(TILE * d.C);
-    if (batched && chunk <= 1) {
-        if (d.autoScale) hipLaunchKernelGGL((k_pren4_level<R, true>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
-        else hipLaunchKernelGGL((k_pren4_level<R, false>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
-    } else {
-        // a fused list (chunk = the whole list) or a batched level walked in
-        // chunks of independent jobs
-        if (!batched || chunk > njobs) chunk = njobs;
-        const dim3 grid(tiles, (njobs + chunk - 1) / chunk);
-        if (d.autoScale) hipLaunchKernelGGL((k_pren4<R, true>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
-        else hipLaunchKernelGGL((k_pren4<R, false>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.C, d.catStride, m, dummyX, dummyS, dummyW, dummyE, cw, pw, partial);
     }
 }
 
@@ -3282,13 +2181,6 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
     template void launch_pre<R>(const PreJob*, const PreJob*, int, const Dims&, int, void*,      \
                                 const void*, void*, const double*, const double*, double*,       \
                                 hipStream_t);                                                    \
-    template void launch_gather_node_matrices<R>(const NodeJob*, int, const Dims&, const void*, \
-                                                 void*, hipStream_t, bool);                  \
-    template void launch_qgrad<R>(const NodeJob*, int, const Dims&, const void*, const double*,  \
-                                  const double*, void*, const double*, const double*, double*,  \
-                                  hipStream_t);                                                 \
-    template void launch_root_freq_grad<R>(const void*, const double*, const double*,          \
-                                           const double*, double*, const Dims&, hipStream_t);   \
     template void launch_root_multi<R>(const RootSubset*, int, const double*, double*, double*,  \
                                        const Dims&, hipStream_t);                               \
     template void launch_root_per_category<R>(const void*, const double*, const double*,        \
@@ -3296,9 +2188,6 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
     template void launch_qgrad_generic<R>(const PreJob*, int, const Dims&, const double*,       \
                                           const double*, const double*, const double*, double*,  \
                                           double*, int, double*, hipStream_t);                  \
-    template void launch_pre_nodes<R>(const NodeJob*, int, bool, int, const Dims&, const void*, \
-                                      void*, const double*, const double*, double*,         \
-                                      hipStream_t);                                         \
     template void launch_pre_levels<R>(const PreJob*, const int*, int, const Dims&, const void*, \
                                        void*, const double*, const double*, double*,             \
                                        hipStream_t);                                             \

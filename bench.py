@@ -57,6 +57,11 @@ sys.path.insert(0, os.path.join(REPO, "beagle-lib_amd"))
 HBM_PEAK_GBS = 8000.0
 N_TAXA, N_PATTERNS, SEED = 128, 100_000, 1
 SHAPES = ("caterpillar", "balanced")
+# the adjoint workload the metric is quoted on: 4-state GTR + Gamma(4),
+# a 512-taxon Yule tree, 50,000 patterns
+ADJ = dict(taxa=512, patterns=50_000, shape="yule")
+# the plain FP64 pruning workload: 16 taxa, 10,000 patterns, root logL only
+PLAIN = dict(taxa=16, patterns=10_000, shape="yule")
 METRIC = "logL+gradient evals/s & partials GB/s vs plain FP64, at ≤1e-6 rel err"
 SHORT = 2   # bytes of a power-of-two scale exponent
 
@@ -120,6 +125,126 @@ def problems(patterns=N_PATTERNS):
     return [synth.make_problem(SEED, N_TAXA, patterns, S=4, shape=s) for s in SHAPES]
 
 
+def adjoint_problem(patterns=None):
+    from beagle_amd import synth
+    return synth.make_problem(SEED, ADJ["taxa"], patterns or ADJ["patterns"], S=4,
+                              shape=ADJ["shape"])
+
+
+def full_gradient_bench(lib, steps, warmup):
+    """The metric's own workload: one evaluation = logL, dlogL/dt and
+    d2logL/dt2 of all 1,022 branches, dlogL/dQ (4 x 4) and dlogL/dpi_root
+    of the 512-taxon tree (matrices + post-order + root + fused pre-order
+    and derivative pass + rate-matrix and root-frequency gradients).
+
+    value path: FP32 partials, power-of-two rescaling, the evaluation
+    captured once as a HIP graph and replayed with its branch lengths set
+    before every replay, one evaluation after the other (a gradient-based
+    sampler's chain: each evaluation's result is collected before the next
+    is launched).  Plain path: the FP64 beagle.h calls as a client makes
+    them (manual scale buffers; post-order, root, beagleUpdatePrePartials,
+    two beagleCalculateEdgeDerivatives) plus the FP64 rate-matrix gradient,
+    every call waiting for its result.  Errors of the value path are taken
+    against the plain path."""
+    import numpy as np
+    import torch
+    from beagle_amd import api, driver
+    prob = adjoint_problem()
+    asyn = api.FLAG_COMPUTATION_ASYNCH
+    ev = driver.Evaluator(lib, prob, precision="single", scaling="auto", extra_flags=asyn)
+    lengths = prob.tree.lengths
+    t0, w = time.perf_counter(), 0
+    while w < warmup or time.perf_counter() - t0 < 1.0:
+        ev.fused_gradient(rate_matrix=True)
+        w += 1
+    torch.cuda.synchronize()
+    ne = max(steps // 2, 5)
+    t0 = time.perf_counter()
+    for _ in range(ne):
+        eager = ev.fused_gradient(rate_matrix=True)
+    torch.cuda.synchronize()
+    eager_rate = ne / (time.perf_counter() - t0)
+    ev.capture_fused_gradient(rate_matrix=True)
+    for _ in range(max(warmup, 3)):
+        ev.replay_fused_gradient(lengths)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res32 = ev.replay_fused_gradient(lengths)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    identical = all(np.array_equal(np.asarray(a), np.asarray(b)) for a, b in zip(res32, eager))
+    phases = phase_timing(ev, steps)
+    ops, post, _, _ = ev.fused_operations()
+    post_idx = dict(zip(map(tuple, ops.tolist()), post.tolist()))
+    pre_bytes = preorder_bytes(prob, ops, lambda o: post_idx[tuple(o.tolist())], 4, True)
+    post_bytes = postorder_bytes(prob, 4, True)
+    ev.close()
+
+    ev = driver.Evaluator(lib, prob, scaling="manual")
+
+    def plain():
+        lnl, g1, g2 = ev.gradient()
+        q, pi = ev.rate_matrix_gradient()
+        return lnl, g1, g2, q, pi
+
+    for _ in range(2):
+        plain()
+    n64 = max(steps // 4, 3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n64):
+        res64 = plain()
+    plain_rate = n64 / (time.perf_counter() - t0)
+    ev.close()
+    value = steps / dt
+    err = {"logL": rel(res32[0], res64[0]), "dlogL_dt": grad_err(res32[1], res64[1]),
+           "d2logL_dt2": grad_err(res32[2], res64[2]), "dlogL_dQ": grad_err(res32[3], res64[3]),
+           "dlogL_dpi": grad_err(res32[4], res64[4])}
+    return {"value": value, "ms_per_eval": dt * 1e3 / steps, "eager_evals_per_s": eager_rate,
+            "graph_results_identical_to_eager": bool(identical), "error_vs_fp64": err,
+            "phases_ms": phases, "preorder_bytes": pre_bytes, "postorder_bytes": post_bytes,
+            "plain_fp64": {"evals_per_s": plain_rate,
+                           "what": "FP64 beagle.h calls with manual scale buffers: "
+                                   "beagleUpdateTransitionMatrices, beagleResetScaleFactors, "
+                                   "beagleUpdatePartials, beagleCalculateRootLogLikelihoods, "
+                                   "beagleSetRootPrePartials, beagleUpdatePrePartials, two "
+                                   "beagleCalculateEdgeDerivatives, then "
+                                   "beagleExtCalculateRateMatrixGradient in FP64; each call "
+                                   "waits for its result",
+                           "speedup_of_value": value / plain_rate},
+            "logL": res64[0]}
+
+
+def phase_timing(ev, steps):
+    """Device time of each phase of one evaluation (HIP events on the
+    instance's stream, ASYNCH instance, averaged over `steps` evaluations):
+    matrices, post-order, root + fused pre-order/derivative pass, rate-matrix
+    and root-frequency gradient."""
+    import numpy as np
+    import torch
+    st = torch.cuda.ExternalStream(ev.inst.stream())
+    ops, post, d1, d2 = ev.fused_operations()
+    names = ("matrices", "postorder", "root_preorder_derivatives", "rate_matrix_gradient")
+    acc = {k: [] for k in names}
+    root = ev.prob.tree.root
+    for _ in range(max(steps // 2, 5)):
+        ev_ = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        ev_[0].record(st)
+        ev.update_matrices()
+        ev_[1].record(st)
+        ev.post_order()
+        ev_[2].record(st)
+        ev.inst.log_likelihood_and_derivatives(root, ev.pre_base + root, ops, post, d1, d2)
+        ev_[3].record(st)
+        ev.rate_matrix_gradient()
+        ev_[4].record(st)
+        torch.cuda.synchronize()
+        for k, name in enumerate(names):
+            acc[name].append(ev_[k].elapsed_time(ev_[k + 1]))
+    return {k: float(np.median(v)) for k, v in acc.items()}
+
+
 def fused_eval(ev):
     """One logL + gradient evaluation through the library's extensions:
     matrices, post-order, then one call for the root logL, the root
@@ -127,54 +252,128 @@ def fused_eval(ev):
     return ev.fused_gradient()
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """Cores this process may use: the affinity mask, capped by the job's
+    thread allowance (OMP_NUM_THREADS) where one is set."""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else n
+
+
 def cpu_baseline(budget_s):
-    """The reference CPU plugin (threaded SSE, double) on this host's cores:
-    logL + gradient evaluations of the balanced tree through the beagle.h
-    calls (post-order, root, beagleUpdatePrePartials, two
-    beagleCalculateEdgeDerivatives) on a `sample`-pattern alignment from the
-    same generators, repeated for about budget_s seconds, scaled to 100,000
-    patterns (the work is linear in the pattern count).  Falls back to the
-    oracle (one thread) without the reference build.  Runs in its own
-    process (both libraries export beagle*)."""
+    """The reference CPU plugin (CPU-4State-SSE-Double, threaded) on this
+    host's cores, on the adjoint workload's generators (512-taxon Yule tree,
+    4-state GTR + Gamma(4)) at a pattern sample sized to the budget: one
+    evaluation = the post-order + root phase (beagleUpdateTransitionMatrices,
+    beagleUpdatePartials with manual scale buffers,
+    beagleCalculateRootLogLikelihoods) and the adjoint phase
+    (beagleUpdatePrePartials, beagleCalculateEdgeDerivatives with r_c Q and
+    r_c^2 Q^2, beagleCalculateCrossProductDerivative, the reference's
+    substitution-parameter gradient).  1 thread and all cores
+    (beagleSetCPUThreadCount; the reference runs the adjoint phase on one
+    thread either way), 2 warm-ups, then at least 10 evaluations per thread
+    count: median and best.  Rates are scaled to 50,000 patterns (the work is
+    linear in the pattern count).  Falls back to the oracle (one thread)
+    without the reference build.  Runs in its own process (both libraries
+    export beagle*)."""
+    import numpy as np
     from beagle_amd import api, driver
     ref_path = os.path.join(REPO, "oracle", "_ref", "libhmsbeagle.so")
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    have_ref = os.path.exists(ref_path)
+    cores = host_cores() if have_ref else 1
 
-    def make(patterns):
-        prob = problems(patterns)[1]
-        if os.path.exists(ref_path):
+    def make(patterns, threads):
+        prob = adjoint_problem(patterns)
+        if have_ref:
             lib = api.Library(ref_path, "beagle")
             flags = api.FLAG_PROCESSOR_CPU | api.FLAG_VECTOR_SSE | api.FLAG_THREADING_CPP
-            ev = driver.Evaluator(lib, prob, extra_flags=flags, transpose="none")
-            lib.call("SetCPUThreadCount", ev.inst.h, cores)
-            return ev, "reference", ev.inst.impl_name, cores
+            ev = driver.Evaluator(lib, prob, extra_flags=flags, transpose="none",
+                                  scaling="manual")
+            lib.call("SetCPUThreadCount", ev.inst.h, threads)
+            return ev
         lib = api.Library(os.path.join(REPO, "oracle", "liboracle.so"), "orc")
-        return driver.Evaluator(lib, prob, transpose="none"), "port", "oracle", 1
+        return driver.Evaluator(lib, prob, transpose="none", scaling="manual")
 
-    # size the sample from one small evaluation
-    ev, kind, name, used = make(1000)
-    t0 = time.perf_counter()
-    ev.gradient()
-    per_pattern = (time.perf_counter() - t0) / 1000
+    def phases(ev):
+        t0 = time.perf_counter()
+        ev.log_likelihood()
+        t1 = time.perf_counter()
+        ev.pre_order()
+        ev.edge_derivatives(False, per_site=False)
+        ev.edge_derivatives(True, per_site=False)
+        ev.cross_products()
+        return t1 - t0, time.perf_counter() - t1
+
+    # size the sample from one small single-thread evaluation: the
+    # single-thread run takes about half the budget
+    ev = make(500, 1)
+    phases(ev)
+    per_pattern = sum(phases(ev)) / 500
     ev.close()
-    sample = int(min(N_PATTERNS, max(1000, budget_s / 3 / per_pattern)) // 1000 * 1000)
-    ev, kind, name, used = make(sample)
-    ev.gradient()
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        ev.gradient()
-        n += 1
-        if time.perf_counter() - t0 > budget_s and n >= 2:
-            break
-    dt = time.perf_counter() - t0
-    ev.close()
-    rate = n / dt * sample / N_PATTERNS
-    return {"value": rate, "unit": "evals/s", "cores": used, "kind": kind,
-            "sample": f"{n} FP64 logL+gradient evaluations of the balanced 128-taxon tree, "
-                      f"{sample}-pattern alignment from the same generators, {name} ({dt:.1f} s; "
-                      f"threads set with beagleSetCPUThreadCount, the pre-order and "
-                      f"derivative calls run on one), rate scaled by {sample}/100000"}
+    sample = int(max(500, min(ADJ["patterns"], budget_s / 2 / 12 / per_pattern)) // 100 * 100)
+    scale = sample / ADJ["patterns"]
+    runs = {}
+    for threads in sorted({1, cores}):
+        ev = make(sample, threads)
+        for _ in range(2):
+            phases(ev)
+        t = [phases(ev) for _ in range(10)]
+        ev.close()
+        tot = [a + b for a, b in t]
+        runs[threads] = {
+            "threads": threads,
+            "evals_per_s_median": scale / float(np.median(tot)),
+            "evals_per_s_best": scale / float(np.min(tot)),
+            "postorder_root_ms_median": float(np.median([a for a, _ in t])) / scale * 1e3,
+            "adjoint_ms_median": float(np.median([b for _, b in t])) / scale * 1e3}
+    best = runs[cores]
+    return {"value": best["evals_per_s_median"], "unit": "evals/s", "cores": cores,
+            "kind": "reference" if have_ref else "port",
+            "sample": f"the adjoint workload's generators at {sample} patterns (512-taxon Yule "
+                      f"tree), {'CPU-4State-SSE-Double' if have_ref else 'the oracle'}, FP64 with "
+                      f"manual scale buffers; 2 warm-ups + 10 timed evaluations per thread "
+                      f"count; rates scaled by {sample}/{ADJ['patterns']}",
+            "cpu_model": cpu_model(), "host_cpus_in_affinity": len(os.sched_getaffinity(0)),
+            "one_thread": runs[1], "all_cores": best}
+
+
+def cpu_baseline_plain(budget_s):
+    """The plain FP64 pruning workload (16 taxa, 10,000 patterns, root logL
+    only) on the reference CPU plugin, 1 thread and all cores: median
+    evaluations/s over about budget_s seconds each."""
+    import numpy as np
+    from beagle_amd import api, driver, synth
+    ref_path = os.path.join(REPO, "oracle", "_ref", "libhmsbeagle.so")
+    if not os.path.exists(ref_path):
+        return None
+    prob = synth.make_problem(SEED, PLAIN["taxa"], PLAIN["patterns"], S=4, shape=PLAIN["shape"])
+    out = {}
+    for threads in sorted({1, host_cores()}):
+        lib = api.Library(ref_path, "beagle")
+        flags = api.FLAG_PROCESSOR_CPU | api.FLAG_VECTOR_SSE | api.FLAG_THREADING_CPP
+        ev = driver.Evaluator(lib, prob, extra_flags=flags, transpose="none", preorder=False)
+        lib.call("SetCPUThreadCount", ev.inst.h, threads)
+        for _ in range(3):
+            ev.log_likelihood()
+        t, t_end = [], time.perf_counter() + budget_s
+        while time.perf_counter() < t_end or len(t) < 10:
+            t0 = time.perf_counter()
+            ev.log_likelihood()
+            t.append(time.perf_counter() - t0)
+        ev.close()
+        out[str(threads)] = {"evals_per_s_median": 1.0 / float(np.median(t)),
+                             "evals_per_s_best": 1.0 / float(np.min(t))}
+    return out
 
 
 CODON = dict(S=61, taxa=64, patterns=20_000, shape="yule")
@@ -371,7 +570,7 @@ def cpu_baseline_codon(budget_s):
     ref_path = os.path.join(REPO, "oracle", "_ref", "libhmsbeagle.so")
     if not os.path.exists(ref_path):
         return None
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = host_cores()
     sample = 1000
     lib = api.Library(ref_path, "beagle")
     flags = api.FLAG_PROCESSOR_CPU | api.FLAG_VECTOR_SSE | api.FLAG_THREADING_CPP
@@ -519,124 +718,59 @@ def roofline_timing(ev, stream, prob, steps):
     return kernel_ms, pre_bytes / (kernel_ms * 1e-3) / 1e9, pre_bytes
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-baseline-only", action="store_true")
-    ap.add_argument("--only", choices=("codon", "partitions", "roofline", "plain-post"),
-                    help="run one workload alone (for profiling): the codon or partition "
-                         "workload, the roofline timing of the dominant kernel, or the "
-                         "plain path's FP64 post-order traversal")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run a plain 1 GiB -> 1 GiB elementwise copy kernel, the "
-                         "reference size for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
-    args = ap.parse_args()
-
-    if args.cpu_baseline_only:
-        res = cpu_baseline(args.cpu_budget)
-        res["codon"] = cpu_baseline_codon(min(5.0, args.cpu_budget / 3))
-        print(json.dumps(res))
-        return
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(patterns shard across GPUs with one all-reduce of the per-pattern sums "
-              "per evaluation)", file=sys.stderr)
-        sys.exit(2)
-
+def tree128_bench(lib, steps, warmup):
+    """The 128-taxon, 100,000-pattern workload (caterpillar and balanced
+    trees): logL + dlogL/dt and d2logL/dt2 of all 254 branches per
+    evaluation.  FP32: each tree's evaluation replayed as a HIP graph, the
+    two trees as independent chains (a tree's next evaluation launched once
+    its result is collected, while the other tree's runs); the same
+    evaluations as eager calls; the plain FP64 beagle.h calls and the FP64
+    fused extension; errors against the plain path."""
     import numpy as np
     import torch
-    import beagle_amd
     from beagle_amd import api, driver
-
-    assert torch.cuda.is_available(), "needs an MI355X"
-    lib = beagle_amd.load()
-    if args.only == "codon":
-        print(json.dumps({"codon": codon_bench(lib, args.steps)}))
-        return
-    if args.only == "partitions":
-        print(json.dumps({"partitions": partitions_bench(lib, max(args.steps // 10, 3))}))
-        return
     probs = problems()
     asyn = api.FLAG_COMPUTATION_ASYNCH
-    if args.only == "plain-post":
-        ms, gbs = plain_post_timing(lib, probs[0], args.steps)
-        if args.calibrate_copy:
-            calibrate_copy()
-        print(json.dumps({"plain_post": {"ms": ms, "GBps": gbs,
-                                         "algorithmic_bytes": postorder_bytes(probs[0], 8)}}))
-        return
-    if args.only == "roofline":
-        ev = driver.Evaluator(lib, probs[0], precision="single", scaling="auto", extra_flags=asyn)
-        for _ in range(3):
-            fused_eval(ev)
-        kernel_ms, achieved, pre_bytes = roofline_timing(
-            ev, torch.cuda.ExternalStream(ev.inst.stream()), probs[0], args.steps)
-        ev.close()
-        if args.calibrate_copy:
-            calibrate_copy()
-        print(json.dumps({"roofline": {"kernel_ms": kernel_ms, "achieved": achieved,
-                                       "algorithmic_bytes": pre_bytes}}))
-        return
-
-    # ---- value: FP32 fused logL + gradient, both trees ----
     evs = [driver.Evaluator(lib, p, precision="single", scaling="auto", extra_flags=asyn)
            for p in probs]
-    streams = [torch.cuda.ExternalStream(ev.inst.stream()) for ev in evs]
-    # warm-up: at least W steps and one second of work, so clocks have ramped
     t0, w = time.perf_counter(), 0
-    while w < args.warmup or time.perf_counter() - t0 < 1.0:
+    while w < warmup or time.perf_counter() - t0 < 1.0:
         for ev in evs:
             fused_eval(ev)
         w += 1
     torch.cuda.synchronize()
-    # eager calls, one evaluation after the other
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         eager32 = [fused_eval(evs[0]), fused_eval(evs[1])]
     torch.cuda.synchronize()
-    eager_rate = 2 * args.steps / (time.perf_counter() - t0)
-    # the batched path: each tree's evaluation captured once as a HIP graph
-    # and replayed on its instance's stream with its branch lengths set per
-    # evaluation; the trees are independent chains, so each tree's next
-    # evaluation is launched as soon as its previous result has been
-    # collected, while the other tree's evaluation runs
+    eager_rate = 2 * steps / (time.perf_counter() - t0)
     lengths = [ev.prob.tree.lengths for ev in evs]
     for ev in evs:
         ev.capture_fused_gradient()
 
-    def chains(steps):
+    def chains(n):
         for ev, x in zip(evs, lengths):
             ev.replay_fused_gradient(x, wait=False)
         out = None
-        for step in range(steps):
+        for step in range(n):
             out = []
             for ev, x in zip(evs, lengths):
                 out.append(ev.collect_fused_gradient())
-                if step + 1 < steps:
+                if step + 1 < n:
                     ev.replay_fused_gradient(x, wait=False)
         return out
 
-    chains(max(args.warmup, 3))
+    chains(max(warmup, 3))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res32 = chains(args.steps)
+    res32 = chains(steps)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_per_step = dt * 1e3 / args.steps
-    value = 2 * args.steps / dt
-    graph_identical = all(a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
-                          for a, b in zip(res32, eager32))
-    kernel_ms, achieved, pre_bytes = roofline_timing(evs[0], streams[0], probs[0], args.steps)
+    value = 2 * steps / (time.perf_counter() - t0)
+    identical = all(a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+                    for a, b in zip(res32, eager32))
     for ev in evs:
         ev.close()
-
-    # ---- FP64: plain API path and fused path (errors are taken against
-    # the plain path) ----
-    n64 = max(args.steps // 3, 3)
+    n64 = max(steps // 3, 3)
     rates, res64 = {}, {}
     for name in ("api", "fused"):
         evs = [driver.Evaluator(lib, p, extra_flags=asyn) for p in probs]
@@ -656,15 +790,148 @@ def main():
     err = {"logL": max(rel(a[0], b[0]) for a, b in zip(res32, api64)),
            "dlogL_dt": max(grad_err(a[1], b[1]) for a, b in zip(res32, api64)),
            "d2logL_dt2": max(grad_err(a[2], b[2]) for a, b in zip(res32, api64))}
-    fused64_diff = max(max(grad_err(a[1], b[1]), grad_err(a[2], b[2]), rel(a[0], b[0]))
-                       for a, b in zip(res64["fused"], api64))
+    post_ms, post_gbs = plain_post_timing(lib, probs[0], steps)
+    return {"workload": "4-state GTR+G4, 128 taxa, 100,000 patterns, caterpillar + balanced "
+                        "trees; logL + dlogL/dt and d2logL/dt2 of all 254 branches",
+            "evals_per_s": value, "eager_evals_per_s": eager_rate,
+            "graph_results_identical_to_eager": bool(identical), "error_vs_fp64": err,
+            "plain_fp64": {"api_evals_per_s": rates["api"],
+                           "what": "beagle.h calls: update matrices, post-order, root, "
+                                   "beagleUpdatePrePartials, two beagleCalculateEdgeDerivatives",
+                           "speedup_of_fp32": value / rates["api"],
+                           "fused_evals_per_s": rates["fused"],
+                           "postorder_GBps": post_gbs,
+                           "postorder_frac_of_hbm_peak": post_gbs / HBM_PEAK_GBS,
+                           "postorder_ms": post_ms,
+                           "postorder_kernel": "k_post4<double,false>: caterpillar, 127 "
+                                               "operations, one launch"},
+            "logL": [r[0] for r in api64]}
 
-    post_ms, post_gbs = plain_post_timing(lib, probs[0], args.steps)
 
-    checkpointed = checkpoint_bench(lib, probs, max(args.steps // 3, 3))
+def plain_workload_bench(lib, steps):
+    """The plain FP64 pruning workload (16 taxa, 10,000 patterns, root logL
+    only): the beagle.h calls (matrices, post-order, root) one evaluation
+    after the other, eagerly and replayed as a HIP graph."""
+    import numpy as np
+    import torch
+    from beagle_amd import api, driver, synth
+    prob = synth.make_problem(SEED, PLAIN["taxa"], PLAIN["patterns"], S=4, shape=PLAIN["shape"])
+    ev = driver.Evaluator(lib, prob, preorder=False)
+    for _ in range(5):
+        lnl = ev.log_likelihood()
+    n = max(steps * 5, 50)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        lnl = ev.log_likelihood()
+    eager = n / (time.perf_counter() - t0)
+    res = np.zeros(1)
+    ev.inst.begin_capture()
+    ev.update_matrices()
+    ev.post_order()
+    ev.inst.root_log_likelihood(prob.tree.root, 0, 0, api.OP_NONE, out=res)
+    g = ev.inst.end_capture()
+    for _ in range(5):
+        ev.inst.launch_graph(g)
+        ev.inst.wait_graph(g)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        ev.inst.set_graph_branch_lengths(g, prob.tree.lengths[:2 * prob.tree.n - 2])
+        ev.inst.launch_graph(g)
+        ev.inst.wait_graph(g)
+    graph = n / (time.perf_counter() - t0)
+    same = float(res[0]) == lnl
+    ev.close()
+    return {"workload": "4-state GTR+G4, 16 taxa (Yule), 10,000 patterns, FP64, root logL only",
+            "eager_evals_per_s": eager, "graph_evals_per_s": graph, "logL": lnl,
+            "graph_identical_to_eager": bool(same)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline-only", action="store_true")
+    ap.add_argument("--only", choices=("headline", "tree128", "codon", "partitions", "roofline",
+                                       "plain-post"),
+                    help="run one workload alone (for profiling): the metric's adjoint "
+                         "workload, the 128-taxon workload, the codon or partition workload, "
+                         "the roofline timing of the dominant kernel, or the plain path's "
+                         "FP64 post-order traversal")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB -> 1 GiB elementwise copy kernel, the "
+                         "reference size for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    args = ap.parse_args()
+
+    if args.cpu_baseline_only:
+        res = cpu_baseline(args.cpu_budget)
+        res["plain_workload"] = cpu_baseline_plain(1.0)
+        res["codon"] = cpu_baseline_codon(min(4.0, args.cpu_budget / 3))
+        print(json.dumps(res))
+        return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(patterns shard across GPUs with one all-reduce of the per-pattern sums "
+              "per evaluation)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import beagle_amd
+    from beagle_amd import api, driver
+
+    assert torch.cuda.is_available(), "needs an MI355X"
+    lib = beagle_amd.load()
+    if args.only == "headline":
+        print(json.dumps(full_gradient_bench(lib, args.steps, args.warmup)))
+        return
+    if args.only == "tree128":
+        print(json.dumps(tree128_bench(lib, args.steps, args.warmup)))
+        return
+    if args.only == "codon":
+        print(json.dumps({"codon": codon_bench(lib, args.steps)}))
+        return
+    if args.only == "partitions":
+        print(json.dumps({"partitions": partitions_bench(lib, max(args.steps // 10, 3))}))
+        return
+    if args.only == "plain-post":
+        probs = problems()
+        ms, gbs = plain_post_timing(lib, probs[0], args.steps)
+        if args.calibrate_copy:
+            calibrate_copy()
+        print(json.dumps({"plain_post": {"ms": ms, "GBps": gbs,
+                                         "algorithmic_bytes": postorder_bytes(probs[0], 8)}}))
+        return
+    asyn = api.FLAG_COMPUTATION_ASYNCH
+    if args.only == "roofline":
+        prob = adjoint_problem()
+        ev = driver.Evaluator(lib, prob, precision="single", scaling="auto", extra_flags=asyn)
+        for _ in range(3):
+            fused_eval(ev)
+        kernel_ms, achieved, pre_bytes = roofline_timing(
+            ev, torch.cuda.ExternalStream(ev.inst.stream()), prob, args.steps)
+        ev.close()
+        if args.calibrate_copy:
+            calibrate_copy()
+        print(json.dumps({"roofline": {"kernel_ms": kernel_ms, "achieved": achieved,
+                                       "algorithmic_bytes": pre_bytes}}))
+        return
+
+    head = full_gradient_bench(lib, args.steps, args.warmup)
+    prob = adjoint_problem()
+    ev = driver.Evaluator(lib, prob, precision="single", scaling="auto", extra_flags=asyn)
+    for _ in range(3):
+        fused_eval(ev)
+    kernel_ms, achieved, pre_bytes = roofline_timing(
+        ev, torch.cuda.ExternalStream(ev.inst.stream()), prob, args.steps)
+    ev.close()
+    tree128 = tree128_bench(lib, args.steps, args.warmup)
+    plain16 = plain_workload_bench(lib, args.steps)
+    checkpointed = checkpoint_bench(lib, problems(), max(args.steps // 3, 3))
     codon = codon_bench(lib, args.steps)
     partitions = partitions_bench(lib, max(args.steps // 10, 3))
-
     if args.calibrate_copy:
         calibrate_copy()
 
@@ -677,12 +944,18 @@ def main():
             cpu = json.loads(r.stdout.strip().splitlines()[-1])
         else:
             print(r.stderr[-2000:], file=sys.stderr)
+    if cpu:
+        if cpu.get("codon"):
+            codon["cpu_reference"] = cpu.pop("codon")
+        if cpu.get("plain_workload"):
+            plain16["cpu_reference"] = cpu.pop("plain_workload")
 
     traffic = None
-    tpath = os.path.join(REPO, "profiles", "pren4f_fp32_caterpillar_traffic.json")
+    tpath = os.path.join(REPO, "profiles", "pren4f_fp32_adjoint_512taxa_traffic.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
 
+    value = head.pop("value")
     out = {
         "metric": METRIC,
         "value": value,
@@ -690,49 +963,34 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_eval"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generators; no dataset)",
-        "config": {"workload": "4-state GTR+G4, 128 taxa, 100,000 patterns, caterpillar + "
-                               "balanced trees; logL + dlogL/dt and d2logL/dt2 of all 254 "
-                               "branches per evaluation",
-                   "taxa": N_TAXA, "patterns": N_PATTERNS, "categories": 4,
-                   "evals_per_step": 2,
+        "config": {"workload": "adjoint: 4-state GTR+G4, 512-taxon Yule tree, 50,000 patterns; "
+                               "one evaluation = logL + dlogL/dt and d2logL/dt2 of all 1,022 "
+                               "branches + dlogL/dQ + dlogL/dpi_root",
+                   "taxa": ADJ["taxa"], "patterns": ADJ["patterns"], "categories": 4,
+                   "steps_are": "evaluations, one after the other",
                    "path": "FP32 partials, power-of-two rescaling, fused pre-order + "
-                           "derivative extension; each tree's evaluation replayed as a HIP "
-                           "graph, the two trees as independent chains (a tree's next "
-                           "evaluation launched once its result is collected)"},
-        "eager_evals_per_s": eager_rate,
-        "graph_results_identical_to_eager": graph_identical,
+                           "derivative extension and rate-matrix gradient; the evaluation "
+                           "replayed as one HIP graph with its branch lengths set per replay"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_pren4f<4,true>: node-fused FP32 pre-order + derivative "
-                               "traversal of the caterpillar (127 node jobs = 254 operations, "
-                               "one launch)",
+                               "traversal of the 512-taxon tree (511 node jobs = 1,022 "
+                               "operations)",
                      "kernel_ms": kernel_ms, "algorithmic_bytes": pre_bytes},
-        "error_vs_fp64": err,
-        "plain_fp64": {
-            "api_evals_per_s": rates["api"],
-            "what": "beagle.h calls: update matrices, post-order, root, "
-                    "beagleUpdatePrePartials, two beagleCalculateEdgeDerivatives",
-            "speedup_of_value": value / rates["api"],
-            "fused_evals_per_s": rates["fused"],
-            "fused_vs_api_max_rel_diff": fused64_diff,
-            "postorder_GBps": post_gbs,
-            "postorder_frac_of_hbm_peak": post_gbs / HBM_PEAK_GBS,
-            "postorder_ms": post_ms,
-            "postorder_kernel": "k_post4<double,false>: caterpillar, 127 operations, one launch"},
-        "logL": [r[0] for r in api64],
         "cpu_baseline": cpu,
+        "adjoint": head,
+        "tree128": tree128,
+        "plain_workload": plain16,
         "checkpointed": checkpointed,
         "codon": codon,
         "partitions": partitions,
     }
-    if cpu and cpu.get("codon"):
-        codon["cpu_reference"] = cpu.pop("codon")
     print(json.dumps(out))
 
 
