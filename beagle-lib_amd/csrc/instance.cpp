@@ -1827,22 +1827,24 @@ int Instance::rateMatrixGradient(int e, const int* ops, int count, const int* po
     void* jm = grow(dJobMats_, jobMatsCap_, (size_t)n * perJob * realSize());
     const int ntiles = (dims_.P + 63) / 64;
     const int nb = (dims_.P + 255) / 256;
-    const int chunks = qgrad_chunks(dims_, n);
-    const size_t nq = (size_t)16 * ntiles * chunks * C, nf = (size_t)S * nb;
+    const int cols = qgrad_partial_cols(dims_, n);
+    const size_t nq = (size_t)16 * cols, nf = (size_t)S * nb;
     double* wk = reinterpret_cast<double*>(
         grow(dWork_, workCap_, (nq + nf + 2 * (16 + S)) * sizeof(double)));
     double* sums = wk + nq + nf;
     double* out = sums + 16 + S;
     const double* cw = dCatWeights_ + (size_t)wIdx * C;
-    if (dp_) launch_gather_node_matrices<double>(dj, n, dims_, dMatrices_, jm, stream_);
-    else launch_gather_node_matrices<float>(dj, n, dims_, dMatrices_, jm, stream_);
+    if (C != 4) {   // the category-per-wave kernel reads gathered job matrices
+        if (dp_) launch_gather_node_matrices<double>(dj, n, dims_, dMatrices_, jm, stream_);
+        else launch_gather_node_matrices<float>(dj, n, dims_, dMatrices_, jm, stream_);
+    }
     if (dp_)
-        launch_qgrad<double>(dj, n, dims_, jm, dF_, eig, dDummy_, cw, dPatternWeights_, wk,
-                             stream_);
+        launch_qgrad<double>(dj, n, dims_, jm, dMatrices_, dF_, eig, dDummy_, cw,
+                             dPatternWeights_, wk, stream_);
     else
-        launch_qgrad<float>(dj, n, dims_, jm, dF_, eig, dDummy_, cw, dPatternWeights_, wk,
-                            stream_);
-    launch_reduce_rows(wk, ntiles * chunks * C, 16, sums, stream_);
+        launch_qgrad<float>(dj, n, dims_, jm, dMatrices_, dF_, eig, dDummy_, cw,
+                            dPatternWeights_, wk, stream_);
+    launch_reduce_rows(wk, cols, 16, sums, stream_);
     const bool freqs = outFreqs != nullptr;
     if (freqs) {
         if (!validBuffer(rootIdx) || !dPartials_[rootIdx] || !validEigen(fIdx) || hFreqs_[fIdx].empty())

@@ -282,14 +282,16 @@ void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, 
 // Rate-matrix gradient over node jobs (4 states): per-tile, per-category
 // sums of H (16 rows of tiles * C); jobF holds F per (job, category, child)
 // as [job][category][2][16], eig = V | V^-1 row-major.
-// Chunks of the job list k_qgrad4 spreads over its grid's second dimension;
-// partial holds [16][tiles * chunks * C] sums.
+// Chunks of the job list the 4-state Q-gradient kernels spread over their
+// grid's second dimension, and the columns of their [16][cols] partial sums.
 int qgrad_chunks(const Dims& d, int njobs);
+int qgrad_partial_cols(const Dims& d, int njobs);
+// jobMats: the gathered job matrices (C != 4); mats: the matrix slab (C = 4);
 // factors: the factor slab (NodeJob::mF offsets); eig: [V | V^-1] (FP64)
 template <typename R>
 void launch_qgrad(const NodeJob* djobs, int njobs, const Dims& d, const void* jobMats,
-                  const double* factors, const double* eig, void* dummy, const double* cw,
-                  const double* pw, double* partial, hipStream_t st);
+                  const void* mats, const double* factors, const double* eig, void* dummy,
+                  const double* cw, const double* pw, double* partial, hipStream_t st);
 // out[0..15] = V^-T H V^T from the summed H (4 states); out[16..16+nf) =
 // sums[16..16+nf) (the root-frequency gradient, passed through)
 void launch_qgrad_finish(const double* sums, const double* eig, int nf, double* out,

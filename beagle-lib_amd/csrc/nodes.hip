@@ -810,18 +810,201 @@ __global__ __launch_bounds__(512) void k_qgrad4(const NodeJob* __restrict__ jobs
     }
 }
 
+// The same sums with one wave per 16 patterns holding all four categories
+// (lane = 16 c + j, pattern 16 w + j of wave w): the per-pattern likelihood
+// L_p = sum_c w_c sum_i u_i (P_a x_a)_i (P_b x_b)_i is two lane exchanges
+// (lanes 16 and 32 apart) instead of an LDS exchange and a workgroup
+// barrier per job, so waves run independently; V and V^-1 are wave-uniform
+// (scalar registers).  Each wave walks one chunk of the (independent) jobs
+// with the next job's loads in flight: its pattern operands into the other
+// of two register sets (the loop is unrolled by two), and its matrices P_a,
+// P_b (all categories, [2][C][4][5]) and Frechet factors ([2][C][16]),
+// loaded by the wave's lanes together, staged through a wave-private LDS
+// slot when the job starts (LDS reads wait on their own counter, so the
+// loads in flight are not waited for).  H is summed over the 64 lanes at
+// the end.
+template <typename R>
+struct QWOps {
+    R u[4], xa[4], xb[4];
+    int sa, sb;
+    typename Vec16<R>::T m0, m1;   // this lane's share of the job's matrices
+    dvec2 f;                       // this lane's share of its Frechet factors
+};
+
+template <typename R>
+__device__ __forceinline__ void qw_issue(const NodeJob& J, unsigned off, int p, int lane,
+                                         const R* __restrict__ mats,
+                                         const double* __restrict__ F, QWOps<R>& o) {
+    constexpr int V = Vec16<R>::N, NV = 2 * 80 / V;   // vectors of the two matrices
+    load4(at<R>(J.parent, off), o.u);
+    if (J.k[0] == 1) o.sa = *at<int>(J.x[0], 4u * p);
+    else load4(at<R>(J.x[0], off), o.xa);
+    if (J.k[1] == 1) o.sb = *at<int>(J.x[1], 4u * p);
+    else load4(at<R>(J.x[1], off), o.xb);
+    // vectors l and (FP64: 80 vectors) l + 64 of [P_a | P_b] (each 80 reals,
+    // all categories)
+    const int l0 = min(lane, NV - 1);
+    const int h0 = l0 < NV / 2 ? 0 : 1;
+    o.m0 = *(const GLOBAL typename Vec16<R>::T*)(gin<R>(mats) + J.moP[h0] + (l0 - h0 * NV / 2) * V);
+    if (NV > 64) {
+        const int l1 = min(lane + 64, NV - 1);
+        const int h1 = l1 < NV / 2 ? 0 : 1;
+        o.m1 = *(const GLOBAL typename Vec16<R>::T*)(gin<R>(mats) + J.moP[h1] +
+                                                      (l1 - h1 * NV / 2) * V);
+    }
+    // factors: lanes 0-31 child a's [C][16], lanes 32-63 child b's
+    o.f = *(const GLOBAL dvec2*)(gin<double>(F) + J.mF[lane >> 5] + 2 * (lane & 31));
+}
+
+template <typename R>
+__device__ __forceinline__ void qw_stage(const QWOps<R>& o, int lane, R* sm, double* sf) {
+    constexpr int V = Vec16<R>::N, NV = 2 * 80 / V;
+    *(typename Vec16<R>::T*)(sm + min(lane, NV - 1) * V) = o.m0;
+    if (NV > 64) *(typename Vec16<R>::T*)(sm + min(lane + 64, NV - 1) * V) = o.m1;
+    *(dvec2*)(sf + 2 * lane) = o.f;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+template <typename R>
+__device__ __forceinline__ void qw_compute(const NodeJob& J, const QWOps<R>& o, const R* sm,
+                                           const double* sf, int c, bool in, double w,
+                                           double wc, const double (&V)[16],
+                                           const double (&Vi)[16], double (&H)[16]) {
+    const bool tipA = J.k[0] == 1, tipB = J.k[1] == 1;
+    const R* Pa = sm + c * 20;
+    const R* Pb = sm + 80 + c * 20;
+    // a tip reads the column of its state (column 4: the pad column of a
+    // missing state); its own vector is one-hot, and a missing state's pad
+    // column does not depend on Q, so it contributes nothing
+    const int ca = min((unsigned)o.sa, 4u), cb = min((unsigned)o.sb, 4u);
+    R ya[4], yb[4];   // P x, in the partials' precision as the products are formed
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        ya[i] = tipA ? Pa[i * 5 + ca]
+                     : Pa[i * 5] * o.xa[0] + Pa[i * 5 + 1] * o.xa[1] + Pa[i * 5 + 2] * o.xa[2] +
+                           Pa[i * 5 + 3] * o.xa[3];
+        yb[i] = tipB ? Pb[i * 5 + cb]
+                     : Pb[i * 5] * o.xb[0] + Pb[i * 5 + 1] * o.xb[1] + Pb[i * 5 + 2] * o.xb[2] +
+                           Pb[i * 5 + 3] * o.xb[3];
+    }
+    double L = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) L += (double)o.u[i] * (double)ya[i] * (double)yb[i];
+    L *= wc;
+    L += __shfl_xor(L, 16, 64);
+    L += __shfl_xor(L, 32, 64);
+    const double coef = in ? w / L : 0.0;
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+        const bool tip = ch == 0 ? tipA : tipB;
+        const int st = ch == 0 ? ca : cb;
+        double al[4], be[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double a = 0.0, b = 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double top = (double)o.u[i] * (double)(ch == 0 ? yb[i] : ya[i]);
+                a += V[i * 4 + q] * top;
+                const double x = (double)(ch == 0 ? o.xa[i] : o.xb[i]);
+                b += Vi[q * 4 + i] * (tip ? (st == i ? 1.0 : 0.0) : x);
+            }
+            al[q] = coef * a;
+            be[q] = b;
+        }
+        const double* Fc = sf + ch * 64 + c * 16;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) H[q * 4 + r] += Fc[q * 4 + r] * (al[q] * be[r]);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_qgrad4w(const NodeJob* __restrict__ jobs, int njobs,
+                                                 int P, long cs, const R* __restrict__ mats,
+                                                 const double* __restrict__ F,
+                                                 const double* __restrict__ eig,
+                                                 const double* __restrict__ cw,
+                                                 const double* __restrict__ pw,
+                                                 double* __restrict__ partial) {
+    // LDS per wave: the job's matrices [2][4][20] reals, factors [2][4][16]
+    __shared__ double sfb[4][128];
+    __shared__ R smb[4][160];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = lane >> 4, j = lane & 15;
+    const int wave = blockIdx.x * 4 + wv;
+    const int nwaves = gridDim.x * 4;
+    const int p = wave * 16 + j;            // < Ppad (a multiple of 64)
+    const bool in = p < P;
+    const double wc = cw[c];
+    const double w = pw[p] * wc;            // pattern weights are zero past P
+    const unsigned off = (unsigned)((c * cs + (long)p * 4) * (long)sizeof(R));
+    R* sm = smb[wv];
+    double* sf = sfb[wv];
+    double V[16], Vi[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        V[q] = eig[q];
+        Vi[q] = eig[16 + q];
+    }
+    double H[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) H[q] = 0.0;
+    const int chunk = (njobs + gridDim.y - 1) / gridDim.y;
+    const int o0 = blockIdx.y * chunk, o1 = min(njobs, o0 + chunk);
+    QWOps<R> A, B;
+    if (o0 < o1) qw_issue<R>(jobs[o0], off, p, lane, mats, F, A);
+    for (int o = o0; o < o1; o += 2) {
+        const NodeJob& Ja = jobs[o];
+        qw_stage<R>(A, lane, sm, sf);
+        if (o + 1 < o1) qw_issue<R>(jobs[o + 1], off, p, lane, mats, F, B);
+        qw_compute<R>(Ja, A, sm, sf, c, in, w, wc, V, Vi, H);
+        if (o + 1 >= o1) break;
+        const NodeJob& Jb = jobs[o + 1];
+        qw_stage<R>(B, lane, sm, sf);
+        if (o + 2 < o1) qw_issue<R>(jobs[o + 2], off, p, lane, mats, F, A);
+        qw_compute<R>(Jb, B, sm, sf, c, in, w, wc, V, Vi, H);
+    }
+    const long cols = (long)nwaves * gridDim.y;
+    const long col = (long)blockIdx.y * nwaves + wave;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const double sum = wave_sum_dpp(H[q]);
+        if (lane == 0) partial[q * cols + col] = sum;
+    }
+}
+
+// Chunks of the job list the Q-gradient kernels spread over the grid's
+// second dimension: about 16 waves per SIMD (1,024 SIMDs), at least 16 jobs
+// per chunk.
 int qgrad_chunks(const Dims& d, int njobs) {
-    // enough workgroups for about eight per CU (256 CUs), at least 16 jobs each
     const int tiles = (d.P + TILE - 1) / TILE;
-    const int want = (2048 + tiles - 1) / tiles;
+    const int waves = d.C == 4 ? 4 * tiles : tiles * d.C;
+    const int want = (16384 + waves - 1) / waves;
     return std::max(1, std::min(want, (njobs + 15) / 16));
+}
+
+int qgrad_partial_cols(const Dims& d, int njobs) {
+    const int tiles = (d.P + TILE - 1) / TILE;
+    const int chunks = qgrad_chunks(d, njobs);
+    return d.C == 4 ? 4 * tiles * chunks : tiles * chunks * d.C;
 }
 
 template <typename R>
 void launch_qgrad(const NodeJob* djobs, int njobs, const Dims& d, const void* jobMats,
-                  const double* jobF, const double* eig, void* dummy, const double* cw,
-                  const double* pw, double* partial, hipStream_t st) {
+                  const void* mats, const double* jobF, const double* eig, void* dummy,
+                  const double* cw, const double* pw, double* partial, hipStream_t st) {
     if (njobs <= 0) return;
+    if (d.C == 4) {
+        const int tiles = (d.P + TILE - 1) / TILE;
+        hipLaunchKernelGGL((k_qgrad4w<R>), dim3(tiles, qgrad_chunks(d, njobs)), dim3(256), 0, st,
+                           djobs, njobs, d.P, d.catStride, reinterpret_cast<const R*>(mats), jobF,
+                           eig, cw, pw, partial);
+        return;
+    }
     const R* dummyX = reinterpret_cast<const R*>(dummy);
     R* dummyW = const_cast<R*>(dummyX) + 8 * TILE * 4;
     const double* dummyD = reinterpret_cast<const double*>(dummyW + 8 * TILE * 4);
@@ -967,9 +1150,9 @@ void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, 
 #define INSTANTIATE(R)                                                                          \
     template void launch_gather_node_matrices<R>(const NodeJob*, int, const Dims&, const void*, \
                                                  void*, hipStream_t, bool);                     \
-    template void launch_qgrad<R>(const NodeJob*, int, const Dims&, const void*, const double*,  \
-                                  const double*, void*, const double*, const double*, double*,  \
-                                  hipStream_t);                                                 \
+    template void launch_qgrad<R>(const NodeJob*, int, const Dims&, const void*, const void*,   \
+                                  const double*, const double*, void*, const double*,           \
+                                  const double*, double*, hipStream_t);                         \
     template void launch_root_freq_grad<R>(const void*, const double*, const double*,          \
                                            const double*, double*, const Dims&, hipStream_t);   \
     template void launch_pre_nodes<R>(const NodeJob*, int, bool, int, const Dims&, const void*, \
