@@ -158,6 +158,8 @@ SIGNATURES = {
     "ExtCalculateLogLikelihoodAndDerivatives": (_I, [_I, _I, _I, _I, _I, _I, _IP, _I, _IP, _IP,
                                                      _IP, _DP, _DP, _DP, _DP]),
     "ExtCalculateRateMatrixGradient": (_I, [_I, _I, _IP, _I, _IP, _I, _I, _I, _DP, _DP]),
+    "ExtCalculateLogLikelihoodAndGradient": (_I, [_I, _I, _I, _I, _I, _I, _IP, _I, _IP, _IP,
+                                                  _IP, _I, _DP, _DP, _DP, _DP, _DP, _DP]),
     "ExtCalculateRootLogLikelihoodsNoAlloc": (_I, [_I, _IP, _IP, _IP, _IP, _I, _DP]),
 }
 
@@ -551,6 +553,29 @@ class Instance:
                 s1.ctypes.data_as(c_double_p), sq.ctypes.data_as(c_double_p),
                 s2.ctypes.data_as(c_double_p))
         return float(lnl[0]), s1, sq, s2
+
+    def log_likelihood_and_gradient(self, root, root_pre, ops, post, d1, d2, eigen=0,
+                                    weights_index=0, freqs_index=0, cumulative=OP_NONE,
+                                    out=None):
+        """Extension: log_likelihood_and_derivatives() and
+        rate_matrix_gradient(eigen, ops, post, root=root) with one wait.
+        Returns (logL, s1, sq, s2, dlogL/dQ [S][S], dlogL/dpi_root [S]).
+        out: caller-owned float64 arrays (logL[1], s1[n], sq[n], s2[n],
+        q[S*S], pi[S]), needed while capturing a graph."""
+        a = _ip(np.asarray(ops).reshape(-1))
+        n = len(a[0]) // 7
+        b, c, d = _ip(post), _ip(d1), _ip(d2)
+        S = self.S
+        if out is None:
+            out = (np.zeros(1), np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(S * S),
+                   np.zeros(S))
+        lnl, s1, sq, s2, gq, gf = out
+        self._c("ExtCalculateLogLikelihoodAndGradient", root, weights_index, freqs_index,
+                cumulative, root_pre, a[1], n, b[1], c[1], d[1], eigen,
+                lnl.ctypes.data_as(c_double_p), s1.ctypes.data_as(c_double_p),
+                sq.ctypes.data_as(c_double_p), s2.ctypes.data_as(c_double_p),
+                gq.ctypes.data_as(c_double_p), gf.ctypes.data_as(c_double_p))
+        return float(lnl[0]), s1, sq, s2, gq.reshape(S, S), gf
 
     def checkpointed_gradient(self, ops, d1, d2, buffers, interval=0, weights_index=0,
                               freqs_index=0, plan_only=False):

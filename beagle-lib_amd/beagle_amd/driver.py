@@ -223,22 +223,29 @@ class Evaluator:
         pre-order + derivative extensions (this package's library only):
         matrices and the post-order traversal, then one call for the root
         log-likelihood, root pre-order partials and the derivative pass;
-        with rate_matrix also dlogL/dQ and dlogL/dpi_root."""
+        with rate_matrix also dlogL/dQ and dlogL/dpi_root (True: in the same
+        call, beagleExtCalculateLogLikelihoodAndGradient; "separate": a
+        second call, beagleExtCalculateRateMatrixGradient)."""
         ops, post, d1, d2 = self.fused_operations(keep_tips)
         self.update_matrices(lengths)
         self.post_order()
         manual = self.scaling in ("manual", "manual_log")
         cum = self.n - 1 if manual else api.OP_NONE
-        logl, s1, sq, s2 = self.inst.log_likelihood_and_derivatives(
-            self.prob.tree.root, self.pre_base + self.prob.tree.root, ops, post, d1, d2,
-            cumulative=cum)
+        root, root_pre = self.prob.tree.root, self.pre_base + self.prob.tree.root
+        if rate_matrix is True:
+            logl, s1, sq, s2, q, pi = self.inst.log_likelihood_and_gradient(
+                root, root_pre, ops, post, d1, d2, cumulative=cum)
+        else:
+            logl, s1, sq, s2 = self.inst.log_likelihood_and_derivatives(
+                root, root_pre, ops, post, d1, d2, cumulative=cum)
         g1 = np.zeros(2 * self.n - 1)
         g2 = np.zeros(2 * self.n - 1)
         g1[post] = s1
         g2[post] = s2 - sq
         nodes = self.branch_nodes()
-        if rate_matrix:
+        if rate_matrix == "separate":
             q, pi = self.rate_matrix_gradient()
+        if rate_matrix:
             return logl, g1[nodes], g2[nodes], q, pi
         return logl, g1[nodes], g2[nodes]
 
@@ -274,11 +281,9 @@ class Evaluator:
         buffers have their size."""
         if self.matrices != "eigen":
             raise ValueError("graphs record matrices computed from the eigen-system")
-        self.fused_gradient(keep_tips=keep_tips)
+        self.fused_gradient(keep_tips=keep_tips, rate_matrix=rate_matrix)
         ops, post, d1, d2 = self.fused_operations(keep_tips)
         S = self.prob.S
-        if rate_matrix:
-            self.rate_matrix_gradient()
         n = len(post)
         self._gout = (np.zeros(1), np.zeros(n), np.zeros(n), np.zeros(n))
         self._qout = (np.zeros(S * S), np.zeros(S)) if rate_matrix else None
@@ -289,12 +294,16 @@ class Evaluator:
         try:
             self.update_matrices()
             self.post_order()
-            self.inst.log_likelihood_and_derivatives(
-                self.prob.tree.root, self.pre_base + self.prob.tree.root, ops, post, d1, d2,
-                cumulative=cum, out=self._gout)
-            if rate_matrix:
-                self.inst.rate_matrix_gradient(0, ops, post, root=self.prob.tree.root,
-                                               out=self._qout)
+            root, root_pre = self.prob.tree.root, self.pre_base + self.prob.tree.root
+            if rate_matrix is True:
+                self.inst.log_likelihood_and_gradient(root, root_pre, ops, post, d1, d2,
+                                                      cumulative=cum,
+                                                      out=self._gout + self._qout)
+            else:
+                self.inst.log_likelihood_and_derivatives(root, root_pre, ops, post, d1, d2,
+                                                         cumulative=cum, out=self._gout)
+            if rate_matrix == "separate":
+                self.inst.rate_matrix_gradient(0, ops, post, root=root, out=self._qout)
         finally:
             self._graph = self.inst.end_capture()
         return self._graph

@@ -644,6 +644,25 @@ int beagleExtCalculateLogLikelihoodAndDerivatives(
     });
 }
 
+int beagleExtCalculateLogLikelihoodAndGradient(
+    int instance, int rootBufferIndex, int categoryWeightsIndex, int stateFrequenciesIndex,
+    int cumulativeScaleIndex, int rootPreBufferIndex, const BeagleOperation* operations,
+    int operationCount, const int* postBufferIndices, const int* firstDerivativeIndices,
+    const int* secondDerivativeIndices, int eigenIndex, double* outLogLikelihood,
+    double* outSumFirstDerivatives, double* outSumSquaredFirstDerivatives,
+    double* outSumSecondDerivatives, double* outGradientRateMatrix,
+    double* outGradientFrequencies) {
+    if (!outLogLikelihood) return BEAGLE_ERROR_OUT_OF_RANGE;
+    return withInstance(instance, [&](Instance* in) {
+        return in->logLikelihoodAndGradient(
+            rootBufferIndex, categoryWeightsIndex, stateFrequenciesIndex, cumulativeScaleIndex,
+            rootPreBufferIndex, reinterpret_cast<const int*>(operations), operationCount,
+            postBufferIndices, firstDerivativeIndices, secondDerivativeIndices, eigenIndex,
+            outLogLikelihood, outSumFirstDerivatives, outSumSquaredFirstDerivatives,
+            outSumSecondDerivatives, outGradientRateMatrix, outGradientFrequencies);
+    });
+}
+
 int beagleExtCalculateRateMatrixGradient(int instance, int eigenIndex,
                                           const BeagleOperation* operations, int operationCount,
                                           const int* postBufferIndices, int categoryWeightsIndex,

@@ -1576,19 +1576,46 @@ const double* Instance::stageOut(const double* dsrc, size_t n, size_t at) {
 
 // The fused 4-state derivative pass, enqueued; with `reduce` the per-
 // operation sums (3 per operation) end in *dSums on the device.
+//
+// With qg (4 states, FP32, 1, 2 or 4 categories) the pass also forms the
+// rate-matrix gradient sums H of eigen-system qg->eigen: qg->slots slots of
+// qg->hacc ([slot][16], at most node jobs x tiles); the work buffer also
+// holds their block sums, the root-frequency partials, the reduced sums and
+// the final gradient.  qg->fused tells whether the node kernels ran
+// (otherwise nothing of the gradient was formed).
 int Instance::enqueuePreDerivatives(const int* ops, int count, const int* postIdx,
                                     const int* d1Idx, const int* d2Idx, int wIdx, bool reduce,
-                                    const double** dSums) {
+                                    const double** dSums, QGradPass* qg) {
     int rc = buildPreJobs(ops, count, false, true);
     if (rc != BEAGLE_SUCCESS || count == 0) return rc;
     rc = attachOwnOperands(count, postIdx, d1Idx, d2Idx);
     if (rc != BEAGLE_SUCCESS) return rc;
     const int ntiles = (dims_.P + 63) / 64;
     const size_t nPart = (size_t)count * 3 * ntiles;
-    double* wk = reinterpret_cast<double*>(grow(dWork_, workCap_, (nPart + 3 * (size_t)count) * sizeof(double)));
+    const int S = dims_.S, nb = (dims_.P + 255) / 256;
+    const long cap = qg ? (long)(count / 2 + 1) * ntiles : 0;
+    const size_t nq = qg ? 16 * ((size_t)cap + (size_t)cap / 128 + 1) + (size_t)S * nb +
+                               2 * (16 + (size_t)S)
+                         : 0;
+    double* wk = reinterpret_cast<double*>(
+        grow(dWork_, workCap_, (nPart + 3 * (size_t)count + nq) * sizeof(double)));
     double* sums = wk + nPart;
     preWeights_ = wIdx;
-    if (!launchPreNodes(ops, count, postIdx, wk)) launchPre(ops, BEAGLE_OP_COUNT, count, wk);
+    bool nodes;
+    if (qg) {
+        qg->hacc = sums + 3 * (size_t)count;
+        qg->slotPart = qg->hacc + 16 * (size_t)cap;
+        qg->freqPart = qg->slotPart + 16 * ((size_t)cap / 128 + 1);
+        qg->sums = qg->freqPart + (size_t)S * nb;
+        qg->out = qg->sums + 16 + S;
+        NodeQGrad q{dF_, dVVi_ + (size_t)qg->eigen * 2 * S * S, qg->hacc, cap, 0};
+        nodes = launchPreNodes(ops, count, postIdx, wk, &q);
+        qg->fused = nodes;
+        qg->slots = q.used;
+    } else {
+        nodes = launchPreNodes(ops, count, postIdx, wk);
+    }
+    if (!nodes) launchPre(ops, BEAGLE_OP_COUNT, count, wk);
     if (reduce) launch_reduce_rows(wk, ntiles, 3 * count, sums, stream_);
     HIPCHECK(hipGetLastError());
     *dSums = sums;
@@ -1638,6 +1665,121 @@ int Instance::logLikelihoodAndDerivatives(int rootIdx, int wIdx, int fIdx, int c
         if (outD1Sq) outD1Sq[o] = h[3 * o + 1];
         if (outD2) outD2[o] = h[3 * o + 2];
     }
+    return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+}
+
+// The operands of a rate-matrix gradient of eigen-system e: every
+// operation's branch matrix computed from e (its Frechet factors were then
+// written beside it), category weights wIdx set.
+int Instance::checkQGradOperands(int e, const int* ops, int count, int wIdx) {
+    if (!validEigen(e) || !eigenSet_[e] || hEigen_[e].lam.empty() || !validEigen(wIdx) ||
+        hCatWeights_[wIdx].empty())
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    for (int o = 0; o < count; ++o) {
+        const int m = ops[(size_t)o * BEAGLE_OP_COUNT + 4];
+        if (!validMatrix(m)) return BEAGLE_ERROR_OUT_OF_RANGE;
+        const MatrixOrigin& org = matOrigin_[m];
+        if (org.eigen != e || hCatRates_[org.rates].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
+    }
+    return BEAGLE_SUCCESS;
+}
+
+// Extension: logLikelihoodAndDerivatives and rateMatrixGradient with one
+// wait (see beagle_amd_ext.h).  On the FP32 4-state node path (1, 2 or 4
+// categories) the rate-matrix gradient is formed inside the derivative
+// pass from the operands it already holds; otherwise the two calls run in
+// turn.
+int Instance::logLikelihoodAndGradient(int rootIdx, int wIdx, int fIdx, int cumIdx,
+                                       int rootPreIdx, const int* ops, int count,
+                                       const int* postIdx, const int* d1Idx, const int* d2Idx,
+                                       int e, double* outLogL, double* outD1, double* outD1Sq,
+                                       double* outD2, double* outQ, double* outFreqs) {
+    const int S = dims_.S;
+    const bool fused = S == 4 && !dp_ && node_f32_supported(dims_.C) && !preTransposed_ &&
+                       !complex_ && d2Idx != nullptr && dF_ != nullptr && count > 0 &&
+                       partialsBytes_ < (size_t(1) << 32);
+    if (!fused) {
+        int rc = logLikelihoodAndDerivatives(rootIdx, wIdx, fIdx, cumIdx, rootPreIdx, ops, count,
+                                             postIdx, d1Idx, d2Idx, outLogL, outD1, outD1Sq,
+                                             outD2);
+        if (rc != BEAGLE_SUCCESS && rc != BEAGLE_ERROR_FLOATING_POINT) return rc;
+        const int rq = rateMatrixGradient(e, ops, count, postIdx, wIdx, rootIdx, fIdx, outQ,
+                                          outFreqs);
+        return rq != BEAGLE_SUCCESS ? rq : rc;
+    }
+    int rc = checkQGradOperands(e, ops, count, wIdx);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    const bool freqs = outFreqs != nullptr;
+    if (freqs && (!validBuffer(rootIdx) || !dPartials_[rootIdx] || !validEigen(fIdx) ||
+                  hFreqs_[fIdx].empty()))
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    rc = enqueueRoot(rootIdx, wIdx, fIdx, cumIdx);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    rc = setRootPrePartials(&rootPreIdx, &fIdx, 1);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    const double* sums = nullptr;
+    QGradPass qg{e, false, nullptr, nullptr, nullptr, nullptr, nullptr, 0};
+    rc = enqueuePreDerivatives(ops, count, postIdx, d1Idx, d2Idx, wIdx, true, &sums, &qg);
+    if (rc != BEAGLE_SUCCESS) return rc;
+    if (!qg.fused) {
+        // the operations do not pair into node jobs: the derivative results
+        // as logLikelihoodAndDerivatives delivers them, then the separate
+        // gradient
+        if (capturing_) captureBreak("the rate-matrix gradient needs node-paired operations");
+        reserveStage(1 + 3 * (size_t)count);
+        const double* hl = stageOut(dResult_, 1, 0);
+        const double* h = stageOut(sums, 3 * (size_t)count, 1);
+        sync();
+        lastLogL_ = hl[0];
+        *outLogL = lastLogL_;
+        for (int o = 0; o < count; ++o) {
+            if (outD1) outD1[o] = h[3 * o];
+            if (outD1Sq) outD1Sq[o] = h[3 * o + 1];
+            if (outD2) outD2[o] = h[3 * o + 2];
+        }
+        const int rq = rateMatrixGradient(e, ops, count, postIdx, wIdx, rootIdx, fIdx, outQ,
+                                          outFreqs);
+        if (rq != BEAGLE_SUCCESS) return rq;
+        return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
+    }
+    const double* cw = dCatWeights_ + (size_t)wIdx * dims_.C;
+    const double* eig = dVVi_ + (size_t)e * 2 * S * S;
+    launch_qgrad_slot_sums(qg.hacc, qg.slots, qg.slotPart, qg.sums, stream_);
+    if (freqs) {
+        const int nb = (dims_.P + 255) / 256;
+        launch_root_freq_grad<float>(dPartials_[rootIdx], cw, dFreqs_ + (size_t)fIdx * S,
+                                     dPatternWeights_, qg.freqPart, dims_, stream_);
+        launch_reduce_rows(qg.freqPart, nb, S, qg.sums + 16, stream_);
+    }
+    launch_qgrad_finish(qg.sums, eig, freqs ? S : 0, qg.out, stream_);
+    HIPCHECK(hipGetLastError());
+    if (capturing_) {
+        cap_.outs.push_back({outLogL, captureResult(dResult_, 1), 1, 1, true});
+        const size_t off = captureResult(sums, 3 * (size_t)count);
+        double* dst[3] = {outD1, outD1Sq, outD2};
+        for (int k = 0; k < 3; ++k)
+            if (dst[k]) cap_.outs.push_back({dst[k], off + k * sizeof(double), count, 3, false});
+        if (outQ) cap_.outs.push_back({outQ, captureResult(qg.out, 16), 16, 1, false});
+        if (freqs) cap_.outs.push_back({outFreqs, captureResult(qg.out + 16, S), S, 1, false});
+        return BEAGLE_SUCCESS;
+    }
+    const size_t nd = 3 * (size_t)count;
+    reserveStage(1 + nd + 16 + S);
+    const double* hl = stageOut(dResult_, 1, 0);
+    const double* h = stageOut(sums, nd, 1);
+    const double* hq = stageOut(qg.out, 16 + (freqs ? S : 0), 1 + nd);
+    sync();
+    lastLogL_ = hl[0];
+    *outLogL = lastLogL_;
+    for (int o = 0; o < count; ++o) {
+        if (outD1) outD1[o] = h[3 * o];
+        if (outD1Sq) outD1Sq[o] = h[3 * o + 1];
+        if (outD2) outD2[o] = h[3 * o + 2];
+    }
+    if (outQ)
+        for (int q = 0; q < 16; ++q) outQ[q] = hq[q];
+    if (freqs)
+        for (int i = 0; i < S; ++i) outFreqs[i] = hq[16 + i];
     return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
 }
 
@@ -1706,15 +1848,27 @@ bool Instance::makeNodeJobs(const int* ops, int count, const int* postIdx,
 // (makeNodeJobs).  Jobs are planned like operations: wide dependency
 // levels batched, narrow runs fused with the parent forwarded from
 // registers.
-bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, double* partial) {
+//
+// With q the jobs also form the rate-matrix gradient (4 states, FP32; the
+// operations' matrices checked by checkQGradOperands): each job carries its
+// branches' factor offsets, and every run (each timed candidate too) writes
+// its H slots from the first, so the last run's slots are what remain.
+bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, double* partial,
+                              NodeQGrad* q) {
     if (partialsBytes_ >= (size_t(1) << 32)) return false;   // 32-bit offsets in the kernel
     std::vector<NodeJob> nodes;
     std::vector<Access> acc;
     if (!makeNodeJobs(ops, count, postIdx, nodes, acc)) return false;
     const int n = (int)nodes.size();
+    if (q) {
+        for (NodeJob& J : nodes)
+            for (int ch = 0; ch < 2; ++ch)
+                J.mF[ch] = ops[(size_t)J.oidx[ch] * BEAGLE_OP_COUNT + 4] * dims_.C * 16;
+    }
     std::vector<int> level;
     const int nl = dependencyLevels(acc, bufferCount_, level);
     auto run = [&](int k) {
+        if (q) q->used = 0;
         std::vector<Segment> plan;
         if (kPlans[k].fused) {
             Segment sg{false, {}};
@@ -1723,9 +1877,9 @@ bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, dou
         } else {
             plan = planSegments(level, nl, kPlans[k].narrow);
         }
-        launchNodePlan(nodes, plan, partial);
+        launchNodePlan(nodes, plan, partial, q);
     };
-    const int k = tunedChoice(shapeKey(2, n, nl), kNumPlans, forcedPlan(), run);
+    const int k = tunedChoice(shapeKey(q ? 6 : 2, n, nl), kNumPlans, forcedPlan(), run);
     if (k >= 0) run(k);
     return true;
 }
@@ -1733,7 +1887,7 @@ bool Instance::launchPreNodes(const int* ops, int count, const int* postIdx, dou
 // Launch node jobs as planned: fused segments forward each job's parent from
 // the previous job's registers where it is that job's output.
 void Instance::launchNodePlan(const std::vector<NodeJob>& nodes, const std::vector<Segment>& plan,
-                              double* partial) {
+                              double* partial, NodeQGrad* q) {
     std::vector<NodeJob> all;
     struct Launch { int first, n; bool batched; };
     std::vector<Launch> launches;
@@ -1760,7 +1914,7 @@ void Instance::launchNodePlan(const std::vector<NodeJob>& nodes, const std::vect
                                             node_f32_supported(dims_.C));
     for (const Launch& L : launches) {
         if (dp_) launch_pre_nodes<double>(d + L.first, L.n, L.batched, levelChunk_, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
-        else launch_pre_nodes<float>(d + L.first, L.n, L.batched, levelChunk_, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
+        else launch_pre_nodes<float>(d + L.first, L.n, L.batched, levelChunk_, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_, q);
     }
 }
 

@@ -112,6 +112,11 @@ public:
                                     const int* ops, int count, const int* postIdx,
                                     const int* d1Idx, const int* d2Idx, double* outLogL,
                                     double* outD1, double* outD1Sq, double* outD2);
+    int logLikelihoodAndGradient(int rootIdx, int wIdx, int fIdx, int cumIdx, int rootPreIdx,
+                                 const int* ops, int count, const int* postIdx, const int* d1Idx,
+                                 const int* d2Idx, int eigenIdx, double* outLogL, double* outD1,
+                                 double* outD1Sq, double* outD2, double* outQ,
+                                 double* outFreqs);
     int checkpointedGradient(const int* ops, int count, const int* d1Idx, const int* d2Idx,
                              int wIdx, int fIdx, const int* pool, int poolCount, int interval,
                              int* outInterval, int* outPeak, double* outLogL, double* outD1,
@@ -250,12 +255,24 @@ private:
     // dependency levels of a post-order list (reads children and a read
     // scale buffer; writes the destination and a write scale buffer)
     int postLevels(const int* ops, int stride, int count, std::vector<int>& level);
-    bool launchPreNodes(const int* ops, int count, const int* postIdx, double* partial);
+    bool launchPreNodes(const int* ops, int count, const int* postIdx, double* partial,
+                        NodeQGrad* q = nullptr);
     bool makeNodeJobs(const int* ops, int count, const int* postIdx, std::vector<NodeJob>& nodes,
                       std::vector<Access>& acc);
     int enqueueRoot(int b, int w, int f, int cum);
+    // The rate-matrix gradient requested from the derivative pass: eigen
+    // system in, device pointers out (fused false: the pass could not form
+    // it, it must be computed separately)
+    struct QGradPass {
+        int eigen;
+        bool fused;
+        double *hacc, *slotPart, *freqPart, *sums, *out;
+        long slots;   // H slots the pass wrote
+    };
     int enqueuePreDerivatives(const int* ops, int count, const int* postIdx, const int* d1Idx,
-                              const int* d2Idx, int wIdx, bool reduce, const double** dSums);
+                              const int* d2Idx, int wIdx, bool reduce, const double** dSums,
+                              QGradPass* qg = nullptr);
+    int checkQGradOperands(int e, const int* ops, int count, int wIdx);
     void reserveStage(size_t n);
     const double* stageOut(const double* dsrc, size_t n, size_t at);
     static constexpr int kNarrowLevel = 8;   // levels with fewer ops run fused
@@ -303,7 +320,7 @@ private:
     template <typename F>
     int tunedChoice(uint64_t key, int ncand, int forced, F&& run);
     void launchNodePlan(const std::vector<NodeJob>& nodes, const std::vector<Segment>& plan,
-                        double* partial);
+                        double* partial, NodeQGrad* q = nullptr);
     void launchPrePlan(const std::vector<Segment>& plan, const double* cw, double* partial);     // MFMA post-order: 0 partials pipelined, 1 whole tile up front
     int levelChunk_ = 1;      // batched level: 1 one op per workgroup, k > 1 chunks of k ops
     Dims dims_;

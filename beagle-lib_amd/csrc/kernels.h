@@ -112,6 +112,23 @@ struct ConvolveJob {
     const void* A; const void* B; void* out;
 };
 
+// The rate-matrix gradient formed inside the FP32 node pass (4 states,
+// C = 1, 2, 4): Frechet factors (NodeJob::mF offsets into F), V | V^-1
+// (FP64, row-major), and H slots hacc [capacity][16]: each launch writes
+// slots [used, used + rows * tiles) once (rows = its grid's job rows) and
+// advances used; a pass of n jobs needs at most n * tiles slots.
+struct NodeQGrad {
+    const double* F;
+    const double* eig;
+    double* hacc;
+    long capacity;
+    long used;
+};
+// Sum of n H slots ([n][16]) into out[16], through part: [ceil(n / 128)][16]
+// block sums.
+void launch_qgrad_slot_sums(const double* slots, long n, double* part, double* out,
+                            hipStream_t st);
+
 // One node of an edge-derivative or cross-product request.
 struct EdgeJob {
     const void* pre;       // pre-order partials of the node
@@ -278,7 +295,7 @@ void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, 
 template <typename R>
 void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, const Dims& d,
                       const void* jobMats, void* dummy, const double* cw, const double* pw,
-                      double* partial, hipStream_t st);
+                      double* partial, hipStream_t st, NodeQGrad* q = nullptr);
 // Rate-matrix gradient over node jobs (4 states): per-tile, per-category
 // sums of H (16 rows of tiles * C); jobF holds F per (job, category, child)
 // as [job][category][2][16], eig = V | V^-1 row-major.

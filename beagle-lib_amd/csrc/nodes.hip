@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <stdexcept>
 #include <type_traits>
 
 namespace hipbeagle {
@@ -33,6 +34,9 @@ namespace hipbeagle {
 // separates the lanes.
 constexpr int NODE_MATS = 6 * 20;
 constexpr int NODE_F32_BLOCK = 128;       // floats of a wave's block in the FP32 node kernels
+// ... and with the rate-matrix gradient: the matrices, then the factors of
+// the job's two branches for the wave's category ([2][16] doubles)
+constexpr int NODE_F32_QBLOCK = NODE_F32_BLOCK + 64;
 constexpr int NODE_MAT_BLOCK = 64 * 16;   // bytes of one wave's block
 
 // 32-bit addressing: a wave-uniform base pointer plus a per-lane byte
@@ -54,6 +58,7 @@ struct NodeOps4 {
     int sa, sb;                // children's tip states
     int eu, ea, eb;            // exponents (automatic scaling)
     typename Vec16<R>::T mv;   // this lane's share of the job's matrices
+    dvec2 fv;                  // with the rate-matrix gradient: a share of its factors
 };
 
 // Byte offsets: `off` of this lane's 4-state vector in a partials buffer,
@@ -381,6 +386,73 @@ __device__ __forceinline__ void mfma_cols(const float* m, int r, const float (&u
     for (int i = 0; i < 4; ++i) t[i] = acc[i];
 }
 
+// ---------------------------------------------------------------------------
+// FP32 rate-matrix gradient (4 states) on the matrix cores: the sums of
+// qw_compute (H += F o ((V^T u)(V^-1 x)^T) w_c w_p / L_p per branch) for
+// lanes that hold one pattern of one category each, the four lanes of every
+// 4-lane block the same category.  In v_mfma_f32_4x4x1f32 three chains of
+// instructions give per block, with lane 4b + s holding component s of the
+// block's four patterns i (the products transpose):
+//   alpha_i = V^T (coef_i top_i)   A = the lane's own vector, B = V[k][s]
+//   beta_i  = V^-1 x_i             A = the lane's own vector, B = V^-1[s][k]
+//   M^T     = sum_i beta_i alpha_i^T   (A = beta_i, B = alpha_i; lane 4b + s
+//             ends with row s of M = sum_i alpha_i beta_i^T)
+// and the lane adds F[s][.] o M[s][.] to its four FP64 accumulators (F of
+// its category and branch).  V and V^-1 enter rounded to FP32: on the
+// 512-taxon, 50,000-pattern workload dlogL/dQ stays within 2.5e-8 of the
+// FP64 path (relative to its largest entry), as with V split into FP32
+// hi + lo parts (2.1e-8), at 12 instructions per branch instead of 20.
+// Lanes with the same s are summed at the end.
+
+struct QgVec {
+    float vh[4];   // V[k][s], s = lane % 4
+    float wh[4];   // V^-1[s][k]
+};
+
+__device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&x)[4],
+                                            const QgVec& v, const double* f, double (&Hr)[4]) {
+    fvec4 al = {0.0f, 0.0f, 0.0f, 0.0f}, be = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        al = __builtin_amdgcn_mfma_f32_4x4x1f32(ta[k], v.vh[k], al, 0, 0, 0);
+        be = __builtin_amdgcn_mfma_f32_4x4x1f32(x[k], v.wh[k], be, 0, 0, 0);
+    }
+    fvec4 m = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) m = __builtin_amdgcn_mfma_f32_4x4x1f32(be[i], al[i], m, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Hr[i] += f[i] * (double)m[i];
+}
+
+// The same with V^T and V^-1 (FP32, [2][4][4]: row s of each) in LDS, read
+// where the products need them rather than held in registers.
+__device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&x)[4],
+                                            const float* sv, int s, const double* f,
+                                            double (&Hr)[4]) {
+    QgVec v;
+    const fvec4 a = *(const fvec4*)(sv + 4 * s), b = *(const fvec4*)(sv + 16 + 4 * s);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v.vh[k] = a[k];
+        v.wh[k] = b[k];
+    }
+    qgrad_block(ta, x, v, f, Hr);
+}
+
+// V^T and V^-1 rows into LDS (threads 0-31 of the workgroup)
+__device__ __forceinline__ void qgrad_vectors_lds(const double* __restrict__ eig, float* sv) {
+    const int t = threadIdx.x;
+    if (t < 32) sv[t] = (float)(t < 16 ? eig[(t & 3) * 4 + (t >> 2)] : eig[t]);
+}
+
+__device__ __forceinline__ void qgrad_vectors(const double* __restrict__ eig, int s, QgVec& v) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v.vh[k] = (float)eig[k * 4 + s];
+        v.wh[k] = (float)eig[16 + s * 4 + k];
+    }
+}
+
 // t = M x for a child: its partials (matrix cores), or for a tip the
 // column of its state (s in [0, 4], a per-lane read; column 4 is the pad
 // column of a missing state).
@@ -431,10 +503,11 @@ __device__ __forceinline__ T unset() {
     return x;
 }
 
-template <bool AUTO>
+template <bool AUTO, bool QG = false>
 __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float* __restrict__ jm,
                                                   int c, unsigned off, int p, int lane,
-                                                  NodeOps4<float>& o) {
+                                                  NodeOps4<float>& o,
+                                                  const double* __restrict__ F = nullptr) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         o.u[i] = unset<float>();
@@ -455,6 +528,13 @@ __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float*
         const int l = min(lane, NODE_MATS / 4 - 1);
         o.mv = *at<fvec4>(jm, (unsigned)((J.mbase + c * NODE_MATS + l * 4) * (int)sizeof(float)));
     }
+    if (QG) {
+        // lanes 0-7: child a's 16 factors of category c, lanes 8-15: child
+        // b's (lanes past 15 repeat lane 15's pair; a select of the two
+        // fields, not a lane-dependent index into the job)
+        const int fl = min(lane, 15);
+        o.fv = *at<dvec2>(F, (unsigned)(((fl >> 3 ? J.mF[1] : J.mF[0]) + c * 16 + 2 * (fl & 7)) * 8));
+    }
     if (AUTO) {
         if (J.kPar == 0 && J.eParent != nullptr) o.eu = *at<short>(J.eParent, 2u * p);
         if (J.ex[0] != nullptr) o.ea = *at<short>(J.ex[0], 2u * p);
@@ -462,13 +542,16 @@ __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float*
     }
 }
 
-template <int C, bool AUTO>
+template <int C, bool AUTO, bool QG = false, int NPAR = 2>
 __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps4<float>& o,
                                                  float (&pa)[4], float (&pb)[4], int& pea,
                                                  int& peb, float* smat, float* sx, int& k, int c,
                                                  int lane, int p, int P, unsigned off,
                                                  const float (&wc)[C], float w, double* partial,
-                                                 int tile, int ntiles) {
+                                                 int tile, int ntiles, const float* sv,
+                                                 double (&Hr)[4]) {
+    constexpr int NQ = QG ? 9 : 8;   // quantities exchanged per pattern and category
+    constexpr int BLK = QG ? NODE_F32_QBLOCK : NODE_F32_BLOCK;
     const bool in = p < P;
     float u[4];
 #pragma unroll
@@ -478,9 +561,13 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     // tip states are stored in [0, 4]; the clamp keeps the column index in
     // the matrix whatever a lane without a tip read
     const int sa = min((unsigned)o.sa, 4u), sb = min((unsigned)o.sb, 4u);
-    const int par = k & 1;
-    float* lm = smat + (c * 2 + par) * NODE_F32_BLOCK;
+    // NPAR 2: alternate buffers from job to job (one barrier per job);
+    // NPAR 1: one job per workgroup
+    const int par = NPAR == 2 ? (k & 1) : 0;
+    float* lm = smat + (c * NPAR + par) * BLK;
+    double* lf = reinterpret_cast<double*>(lm + NODE_F32_BLOCK);   // QG: [2][16] factors
     *(fvec4*)(lm + min(lane, NODE_MATS / 4 - 1) * 4) = o.mv;
+    if (QG) *(dvec2*)(lf + 2 * min(lane, 15)) = o.fv;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
@@ -492,13 +579,15 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     child_product(tipB, sb, r, o.xb, lm + 20, t);  // P_b x_b
 #pragma unroll
     for (int i = 0; i < 4; ++i) ua[i] = u[i] * t[i];
+    // QG: this category's likelihood of the pattern, sum pre(v) P_a x_a P_b x_b
+    const float Lc = QG ? ub[0] * t[0] + ub[1] * t[1] + ub[2] * t[2] + ub[3] * t[3] : 0.0f;
     mfma_cols(lm, r, ua, va);                      // pre(a) = P_a^T (pre(v) * P_b x_b)
     mfma_cols(lm + 20, r, ub, vb);                 // pre(b) = P_b^T (pre(v) * P_a x_a)
     float dA, n1A, n2A, dB, n1B, n2B;
     branch_sums_f32(tipA, sa, r, va, o.xa, lm + 40, lm + 60, dA, n1A, n2A);
     branch_sums_f32(tipB, sb, r, vb, o.xb, lm + 80, lm + 100, dB, n1B, n2B);
-    // exchange: max a, max b, den/n1/n2 of a, of b
-    float* buf = sx + par * 8 * C * TILE;
+    // exchange: max a, max b, den/n1/n2 of a, of b (QG: and Lc)
+    float* buf = sx + par * NQ * C * TILE;
     ++k;
     float* mine = buf + c * TILE + lane;
     mine[0 * C * TILE] = in ? fmaxf(fmaxf(va[0], va[1]), fmaxf(va[2], va[3])) : 0.0f;
@@ -509,9 +598,39 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     mine[5 * C * TILE] = dB;
     mine[6 * C * TILE] = n1B;
     mine[7 * C * TILE] = n2B;
+    if (QG) mine[8 * C * TILE] = Lc;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    if (QG) {
+        // rate-matrix gradient of both branches: top_a = pre(v) * P_b x_b
+        // (ua), top_b = pre(v) * P_a x_a (ub); L_p = sum_c w_c Lc, the same
+        // order of additions in every wave; scale factors cancel in
+        // top x^T / L_p
+        float L = 0.0f, wcc = wc[0];
+#pragma unroll
+        for (int cc = 0; cc < C; ++cc) {
+            L += wc[cc] * buf[(8 * C + cc) * TILE + lane];
+            if (c == cc) wcc = wc[cc];
+        }
+        const float coef = in ? (w * wcc) / L : 0.0f;
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+            const bool tip = ch == 0 ? tipA : tipB;
+            const int st = ch == 0 ? sa : sb;
+            float ta[4], x[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                // a tip's vector is one-hot; a missing state (4) reads the
+                // pad column, which does not depend on Q: zero; lanes past
+                // the patterns add nothing
+                ta[i] = in ? coef * (ch == 0 ? ua[i] : ub[i]) : 0.0f;
+                const float xv = ch == 0 ? o.xa[i] : o.xb[i];
+                x[i] = !in ? 0.0f : tip ? (st == i ? 1.0f : 0.0f) : xv;
+            }
+            qgrad_block(ta, x, sv, r, lf + ch * 16 + r * 4, Hr);
+        }
+    }
     const bool keepA = in && J.dest[0] != nullptr;
     const bool keepB = in && J.dest[1] != nullptr;
     int outA = 0, outB = 0;
@@ -570,11 +689,11 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     peb = outB;
 }
 
-#define NODE_F32_SETUP                                                                        \
-    extern __shared__ double sdv[]; /* [2][8][C][TILE] floats, then [C][2] matrix blocks */   \
+#define NODE_F32_SETUP(NQ, NPAR)                                                              \
+    extern __shared__ double sdv[]; /* [NPAR][NQ][C][TILE] floats, [C][NPAR] matrix blocks */ \
     static_assert(NODE_MATS <= NODE_F32_BLOCK, "matrix block");                               \
     float* sx = reinterpret_cast<float*>(sdv);                                                \
-    float* smat = sx + 2 * 8 * C * TILE;                                                      \
+    float* smat = sx + (NPAR) * (NQ) * C * TILE;                                              \
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);                          \
     const int lane = threadIdx.x & 63;                                                        \
     const int tile = blockIdx.x;                                                              \
@@ -588,61 +707,123 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     int k = 0, pea = 0, peb = 0;                                                              \
     float pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
 
+// With the rate-matrix gradient (QG): the workgroup's H (the lanes' shares
+// summed over its jobs, then over the lanes with the same lane % 4 and over
+// the category waves in order) is written to its own slot of hacc
+// ([slot][16], slot = base + chunk * tiles + tile); every slot of a pass is
+// written once, by one workgroup, so the pass needs no zeroed accumulator
+// and its sums do not depend on the timing of the workgroups.
+template <int C>
+__device__ __forceinline__ void qgrad_store(double (&Hr)[4], int c, int lane, double* red,
+                                            double* __restrict__ hacc, long slot) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int m = 4; m < 64; m <<= 1) Hr[i] += __shfl_xor(Hr[i], m, 64);
+    }
+    if (lane < 4) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) red[c * 16 + lane * 4 + i] = Hr[i];
+    }
+    __syncthreads();
+    if (c == 0 && lane < 16) {
+        double h = red[lane];
+#pragma unroll
+        for (int cc = 1; cc < C; ++cc) h += red[cc * 16 + lane];
+        gout<double>(hacc)[slot * 16 + lane] = h;
+    }
+}
+
+// LDS of the FP32 node kernels: the exchange and the waves' matrix (and
+// factor) blocks, NPAR of each (2 for a list of jobs, 1 for one job per
+// workgroup), and with QG the workgroup's H by category and V^T | V^-1
+template <int C, bool QG, int NPAR>
+constexpr size_t node_f32_lds() {
+    return (size_t)(NPAR * (QG ? 9 : 8) * C * TILE +
+                    C * NPAR * (QG ? NODE_F32_QBLOCK : NODE_F32_BLOCK)) * sizeof(float) +
+           (QG ? 16 * C * sizeof(double) + 32 * sizeof(float) : 0);
+}
+
 // Each job's operands are loaded when the job starts (node_issue_direct):
 // at most 64 VGPRs, so eight waves per SIMD and a 100,000-pattern grid
 // resident at once, which covers the loads' latency better than loading
 // the next job ahead in more registers does.
-template <int C, bool AUTO>
-__global__ __launch_bounds__(64 * C, 7) void k_pren4f(const NodeJob* __restrict__ jobs, int njobs,
-                                                      int chunk, int P, long cs,
-                                                      const float* __restrict__ jm,
-                                                      const double* __restrict__ cw,
-                                                      const double* __restrict__ pw,
-                                                      double* partial) {
-    NODE_F32_SETUP
+template <int C, bool AUTO, bool QG>
+__global__ __launch_bounds__(64 * C, QG ? 4 : 7) void k_pren4f(
+    const NodeJob* __restrict__ jobs, int njobs, int chunk, int P, long cs,
+    const float* __restrict__ jm, const double* __restrict__ cw, const double* __restrict__ pw,
+    double* partial, const double* __restrict__ F, const double* __restrict__ eig,
+    double* __restrict__ hacc, long base) {
+    NODE_F32_SETUP(QG ? 9 : 8, 2)
+    double* red = reinterpret_cast<double*>(smat + C * 2 * NODE_F32_QBLOCK);
+    float* sv = reinterpret_cast<float*>(red + 16 * C);
+    double Hr[4] = {0.0, 0.0, 0.0, 0.0};
+    if (QG) qgrad_vectors_lds(eig, sv);   // read after the first job's barrier
     jobs += blockIdx.y * chunk;
     njobs = min(chunk, njobs - (int)blockIdx.y * chunk);
     for (int o = 0; o < njobs; ++o) {
         const NodeJob J = jobs[o];
         NodeOps4<float> S0;
-        node_issue_direct<AUTO>(J, jm, c, off, p, lane, S0);
-        node_compute_f32<C, AUTO>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, wc, w,
-                                  partial, tile, ntiles);
+        node_issue_direct<AUTO, QG>(J, jm, c, off, p, lane, S0, F);
+        node_compute_f32<C, AUTO, QG>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
+                                      wc, w, partial, tile, ntiles, sv, Hr);
     }
+    if (QG) qgrad_store<C>(Hr, c, lane, red, hacc, base + (long)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
-template <int C, bool AUTO>
-__global__ __launch_bounds__(64 * C) void k_pren4f_level(const NodeJob* __restrict__ jobs,
-                                                         int P, long cs,
-                                                         const float* __restrict__ jm,
-                                                         const double* __restrict__ cw,
-                                                         const double* __restrict__ pw,
-                                                         double* partial) {
-    NODE_F32_SETUP
+template <int C, bool AUTO, bool QG>
+__global__ __launch_bounds__(64 * C, QG ? 7 : 8) void k_pren4f_level(
+    const NodeJob* __restrict__ jobs, int P, long cs, const float* __restrict__ jm,
+    const double* __restrict__ cw, const double* __restrict__ pw, double* partial,
+    const double* __restrict__ F, const double* __restrict__ eig, double* __restrict__ hacc,
+    long base) {
+    NODE_F32_SETUP(QG ? 9 : 8, 1)
     NodeOps4<float> S0;
+    double* red = reinterpret_cast<double*>(smat + C * NODE_F32_QBLOCK);
+    float* sv = reinterpret_cast<float*>(red + 16 * C);
+    double Hr[4] = {0.0, 0.0, 0.0, 0.0};
+    if (QG) qgrad_vectors_lds(eig, sv);   // read after the job's barrier
     const NodeJob J = jobs[blockIdx.y];
-    node_issue_direct<AUTO>(J, jm, c, off, p, lane, S0);
-    node_compute_f32<C, AUTO>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off, wc, w,
-                              partial, tile, ntiles);
+    node_issue_direct<AUTO, QG>(J, jm, c, off, p, lane, S0, F);
+    node_compute_f32<C, AUTO, QG, 1>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
+                                     wc, w, partial, tile, ntiles, sv, Hr);
+    if (QG) qgrad_store<C>(Hr, c, lane, red, hacc, base + (long)blockIdx.y * gridDim.x + blockIdx.x);
 }
 #undef NODE_F32_SETUP
 
-template <int C>
+// sum over the four categories of a pattern held by lanes j, j + 16, j + 32,
+// j + 48 (the same order of additions in every lane)
+__device__ __forceinline__ float cat_sum(float v) {
+    v += __shfl_xor(v, 16, 64);
+    return v + __shfl_xor(v, 32, 64);
+}
+
+template <int C, bool QG>
 static void launch_pre_nodes_f32(const NodeJob* djobs, int njobs, bool batched, int chunk,
                                  const Dims& d, const float* m, const double* cw,
-                                 const double* pw, double* partial, hipStream_t st) {
+                                 const double* pw, double* partial, hipStream_t st,
+                                 NodeQGrad* q) {
     const int tiles = (d.P + TILE - 1) / TILE;
-    const size_t lds = (size_t)(2 * 8 * C * TILE + C * 2 * NODE_F32_BLOCK) * sizeof(float);
     const dim3 block(TILE * C);
+    const double* F = QG ? q->F : nullptr;
+    const double* eig = QG ? q->eig : nullptr;
+    double* hacc = QG ? q->hacc : nullptr;
+    const long base = QG ? q->used : 0;
+    int rows;
     if (batched && chunk <= 1) {
-        if (d.autoScale) hipLaunchKernelGGL((k_pren4f_level<C, true>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial);
-        else hipLaunchKernelGGL((k_pren4f_level<C, false>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial);
+        rows = njobs;
+        const size_t lds = node_f32_lds<C, QG, 1>();
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4f_level<C, true, QG>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
+        else hipLaunchKernelGGL((k_pren4f_level<C, false, QG>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
     } else {
         if (!batched || chunk > njobs) chunk = njobs;
-        const dim3 grid(tiles, (njobs + chunk - 1) / chunk);
-        if (d.autoScale) hipLaunchKernelGGL((k_pren4f<C, true>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial);
-        else hipLaunchKernelGGL((k_pren4f<C, false>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial);
+        rows = (njobs + chunk - 1) / chunk;
+        const dim3 grid(tiles, rows);
+        const size_t lds = node_f32_lds<C, QG, 2>();
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4f<C, true, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
+        else hipLaunchKernelGGL((k_pren4f<C, false, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
     }
+    if (QG) q->used += (long)rows * tiles;
 }
 
 // ---------------------------------------------------------------------------
@@ -977,6 +1158,115 @@ __global__ __launch_bounds__(256) void k_qgrad4w(const NodeJob* __restrict__ job
     }
 }
 
+// t = M x with M row-major [4][5] (the model's layout), lane = pattern
+__device__ __forceinline__ void mfma_rows5(const float* m, int r, const float (&x)[4],
+                                           float (&t)[4]) {
+    fvec4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m[r * 5 + k], x[k], acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t[i] = acc[i];
+}
+
+__device__ __forceinline__ void qm_compute(const NodeJob& J, const QWOps<float>& o,
+                                           const float* sm, const double* sf, int c, int lane,
+                                           bool in, float w, float wc, const QgVec& v,
+                                           double (&Hr)[4]) {
+    const bool tipA = J.k[0] == 1, tipB = J.k[1] == 1;
+    const float* Pa = sm + c * 20;
+    const float* Pb = sm + 80 + c * 20;
+    const int r = lane & 3;
+    const int ca = min((unsigned)o.sa, 4u), cb = min((unsigned)o.sb, 4u);
+    float ya[4], yb[4];
+    if (tipA) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ya[i] = Pa[i * 5 + ca];
+    } else {
+        mfma_rows5(Pa, r, o.xa, ya);
+    }
+    if (tipB) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) yb[i] = Pb[i * 5 + cb];
+    } else {
+        mfma_rows5(Pb, r, o.xb, yb);
+    }
+    float L = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) L += o.u[i] * ya[i] * yb[i];
+    L = cat_sum(wc * L);
+    const float coef = in ? (w * wc) / L : 0.0f;
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+        const bool tip = ch == 0 ? tipA : tipB;
+        const int st = ch == 0 ? ca : cb;
+        float ta[4], x[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            // a tip's vector is one-hot; a missing state (4) reads the pad
+            // column, which does not depend on Q: zero; lanes past the
+            // patterns add nothing
+            ta[i] = in ? coef * (o.u[i] * (ch == 0 ? yb[i] : ya[i])) : 0.0f;
+            const float xv = ch == 0 ? o.xa[i] : o.xb[i];
+            x[i] = !in ? 0.0f : tip ? (st == i ? 1.0f : 0.0f) : xv;
+        }
+        qgrad_block(ta, x, v, sf + ch * 64 + c * 16 + r * 4, Hr);
+    }
+}
+
+// grid (tiles of 64 patterns, chunks of the job list), 4 waves of 16
+// patterns x 4 categories; partial rows [16][cols], column (chunk, wave)
+__global__ __launch_bounds__(256, 4) void k_qgrad4m(const NodeJob* __restrict__ jobs, int njobs,
+                                                 int P, long cs, const float* __restrict__ mats,
+                                                 const double* __restrict__ F,
+                                                 const double* __restrict__ eig,
+                                                 const double* __restrict__ cw,
+                                                 const double* __restrict__ pw,
+                                                 double* __restrict__ partial) {
+    __shared__ double sfb[4][128];
+    __shared__ float smb[4][160];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = lane >> 4, j = lane & 15;
+    const int wave = blockIdx.x * 4 + wv;
+    const int nwaves = gridDim.x * 4;
+    const int p = wave * 16 + j;            // < Ppad (a multiple of 64)
+    const bool in = p < P;
+    const float wc = (float)cw[c];
+    const float w = (float)pw[p];           // pattern weights are zero past P
+    const unsigned off = (unsigned)((c * cs + (long)p * 4) * (long)sizeof(float));
+    float* sm = smb[wv];
+    double* sf = sfb[wv];
+    QgVec v;
+    qgrad_vectors(eig, lane & 3, v);
+    double Hr[4] = {0.0, 0.0, 0.0, 0.0};
+    const int chunk = (njobs + gridDim.y - 1) / gridDim.y;
+    const int o0 = blockIdx.y * chunk, o1 = min(njobs, o0 + chunk);
+    QWOps<float> A, B;
+    if (o0 < o1) qw_issue<float>(jobs[o0], off, p, lane, mats, F, A);
+    for (int o = o0; o < o1; o += 2) {
+        const NodeJob& Ja = jobs[o];
+        qw_stage<float>(A, lane, sm, sf);
+        if (o + 1 < o1) qw_issue<float>(jobs[o + 1], off, p, lane, mats, F, B);
+        qm_compute(Ja, A, sm, sf, c, lane, in, w, wc, v, Hr);
+        if (o + 1 >= o1) break;
+        const NodeJob& Jb = jobs[o + 1];
+        qw_stage<float>(B, lane, sm, sf);
+        if (o + 2 < o1) qw_issue<float>(jobs[o + 2], off, p, lane, mats, F, A);
+        qm_compute(Jb, B, sm, sf, c, lane, in, w, wc, v, Hr);
+    }
+    // lanes with the same lane % 4 hold the same entries of H
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int m = 4; m < 64; m <<= 1) Hr[i] += __shfl_xor(Hr[i], m, 64);
+    }
+    const long cols = (long)nwaves * gridDim.y;
+    const long col = (long)blockIdx.y * nwaves + wave;
+    if (lane < 4) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) partial[(lane * 4 + i) * cols + col] = Hr[i];
+    }
+}
+
 // Chunks of the job list the Q-gradient kernels spread over the grid's
 // second dimension: about 16 waves per SIMD (1,024 SIMDs), at least 16 jobs
 // per chunk.
@@ -1000,6 +1290,12 @@ void launch_qgrad(const NodeJob* djobs, int njobs, const Dims& d, const void* jo
     if (njobs <= 0) return;
     if (d.C == 4) {
         const int tiles = (d.P + TILE - 1) / TILE;
+        if (std::is_same<R, float>::value) {
+            hipLaunchKernelGGL(k_qgrad4m, dim3(tiles, qgrad_chunks(d, njobs)), dim3(256), 0, st,
+                               djobs, njobs, d.P, d.catStride,
+                               reinterpret_cast<const float*>(mats), jobF, eig, cw, pw, partial);
+            return;
+        }
         hipLaunchKernelGGL((k_qgrad4w<R>), dim3(tiles, qgrad_chunks(d, njobs)), dim3(256), 0, st,
                            djobs, njobs, d.P, d.catStride, reinterpret_cast<const R*>(mats), jobF,
                            eig, cw, pw, partial);
@@ -1016,6 +1312,50 @@ void launch_qgrad(const NodeJob* djobs, int njobs, const Dims& d, const void* jo
     hipLaunchKernelGGL((k_qgrad4<R>), dim3(tiles, chunks), dim3(TILE * d.C), lds, st, djobs, njobs, d.P,
                        d.C, d.catStride, reinterpret_cast<const R*>(jobMats), jobF, eig, dummyX,
                        dummyS, cw, pw, partial);
+}
+
+// Sum of the node pass's H slots ([n][16] doubles) in two fixed-order
+// steps: blocks of 16 slots x 16 entries, each thread adding at most
+// kSlotSumLoads slots with its loads issued together; then one block per
+// entry adds the block sums.
+constexpr int kSlotSumLoads = 8;
+
+__global__ __launch_bounds__(256) void k_slot_sums16(const double* __restrict__ in, long n,
+                                                     double* __restrict__ part) {
+    __shared__ double sh[16][16];
+    const int j = threadIdx.x & 15, r = threadIdx.x >> 4;
+    double x[kSlotSumLoads];
+#pragma unroll
+    for (int q = 0; q < kSlotSumLoads; ++q) {
+        const long i = ((long)blockIdx.x * kSlotSumLoads + q) * 16 + r;
+        x[q] = i < n ? in[i * 16 + j] : 0.0;
+    }
+    double v = x[0];
+#pragma unroll
+    for (int q = 1; q < kSlotSumLoads; ++q) v += x[q];
+    sh[r][j] = v;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double s = sh[0][j];
+        for (int q = 1; q < 16; ++q) s += sh[q][j];
+        part[blockIdx.x * 16 + j] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_slot_column_sums(const double* __restrict__ part,
+                                                          int nb, double* __restrict__ out) {
+    __shared__ double sh[4];
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) v += part[(long)b * 16 + blockIdx.x];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+void launch_qgrad_slot_sums(const double* slots, long n, double* part, double* out,
+                            hipStream_t st) {
+    const int nb = (int)std::max<long>(1, (n + 16 * kSlotSumLoads - 1) / (16 * kSlotSumLoads));
+    hipLaunchKernelGGL(k_slot_sums16, dim3(nb), dim3(256), 0, st, slots, n, part);
+    hipLaunchKernelGGL(k_slot_column_sums, dim3(16), dim3(256), 0, st, part, nb, out);
 }
 
 // dlogL/dQ = V^-T H V^T (one wave), the root-frequency sums copied after it.
@@ -1114,7 +1454,7 @@ void launch_gather_node_matrices(const NodeJob* djobs, int njobs, const Dims& d,
 template <typename R>
 void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, const Dims& d,
                       const void* jobMats, void* dummy, const double* cw, const double* pw,
-                      double* partial, hipStream_t st) {
+                      double* partial, hipStream_t st, NodeQGrad* q) {
     if (njobs <= 0) return;
     const R* dummyX = reinterpret_cast<const R*>(dummy);
     R* dummyW = const_cast<R*>(dummyX) + 8 * TILE * 4;
@@ -1124,11 +1464,21 @@ void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, 
     const R* m = reinterpret_cast<const R*>(jobMats);
     if (std::is_same<R, float>::value && node_f32_supported(d.C)) {
         const float* mf = reinterpret_cast<const float*>(m);
-        if (d.C == 4) launch_pre_nodes_f32<4>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st);
-        else if (d.C == 2) launch_pre_nodes_f32<2>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st);
-        else launch_pre_nodes_f32<1>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st);
+        if (q) {
+            // a pass never needs more slots than jobs x tiles
+            if (q->used + (long)njobs * ((d.P + TILE - 1) / TILE) > q->capacity)
+                throw std::runtime_error("rate-matrix gradient slots exhausted");
+            if (d.C == 4) launch_pre_nodes_f32<4, true>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st, q);
+            else if (d.C == 2) launch_pre_nodes_f32<2, true>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st, q);
+            else launch_pre_nodes_f32<1, true>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st, q);
+        } else {
+            if (d.C == 4) launch_pre_nodes_f32<4, false>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st, q);
+            else if (d.C == 2) launch_pre_nodes_f32<2, false>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st, q);
+            else launch_pre_nodes_f32<1, false>(djobs, njobs, batched, chunk, d, mf, cw, pw, partial, st, q);
+        }
         return;
     }
+    if (q) throw std::runtime_error("the rate-matrix gradient pass needs FP32 with 1, 2 or 4 categories");
     const int tiles = (d.P + TILE - 1) / TILE;
     const size_t lds = (size_t)2 * d.C * 8 * TILE * sizeof(double) +
                        (size_t)d.C * 2 * NODE_MAT_BLOCK;
@@ -1157,7 +1507,7 @@ void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, 
                                            const double*, double*, const Dims&, hipStream_t);   \
     template void launch_pre_nodes<R>(const NodeJob*, int, bool, int, const Dims&, const void*, \
                                       void*, const double*, const double*, double*,             \
-                                      hipStream_t);
+                                      hipStream_t, NodeQGrad*);
 
 INSTANTIATE(double)
 INSTANTIATE(float)

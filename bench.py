@@ -218,27 +218,33 @@ def full_gradient_bench(lib, steps, warmup):
 
 def phase_timing(ev, steps):
     """Device time of each phase of one evaluation (HIP events on the
-    instance's stream, ASYNCH instance, averaged over `steps` evaluations):
-    matrices, post-order, root + fused pre-order/derivative pass, rate-matrix
-    and root-frequency gradient."""
+    instance's stream, ASYNCH instance, medians over `steps` evaluations):
+    matrices, post-order, then the one call that forms the root logL, the
+    pre-order partials, the branch derivatives and the rate-matrix and
+    root-frequency gradients (beagleExtCalculateLogLikelihoodAndGradient),
+    and for comparison the same work as two calls (the derivative pass,
+    then beagleExtCalculateRateMatrixGradient's own pass)."""
     import numpy as np
     import torch
     st = torch.cuda.ExternalStream(ev.inst.stream())
     ops, post, d1, d2 = ev.fused_operations()
-    names = ("matrices", "postorder", "root_preorder_derivatives", "rate_matrix_gradient")
+    names = ("matrices", "postorder", "root_preorder_derivatives_rate_matrix",
+             "two_calls_root_preorder_derivatives", "two_calls_rate_matrix_gradient")
     acc = {k: [] for k in names}
     root = ev.prob.tree.root
     for _ in range(max(steps // 2, 5)):
-        ev_ = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        ev_ = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
         ev_[0].record(st)
         ev.update_matrices()
         ev_[1].record(st)
         ev.post_order()
         ev_[2].record(st)
-        ev.inst.log_likelihood_and_derivatives(root, ev.pre_base + root, ops, post, d1, d2)
+        ev.inst.log_likelihood_and_gradient(root, ev.pre_base + root, ops, post, d1, d2)
         ev_[3].record(st)
-        ev.rate_matrix_gradient()
+        ev.inst.log_likelihood_and_derivatives(root, ev.pre_base + root, ops, post, d1, d2)
         ev_[4].record(st)
+        ev.rate_matrix_gradient()
+        ev_[5].record(st)
         torch.cuda.synchronize()
         for k, name in enumerate(names):
             acc[name].append(ev_[k].elapsed_time(ev_[k + 1]))

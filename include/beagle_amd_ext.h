@@ -217,6 +217,37 @@ int beagleExtCalculateRateMatrixGradient(int instance, int eigenIndex,
                                           double* outGradientRateMatrix,
                                           double* outGradientFrequencies);
 
+/* beagleExtCalculateLogLikelihoodAndDerivatives and
+ * beagleExtCalculateRateMatrixGradient in one call with one wait:
+ *   beagleExtCalculateLogLikelihoodAndDerivatives(instance, rootBufferIndex,
+ *       categoryWeightsIndex, stateFrequenciesIndex, cumulativeScaleIndex,
+ *       rootPreBufferIndex, operations, operationCount, postBufferIndices,
+ *       firstDerivativeIndices, secondDerivativeIndices, outLogLikelihood,
+ *       outSumFirstDerivatives, outSumSquaredFirstDerivatives,
+ *       outSumSecondDerivatives);
+ *   beagleExtCalculateRateMatrixGradient(instance, eigenIndex, operations,
+ *       operationCount, postBufferIndices, categoryWeightsIndex,
+ *       rootBufferIndex, stateFrequenciesIndex, outGradientRateMatrix,
+ *       outGradientFrequencies);
+ * the full gradient a sampler of branch lengths and substitution-model
+ * parameters needs at each step.  For 4 states in FP32 with 1, 2 or 4
+ * categories and node-paired operations the rate-matrix gradient is formed
+ * inside the derivative pass, from the partials and matrices that pass
+ * already reads (no second traversal); otherwise the two calls run in turn.
+ * Same arguments, outputs and errors as the two calls; recorded by
+ * beagleExtBeginCapture on the fused path.  The summation order of the
+ * rate-matrix sums follows the traversal plan the instance picks, so with
+ * autotuning on the first (tuning) call may differ from later ones in the
+ * last bits.  No counterpart in the reference. */
+int beagleExtCalculateLogLikelihoodAndGradient(
+    int instance, int rootBufferIndex, int categoryWeightsIndex, int stateFrequenciesIndex,
+    int cumulativeScaleIndex, int rootPreBufferIndex, const BeagleOperation* operations,
+    int operationCount, const int* postBufferIndices, const int* firstDerivativeIndices,
+    const int* secondDerivativeIndices, int eigenIndex, double* outLogLikelihood,
+    double* outSumFirstDerivatives, double* outSumSquaredFirstDerivatives,
+    double* outSumSecondDerivatives, double* outGradientRateMatrix,
+    double* outGradientFrequencies);
+
 /* beagleCalculateRootLogLikelihoods for one root (count = 1, a category
  * weights index >= 0) that allocates no device memory.  No counterpart in
  * the reference: for callers that cannot call hipMalloc (serving loops,

@@ -736,6 +736,71 @@ def test_rate_matrix_gradient_against_oracle_finite_differences(amd, oracle, sca
         assert abs(gpi[i] - fd) <= tol * max(abs(fd), np.abs(gpi).max()), (i, gpi[i], fd)
 
 
+@pytest.mark.parametrize("C,scaling,keep_tips,precision,tips_as",
+                         [(4, "auto", False, "single", "states"),
+                          (4, "none", True, "single", "partials"),
+                          (2, "auto", False, "single", "states"),
+                          (1, "auto", True, "single", "states"),
+                          (3, "auto", False, "single", "states"),
+                          (4, "auto", False, "double", "states")])
+def test_log_likelihood_and_gradient_one_call(amd, C, scaling, keep_tips, precision, tips_as):
+    """beagleExtCalculateLogLikelihoodAndGradient against the two calls it
+    stands for (beagleExtCalculateLogLikelihoodAndDerivatives, then
+    beagleExtCalculateRateMatrixGradient) on the same instance: logL,
+    branch derivatives and dlogL/dpi_root bit for bit; dlogL/dQ (formed
+    inside the FP32 derivative pass for 1, 2 and 4 categories, by the two
+    calls otherwise) within 1e-6 of the largest component in FP32 and bit
+    for bit on the two-call fallbacks (3 categories, FP64)."""
+    prob = synth.make_problem(71 + C, 23, 1500, S=4, C=C, shape="yule", missing=0.05,
+                              mean_length=0.2)
+    ev = driver.Evaluator(amd, prob, scaling=scaling, precision=precision, tips_as=tips_as)
+    sep = ev.fused_gradient(keep_tips=keep_tips, rate_matrix="separate")
+    ev.fused_gradient(keep_tips=keep_tips, rate_matrix=True)   # plan tuned once
+    one = ev.fused_gradient(keep_tips=keep_tips, rate_matrix=True)
+    again = ev.fused_gradient(keep_tips=keep_tips, rate_matrix=True)
+    ev.close()
+    assert one[0] == sep[0]
+    for k in (1, 2, 4):
+        np.testing.assert_array_equal(one[k], sep[k])
+    for a, b in zip(one, again):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+    fused = precision == "single" and C in (1, 2, 4)
+    if fused:
+        assert np.abs(one[3] - sep[3]).max() <= 1e-6 * np.abs(sep[3]).max()
+    else:
+        np.testing.assert_array_equal(one[3], sep[3])
+
+
+def test_log_likelihood_and_gradient_against_oracle_finite_differences(amd, oracle):
+    """The one-call FP32 gradient (dlogL/dQ formed inside the derivative
+    pass) against fourth-order differences of the oracle's log-likelihood
+    along random directions, as the separate rate-matrix gradient is
+    checked; and dlogL/dt against the oracle's edge derivatives."""
+    prob = synth.make_problem(61, 9, 400, S=4, shape="yule", missing=0.05, mean_length=0.2)
+    ev = driver.Evaluator(amd, prob, scaling="auto", precision="single")
+    lnl, d1, d2, G, gpi = ev.fused_gradient(rate_matrix=True)
+    ev.close()
+    ref = driver.Evaluator(oracle, prob)
+    want = ref.gradient()
+    ref.close()
+    assert rel(lnl, want[0]) < 1e-6
+    assert np.abs(d1 - want[1]).max() <= 1e-5 * np.abs(want[1]).max()
+    rng = np.random.default_rng(5)
+    h = 2.5e-5
+
+    def fd4(f):
+        return (8 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12 * h)
+
+    for _ in range(3):
+        E = rng.normal(size=(4, 4))
+        fd = fd4(lambda s: _oracle_loglik(oracle, _with_model(prob, Q=prob.model.Q + s * E)))
+        assert abs(float(np.sum(G * E)) - fd) <= 2e-5 * max(abs(fd), np.abs(G).max())
+    for i in range(4):
+        e = np.eye(4)[i]
+        fd = fd4(lambda s: _oracle_loglik(oracle, _with_model(prob, freqs=prob.model.freqs + s * e)))
+        assert abs(gpi[i] - fd) <= 2e-5 * max(abs(fd), np.abs(gpi).max())
+
+
 @pytest.mark.parametrize("S,C,scaling,precision", [(20, 4, "none", "double"),
                                                    (20, 4, "auto", "single"),
                                                    (20, 3, "auto", "double"),
@@ -1363,6 +1428,12 @@ def test_adjoint_full_size_512_taxa(amd):
         assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
     assert np.abs(q32 - q64).max() <= 1e-6 * np.abs(q64).max()
     assert np.abs(pi32 - pi64).max() <= 1e-6 * np.abs(pi64).max()
+    ev = driver.Evaluator(amd, prob, precision="single", scaling="auto")
+    one = ev.fused_gradient(rate_matrix=True)
+    ev.close()
+    assert one[0] == f32[0]
+    assert np.abs(one[3] - q64).max() <= 1e-6 * np.abs(q64).max()
+    np.testing.assert_array_equal(one[4], pi32)
 
 
 @pytest.mark.parametrize("S,precision", [(4, "single"), (20, "single")])
@@ -1463,6 +1534,7 @@ def test_graph_replays_rate_matrix_gradient(amd, oracle, precision, scaling):
     rng = np.random.default_rng(21)
     lens = [prob.tree.lengths * rng.uniform(0.5, 1.5, prob.tree.lengths.shape) for _ in range(3)]
     ev = driver.Evaluator(amd, prob, precision=precision, scaling=scaling)
+    ev.fused_gradient(rate_matrix=True)   # the traversal plan tuned before the references
     want = [ev.fused_gradient(x, rate_matrix=True) for x in lens]
     ev.capture_fused_gradient(rate_matrix=True)
     for x, w in zip(lens, want):
