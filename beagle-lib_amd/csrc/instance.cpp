@@ -1204,8 +1204,8 @@ void Instance::launchPostPlan(const std::vector<Segment>& plan) {
         PostJob* h = all.data() + L.first;
         if (L.batched && levelChunk_ <= 1) {
             const int start[2] = {0, L.n};
-            if (dp_) launch_post_levels<double>(d + L.first, start, 1, dims_, dMatrices_, stream_);
-            else launch_post_levels<float>(d + L.first, start, 1, dims_, dMatrices_, stream_);
+            if (dp_) launch_post_levels<double>(d + L.first, h, start, 1, dims_, dMatrices_, stream_);
+            else launch_post_levels<float>(d + L.first, h, start, 1, dims_, dMatrices_, stream_);
         } else {
             // fused list, or a batched level walked in chunks of independent ops
             const int chunk = L.batched ? levelChunk_ : 0;

@@ -274,10 +274,11 @@ void launch_post(const PostJob* djobs, const PostJob* hjobs, int njobs, int chun
                  const void* mats, void* dummy, hipStream_t st);
 inline size_t dummy_bytes() { return 2 * 8 * 64 * 4 * sizeof(double) + 64 * (8 + 4) + 128 * 2; }
 // 4-state post-order by dependency level: jobs are ordered by level and
-// level l occupies [levelStart[l], levelStart[l+1]).
+// level l occupies [levelStart[l], levelStart[l+1]); hjobs is the host copy
+// of djobs (the launch shape of each level depends on its operands).
 template <typename R>
-void launch_post_levels(const PostJob* djobs, const int* levelStart, int nlevels, const Dims& d,
-                        const void* mats, hipStream_t st);
+void launch_post_levels(const PostJob* djobs, const PostJob* hjobs, const int* levelStart,
+                        int nlevels, const Dims& d, const void* mats, hipStream_t st);
 // partial != null: fused derivative sums, 3 x ntiles doubles per job
 // (ntiles = ceil(P/64)), reduced afterwards with launch_reduce_rows.
 template <typename R>

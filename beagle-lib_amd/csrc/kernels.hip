@@ -22,7 +22,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <cstdlib>
 #include <type_traits>
 
 namespace hipbeagle {
@@ -626,6 +625,73 @@ __global__ __launch_bounds__(512) void k_post4_level(const PostJob* __restrict__
                               J.accB, J.e1, J.e2, J.ed);
     }
     if (in) store4(gout<R>(J.dest) + off, v);
+}
+
+// The same with T pattern tiles per workgroup: grid = (ceil(tiles / T),
+// operations).  Each wave (one category) issues the loads of its T tiles
+// before any arithmetic, so the latency every workgroup pays before its
+// first useful byte (the job record, the matrices, the first operands) is
+// paid once per T tiles, and the T tiles' category maxima share one LDS
+// exchange and one barrier.  Tiles past the last are clamped for their
+// loads and store nothing.
+template <typename R, bool AUTO, int T>
+__global__ __launch_bounds__(512) void k_post4_level_mt(const PostJob* __restrict__ jobs, int P,
+                                                        int C, long cs,
+                                                        const R* __restrict__ mats) {
+    __shared__ R smax[T][8][TILE];
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int ntiles = (P + TILE - 1) / TILE;
+    const PostJob J = jobs[blockIdx.y];
+    Operand4<R> A[T], B[T];
+    int p[T];
+    bool in[T];
+#pragma unroll
+    for (int i = 0; i < T; ++i) {
+        const int t = (int)blockIdx.x * T + i;
+        p[i] = min(t, ntiles - 1) * TILE + lane;
+        in[i] = t < ntiles && p[i] < P && p[i] >= J.p0 && p[i] < J.p1;
+        const long off = c * cs + (long)p[i] * 4;
+        fetch4(J.x1, J.s1, off, p[i], A[i]);
+        fetch4(J.x2, J.s2, off, p[i], B[i]);
+    }
+    const R* m1 = mats + J.mo1 + c * 20;
+    const R* m2 = mats + J.mo2 + c * 20;
+    R v[T][4];
+#pragma unroll
+    for (int i = 0; i < T; ++i) {
+        R a[4], b[4];
+        apply4(m1, J.s1 != nullptr, A[i], a);
+        apply4(m2, J.s2 != nullptr, B[i], b);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[i][q] = a[q] * b[q];
+        if (J.mode == SCALE_READ) {
+            const R sr = (R)gin<double>(J.scaleRead)[p[i]];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[i][q] = v[i][q] / sr;
+        }
+    }
+    if (AUTO || J.mode == SCALE_WRITE) {
+#pragma unroll
+        for (int i = 0; i < T; ++i) {
+            const R m = fmax(fmax(v[i][0], v[i][1]), fmax(v[i][2], v[i][3]));
+            smax[i][c][lane] = in[i] ? m : (R)0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+#pragma unroll
+        for (int i = 0; i < T; ++i) {
+            R mx = smax[i][0][lane];
+            for (int cc = 1; cc < C; ++cc) mx = fmax(mx, smax[i][cc][lane]);
+            if (in[i])
+                rescale4<R, AUTO>(v[i], mx, c, p[i], J.scaleWrite, J.logScalers, J.cumulative,
+                                  J.accA, J.accB, J.e1, J.e2, J.ed);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+        if (in[i]) store4(gout<R>(J.dest) + c * cs + (long)p[i] * 4, v[i]);
 }
 
 // Pre-order traversal, pipelined like the post-order one (same fixed
@@ -1255,22 +1321,49 @@ __global__ __launch_bounds__(WG) void k_rescalex(R* __restrict__ dest, int p0, i
     scaleWrite[p] = w;
 }
 
+// Waves a level launch should leave per tile-count step (two rounds of
+// eight waves on each of the 1,024 SIMDs).
+constexpr long kPostLevelWaves = 16384;
+
 // Traversal launchers: the fused 4-state kernels, or the generic-S path
 // whose grids come from the host copy of each job.
 template <typename R>
-void launch_post_levels(const PostJob* djobs, const int* levelStart, int nlevels, const Dims& d,
-                        const void* mats, hipStream_t st) {
+void launch_post_levels(const PostJob* djobs, const PostJob* hjobs, const int* levelStart,
+                        int nlevels, const Dims& d, const void* mats, hipStream_t st) {
+    const int tiles = (d.P + TILE - 1) / TILE;
+    const R* m = reinterpret_cast<const R*>(mats);
     for (int l = 0; l < nlevels; ++l) {
         const int n = levelStart[l + 1] - levelStart[l];
-        dim3 grid((d.P + TILE - 1) / TILE, n);
-        if (d.autoScale)
-            hipLaunchKernelGGL((k_post4_level<R, true>), grid, dim3(TILE * d.C), 0, st,
-                               djobs + levelStart[l], d.P, d.C, d.catStride,
-                               reinterpret_cast<const R*>(mats));
-        else
-            hipLaunchKernelGGL((k_post4_level<R, false>), grid, dim3(TILE * d.C), 0, st,
-                               djobs + levelStart[l], d.P, d.C, d.catStride,
-                               reinterpret_cast<const R*>(mats));
+        const PostJob* lj = djobs + levelStart[l];
+        // FP32 tiles per workgroup: 4 where most operations join two tips
+        // (each tile moves few bytes), else 2, as long as about two rounds
+        // of waves stay on the chip (kPostLevelWaves); 1 for narrow levels
+        // and FP64 (two tiles' FP64 operands take 115 registers).
+        int tipTip = 0;
+        for (int o = levelStart[l]; o < levelStart[l + 1]; ++o)
+            tipTip += hjobs[o].s1 != nullptr && hjobs[o].s2 != nullptr;
+        const long waves = (long)n * tiles * d.C;
+        int T = sizeof(R) == 4 && waves >= 2 * kPostLevelWaves ? 2 : 1;
+        if (T == 2 && 2 * tipTip >= n && waves >= 4 * kPostLevelWaves) T = 4;
+        const dim3 block(TILE * d.C);
+        if constexpr (sizeof(R) == 4) {
+            if (T > 1) {
+                const dim3 grid((tiles + T - 1) / T, n);
+                if (T == 4) {
+                    if (d.autoScale) hipLaunchKernelGGL((k_post4_level_mt<R, true, 4>), grid, block, 0, st, lj, d.P, d.C, d.catStride, m);
+                    else hipLaunchKernelGGL((k_post4_level_mt<R, false, 4>), grid, block, 0, st, lj, d.P, d.C, d.catStride, m);
+                } else {
+                    if (d.autoScale) hipLaunchKernelGGL((k_post4_level_mt<R, true, 2>), grid, block, 0, st, lj, d.P, d.C, d.catStride, m);
+                    else hipLaunchKernelGGL((k_post4_level_mt<R, false, 2>), grid, block, 0, st, lj, d.P, d.C, d.catStride, m);
+                }
+                continue;
+            }
+        }
+        {
+            const dim3 grid(tiles, n);
+            if (d.autoScale) hipLaunchKernelGGL((k_post4_level<R, true>), grid, block, 0, st, lj, d.P, d.C, d.catStride, m);
+            else hipLaunchKernelGGL((k_post4_level<R, false>), grid, block, 0, st, lj, d.P, d.C, d.catStride, m);
+        }
     }
 }
 
@@ -2176,7 +2269,7 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
     template void launch_post<R>(const PostJob*, const PostJob*, int, int, const Dims&,         \
                                  const void*,                                                    \
                                  void*, hipStream_t);                                            \
-    template void launch_post_levels<R>(const PostJob*, const int*, int, const Dims&,            \
+    template void launch_post_levels<R>(const PostJob*, const PostJob*, const int*, int, const Dims&,            \
                                         const void*, hipStream_t);                               \
     template void launch_pre<R>(const PreJob*, const PreJob*, int, const Dims&, int, void*,      \
                                 const void*, void*, const double*, const double*, double*,       \

@@ -409,13 +409,37 @@ struct QgVec {
     float wh[4];   // V^-1[s][k]
 };
 
+// A tip child's beta needs no products: beta_i = column s_i of V^-1, so
+// lane 4b + j takes V^-1[j][s_i] with s_i (code: the state, 4 for a
+// missing state or a lane past the patterns) read from lane i of its quad.
+__device__ __forceinline__ float tip_beta1(int ci, const QgVec& v) {
+    const float lo = (ci & 1) ? v.wh[1] : v.wh[0], hi = (ci & 1) ? v.wh[3] : v.wh[2];
+    return ci > 3 ? 0.0f : (ci & 2) ? hi : lo;
+}
+
+__device__ __forceinline__ fvec4 tip_beta(int code, const QgVec& v) {
+    fvec4 be;   // quad_perm [i,i,i,i]: lane i of the quad
+    be[0] = tip_beta1(__builtin_amdgcn_mov_dpp(code, 0x00, 0xf, 0xf, false), v);
+    be[1] = tip_beta1(__builtin_amdgcn_mov_dpp(code, 0x55, 0xf, 0xf, false), v);
+    be[2] = tip_beta1(__builtin_amdgcn_mov_dpp(code, 0xAA, 0xf, 0xf, false), v);
+    be[3] = tip_beta1(__builtin_amdgcn_mov_dpp(code, 0xFF, 0xf, 0xf, false), v);
+    return be;
+}
+
 __device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&x)[4],
-                                            const QgVec& v, const double* f, double (&Hr)[4]) {
+                                            const QgVec& v, const double* f, double (&Hr)[4],
+                                            bool tip = false, int code = 0) {
     fvec4 al = {0.0f, 0.0f, 0.0f, 0.0f}, be = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (tip) {
+        be = tip_beta(code, v);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        al = __builtin_amdgcn_mfma_f32_4x4x1f32(ta[k], v.vh[k], al, 0, 0, 0);
-        be = __builtin_amdgcn_mfma_f32_4x4x1f32(x[k], v.wh[k], be, 0, 0, 0);
+        for (int k = 0; k < 4; ++k) al = __builtin_amdgcn_mfma_f32_4x4x1f32(ta[k], v.vh[k], al, 0, 0, 0);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            al = __builtin_amdgcn_mfma_f32_4x4x1f32(ta[k], v.vh[k], al, 0, 0, 0);
+            be = __builtin_amdgcn_mfma_f32_4x4x1f32(x[k], v.wh[k], be, 0, 0, 0);
+        }
     }
     fvec4 m = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -428,7 +452,7 @@ __device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&
 // where the products need them rather than held in registers.
 __device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&x)[4],
                                             const float* sv, int s, const double* f,
-                                            double (&Hr)[4]) {
+                                            double (&Hr)[4], bool tip, int code) {
     QgVec v;
     const fvec4 a = *(const fvec4*)(sv + 4 * s), b = *(const fvec4*)(sv + 16 + 4 * s);
 #pragma unroll
@@ -436,13 +460,19 @@ __device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&
         v.vh[k] = a[k];
         v.wh[k] = b[k];
     }
-    qgrad_block(ta, x, v, f, Hr);
+    qgrad_block(ta, x, v, f, Hr, tip, code);
 }
 
-// V^T and V^-1 rows into LDS (threads 0-31 of the workgroup)
-__device__ __forceinline__ void qgrad_vectors_lds(const double* __restrict__ eig, float* sv) {
-    const int t = threadIdx.x;
-    if (t < 32) sv[t] = (float)(t < 16 ? eig[(t & 3) * 4 + (t >> 2)] : eig[t]);
+// V^T and V^-1 rows (FP32) for LDS: thread t < 32 loads entry t
+// (qgrad_vector_load) and stores it (qgrad_vector_store) once the job's
+// operand loads are issued, so the wait for it does not hold them back.
+__device__ __forceinline__ double qgrad_vector_load(const double* __restrict__ eig) {
+    const int t = min((int)threadIdx.x, 31);
+    return eig[t < 16 ? (t & 3) * 4 + (t >> 2) : t];
+}
+
+__device__ __forceinline__ void qgrad_vector_store(double e, float* sv) {
+    if (threadIdx.x < 32) sv[threadIdx.x] = (float)e;
 }
 
 __device__ __forceinline__ void qgrad_vectors(const double* __restrict__ eig, int s, QgVec& v) {
@@ -489,10 +519,6 @@ __device__ __forceinline__ void branch_sums_f32(bool tip, int s, int r, const fl
     n2 = dot4(pre, t);
 }
 
-// Operand loads of an FP32 node job without the dummy tile: uniform
-// branches skip what the job does not read (a forwarded parent, the
-// partials of a tip child, the states of a partials child).  Used where no
-// later job's loads are in flight, so the branches cost no extra waits.
 // A register with no particular value and no instruction to set it, for
 // operand slots a job does not load (its uses are selected or branched
 // away by the same job fields that skip the load).
@@ -503,27 +529,21 @@ __device__ __forceinline__ T unset() {
     return x;
 }
 
+// Operand loads of an FP32 node job without the dummy tile.
 template <bool AUTO, bool QG = false>
 __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float* __restrict__ jm,
                                                   int c, unsigned off, int p, int lane,
                                                   NodeOps4<float>& o,
                                                   const double* __restrict__ F = nullptr) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        o.u[i] = unset<float>();
-        o.xa[i] = unset<float>();
-        o.xb[i] = unset<float>();
-    }
-    o.sa = unset<int>();
-    o.sb = unset<int>();
-    o.eu = unset<int>();
-    o.ea = unset<int>();
-    o.eb = unset<int>();
-    if (J.kPar == 0) load4(at<float>(J.parent, off), o.u);
-    if (J.k[0] == 0) load4(at<float>(J.x[0], off), o.xa);
-    else o.sa = *at<int>(J.x[0], 4u * p);
-    if (J.k[1] == 0) load4(at<float>(J.x[1], off), o.xb);
-    else o.sb = *at<int>(J.x[1], 4u * p);
+    // every operand slot is loaded, from an empty range (reads 0, no
+    // traffic) where the job has no such operand: no branches, so no value
+    // merges that would make the compiler wait for earlier loads before
+    // issuing the later ones
+    load4_range<float>(range_if(J.parent, J.kPar == 0, jm), off, o.u);
+    load4_range<float>(range_if(J.x[0], J.k[0] == 0, jm), off, o.xa);
+    load4_range<float>(range_if(J.x[1], J.k[1] == 0, jm), off, o.xb);
+    o.sa = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[0], J.k[0] != 0, jm), 4u * p, 0, 0);
+    o.sb = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[1], J.k[1] != 0, jm), 4u * p, 0, 0);
     {
         const int l = min(lane, NODE_MATS / 4 - 1);
         o.mv = *at<fvec4>(jm, (unsigned)((J.mbase + c * NODE_MATS + l * 4) * (int)sizeof(float)));
@@ -536,9 +556,12 @@ __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float*
         o.fv = *at<dvec2>(F, (unsigned)(((fl >> 3 ? J.mF[1] : J.mF[0]) + c * 16 + 2 * (fl & 7)) * 8));
     }
     if (AUTO) {
-        if (J.kPar == 0 && J.eParent != nullptr) o.eu = *at<short>(J.eParent, 2u * p);
-        if (J.ex[0] != nullptr) o.ea = *at<short>(J.ex[0], 2u * p);
-        if (J.ex[1] != nullptr) o.eb = *at<short>(J.ex[1], 2u * p);
+        o.eu = (short)__builtin_amdgcn_raw_buffer_load_b16(
+            range_if(J.eParent, J.kPar == 0 && J.eParent != nullptr, jm), 2u * p, 0, 0);
+        o.ea = (short)__builtin_amdgcn_raw_buffer_load_b16(range_if(J.ex[0], J.ex[0] != nullptr, jm),
+                                                           2u * p, 0, 0);
+        o.eb = (short)__builtin_amdgcn_raw_buffer_load_b16(range_if(J.ex[1], J.ex[1] != nullptr, jm),
+                                                           2u * p, 0, 0);
     }
 }
 
@@ -602,35 +625,6 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    if (QG) {
-        // rate-matrix gradient of both branches: top_a = pre(v) * P_b x_b
-        // (ua), top_b = pre(v) * P_a x_a (ub); L_p = sum_c w_c Lc, the same
-        // order of additions in every wave; scale factors cancel in
-        // top x^T / L_p
-        float L = 0.0f, wcc = wc[0];
-#pragma unroll
-        for (int cc = 0; cc < C; ++cc) {
-            L += wc[cc] * buf[(8 * C + cc) * TILE + lane];
-            if (c == cc) wcc = wc[cc];
-        }
-        const float coef = in ? (w * wcc) / L : 0.0f;
-#pragma unroll
-        for (int ch = 0; ch < 2; ++ch) {
-            const bool tip = ch == 0 ? tipA : tipB;
-            const int st = ch == 0 ? sa : sb;
-            float ta[4], x[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                // a tip's vector is one-hot; a missing state (4) reads the
-                // pad column, which does not depend on Q: zero; lanes past
-                // the patterns add nothing
-                ta[i] = in ? coef * (ch == 0 ? ua[i] : ub[i]) : 0.0f;
-                const float xv = ch == 0 ? o.xa[i] : o.xb[i];
-                x[i] = !in ? 0.0f : tip ? (st == i ? 1.0f : 0.0f) : xv;
-            }
-            qgrad_block(ta, x, sv, r, lf + ch * 16 + r * 4, Hr);
-        }
-    }
     const bool keepA = in && J.dest[0] != nullptr;
     const bool keepB = in && J.dest[1] != nullptr;
     int outA = 0, outB = 0;
@@ -686,7 +680,37 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
         pb[i] = vb[i];
     }
     pea = outA;
-    peb = outB;
+    peb = outB;    if (QG) {
+        // rate-matrix gradient of both branches (after the stores, which
+        // then drain meanwhile): top_a = pre(v) * P_b x_b
+        // (ua), top_b = pre(v) * P_a x_a (ub); L_p = sum_c w_c Lc, the same
+        // order of additions in every wave; scale factors cancel in
+        // top x^T / L_p
+        float L = 0.0f, wcc = wc[0];
+#pragma unroll
+        for (int cc = 0; cc < C; ++cc) {
+            L += wc[cc] * buf[(8 * C + cc) * TILE + lane];
+            if (c == cc) wcc = wc[cc];
+        }
+        const float coef = in ? (w * wcc) / L : 0.0f;
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+            const bool tip = ch == 0 ? tipA : tipB;
+            const int st = ch == 0 ? sa : sb;
+            float ta[4], x[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                // a tip's vector is one-hot; a missing state (4) reads the
+                // pad column, which does not depend on Q: zero; lanes past
+                // the patterns add nothing
+                ta[i] = in ? coef * (ch == 0 ? ua[i] : ub[i]) : 0.0f;
+                const float xv = ch == 0 ? o.xa[i] : o.xb[i];
+                x[i] = !in ? 0.0f : tip ? (st == i ? 1.0f : 0.0f) : xv;
+            }
+            const int code = in && st < 4 ? st : 4;
+            qgrad_block(ta, x, sv, r, lf + ch * 16 + r * 4, Hr, tip, code);
+        }
+    }
 }
 
 #define NODE_F32_SETUP(NQ, NPAR)                                                              \
@@ -707,6 +731,25 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     int k = 0, pea = 0, peb = 0;                                                              \
     float pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
 
+// v summed over the 16 lanes with the same lane % 4: two row rotations
+// (DPP), then the row pairs and the wave halves swapped (permlane16/32
+// swaps); lanes 0-3 hold the sums for s = 0-3.
+__device__ __forceinline__ double swap_sum(double v, bool half32) {
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    const auto a = half32 ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false)
+                          : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = half32 ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false)
+                          : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return __hiloint2double((int)b[0], (int)a[0]) + __hiloint2double((int)b[1], (int)a[1]);
+}
+
+__device__ __forceinline__ double sum_by_s(double v) {
+    v += dpp_move<0x124, 0xf>(v);   // row_ror:4
+    v += dpp_move<0x128, 0xf>(v);   // row_ror:8: the row's four lanes of this s
+    v = swap_sum(v, false);         // rows 0 + 1, 2 + 3
+    return swap_sum(v, true);       // halves
+}
+
 // With the rate-matrix gradient (QG): the workgroup's H (the lanes' shares
 // summed over its jobs, then over the lanes with the same lane % 4 and over
 // the category waves in order) is written to its own slot of hacc
@@ -717,10 +760,7 @@ template <int C>
 __device__ __forceinline__ void qgrad_store(double (&Hr)[4], int c, int lane, double* red,
                                             double* __restrict__ hacc, long slot) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-#pragma unroll
-        for (int m = 4; m < 64; m <<= 1) Hr[i] += __shfl_xor(Hr[i], m, 64);
-    }
+    for (int i = 0; i < 4; ++i) Hr[i] = sum_by_s(Hr[i]);
     if (lane < 4) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) red[c * 16 + lane * 4 + i] = Hr[i];
@@ -758,13 +798,14 @@ __global__ __launch_bounds__(64 * C, QG ? 4 : 7) void k_pren4f(
     double* red = reinterpret_cast<double*>(smat + C * 2 * NODE_F32_QBLOCK);
     float* sv = reinterpret_cast<float*>(red + 16 * C);
     double Hr[4] = {0.0, 0.0, 0.0, 0.0};
-    if (QG) qgrad_vectors_lds(eig, sv);   // read after the first job's barrier
+    const double ev = QG ? qgrad_vector_load(eig) : 0.0;
     jobs += blockIdx.y * chunk;
     njobs = min(chunk, njobs - (int)blockIdx.y * chunk);
     for (int o = 0; o < njobs; ++o) {
         const NodeJob J = jobs[o];
         NodeOps4<float> S0;
         node_issue_direct<AUTO, QG>(J, jm, c, off, p, lane, S0, F);
+        if (QG && o == 0) qgrad_vector_store(ev, sv);   // read after the job's barrier
         node_compute_f32<C, AUTO, QG>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
                                       wc, w, partial, tile, ntiles, sv, Hr);
     }
@@ -782,9 +823,10 @@ __global__ __launch_bounds__(64 * C, QG ? 7 : 8) void k_pren4f_level(
     double* red = reinterpret_cast<double*>(smat + C * NODE_F32_QBLOCK);
     float* sv = reinterpret_cast<float*>(red + 16 * C);
     double Hr[4] = {0.0, 0.0, 0.0, 0.0};
-    if (QG) qgrad_vectors_lds(eig, sv);   // read after the job's barrier
+    const double ev = QG ? qgrad_vector_load(eig) : 0.0;
     const NodeJob J = jobs[blockIdx.y];
     node_issue_direct<AUTO, QG>(J, jm, c, off, p, lane, S0, F);
+    if (QG) qgrad_vector_store(ev, sv);   // read after the job's barrier
     node_compute_f32<C, AUTO, QG, 1>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
                                      wc, w, partial, tile, ntiles, sv, Hr);
     if (QG) qgrad_store<C>(Hr, c, lane, red, hacc, base + (long)blockIdx.y * gridDim.x + blockIdx.x);
@@ -1209,7 +1251,7 @@ __device__ __forceinline__ void qm_compute(const NodeJob& J, const QWOps<float>&
             const float xv = ch == 0 ? o.xa[i] : o.xb[i];
             x[i] = !in ? 0.0f : tip ? (st == i ? 1.0f : 0.0f) : xv;
         }
-        qgrad_block(ta, x, v, sf + ch * 64 + c * 16 + r * 4, Hr);
+        qgrad_block(ta, x, v, sf + ch * 64 + c * 16 + r * 4, Hr, tip, in && st < 4 ? st : 4);
     }
 }
 
