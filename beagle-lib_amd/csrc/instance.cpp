@@ -347,11 +347,10 @@ void Instance::toHost(double* dst, const void* src, size_t n) {
 // before its previous copy finished.
 void* Instance::upload(const void* data, size_t bytes) {
     if (capturing_) {
-        // the graph's own staging: replays copy from it again
+        // the graph's own copy, in its device arena: the whole arena is
+        // copied once when the capture ends, so replays copy nothing
         const size_t off = captureAlloc(bytes);
         std::memcpy(cap_.host + off, data, bytes);
-        HIPCHECK(hipMemcpyAsync(cap_.dev + off, cap_.host + off, bytes, hipMemcpyHostToDevice,
-                                stream_));
         lastUploadOff_ = off;
         return cap_.dev + off;
     }
@@ -674,7 +673,15 @@ int Instance::matrixUpdate(const std::vector<MatrixJob>& jobs) {
     if (jobs.empty()) return BEAGLE_SUCCESS;
     const MatrixJob* d = reinterpret_cast<const MatrixJob*>(
         upload(jobs.data(), jobs.size() * sizeof(MatrixJob)));
-    if (capturing_) cap_.matJobs.push_back({lastUploadOff_, (int)jobs.size()});
+    if (capturing_) {
+        // replays read each job's length from the graph's host memory,
+        // where setGraphBranchLengths writes it
+        cap_.matJobs.push_back({lastUploadOff_, (int)jobs.size()});
+        MatrixJob* h = reinterpret_cast<MatrixJob*>(cap_.host + lastUploadOff_);
+        for (size_t j = 0; j < jobs.size(); ++j)
+            h[j].tHost = reinterpret_cast<const double*>(
+                cap_.hostDev + ((char*)&h[j].t - cap_.host));
+    }
     if (dp_) launch_matrices<double>(d, (int)jobs.size(), dims_.S, dims_.C, stream_);
     else launch_matrices<float>(d, (int)jobs.size(), dims_.S, dims_.C, stream_);
     HIPCHECK(hipGetLastError());
@@ -709,6 +716,7 @@ int Instance::updateTransitionMatrices(int e, const int* probIdx, const int* d1I
         J.t = t[u];
         J.cat = -1;
         J.F = dF_ ? dF_ + (size_t)probIdx[u] * dims_.C * 16 : nullptr;
+        J.tHost = nullptr;
     }
     for (int u = 0; u < count; ++u) {
         matOrigin_[probIdx[u]] = MatrixOrigin{e, 0, t[u]};
@@ -745,6 +753,7 @@ int Instance::updateTransitionMatricesWithModelCategories(const int* eig, const 
             J.t = t[u];
             J.cat = c;
             J.F = nullptr;
+            J.tHost = nullptr;
             jobs.push_back(J);
         }
         matOrigin_[probIdx[u]] = MatrixOrigin{};   // several eigen-systems
@@ -777,6 +786,7 @@ int Instance::updateTransitionMatricesWithMultipleModels(const int* eig, const i
         J.t = t[u];
         J.cat = -1;
         J.F = dF_ ? dF_ + (size_t)probIdx[u] * dims_.C * 16 : nullptr;
+        J.tHost = nullptr;
     }
     for (int u = 0; u < count; ++u) {
         matOrigin_[probIdx[u]] = MatrixOrigin{eig[u], rateIdx[u], t[u]};
@@ -2654,10 +2664,12 @@ size_t Instance::captureAlloc(size_t bytes) {
     return off;
 }
 
+// An output of the captured calls, written by a copy kernel into the
+// graph's host memory at its place in the stream (no copy node, whose
+// engine would add its own latency to every replay).
 size_t Instance::captureResult(const double* dsrc, size_t n) {
     const size_t off = captureAlloc(n * sizeof(double));
-    HIPCHECK(hipMemcpyAsync(cap_.host + off, dsrc, n * sizeof(double), hipMemcpyDeviceToHost,
-                            stream_));
+    launch_copy_to_host(reinterpret_cast<double*>(cap_.hostDev + off), dsrc, (long)n, stream_);
     return off;
 }
 
@@ -2670,7 +2682,9 @@ int Instance::beginCapture() {
     if (capturing_) return BEAGLE_ERROR_GENERAL;
     sync();
     cap_ = Graph{};
-    HIPCHECK(hipHostMalloc((void**)&cap_.host, kCaptureArena, hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc((void**)&cap_.host, kCaptureArena,
+                           hipHostMallocCoherent | hipHostMallocMapped));
+    HIPCHECK(hipHostGetDevicePointer((void**)&cap_.hostDev, cap_.host, 0));
     HIPCHECK(hipMalloc((void**)&cap_.dev, kCaptureArena));
     captureError_.clear();
     HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
@@ -2693,7 +2707,12 @@ int Instance::endCapture(int* outGraph) {
     }
     const hipError_t ie = hipGraphInstantiate(&cap_.exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (ie != hipSuccess) {
+    // the job lists the captured calls uploaded, to the device once
+    const hipError_t ce = ie == hipSuccess ? hipMemcpy(cap_.dev, cap_.host, cap_.used,
+                                                       hipMemcpyHostToDevice)
+                                           : hipSuccess;
+    if (ie != hipSuccess || ce != hipSuccess) {
+        if (ie == hipSuccess) (void)hipGraphExecDestroy(cap_.exec);
         (void)hipGetLastError();
         (void)hipHostFree(cap_.host);
         (void)hipFree(cap_.dev);

@@ -199,8 +199,9 @@ private:
     };
     struct Graph {
         hipGraphExec_t exec = nullptr;
-        char* host = nullptr;
-        char* dev = nullptr;
+        char* host = nullptr;      // pinned, coherent: the replays' outputs and lengths
+        char* hostDev = nullptr;   // the same memory as the device addresses it
+        char* dev = nullptr;       // the replays' job lists (copied once, at the end)
         size_t used = 0;
         std::vector<CapturedOut> outs;
         std::vector<std::pair<size_t, int>> matJobs;   // (host offset, count) of matrix jobs

@@ -96,6 +96,8 @@ struct MatrixJob {
     int cplx;                         // complex eigen-system (conjugate pairs as 2 x 2 blocks)
     double* F;                        // 4 real states: the branch's Frechet factors
                                       //   [C][4][4] for the rate-matrix gradient, or null
+    const double* tHost;              // non-null in a captured graph: t read from this
+                                      //   (host) address at every replay
 };
 
 // One root of a several-root likelihood (calcRootLogLikelihoodsMulti).
@@ -261,6 +263,8 @@ void launch_post_bf16(const PostJob* djobs, int njobs, const Dims& d, const floa
                       int terms, hipStream_t st);
 
 // ---- launchers (kernels.hip) ----
+// n doubles from device memory to (device-visible) pinned host memory.
+void launch_copy_to_host(double* dstHost, const double* src, long n, hipStream_t st);
 template <typename R>
 void launch_matrices(const MatrixJob* jobs, int njobs, int S, int C, hipStream_t st);
 template <typename R>

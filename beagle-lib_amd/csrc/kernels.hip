@@ -118,7 +118,8 @@ __global__ __launch_bounds__(WG) void k_matrices(const MatrixJob* __restrict__ j
     double* ex = sh;
     double* e1 = sh + S;
     double* e2 = sh + 2 * S;
-    const MatrixJob J = jobs[blockIdx.x];
+    MatrixJob J = jobs[blockIdx.x];
+    if (J.tHost) J.t = *J.tHost;   // a graph replay: the length set since capture
     if (J.cat >= 0 && blockIdx.y != 0) return;   // one category per job
     const int c = J.cat >= 0 ? J.cat : (int)blockIdx.y;
     const double r = J.cat >= 0 ? 1.0 : J.rates[c];
@@ -211,6 +212,18 @@ __global__ __launch_bounds__(WG) void k_matrices(const MatrixJob* __restrict__ j
         if (D1) D1[i * W + S] = (R)0.0;
         if (D2) D2[i * W + S] = (R)0.0;
     }
+}
+
+__global__ __launch_bounds__(WG) void k_copy_to_host(double* __restrict__ dst,
+                                                     const double* __restrict__ src, long n) {
+    for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
+        dst[i] = src[i];
+}
+
+void launch_copy_to_host(double* dstHost, const double* src, long n, hipStream_t st) {
+    if (n <= 0) return;
+    const int nb = (int)std::min<long>((n + WG - 1) / WG, 64);
+    hipLaunchKernelGGL(k_copy_to_host, dim3(nb), dim3(WG), 0, st, dstHost, src, n);
 }
 
 template <typename R>
