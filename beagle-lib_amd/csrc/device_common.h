@@ -115,6 +115,20 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t range_if(const void* base, boo
     return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, n, 0x00020000);
 }
 
+// Buffer resource over `bytes` bytes at `base` when `on`, else an empty
+// range over `valid`: stores to an empty range (or past `bytes`) are
+// dropped, so a store a step does not need costs one instruction and no
+// branch.  Base and size are wave-uniform.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t range_n(const void* base, bool on,
+                                                          unsigned bytes, const void* valid) {
+    const unsigned long long a = reinterpret_cast<unsigned long long>(on ? base : valid);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    const int n = __builtin_amdgcn_readfirstlane(on ? (int)bytes : 0);
+    void* b = reinterpret_cast<void*>(((unsigned long long)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, n, 0x00020000);
+}
+
 // four reals at byte offset `at` of a buffer resource
 template <typename R>
 __device__ __forceinline__ void load4_range(__amdgpu_buffer_rsrc_t r, unsigned at, R (&v)[4]) {

@@ -35,8 +35,8 @@ namespace hipbeagle {
 constexpr int NODE_MATS = 6 * 20;
 constexpr int NODE_F32_BLOCK = 128;       // floats of a wave's block in the FP32 node kernels
 // ... and with the rate-matrix gradient: the matrices, then the factors of
-// the job's two branches for the wave's category ([2][16] doubles)
-constexpr int NODE_F32_QBLOCK = NODE_F32_BLOCK + 64;
+// the job's two branches for the wave's category ([2][16] floats)
+constexpr int NODE_F32_QBLOCK = NODE_F32_BLOCK + 32;
 constexpr int NODE_MAT_BLOCK = 64 * 16;   // bytes of one wave's block
 
 // 32-bit addressing: a wave-uniform base pointer plus a per-lane byte
@@ -426,8 +426,11 @@ __device__ __forceinline__ fvec4 tip_beta(int code, const QgVec& v) {
     return be;
 }
 
+// f: the lane's four factors F[s][.] (FP64, or FP32 where a job's sums are
+// formed in FP32 and added in FP64 afterwards); H accumulates in T.
+template <typename FT, typename T>
 __device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&x)[4],
-                                            const QgVec& v, const double* f, double (&Hr)[4],
+                                            const QgVec& v, const FT* f, T (&Hr)[4],
                                             bool tip = false, int code = 0) {
     fvec4 al = {0.0f, 0.0f, 0.0f, 0.0f}, be = {0.0f, 0.0f, 0.0f, 0.0f};
     if (tip) {
@@ -445,14 +448,15 @@ __device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&
 #pragma unroll
     for (int i = 0; i < 4; ++i) m = __builtin_amdgcn_mfma_f32_4x4x1f32(be[i], al[i], m, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) Hr[i] += f[i] * (double)m[i];
+    for (int i = 0; i < 4; ++i) Hr[i] += (T)f[i] * (T)m[i];
 }
 
 // The same with V^T and V^-1 (FP32, [2][4][4]: row s of each) in LDS, read
 // where the products need them rather than held in registers.
+template <typename FT, typename T>
 __device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&x)[4],
-                                            const float* sv, int s, const double* f,
-                                            double (&Hr)[4], bool tip, int code) {
+                                            const float* sv, int s, const FT* f, T (&Hr)[4],
+                                            bool tip, int code) {
     QgVec v;
     const fvec4 a = *(const fvec4*)(sv + 4 * s), b = *(const fvec4*)(sv + 16 + 4 * s);
 #pragma unroll
@@ -534,34 +538,45 @@ template <bool AUTO, bool QG = false>
 __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float* __restrict__ jm,
                                                   int c, unsigned off, int p, int lane,
                                                   NodeOps4<float>& o,
-                                                  const double* __restrict__ F = nullptr) {
+                                                  const double* __restrict__ F = nullptr,
+                                                  bool valid = true) {
     // every operand slot is loaded, from an empty range (reads 0, no
-    // traffic) where the job has no such operand: no branches, so no value
-    // merges that would make the compiler wait for earlier loads before
-    // issuing the later ones
-    load4_range<float>(range_if(J.parent, J.kPar == 0, jm), off, o.u);
-    load4_range<float>(range_if(J.x[0], J.k[0] == 0, jm), off, o.xa);
-    load4_range<float>(range_if(J.x[1], J.k[1] == 0, jm), off, o.xb);
-    o.sa = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[0], J.k[0] != 0, jm), 4u * p, 0, 0);
-    o.sb = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[1], J.k[1] != 0, jm), 4u * p, 0, 0);
+    // traffic) where the job has no such operand (or, !valid, for no job):
+    // no branches, so no value merges that would make the compiler wait
+    // for earlier loads before issuing the later ones
+    load4_range<float>(range_if(J.parent, valid && J.kPar == 0, jm), off, o.u);
+    load4_range<float>(range_if(J.x[0], valid && J.k[0] == 0, jm), off, o.xa);
+    load4_range<float>(range_if(J.x[1], valid && J.k[1] == 0, jm), off, o.xb);
+    o.sa = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[0], valid && J.k[0] != 0, jm),
+                                                     4u * p, 0, 0);
+    o.sb = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[1], valid && J.k[1] != 0, jm),
+                                                     4u * p, 0, 0);
     {
         const int l = min(lane, NODE_MATS / 4 - 1);
-        o.mv = *at<fvec4>(jm, (unsigned)((J.mbase + c * NODE_MATS + l * 4) * (int)sizeof(float)));
+        const auto q = __builtin_amdgcn_raw_buffer_load_b128(
+            range_if(jm, valid, jm),
+            (unsigned)((J.mbase + c * NODE_MATS + l * 4) * (int)sizeof(float)), 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o.mv[i] = __uint_as_float(q[i]);
     }
     if (QG) {
         // lanes 0-7: child a's 16 factors of category c, lanes 8-15: child
         // b's (lanes past 15 repeat lane 15's pair; a select of the two
         // fields, not a lane-dependent index into the job)
         const int fl = min(lane, 15);
-        o.fv = *at<dvec2>(F, (unsigned)(((fl >> 3 ? J.mF[1] : J.mF[0]) + c * 16 + 2 * (fl & 7)) * 8));
+        const auto q = __builtin_amdgcn_raw_buffer_load_b128(
+            range_if(F, valid, jm),
+            (unsigned)(((fl >> 3 ? J.mF[1] : J.mF[0]) + c * 16 + 2 * (fl & 7)) * 8), 0, 0);
+        o.fv[0] = __hiloint2double((int)q[1], (int)q[0]);
+        o.fv[1] = __hiloint2double((int)q[3], (int)q[2]);
     }
     if (AUTO) {
         o.eu = (short)__builtin_amdgcn_raw_buffer_load_b16(
-            range_if(J.eParent, J.kPar == 0 && J.eParent != nullptr, jm), 2u * p, 0, 0);
-        o.ea = (short)__builtin_amdgcn_raw_buffer_load_b16(range_if(J.ex[0], J.ex[0] != nullptr, jm),
-                                                           2u * p, 0, 0);
-        o.eb = (short)__builtin_amdgcn_raw_buffer_load_b16(range_if(J.ex[1], J.ex[1] != nullptr, jm),
-                                                           2u * p, 0, 0);
+            range_if(J.eParent, valid && J.kPar == 0 && J.eParent != nullptr, jm), 2u * p, 0, 0);
+        o.ea = (short)__builtin_amdgcn_raw_buffer_load_b16(
+            range_if(J.ex[0], valid && J.ex[0] != nullptr, jm), 2u * p, 0, 0);
+        o.eb = (short)__builtin_amdgcn_raw_buffer_load_b16(
+            range_if(J.ex[1], valid && J.ex[1] != nullptr, jm), 2u * p, 0, 0);
     }
 }
 
@@ -572,7 +587,7 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
                                                  int lane, int p, int P, unsigned off,
                                                  const float (&wc)[C], float w, double* partial,
                                                  int tile, int ntiles, const float* sv,
-                                                 double (&Hr)[4]) {
+                                                 float (&Hj)[4]) {
     constexpr int NQ = QG ? 9 : 8;   // quantities exchanged per pattern and category
     constexpr int BLK = QG ? NODE_F32_QBLOCK : NODE_F32_BLOCK;
     const bool in = p < P;
@@ -588,9 +603,13 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     // NPAR 1: one job per workgroup
     const int par = NPAR == 2 ? (k & 1) : 0;
     float* lm = smat + (c * NPAR + par) * BLK;
-    double* lf = reinterpret_cast<double*>(lm + NODE_F32_BLOCK);   // QG: [2][16] factors
+    float* lf = lm + NODE_F32_BLOCK;   // QG: [2][16] factors (FP32)
     *(fvec4*)(lm + min(lane, NODE_MATS / 4 - 1) * 4) = o.mv;
-    if (QG) *(dvec2*)(lf + 2 * min(lane, 15)) = o.fv;
+    if (QG) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const f2 fq = {(float)o.fv[0], (float)o.fv[1]};
+        *(f2*)(lf + 2 * min(lane, 15)) = fq;
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
@@ -625,8 +644,6 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    const bool keepA = in && J.dest[0] != nullptr;
-    const bool keepB = in && J.dest[1] != nullptr;
     int outA = 0, outB = 0;
     if (AUTO) {
         float ma = buf[lane], mb = buf[C * TILE + lane];
@@ -644,48 +661,83 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
         }
         outA = ea + eu + (J.ex[1] ? o.eb : 0);
         outB = eb + eu + (J.ex[0] ? o.ea : 0);
-        if (keepA) *at_w<short>(J.ed[0], 2u * p) = (short)outA;
-        if (keepB) *at_w<short>(J.ed[1], 2u * p) = (short)outB;
+        // wave 0 stores the exponents; patterns past P fall outside the range
+        const unsigned pe = (unsigned)P * 2u;
+        __builtin_amdgcn_raw_buffer_store_b16(
+            (unsigned short)outA,
+            range_n(J.ed[0], c == 0 && J.dest[0] != nullptr && J.ed[0] != nullptr, pe, partial),
+            2u * p, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b16(
+            (unsigned short)outB,
+            range_n(J.ed[1], c == 0 && J.dest[1] != nullptr && J.ed[1] != nullptr, pe, partial),
+            2u * p, 0, 0);
     }
     // tile sums of child ch (w r1, w r1^2, w r2, r = N / D over categories)
     // by wave (ch + 2 par) % C
 #pragma unroll
     for (int ch = 0; ch < 2; ++ch) {
-        if (c != (ch + 2 * par) % C) continue;
-        const float* q = buf + (2 + 3 * ch) * C * TILE + lane;
-        float D = wc[0] * q[0], N1 = wc[0] * q[C * TILE], N2 = wc[0] * q[2 * C * TILE];
+        const bool mine_ch = c == (ch + 2 * par) % C;
+        float s1 = 0.0f, sq = 0.0f, s2 = 0.0f;
+        if (mine_ch) {
+            const float* q = buf + (2 + 3 * ch) * C * TILE + lane;
+            float D = wc[0] * q[0], N1 = wc[0] * q[C * TILE], N2 = wc[0] * q[2 * C * TILE];
 #pragma unroll
-        for (int cc = 1; cc < C; ++cc) {
-            D += wc[cc] * q[cc * TILE];
-            N1 += wc[cc] * q[(C + cc) * TILE];
-            N2 += wc[cc] * q[(2 * C + cc) * TILE];
+            for (int cc = 1; cc < C; ++cc) {
+                D += wc[cc] * q[cc * TILE];
+                N1 += wc[cc] * q[(C + cc) * TILE];
+                N2 += wc[cc] * q[(2 * C + cc) * TILE];
+            }
+            const float inv = 1.0f / D;
+            const float r1 = in ? N1 * inv : 0.0f, r2 = in ? N2 * inv : 0.0f;
+            const float a = w * r1;
+            s1 = wave_sum_f32(a);
+            sq = wave_sum_f32(a * r1);
+            s2 = wave_sum_f32(w * r2);
         }
-        const float inv = 1.0f / D;
-        const float r1 = in ? N1 * inv : 0.0f, r2 = in ? N2 * inv : 0.0f;
-        const float a = w * r1;
-        const float s1 = wave_sum_f32(a), sq = wave_sum_f32(a * r1), s2 = wave_sum_f32(w * r2);
-        if (lane == 0) {
-            GLOBAL double* pt = gout<double>(partial);
-            const long row = (long)J.oidx[ch] * 3;
-            pt[(row + 0) * ntiles + tile] = (double)s1;
-            pt[(row + 1) * ntiles + tile] = (double)sq;
-            pt[(row + 2) * ntiles + tile] = (double)s2;
+        // every lane stores the same wave sums (one write per address);
+        // the other waves' stores fall in an empty range
+        const auto rp = range_n(partial, mine_ch, 0xffffffffu, partial);
+        const unsigned row = (unsigned)J.oidx[ch] * 3u;
+        const double sv3[3] = {(double)s1, (double)sq, (double)s2};
+#pragma unroll
+        for (int k3 = 0; k3 < 3; ++k3) {
+            typedef unsigned u2 __attribute__((ext_vector_type(2)));
+            const u2 d = {(unsigned)__double2loint(sv3[k3]), (unsigned)__double2hiint(sv3[k3])};
+            __builtin_amdgcn_raw_buffer_store_b64(
+                d, rp, ((row + k3) * (unsigned)ntiles + (unsigned)tile) * 8u, 0, 0);
         }
     }
-    if (keepA) store4(at_w<float>(J.dest[0], off), va);
-    if (keepB) store4(at_w<float>(J.dest[1], off), vb);
+    {
+        // pre-order partials of the children that keep them; patterns past
+        // P fall outside the category's range
+        typedef unsigned u4 __attribute__((ext_vector_type(4)));
+        const unsigned pb = (unsigned)P * 16u;
+        const unsigned cat = off - 16u * (unsigned)p;   // the category's first byte
+        const u4 qa = {__float_as_uint(va[0]), __float_as_uint(va[1]), __float_as_uint(va[2]),
+                       __float_as_uint(va[3])};
+        const u4 qb = {__float_as_uint(vb[0]), __float_as_uint(vb[1]), __float_as_uint(vb[2]),
+                       __float_as_uint(vb[3])};
+        __builtin_amdgcn_raw_buffer_store_b128(
+            qa, range_n((const char*)J.dest[0] + cat, J.dest[0] != nullptr, pb, partial), 16u * p, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(
+            qb, range_n((const char*)J.dest[1] + cat, J.dest[1] != nullptr, pb, partial), 16u * p, 0, 0);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         pa[i] = va[i];
         pb[i] = vb[i];
     }
     pea = outA;
-    peb = outB;    if (QG) {
+    peb = outB;
+    if (QG) {
         // rate-matrix gradient of both branches (after the stores, which
         // then drain meanwhile): top_a = pre(v) * P_b x_b
         // (ua), top_b = pre(v) * P_a x_a (ub); L_p = sum_c w_c Lc, the same
         // order of additions in every wave; scale factors cancel in
-        // top x^T / L_p
+        // top x^T / L_p.  The job's sums are formed in FP32 (Hj) and added
+        // in FP64 by the caller.
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Hj[i] = 0.0f;
         float L = 0.0f, wcc = wc[0];
 #pragma unroll
         for (int cc = 0; cc < C; ++cc) {
@@ -708,7 +760,7 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
                 x[i] = !in ? 0.0f : tip ? (st == i ? 1.0f : 0.0f) : xv;
             }
             const int code = in && st < 4 ? st : 4;
-            qgrad_block(ta, x, sv, r, lf + ch * 16 + r * 4, Hr, tip, code);
+            qgrad_block(ta, x, sv, r, lf + ch * 16 + r * 4, Hj, tip, code);
         }
     }
 }
@@ -756,6 +808,30 @@ __device__ __forceinline__ double sum_by_s(double v) {
 // ([slot][16], slot = base + chunk * tiles + tile); every slot of a pass is
 // written once, by one workgroup, so the pass needs no zeroed accumulator
 // and its sums do not depend on the timing of the workgroups.
+// The same for one job's FP32 sums (one-job workgroups): the row sums of
+// each s by two DPP rotations, the rows' and waves' sums (16 per wave)
+// through LDS, added in FP64 in a fixed order by wave 0.
+template <int C>
+__device__ __forceinline__ void qgrad_store_f32(float (&Hj)[4], int c, int lane, float* red,
+                                                double* __restrict__ hacc, long slot) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        Hj[i] += dpp_movf<0x124, 0xf>(Hj[i]);   // row_ror:4
+        Hj[i] += dpp_movf<0x128, 0xf>(Hj[i]);   // row_ror:8: the row's sum for this s
+    }
+    if ((lane & 15) < 4) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) red[(c * 4 + (lane >> 4)) * 16 + (lane & 3) * 4 + i] = Hj[i];
+    }
+    __syncthreads();
+    if (c == 0 && lane < 16) {
+        double h = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4 * C; ++q) h += (double)red[q * 16 + lane];
+        gout<double>(hacc)[slot * 16 + lane] = h;
+    }
+}
+
 template <int C>
 __device__ __forceinline__ void qgrad_store(double (&Hr)[4], int c, int lane, double* red,
                                             double* __restrict__ hacc, long slot) {
@@ -781,7 +857,7 @@ template <int C, bool QG, int NPAR>
 constexpr size_t node_f32_lds() {
     return (size_t)(NPAR * (QG ? 9 : 8) * C * TILE +
                     C * NPAR * (QG ? NODE_F32_QBLOCK : NODE_F32_BLOCK)) * sizeof(float) +
-           (QG ? 16 * C * sizeof(double) + 32 * sizeof(float) : 0);
+           (QG ? 64 * C * sizeof(float) + 32 * sizeof(float) : 0);
 }
 
 // Each job's operands are loaded when the job starts (node_issue_direct):
@@ -806,8 +882,61 @@ __global__ __launch_bounds__(64 * C, QG ? 4 : 7) void k_pren4f(
         NodeOps4<float> S0;
         node_issue_direct<AUTO, QG>(J, jm, c, off, p, lane, S0, F);
         if (QG && o == 0) qgrad_vector_store(ev, sv);   // read after the job's barrier
+        float Hj[4];
         node_compute_f32<C, AUTO, QG>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
-                                      wc, w, partial, tile, ntiles, sv, Hr);
+                                      wc, w, partial, tile, ntiles, sv, Hj);
+        if (QG) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) Hr[i] += (double)Hj[i];
+        }
+    }
+    if (QG) qgrad_store<C>(Hr, c, lane, red, hacc, base + (long)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// A batched level walked in chunks of independent jobs, software-
+// pipelined: while job o computes, the operands of job o + 1 are in flight
+// (two operand register sets, the loop unrolled by two so no loaded
+// register is copied).  Every job issues the same memory operations -- the
+// loads and stores a job does not need go to empty buffer ranges, and the
+// one past the chunk's end loads nothing -- so the compiler's waits count
+// exactly the next job's loads and never the stores.
+template <int C, bool AUTO, bool QG>
+__global__ __launch_bounds__(64 * C, QG ? 3 : 4) void k_pren4f_pipe(
+    const NodeJob* __restrict__ jobs, int njobs, int chunk, int P, long cs,
+    const float* __restrict__ jm, const double* __restrict__ cw, const double* __restrict__ pw,
+    double* partial, const double* __restrict__ F, const double* __restrict__ eig,
+    double* __restrict__ hacc, long base) {
+    NODE_F32_SETUP(QG ? 9 : 8, 2)
+    double* red = reinterpret_cast<double*>(smat + C * 2 * NODE_F32_QBLOCK);
+    float* sv = reinterpret_cast<float*>(red + 16 * C);
+    double Hr[4] = {0.0, 0.0, 0.0, 0.0};
+    const double ev = QG ? qgrad_vector_load(eig) : 0.0;
+    jobs += blockIdx.y * chunk;
+    const int n = min(chunk, njobs - (int)blockIdx.y * chunk);
+    NodeOps4<float> A, B;
+    NodeJob JA = jobs[0], JB;
+    node_issue_direct<AUTO, QG>(JA, jm, c, off, p, lane, A, F);
+    if (QG) qgrad_vector_store(ev, sv);   // read after the first job's barrier
+    for (int o = 0;; o += 2) {
+        JB = jobs[min(o + 1, n - 1)];
+        node_issue_direct<AUTO, QG>(JB, jm, c, off, p, lane, B, F, o + 1 < n);
+        float Hj[4];
+        node_compute_f32<C, AUTO, QG>(JA, A, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
+                                      wc, w, partial, tile, ntiles, sv, Hj);
+        if (QG) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) Hr[i] += (double)Hj[i];
+        }
+        if (o + 1 >= n) break;
+        JA = jobs[min(o + 2, n - 1)];
+        node_issue_direct<AUTO, QG>(JA, jm, c, off, p, lane, A, F, o + 2 < n);
+        node_compute_f32<C, AUTO, QG>(JB, B, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
+                                      wc, w, partial, tile, ntiles, sv, Hj);
+        if (QG) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) Hr[i] += (double)Hj[i];
+        }
+        if (o + 2 >= n) break;
     }
     if (QG) qgrad_store<C>(Hr, c, lane, red, hacc, base + (long)blockIdx.y * gridDim.x + blockIdx.x);
 }
@@ -820,16 +949,17 @@ __global__ __launch_bounds__(64 * C, QG ? 7 : 8) void k_pren4f_level(
     long base) {
     NODE_F32_SETUP(QG ? 9 : 8, 1)
     NodeOps4<float> S0;
-    double* red = reinterpret_cast<double*>(smat + C * NODE_F32_QBLOCK);
-    float* sv = reinterpret_cast<float*>(red + 16 * C);
-    double Hr[4] = {0.0, 0.0, 0.0, 0.0};
+    float* red = smat + C * NODE_F32_QBLOCK;   // [C][4 rows][16]
+    float* sv = red + 64 * C;
+    float Hj[4];
     const double ev = QG ? qgrad_vector_load(eig) : 0.0;
     const NodeJob J = jobs[blockIdx.y];
     node_issue_direct<AUTO, QG>(J, jm, c, off, p, lane, S0, F);
     if (QG) qgrad_vector_store(ev, sv);   // read after the job's barrier
     node_compute_f32<C, AUTO, QG, 1>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
-                                     wc, w, partial, tile, ntiles, sv, Hr);
-    if (QG) qgrad_store<C>(Hr, c, lane, red, hacc, base + (long)blockIdx.y * gridDim.x + blockIdx.x);
+                                     wc, w, partial, tile, ntiles, sv, Hj);
+    if (QG)
+        qgrad_store_f32<C>(Hj, c, lane, red, hacc, base + (long)blockIdx.y * gridDim.x + blockIdx.x);
 }
 #undef NODE_F32_SETUP
 
@@ -857,9 +987,18 @@ static void launch_pre_nodes_f32(const NodeJob* djobs, int njobs, bool batched, 
         const size_t lds = node_f32_lds<C, QG, 1>();
         if (d.autoScale) hipLaunchKernelGGL((k_pren4f_level<C, true, QG>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
         else hipLaunchKernelGGL((k_pren4f_level<C, false, QG>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
-    } else {
-        if (!batched || chunk > njobs) chunk = njobs;
+    } else if (batched) {
+        // chunks of independent jobs: the pipelined walk
+        chunk = std::min(chunk, njobs);
         rows = (njobs + chunk - 1) / chunk;
+        const dim3 grid(tiles, rows);
+        const size_t lds = node_f32_lds<C, QG, 2>();
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4f_pipe<C, true, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
+        else hipLaunchKernelGGL((k_pren4f_pipe<C, false, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
+    } else {
+        // a fused list: one chunk, parents forwarded from registers
+        chunk = njobs;
+        rows = 1;
         const dim3 grid(tiles, rows);
         const size_t lds = node_f32_lds<C, QG, 2>();
         if (d.autoScale) hipLaunchKernelGGL((k_pren4f<C, true, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
