@@ -288,6 +288,7 @@ class Evaluator:
         self._gout = (np.zeros(1), np.zeros(n), np.zeros(n), np.zeros(n))
         self._qout = (np.zeros(S * S), np.zeros(S)) if rate_matrix else None
         self._gpost = post
+        self._gperm = None
         manual = self.scaling in ("manual", "manual_log")
         cum = self.n - 1 if manual else api.OP_NONE
         self.inst.begin_capture()
@@ -322,12 +323,13 @@ class Evaluator:
         captured rate-matrix gradient also (dlogL/dQ, dlogL/dpi_root)."""
         self.inst.wait_graph(self._graph)
         lnl, s1, sq, s2 = self._gout
-        g1 = np.zeros(2 * self.n - 1)
-        g2 = np.zeros(2 * self.n - 1)
-        g1[self._gpost] = s1
-        g2[self._gpost] = s2 - sq
-        nodes = self.branch_nodes()
-        res = (float(lnl[0]), g1[nodes], g2[nodes])
+        if getattr(self, "_gperm", None) is None:
+            # position in the operation list of each branch's node
+            at = np.empty(2 * self.n - 1, dtype=np.int64)
+            at[self._gpost] = np.arange(len(self._gpost))
+            self._gperm = at[np.asarray(self.branch_nodes())]
+        perm = self._gperm
+        res = (float(lnl[0]), s1[perm], (s2 - sq)[perm])
         if self._qout is not None:
             S = self.prob.S
             res += (self._qout[0].reshape(S, S).copy(), self._qout[1].copy())

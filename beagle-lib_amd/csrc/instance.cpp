@@ -1590,8 +1590,10 @@ const double* Instance::stageOut(const double* dsrc, size_t n, size_t at) {
 // With qg (4 states, FP32, 1, 2 or 4 categories) the pass also forms the
 // rate-matrix gradient sums H of eigen-system qg->eigen: qg->slots slots of
 // qg->hacc ([slot][16], at most node jobs x tiles); the work buffer also
-// holds their block sums, the root-frequency partials, the reduced sums and
-// the final gradient.  qg->fused tells whether the node kernels ran
+// holds their block sums, the root's workgroup rows (qg->rootRows, [1 + S]
+// [P / 256]), H (qg->sums) and the call's outputs in one block (qg->out:
+// logL, dlogL/dpi, dlogL/dQ, then the 3 sums per operation, which the
+// reduction writes there).  qg->fused tells whether the node kernels ran
 // (otherwise nothing of the gradient was formed).
 int Instance::enqueuePreDerivatives(const int* ops, int count, const int* postIdx,
                                     const int* d1Idx, const int* d2Idx, int wIdx, bool reduce,
@@ -1604,8 +1606,8 @@ int Instance::enqueuePreDerivatives(const int* ops, int count, const int* postId
     const size_t nPart = (size_t)count * 3 * ntiles;
     const int S = dims_.S, nb = (dims_.P + 255) / 256;
     const long cap = qg ? (long)(count / 2 + 1) * ntiles : 0;
-    const size_t nq = qg ? 16 * ((size_t)cap + (size_t)cap / 128 + 1) + (size_t)S * nb +
-                               2 * (16 + (size_t)S)
+    const size_t nq = qg ? 16 * ((size_t)cap + (size_t)cap / 128 + 1) + (size_t)(1 + S) * nb +
+                               16 + (1 + (size_t)S + 16 + 3 * (size_t)count)
                          : 0;
     double* wk = reinterpret_cast<double*>(
         grow(dWork_, workCap_, (nPart + 3 * (size_t)count + nq) * sizeof(double)));
@@ -1615,9 +1617,10 @@ int Instance::enqueuePreDerivatives(const int* ops, int count, const int* postId
     if (qg) {
         qg->hacc = sums + 3 * (size_t)count;
         qg->slotPart = qg->hacc + 16 * (size_t)cap;
-        qg->freqPart = qg->slotPart + 16 * ((size_t)cap / 128 + 1);
-        qg->sums = qg->freqPart + (size_t)S * nb;
-        qg->out = qg->sums + 16 + S;
+        qg->rootRows = qg->slotPart + 16 * ((size_t)cap / 128 + 1);
+        qg->sums = qg->rootRows + (size_t)(1 + S) * nb;
+        qg->out = qg->sums + 16;
+        sums = qg->out + 1 + S + 16;   // the outputs' block: [logL | S | 16 | 3 count]
         NodeQGrad q{dF_, dVVi_ + (size_t)qg->eigen * 2 * S * S, qg->hacc, cap, 0};
         nodes = launchPreNodes(ops, count, postIdx, wk, &q);
         qg->fused = nodes;
@@ -1723,8 +1726,12 @@ int Instance::logLikelihoodAndGradient(int rootIdx, int wIdx, int fIdx, int cumI
     if (freqs && (!validBuffer(rootIdx) || !dPartials_[rootIdx] || !validEigen(fIdx) ||
                   hFreqs_[fIdx].empty()))
         return BEAGLE_ERROR_OUT_OF_RANGE;
-    rc = enqueueRoot(rootIdx, wIdx, fIdx, cumIdx);
-    if (rc != BEAGLE_SUCCESS) return rc;
+    // the root's validity first: its kernel runs after the pass
+    if (!validBuffer(rootIdx) || !dPartials_[rootIdx] || !validEigen(fIdx) ||
+        hFreqs_[fIdx].empty() || dims_.sStride != 1)
+        return BEAGLE_ERROR_OUT_OF_RANGE;
+    const int cs = always_ ? rootIdx - tipCount_ : (auto_ ? BEAGLE_OP_NONE : cumIdx);
+    if (cs != BEAGLE_OP_NONE && !validScale(cs)) return BEAGLE_ERROR_OUT_OF_RANGE;
     rc = setRootPrePartials(&rootPreIdx, &fIdx, 1);
     if (rc != BEAGLE_SUCCESS) return rc;
     const double* sums = nullptr;
@@ -1732,10 +1739,12 @@ int Instance::logLikelihoodAndGradient(int rootIdx, int wIdx, int fIdx, int cumI
     rc = enqueuePreDerivatives(ops, count, postIdx, d1Idx, d2Idx, wIdx, true, &sums, &qg);
     if (rc != BEAGLE_SUCCESS) return rc;
     if (!qg.fused) {
-        // the operations do not pair into node jobs: the derivative results
-        // as logLikelihoodAndDerivatives delivers them, then the separate
-        // gradient
+        // the operations do not pair into node jobs: logL, the derivative
+        // results as logLikelihoodAndDerivatives delivers them, then the
+        // separate gradient
         if (capturing_) captureBreak("the rate-matrix gradient needs node-paired operations");
+        rc = enqueueRoot(rootIdx, wIdx, fIdx, cumIdx);
+        if (rc != BEAGLE_SUCCESS) return rc;
         reserveStage(1 + 3 * (size_t)count);
         const double* hl = stageOut(dResult_, 1, 0);
         const double* h = stageOut(sums, 3 * (size_t)count, 1);
@@ -1752,44 +1761,46 @@ int Instance::logLikelihoodAndGradient(int rootIdx, int wIdx, int fIdx, int cumI
         if (rq != BEAGLE_SUCCESS) return rq;
         return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
     }
+    // the root after the pass (nothing in it reads logL): its workgroup
+    // rows of w_p log L_p and of the root-frequency gradient, reduced in one
+    // launch into the outputs' block [logL | dlogL/dpi | dlogL/dQ | sums]
     const double* cw = dCatWeights_ + (size_t)wIdx * dims_.C;
     const double* eig = dVVi_ + (size_t)e * 2 * S * S;
+    const short* ex = auto_ ? dExp_[rootIdx] : nullptr;
+    const int nb = launch_root_rows<float>(dPartials_[rootIdx], ex, cw,
+                                           dFreqs_ + (size_t)fIdx * S,
+                                           cs >= 0 ? scale(cs) : nullptr, dPatternWeights_,
+                                           dSiteL_, qg.rootRows, dims_, stream_);
+    launch_reduce_rows(qg.rootRows, nb, 1 + S, qg.out, stream_);
     launch_qgrad_slot_sums(qg.hacc, qg.slots, qg.slotPart, qg.sums, stream_);
-    if (freqs) {
-        const int nb = (dims_.P + 255) / 256;
-        launch_root_freq_grad<float>(dPartials_[rootIdx], cw, dFreqs_ + (size_t)fIdx * S,
-                                     dPatternWeights_, qg.freqPart, dims_, stream_);
-        launch_reduce_rows(qg.freqPart, nb, S, qg.sums + 16, stream_);
-    }
-    launch_qgrad_finish(qg.sums, eig, freqs ? S : 0, qg.out, stream_);
+    launch_qgrad_finish(qg.sums, eig, 0, qg.out + 1 + S, stream_);
     HIPCHECK(hipGetLastError());
+    const size_t nOut = 1 + (size_t)S + 16 + 3 * (size_t)count;
+    const size_t dOff = (1 + (size_t)S + 16) * sizeof(double);   // the sums' first byte
+    double* dst[3] = {outD1, outD1Sq, outD2};
     if (capturing_) {
-        cap_.outs.push_back({outLogL, captureResult(dResult_, 1), 1, 1, true});
-        const size_t off = captureResult(sums, 3 * (size_t)count);
-        double* dst[3] = {outD1, outD1Sq, outD2};
+        const size_t off = captureResult(qg.out, nOut);
+        cap_.outs.push_back({outLogL, off, 1, 1, true});
+        if (freqs) cap_.outs.push_back({outFreqs, off + sizeof(double), S, 1, false});
+        if (outQ) cap_.outs.push_back({outQ, off + (1 + (size_t)S) * sizeof(double), 16, 1, false});
         for (int k = 0; k < 3; ++k)
-            if (dst[k]) cap_.outs.push_back({dst[k], off + k * sizeof(double), count, 3, false});
-        if (outQ) cap_.outs.push_back({outQ, captureResult(qg.out, 16), 16, 1, false});
-        if (freqs) cap_.outs.push_back({outFreqs, captureResult(qg.out + 16, S), S, 1, false});
+            if (dst[k]) cap_.outs.push_back({dst[k], off + dOff + k * sizeof(double), count, 3, false});
         return BEAGLE_SUCCESS;
     }
-    const size_t nd = 3 * (size_t)count;
-    reserveStage(1 + nd + 16 + S);
-    const double* hl = stageOut(dResult_, 1, 0);
-    const double* h = stageOut(sums, nd, 1);
-    const double* hq = stageOut(qg.out, 16 + (freqs ? S : 0), 1 + nd);
+    const double* h = stageOut(qg.out, nOut, 0);
     sync();
-    lastLogL_ = hl[0];
+    lastLogL_ = h[0];
     *outLogL = lastLogL_;
-    for (int o = 0; o < count; ++o) {
-        if (outD1) outD1[o] = h[3 * o];
-        if (outD1Sq) outD1Sq[o] = h[3 * o + 1];
-        if (outD2) outD2[o] = h[3 * o + 2];
-    }
-    if (outQ)
-        for (int q = 0; q < 16; ++q) outQ[q] = hq[q];
     if (freqs)
-        for (int i = 0; i < S; ++i) outFreqs[i] = hq[16 + i];
+        for (int i = 0; i < S; ++i) outFreqs[i] = h[1 + i];
+    if (outQ)
+        for (int q = 0; q < 16; ++q) outQ[q] = h[1 + S + q];
+    const double* hd = h + 1 + S + 16;
+    for (int o = 0; o < count; ++o) {
+        if (outD1) outD1[o] = hd[3 * o];
+        if (outD1Sq) outD1Sq[o] = hd[3 * o + 1];
+        if (outD2) outD2[o] = hd[3 * o + 2];
+    }
     return lastLogL_ != lastLogL_ ? BEAGLE_ERROR_FLOATING_POINT : BEAGLE_SUCCESS;
 }
 

@@ -267,7 +267,7 @@ private:
     struct QGradPass {
         int eigen;
         bool fused;
-        double *hacc, *slotPart, *freqPart, *sums, *out;
+        double *hacc, *slotPart, *rootRows, *sums, *out;
         long slots;   // H slots the pass wrote
     };
     int enqueuePreDerivatives(const int* ops, int count, const int* postIdx, const int* d1Idx,

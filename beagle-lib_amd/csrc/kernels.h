@@ -356,6 +356,12 @@ void launch_root_range(const void* root, const short* rootExp, const double* cat
                        const double* freqs, const double* cumulative,
                        const double* patternWeights, double* siteLogL, double* blockSums,
                        double* result, int p0, int p1, const Dims& d, hipStream_t st);
+// k_root's workgroup sums without the reduction (4 states, sStride 1):
+// rows [1 + 4][nb], w_p log L_p then the root-frequency gradient; returns nb.
+template <typename R>
+int launch_root_rows(const void* root, const short* rootExp, const double* catWeights,
+                     const double* freqs, const double* cumulative, const double* patternWeights,
+                     double* siteLogL, double* rows, const Dims& d, hipStream_t st);
 template <typename R>
 void launch_root(const void* root, const short* rootExp, const double* catWeights,
                  const double* freqs, const double* cumulative, const double* patternWeights,

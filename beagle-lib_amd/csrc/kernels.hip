@@ -1507,10 +1507,12 @@ __global__ __launch_bounds__(WG) void k_root(const R* __restrict__ root,
                                              const double* __restrict__ pw,
                                              const short* __restrict__ ex,
                                              double* __restrict__ site, double* __restrict__ bsum,
-                                             int p0, int p1, Dims d) {
+                                             int p0, int p1, Dims d,
+                                             double* __restrict__ fpart = nullptr) {
     __shared__ double sh[WG / 64];
     const int p = p0 + blockIdx.x * WG + threadIdx.x;
     double contrib = 0.0;
+    double fg[4] = {0.0, 0.0, 0.0, 0.0};   // fpart (4 states): w_p t_i / L_p
     if (p < p1) {
         double sum = 0.0;
         if (d.S == 4) {
@@ -1521,6 +1523,8 @@ __global__ __launch_bounds__(WG) void k_root(const R* __restrict__ root,
                 for (int i = 0; i < 4; ++i) t[i] += (double)v[i] * cw[c];
             }
             for (int i = 0; i < 4; ++i) sum += freqs[i] * t[i];
+            if (fpart)
+                for (int i = 0; i < 4; ++i) fg[i] = pw[p] * t[i] / sum;
         } else {
             for (int i = 0; i < d.S; ++i) {
                 double t = 0.0;
@@ -1537,6 +1541,14 @@ __global__ __launch_bounds__(WG) void k_root(const R* __restrict__ root,
     }
     const double b = block_sum(contrib, sh);
     if (threadIdx.x == 0) bsum[blockIdx.x] = b;
+    if (fpart) {
+        // dlogL/dpi_i through the root distribution, per workgroup: rows
+        // [4][gridDim.x] (scale factors cancel in t_i / L_p)
+        for (int i = 0; i < 4; ++i) {
+            const double f = block_sum(fg[i], sh);
+            if (threadIdx.x == 0) fpart[(long)i * gridDim.x + blockIdx.x] = f;
+        }
+    }
 }
 
 // Several roots combined into one likelihood per pattern
@@ -1697,6 +1709,20 @@ void launch_root(const void* root, const short* rootExp, const double* catWeight
     const int nb = root_blocks_and_launch<R>(root, rootExp, catWeights, freqs, cumulative,
                                              patternWeights, siteLogL, blockSums, 0, d.P, d, st);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(WG), 0, st, blockSums, nb, 1, result, 0);
+}
+
+// The root's workgroup sums for a later reduction: rows[0][nb] of
+// w_p site_p and, 4 states, rows[1..4][nb] of the root-frequency gradient.
+// Returns nb.
+template <typename R>
+int launch_root_rows(const void* root, const short* rootExp, const double* catWeights,
+                     const double* freqs, const double* cumulative, const double* patternWeights,
+                     double* siteLogL, double* rows, const Dims& d, hipStream_t st) {
+    const int nb = (d.P + WG - 1) / WG;
+    hipLaunchKernelGGL(k_root<R>, dim3(nb), dim3(WG), 0, st, reinterpret_cast<const R*>(root),
+                       catWeights, freqs, cumulative, patternWeights, rootExp, siteLogL, rows, 0,
+                       d.P, d, rows + nb);
+    return nb;
 }
 
 template <typename R>
@@ -2297,6 +2323,9 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
     template void launch_pre_levels<R>(const PreJob*, const int*, int, const Dims&, const void*, \
                                        void*, const double*, const double*, double*,             \
                                        hipStream_t);                                             \
+    template int launch_root_rows<R>(const void*, const short*, const double*, const double*,    \
+                                     const double*, const double*, double*, double*,            \
+                                     const Dims&, hipStream_t);                                 \
     template void launch_root<R>(const void*, const short*, const double*, const double*,        \
                                  const double*, const double*, double*, double*, double*,        \
                                  const Dims&, hipStream_t);                                      \
