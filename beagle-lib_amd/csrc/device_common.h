@@ -129,6 +129,20 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t range_n(const void* base, bool
     return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, n, 0x00020000);
 }
 
+// Buffer resource of n bytes at `base` (n from a job record, so already
+// wave-uniform; -1 the whole buffer, 0 an empty range: no access reaches
+// memory, so a null base is never dereferenced).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t range_w(const void* base, int n) {
+    // readfirstlane states the uniformity (a no-op on scalar registers), so
+    // no access needs a loop over distinct resources
+    const unsigned long long a = reinterpret_cast<unsigned long long>(base);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    void* b = reinterpret_cast<void*>(((unsigned long long)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, __builtin_amdgcn_readfirstlane(n),
+                                             0x00020000);
+}
+
 // four reals at byte offset `at` of a buffer resource
 template <typename R>
 __device__ __forceinline__ void load4_range(__amdgpu_buffer_rsrc_t r, unsigned at, R (&v)[4]) {

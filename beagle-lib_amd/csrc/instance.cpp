@@ -1859,6 +1859,16 @@ bool Instance::makeNodeJobs(const int* ops, int count, const int* postIdx,
         const int* qa = ops + (size_t)pairs[j][0] * BEAGLE_OP_COUNT;
         const int* qb = ops + (size_t)pairs[j][1] * BEAGLE_OP_COUNT;
         acc[j] = Access{{qa[3], -1, -1}, {qa[0], qb[0]}};
+        // the FP32 kernels' buffer ranges (a forwarded parent: launchNodePlan)
+        N.rU = -1;
+        N.rEu = N.eParent ? -1 : 0;
+        for (int ch = 0; ch < 2; ++ch) {
+            N.rX[ch] = N.k[ch] == 0 ? -1 : 0;
+            N.rS[ch] = N.k[ch] != 0 ? -1 : 0;
+            N.rEx[ch] = N.ex[ch] ? -1 : 0;
+            N.rD[ch] = N.dest[ch] ? dims_.P * 16 : 0;
+            N.rEd[ch] = N.dest[ch] && N.ed[ch] ? dims_.P * 2 : 0;
+        }
     }
     return true;
 }
@@ -1921,6 +1931,7 @@ void Instance::launchNodePlan(const std::vector<NodeJob>& nodes, const std::vect
                 const NodeJob& Pv = all[j - 1];
                 if (Pv.dest[0] && J.parent == Pv.dest[0]) J.kPar = 2;
                 else if (Pv.dest[1] && J.parent == Pv.dest[1]) J.kPar = 3;
+                if (J.kPar != 0) J.rU = J.rEu = 0;   // forwarded: nothing loaded
             }
         }
         launches.push_back({first, (int)s.ops.size(), s.batched});

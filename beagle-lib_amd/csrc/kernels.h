@@ -82,6 +82,13 @@ struct NodeJob {
     int oidx[2];               // operation index of each child in the derivative output
     int mbase;                 // this job's block in the gathered matrices (reals)
     int mF[2];                 // the children's branch factors in the factor slab (doubles)
+    // buffer-range sizes of the FP32 node kernels' accesses, set on the host
+    // (-1: the whole buffer, 0: an empty range, so the access is skipped):
+    // loads of the parent, each child's partials and states, the exponents;
+    // stores of each child's pre-order partials (P patterns, bytes) and
+    // exponents (bytes)
+    int rU, rX[2], rS[2], rEu, rEx[2];
+    int rD[2], rEd[2];
 };
 
 // One transition-matrix update: P, dP/dt, d2P/dt2 for all categories.

@@ -409,40 +409,16 @@ struct QgVec {
     float wh[4];   // V^-1[s][k]
 };
 
-// A tip child's beta needs no products: beta_i = column s_i of V^-1, so
-// lane 4b + j takes V^-1[j][s_i] with s_i (code: the state, 4 for a
-// missing state or a lane past the patterns) read from lane i of its quad.
-__device__ __forceinline__ float tip_beta1(int ci, const QgVec& v) {
-    const float lo = (ci & 1) ? v.wh[1] : v.wh[0], hi = (ci & 1) ? v.wh[3] : v.wh[2];
-    return ci > 3 ? 0.0f : (ci & 2) ? hi : lo;
-}
-
-__device__ __forceinline__ fvec4 tip_beta(int code, const QgVec& v) {
-    fvec4 be;   // quad_perm [i,i,i,i]: lane i of the quad
-    be[0] = tip_beta1(__builtin_amdgcn_mov_dpp(code, 0x00, 0xf, 0xf, false), v);
-    be[1] = tip_beta1(__builtin_amdgcn_mov_dpp(code, 0x55, 0xf, 0xf, false), v);
-    be[2] = tip_beta1(__builtin_amdgcn_mov_dpp(code, 0xAA, 0xf, 0xf, false), v);
-    be[3] = tip_beta1(__builtin_amdgcn_mov_dpp(code, 0xFF, 0xf, 0xf, false), v);
-    return be;
-}
-
 // f: the lane's four factors F[s][.] (FP64, or FP32 where a job's sums are
 // formed in FP32 and added in FP64 afterwards); H accumulates in T.
 template <typename FT, typename T>
 __device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&x)[4],
-                                            const QgVec& v, const FT* f, T (&Hr)[4],
-                                            bool tip = false, int code = 0) {
+                                            const QgVec& v, const FT* f, T (&Hr)[4]) {
     fvec4 al = {0.0f, 0.0f, 0.0f, 0.0f}, be = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (tip) {
-        be = tip_beta(code, v);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) al = __builtin_amdgcn_mfma_f32_4x4x1f32(ta[k], v.vh[k], al, 0, 0, 0);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            al = __builtin_amdgcn_mfma_f32_4x4x1f32(ta[k], v.vh[k], al, 0, 0, 0);
-            be = __builtin_amdgcn_mfma_f32_4x4x1f32(x[k], v.wh[k], be, 0, 0, 0);
-        }
+    for (int k = 0; k < 4; ++k) {
+        al = __builtin_amdgcn_mfma_f32_4x4x1f32(ta[k], v.vh[k], al, 0, 0, 0);
+        be = __builtin_amdgcn_mfma_f32_4x4x1f32(x[k], v.wh[k], be, 0, 0, 0);
     }
     fvec4 m = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -455,8 +431,7 @@ __device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&
 // where the products need them rather than held in registers.
 template <typename FT, typename T>
 __device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&x)[4],
-                                            const float* sv, int s, const FT* f, T (&Hr)[4],
-                                            bool tip, int code) {
+                                            const float* sv, int s, const FT* f, T (&Hr)[4]) {
     QgVec v;
     const fvec4 a = *(const fvec4*)(sv + 4 * s), b = *(const fvec4*)(sv + 16 + 4 * s);
 #pragma unroll
@@ -464,7 +439,7 @@ __device__ __forceinline__ void qgrad_block(const float (&ta)[4], const float (&
         v.vh[k] = a[k];
         v.wh[k] = b[k];
     }
-    qgrad_block(ta, x, v, f, Hr, tip, code);
+    qgrad_block(ta, x, v, f, Hr);
 }
 
 // V^T and V^-1 rows (FP32) for LDS: thread t < 32 loads entry t
@@ -475,8 +450,12 @@ __device__ __forceinline__ double qgrad_vector_load(const double* __restrict__ e
     return eig[t < 16 ? (t & 3) * 4 + (t >> 2) : t];
 }
 
+// ... and after them the rows of a tip's one-hot vectors: e_0 .. e_3 and a
+// zero row (a missing state or a lane past the patterns), sv[32 + 4 code]
 __device__ __forceinline__ void qgrad_vector_store(double e, float* sv) {
-    if (threadIdx.x < 32) sv[threadIdx.x] = (float)e;
+    const int t = threadIdx.x;
+    if (t < 32) sv[t] = (float)e;
+    else if (t < 52) sv[t] = (t - 32) % 5 == 0 && t < 48 ? 1.0f : 0.0f;
 }
 
 __device__ __forceinline__ void qgrad_vectors(const double* __restrict__ eig, int s, QgVec& v) {
@@ -544,12 +523,12 @@ __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float*
     // traffic) where the job has no such operand (or, !valid, for no job):
     // no branches, so no value merges that would make the compiler wait
     // for earlier loads before issuing the later ones
-    load4_range<float>(range_if(J.parent, valid && J.kPar == 0, jm), off, o.u);
-    load4_range<float>(range_if(J.x[0], valid && J.k[0] == 0, jm), off, o.xa);
-    load4_range<float>(range_if(J.x[1], valid && J.k[1] == 0, jm), off, o.xb);
-    o.sa = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[0], valid && J.k[0] != 0, jm),
+    load4_range<float>(range_w(J.parent, valid ? J.rU : 0), off, o.u);
+    load4_range<float>(range_w(J.x[0], valid ? J.rX[0] : 0), off, o.xa);
+    load4_range<float>(range_w(J.x[1], valid ? J.rX[1] : 0), off, o.xb);
+    o.sa = (int)__builtin_amdgcn_raw_buffer_load_b32(range_w(J.x[0], valid ? J.rS[0] : 0),
                                                      4u * p, 0, 0);
-    o.sb = (int)__builtin_amdgcn_raw_buffer_load_b32(range_if(J.x[1], valid && J.k[1] != 0, jm),
+    o.sb = (int)__builtin_amdgcn_raw_buffer_load_b32(range_w(J.x[1], valid ? J.rS[1] : 0),
                                                      4u * p, 0, 0);
     {
         const int l = min(lane, NODE_MATS / 4 - 1);
@@ -572,11 +551,11 @@ __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float*
     }
     if (AUTO) {
         o.eu = (short)__builtin_amdgcn_raw_buffer_load_b16(
-            range_if(J.eParent, valid && J.kPar == 0 && J.eParent != nullptr, jm), 2u * p, 0, 0);
+            range_w(J.eParent, valid ? J.rEu : 0), 2u * p, 0, 0);
         o.ea = (short)__builtin_amdgcn_raw_buffer_load_b16(
-            range_if(J.ex[0], valid && J.ex[0] != nullptr, jm), 2u * p, 0, 0);
+            range_w(J.ex[0], valid ? J.rEx[0] : 0), 2u * p, 0, 0);
         o.eb = (short)__builtin_amdgcn_raw_buffer_load_b16(
-            range_if(J.ex[1], valid && J.ex[1] != nullptr, jm), 2u * p, 0, 0);
+            range_w(J.ex[1], valid ? J.rEx[1] : 0), 2u * p, 0, 0);
     }
 }
 
@@ -662,14 +641,13 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
         outA = ea + eu + (J.ex[1] ? o.eb : 0);
         outB = eb + eu + (J.ex[0] ? o.ea : 0);
         // wave 0 stores the exponents; patterns past P fall outside the range
-        const unsigned pe = (unsigned)P * 2u;
         __builtin_amdgcn_raw_buffer_store_b16(
             (unsigned short)outA,
-            range_n(J.ed[0], c == 0 && J.dest[0] != nullptr && J.ed[0] != nullptr, pe, partial),
+            range_w(J.ed[0], c == 0 ? J.rEd[0] : 0),
             2u * p, 0, 0);
         __builtin_amdgcn_raw_buffer_store_b16(
             (unsigned short)outB,
-            range_n(J.ed[1], c == 0 && J.dest[1] != nullptr && J.ed[1] != nullptr, pe, partial),
+            range_w(J.ed[1], c == 0 ? J.rEd[1] : 0),
             2u * p, 0, 0);
     }
     // tile sums of child ch (w r1, w r1^2, w r2, r = N / D over categories)
@@ -687,7 +665,7 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
                 N1 += wc[cc] * q[(C + cc) * TILE];
                 N2 += wc[cc] * q[(2 * C + cc) * TILE];
             }
-            const float inv = 1.0f / D;
+            const float inv = __builtin_amdgcn_rcpf(D);   // 1 ulp
             const float r1 = in ? N1 * inv : 0.0f, r2 = in ? N2 * inv : 0.0f;
             const float a = w * r1;
             s1 = wave_sum_f32(a);
@@ -696,7 +674,7 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
         }
         // every lane stores the same wave sums (one write per address);
         // the other waves' stores fall in an empty range
-        const auto rp = range_n(partial, mine_ch, 0xffffffffu, partial);
+        const auto rp = range_w(partial, mine_ch ? -1 : 0);
         const unsigned row = (unsigned)J.oidx[ch] * 3u;
         const double sv3[3] = {(double)s1, (double)sq, (double)s2};
 #pragma unroll
@@ -711,16 +689,15 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
         // pre-order partials of the children that keep them; patterns past
         // P fall outside the category's range
         typedef unsigned u4 __attribute__((ext_vector_type(4)));
-        const unsigned pb = (unsigned)P * 16u;
         const unsigned cat = off - 16u * (unsigned)p;   // the category's first byte
         const u4 qa = {__float_as_uint(va[0]), __float_as_uint(va[1]), __float_as_uint(va[2]),
                        __float_as_uint(va[3])};
         const u4 qb = {__float_as_uint(vb[0]), __float_as_uint(vb[1]), __float_as_uint(vb[2]),
                        __float_as_uint(vb[3])};
         __builtin_amdgcn_raw_buffer_store_b128(
-            qa, range_n((const char*)J.dest[0] + cat, J.dest[0] != nullptr, pb, partial), 16u * p, 0, 0);
+            qa, range_w((const char*)J.dest[0] + cat, J.rD[0]), 16u * p, 0, 0);
         __builtin_amdgcn_raw_buffer_store_b128(
-            qb, range_n((const char*)J.dest[1] + cat, J.dest[1] != nullptr, pb, partial), 16u * p, 0, 0);
+            qb, range_w((const char*)J.dest[1] + cat, J.rD[1]), 16u * p, 0, 0);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -744,23 +721,28 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
             L += wc[cc] * buf[(8 * C + cc) * TILE + lane];
             if (c == cc) wcc = wc[cc];
         }
-        const float coef = in ? (w * wcc) / L : 0.0f;
+        // coef is zero past the patterns (their partials are zero or
+        // finite, never written by the passes): ta is then zero there
+        const float coef = in ? (w * wcc) * __builtin_amdgcn_rcpf(L) : 0.0f;
 #pragma unroll
         for (int ch = 0; ch < 2; ++ch) {
             const bool tip = ch == 0 ? tipA : tipB;
             const int st = ch == 0 ? sa : sb;
             float ta[4], x[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < 4; ++i) ta[i] = coef * (ch == 0 ? ua[i] : ub[i]);
+            if (tip) {
                 // a tip's vector is one-hot; a missing state (4) reads the
-                // pad column, which does not depend on Q: zero; lanes past
-                // the patterns add nothing
-                ta[i] = in ? coef * (ch == 0 ? ua[i] : ub[i]) : 0.0f;
-                const float xv = ch == 0 ? o.xa[i] : o.xb[i];
-                x[i] = !in ? 0.0f : tip ? (st == i ? 1.0f : 0.0f) : xv;
+                // pad column, which does not depend on Q: the zero row;
+                // lanes past the patterns add nothing
+                const fvec4 e = *(const fvec4*)(sv + 32 + 4 * (in && st < 4 ? st : 4));
+#pragma unroll
+                for (int i = 0; i < 4; ++i) x[i] = e[i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) x[i] = in ? (ch == 0 ? o.xa[i] : o.xb[i]) : 0.0f;
             }
-            const int code = in && st < 4 ? st : 4;
-            qgrad_block(ta, x, sv, r, lf + ch * 16 + r * 4, Hj, tip, code);
+            qgrad_block(ta, x, sv, r, lf + ch * 16 + r * 4, Hj);
         }
     }
 }
@@ -857,7 +839,7 @@ template <int C, bool QG, int NPAR>
 constexpr size_t node_f32_lds() {
     return (size_t)(NPAR * (QG ? 9 : 8) * C * TILE +
                     C * NPAR * (QG ? NODE_F32_QBLOCK : NODE_F32_BLOCK)) * sizeof(float) +
-           (QG ? 64 * C * sizeof(float) + 32 * sizeof(float) : 0);
+           (QG ? 64 * C * sizeof(float) + 52 * sizeof(float) : 0);
 }
 
 // Each job's operands are loaded when the job starts (node_issue_direct):
@@ -1390,7 +1372,7 @@ __device__ __forceinline__ void qm_compute(const NodeJob& J, const QWOps<float>&
             const float xv = ch == 0 ? o.xa[i] : o.xb[i];
             x[i] = !in ? 0.0f : tip ? (st == i ? 1.0f : 0.0f) : xv;
         }
-        qgrad_block(ta, x, v, sf + ch * 64 + c * 16 + r * 4, Hr, tip, in && st < 4 ? st : 4);
+        qgrad_block(ta, x, v, sf + ch * 64 + c * 16 + r * 4, Hr);
     }
 }
 
