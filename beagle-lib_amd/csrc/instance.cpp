@@ -506,10 +506,11 @@ int Instance::setRootPrePartials(const int* bufIdx, const int* fIdx, int count) 
     const int b = bufIdx[0], f = fIdx[0];
     if (!validBuffer(b) || !validEigen(f) || hFreqs_[f].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
     ensurePartials(b);
-    if (dp_) launch_tile_vector<double>(dPartials_[b], dFreqs_ + (size_t)f * dims_.S, dims_, stream_);
-    else launch_tile_vector<float>(dPartials_[b], dFreqs_ + (size_t)f * dims_.S, dims_, stream_);
+    // the same launch zeroes the buffer's scaling exponents
+    short* ex = auto_ ? dExp_[b] : nullptr;
+    if (dp_) launch_tile_vector<double>(dPartials_[b], dFreqs_ + (size_t)f * dims_.S, dims_, stream_, ex);
+    else launch_tile_vector<float>(dPartials_[b], dFreqs_ + (size_t)f * dims_.S, dims_, stream_, ex);
     HIPCHECK(hipGetLastError());
-    clearExponent(b);
     return BEAGLE_SUCCESS;
 }
 

@@ -416,6 +416,7 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
 template <typename R>
 void launch_pad_set(void* dst, long n, R v, hipStream_t st);
 template <typename R>
-void launch_tile_vector(void* dst, const double* v, const Dims& d, hipStream_t st);
+void launch_tile_vector(void* dst, const double* v, const Dims& d, hipStream_t st,
+                        short* ex = nullptr);
 
 }  // namespace hipbeagle

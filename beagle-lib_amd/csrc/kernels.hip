@@ -2244,9 +2244,11 @@ void launch_fill(double* dst, double v, long n, hipStream_t st) {
 // dst[c][p][i] = v[i] for every category and pattern (root pre-order
 // partials = state frequencies).
 template <typename R>
-__global__ __launch_bounds__(WG) void k_tile_vector(R* dst, const double* __restrict__ v, Dims d) {
+__global__ __launch_bounds__(WG) void k_tile_vector(R* dst, const double* __restrict__ v, Dims d,
+                                                    short* __restrict__ ex) {
     const long t = (long)blockIdx.x * WG + threadIdx.x;
     const long n = (long)d.C * d.Ppad * d.S;
+    if (ex && t < d.Ppad) ex[t] = 0;   // the buffer's scaling exponents
     if (t >= n) return;
     const int i = t % d.S;
     const long cp = t / d.S;
@@ -2255,10 +2257,10 @@ __global__ __launch_bounds__(WG) void k_tile_vector(R* dst, const double* __rest
 }
 
 template <typename R>
-void launch_tile_vector(void* dst, const double* v, const Dims& d, hipStream_t st) {
+void launch_tile_vector(void* dst, const double* v, const Dims& d, hipStream_t st, short* ex) {
     const long n = (long)d.C * d.Ppad * d.S;
     hipLaunchKernelGGL(k_tile_vector<R>, dim3((n + WG - 1) / WG), dim3(WG), 0, st,
-                       reinterpret_cast<R*>(dst), v, d);
+                       reinterpret_cast<R*>(dst), v, d, ex);
 }
 
 template <typename R>
@@ -2346,7 +2348,7 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
     template void launch_transpose<R>(const void* const*, void* const*, int, int, int,           \
                                       hipStream_t);                                              \
     template void launch_pad_set<R>(void*, long, R, hipStream_t);                               \
-    template void launch_tile_vector<R>(void*, const double*, const Dims&, hipStream_t);
+    template void launch_tile_vector<R>(void*, const double*, const Dims&, hipStream_t, short*);
 
 INSTANTIATE(double)
 INSTANTIATE(float)
