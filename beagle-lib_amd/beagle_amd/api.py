@@ -561,7 +561,9 @@ class Instance:
         rate_matrix_gradient(eigen, ops, post, root=root) with one wait.
         Returns (logL, s1, sq, s2, dlogL/dQ [S][S], dlogL/dpi_root [S]).
         out: caller-owned float64 arrays (logL[1], s1[n], sq[n], s2[n],
-        q[S*S], pi[S]), needed while capturing a graph."""
+        q[S*S], pi[S]), needed while capturing a graph; False: enqueue only,
+        nothing returned (no copy-back or wait on an asynchronous
+        instance)."""
         a = _ip(np.asarray(ops).reshape(-1))
         n = len(a[0]) // 7
         b, c, d = _ip(post), _ip(d1), _ip(d2)
@@ -569,6 +571,12 @@ class Instance:
         if out is None:
             out = (np.zeros(1), np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(S * S),
                    np.zeros(S))
+        if out is False:
+            # enqueue only (no outputs, no copy-back)
+            self._c("ExtCalculateLogLikelihoodAndGradient", root, weights_index, freqs_index,
+                    cumulative, root_pre, a[1], n, b[1], c[1], d[1], eigen,
+                    None, None, None, None, None, None)
+            return None
         lnl, s1, sq, s2, gq, gf = out
         self._c("ExtCalculateLogLikelihoodAndGradient", root, weights_index, freqs_index,
                 cumulative, root_pre, a[1], n, b[1], c[1], d[1], eigen,

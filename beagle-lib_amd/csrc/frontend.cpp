@@ -652,7 +652,11 @@ int beagleExtCalculateLogLikelihoodAndGradient(
     double* outSumFirstDerivatives, double* outSumSquaredFirstDerivatives,
     double* outSumSecondDerivatives, double* outGradientRateMatrix,
     double* outGradientFrequencies) {
-    if (!outLogLikelihood) return BEAGLE_ERROR_OUT_OF_RANGE;
+    // every output NULL: enqueue only (checked by the instance)
+    const bool none = !outLogLikelihood && !outSumFirstDerivatives &&
+                      !outSumSquaredFirstDerivatives && !outSumSecondDerivatives &&
+                      !outGradientRateMatrix && !outGradientFrequencies;
+    if (!outLogLikelihood && !none) return BEAGLE_ERROR_OUT_OF_RANGE;
     return withInstance(instance, [&](Instance* in) {
         return in->logLikelihoodAndGradient(
             rootBufferIndex, categoryWeightsIndex, stateFrequenciesIndex, cumulativeScaleIndex,

@@ -1712,7 +1712,9 @@ int Instance::logLikelihoodAndGradient(int rootIdx, int wIdx, int fIdx, int cumI
     const bool fused = S == 4 && !dp_ && node_f32_supported(dims_.C) && !preTransposed_ &&
                        !complex_ && d2Idx != nullptr && dF_ != nullptr && count > 0 &&
                        partialsBytes_ < (size_t(1) << 32);
+    const bool none = outLogL == nullptr;   // every output NULL (the front end checks)
     if (!fused) {
+        if (none) return BEAGLE_ERROR_OUT_OF_RANGE;
         int rc = logLikelihoodAndDerivatives(rootIdx, wIdx, fIdx, cumIdx, rootPreIdx, ops, count,
                                              postIdx, d1Idx, d2Idx, outLogL, outD1, outD1Sq,
                                              outD2);
@@ -1744,6 +1746,7 @@ int Instance::logLikelihoodAndGradient(int rootIdx, int wIdx, int fIdx, int cumI
         // results as logLikelihoodAndDerivatives delivers them, then the
         // separate gradient
         if (capturing_) captureBreak("the rate-matrix gradient needs node-paired operations");
+        if (none) return BEAGLE_ERROR_OUT_OF_RANGE;
         rc = enqueueRoot(rootIdx, wIdx, fIdx, cumIdx);
         if (rc != BEAGLE_SUCCESS) return rc;
         reserveStage(1 + 3 * (size_t)count);
@@ -1776,6 +1779,12 @@ int Instance::logLikelihoodAndGradient(int rootIdx, int wIdx, int fIdx, int cumI
     launch_qgrad_slot_sums(qg.hacc, qg.slots, qg.slotPart, qg.sums, stream_);
     launch_qgrad_finish(qg.sums, eig, 0, qg.out + 1 + S, stream_);
     HIPCHECK(hipGetLastError());
+    if (none) {
+        // nothing to return: no copy; an asynchronous instance returns
+        // without waiting
+        finishCall();
+        return BEAGLE_SUCCESS;
+    }
     const size_t nOut = 1 + (size_t)S + 16 + 3 * (size_t)count;
     const size_t dOff = (1 + (size_t)S + 16) * sizeof(double);   // the sums' first byte
     double* dst[3] = {outD1, outD1Sq, outD2};

@@ -36,13 +36,14 @@ partitions: 48 partitions with 4-, 20- and 61-state models and their own
 trees, FP32 logL + gradient per partition, eager calls against HIP-graph
 replays launched across the instances' streams.
 
-roofline: the dominant kernel of the value path, the node-fused FP32
-pre-order + derivative traversal of the caterpillar (one launch of 127
-node jobs, i.e. 254 pre-order operations), timed with HIP events on the
-library's stream in a pass after the timed loop, through the extension
-with no outputs so that no reduction, copy-back or host wait falls inside
-the bracket; algorithmic bytes per launch from preorder_bytes().  Inputs
-are resident in HBM before timing.
+roofline: the dominant kernel of the value path, the FP32 node kernel of
+the one-call gradient (pre-order partials + branch derivatives + rate-
+matrix sums of the 512-taxon tree's 511 node jobs, one launch per
+dependency level), timed per pass with HIP events on the library's stream
+after the timed loop: the one call captured alone as a HIP graph and its
+replays bracketed back to back, so no host work falls inside the bracket;
+algorithmic bytes per pass from preorder_bytes() plus the root's read.  Inputs are resident in
+HBM before timing.
 """
 import argparse
 import json
@@ -694,32 +695,48 @@ def calibrate_copy():
 
 
 def roofline_timing(ev, stream, prob, steps):
-    """The dominant kernel alone: the caterpillar's fused pre-order launch
-    bracketed by HIP events on the library's stream, through the extension
-    with no outputs (no reduction, copy-back or host wait inside the
-    bracket; the job upload and the matrix gather, a few microseconds, are).
-    The post-order traversal is enqueued just before, without waiting for
-    it, so the host's preparation of the launch overlaps device work and the
-    first event is not stamped ahead of it.  Returns (mean ms per launch,
-    algorithmic GB/s, algorithmic bytes per launch)."""
+    """The dominant kernel of the value path: the one-call gradient's
+    node pass (pre-order partials, branch derivatives and rate-matrix sums of
+    all 511 node jobs: for this tree one launch of the chained node kernel
+    k_pren4f for the narrow top levels, 13 launches of k_pren4f_level, one
+    per dependency level, and one k_pren4f launch for the narrow bottom
+    levels, as the tuner picks them).  The call (beagleExtCalculateLogLikelihoodAndGradient)
+    is captured alone as a HIP graph and each replay is bracketed by HIP
+    events on the library's stream, back to back, so no host work falls
+    inside the bracket; the root, the reductions and the result copy that
+    follow the pass (about 4% of the bracket) are inside it.  Returns (mean
+    ms per pass, algorithmic GB/s, algorithmic bytes per pass:
+    preorder_bytes() plus one read of the root partials)."""
     import numpy as np
     import torch
-    ops, post, _, _ = ev.fused_operations()
+    ops, post, d1, d2 = ev.fused_operations()
     post_idx = dict(zip(map(tuple, ops.tolist()), post.tolist()))
     pre_bytes = preorder_bytes(prob, ops, lambda o: post_idx[tuple(o.tolist())], 4, True)
+    pre_bytes += prob.C * prob.P * prob.S * 4 + SHORT * prob.P   # the root's read
+    root = prob.tree.root
+    n, S = len(post), prob.S
+    outs = (np.zeros(1), np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(S * S), np.zeros(S))
+    ev.fused_gradient(rate_matrix=True)
+    ev.inst.begin_capture()
+    try:
+        ev.inst.log_likelihood_and_gradient(root, ev.pre_base + root, ops, post, d1, d2,
+                                            out=outs)
+    finally:
+        graph = ev.inst.end_capture()
+    for _ in range(3):
+        ev.inst.launch_graph(graph)
+    ev.inst.wait_graph(graph)
     pairs = []
     for _ in range(max(steps, 10)):
-        ev.update_matrices()
-        ev.post_order()
-        ev.set_root_pre()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        ev.inst.pre_partials_with_derivatives(ops, post, [ev.d1_index] * len(post),
-                                              [ev.d2_index] * len(post), outputs=False)
+        ev.inst.launch_graph(graph)
         e1.record(stream)
         pairs.append((e0, e1))
+    ev.inst.wait_graph(graph)
     torch.cuda.synchronize()
+    ev.inst.destroy_graph(graph)
     kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
     return kernel_ms, pre_bytes / (kernel_ms * 1e-3) / 1e9, pre_bytes
 
@@ -985,9 +1002,13 @@ def main():
                            "replayed as one HIP graph with its branch lengths set per replay"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_pren4f<4,true>: node-fused FP32 pre-order + derivative "
-                               "traversal of the 512-taxon tree (511 node jobs = 1,022 "
-                               "operations)",
+                     "kernel": "k_pren4f_level<4,true,true>: the one-call gradient's node "
+                               "pass of the 512-taxon tree (511 node jobs = 1,022 branches; "
+                               "pre-order partials, branch derivatives, rate-matrix sums): "
+                               "13 level launches plus two chained k_pren4f launches for "
+                               "the narrow top and bottom levels per pass; kernel_ms is "
+                               "per pass, bracketing one graph replay of the call "
+                               "(root, reductions and result copy included)",
                      "kernel_ms": kernel_ms, "algorithmic_bytes": pre_bytes},
         "cpu_baseline": cpu,
         "adjoint": head,

@@ -235,7 +235,10 @@ int beagleExtCalculateRateMatrixGradient(int instance, int eigenIndex,
  * inside the derivative pass, from the partials and matrices that pass
  * already reads (no second traversal); otherwise the two calls run in turn.
  * Same arguments, outputs and errors as the two calls; recorded by
- * beagleExtBeginCapture on the fused path.  The summation order of the
+ * beagleExtBeginCapture on the fused path.  With every output pointer NULL
+ * (fused path only; otherwise BEAGLE_ERROR_OUT_OF_RANGE) the work is
+ * enqueued and nothing is copied back: an instance created with
+ * BEAGLE_FLAG_COMPUTATION_ASYNCH returns without waiting.  The summation order of the
  * rate-matrix sums follows the traversal plan the instance picks, so with
  * autotuning on the first (tuning) call may differ from later ones in the
  * last bits.  No counterpart in the reference. */

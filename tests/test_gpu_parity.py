@@ -771,6 +771,37 @@ def test_log_likelihood_and_gradient_one_call(amd, C, scaling, keep_tips, precis
         np.testing.assert_array_equal(one[3], sep[3])
 
 
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_log_likelihood_and_gradient_enqueue_only(amd, precision):
+    """beagleExtCalculateLogLikelihoodAndGradient with every output NULL:
+    on the fused FP32 path the work is enqueued (success) and the next call
+    with outputs returns what a call without the enqueue-only one returns,
+    bit for bit; on the FP64 two-call path the call is refused with
+    BEAGLE_ERROR_OUT_OF_RANGE."""
+    prob = synth.make_problem(83, 23, 1500, S=4, shape="yule", missing=0.05, mean_length=0.2)
+    ev = driver.Evaluator(amd, prob, scaling="auto", precision=precision,
+                          extra_flags=api.FLAG_COMPUTATION_ASYNCH)
+    ref = ev.fused_gradient(rate_matrix=True)
+    ref = ev.fused_gradient(rate_matrix=True)   # after the tuning call
+    ops, post, d1, d2 = ev.fused_operations()
+    root = prob.tree.root
+    if precision == "double":
+        with pytest.raises(api.BeagleError) as e:
+            ev.inst.log_likelihood_and_gradient(root, ev.pre_base + root, ops, post, d1, d2,
+                                                out=False)
+        assert e.value.code == api.ERROR_OUT_OF_RANGE
+        ev.close()
+        return
+    for _ in range(3):
+        assert ev.inst.log_likelihood_and_gradient(root, ev.pre_base + root, ops, post, d1,
+                                                   d2, out=False) is None
+    got = ev.fused_gradient(rate_matrix=True)
+    ev.close()
+    assert got[0] == ref[0]
+    for a, b in zip(got[1:], ref[1:]):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+
+
 def test_log_likelihood_and_gradient_against_oracle_finite_differences(amd, oracle):
     """The one-call FP32 gradient (dlogL/dQ formed inside the derivative
     pass) against fourth-order differences of the oracle's log-likelihood
