@@ -334,20 +334,41 @@ __device__ __forceinline__ float dpp_movf(float v) {
         __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
 }
 
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_movf_rows(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, ROWS, 0xf, false));
+// Three sums over the 64 lanes of a wave at once, each in the order of a
+// pairwise tree (lane pairs, quads, row halves, rows, then (R0 + R1) +
+// (R2 + R3) over the four 16-lane rows): the first step leaves a's pair sums
+// in the even lanes and b's in the odd ones of one register, c's in the
+// even lanes of a second; the quad and row steps keep the lane parity; the
+// two registers' row totals are then added across rows by one permlane16
+// and one permlane32 swap.  a ends in lane 14, b in lane 15, c in lane 30.
+__device__ __forceinline__ void wave_sum3_f32(float a, float b, float c, float& sa, float& sb,
+                                              float& sc) {
+    const bool odd = threadIdx.x & 1;
+    float x = (odd ? b : a) + dpp_movf<0xB1, 0xf>(odd ? a : b);
+    float y = c + dpp_movf<0xB1, 0xf>(c);
+    x += dpp_movf<0x4E, 0xf>(x);
+    y += dpp_movf<0x4E, 0xf>(y);
+    x += dpp_movf<0x114, 0xf>(x);
+    y += dpp_movf<0x114, 0xf>(y);
+    x += dpp_movf<0x118, 0xf>(x);
+    y += dpp_movf<0x118, 0xf>(y);
+    // rows: [x0, y0, x2, y2] and [x1, y1, x3, y3]
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false,
+                                                    false);
+    const float r = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(r), __float_as_uint(r), false,
+                                                    false);
+    const float t = __uint_as_float(q[0]) + __uint_as_float(q[1]);
+    sa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), 14));
+    sb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), 15));
+    sc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), 30));
 }
 
-// Sum over the 64 lanes, the steps of wave_sum_dpp in FP32.
-__device__ __forceinline__ float wave_sum_f32(float v) {
-    v += dpp_movf<0xB1, 0xf>(v);
-    v += dpp_movf<0x4E, 0xf>(v);
-    v += dpp_movf<0x114, 0xf>(v);
-    v += dpp_movf<0x118, 0xf>(v);
-    v += dpp_movf_rows<0x142, 0xa>(v);
-    v += dpp_movf_rows<0x143, 0xc>(v);
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+// Largest of four non-negative floats: their bit patterns order as the
+// values do (+0 lowest), so integer maxima (no NaN quieting of the inputs).
+__device__ __forceinline__ float max4_nonneg(const float (&v)[4]) {
+    return __int_as_float(max(max(__float_as_int(v[0]), __float_as_int(v[1])),
+                              max(__float_as_int(v[2]), __float_as_int(v[3]))));
 }
 
 // The FP32 node kernels read each matrix of a job column by column from the
@@ -611,8 +632,8 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     float* buf = sx + par * NQ * C * TILE;
     ++k;
     float* mine = buf + c * TILE + lane;
-    mine[0 * C * TILE] = in ? fmaxf(fmaxf(va[0], va[1]), fmaxf(va[2], va[3])) : 0.0f;
-    mine[1 * C * TILE] = in ? fmaxf(fmaxf(vb[0], vb[1]), fmaxf(vb[2], vb[3])) : 0.0f;
+    mine[0 * C * TILE] = in ? max4_nonneg(va) : 0.0f;
+    mine[1 * C * TILE] = in ? max4_nonneg(vb) : 0.0f;
     mine[2 * C * TILE] = dA;
     mine[3 * C * TILE] = n1A;
     mine[4 * C * TILE] = n2A;
@@ -625,12 +646,13 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     int outA = 0, outB = 0;
     if (AUTO) {
-        float ma = buf[lane], mb = buf[C * TILE + lane];
+        int ima = __float_as_int(buf[lane]), imb = __float_as_int(buf[C * TILE + lane]);
 #pragma unroll
         for (int cc = 1; cc < C; ++cc) {
-            ma = fmaxf(ma, buf[cc * TILE + lane]);
-            mb = fmaxf(mb, buf[(C + cc) * TILE + lane]);
+            ima = max(ima, __float_as_int(buf[cc * TILE + lane]));
+            imb = max(imb, __float_as_int(buf[(C + cc) * TILE + lane]));
         }
+        const float ma = __int_as_float(ima), mb = __int_as_float(imb);
         const int ea = ma > 0.0f ? exponent_of(ma) : 0;
         const int eb = mb > 0.0f ? exponent_of(mb) : 0;
 #pragma unroll
@@ -668,9 +690,7 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
             const float inv = __builtin_amdgcn_rcpf(D);   // 1 ulp
             const float r1 = in ? N1 * inv : 0.0f, r2 = in ? N2 * inv : 0.0f;
             const float a = w * r1;
-            s1 = wave_sum_f32(a);
-            sq = wave_sum_f32(a * r1);
-            s2 = wave_sum_f32(w * r2);
+            wave_sum3_f32(a, a * r1, w * r2, s1, sq, s2);
         }
         // every lane stores the same wave sums (one write per address);
         // the other waves' stores fall in an empty range
