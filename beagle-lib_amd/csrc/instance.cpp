@@ -1947,13 +1947,14 @@ void Instance::launchNodePlan(const std::vector<NodeJob>& nodes, const std::vect
         launches.push_back({first, (int)s.ops.size(), s.batched});
     }
     const double* cw = dCatWeights_ + (size_t)preWeights_ * dims_.C;
-    const size_t perJob = (size_t)dims_.C * 6 * 20;
+    const bool f32 = !dp_ && node_f32_supported(dims_.C);
+    const size_t perJob = (size_t)dims_.C * (f32 ? kNodeJobF32 : kNodeJobMats);
     for (size_t j = 0; j < all.size(); ++j) all[j].mbase = (int)(j * perJob);
     void* jm = grow(dJobMats_, jobMatsCap_, all.size() * perJob * realSize());
     auto d = reinterpret_cast<const NodeJob*>(uploadJobs(all.data(), all.size() * sizeof(NodeJob)));
     if (dp_) launch_gather_node_matrices<double>(d, (int)all.size(), dims_, dMatrices_, jm, stream_);
     else launch_gather_node_matrices<float>(d, (int)all.size(), dims_, dMatrices_, jm, stream_,
-                                            node_f32_supported(dims_.C));
+                                            f32, q ? q->F : nullptr);
     for (const Launch& L : launches) {
         if (dp_) launch_pre_nodes<double>(d + L.first, L.n, L.batched, levelChunk_, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_);
         else launch_pre_nodes<float>(d + L.first, L.n, L.batched, levelChunk_, dims_, jm, dDummy_, cw, dPatternWeights_, partial, stream_, q);

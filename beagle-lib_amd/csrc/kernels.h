@@ -350,11 +350,16 @@ void launch_root_freq_grad(const void* root, const double* cw, const double* fre
 // jobMats as [job][category][6][20] reals, so a job's matrices for one
 // category are contiguous; job j's block starts at djobs[j].mbase.  Each
 // matrix keeps its [4 rows][5 columns] order, or with colMajor becomes
-// [5 columns][4 rows] (the FP32 node kernels, node_f32_supported).
+// [5 columns][4 rows] (the FP32 node kernels, node_f32_supported), and
+// then a category's block is kNodeJobF32 reals: the six matrices, 8 of
+// padding and the job's two branches' Frechet factors [2][16] (FP32, from
+// F at NodeJob::mF; zero without F).
+constexpr int kNodeJobMats = 6 * 20;
+constexpr int kNodeJobF32 = 160;
 template <typename R>
 void launch_gather_node_matrices(const NodeJob* djobs, int njobs, const Dims& d,
                                  const void* mats, void* jobMats, hipStream_t st,
-                                 bool colMajor = false);
+                                 bool colMajor = false, const double* F = nullptr);
 // 4-state FP32 node jobs with these category counts run on the kernels that
 // read column-major job matrices
 inline bool node_f32_supported(int C) { return C == 1 || C == 2 || C == 4; }

@@ -32,11 +32,16 @@ namespace hipbeagle {
 // of scalar registers.  Every lane stores its vector (the block holds 64 of
 // them; lanes past the matrices repeat the last one) so no branch
 // separates the lanes.
-constexpr int NODE_MATS = 6 * 20;
+constexpr int NODE_MATS = kNodeJobMats;
 constexpr int NODE_F32_BLOCK = 128;       // floats of a wave's block in the FP32 node kernels
 // ... and with the rate-matrix gradient: the matrices, then the factors of
 // the job's two branches for the wave's category ([2][16] floats)
 constexpr int NODE_F32_QBLOCK = NODE_F32_BLOCK + 32;
+// A category's block of gathered job data for the FP32 node kernels: the
+// six matrices column-major, padding, the factors -- the layout of the
+// wave's LDS block, so it is staged as it is loaded (40 lanes x 16 bytes).
+constexpr int NODE_JM = kNodeJobF32;
+static_assert(NODE_JM == NODE_F32_QBLOCK, "FP32 job block = LDS block");
 constexpr int NODE_MAT_BLOCK = 64 * 16;   // bytes of one wave's block
 
 // 32-bit addressing: a wave-uniform base pointer plus a per-lane byte
@@ -57,8 +62,7 @@ struct NodeOps4 {
     R xa[4], xb[4];            // children's post-order partials
     int sa, sb;                // children's tip states
     int eu, ea, eb;            // exponents (automatic scaling)
-    typename Vec16<R>::T mv;   // this lane's share of the job's matrices
-    dvec2 fv;                  // with the rate-matrix gradient: a share of its factors
+    typename Vec16<R>::T mv;   // this lane's share of the job's matrices (and factors)
 };
 
 // Byte offsets: `off` of this lane's 4-state vector in a partials buffer,
@@ -537,9 +541,7 @@ __device__ __forceinline__ T unset() {
 template <bool AUTO, bool QG = false>
 __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float* __restrict__ jm,
                                                   int c, unsigned off, int p, int lane,
-                                                  NodeOps4<float>& o,
-                                                  const double* __restrict__ F = nullptr,
-                                                  bool valid = true) {
+                                                  NodeOps4<float>& o, bool valid = true) {
     // every operand slot is loaded, from an empty range (reads 0, no
     // traffic) where the job has no such operand (or, !valid, for no job):
     // no branches, so no value merges that would make the compiler wait
@@ -552,23 +554,15 @@ __device__ __forceinline__ void node_issue_direct(const NodeJob& J, const float*
     o.sb = (int)__builtin_amdgcn_raw_buffer_load_b32(range_w(J.x[1], valid ? J.rS[1] : 0),
                                                      4u * p, 0, 0);
     {
-        const int l = min(lane, NODE_MATS / 4 - 1);
+        // the category's block: matrices (lanes 0-29), with the rate-matrix
+        // gradient padding and factors too (lanes 30-39); later lanes repeat
+        // the last vector
+        const int l = min(lane, (QG ? NODE_JM : NODE_MATS) / 4 - 1);
         const auto q = __builtin_amdgcn_raw_buffer_load_b128(
             range_if(jm, valid, jm),
-            (unsigned)((J.mbase + c * NODE_MATS + l * 4) * (int)sizeof(float)), 0, 0);
+            (unsigned)((J.mbase + c * NODE_JM + l * 4) * (int)sizeof(float)), 0, 0);
 #pragma unroll
         for (int i = 0; i < 4; ++i) o.mv[i] = __uint_as_float(q[i]);
-    }
-    if (QG) {
-        // lanes 0-7: child a's 16 factors of category c, lanes 8-15: child
-        // b's (lanes past 15 repeat lane 15's pair; a select of the two
-        // fields, not a lane-dependent index into the job)
-        const int fl = min(lane, 15);
-        const auto q = __builtin_amdgcn_raw_buffer_load_b128(
-            range_if(F, valid, jm),
-            (unsigned)(((fl >> 3 ? J.mF[1] : J.mF[0]) + c * 16 + 2 * (fl & 7)) * 8), 0, 0);
-        o.fv[0] = __hiloint2double((int)q[1], (int)q[0]);
-        o.fv[1] = __hiloint2double((int)q[3], (int)q[2]);
     }
     if (AUTO) {
         o.eu = (short)__builtin_amdgcn_raw_buffer_load_b16(
@@ -604,12 +598,7 @@ __device__ __forceinline__ void node_compute_f32(const NodeJob& J, const NodeOps
     const int par = NPAR == 2 ? (k & 1) : 0;
     float* lm = smat + (c * NPAR + par) * BLK;
     float* lf = lm + NODE_F32_BLOCK;   // QG: [2][16] factors (FP32)
-    *(fvec4*)(lm + min(lane, NODE_MATS / 4 - 1) * 4) = o.mv;
-    if (QG) {
-        typedef float f2 __attribute__((ext_vector_type(2)));
-        const f2 fq = {(float)o.fv[0], (float)o.fv[1]};
-        *(f2*)(lf + 2 * min(lane, 15)) = fq;
-    }
+    *(fvec4*)(lm + min(lane, BLK / 4 - 1) * 4) = o.mv;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
@@ -870,8 +859,7 @@ template <int C, bool AUTO, bool QG>
 __global__ __launch_bounds__(64 * C, QG ? 4 : 7) void k_pren4f(
     const NodeJob* __restrict__ jobs, int njobs, int chunk, int P, long cs,
     const float* __restrict__ jm, const double* __restrict__ cw, const double* __restrict__ pw,
-    double* partial, const double* __restrict__ F, const double* __restrict__ eig,
-    double* __restrict__ hacc, long base) {
+    double* partial, const double* __restrict__ eig, double* __restrict__ hacc, long base) {
     NODE_F32_SETUP(QG ? 9 : 8, 2)
     double* red = reinterpret_cast<double*>(smat + C * 2 * NODE_F32_QBLOCK);
     float* sv = reinterpret_cast<float*>(red + 16 * C);
@@ -882,7 +870,7 @@ __global__ __launch_bounds__(64 * C, QG ? 4 : 7) void k_pren4f(
     for (int o = 0; o < njobs; ++o) {
         const NodeJob J = jobs[o];
         NodeOps4<float> S0;
-        node_issue_direct<AUTO, QG>(J, jm, c, off, p, lane, S0, F);
+        node_issue_direct<AUTO, QG>(J, jm, c, off, p, lane, S0);
         if (QG && o == 0) qgrad_vector_store(ev, sv);   // read after the job's barrier
         float Hj[4];
         node_compute_f32<C, AUTO, QG>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
@@ -906,8 +894,7 @@ template <int C, bool AUTO, bool QG>
 __global__ __launch_bounds__(64 * C, QG ? 3 : 4) void k_pren4f_pipe(
     const NodeJob* __restrict__ jobs, int njobs, int chunk, int P, long cs,
     const float* __restrict__ jm, const double* __restrict__ cw, const double* __restrict__ pw,
-    double* partial, const double* __restrict__ F, const double* __restrict__ eig,
-    double* __restrict__ hacc, long base) {
+    double* partial, const double* __restrict__ eig, double* __restrict__ hacc, long base) {
     NODE_F32_SETUP(QG ? 9 : 8, 2)
     double* red = reinterpret_cast<double*>(smat + C * 2 * NODE_F32_QBLOCK);
     float* sv = reinterpret_cast<float*>(red + 16 * C);
@@ -917,11 +904,11 @@ __global__ __launch_bounds__(64 * C, QG ? 3 : 4) void k_pren4f_pipe(
     const int n = min(chunk, njobs - (int)blockIdx.y * chunk);
     NodeOps4<float> A, B;
     NodeJob JA = jobs[0], JB;
-    node_issue_direct<AUTO, QG>(JA, jm, c, off, p, lane, A, F);
+    node_issue_direct<AUTO, QG>(JA, jm, c, off, p, lane, A);
     if (QG) qgrad_vector_store(ev, sv);   // read after the first job's barrier
     for (int o = 0;; o += 2) {
         JB = jobs[min(o + 1, n - 1)];
-        node_issue_direct<AUTO, QG>(JB, jm, c, off, p, lane, B, F, o + 1 < n);
+        node_issue_direct<AUTO, QG>(JB, jm, c, off, p, lane, B, o + 1 < n);
         float Hj[4];
         node_compute_f32<C, AUTO, QG>(JA, A, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
                                       wc, w, partial, tile, ntiles, sv, Hj);
@@ -931,7 +918,7 @@ __global__ __launch_bounds__(64 * C, QG ? 3 : 4) void k_pren4f_pipe(
         }
         if (o + 1 >= n) break;
         JA = jobs[min(o + 2, n - 1)];
-        node_issue_direct<AUTO, QG>(JA, jm, c, off, p, lane, A, F, o + 2 < n);
+        node_issue_direct<AUTO, QG>(JA, jm, c, off, p, lane, A, o + 2 < n);
         node_compute_f32<C, AUTO, QG>(JB, B, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
                                       wc, w, partial, tile, ntiles, sv, Hj);
         if (QG) {
@@ -947,8 +934,7 @@ template <int C, bool AUTO, bool QG>
 __global__ __launch_bounds__(64 * C, QG ? 7 : 8) void k_pren4f_level(
     const NodeJob* __restrict__ jobs, int P, long cs, const float* __restrict__ jm,
     const double* __restrict__ cw, const double* __restrict__ pw, double* partial,
-    const double* __restrict__ F, const double* __restrict__ eig, double* __restrict__ hacc,
-    long base) {
+    const double* __restrict__ eig, double* __restrict__ hacc, long base) {
     NODE_F32_SETUP(QG ? 9 : 8, 1)
     NodeOps4<float> S0;
     float* red = smat + C * NODE_F32_QBLOCK;   // [C][4 rows][16]
@@ -956,7 +942,7 @@ __global__ __launch_bounds__(64 * C, QG ? 7 : 8) void k_pren4f_level(
     float Hj[4];
     const double ev = QG ? qgrad_vector_load(eig) : 0.0;
     const NodeJob J = jobs[blockIdx.y];
-    node_issue_direct<AUTO, QG>(J, jm, c, off, p, lane, S0, F);
+    node_issue_direct<AUTO, QG>(J, jm, c, off, p, lane, S0);
     if (QG) qgrad_vector_store(ev, sv);   // read after the job's barrier
     node_compute_f32<C, AUTO, QG, 1>(J, S0, pa, pb, pea, peb, smat, sx, k, c, lane, p, P, off,
                                      wc, w, partial, tile, ntiles, sv, Hj);
@@ -979,7 +965,6 @@ static void launch_pre_nodes_f32(const NodeJob* djobs, int njobs, bool batched, 
                                  NodeQGrad* q) {
     const int tiles = (d.P + TILE - 1) / TILE;
     const dim3 block(TILE * C);
-    const double* F = QG ? q->F : nullptr;
     const double* eig = QG ? q->eig : nullptr;
     double* hacc = QG ? q->hacc : nullptr;
     const long base = QG ? q->used : 0;
@@ -987,24 +972,24 @@ static void launch_pre_nodes_f32(const NodeJob* djobs, int njobs, bool batched, 
     if (batched && chunk <= 1) {
         rows = njobs;
         const size_t lds = node_f32_lds<C, QG, 1>();
-        if (d.autoScale) hipLaunchKernelGGL((k_pren4f_level<C, true, QG>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
-        else hipLaunchKernelGGL((k_pren4f_level<C, false, QG>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4f_level<C, true, QG>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial, eig, hacc, base);
+        else hipLaunchKernelGGL((k_pren4f_level<C, false, QG>), dim3(tiles, njobs), block, lds, st, djobs, d.P, d.catStride, m, cw, pw, partial, eig, hacc, base);
     } else if (batched) {
         // chunks of independent jobs: the pipelined walk
         chunk = std::min(chunk, njobs);
         rows = (njobs + chunk - 1) / chunk;
         const dim3 grid(tiles, rows);
         const size_t lds = node_f32_lds<C, QG, 2>();
-        if (d.autoScale) hipLaunchKernelGGL((k_pren4f_pipe<C, true, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
-        else hipLaunchKernelGGL((k_pren4f_pipe<C, false, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4f_pipe<C, true, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, eig, hacc, base);
+        else hipLaunchKernelGGL((k_pren4f_pipe<C, false, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, eig, hacc, base);
     } else {
         // a fused list: one chunk, parents forwarded from registers
         chunk = njobs;
         rows = 1;
         const dim3 grid(tiles, rows);
         const size_t lds = node_f32_lds<C, QG, 2>();
-        if (d.autoScale) hipLaunchKernelGGL((k_pren4f<C, true, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
-        else hipLaunchKernelGGL((k_pren4f<C, false, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, F, eig, hacc, base);
+        if (d.autoScale) hipLaunchKernelGGL((k_pren4f<C, true, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, eig, hacc, base);
+        else hipLaunchKernelGGL((k_pren4f<C, false, QG>), grid, block, lds, st, djobs, njobs, chunk, d.P, d.catStride, m, cw, pw, partial, eig, hacc, base);
     }
     if (QG) q->used += (long)rows * tiles;
 }
@@ -1608,30 +1593,42 @@ template <typename R>
 __global__ __launch_bounds__(WG) void k_gather_node_mats(const NodeJob* __restrict__ jobs,
                                                          int njobs, int C,
                                                          const R* __restrict__ mats,
-                                                         R* __restrict__ out, bool colMajor) {
+                                                         R* __restrict__ out, bool colMajor,
+                                                         const double* __restrict__ F) {
+    const int stride = colMajor ? NODE_JM : NODE_MATS;   // reals of a category's block
     const long t = (long)blockIdx.x * WG + threadIdx.x;
-    const long per = (long)C * NODE_MATS;
+    const long per = (long)C * stride;
     if (t >= per * njobs) return;
     const int j = (int)(t / per);
     const int rem = (int)(t - (long)j * per);
-    const int c = rem / NODE_MATS, r = rem - c * NODE_MATS;
-    const int m = r / 20, e = r - m * 20;
-    // row-major [4][5] as stored, or column-major [5][4]
-    const int d = colMajor ? m * 20 + (e % 5) * 4 + e / 5 : r;
+    const int c = rem / stride, r = rem - c * stride;
     const NodeJob& J = jobs[j];
-    const int mo = m == 0 ? J.moP[0] : m == 1 ? J.moP[1] : m == 2 ? J.moD1[0]
-                 : m == 3 ? J.moD2[0] : m == 4 ? J.moD1[1] : J.moD2[1];
-    out[(long)J.mbase + c * NODE_MATS + d] = mats[(long)mo + c * 20 + e];
+    R v = (R)0;
+    int d = r;
+    if (r < NODE_MATS) {
+        const int m = r / 20, e = r - m * 20;
+        // row-major [4][5] as stored, or column-major [5][4]
+        d = colMajor ? m * 20 + (e % 5) * 4 + e / 5 : r;
+        const int mo = m == 0 ? J.moP[0] : m == 1 ? J.moP[1] : m == 2 ? J.moD1[0]
+                     : m == 3 ? J.moD2[0] : m == 4 ? J.moD1[1] : J.moD2[1];
+        v = mats[(long)mo + c * 20 + e];
+    } else if (r >= NODE_F32_BLOCK && F) {
+        // the branches' Frechet factors of category c, [child][16]
+        const int f = r - NODE_F32_BLOCK;
+        v = (R)F[(long)J.mF[f >> 4] + c * 16 + (f & 15)];
+    }
+    out[(long)J.mbase + c * stride + d] = v;
 }
 
 template <typename R>
 void launch_gather_node_matrices(const NodeJob* djobs, int njobs, const Dims& d,
-                                 const void* mats, void* jobMats, hipStream_t st, bool colMajor) {
-    const long n = (long)njobs * d.C * NODE_MATS;
+                                 const void* mats, void* jobMats, hipStream_t st, bool colMajor,
+                                 const double* F) {
+    const long n = (long)njobs * d.C * (colMajor ? NODE_JM : NODE_MATS);
     if (n <= 0) return;
     hipLaunchKernelGGL(k_gather_node_mats<R>, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, st,
                        djobs, njobs, d.C, reinterpret_cast<const R*>(mats),
-                       reinterpret_cast<R*>(jobMats), colMajor);
+                       reinterpret_cast<R*>(jobMats), colMajor, F);
 }
 
 template <typename R>
@@ -1682,7 +1679,7 @@ void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, 
 
 #define INSTANTIATE(R)                                                                          \
     template void launch_gather_node_matrices<R>(const NodeJob*, int, const Dims&, const void*, \
-                                                 void*, hipStream_t, bool);                     \
+                                                 void*, hipStream_t, bool, const double*);                     \
     template void launch_qgrad<R>(const NodeJob*, int, const Dims&, const void*, const void*,   \
                                   const double*, const double*, void*, const double*,           \
                                   const double*, double*, hipStream_t);                         \
