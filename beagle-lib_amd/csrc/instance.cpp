@@ -212,6 +212,8 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
         // looked up once per device with that device current (null: the entry
         // point reports BEAGLE_ERROR_GENERAL)
         noAllocScratch_ = noalloc_scratch(device_);
+        HIPCHECK(hipDeviceGetAttribute(&maxLdsPerBlock_, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                       device_));
         partStart_ = {0, patternCount};
         if (const char* m = std::getenv("BEAGLE_AMD_TRAVERSAL")) traversalMode_ = std::atoi(m);
         if (const char* m = std::getenv("BEAGLE_AMD_LEVEL_CHUNK")) levelChunk_ = std::max(1, std::atoi(m));
@@ -2090,6 +2092,28 @@ int Instance::rateMatrixGradientGeneric(int e, const int* ops, int count, const 
     if (freqs && (!validBuffer(rootIdx) || !dPartials_[rootIdx] || !validEigen(fIdx) ||
                   hFreqs_[fIdx].empty()))
         return BEAGLE_ERROR_OUT_OF_RANGE;
+    // FP32 partials with 5-64 states and at most 4 categories: the matrix
+    // cores, with each branch's P_sib and P packed (as a partials operand's
+    // matrix, or by state for a tip)
+    const bool mfmaQ = !dp_ && mfmaAdjoint() && C <= 4 &&
+                       qgrad_mfma_lds(dims_) <= (size_t)maxLdsPerBlock_;
+    const float* pack = nullptr;
+    if (mfmaQ) {
+        const size_t per = packFloats();
+        std::vector<PackJob> pj(2 * (size_t)count);
+        for (int o = 0; o < count; ++o) {
+            PreJob& J = hPre_[o];
+            J.pkSib = (int)(2 * o * per);
+            J.pkSelf = (int)((2 * o + 1) * per);
+            pj[2 * o] = PackJob{J.moSib, J.pkSib, J.ss ? PACK_ROWS : PACK_OPERAND};
+            pj[2 * o + 1] = PackJob{J.moSelf, J.pkSelf, J.sself ? PACK_ROWS : PACK_OPERAND};
+        }
+        float* pk = static_cast<float*>(
+            grow(reinterpret_cast<void*&>(dPack_), packCap_, pj.size() * per * sizeof(float)));
+        auto dpj = reinterpret_cast<const PackJob*>(upload(pj.data(), pj.size() * sizeof(PackJob)));
+        launch_pack_matrices<float>(dpj, (int)pj.size(), S, C, dMatrices_, pk, stream_);
+        pack = pk;
+    }
     // one upload: jobs | V, V^-1, eigenvalues | tau
     const size_t S2 = (size_t)S * S;
     const size_t jobBytes = (size_t)count * sizeof(PreJob);
@@ -2106,7 +2130,11 @@ int Instance::rateMatrixGradientGeneric(int e, const int* ops, int count, const 
     // work: coef [count][Ppad] | X chunks [count][C][chunks][S^2] |
     // z [S^2][count C] | root-frequency block sums [S][nb] | sums [S^2 + S]
     const int ntiles = dims_.Ppad / 32;
-    const int chunks = std::max(1, std::min(ntiles, (1024 + count * C - 1) / (count * C)));
+    // chunks of tiles per branch (and category): enough workgroups to fill
+    // the device, X partial sums of at most a few thousand patterns each
+    const int chunks = mfmaQ ? std::max(1, std::min(ntiles, std::max((2048 + count - 1) / count,
+                                                                     (ntiles + 39) / 40)))
+                             : std::max(1, std::min(ntiles, (1024 + count * C - 1) / (count * C)));
     const int nb = (dims_.P + 255) / 256;
     const size_t nCoef = (size_t)count * dims_.Ppad, nX = (size_t)count * C * chunks * S2,
                  nZ = S2 * count * C, nF = (size_t)S * nb;
@@ -2121,7 +2149,10 @@ int Instance::rateMatrixGradientGeneric(int e, const int* ops, int count, const 
     const PreJob* dj = reinterpret_cast<const PreJob*>(dblob);
     const double* deig = reinterpret_cast<const double*>(dblob + eOff);
     const double* dtau = reinterpret_cast<const double*>(dblob + tOff);
-    if (dp_)
+    if (mfmaQ)
+        launch_qgrad_mfma(dj, count, dims_, pack, cw, dPatternWeights_, deig, dtau, xs, chunks, z,
+                          stream_);
+    else if (dp_)
         launch_qgrad_generic<double>(dj, count, dims_, cw, dPatternWeights_, deig, dtau, coef, xs,
                                      chunks, z, stream_);
     else

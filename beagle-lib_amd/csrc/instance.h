@@ -359,6 +359,7 @@ private:
     size_t stageCap_ = 0;             // doubles
     void* dJobMats_ = nullptr;        // grow-only gathered matrices of node jobs
     size_t jobMatsCap_ = 0;
+    int maxLdsPerBlock_ = 0;          // the device's LDS limit per workgroup (bytes)
     void* dSites_ = nullptr;          // grow-only per-site derivative ratios (MFMA path)
     size_t sitesCap_ = 0;
     float* dPack_ = nullptr;          // grow-only MFMA-packed matrices of a traversal

@@ -333,6 +333,18 @@ template <typename R>
 void launch_qgrad_generic(const PreJob* djobs, int nops, const Dims& d, const double* cw,
                           const double* pw, const double* eig, const double* tau, double* coef,
                           double* xs, int chunks, double* z, hipStream_t st);
+// The chunks of X summed and transformed: z = (V^T X V^-T) o F (k_qgx_eigen).
+void launch_qgx_eigen(const double* xs, int chunks, int nops, int S, int C, const double* eig,
+                      const double* tau, double* z, hipStream_t st);
+// The same gradient for S = 5...64 with FP32 partials on the matrix cores:
+// the jobs carry packed P_sib (pkSib) and P (pkSelf), each PACK_OPERAND or,
+// for a tip-state operand, PACK_ROWS; X chunks as in launch_qgrad_generic
+// (no coef work), then launch_qgx_eigen.  qgrad_mfma_lds: the LDS a
+// workgroup needs (C waves of one branch).
+size_t qgrad_mfma_lds(const Dims& d);
+void launch_qgrad_mfma(const PreJob* djobs, int nops, const Dims& d, const float* pack,
+                       const double* cw, const double* pw, const double* eig,
+                       const double* tau, double* xs, int chunks, double* z, hipStream_t st);
 // Several roots, one likelihood per pattern: site[p], block sums of
 // w_p site[p] in bsum (ceil(P / WG) of them).
 template <typename R>

@@ -1200,6 +1200,12 @@ __global__ __launch_bounds__(WG) void k_qgx_eigen(const double* __restrict__ xs,
     }
 }
 
+void launch_qgx_eigen(const double* xs, int chunks, int nops, int S, int C, const double* eig,
+                      const double* tau, double* z, hipStream_t st) {
+    hipLaunchKernelGGL(k_qgx_eigen, dim3(C, nops), dim3(WG), (size_t)S * S * sizeof(double), st, xs,
+                       chunks, nops, S, C, eig, tau, z);
+}
+
 template <typename R>
 void launch_qgrad_generic(const PreJob* djobs, int nops, const Dims& d, const double* cw,
                           const double* pw, const double* eig, const double* tau, double* coef,
@@ -1211,8 +1217,7 @@ void launch_qgrad_generic(const PreJob* djobs, int nops, const Dims& d, const do
     const int per = (ntiles + chunks - 1) / chunks;
     hipLaunchKernelGGL(k_qgx_outer<R>, dim3(chunks, d.C, nops), dim3(WG), qgx_lds_bytes<R>(d.S, false), st,
                        djobs, d, cw, coef, per, xs);
-    hipLaunchKernelGGL(k_qgx_eigen, dim3(d.C, nops), dim3(WG), (size_t)d.S * d.S * sizeof(double),
-                       st, xs, chunks, nops, d.S, d.C, eig, tau, z);
+    launch_qgx_eigen(xs, chunks, nops, d.S, d.C, eig, tau, z, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -697,6 +697,190 @@ void launch_deriv_mfma(const DerivJob* djobs, int njobs, int nd, const Dims& d,
 }
 
 // ---------------------------------------------------------------------------
+// Rate-matrix gradient (S = 5...64, FP32 partials) on the matrix cores: the
+// X_c = sum_p (w_c w_p / L_p) top_p x_p^T of one branch (a pre-order
+// operation with its own post-order operand attached) over a chunk of
+// 32-pattern tiles, one wave per category (BeagleCPUImpl.hpp:2122-2237
+// holds the cross products these sums make exact).  Per tile:
+//   y   = P_sib x_sib, Px = P x           (one stream of up to two children,
+//                                          or a tip's column)
+//   top = pre(parent) * y, L_c = sum_i top_i (P x)_i   (accumulator layout;
+//                                          the lane pair h = 0, 1 adds)
+//   L   = sum_c w_c L_c                   (LDS, double-buffered by tile)
+//   X_c += (coef top) x^T                 (v_mfma_f32_32x32x2f32, k = two
+//                                          patterns per instruction)
+// For the last product the scaled top vectors and the own operand x (its
+// partials, or the one-hot of a tip state; a missing state's zero row, as
+// its pad column does not depend on Q) are staged in the wave's LDS block
+// as [state][pattern]: lane l then reads A[i][k] = T[32 ib + l % 32][k0 + l / 32]
+// and B[k][j] = X[32 jb + l % 32][k0 + l / 32].  X_c is summed in FP32 over
+// the chunk (every term is non-negative) and written as FP64 to
+// out[o][c][chunk][S^2]; the chunks are added in FP64 by k_qgx_eigen.  At
+// most 4 categories (one wave per SIMD, so the accumulators may take the
+// AGPRs beside 256 VGPRs).
+constexpr int QGM_LD = PTILE + 1;   // row stride of the staged tiles (floats)
+
+__host__ __device__ constexpr size_t qgm_lds_bytes(int MB, int C) {
+    return ((size_t)C * 2 * MB * 32 * QGM_LD + 2 * 4 * 32) * sizeof(float);
+}
+
+template <int MB, int KG>
+__global__ __launch_bounds__(256) void k_qgm(const PreJob* __restrict__ jobs, Dims d,
+                                             const float* __restrict__ pack,
+                                             const double* __restrict__ cw,
+                                             const double* __restrict__ pw, int tilesPerChunk,
+                                             double* __restrict__ out) {
+    extern __shared__ float qsm[];
+    constexpr int RW = MB * 32;   // staged rows
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int l = threadIdx.x & 63, n = l & 31, h = l >> 5;
+    const int S = d.S, C = d.C;
+    const int o = blockIdx.y;
+    const PreJob& J = jobs[o];
+    float* T = qsm + (size_t)c * 2 * RW * QGM_LD;
+    float* X = T + RW * QGM_LD;
+    float* red = qsm + (size_t)C * 2 * RW * QGM_LD;   // [2 tiles][4 categories][32]
+    const long per = mfma_pack_reals(S);
+    const float* psib = pack + J.pkSib + c * per;
+    const float* pself = pack + J.pkSelf + c * per;
+    const double wc = cw[c];
+    floatx16 acc[MB][MB];
+#pragma unroll
+    for (int ib = 0; ib < MB; ++ib)
+#pragma unroll
+        for (int jb = 0; jb < MB; ++jb) acc[ib][jb] = floatx16{0};
+    const int ntiles = d.Ppad / PTILE;
+    const int t0 = blockIdx.x * tilesPerChunk, t1 = min(ntiles, t0 + tilesPerChunk);
+    for (int tile = t0; tile < t1; ++tile) {
+        const int tp0 = tile * PTILE, p = tp0 + n;
+        const long tbase = pbase(d, c, tp0);
+        floatx16 ys[MB], yo[MB];
+        {
+            floatx16 y[2][MB];
+            const GLOBAL float* xs = gin<float>(J.xs) + tbase + n;
+            const GLOBAL float* xo = gin<float>(J.xself) + tbase + n;
+            const GLOBAL floatx4* as = (const GLOBAL floatx4*)gin<float>(psib) + l;
+            const GLOBAL floatx4* ao = (const GLOBAL floatx4*)gin<float>(pself) + l;
+            if (!J.ss && !J.sself) {
+                stream_mfma<MB, KG, 2, false>(xs, as, xo, ao, h, S, y);
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
+                    ys[mb] = y[0][mb];
+                    yo[mb] = y[1][mb];
+                }
+            } else if (!J.ss) {
+                stream_mfma<MB, KG, 1, false>(xs, as, xs, as, h, S, y);
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) ys[mb] = y[0][mb];
+                state_rows<MB>(pself, J.sself, p, h, yo);
+            } else if (!J.sself) {
+                stream_mfma<MB, KG, 1, false>(xo, ao, xo, ao, h, S, y);
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) yo[mb] = y[0][mb];
+                state_rows<MB>(psib, J.ss, p, h, ys);
+            } else {
+                state_rows<MB>(psib, J.ss, p, h, ys);
+                state_rows<MB>(pself, J.sself, p, h, yo);
+            }
+        }
+        floatx16 par[MB];
+        load_acc<MB>(J.parent, tbase, n, h, S, par);
+        float lc = 0.f;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+            ys[mb] *= par[mb];   // top
+#pragma unroll
+            for (int r = 0; r < 16; ++r) lc += ys[mb][r] * yo[mb][r];
+        }
+        lc += __shfl_xor(lc, 32);
+        float* rb = red + (tile & 1) * 4 * 32;
+        if (h == 0) rb[c * 32 + n] = lc;
+        __syncthreads();
+        double L = 0.0;
+        for (int cc = 0; cc < C; ++cc) L += cw[cc] * (double)rb[cc * 32 + n];
+        const float coef = (p < d.P && L != 0.0) ? (float)(pw[p] * wc / L) : 0.f;
+        // stage coef top and x as [state][pattern]; the wave's previous
+        // tile's reads of its block ended before this tile's barrier
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                T[(mb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)) * QGM_LD + n] = coef * ys[mb][r];
+        if (J.sself) {
+            const int s = gin<int>(J.sself)[p];
+#pragma unroll
+            for (int u = 0; u < RW / 2; ++u) {
+                const int j = 2 * u + h;
+                X[j * QGM_LD + n] = s == j ? 1.f : 0.f;
+            }
+        } else {
+            const GLOBAL float* xo = gin<float>(J.xself) + tbase + n;
+#pragma unroll
+            for (int u = 0; u < RW / 2; ++u) {
+                const int j = 2 * u + h;
+                X[j * QGM_LD + n] = j < S ? xo[j * PTILE] : 0.f;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+#pragma unroll
+        for (int kk = 0; kk < PTILE / 2; ++kk) {
+            float a[MB], b[MB];
+#pragma unroll
+            for (int ib = 0; ib < MB; ++ib) {
+                a[ib] = T[(ib * 32 + n) * QGM_LD + 2 * kk + h];
+                b[ib] = X[(ib * 32 + n) * QGM_LD + 2 * kk + h];
+            }
+#pragma unroll
+            for (int ib = 0; ib < MB; ++ib)
+#pragma unroll
+                for (int jb = 0; jb < MB; ++jb)
+                    acc[ib][jb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[ib], b[jb], acc[ib][jb],
+                                                                       0, 0, 0);
+        }
+        // the block's reads done before the next tile writes it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    }
+    double* dst = out + (((long)o * C + c) * gridDim.x + blockIdx.x) * S * S;
+#pragma unroll
+    for (int ib = 0; ib < MB; ++ib)
+#pragma unroll
+        for (int jb = 0; jb < MB; ++jb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = ib * 32 + 8 * (r >> 2) + 4 * h + (r & 3), j = jb * 32 + n;
+                if (i < S && j < S) dst[i * S + j] = (double)acc[ib][jb][r];
+            }
+}
+
+size_t qgrad_mfma_lds(const Dims& d) { return qgm_lds_bytes(mfma_mblocks(d.S), d.C); }
+
+void launch_qgrad_mfma(const PreJob* djobs, int nops, const Dims& d, const float* pack,
+                       const double* cw, const double* pw, const double* eig,
+                       const double* tau, double* xs, int chunks, double* z, hipStream_t st) {
+    if (nops <= 0) return;
+    const int ntiles = d.Ppad / PTILE;
+    const int per = (ntiles + chunks - 1) / chunks;
+    const dim3 grid(chunks, nops), block(64 * d.C);
+    const size_t lds = qgrad_mfma_lds(d);
+    const int KG = mfma_kgroups(d.S);
+#define QGM(M, K)                                                                              \
+    case K:                                                                                    \
+        hipLaunchKernelGGL((k_qgm<M, K>), grid, block, lds, st, djobs, d, pack, cw, pw, per, xs); \
+        break;
+    switch (KG) {
+        QGM(1, 1) QGM(1, 2) QGM(1, 3) QGM(1, 4)
+        QGM(2, 5) QGM(2, 6) QGM(2, 7) QGM(2, 8)
+        default: break;
+    }
+#undef QGM
+    launch_qgx_eigen(xs, chunks, nops, d.S, d.C, eig, tau, z, st);
+}
+
+// ---------------------------------------------------------------------------
 // FP64: v_mfma_f64_16x16x4f64.  Lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16]; element r of its accumulator is D[4 r + l / 16][l % 16]
 // (rows interleave across the four lane groups).  A wavefront owns one

@@ -874,6 +874,29 @@ def test_rate_matrix_gradient_generic_against_oracle_finite_differences(amd, ora
         assert abs(gpi[i] - fd) <= tol * max(abs(fd), np.abs(gpi).max()), (i, gpi[i], fd)
 
 
+@pytest.mark.parametrize("S,C,tips_as", [(20, 4, "states"), (61, 2, "partials"),
+                                         (33, 3, "states"), (5, 1, "states"),
+                                         (64, 4, "partials")])
+def test_rate_matrix_gradient_matrix_cores_against_fp64(amd, S, C, tips_as):
+    """The rate-matrix gradient of an FP32 instance with 5-64 states and at
+    most 4 categories (the matrix-core kernel k_qgm) against the FP64
+    instance's (vector units) on the same problem: dlogL/dQ within 1e-6 of
+    its largest entry, dlogL/dpi_root likewise; state counts on both sides
+    of a 32-state block, tips as states (with missing ones) and as
+    partials, pattern counts that leave a partial last tile."""
+    prob = synth.make_problem(400 + S + C, 24, 1900, S=S, C=C, shape="yule", missing=0.05,
+                              mean_length=0.2)
+    res = []
+    for precision, scaling in (("single", "auto"), ("double", "none")):
+        ev = driver.Evaluator(amd, prob, scaling=scaling, precision=precision, tips_as=tips_as)
+        ev.fused_gradient(keep_tips=False)
+        res.append(ev.rate_matrix_gradient())
+        ev.close()
+    (g32, p32), (g64, p64) = res
+    assert np.abs(g32 - g64).max() <= 1e-6 * np.abs(g64).max()
+    assert np.abs(p32 - p64).max() <= 1e-6 * np.abs(p64).max()
+
+
 @pytest.mark.parametrize("precision,S", [("double", 4), ("single", 4), ("double", 20)])
 def test_torch_autograd_binding(amd, oracle, precision, S):
     """beagle_amd.torch_fn: logL of (branch lengths, GTR(exch, freqs)) and
