@@ -2135,7 +2135,7 @@ int Instance::rateMatrixGradientGeneric(int e, const int* ops, int count, const 
     const int chunks = mfmaQ ? std::max(1, std::min(ntiles, std::max((2048 + count - 1) / count,
                                                                      (ntiles + 39) / 40)))
                              : std::max(1, std::min(ntiles, (1024 + count * C - 1) / (count * C)));
-    const int nb = (dims_.P + 255) / 256;
+    const int nb = root_freq_wide_blocks(dims_);
     const size_t nCoef = (size_t)count * dims_.Ppad, nX = (size_t)count * C * chunks * S2,
                  nZ = S2 * count * C, nF = (size_t)S * nb;
     double* wk = reinterpret_cast<double*>(
@@ -2160,8 +2160,8 @@ int Instance::rateMatrixGradientGeneric(int e, const int* ops, int count, const 
                                     chunks, z, stream_);
     launch_reduce_rows(z, count * C, (int)S2, sums, stream_);
     if (freqs) {
-        if (dp_) launch_root_freq_grad<double>(dPartials_[rootIdx], cw, dFreqs_ + (size_t)fIdx * S, dPatternWeights_, fb, dims_, stream_);
-        else launch_root_freq_grad<float>(dPartials_[rootIdx], cw, dFreqs_ + (size_t)fIdx * S, dPatternWeights_, fb, dims_, stream_);
+        if (dp_) launch_root_freq_grad_wide<double>(dPartials_[rootIdx], cw, dFreqs_ + (size_t)fIdx * S, dPatternWeights_, fb, dims_, stream_);
+        else launch_root_freq_grad_wide<float>(dPartials_[rootIdx], cw, dFreqs_ + (size_t)fIdx * S, dPatternWeights_, fb, dims_, stream_);
         launch_reduce_rows(fb, nb, S, sums + S2, stream_);
     }
     HIPCHECK(hipGetLastError());

@@ -358,6 +358,11 @@ void launch_root_per_category(const void* root, const double* freqs, const doubl
 template <typename R>
 void launch_root_freq_grad(const void* root, const double* cw, const double* freqs,
                            const double* pw, double* partial, const Dims& d, hipStream_t st);
+// The same for many states, rows i of [S][root_freq_wide_blocks(d)].
+int root_freq_wide_blocks(const Dims& d);
+template <typename R>
+void launch_root_freq_grad_wide(const void* root, const double* cw, const double* freqs,
+                                const double* pw, double* partial, const Dims& d, hipStream_t st);
 // Copies each node job's six matrices (P_a, P_b, D1_a, D2_a, D1_b, D2_b) into
 // jobMats as [job][category][6][20] reals, so a job's matrices for one
 // category are contiguous; job j's block starts at djobs[j].mbase.  Each

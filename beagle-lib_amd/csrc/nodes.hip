@@ -1589,6 +1589,45 @@ void launch_root_freq_grad(const void* root, const double* cw, const double* fre
                        reinterpret_cast<const R*>(root), cw, freqs, pw, partial, d);
 }
 
+// The same sums for many states: one wave per 64 patterns and group of 16
+// states, grid (ceil(P / 64), ceil(S / 16)); every lane forms its pattern's
+// likelihood, then each of the group's states is summed over the wave by
+// DPP steps (no workgroup barrier per state).  Rows i of [S][ceil(P / 64)].
+template <typename R>
+__global__ __launch_bounds__(64) void k_root_freq_grad_wide(const R* __restrict__ root,
+                                                            const double* __restrict__ cw,
+                                                            const double* __restrict__ freqs,
+                                                            const double* __restrict__ pw,
+                                                            double* __restrict__ partial, Dims d) {
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    const bool in = p < d.P;
+    const int q = in ? p : 0;   // lanes past the patterns read pattern 0 and add 0
+    double L = 0.0;
+    for (int i = 0; i < d.S; ++i) {
+        double g = 0.0;
+        for (int c = 0; c < d.C; ++c) g += cw[c] * (double)root[pbase(d, c, q) + (long)i * d.sStride];
+        L += freqs[i] * g;
+    }
+    const int i1 = min(d.S, (int)(blockIdx.y + 1) * 16);
+    for (int i = blockIdx.y * 16; i < i1; ++i) {
+        double g = 0.0;
+        for (int c = 0; c < d.C; ++c) g += cw[c] * (double)root[pbase(d, c, q) + (long)i * d.sStride];
+        const double s = wave_sum_dpp(in ? pw[q] * g / L : 0.0);
+        if (threadIdx.x == 0) partial[(long)i * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+int root_freq_wide_blocks(const Dims& d) { return (d.P + 63) / 64; }
+
+template <typename R>
+void launch_root_freq_grad_wide(const void* root, const double* cw, const double* freqs,
+                                const double* pw, double* partial, const Dims& d,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(k_root_freq_grad_wide<R>, dim3(root_freq_wide_blocks(d), (d.S + 15) / 16),
+                       dim3(64), 0, st, reinterpret_cast<const R*>(root), cw, freqs, pw, partial,
+                       d);
+}
+
 template <typename R>
 __global__ __launch_bounds__(WG) void k_gather_node_mats(const NodeJob* __restrict__ jobs,
                                                          int njobs, int C,
@@ -1684,7 +1723,9 @@ void launch_pre_nodes(const NodeJob* djobs, int njobs, bool batched, int chunk, 
                                   const double*, const double*, void*, const double*,           \
                                   const double*, double*, hipStream_t);                         \
     template void launch_root_freq_grad<R>(const void*, const double*, const double*,          \
-                                           const double*, double*, const Dims&, hipStream_t);   \
+                                           const double*, double*, const Dims&, hipStream_t);     \
+    template void launch_root_freq_grad_wide<R>(const void*, const double*, const double*,     \
+                                                const double*, double*, const Dims&, hipStream_t);   \
     template void launch_pre_nodes<R>(const NodeJob*, int, bool, int, const Dims&, const void*, \
                                       void*, const double*, const double*, double*,             \
                                       hipStream_t, NodeQGrad*);
