@@ -14,8 +14,8 @@
 // accumulator of M block mb, output state 32 mb + 8 (r / 4) + 4 (l / 32) +
 // r % 4.  Inner states are assigned to instructions as k = 2 step + l / 32;
 // the repacked matrix follows the same assignment, padding with zeros
-// (states >= S), so a partials read past the last state is clamped to the
-// last state and multiplied by zero.  A tip-state child instead reads its
+// (states >= S), and a partials read past the last state falls outside the
+// tile's buffer resource and reads 0.  A tip-state child instead reads its
 // column P[., s] from the matrix repacked as rows by state (row S is the pad
 // column, 1 for P(t)).
 //
@@ -39,6 +39,51 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
     hi = (__bf16)x;
     lo = (__bf16)(x - (float)hi);
+}
+
+// ---------------------------------------------------------------------------
+// Operand access through buffer resources.  Base and size are wave-uniform
+// (job fields, the category, the tile), so a resource sits in scalar
+// registers and every load is one instruction with a per-lane offset fixed
+// for the kernel and a compile-time immediate: no 64-bit address arithmetic
+// per load.  A load past a resource's size reads 0 and a store past it is
+// dropped, which gives the padded states (rows >= S) for free.
+
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+__device__ __forceinline__ Rsrc rsrc(const void* base, int bytes) {
+    const unsigned long long a = reinterpret_cast<unsigned long long>(base);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    void* b = reinterpret_cast<void*>(((unsigned long long)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(b, (short)0, __builtin_amdgcn_readfirstlane(bytes),
+                                             0x00020000);
+}
+
+// One category's S x PTILE tile of a partials buffer as two resources of up
+// to 32 rows (states), so that every row offset within a resource fits the
+// 12-bit immediate of the instruction: rows >= S fall outside both.
+struct Tile {
+    Rsrc r[2];
+};
+__device__ __forceinline__ Tile tile_of(const void* buf, long tbase, int S) {
+    const float* b = (const float*)buf + tbase;
+    return Tile{{rsrc(b, min(S, 32) * PTILE * 4), rsrc(b + 32 * PTILE, max(S - 32, 0) * PTILE * 4)}};
+}
+// element (row, lane's pattern) of a tile; `vo` holds the lane's byte offset
+// (pattern n, plus the lane's row offset), `row` is compile-time
+__device__ __forceinline__ float tile_load(const Tile& t, int row, unsigned vo) {
+    return __uint_as_float(
+        __builtin_amdgcn_raw_buffer_load_b32(t.r[row >> 5], vo + (row & 31) * PTILE * 4, 0, 0));
+}
+__device__ __forceinline__ void tile_store(const Tile& t, int row, unsigned vo, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), t.r[row >> 5],
+                                          vo + (row & 31) * PTILE * 4, 0, 0);
+}
+// four floats at byte offset vo + off of a packed-matrix resource (`off`
+// compile-time, placed in the scalar offset)
+__device__ __forceinline__ floatx4 pack_load4(Rsrc r, unsigned vo, int off) {
+    return __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, off, 0));
 }
 
 // ---------------------------------------------------------------------------
@@ -129,17 +174,16 @@ void launch_pack_matrices(const PackJob* jobs, int njobs, int S, int C, const vo
 // post-order
 
 // Operands of one pipeline stage, k group g of one partials child: the
-// matrix (A) and the partials (B).
+// matrix (A; lane offset va = 16 l) and the partials (B; lane offset
+// vx = 4 l, which is row h and pattern n of the tile).
 template <int MB>
-__device__ __forceinline__ void load_a(const GLOBAL floatx4* a, int g, int KG,
-                                       floatx4 (&am)[MB]) {
+__device__ __forceinline__ void load_a(Rsrc a, unsigned va, int g, int KG, floatx4 (&am)[MB]) {
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) am[mb] = a[(mb * KG + g) * 64];
+    for (int mb = 0; mb < MB; ++mb) am[mb] = pack_load4(a, va, (mb * KG + g) * 1024);
 }
-__device__ __forceinline__ void load_x(const GLOBAL float* xb, int g, int h, int S,
-                                       float (&b)[4]) {
+__device__ __forceinline__ void load_x(const Tile& x, unsigned vx, int g, float (&b)[4]) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) b[q] = xb[min(8 * g + 2 * q + h, S - 1) * PTILE];
+    for (int q = 0; q < 4; ++q) b[q] = tile_load(x, 8 * g + 2 * q, vx);
 }
 
 template <int MB>
@@ -155,19 +199,19 @@ __device__ __forceinline__ void mfma_stage(floatx16 (&y)[MB], const floatx4 (&am
 // Split-BF16 stages: k step t (16 inner states) of one partials child.
 // A (v_mfma_f32_32x32x16_bf16): lane l holds P[32 mb + l % 32][16 t + 8 (l / 32) + e]
 // as packed hi and lo parts; B: X[16 t + 8 (l / 32) + e][pattern l % 32].
+// lane offsets: va = 16 l, vx = 1024 h + 4 n (row 8 h, pattern n)
 template <int MB>
-__device__ __forceinline__ void load_a16(const GLOBAL u32x4* a, int t, int KT, u32x4 (&ah)[MB],
+__device__ __forceinline__ void load_a16(Rsrc a, unsigned va, int t, int KT, u32x4 (&ah)[MB],
                                          u32x4 (&al)[MB]) {
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
-        ah[mb] = a[((mb * KT + t) * 2 + 0) * 64];
-        al[mb] = a[((mb * KT + t) * 2 + 1) * 64];
+        ah[mb] = __builtin_amdgcn_raw_buffer_load_b128(a, va, ((mb * KT + t) * 2 + 0) * 1024, 0);
+        al[mb] = __builtin_amdgcn_raw_buffer_load_b128(a, va, ((mb * KT + t) * 2 + 1) * 1024, 0);
     }
 }
-__device__ __forceinline__ void load_x16(const GLOBAL float* xb, int t, int h, int S,
-                                         float (&b)[8]) {
+__device__ __forceinline__ void load_x16(const Tile& x, unsigned vx, int t, float (&b)[8]) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) b[e] = xb[min(16 * t + 8 * h + e, S - 1) * PTILE];
+    for (int e = 0; e < 8; ++e) b[e] = tile_load(x, 16 * t + e, vx);
 }
 // NT = 3: Pl Xh + Ph Xl + Ph Xh (small terms first); NT = 4 adds Pl Xl.
 template <int MB, int NT>
@@ -195,9 +239,8 @@ __device__ __forceinline__ void mfma16_stage(floatx16 (&y)[MB], const u32x4 (&ah
 // Split-BF16 stream of NA partials children, KT stages each, operands PF
 // stages ahead (as stream_mfma).
 template <int MB, int KT, int NA, int NT>
-__device__ __forceinline__ void stream_bf16(const GLOBAL float* x0, const GLOBAL u32x4* a0,
-                                            const GLOBAL float* x1, const GLOBAL u32x4* a1,
-                                            int h, int S, floatx16 (&y)[2][MB]) {
+__device__ __forceinline__ void stream_bf16(const Tile& x0, Rsrc a0, const Tile& x1, Rsrc a1,
+                                            unsigned vx, unsigned va, floatx16 (&y)[2][MB]) {
     constexpr int PF = 2, T = NA * KT;
 #pragma unroll
     for (int k = 0; k < NA; ++k)
@@ -207,8 +250,8 @@ __device__ __forceinline__ void stream_bf16(const GLOBAL float* x0, const GLOBAL
     float b[PF][8];
 #pragma unroll
     for (int j = 0; j < PF && j < T; ++j) {
-        load_a16<MB>(j < KT ? a0 : a1, j % KT, KT, ah[j], al[j]);
-        load_x16(j < KT ? x0 : x1, j % KT, h, S, b[j]);
+        load_a16<MB>(j < KT ? a0 : a1, va, j % KT, KT, ah[j], al[j]);
+        load_x16(j < KT ? x0 : x1, vx, j % KT, b[j]);
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -216,8 +259,8 @@ __device__ __forceinline__ void stream_bf16(const GLOBAL float* x0, const GLOBAL
         mfma16_stage<MB, NT>(y[j / KT], ah[j % PF], al[j % PF], b[j % PF]);
         if (j + PF < T) {
             const int t = (j + PF) % KT;
-            load_a16<MB>(j + PF < KT ? a0 : a1, t, KT, ah[j % PF], al[j % PF]);
-            load_x16(j + PF < KT ? x0 : x1, t, h, S, b[j % PF]);
+            load_a16<MB>(j + PF < KT ? a0 : a1, va, t, KT, ah[j % PF], al[j % PF]);
+            load_x16(j + PF < KT ? x0 : x1, vx, t, b[j % PF]);
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -248,9 +291,8 @@ __device__ __forceinline__ void state_rows(const float* pk, const int* states, i
 // ahead.  Straight-line code, so the wait before each stage covers only
 // what that stage reads.
 template <int MB, int KG, int NA, bool XU>
-__device__ __forceinline__ void stream_mfma(const GLOBAL float* x0, const GLOBAL floatx4* a0,
-                                            const GLOBAL float* x1, const GLOBAL floatx4* a1,
-                                            int h, int S, floatx16 (&y)[2][MB]) {
+__device__ __forceinline__ void stream_mfma(const Tile& x0, Rsrc a0, const Tile& x1, Rsrc a1,
+                                            unsigned vx, unsigned va, floatx16 (&y)[2][MB]) {
     constexpr int PF = 3, T = NA * KG;
     constexpr int XB = XU ? T : PF;     // partials buffers
 #pragma unroll
@@ -261,12 +303,12 @@ __device__ __forceinline__ void stream_mfma(const GLOBAL float* x0, const GLOBAL
     float b[XB][4];
 #pragma unroll
     for (int j = 0; j < PF && j < T; ++j) {
-        load_a<MB>(j < KG ? a0 : a1, j % KG, KG, am[j]);
-        if (!XU) load_x(j < KG ? x0 : x1, j % KG, h, S, b[j]);
+        load_a<MB>(j < KG ? a0 : a1, va, j % KG, KG, am[j]);
+        if (!XU) load_x(j < KG ? x0 : x1, vx, j % KG, b[j]);
     }
     if (XU) {
 #pragma unroll
-        for (int j = 0; j < T; ++j) load_x(j < KG ? x0 : x1, j % KG, h, S, b[j % XB]);
+        for (int j = 0; j < T; ++j) load_x(j < KG ? x0 : x1, vx, j % KG, b[j % XB]);
     }
     // keep the stage order: the scheduler would otherwise regroup the loads
     // and wait for all of them before the first instruction
@@ -276,8 +318,8 @@ __device__ __forceinline__ void stream_mfma(const GLOBAL float* x0, const GLOBAL
         mfma_stage<MB>(y[j / KG], am[j % PF], b[j % XB]);
         if (j + PF < T) {
             const int g = (j + PF) % KG;
-            load_a<MB>(j + PF < KG ? a0 : a1, g, KG, am[j % PF]);
-            if (!XU) load_x(j + PF < KG ? x0 : x1, g, h, S, b[j % XB]);
+            load_a<MB>(j + PF < KG ? a0 : a1, va, g, KG, am[j % PF]);
+            if (!XU) load_x(j + PF < KG ? x0 : x1, vx, g, b[j % XB]);
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -357,14 +399,14 @@ __device__ __forceinline__ void rescale_store(floatx16 (&v)[MB], const ScaleOut&
             }
         }
     }
-    GLOBAL float* out = gout<float>(J.dest) + tbase + n;
+    // rows >= S fall outside the tile's resources; a lane past the
+    // operation's patterns stores out of range
+    const Tile out = tile_of(J.dest, tbase, S);
+    const unsigned vo = in ? 4 * n + 512 * h : 0x40000000u;
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int i = mb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-            if (i < S && in) out[i * PTILE] = v[mb][r];
-        }
+        for (int r = 0; r < 16; ++r) tile_store(out, mb * 32 + 8 * (r >> 2) + (r & 3), vo, v[mb][r]);
 }
 
 // MODE 0: FP32 matrix cores, partials pipelined; 1: FP32, whole partials
@@ -386,24 +428,18 @@ __global__ __launch_bounds__(512) void k_postm(const PostJob* __restrict__ jobs,
     const long tbase = pbase(d, c, tp0);
     // the partials children in list order: stream 0 is child 1 unless that
     // child is tip states, stream 1 is child 2 when both are partials
-    const GLOBAL float* x1 = gin<float>(J.x1) + tbase + n;
-    const GLOBAL float* x2 = gin<float>(J.x2) + tbase + n;
-    const GLOBAL float* p1 = gin<float>(pack + J.pk1 + c * per);
-    const GLOBAL float* p2 = gin<float>(pack + J.pk2 + c * per);
-    const GLOBAL float* xs0 = J.s1 ? x2 : x1;
-    const GLOBAL float* ps0 = J.s1 ? p2 : p1;
+    const Tile x0 = tile_of(J.s1 ? J.x2 : J.x1, tbase, S), x2 = tile_of(J.x2, tbase, S);
+    const Rsrc a0 = rsrc(pack + (J.s1 ? J.pk2 : J.pk1) + c * per, -1);
+    const Rsrc a2 = rsrc(pack + J.pk2 + c * per, -1);
     floatx16 y[2][MB];
     const int na = (J.s1 ? 0 : 1) + (J.s2 ? 0 : 1);
     if (MODE <= 1) {
-        const GLOBAL floatx4* a0 = (const GLOBAL floatx4*)ps0 + l;
-        const GLOBAL floatx4* a2 = (const GLOBAL floatx4*)p2 + l;
-        if (na == 2) stream_mfma<MB, KX, 2, MODE == 1>(xs0, a0, x2, a2, h, S, y);
-        else if (na == 1) stream_mfma<MB, KX, 1, MODE == 1>(xs0, a0, x2, a2, h, S, y);
+        if (na == 2) stream_mfma<MB, KX, 2, MODE == 1>(x0, a0, x2, a2, 4 * l, 16 * l, y);
+        else if (na == 1) stream_mfma<MB, KX, 1, MODE == 1>(x0, a0, x2, a2, 4 * l, 16 * l, y);
     } else {
-        const GLOBAL u32x4* a0 = (const GLOBAL u32x4*)ps0 + l;
-        const GLOBAL u32x4* a2 = (const GLOBAL u32x4*)p2 + l;
-        if (na == 2) stream_bf16<MB, KX, 2, MODE>(xs0, a0, x2, a2, h, S, y);
-        else if (na == 1) stream_bf16<MB, KX, 1, MODE>(xs0, a0, x2, a2, h, S, y);
+        const unsigned vx = 1024 * h + 4 * n;
+        if (na == 2) stream_bf16<MB, KX, 2, MODE>(x0, a0, x2, a2, vx, 16 * l, y);
+        else if (na == 1) stream_bf16<MB, KX, 1, MODE>(x0, a0, x2, a2, vx, 16 * l, y);
     }
     // tip-state children take the accumulator slots the stream left free
     if (J.s2) state_rows<MB>(pack + J.pk2 + c * per, J.s2, p, h, y[1]);
@@ -476,17 +512,16 @@ void launch_post_bf16(const PostJob* djobs, int njobs, const Dims& d, const floa
 
 // z = M^T u, u in the accumulator layout; A operands loaded PF groups ahead.
 template <int MB>
-__device__ __forceinline__ void gemm_from_acc(const float* pk, int l, const floatx16 (&u)[MB],
+__device__ __forceinline__ void gemm_from_acc(Rsrc a, unsigned va, const floatx16 (&u)[MB],
                                               floatx16 (&z)[MB]) {
     constexpr int NG = MB * 4, PF = 3;
-    const GLOBAL floatx4* a = (const GLOBAL floatx4*)gin<float>(pk) + l;
 #pragma unroll
     for (int mo = 0; mo < MB; ++mo) z[mo] = floatx16{0};
     floatx4 am[PF][MB];
 #pragma unroll
     for (int j = 0; j < PF && j < NG; ++j)
 #pragma unroll
-        for (int mo = 0; mo < MB; ++mo) am[j][mo] = a[(mo * NG + j) * 64];
+        for (int mo = 0; mo < MB; ++mo) am[j][mo] = pack_load4(a, va, (mo * NG + j) * 1024);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
@@ -500,24 +535,21 @@ __device__ __forceinline__ void gemm_from_acc(const float* pk, int l, const floa
         }
         if (g + PF < NG) {
 #pragma unroll
-            for (int mo = 0; mo < MB; ++mo) am[g % PF][mo] = a[(mo * NG + g + PF) * 64];
+            for (int mo = 0; mo < MB; ++mo)
+                am[g % PF][mo] = pack_load4(a, va, (mo * NG + g + PF) * 1024);
         }
         __builtin_amdgcn_sched_barrier(0);
     }
 }
 
-// values of a partials buffer in the accumulator layout (rows >= S read 0)
+// values of a partials tile in the accumulator layout (rows >= S read 0);
+// lane offset vo = 4 n + 512 h (row 4 h, pattern n)
 template <int MB>
-__device__ __forceinline__ void load_acc(const void* x, long tbase, int n, int h, int S,
-                                         floatx16 (&v)[MB]) {
-    const GLOBAL float* xb = gin<float>(x) + tbase + n;
+__device__ __forceinline__ void load_acc(const Tile& t, unsigned vo, floatx16 (&v)[MB]) {
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int i = mb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-            v[mb][r] = i < S ? xb[i * PTILE] : 0.f;
-        }
+        for (int r = 0; r < 16; ++r) v[mb][r] = tile_load(t, mb * 32 + 8 * (r >> 2) + (r & 3), vo);
 }
 
 template <int MB, int KG, bool AUTO>
@@ -535,19 +567,19 @@ __global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, D
     const long per = mfma_pack_reals(S);
     const long tbase = pbase(d, c, tp0);
     floatx16 par[MB];
-    load_acc<MB>(J.parent, tbase, n, h, S, par);
+    load_acc<MB>(tile_of(J.parent, tbase, S), 4 * n + 512 * h, par);
     floatx16 y[2][MB];
     const float* psib = pack + J.pkSib + c * per;
     if (J.ss) {
         state_rows<MB>(psib, J.ss, p, h, y[0]);
     } else {
-        const GLOBAL float* xs = gin<float>(J.xs) + tbase + n;
-        const GLOBAL floatx4* as = (const GLOBAL floatx4*)gin<float>(psib) + l;
-        stream_mfma<MB, KG, 1, false>(xs, as, xs, as, h, S, y);
+        const Tile xs = tile_of(J.xs, tbase, S);
+        const Rsrc as = rsrc(psib, -1);
+        stream_mfma<MB, KG, 1, false>(xs, as, xs, as, 4 * l, 16 * l, y);
     }
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) y[0][mb] *= par[mb];
-    gemm_from_acc<MB>(pack + J.pkSelf + c * per, l, y[0], y[1]);
+    gemm_from_acc<MB>(rsrc(pack + J.pkSelf + c * per, -1), 16 * l, y[0], y[1]);
     const ScaleOut so{J.mode, J.logScalers, nullptr, J.scaleWrite, nullptr, J.accA,
                       J.accB, J.e1, J.e2, J.ed, J.dest};
     rescale_store<MB, AUTO>(y[1], so, tbase, S, d.C, c, n, h, p, in, red);
@@ -612,11 +644,10 @@ __global__ __launch_bounds__(512) void k_derivm(const DerivJob* __restrict__ job
         state_rows<MB>(p1, J.states, p, h, y[0]);
         if (ND == 2) state_rows<MB>(p2, J.states, p, h, y[1]);
     } else {
-        const GLOBAL float* x = gin<float>(J.post) + tbase + n;
-        stream_mfma<MB, KG, ND, false>(x, (const GLOBAL floatx4*)gin<float>(p1) + l, x,
-                                       (const GLOBAL floatx4*)gin<float>(p2) + l, h, S, y);
+        const Tile x = tile_of(J.post, tbase, S);
+        stream_mfma<MB, KG, ND, false>(x, rsrc(p1, -1), x, rsrc(p2, -1), 4 * l, 16 * l, y);
     }
-    load_acc<MB>(J.pre, tbase, n, h, S, z);
+    load_acc<MB>(tile_of(J.pre, tbase, S), 4 * n + 512 * h, z);
     float den = 0.f, n1 = 0.f, n2 = 0.f;
     if (J.states) {
         const int sd = st % S;
@@ -631,7 +662,7 @@ __global__ __launch_bounds__(512) void k_derivm(const DerivJob* __restrict__ job
             }
     } else {
         floatx16 x[MB];
-        load_acc<MB>(J.post, tbase, n, h, S, x);
+        load_acc<MB>(tile_of(J.post, tbase, S), 4 * n + 512 * h, x);
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
@@ -757,24 +788,22 @@ __global__ __launch_bounds__(256) void k_qgm(const PreJob* __restrict__ jobs, Di
         floatx16 ys[MB], yo[MB];
         {
             floatx16 y[2][MB];
-            const GLOBAL float* xs = gin<float>(J.xs) + tbase + n;
-            const GLOBAL float* xo = gin<float>(J.xself) + tbase + n;
-            const GLOBAL floatx4* as = (const GLOBAL floatx4*)gin<float>(psib) + l;
-            const GLOBAL floatx4* ao = (const GLOBAL floatx4*)gin<float>(pself) + l;
+            const Tile xs = tile_of(J.xs, tbase, S), xo = tile_of(J.xself, tbase, S);
+            const Rsrc as = rsrc(psib, -1), ao = rsrc(pself, -1);
             if (!J.ss && !J.sself) {
-                stream_mfma<MB, KG, 2, false>(xs, as, xo, ao, h, S, y);
+                stream_mfma<MB, KG, 2, false>(xs, as, xo, ao, 4 * l, 16 * l, y);
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb) {
                     ys[mb] = y[0][mb];
                     yo[mb] = y[1][mb];
                 }
             } else if (!J.ss) {
-                stream_mfma<MB, KG, 1, false>(xs, as, xs, as, h, S, y);
+                stream_mfma<MB, KG, 1, false>(xs, as, xs, as, 4 * l, 16 * l, y);
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb) ys[mb] = y[0][mb];
                 state_rows<MB>(pself, J.sself, p, h, yo);
             } else if (!J.sself) {
-                stream_mfma<MB, KG, 1, false>(xo, ao, xo, ao, h, S, y);
+                stream_mfma<MB, KG, 1, false>(xo, ao, xo, ao, 4 * l, 16 * l, y);
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb) yo[mb] = y[0][mb];
                 state_rows<MB>(psib, J.ss, p, h, ys);
@@ -784,7 +813,7 @@ __global__ __launch_bounds__(256) void k_qgm(const PreJob* __restrict__ jobs, Di
             }
         }
         floatx16 par[MB];
-        load_acc<MB>(J.parent, tbase, n, h, S, par);
+        load_acc<MB>(tile_of(J.parent, tbase, S), 4 * n + 512 * h, par);
         float lc = 0.f;
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) {
@@ -814,12 +843,9 @@ __global__ __launch_bounds__(256) void k_qgm(const PreJob* __restrict__ jobs, Di
                 X[j * QGM_LD + n] = s == j ? 1.f : 0.f;
             }
         } else {
-            const GLOBAL float* xo = gin<float>(J.xself) + tbase + n;
+            const Tile xo = tile_of(J.xself, tbase, S);
 #pragma unroll
-            for (int u = 0; u < RW / 2; ++u) {
-                const int j = 2 * u + h;
-                X[j * QGM_LD + n] = j < S ? xo[j * PTILE] : 0.f;
-            }
+            for (int u = 0; u < RW / 2; ++u) X[(2 * u + h) * QGM_LD + n] = tile_load(xo, 2 * u, 4 * l);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
