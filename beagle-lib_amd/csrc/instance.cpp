@@ -1288,8 +1288,8 @@ int Instance::buildPreJobs(const int* ops, int count, bool byPartition, bool all
 // Launch the pre-order jobs in hPre_ (built from `ops`): the generic-S
 // path one launch per op; for 4 states a level plan of batched and fused
 // segments (see planSegments).  partial != null adds the derivative sums.
-// S > 4: pre-order and edge derivatives on the FP32 matrix cores (FP32
-// partials, whatever the post-order engine) or the FP64 ones.
+// S > 4: pre-order and edge derivatives on the matrix cores of the
+// instance's engine: FP32, split BF16 (beagleExtSetSplitBF16) or FP64.
 bool Instance::mfmaAdjoint() const { return mfma_post_supported(dims_.S, dims_.C); }
 
 // floats per packed child matrix (all categories) of the MFMA kernels
@@ -1308,8 +1308,10 @@ void Instance::launchPreMfma(const int* ops, int stride, int count) {
         PreJob& J = hPre_[o];
         J.pkSelf = (int)(2 * o * per);
         J.pkSib = (int)((2 * o + 1) * per);
-        pj[2 * o] = PackJob{J.moSelf, J.pkSelf, PACK_PRE_SELF | tin};
-        pj[2 * o + 1] = PackJob{J.moSib, J.pkSib, J.ss ? PACK_ROWS : PACK_OPERAND};
+        const bool bf = splitBf16_ && !dp_;
+        pj[2 * o] = PackJob{J.moSelf, J.pkSelf, (bf ? PACK_PRE_SELF_BF16 : PACK_PRE_SELF) | tin};
+        pj[2 * o + 1] =
+            PackJob{J.moSib, J.pkSib, J.ss ? PACK_ROWS : bf ? PACK_OPERAND_BF16 : PACK_OPERAND};
     }
     float* pack = static_cast<float*>(
         grow(reinterpret_cast<void*&>(dPack_), packCap_, pj.size() * per * sizeof(float)));
@@ -1334,7 +1336,8 @@ void Instance::launchPreMfma(const int* ops, int stride, int count) {
     auto d = reinterpret_cast<const PreJob*>(uploadJobs(sorted.data(), count * sizeof(PreJob)));
     for (int l = 0; l < nl; ++l) {
         if (dp_) launch_pre_mfma64(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
-        else launch_pre_mfma(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
+        else launch_pre_mfma(d + start[l], start[l + 1] - start[l], dims_, pack,
+                             splitBf16_ ? bf16Terms_ : 0, stream_);
     }
 }
 
@@ -1361,7 +1364,8 @@ int Instance::enqueueDerivMfma(const int* postIdx, const int* preIdx, const int*
         J.states = dStates_[po];
         J.post = dStates_[po] ? nullptr : dPartials_[po];
         J.row = n;
-        const int kind = (J.states ? PACK_ROWS : PACK_OPERAND) | tin;
+        const int kind =
+            (J.states ? PACK_ROWS : splitBf16_ && !dp_ ? PACK_OPERAND_BF16 : PACK_OPERAND) | tin;
         J.pk1 = (int)((size_t)nd * n * per);
         pj[(size_t)nd * n] = PackJob{(int)(d1[n] * mstride), J.pk1, kind};
         J.pk2 = J.pk1;
@@ -1386,7 +1390,7 @@ int Instance::enqueueDerivMfma(const int* postIdx, const int* preIdx, const int*
                             dPatternWeights_, sites, wk, stream_);
     else
         launch_deriv_mfma(dj, count, nd, dims_, pack, dCatWeights_ + (size_t)wIdx * dims_.C,
-                          dPatternWeights_, sites, wk, stream_);
+                          dPatternWeights_, sites, wk, splitBf16_ ? bf16Terms_ : 0, stream_);
     launch_reduce_rows(wk, ntiles, NR * count, wk + nPart, stream_);
     HIPCHECK(hipGetLastError());
     *dSums = wk + nPart;

@@ -183,6 +183,7 @@ enum PackKind : int {
     PACK_ROWS = 1,          // M[., s] by state s (tip-state operand)
     PACK_OPERAND_BF16 = 2,  // PACK_OPERAND as hi/lo BF16 parts
     PACK_PRE_SELF = 3,      // A operand of M^T u with u held in accumulator registers
+    PACK_PRE_SELF_BF16 = 4, // PACK_PRE_SELF for the BF16 matrix cores, as hi/lo parts
     // added to a kind: the stored matrix is the transpose (the reference GPU
     // convention for pre-order self and derivative matrices, S > 4)
     PACK_TRANSPOSED_INPUT = 8,
@@ -238,18 +239,20 @@ struct DerivJob {
     int row;
 };
 
-// Pre-order (S > 4, FP32) per dependency level, grid (Ppad/32, njobs): the
-// jobs' pkSelf (PACK_PRE_SELF) and pkSib (PACK_OPERAND / PACK_ROWS) offsets
-// in the pack slab.
+// Pre-order (S > 4, FP32 partials) per dependency level, grid (Ppad/32,
+// njobs): the jobs' pkSelf (PACK_PRE_SELF) and pkSib (PACK_OPERAND /
+// PACK_ROWS) offsets in the pack slab.  terms = 3 or 4: split-BF16 matrix
+// cores (PACK_PRE_SELF_BF16, PACK_OPERAND_BF16), 0: FP32 matrix cores.
 void launch_pre_mfma(const PreJob* djobs, int njobs, const Dims& d, const float* pack,
-                     hipStream_t st);
-// Edge derivatives (S > 4, FP32): nd = 1 or 2 derivative matrices; per job
-// and 32-pattern tile, partial sums rows [row * NR + k][tile] (NR = 2: sum w
-// r1, sum w r1^2; NR = 3: also sum w r2) with r = N / D over categories;
-// sites (nd = 1, may be null): r1 per job and pattern.
+                     int terms, hipStream_t st);
+// Edge derivatives (S > 4, FP32 partials): nd = 1 or 2 derivative matrices;
+// per job and 32-pattern tile, partial sums rows [row * NR + k][tile] (NR =
+// 2: sum w r1, sum w r1^2; NR = 3: also sum w r2) with r = N / D over
+// categories; sites (nd = 1, may be null): r1 per job and pattern.  terms
+// as launch_pre_mfma (matrices PACK_OPERAND_BF16 when 3 or 4).
 void launch_deriv_mfma(const DerivJob* djobs, int njobs, int nd, const Dims& d,
                        const float* pack, const double* cw, const double* pw, double* sites,
-                       double* partial, hipStream_t st);
+                       double* partial, int terms, hipStream_t st);
 
 // FP64 partials on v_mfma_f64_16x16x4f64: the matrices packed with
 // launch_pack_matrices64 (pack offsets in floats, even), grid (Ppad/16, njobs).

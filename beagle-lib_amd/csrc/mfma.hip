@@ -141,7 +141,31 @@ __global__ __launch_bounds__(WG) void k_pack_mats(const PackJob* __restrict__ jo
                 const int ee = el & 7, lane = (el >> 3) & 63, part = (el >> 9) & 1;
                 const int blk = el >> 10, t = blk % KT, mb = blk / KT;
                 const int i = mb * 32 + (lane & 31), k = 16 * t + 8 * (lane >> 5) + ee;
-                const float x = (i < S && k < S) ? (float)M[i * (S + 1) + k] : 0.f;
+                const float x = (i < S && k < S) ? at(i, k) : 0.f;
+                __bf16 hi, lo;
+                split_bf16(x, hi, lo);
+                half[u] = __builtin_bit_cast(unsigned short, part ? lo : hi);
+            }
+            v = __builtin_bit_cast(float, (unsigned)half[0] | ((unsigned)half[1] << 16));
+        }
+    } else if (kind == PACK_PRE_SELF_BF16) {
+        // M^T u with u in the accumulator layout on v_mfma_f32_32x32x16_bf16:
+        // [mo][instruction s][part: hi, lo][lane][8 BF16]; slot e of lane
+        // half h in instruction s is accumulator element 8 (s % 2) + e of
+        // block s / 2, i.e. inner state 32 (s / 2) + 16 (s % 2) + 8 (e / 4) +
+        // 4 h + e % 4, which the lane already holds; output row j = 32 mo +
+        // lane % 32; value M[k][j]
+        const int NS = 2 * MB;
+        if (e < MB * NS * 512) {
+            unsigned short half[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int el = 2 * e + u;
+                const int ee = el & 7, lane = (el >> 3) & 63, part = (el >> 9) & 1;
+                const int blk = el >> 10, s = blk % NS, mo = blk / NS;
+                const int k = 32 * (s >> 1) + 16 * (s & 1) + 8 * (ee >> 2) + 4 * (lane >> 5) + (ee & 3);
+                const int j = mo * 32 + (lane & 31);
+                const float x = (j < S && k < S) ? at(k, j) : 0.f;
                 __bf16 hi, lo;
                 split_bf16(x, hi, lo);
                 half[u] = __builtin_bit_cast(unsigned short, part ? lo : hi);
@@ -542,6 +566,32 @@ __device__ __forceinline__ void gemm_from_acc(Rsrc a, unsigned va, const floatx1
     }
 }
 
+// z = M^T u on the BF16 matrix cores (both operands split hi/lo, NT
+// products), u in the accumulator layout: instruction s takes the lane's
+// own elements 8 (s % 2) ... 8 (s % 2) + 7 of block s / 2 as its B operand
+// (PACK_PRE_SELF_BF16 orders M^T the same way); A operands PF instructions
+// ahead.
+template <int MB, int NT>
+__device__ __forceinline__ void gemm_from_acc16(Rsrc a, unsigned va, const floatx16 (&u)[MB],
+                                                floatx16 (&z)[MB]) {
+    constexpr int NS = 2 * MB, PF = 2;
+#pragma unroll
+    for (int mo = 0; mo < MB; ++mo) z[mo] = floatx16{0};
+    u32x4 ah[PF][MB], al[PF][MB];
+#pragma unroll
+    for (int j = 0; j < PF && j < NS; ++j) load_a16<MB>(a, va, j, NS, ah[j], al[j]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        float b[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) b[e] = u[s >> 1][8 * (s & 1) + e];
+        mfma16_stage<MB, NT>(z, ah[s % PF], al[s % PF], b);
+        if (s + PF < NS) load_a16<MB>(a, va, s + PF, NS, ah[s % PF], al[s % PF]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // values of a partials tile in the accumulator layout (rows >= S read 0);
 // lane offset vo = 4 n + 512 h (row 4 h, pattern n)
 template <int MB>
@@ -552,7 +602,9 @@ __device__ __forceinline__ void load_acc(const Tile& t, unsigned vo, floatx16 (&
         for (int r = 0; r < 16; ++r) v[mb][r] = tile_load(t, mb * 32 + 8 * (r >> 2) + (r & 3), vo);
 }
 
-template <int MB, int KG, bool AUTO>
+// MODE 0: FP32 matrix cores (KX = k groups of 8); 3, 4: split BF16 with 3
+// or 4 products (KX = k steps of 16).
+template <int MB, int KX, bool AUTO, int MODE>
 __global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, Dims d,
                                               const float* __restrict__ pack) {
     __shared__ float red[8][32];
@@ -575,34 +627,48 @@ __global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, D
     } else {
         const Tile xs = tile_of(J.xs, tbase, S);
         const Rsrc as = rsrc(psib, -1);
-        stream_mfma<MB, KG, 1, false>(xs, as, xs, as, 4 * l, 16 * l, y);
+        if (MODE == 0) stream_mfma<MB, KX, 1, false>(xs, as, xs, as, 4 * l, 16 * l, y);
+        else stream_bf16<MB, KX, 1, MODE>(xs, as, xs, as, 1024 * h + 4 * n, 16 * l, y);
     }
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) y[0][mb] *= par[mb];
-    gemm_from_acc<MB>(rsrc(pack + J.pkSelf + c * per, -1), 16 * l, y[0], y[1]);
+    const Rsrc aself = rsrc(pack + J.pkSelf + c * per, -1);
+    if (MODE == 0) gemm_from_acc<MB>(aself, 16 * l, y[0], y[1]);
+    else gemm_from_acc16<MB, MODE>(aself, 16 * l, y[0], y[1]);
     const ScaleOut so{J.mode, J.logScalers, nullptr, J.scaleWrite, nullptr, J.accA,
                       J.accB, J.e1, J.e2, J.ed, J.dest};
     rescale_store<MB, AUTO>(y[1], so, tbase, S, d.C, c, n, h, p, in, red);
 }
 
 void launch_pre_mfma(const PreJob* djobs, int njobs, const Dims& d, const float* pack,
-                     hipStream_t st) {
+                     int terms, hipStream_t st) {
     if (njobs <= 0) return;
     const dim3 grid(d.Ppad / PTILE, njobs), block(64 * d.C);
-    const int KG = mfma_kgroups(d.S);
-#define PREM(M, K)                                                                             \
+#define PREM_V(M, K, A, MODE)                                                                  \
+    hipLaunchKernelGGL((k_prem<M, K, A, MODE>), grid, block, 0, st, djobs, d, pack);
+#define PREM(M, K, MODE)                                                                       \
     case K:                                                                                    \
-        if (d.autoScale)                                                                       \
-            hipLaunchKernelGGL((k_prem<M, K, true>), grid, block, 0, st, djobs, d, pack);     \
-        else                                                                                   \
-            hipLaunchKernelGGL((k_prem<M, K, false>), grid, block, 0, st, djobs, d, pack);    \
+        if (d.autoScale) { PREM_V(M, K, true, MODE) } else { PREM_V(M, K, false, MODE) }       \
         break;
-    switch (KG) {
-        PREM(1, 1) PREM(1, 2) PREM(1, 3) PREM(1, 4)
-        PREM(2, 5) PREM(2, 6) PREM(2, 7) PREM(2, 8)
-        default: break;
+    if (terms == 0) {
+        switch (mfma_kgroups(d.S)) {
+            PREM(1, 1, 0) PREM(1, 2, 0) PREM(1, 3, 0) PREM(1, 4, 0)
+            PREM(2, 5, 0) PREM(2, 6, 0) PREM(2, 7, 0) PREM(2, 8, 0)
+            default: break;
+        }
+    } else if (terms == 4) {
+        switch (mfma_ksteps16(d.S)) {
+            PREM(1, 1, 4) PREM(1, 2, 4) PREM(2, 3, 4) PREM(2, 4, 4)
+            default: break;
+        }
+    } else {
+        switch (mfma_ksteps16(d.S)) {
+            PREM(1, 1, 3) PREM(1, 2, 3) PREM(2, 3, 3) PREM(2, 4, 3)
+            default: break;
+        }
     }
 #undef PREM
+#undef PREM_V
 }
 
 // ---------------------------------------------------------------------------
@@ -619,7 +685,8 @@ __device__ __forceinline__ double half_wave_sum(double v) {
     return v;
 }
 
-template <int MB, int KG, int ND>
+// MODE and KX as k_prem.
+template <int MB, int KX, int ND, int MODE>
 __global__ __launch_bounds__(512) void k_derivm(const DerivJob* __restrict__ jobs, Dims d,
                                                 const float* __restrict__ pack,
                                                 const double* __restrict__ cw,
@@ -645,7 +712,11 @@ __global__ __launch_bounds__(512) void k_derivm(const DerivJob* __restrict__ job
         if (ND == 2) state_rows<MB>(p2, J.states, p, h, y[1]);
     } else {
         const Tile x = tile_of(J.post, tbase, S);
-        stream_mfma<MB, KG, ND, false>(x, rsrc(p1, -1), x, rsrc(p2, -1), 4 * l, 16 * l, y);
+        if (MODE == 0)
+            stream_mfma<MB, KX, ND, false>(x, rsrc(p1, -1), x, rsrc(p2, -1), 4 * l, 16 * l, y);
+        else
+            stream_bf16<MB, KX, ND, MODE>(x, rsrc(p1, -1), x, rsrc(p2, -1), 1024 * h + 4 * n,
+                                          16 * l, y);
     }
     load_acc<MB>(tile_of(J.pre, tbase, S), 4 * n + 512 * h, z);
     float den = 0.f, n1 = 0.f, n2 = 0.f;
@@ -706,25 +777,35 @@ __global__ __launch_bounds__(512) void k_derivm(const DerivJob* __restrict__ job
 
 void launch_deriv_mfma(const DerivJob* djobs, int njobs, int nd, const Dims& d,
                        const float* pack, const double* cw, const double* pw, double* sites,
-                       double* partial, hipStream_t st) {
+                       double* partial, int terms, hipStream_t st) {
     if (njobs <= 0) return;
     const dim3 grid((d.P + PTILE - 1) / PTILE, njobs), block(64 * d.C);
-    const int KG = mfma_kgroups(d.S);
-#define DERIVM(M, K)                                                                           \
+#define DERIVM_V(M, K, N, MODE)                                                                \
+    hipLaunchKernelGGL((k_derivm<M, K, N, MODE>), grid, block, 0, st, djobs, d, pack, cw, pw,  \
+                       sites, partial);
+#define DERIVM(M, K, MODE)                                                                     \
     case K:                                                                                    \
-        if (nd == 2)                                                                           \
-            hipLaunchKernelGGL((k_derivm<M, K, 2>), grid, block, 0, st, djobs, d, pack, cw, pw, \
-                               sites, partial);                                                \
-        else                                                                                   \
-            hipLaunchKernelGGL((k_derivm<M, K, 1>), grid, block, 0, st, djobs, d, pack, cw, pw, \
-                               sites, partial);                                                \
+        if (nd == 2) { DERIVM_V(M, K, 2, MODE) } else { DERIVM_V(M, K, 1, MODE) }              \
         break;
-    switch (KG) {
-        DERIVM(1, 1) DERIVM(1, 2) DERIVM(1, 3) DERIVM(1, 4)
-        DERIVM(2, 5) DERIVM(2, 6) DERIVM(2, 7) DERIVM(2, 8)
-        default: break;
+    if (terms == 0) {
+        switch (mfma_kgroups(d.S)) {
+            DERIVM(1, 1, 0) DERIVM(1, 2, 0) DERIVM(1, 3, 0) DERIVM(1, 4, 0)
+            DERIVM(2, 5, 0) DERIVM(2, 6, 0) DERIVM(2, 7, 0) DERIVM(2, 8, 0)
+            default: break;
+        }
+    } else if (terms == 4) {
+        switch (mfma_ksteps16(d.S)) {
+            DERIVM(1, 1, 4) DERIVM(1, 2, 4) DERIVM(2, 3, 4) DERIVM(2, 4, 4)
+            default: break;
+        }
+    } else {
+        switch (mfma_ksteps16(d.S)) {
+            DERIVM(1, 1, 3) DERIVM(1, 2, 3) DERIVM(2, 3, 3) DERIVM(2, 4, 3)
+            default: break;
+        }
     }
 #undef DERIVM
+#undef DERIVM_V
 }
 
 // ---------------------------------------------------------------------------

@@ -17,8 +17,10 @@ extern "C" {
  * with events.  No counterpart in the reference. */
 int beagleExtGetStream(int instance, void** outStream);
 
-/* Matrix-core engine of the post-order partials products of an FP32
- * instance with 5 to 64 states (no counterpart in the reference).
+/* Matrix-core engine of the partials products of an FP32 instance with 5
+ * to 64 states: post-order, pre-order and edge-derivative operations (no
+ * counterpart in the reference; the rate-matrix gradient stays on the FP32
+ * matrix cores).
  * terms = 0: FP32 matrix cores (the default).  terms = 3: BF16 matrix cores
  * with every FP32 operand split into a BF16 high and low part, x = hi + lo,
  * and three products per pair (lo*hi + hi*lo + hi*hi, FP32 accumulation);

@@ -318,9 +318,11 @@ ADJ_CASES = [
 ]
 
 
-def _adjoint_run(lib, prob, precision, tips_as, scaling, transpose):
+def _adjoint_run(lib, prob, precision, tips_as, scaling, transpose, terms=0):
     ev = driver.Evaluator(lib, prob, precision=precision, tips_as=tips_as, scaling=scaling,
                           matrices="explicit", transpose=transpose)
+    if terms:
+        ev.inst.set_split_bf16(terms)
     out = {"logl": ev.log_likelihood()}
     ev.pre_order()
     out["d1"], out["s1"], out["q1"] = ev.edge_derivatives(False)
@@ -356,6 +358,45 @@ def test_mfma_adjoint_single_matches_oracle(amd, oracle, S, n, P, shape, tips_as
         close_rows(pg[keep], po[keep], S, 1e-5)
 
 
+BF16_ADJ_CASES = [
+    # S, taxa, patterns, shape, tips_as, scaling, transpose, products per hi/lo pair
+    (12, 7, 180, "yule", "states", "auto", "auto", 3),
+    (20, 10, 400, "yule", "states", "none", "auto", 3),
+    (33, 6, 97, "yule", "partials", "manual", "auto", 4),
+    (61, 8, 200, "balanced", "states", "auto", "manual", 3),
+    (61, 7, 150, "yule", "partials", "none", "auto", 4),
+]
+
+
+@pytest.mark.parametrize("S,n,P,shape,tips_as,scaling,transpose,terms", BF16_ADJ_CASES)
+def test_split_bf16_adjoint_matches_oracle(amd, oracle, S, n, P, shape, tips_as, scaling,
+                                           transpose, terms):
+    """beagleExtSetSplitBF16 on the adjoint: post-order, pre-order (P_sib
+    x_sib, then P_self^T fed from the accumulator registers) and edge
+    derivatives on v_mfma_f32_32x32x16_bf16 with every operand split into
+    BF16 hi and lo parts (3 or 4 products per pair), against the FP64
+    oracle: logL within 1e-6 relative, per-site derivative ratios and
+    per-branch sums within 1e-4 of their largest magnitude, pre-order
+    partials (unscaled runs) within 1e-4 of each row's maximum.  The
+    pre-order partials differ from the FP32 engine's, so the BF16 kernels
+    ran."""
+    prob = synth.make_problem(29 + S + n, n, P, S=S, shape=shape, missing=0.05, mean_length=0.2)
+    g = _adjoint_run(amd, prob, "single", tips_as, scaling, transpose, terms)
+    f = _adjoint_run(amd, prob, "single", tips_as, scaling, transpose)
+    o = _adjoint_run(oracle, prob, "double", tips_as,
+                     "manual" if scaling == "auto" else scaling, "auto")
+    assert rel(g["logl"], o["logl"]) < 1e-6
+    for k in ("d1", "s1", "q1", "s2"):
+        a, b = np.asarray(g[k]), np.asarray(o[k])
+        assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max(), (k, np.abs(a - b).max())
+    assert not np.array_equal(np.asarray(g["pre"]), np.asarray(f["pre"]))
+    if scaling == "none":
+        pg = np.asarray(g["pre"]).reshape(-1, S)
+        po = np.asarray(o["pre"]).reshape(-1, S)
+        keep = po.max(axis=1) > 1e-30
+        close_rows(pg[keep], po[keep], S, 1e-4)
+
+
 @pytest.mark.parametrize("S,tips_as,scaling,transpose", [(20, "states", "manual", "auto"),
                                                         (61, "partials", "none", "manual"),
                                                         (33, "states", "auto", "auto")])
@@ -377,15 +418,18 @@ def test_mfma_adjoint_double_matches_oracle(amd, oracle, S, tips_as, scaling, tr
         close_rows(g["pre"], o["pre"], S, 1e-12)
 
 
-@pytest.mark.parametrize("S,transpose", [(20, "auto"), (61, "auto"), (61, "manual")])
-def test_mfma_fused_gradient_single(amd, oracle, S, transpose):
+@pytest.mark.parametrize("S,transpose,terms", [(20, "auto", 0), (61, "auto", 0), (61, "manual", 0),
+                                              (20, "manual", 3), (61, "auto", 3)])
+def test_mfma_fused_gradient_single(amd, oracle, S, transpose, terms):
     """beagleExtCalculateLogLikelihoodAndDerivatives for S > 4 in FP32
-    (matrix-core pre-order, then both derivative matrices in one pass)
-    against the oracle's beagleUpdatePrePartials + two
-    beagleCalculateEdgeDerivatives."""
+    (matrix-core pre-order, then both derivative matrices in one pass; FP32
+    or, terms = 3, split-BF16 matrix cores) against the oracle's
+    beagleUpdatePrePartials + two beagleCalculateEdgeDerivatives."""
     prob = synth.make_problem(37 + S, 9, 300, S=S, missing=0.05, mean_length=0.2)
     ev = driver.Evaluator(amd, prob, precision="single", scaling="auto", matrices="explicit",
                           transpose=transpose)
+    if terms:
+        ev.inst.set_split_bf16(terms)
     lg, g1, g2 = ev.fused_gradient()
     ev.close()
     ev = driver.Evaluator(oracle, prob, scaling="manual", matrices="explicit")
@@ -1434,8 +1478,9 @@ def test_codon_full_size_engines(amd):
     """The 61-state GY94 codon workload at full size (64 taxa, 20,000
     patterns, Gamma(4)), where the oracle would take minutes: the FP32 and
     split-BF16 (3 and 4 products) matrix-core post-order give logL within
-    1e-6 relative of the FP64 matrix-core path, and the FP32 logL + branch
-    gradient is within 1e-6 of FP64 (relative to the largest component)."""
+    1e-6 relative of the FP64 matrix-core path, and the FP32 and split-BF16
+    (3 products: post-order, pre-order and derivatives) logL + branch
+    gradient are within 1e-6 of FP64 (relative to the largest component)."""
     prob = synth.make_problem(3, 64, 20_000, S=61, shape="yule")
     ev = driver.Evaluator(amd, prob)
     l64 = ev.log_likelihood()
@@ -1446,12 +1491,15 @@ def test_codon_full_size_engines(amd):
     for terms in (0, 3, 4):
         ev.inst.set_split_bf16(terms)
         assert rel(ev.log_likelihood(), l64) < 1e-6, terms
+    ev.inst.set_split_bf16(3)
+    gbf = ev.fused_gradient()
     ev.inst.set_split_bf16(0)
     g32 = ev.fused_gradient()
     ev.close()
-    assert rel(g32[0], g64[0]) < 1e-6
-    for a, b in zip(g32[1:], g64[1:]):
-        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
+    for g in (g32, gbf):
+        assert rel(g[0], g64[0]) < 1e-6
+        for a, b in zip(g[1:], g64[1:]):
+            assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
 
 
 def test_adjoint_full_size_512_taxa(amd):
