@@ -19,6 +19,28 @@ __device__ __forceinline__ GLOBAL T* gout(void* p) {
     return (GLOBAL T*)p;
 }
 
+// Job and item of this workgroup under a level work list (TileList in
+// kernels.h): the largest j with first[j] <= blockIdx.x, found by a binary
+// search over wave-uniform (scalar) loads; without a list the grid is
+// (items, jobs).
+template <typename List>
+__device__ __forceinline__ void tile_item(const List& w, int& job, int& item) {
+    if (!w.first) {
+        job = blockIdx.y;
+        item = blockIdx.x;
+        return;
+    }
+    const int b = blockIdx.x;
+    int lo = 0, hi = w.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (w.first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    job = lo;
+    item = b - w.first[lo];
+}
+
 // x = f 2^e with f in [0.5, 1)
 __device__ __forceinline__ int exponent_of(double x) {
     int e;

@@ -222,6 +222,7 @@ Instance::Instance(int tipCount, int partialsBufferCount, int compactBufferCount
             mfmaVariantForced_ = true;
         }
         if (const char* m = std::getenv("BEAGLE_AMD_AUTOTUNE")) autotune_ = std::atoi(m) != 0;
+        if (const char* m = std::getenv("BEAGLE_AMD_TILE_LISTS")) tileLists_ = std::atoi(m) != 0;
         ring_.resize(8);
         for (auto& s : ring_) HIPCHECK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
         jobs_.resize(8);
@@ -1118,6 +1119,35 @@ int Instance::postLevels(const int* ops, int stride, int count, std::vector<int>
     return dependencyLevels(acc, bufferCount_ + scaleCount_, level);
 }
 
+// Work lists (TileList) of the levels [start[l], start[l + 1]) of `jobs`:
+// nothing when every job covers all patterns; otherwise, per level, first[]
+// over its jobs' tiles of tileP patterns (from the tile that holds the job's
+// first pattern) appended to `ints`, off[l] its offset there (-1: the level
+// keeps the (tiles, jobs) grid) and items[l] its tile count.
+constexpr int kTile4 = 64;   // patterns per tile of the 4-state level kernels
+
+template <typename Job>
+static void levelTileLists(const Job* jobs, const int* start, int nl, int P, int tileP,
+                           std::vector<int>& ints, std::vector<int>& off,
+                           std::vector<int>& items) {
+    off.assign(nl, -1);
+    items.assign(nl, 0);
+    for (int l = 0; l < nl; ++l) {
+        bool whole = true;
+        for (int o = start[l]; o < start[l + 1]; ++o) whole &= jobs[o].p0 == 0 && jobs[o].p1 == P;
+        if (whole) continue;
+        off[l] = (int)ints.size();
+        int acc = 0;
+        for (int o = start[l]; o < start[l + 1]; ++o) {
+            ints.push_back(acc);
+            const int t0 = jobs[o].p0 / tileP, t1 = (jobs[o].p1 + tileP - 1) / tileP;
+            acc += std::max(0, t1 - t0);
+        }
+        ints.push_back(acc);
+        items[l] = acc;
+    }
+}
+
 // MFMA post-order (S > 4; FP64, FP32 or split-BF16 matrix cores): every
 // child matrix of the list is repacked once into the pack slab, then each
 // dependency level is one launch over (pattern tiles x ops).  A list whose
@@ -1156,12 +1186,19 @@ void Instance::launchPostMfma(const int* ops, int stride, int count) {
     if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     auto d = reinterpret_cast<const PostJob*>(uploadJobs(sorted.data(), count * sizeof(PostJob)));
+    // operations by partition: each level's (operation, 32-pattern tile) list
+    std::vector<int> lints, loff, litems;
+    if (!dp_ && tileLists_) levelTileLists(sorted.data(), start.data(), nl, dims_.P, PTILE, lints, loff, litems);
+    const int* dl = lints.empty() ? nullptr
+                                  : static_cast<const int*>(uploadJobs(lints.data(), lints.size() * sizeof(int)));
     auto run = [&](int variant) {
         for (int l = 0; l < nl; ++l) {
             const int nj = start[l + 1] - start[l];
+            const TileList w{dl && loff[l] >= 0 ? dl + loff[l] : nullptr, nj};
+            const int it = dl ? litems[l] : 0;
             if (dp_) launch_post_mfma64(d + start[l], nj, dims_, pack, stream_);
-            else if (splitBf16_) launch_post_bf16(d + start[l], nj, dims_, pack, bf16Terms_, stream_);
-            else launch_post_mfma(d + start[l], nj, dims_, pack, variant, stream_);
+            else if (splitBf16_) launch_post_bf16(d + start[l], nj, dims_, pack, bf16Terms_, stream_, w, it);
+            else launch_post_mfma(d + start[l], nj, dims_, pack, variant, stream_, w, it);
         }
     };
     if (dp_ || splitBf16_ || shared) {
@@ -1213,12 +1250,27 @@ void Instance::launchPostPlan(const std::vector<Segment>& plan) {
     }
     if (all.empty()) return;
     auto d = reinterpret_cast<const PostJob*>(uploadJobs(all.data(), all.size() * sizeof(PostJob)));
-    for (const Launch& L : launches) {
+    // batched levels of operations by partition: (operation, tile) lists
+    std::vector<int> lints, loff(launches.size(), -1), litems(launches.size(), 0);
+    for (size_t i = 0; i < launches.size(); ++i) {
+        const Launch& L = launches[i];
+        if (!L.batched || levelChunk_ > 1 || !tileLists_) continue;
+        std::vector<int> o1, i1;
+        const int start[2] = {L.first, L.first + L.n};
+        levelTileLists(all.data(), start, 1, dims_.P, kTile4, lints, o1, i1);
+        loff[i] = o1[0];
+        litems[i] = i1[0];
+    }
+    const int* dl = lints.empty() ? nullptr
+                                  : static_cast<const int*>(uploadJobs(lints.data(), lints.size() * sizeof(int)));
+    for (size_t i = 0; i < launches.size(); ++i) {
+        const Launch& L = launches[i];
         PostJob* h = all.data() + L.first;
         if (L.batched && levelChunk_ <= 1) {
             const int start[2] = {0, L.n};
-            if (dp_) launch_post_levels<double>(d + L.first, h, start, 1, dims_, dMatrices_, stream_);
-            else launch_post_levels<float>(d + L.first, h, start, 1, dims_, dMatrices_, stream_);
+            const TileList w{dl && loff[i] >= 0 ? dl + loff[i] : nullptr, L.n};
+            if (dp_) launch_post_levels<double>(d + L.first, h, start, 1, dims_, dMatrices_, stream_, w, litems[i]);
+            else launch_post_levels<float>(d + L.first, h, start, 1, dims_, dMatrices_, stream_, w, litems[i]);
         } else {
             // fused list, or a batched level walked in chunks of independent ops
             const int chunk = L.batched ? levelChunk_ : 0;
@@ -1334,10 +1386,17 @@ void Instance::launchPreMfma(const int* ops, int stride, int count) {
     if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     auto d = reinterpret_cast<const PreJob*>(uploadJobs(sorted.data(), count * sizeof(PreJob)));
+    // operations by partition: each level's (operation, 32-pattern tile) list
+    std::vector<int> lints, loff, litems;
+    if (!dp_ && tileLists_) levelTileLists(sorted.data(), start.data(), nl, dims_.P, PTILE, lints, loff, litems);
+    const int* dl = lints.empty() ? nullptr
+                                  : static_cast<const int*>(uploadJobs(lints.data(), lints.size() * sizeof(int)));
     for (int l = 0; l < nl; ++l) {
-        if (dp_) launch_pre_mfma64(d + start[l], start[l + 1] - start[l], dims_, pack, stream_);
-        else launch_pre_mfma(d + start[l], start[l + 1] - start[l], dims_, pack,
-                             splitBf16_ ? bf16Terms_ : 0, stream_);
+        const int nj = start[l + 1] - start[l];
+        const TileList w{dl && loff[l] >= 0 ? dl + loff[l] : nullptr, nj};
+        if (dp_) launch_pre_mfma64(d + start[l], nj, dims_, pack, stream_);
+        else launch_pre_mfma(d + start[l], nj, dims_, pack, splitBf16_ ? bf16Terms_ : 0, stream_,
+                             w, dl ? litems[l] : 0);
     }
 }
 
@@ -1478,11 +1537,27 @@ void Instance::launchPrePlan(const std::vector<Segment>& plan, const double* cw,
     }
     if (all.empty()) return;
     auto d = reinterpret_cast<const PreJob*>(uploadJobs(all.data(), all.size() * sizeof(PreJob)));
-    for (const Launch& L : launches) {
+    // batched levels of operations by partition (no derivative sums, which
+    // need every tile's slot): (operation, tile) lists
+    std::vector<int> lints, loff(launches.size(), -1), litems(launches.size(), 0);
+    for (size_t i = 0; i < launches.size() && !partial; ++i) {
+        const Launch& L = launches[i];
+        if (!L.batched || !tileLists_) continue;
+        std::vector<int> o1, i1;
+        const int start[2] = {L.first, L.first + L.n};
+        levelTileLists(all.data(), start, 1, dims_.P, kTile4, lints, o1, i1);
+        loff[i] = o1[0];
+        litems[i] = i1[0];
+    }
+    const int* dl = lints.empty() ? nullptr
+                                  : static_cast<const int*>(uploadJobs(lints.data(), lints.size() * sizeof(int)));
+    for (size_t i = 0; i < launches.size(); ++i) {
+        const Launch& L = launches[i];
         if (L.batched) {
             const int start[2] = {0, L.n};
-            if (dp_) launch_pre_levels<double>(d + L.first, start, 1, dims_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
-            else launch_pre_levels<float>(d + L.first, start, 1, dims_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);
+            const TileList w{dl && loff[i] >= 0 ? dl + loff[i] : nullptr, L.n};
+            if (dp_) launch_pre_levels<double>(d + L.first, start, 1, dims_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_, w, litems[i]);
+            else launch_pre_levels<float>(d + L.first, start, 1, dims_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_, w, litems[i]);
         } else {
             PreJob* h = all.data() + L.first;
             if (dp_) launch_pre<double>(d + L.first, h, L.n, dims_, 0, dScratch_, dMatrices_, dDummy_, cw, dPatternWeights_, partial, stream_);

@@ -312,6 +312,7 @@ private:
         std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;  // per candidate
     };
     bool autotune_ = true;
+    bool tileLists_ = true;   // work lists for level launches over pattern ranges
     std::unordered_map<uint64_t, Tuning> tuned_;
     int tunings_ = 0;   // shape classes tuned so far
     static uint64_t shapeKey(uint64_t tag, int count, int levels);

@@ -171,6 +171,18 @@ __host__ __device__ inline long pbase(const Dims& d, int c, int p) {
                                              : (long)(p / PTILE) * PTILE * d.Sp + p % PTILE);
 }
 
+// Work list of one level launch whose operations cover pattern ranges
+// (operations by partition): workgroup b runs item b - first[j] of job j,
+// first[j] <= b < first[j + 1] counting the items (tiles, or groups of tiles)
+// of the jobs before j, from the tile that holds the job's first pattern.
+// first == nullptr: the grid is (tiles, jobs) over the whole pattern range.
+// (BeagleGPUImpl.hpp:2466-2572 lists (operation, pattern block) entries for
+// one grid in the same way.)
+struct TileList {
+    const int* first;
+    int n;
+};
+
 // One child matrix of a post-order operation repacked for the MFMA kernels
 // (mfma.hip): src is the matrix's element offset in the matrix slab, dst
 // the block's offset in the pack slab (C * mfma_pack_reals(S) floats).
@@ -226,8 +238,9 @@ void launch_pack_matrices(const PackJob* jobs, int njobs, int S, int C, const vo
                           float* pack, hipStream_t st);
 // One launch per dependency level: grid (pattern tiles, njobs).  variant 0:
 // partials loaded a few k groups ahead; 1: the whole partials tile up front.
+// w.first: a work list of `items` 32-pattern tiles (TileList).
 void launch_post_mfma(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
-                      int variant, hipStream_t st);
+                      int variant, hipStream_t st, TileList w = {nullptr, 0}, int items = 0);
 // One node of an MFMA edge-derivative request (S > 4, FP32): its pre- and
 // post-order partials (or tip states) and one or two packed derivative
 // matrices; `row` is its index in the request (output rows row * NR + k).
@@ -244,7 +257,7 @@ struct DerivJob {
 // PACK_ROWS) offsets in the pack slab.  terms = 3 or 4: split-BF16 matrix
 // cores (PACK_PRE_SELF_BF16, PACK_OPERAND_BF16), 0: FP32 matrix cores.
 void launch_pre_mfma(const PreJob* djobs, int njobs, const Dims& d, const float* pack,
-                     int terms, hipStream_t st);
+                     int terms, hipStream_t st, TileList w = {nullptr, 0}, int items = 0);
 // Edge derivatives (S > 4, FP32 partials): nd = 1 or 2 derivative matrices;
 // per job and 32-pattern tile, partial sums rows [row * NR + k][tile] (NR =
 // 2: sum w r1, sum w r1^2; NR = 3: also sum w r2) with r = N / D over
@@ -270,7 +283,7 @@ void launch_deriv_mfma64(const DerivJob* djobs, int njobs, int nd, const Dims& d
 // The same on the BF16 matrix cores with hi/lo split operands (terms: 3 or 4
 // products per split pair); the matrices packed as PACK_OPERAND_BF16.
 void launch_post_bf16(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
-                      int terms, hipStream_t st);
+                      int terms, hipStream_t st, TileList w = {nullptr, 0}, int items = 0);
 
 // ---- launchers (kernels.hip) ----
 // n doubles from device memory to (device-visible) pinned host memory.
@@ -291,8 +304,10 @@ inline size_t dummy_bytes() { return 2 * 8 * 64 * 4 * sizeof(double) + 64 * (8 +
 // level l occupies [levelStart[l], levelStart[l+1]); hjobs is the host copy
 // of djobs (the launch shape of each level depends on its operands).
 template <typename R>
+// w.first (one level only): a work list of `items` 64-pattern tiles.
 void launch_post_levels(const PostJob* djobs, const PostJob* hjobs, const int* levelStart,
-                        int nlevels, const Dims& d, const void* mats, hipStream_t st);
+                        int nlevels, const Dims& d, const void* mats, hipStream_t st,
+                        TileList w = {nullptr, 0}, int items = 0);
 // partial != null: fused derivative sums, 3 x ntiles doubles per job
 // (ntiles = ceil(P/64)), reduced afterwards with launch_reduce_rows.
 template <typename R>
@@ -301,9 +316,11 @@ void launch_pre(const PreJob* djobs, const PreJob* hjobs, int njobs, const Dims&
                 const double* cw, const double* pw, double* partial, hipStream_t st);
 void launch_reduce_rows(const double* in, int n, int rows, double* out, hipStream_t st);
 template <typename R>
+// w.first (one level, partial == null only): a work list of `items` tiles.
 void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, const Dims& d,
                        const void* mats, void* dummy, const double* cw, const double* pw,
-                       double* partial, hipStream_t st);
+                       double* partial, hipStream_t st, TileList w = {nullptr, 0},
+                       int items = 0);
 // Node-fused pre-order + derivative traversal (4 states): batched == true
 // runs the njobs jobs (one dependency level) in one grid over (tiles, jobs),
 // otherwise one launch walks the list in order.

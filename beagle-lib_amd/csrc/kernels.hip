@@ -605,14 +605,16 @@ __global__ __launch_bounds__(512) void k_post4(const PostJob* __restrict__ jobs,
 template <typename R, bool AUTO>
 __global__ __launch_bounds__(512) void k_post4_level(const PostJob* __restrict__ jobs, int P,
                                                      int C, long cs,
-                                                     const R* __restrict__ mats) {
+                                                     const R* __restrict__ mats, TileList w) {
     __shared__ R smax[2][8][TILE];
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int p = blockIdx.x * TILE + lane;
+    int job, item;
+    tile_item(w, job, item);
+    const PostJob J = jobs[job];
+    const int p = ((w.first ? J.p0 / TILE : 0) + item) * TILE + lane;
     const long off = c * cs + (long)p * 4;
     int k = 0;
-    const PostJob J = jobs[blockIdx.y];
     const bool in = p < P && p >= J.p0 && p < J.p1;
     Operand4<R> A, B;
     fetch4(J.x1, J.s1, off, p, A);
@@ -883,11 +885,14 @@ __global__ __launch_bounds__(512) void k_pre4_level(const PreJob* __restrict__ j
                                                     R* dummyW, const short* dummyE,
                                                     const double* __restrict__ cw,
                                                     const double* __restrict__ pw,
-                                                    double* partial) {
+                                                    double* partial, TileList w) {
     extern __shared__ double sdv[];   // [2][C][4][TILE]
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x;
+    int job, item;
+    tile_item(w, job, item);
+    const PreJob J = jobs[job];
+    const int tile = (w.first ? J.p0 / TILE : 0) + item;
     const int p = tile * TILE + lane;
     const long off = c * cs + (long)p * 4;
     const long doff = (long)(c * TILE + lane) * 4;
@@ -895,7 +900,6 @@ __global__ __launch_bounds__(512) void k_pre4_level(const PreJob* __restrict__ j
     int k = 0;
     R prev[4] = {0, 0, 0, 0}, uprev[4] = {0, 0, 0, 0};
     PreOps4<R> S0;
-    const PreJob J = jobs[blockIdx.y];
     pre_issue<R, DERIV>(J, dummyX, dummyS, off, doff, p, lane, S0);
     pre_compute<R, AUTO, DERIV>(J, S0, prev, uprev, mats, sdv, k, c, lane, p, P, C, off, doff,
                                 dummyW, dummyE, cw, pw, partial, tile, ntiles);
@@ -1347,9 +1351,18 @@ constexpr long kPostLevelWaves = 16384;
 // whose grids come from the host copy of each job.
 template <typename R>
 void launch_post_levels(const PostJob* djobs, const PostJob* hjobs, const int* levelStart,
-                        int nlevels, const Dims& d, const void* mats, hipStream_t st) {
+                        int nlevels, const Dims& d, const void* mats, hipStream_t st,
+                        TileList w, int items) {
     const int tiles = (d.P + TILE - 1) / TILE;
     const R* m = reinterpret_cast<const R*>(mats);
+    if (w.first) {
+        // one level over pattern ranges: one tile per workgroup, listed
+        if (items <= 0) return;
+        const PostJob* lj = djobs + levelStart[0];
+        if (d.autoScale) hipLaunchKernelGGL((k_post4_level<R, true>), dim3(items), dim3(TILE * d.C), 0, st, lj, d.P, d.C, d.catStride, m, w);
+        else hipLaunchKernelGGL((k_post4_level<R, false>), dim3(items), dim3(TILE * d.C), 0, st, lj, d.P, d.C, d.catStride, m, w);
+        return;
+    }
     for (int l = 0; l < nlevels; ++l) {
         const int n = levelStart[l + 1] - levelStart[l];
         const PostJob* lj = djobs + levelStart[l];
@@ -1379,8 +1392,8 @@ void launch_post_levels(const PostJob* djobs, const PostJob* hjobs, const int* l
         }
         {
             const dim3 grid(tiles, n);
-            if (d.autoScale) hipLaunchKernelGGL((k_post4_level<R, true>), grid, block, 0, st, lj, d.P, d.C, d.catStride, m);
-            else hipLaunchKernelGGL((k_post4_level<R, false>), grid, block, 0, st, lj, d.P, d.C, d.catStride, m);
+            if (d.autoScale) hipLaunchKernelGGL((k_post4_level<R, true>), grid, block, 0, st, lj, d.P, d.C, d.catStride, m, TileList{nullptr, 0});
+            else hipLaunchKernelGGL((k_post4_level<R, false>), grid, block, 0, st, lj, d.P, d.C, d.catStride, m, TileList{nullptr, 0});
         }
     }
 }
@@ -1428,7 +1441,9 @@ static size_t pre_lds_bytes(int C) { return (size_t)2 * C * 4 * TILE * sizeof(do
 template <typename R>
 void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, const Dims& d,
                        const void* mats, void* dummy, const double* cw, const double* pw,
-                       double* partial, hipStream_t st) {
+                       double* partial, hipStream_t st, TileList w, int items) {
+    if (w.first && (partial || nlevels != 1)) w.first = nullptr;   // sums need every tile's slot
+    if (w.first && items <= 0) return;
     const R* dummyX = reinterpret_cast<const R*>(dummy);
     R* dummyW = const_cast<R*>(dummyX) + 8 * TILE * 4;
     const double* dummyD = reinterpret_cast<const double*>(dummyW + 8 * TILE * 4);
@@ -1437,13 +1452,13 @@ void launch_pre_levels(const PreJob* djobs, const int* levelStart, int nlevels, 
     const R* m = reinterpret_cast<const R*>(mats);
     for (int l = 0; l < nlevels; ++l) {
         const int n = levelStart[l + 1] - levelStart[l];
-        dim3 grid((d.P + TILE - 1) / TILE, n);
+        const dim3 grid = w.first ? dim3(items) : dim3((d.P + TILE - 1) / TILE, n);
         dim3 block(TILE * d.C);
         const size_t lds = pre_lds_bytes(d.C);
         const PreJob* j = djobs + levelStart[l];
 #define PREL(A, D)                                                                                \
     hipLaunchKernelGGL((k_pre4_level<R, A, D>), grid, block, lds, st, j, d.P, d.C, d.catStride, m, \
-                       dummyX, dummyS, dummyW, dummyE, cw, pw, partial)
+                       dummyX, dummyS, dummyW, dummyE, cw, pw, partial, w)
         if (partial) {
             if (d.autoScale) PREL(true, true);
             else PREL(false, true);
@@ -2316,7 +2331,7 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
                                  const void*,                                                    \
                                  void*, hipStream_t);                                            \
     template void launch_post_levels<R>(const PostJob*, const PostJob*, const int*, int, const Dims&,            \
-                                        const void*, hipStream_t);                               \
+                                        const void*, hipStream_t, TileList, int);                \
     template void launch_pre<R>(const PreJob*, const PreJob*, int, const Dims&, int, void*,      \
                                 const void*, void*, const double*, const double*, double*,       \
                                 hipStream_t);                                                    \
@@ -2329,7 +2344,7 @@ void launch_transpose(const void* const* src, void* const* dst, int count, int S
                                           double*, int, double*, hipStream_t);                  \
     template void launch_pre_levels<R>(const PreJob*, const int*, int, const Dims&, const void*, \
                                        void*, const double*, const double*, double*,             \
-                                       hipStream_t);                                             \
+                                       hipStream_t, TileList, int);                              \
     template int launch_root_rows<R>(const void*, const short*, const double*, const double*,    \
                                      const double*, const double*, double*, double*,            \
                                      const Dims&, hipStream_t);                                 \

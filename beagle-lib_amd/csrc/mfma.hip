@@ -438,12 +438,14 @@ __device__ __forceinline__ void rescale_store(floatx16 (&v)[MB], const ScaleOut&
 // (FP32) or k steps of 16 (BF16).
 template <int MB, int KX, bool AUTO, int MODE>
 __global__ __launch_bounds__(512) void k_postm(const PostJob* __restrict__ jobs, Dims d,
-                                               const float* __restrict__ pack) {
+                                               const float* __restrict__ pack, TileList w) {
     __shared__ float red[8][32];
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int l = threadIdx.x & 63, n = l & 31, h = l >> 5;
-    const PostJob& J = jobs[blockIdx.y];
-    const int tp0 = blockIdx.x * PTILE;
+    int job, item;
+    tile_item(w, job, item);
+    const PostJob& J = jobs[job];
+    const int tp0 = ((w.first ? J.p0 / PTILE : 0) + item) * PTILE;
     if (tp0 >= J.p1 || tp0 + PTILE <= J.p0) return;    // uniform over the workgroup
     const int p = tp0 + n;
     const bool in = p >= J.p0 && p < J.p1;
@@ -480,15 +482,15 @@ __global__ __launch_bounds__(512) void k_postm(const PostJob* __restrict__ jobs,
 
 
 void launch_post_mfma(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
-                      int variant, hipStream_t st) {
-    if (njobs <= 0) return;
-    const dim3 grid(d.Ppad / PTILE, njobs), block(64 * d.C);
+                      int variant, hipStream_t st, TileList w, int items) {
+    if (njobs <= 0 || (w.first && items <= 0)) return;
+    const dim3 grid = w.first ? dim3(items) : dim3(d.Ppad / PTILE, njobs), block(64 * d.C);
     const int KG = mfma_kgroups(d.S);
 #define POSTM_V(M, K, A)                                                                       \
     if (variant == 1)                                                                          \
-        hipLaunchKernelGGL((k_postm<M, K, A, 1>), grid, block, 0, st, djobs, d, pack);        \
+        hipLaunchKernelGGL((k_postm<M, K, A, 1>), grid, block, 0, st, djobs, d, pack, w);     \
     else                                                                                       \
-        hipLaunchKernelGGL((k_postm<M, K, A, 0>), grid, block, 0, st, djobs, d, pack);
+        hipLaunchKernelGGL((k_postm<M, K, A, 0>), grid, block, 0, st, djobs, d, pack, w);
 #define POSTM(M, K)                                                                            \
     case K:                                                                                    \
         if (d.autoScale) { POSTM_V(M, K, true) } else { POSTM_V(M, K, false) }                 \
@@ -503,15 +505,15 @@ void launch_post_mfma(const PostJob* djobs, int njobs, const Dims& d, const floa
 }
 
 void launch_post_bf16(const PostJob* djobs, int njobs, const Dims& d, const float* pack,
-                      int terms, hipStream_t st) {
-    if (njobs <= 0) return;
-    const dim3 grid(d.Ppad / PTILE, njobs), block(64 * d.C);
+                      int terms, hipStream_t st, TileList w, int items) {
+    if (njobs <= 0 || (w.first && items <= 0)) return;
+    const dim3 grid = w.first ? dim3(items) : dim3(d.Ppad / PTILE, njobs), block(64 * d.C);
     const int KT = mfma_ksteps16(d.S);
 #define POSTB_V(M, K, A)                                                                       \
     if (terms == 4)                                                                            \
-        hipLaunchKernelGGL((k_postm<M, K, A, 4>), grid, block, 0, st, djobs, d, pack);        \
+        hipLaunchKernelGGL((k_postm<M, K, A, 4>), grid, block, 0, st, djobs, d, pack, w);     \
     else                                                                                       \
-        hipLaunchKernelGGL((k_postm<M, K, A, 3>), grid, block, 0, st, djobs, d, pack);
+        hipLaunchKernelGGL((k_postm<M, K, A, 3>), grid, block, 0, st, djobs, d, pack, w);
 #define POSTB(M, K)                                                                            \
     case K:                                                                                    \
         if (d.autoScale) { POSTB_V(M, K, true) } else { POSTB_V(M, K, false) }                 \
@@ -606,12 +608,14 @@ __device__ __forceinline__ void load_acc(const Tile& t, unsigned vo, floatx16 (&
 // or 4 products (KX = k steps of 16).
 template <int MB, int KX, bool AUTO, int MODE>
 __global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, Dims d,
-                                              const float* __restrict__ pack) {
+                                              const float* __restrict__ pack, TileList w) {
     __shared__ float red[8][32];
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int l = threadIdx.x & 63, n = l & 31, h = l >> 5;
-    const PreJob& J = jobs[blockIdx.y];
-    const int tp0 = blockIdx.x * PTILE;
+    int job, item;
+    tile_item(w, job, item);
+    const PreJob& J = jobs[job];
+    const int tp0 = ((w.first ? J.p0 / PTILE : 0) + item) * PTILE;
     if (tp0 >= J.p1 || tp0 + PTILE <= J.p0) return;    // uniform over the workgroup
     const int p = tp0 + n;
     const bool in = p >= J.p0 && p < J.p1;
@@ -641,11 +645,11 @@ __global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, D
 }
 
 void launch_pre_mfma(const PreJob* djobs, int njobs, const Dims& d, const float* pack,
-                     int terms, hipStream_t st) {
-    if (njobs <= 0) return;
-    const dim3 grid(d.Ppad / PTILE, njobs), block(64 * d.C);
+                     int terms, hipStream_t st, TileList w, int items) {
+    if (njobs <= 0 || (w.first && items <= 0)) return;
+    const dim3 grid = w.first ? dim3(items) : dim3(d.Ppad / PTILE, njobs), block(64 * d.C);
 #define PREM_V(M, K, A, MODE)                                                                  \
-    hipLaunchKernelGGL((k_prem<M, K, A, MODE>), grid, block, 0, st, djobs, d, pack);
+    hipLaunchKernelGGL((k_prem<M, K, A, MODE>), grid, block, 0, st, djobs, d, pack, w);
 #define PREM(M, K, MODE)                                                                       \
     case K:                                                                                    \
         if (d.autoScale) { PREM_V(M, K, true, MODE) } else { PREM_V(M, K, false, MODE) }       \

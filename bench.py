@@ -36,6 +36,12 @@ partitions: 48 partitions with 4-, 20- and 61-state models and their own
 trees, FP32 logL + gradient per partition, eager calls against HIP-graph
 replays launched across the instances' streams.
 
+by_partition: the same partitions on one shared tree, one instance per
+state count holding its partitions as pattern ranges (the calls by
+partition, one launch per dependency level over a work list of (operation,
+pattern tile) items), against the full tiles x operations grid and against
+one instance per partition.
+
 roofline: the dominant kernel of the value path, the FP32 node kernel of
 the one-call gradient (pre-order partials + branch derivatives + rate-
 matrix sums of the 512-taxon tree's 511 node jobs, one launch per
@@ -569,6 +575,134 @@ def partitions_bench(lib, rounds):
             "graph_results_identical_to_eager": bool(same), "rounds": rounds}
 
 
+def by_partition_bench(lib, rounds):
+    """The 48 partitions' state and pattern counts on one shared 128-taxon
+    Yule tree, each partition with its own model (GTR, exchangeabilities
+    ~Exp(1) and pi ~Dirichlet(1) for 20 states, GY94), Gamma(4), FP32.
+    By partition: one instance per state count holding its 16 partitions as
+    pattern ranges; a round = the partitions' transition matrices, one
+    beagleUpdatePartialsByPartition over every partition's operations (one
+    launch per dependency level, a work list of (operation, pattern tile)
+    items), one beagleUpdatePrePartialsByPartition, then root
+    log-likelihoods by partition.  Against one instance per partition
+    making the same calls, and the by-partition instances with the work
+    lists switched off (grid of every tile x every operation)."""
+    import dataclasses
+    import os
+    import numpy as np
+    import torch
+    from beagle_amd import api, driver, synth
+    rng = np.random.default_rng(SEED + 200)
+    tree = synth.make_tree(rng, 128, "yule", 0.1)
+    probs = []
+    for k in range(PARTS["count"]):
+        S = PARTS["states"][k % 3]
+        P = int(round(np.exp(rng.uniform(np.log(PARTS["pmin"]), np.log(PARTS["pmax"])))))
+        probs.append(dataclasses.replace(synth.make_problem(SEED + 3000 + k, 128, P, S=S), tree=tree))
+    n = tree.n
+    nodes, nb = 2 * n - 1, 2 * n - 2
+    post = np.asarray(synth.postorder_ops(tree), np.int32)
+    flags = api.FLAG_EIGEN_REAL | api.FLAG_SCALING_AUTO | api.FLAG_PREORDER_TRANSPOSE_AUTO | \
+        api.FLAG_COMPUTATION_ASYNCH
+
+    def build(group):
+        K = len(group)
+        S = group[0].S
+        sizes = [g.P for g in group]
+        inst = lib.create(n, (n - 1) + nodes, n, S, sum(sizes), K, K * nb, group[0].C, n,
+                          preference=flags, requirement=api.FLAG_PRECISION_SINGLE)
+        for t in range(n):
+            inst.set_tip_states(t, np.concatenate([g.tips[t] for g in group]))
+        for k, g in enumerate(group):
+            m = g.model
+            inst.set_eigen(k, m.evec.reshape(-1), m.ivec.reshape(-1), m.evals)
+            inst.set_state_frequencies(k, m.freqs)
+        inst.set_category_weights(0, group[0].weights)
+        inst.set_category_rates(group[0].rates)
+        inst.set_pattern_weights(np.concatenate([g.pattern_weights for g in group]))
+        inst.set_pattern_partitions(K, np.repeat(np.arange(K), sizes).astype(np.int32))
+        pops, qops = [], []
+        pre = np.asarray(synth.preorder_ops(tree, nodes), np.int32)
+        for k in range(K):
+            a = post.copy()
+            a[:, [4, 6]] += k * nb
+            pops.append(np.hstack([a, np.full((len(a), 1), k), np.full((len(a), 1), -1)]))
+            b = pre.copy()
+            b[:, [4, 6]] += k * nb
+            qops.append(np.hstack([b, np.full((len(b), 1), k), np.full((len(b), 1), -1)]))
+        # root pre-order partials: each partition's frequencies over its range
+        root_pre = np.concatenate([np.tile(g.model.freqs, g.P) for g in group])
+        inst.set_partials(nodes + tree.root, np.tile(root_pre, group[0].C))
+        return inst, np.vstack(pops).astype(np.int32), np.vstack(qops).astype(np.int32)
+
+    def round_by_partition(insts):
+        for inst, group, pops, qops in insts:
+            for k, g in enumerate(group):
+                inst.update_transition_matrices(k, np.arange(nb, dtype=np.int32) + k * nb, None, None,
+                                                tree.lengths[:nb])
+            inst.update_partials_by_partition(pops)
+            inst.update_pre_partials_by_partition(qops)
+        out = []
+        for inst, group, pops, qops in insts:
+            K = len(group)
+            by, _ = inst.root_log_likelihood_by_partition([tree.root] * K, [0] * K, list(range(K)),
+                                                          [-1] * K, list(range(K)))
+            out.append(by)
+        return np.concatenate(out)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, r
+
+    res = {}
+    for lists in ("1", "0"):
+        os.environ["BEAGLE_AMD_TILE_LISTS"] = lists
+        insts = []
+        for S in PARTS["states"]:
+            group = [g for g in probs if g.S == S]
+            inst, pops, qops = build(group)
+            insts.append((inst, group, pops, qops))
+        t, by = timed(lambda: round_by_partition(insts), rounds)
+        res[lists] = (t, by)
+        for inst, *_ in insts:
+            inst.finalize()
+    os.environ.pop("BEAGLE_AMD_TILE_LISTS", None)
+    evs = [driver.Evaluator(lib, g, precision="single", scaling="auto",
+                            extra_flags=api.FLAG_COMPUTATION_ASYNCH) for g in probs]
+    order = [i for S in PARTS["states"] for i, g in enumerate(probs) if g.S == S]
+
+    def round_per_instance():
+        for i in order:
+            evs[i].update_matrices()
+            evs[i].post_order()
+            evs[i].pre_order()
+        return np.array([evs[i].root() for i in order])
+
+    t_inst, by_inst = timed(round_per_instance, rounds)
+    for ev in evs:
+        ev.close()
+    t_list, by_list = res["1"]
+    t_grid, by_grid = res["0"]
+    return {"workload": "48 partitions (state and pattern counts as the partitions workload) on one "
+                        "shared 128-taxon Yule tree, own model each, Gamma(4), FP32: matrices, "
+                        "post-order, pre-order and root logL by partition",
+            "total_patterns": int(sum(g.P for g in probs)),
+            "by_partition_ms_per_round": t_list * 1e3,
+            "by_partition_full_grid_ms_per_round": t_grid * 1e3,
+            "instance_per_partition_ms_per_round": t_inst * 1e3,
+            "work_list_speedup_vs_full_grid": t_grid / t_list,
+            "by_partition_speedup_vs_instance_per_partition": t_inst / t_list,
+            "logl_identical_with_and_without_lists": bool(np.array_equal(by_list, by_grid)),
+            "max_rel_diff_vs_instance_per_partition": float(np.max(np.abs(by_list - by_inst) /
+                                                                   np.abs(by_inst))),
+            "rounds": rounds}
+
+
 def cpu_baseline_codon(budget_s):
     """The reference CPU plugin (threaded SSE, double) on the codon workload:
     log-likelihood evaluations (matrices, post-order, root) of a pattern
@@ -878,7 +1012,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-only", action="store_true")
-    ap.add_argument("--only", choices=("headline", "tree128", "codon", "partitions", "roofline",
+    ap.add_argument("--only", choices=("headline", "tree128", "codon", "partitions", "by_partition", "roofline",
                                        "plain-post"),
                     help="run one workload alone (for profiling): the metric's adjoint "
                          "workload, the 128-taxon workload, the codon or partition workload, "
@@ -919,6 +1053,9 @@ def main():
     if args.only == "partitions":
         print(json.dumps({"partitions": partitions_bench(lib, max(args.steps // 10, 3))}))
         return
+    if args.only == "by_partition":
+        print(json.dumps({"by_partition": by_partition_bench(lib, max(args.steps // 10, 3))}))
+        return
     if args.only == "plain-post":
         probs = problems()
         ms, gbs = plain_post_timing(lib, probs[0], args.steps)
@@ -955,6 +1092,7 @@ def main():
     checkpointed = checkpoint_bench(lib, problems(), max(args.steps // 3, 3))
     codon = codon_bench(lib, args.steps)
     partitions = partitions_bench(lib, max(args.steps // 10, 3))
+    by_partition = by_partition_bench(lib, max(args.steps // 10, 3))
     if args.calibrate_copy:
         calibrate_copy()
 
@@ -1017,6 +1155,7 @@ def main():
         "checkpointed": checkpointed,
         "codon": codon,
         "partitions": partitions,
+        "by_partition": by_partition,
     }
     print(json.dumps(out))
 

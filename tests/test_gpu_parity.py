@@ -478,6 +478,48 @@ def test_mfma_post_order_partitions(amd, oracle):
     assert rel(out[0][1], out[1][1]) < 1e-6
 
 
+@pytest.mark.parametrize("S,traversal", [(4, "1"), (4, None), (20, None), (61, None)])
+def test_partition_work_lists_match_oracle(amd, oracle, monkeypatch, S, traversal):
+    """Operations by partition over ten uneven partitions (one without
+    patterns, some smaller than a tile, ranges that start and end inside
+    tiles): every level launch over pattern ranges runs one workgroup per
+    (operation, tile) item of its work list.  Post-order, root
+    log-likelihoods by partition and pre-order partials by partition against
+    the oracle: logL 1e-6 relative (per partition and total), pre-order
+    partials 1e-5 of each row's maximum.  traversal "1" forces the 4-state
+    level plan (batched levels)."""
+    if traversal:
+        monkeypatch.setenv("BEAGLE_AMD_TRAVERSAL", traversal)
+    sizes = [0, 3, 40, 70, 1, 150, 33, 64, 5, 200]
+    K = len(sizes)
+    prob = synth.make_problem(41 + S, 9, sum(sizes), S=S, missing=0.05, mean_length=0.2)
+    parts = np.repeat(np.arange(K), sizes).astype(np.int32)
+    out = []
+    for lib, prec in ((amd, "single"), (oracle, "double")):
+        ev = driver.Evaluator(lib, prob, precision=prec, matrices="explicit")
+        ev.update_matrices()
+        ev.inst.set_pattern_partitions(K, parts)
+        pops = [list(o) + [q, -1] for q in range(K) for o in synth.postorder_ops(prob.tree)]
+        ev.inst.update_partials_by_partition(np.array(pops, np.int32))
+        r = prob.tree.root
+        by, tot = ev.inst.root_log_likelihood_by_partition([r] * K, [0] * K, [0] * K, [-1] * K,
+                                                           list(range(K)))
+        ev.set_root_pre()
+        pre = ev.preorder_operations()
+        qops = [list(o) + [q, -1] for q in range(K) for o in pre]
+        ev.inst.update_pre_partials_by_partition(np.array(qops, np.int32))
+        pp = [np.asarray(ev.inst.get_partials(ev.pre_base + v)).reshape(-1, S)
+              for v in ev.branch_nodes()[:4]]
+        out.append((np.asarray(by), tot, pp))
+        ev.close()
+    (bg, tg, pg), (bo, to, po) = out
+    assert rel(tg, to) < 1e-6
+    close(bg, bo, 1e-6, 0)
+    for a, b in zip(pg, po):
+        keep = b.max(axis=1) > 1e-30
+        close_rows(a[keep], b[keep], S, 1e-5)
+
+
 def test_matrices_match_oracle(amd, oracle):
     prob = synth.make_problem(3, 8, 100, S=20)
     mats = []
