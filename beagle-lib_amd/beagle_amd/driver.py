@@ -38,6 +38,7 @@ class Evaluator:
                  matrices="eigen", pre_scaling=False):
         self.lib = lib
         self.prob = prob
+        self.precision = precision
         self.scaling = scaling
         self.preorder = preorder
         self.transpose = transpose
@@ -205,13 +206,16 @@ class Evaluator:
 
     def fused_operations(self, keep_tips=False):
         """Pre-order operations and matching post-order buffers for the
-        derivative extension; for 4 states the tips' operations have no
+        derivative extension; for 4 states, and 5-64 states in single
+        precision with tips as states, the tips' operations have no
         destination unless keep_tips (no later operation reads them)."""
         key = bool(keep_tips)
         if self._pre is None or self._pre[0] != key:
             ops = self._transposed_self(synth.preorder_ops(self.prob.tree, self.pre_base))
             post = (ops[:, 0] - self.pre_base).astype(np.int32)
-            if not keep_tips and self.prob.S == 4 and self.prob.C <= 8:
+            S, C = self.prob.S, self.prob.C
+            if not keep_tips and C <= 8 and (S == 4 or (4 < S <= 64 and self.precision == "single"
+                                                        and self.tips_as == "states")):
                 ops[post < self.n, 0] = api.OP_NONE
             n = len(post)
             self._pre = (key, ops, post, np.full(n, self._deriv_index(False), np.int32),

@@ -1352,12 +1352,25 @@ size_t Instance::packFloats() const {
 // MFMA pre-order: P_self^T packed for the register-resident second product
 // (from the stored transpose when the client passes transposed self
 // matrices), P_sib as a post-order child; one launch per dependency level.
-void Instance::launchPreMfma(const int* ops, int stride, int count) {
+void Instance::launchPreMfma(const int* ops, int stride, int count, const TipDeriv* td) {
     const size_t per = packFloats();
     std::vector<PackJob> pj(2 * (size_t)count);
     const int tin = preTransposed_ ? PACK_TRANSPOSED_INPUT : 0;
+    const size_t mstride = matrixBytes_ / realSize();
     for (int o = 0; o < count; ++o) {
         PreJob& J = hPre_[o];
+        J.pkT = J.pkM1 = J.pkM2 = -1;
+        if (td && !dp_ && dStates_[td->postIdx[o]]) {
+            // the tip's columns of P_self and of P_self D1, P_self D2
+            J.sself = dStates_[td->postIdx[o]];
+            const int base = (int)pj.size();
+            J.pkT = (int)(base * per);
+            J.pkM1 = (int)((base + 1) * per);
+            J.pkM2 = (int)((base + 2) * per);
+            pj.push_back(PackJob{J.moSelf, J.pkT, PACK_TIP_COLUMN | tin, 0});
+            pj.push_back(PackJob{J.moSelf, J.pkM1, PACK_TIP_PRODUCT | tin, (int)(td->d1[o] * mstride)});
+            pj.push_back(PackJob{J.moSelf, J.pkM2, PACK_TIP_PRODUCT | tin, (int)(td->d2[o] * mstride)});
+        }
         J.pkSelf = (int)(2 * o * per);
         J.pkSib = (int)((2 * o + 1) * per);
         const bool bf = splitBf16_ && !dp_;
@@ -1391,12 +1404,14 @@ void Instance::launchPreMfma(const int* ops, int stride, int count) {
     if (!dp_ && tileLists_) levelTileLists(sorted.data(), start.data(), nl, dims_.P, PTILE, lints, loff, litems);
     const int* dl = lints.empty() ? nullptr
                                   : static_cast<const int*>(uploadJobs(lints.data(), lints.size() * sizeof(int)));
+    const double* cw = td ? dCatWeights_ + (size_t)td->wIdx * dims_.C : nullptr;
     for (int l = 0; l < nl; ++l) {
         const int nj = start[l + 1] - start[l];
         const TileList w{dl && loff[l] >= 0 ? dl + loff[l] : nullptr, nj};
         if (dp_) launch_pre_mfma64(d + start[l], nj, dims_, pack, stream_);
         else launch_pre_mfma(d + start[l], nj, dims_, pack, splitBf16_ ? bf16Terms_ : 0, stream_,
-                             w, dl ? litems[l] : 0);
+                             w, dl ? litems[l] : 0, cw, dPatternWeights_,
+                             td ? td->partial : nullptr);
     }
 }
 
@@ -1404,52 +1419,63 @@ void Instance::launchPreMfma(const int* ops, int stride, int count) {
 // derivative matrices, enqueued: *dSums receives per node NR = 2 or 3 sums
 // (sum w r1, sum w r1^2, sum w r2), interleaved; sites (one matrix only, may
 // be null) the per-pattern ratios.
+// skip[n] != 0: entry n's sums are already in `work` (tip branches of the
+// fused gradient, written by the pre-order kernel); work: that buffer, sized
+// for all entries, else one is taken here.
 int Instance::enqueueDerivMfma(const int* postIdx, const int* preIdx, const int* d1,
                                const int* d2, int wIdx, int count, double* sites,
-                               double** dSums) {
+                               double** dSums, const char* skip, double* work) {
     const size_t per = packFloats();
     const int nd = d2 ? 2 : 1, NR = d2 ? 3 : 2;
     const size_t mstride = matrixBytes_ / realSize();
     const int tin = preTransposed_ ? PACK_TRANSPOSED_INPUT : 0;
-    std::vector<DerivJob> jobs(count);
-    std::vector<PackJob> pj((size_t)nd * count);
+    std::vector<DerivJob> jobs;
+    std::vector<PackJob> pj;
+    jobs.reserve(count);
+    pj.reserve((size_t)nd * count);
     for (int n = 0; n < count; ++n) {
+        if (skip && skip[n]) continue;
         const int po = postIdx[n], pr = preIdx[n];
         if (!validBuffer(po) || !validBuffer(pr) || !dPartials_[pr] || !validMatrix(d1[n]) ||
             (d2 && !validMatrix(d2[n])) || (!dStates_[po] && !dPartials_[po]))
             return BEAGLE_ERROR_OUT_OF_RANGE;
-        DerivJob& J = jobs[n];
+        jobs.push_back(DerivJob{});
+        DerivJob& J = jobs.back();
         J.pre = dPartials_[pr];
         J.states = dStates_[po];
         J.post = dStates_[po] ? nullptr : dPartials_[po];
         J.row = n;
         const int kind =
             (J.states ? PACK_ROWS : splitBf16_ && !dp_ ? PACK_OPERAND_BF16 : PACK_OPERAND) | tin;
-        J.pk1 = (int)((size_t)nd * n * per);
-        pj[(size_t)nd * n] = PackJob{(int)(d1[n] * mstride), J.pk1, kind};
+        J.pk1 = (int)(pj.size() * per);
+        pj.push_back(PackJob{(int)(d1[n] * mstride), J.pk1, kind, 0});
         J.pk2 = J.pk1;
         if (d2) {
-            J.pk2 = (int)(((size_t)nd * n + 1) * per);
-            pj[(size_t)nd * n + 1] = PackJob{(int)(d2[n] * mstride), J.pk2, kind};
+            J.pk2 = (int)(pj.size() * per);
+            pj.push_back(PackJob{(int)(d2[n] * mstride), J.pk2, kind, 0});
         }
     }
+    const int njobs = (int)jobs.size();
     float* pack = static_cast<float*>(
         grow(reinterpret_cast<void*&>(dPack_), packCap_, pj.size() * per * sizeof(float)));
     const int tileP = dp_ ? 16 : PTILE;          // patterns per derivative tile
     const int ntiles = (dims_.P + tileP - 1) / tileP;
     const size_t nPart = (size_t)NR * count * ntiles;
-    double* wk = reinterpret_cast<double*>(
-        grow(dWork_, workCap_, (nPart + (size_t)NR * count) * sizeof(double)));
-    auto dpj = reinterpret_cast<const PackJob*>(uploadJobs(pj.data(), pj.size() * sizeof(PackJob)));
-    if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
-    else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
-    auto dj = reinterpret_cast<const DerivJob*>(uploadJobs(jobs.data(), count * sizeof(DerivJob)));
-    if (dp_)
-        launch_deriv_mfma64(dj, count, nd, dims_, pack, dCatWeights_ + (size_t)wIdx * dims_.C,
-                            dPatternWeights_, sites, wk, stream_);
-    else
-        launch_deriv_mfma(dj, count, nd, dims_, pack, dCatWeights_ + (size_t)wIdx * dims_.C,
-                          dPatternWeights_, sites, wk, splitBf16_ ? bf16Terms_ : 0, stream_);
+    const size_t need = (nPart + (size_t)NR * count) * sizeof(double);
+    if (work && (work != dWork_ || need > workCap_)) return BEAGLE_ERROR_GENERAL;
+    double* wk = work ? work : reinterpret_cast<double*>(grow(dWork_, workCap_, need));
+    if (njobs > 0) {
+        auto dpj = reinterpret_cast<const PackJob*>(uploadJobs(pj.data(), pj.size() * sizeof(PackJob)));
+        if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+        else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+        auto dj = reinterpret_cast<const DerivJob*>(uploadJobs(jobs.data(), njobs * sizeof(DerivJob)));
+        if (dp_)
+            launch_deriv_mfma64(dj, njobs, nd, dims_, pack, dCatWeights_ + (size_t)wIdx * dims_.C,
+                                dPatternWeights_, sites, wk, stream_);
+        else
+            launch_deriv_mfma(dj, njobs, nd, dims_, pack, dCatWeights_ + (size_t)wIdx * dims_.C,
+                              dPatternWeights_, sites, wk, splitBf16_ ? bf16Terms_ : 0, stream_);
+    }
     launch_reduce_rows(wk, ntiles, NR * count, wk + nPart, stream_);
     HIPCHECK(hipGetLastError());
     *dSums = wk + nPart;
@@ -1585,16 +1611,40 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
     if (!validEigen(wIdx) || hCatWeights_[wIdx].empty()) return BEAGLE_ERROR_OUT_OF_RANGE;
     const bool fused = dims_.S == 4 && dims_.C <= 8 && !preTransposed_ && d2Idx != nullptr;
     if (!fused && mfmaAdjoint() && d2Idx != nullptr) {
-        // S > 4: MFMA pre-order, then both derivative matrices in one pass
-        for (int o = 0; o < count; ++o)
-            if (ops[(size_t)o * BEAGLE_OP_COUNT] == BEAGLE_OP_NONE) return BEAGLE_ERROR_OUT_OF_RANGE;
-        int rc = updatePrePartials(ops, count, BEAGLE_OP_NONE, false);
+        // S > 4: MFMA pre-order, then both derivative matrices in one pass.
+        // FP32 / split BF16: a tip branch's sums come from the pre-order
+        // kernel, from the operands of the tip's pre-order partials, which a
+        // NONE destination leaves unwritten; the derivative pass takes the
+        // other branches.
+        std::vector<char> tip(count, 0);
+        for (int o = 0; o < count; ++o) {
+            const int po = postIdx[o];
+            tip[o] = !dp_ && validBuffer(po) && dStates_[po] != nullptr;
+            if (ops[(size_t)o * BEAGLE_OP_COUNT] == BEAGLE_OP_NONE && !tip[o])
+                return BEAGLE_ERROR_OUT_OF_RANGE;
+            if (!validMatrix(d1Idx[o]) || !validMatrix(d2Idx[o])) return BEAGLE_ERROR_OUT_OF_RANGE;
+        }
+        int rc = buildPreJobs(ops, count, false, true);
         if (rc != BEAGLE_SUCCESS || count == 0) return rc;
-        if (!(outD1 || outD1Sq || outD2)) return BEAGLE_SUCCESS;
+        const bool want = outD1 || outD1Sq || outD2;
+        // the derivative kernels' tiles: 32 patterns (FP32), 16 (FP64)
+        const int tileP = dp_ ? 16 : PTILE;
+        const size_t nPart = (size_t)3 * count * ((dims_.P + tileP - 1) / tileP);
+        double* wk = want ? reinterpret_cast<double*>(grow(dWork_, workCap_,
+                                                           (nPart + (size_t)3 * count) * sizeof(double)))
+                          : nullptr;
+        const TipDeriv td{postIdx, d1Idx, d2Idx, wIdx, wk};
+        launchPreMfma(ops, BEAGLE_OP_COUNT, count, want && !dp_ ? &td : nullptr);
+        HIPCHECK(hipGetLastError());
+        if (!want) {
+            finishCall();
+            return BEAGLE_SUCCESS;
+        }
         std::vector<int> pre(count);
         for (int o = 0; o < count; ++o) pre[o] = ops[(size_t)o * BEAGLE_OP_COUNT];
         double* sums = nullptr;
-        rc = enqueueDerivMfma(postIdx, pre.data(), d1Idx, d2Idx, wIdx, count, nullptr, &sums);
+        rc = enqueueDerivMfma(postIdx, pre.data(), d1Idx, d2Idx, wIdx, count, nullptr, &sums,
+                              tip.data(), wk);
         if (rc != BEAGLE_SUCCESS) return rc;
         return deliverDerivativeSums(sums, count, outD1, outD1Sq, outD2);
     }

@@ -238,9 +238,19 @@ private:
     void launchPostMfma(const int* ops, int stride, int count);
     bool mfmaAdjoint() const;
     size_t packFloats() const;
-    void launchPreMfma(const int* ops, int stride, int count);
+    // tip branches of the fused gradient on the MFMA path: their derivative
+    // sums formed by the pre-order kernel (rows of `partial`)
+    struct TipDeriv {
+        const int* postIdx;
+        const int* d1;
+        const int* d2;
+        int wIdx;
+        double* partial;
+    };
+    void launchPreMfma(const int* ops, int stride, int count, const TipDeriv* td = nullptr);
     int enqueueDerivMfma(const int* postIdx, const int* preIdx, const int* d1, const int* d2,
-                         int wIdx, int count, double* sites, double** dSums);
+                         int wIdx, int count, double* sites, double** dSums,
+                         const char* skip = nullptr, double* work = nullptr);
     int deliverDerivativeSums(const double* sums, int count, double* outD1, double* outD1Sq,
                               double* outD2);
 

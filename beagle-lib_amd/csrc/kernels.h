@@ -62,6 +62,9 @@ struct PreJob {
     int kSelf, moD1, moD2;
     int oidx;                  // row of this operation in the derivative output
     int pkSelf, pkSib;         // MFMA path (S > 4): packed P_self^T and P_sib
+    // MFMA path, tip branch of the fused gradient: PACK_TIP_COLUMN of P_self
+    // and PACK_TIP_PRODUCT of P_self with both derivative matrices (-1: none)
+    int pkT, pkM1, pkM2;
 };
 
 // The two pre-order operations of one node's children a (slot 0) and b
@@ -189,6 +192,7 @@ struct TileList {
 struct PackJob {
     int src, dst;
     int kind;      // PACK_OPERAND: partials child, PACK_ROWS: tip-state child
+    int src2;      // PACK_TIP_PRODUCT: the derivative matrix's element offset
 };
 enum PackKind : int {
     PACK_OPERAND = 0,       // A operand of M x (partials): rows i, inner states k
@@ -196,6 +200,8 @@ enum PackKind : int {
     PACK_OPERAND_BF16 = 2,  // PACK_OPERAND as hi/lo BF16 parts
     PACK_PRE_SELF = 3,      // A operand of M^T u with u held in accumulator registers
     PACK_PRE_SELF_BF16 = 4, // PACK_PRE_SELF for the BF16 matrix cores, as hi/lo parts
+    PACK_TIP_COLUMN = 5,    // P[., s mod S] by state s = 0..S (tip-branch denominators)
+    PACK_TIP_PRODUCT = 6,   // (P D)[., s] by state s = 0..S (tip-branch numerators)
     // added to a kind: the stored matrix is the transpose (the reference GPU
     // convention for pre-order self and derivative matrices, S > 4)
     PACK_TRANSPOSED_INPUT = 8,
@@ -256,8 +262,13 @@ struct DerivJob {
 // njobs): the jobs' pkSelf (PACK_PRE_SELF) and pkSib (PACK_OPERAND /
 // PACK_ROWS) offsets in the pack slab.  terms = 3 or 4: split-BF16 matrix
 // cores (PACK_PRE_SELF_BF16, PACK_OPERAND_BF16), 0: FP32 matrix cores.
+// Jobs with pkT >= 0 (sself: the tip's states) also write their branch's
+// three derivative tile sums to partial (rows oidx * 3 + k, ceil(P/32)
+// tiles), weighted by cw (categories) and pw (patterns).
 void launch_pre_mfma(const PreJob* djobs, int njobs, const Dims& d, const float* pack,
-                     int terms, hipStream_t st, TileList w = {nullptr, 0}, int items = 0);
+                     int terms, hipStream_t st, TileList w = {nullptr, 0}, int items = 0,
+                     const double* cw = nullptr, const double* pw = nullptr,
+                     double* partial = nullptr);
 // Edge derivatives (S > 4, FP32 partials): nd = 1 or 2 derivative matrices;
 // per job and 32-pattern tile, partial sums rows [row * NR + k][tile] (NR =
 // 2: sum w r1, sum w r1^2; NR = 3: also sum w r2) with r = N / D over

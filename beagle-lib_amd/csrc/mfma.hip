@@ -148,6 +148,29 @@ __global__ __launch_bounds__(WG) void k_pack_mats(const PackJob* __restrict__ jo
             }
             v = __builtin_bit_cast(float, (unsigned)half[0] | ((unsigned)half[1] << 16));
         }
+    } else if (kind == PACK_TIP_COLUMN || kind == PACK_TIP_PRODUCT) {
+        // tip branch of the fused gradient, [state s (0..S)][row j]:
+        // COLUMN P[j][s mod S]; PRODUCT (P D)[j][s] = sum_i P[j][i] D[i][s]
+        // (FP64 sum; s = S is the derivative matrix's pad column, 0 for a
+        // transposed input as in the rows-by-state pack)
+        const int W = MB * 32;
+        if (e < (S + 1) * W) {
+            const int s = e / W, j = e - s * W;
+            if (j < S) {
+                if (kind == PACK_TIP_COLUMN) {
+                    v = at(j, s % S);
+                } else {
+                    const R* D = mats + J.src2 + (long)c * S * (S + 1);
+                    double acc = 0.0;
+                    for (int i = 0; i < S; ++i) {
+                        const double dv = tin ? (s < S ? (double)D[s * (S + 1) + i] : 0.0)
+                                              : (double)D[i * (S + 1) + s];
+                        acc += (double)at(j, i) * dv;
+                    }
+                    v = (float)acc;
+                }
+            }
+        }
     } else if (kind == PACK_PRE_SELF_BF16) {
         // M^T u with u in the accumulator layout on v_mfma_f32_32x32x16_bf16:
         // [mo][instruction s][part: hi, lo][lane][8 BF16]; slot e of lane
@@ -604,12 +627,21 @@ __device__ __forceinline__ void load_acc(const Tile& t, unsigned vo, floatx16 (&
         for (int r = 0; r < 16; ++r) v[mb][r] = tile_load(t, mb * 32 + 8 * (r >> 2) + (r & 3), vo);
 }
 
+__device__ __forceinline__ double half_wave_sum(double v) {
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
 // MODE 0: FP32 matrix cores (KX = k groups of 8); 3, 4: split BF16 with 3
 // or 4 products (KX = k steps of 16).
 template <int MB, int KX, bool AUTO, int MODE>
 __global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, Dims d,
-                                              const float* __restrict__ pack, TileList w) {
+                                              const float* __restrict__ pack, TileList w,
+                                              const double* __restrict__ cw,
+                                              const double* __restrict__ pw, double* partial) {
     __shared__ float red[8][32];
+    __shared__ double dred[8][3][32];
     const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int l = threadIdx.x & 63, n = l & 31, h = l >> 5;
     int job, item;
@@ -636,6 +668,54 @@ __global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, D
     }
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) y[0][mb] *= par[mb];
+    if (J.pkT >= 0) {
+        // tip branch of the fused gradient, from u = y[0] without forming
+        // pre(t) = P_t^T u: with the tip's state s, pre(t)_{s mod S} =
+        // u . P_t[., s mod S] and pre(t) . (D_k x) = u . (P_t D_k)[., s]
+        // (BeagleCPUImpl.hpp:2086-2119 forms the same ratios from pre(t));
+        // categories combined in FP64 through LDS, tile sums of w r1, w r1^2
+        // and w r2 as the edge-derivative kernel writes them
+        float s3[3] = {0.f, 0.f, 0.f};
+        const int pks[3] = {J.pkT, J.pkM1, J.pkM2};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            floatx16 t[MB];
+            state_rows<MB>(pack + pks[k] + c * per, J.sself, p, h, t);
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) s3[k] += y[0][mb][r] * t[mb][r];
+            s3[k] += __shfl_xor(s3[k], 32);
+        }
+        const double wc = cw[c];
+        if (h == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dred[c][k][n] = wc * (double)s3[k];
+        }
+        __syncthreads();
+        const int ntp = (d.P + PTILE - 1) / PTILE, tile = tp0 / PTILE;
+        if (c == 0) {
+            double D = 0.0, N1 = 0.0, N2 = 0.0;
+            for (int cc = 0; cc < d.C; ++cc) {
+                D += dred[cc][0][n];
+                N1 += dred[cc][1][n];
+                N2 += dred[cc][2][n];
+            }
+            const bool use = p < d.P && h == 0;
+            const double r1 = use ? N1 / D : 0.0, r2 = use ? N2 / D : 0.0;
+            const double wp = use ? pw[p] : 0.0;
+            const double s1 = half_wave_sum(wp * r1), sq = half_wave_sum(wp * r1 * r1);
+            const double s2 = half_wave_sum(wp * r2);
+            if (l == 0 && tile < ntp) {
+                GLOBAL double* pt = gout<double>(partial);
+                const long row = (long)J.oidx * 3;
+                pt[(row + 0) * ntp + tile] = s1;
+                pt[(row + 1) * ntp + tile] = sq;
+                pt[(row + 2) * ntp + tile] = s2;
+            }
+        }
+    }
+    if (!J.dest) return;   // no pre-order partials kept (uniform)
     const Rsrc aself = rsrc(pack + J.pkSelf + c * per, -1);
     if (MODE == 0) gemm_from_acc<MB>(aself, 16 * l, y[0], y[1]);
     else gemm_from_acc16<MB, MODE>(aself, 16 * l, y[0], y[1]);
@@ -645,11 +725,13 @@ __global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, D
 }
 
 void launch_pre_mfma(const PreJob* djobs, int njobs, const Dims& d, const float* pack,
-                     int terms, hipStream_t st, TileList w, int items) {
+                     int terms, hipStream_t st, TileList w, int items, const double* cw,
+                     const double* pw, double* partial) {
     if (njobs <= 0 || (w.first && items <= 0)) return;
     const dim3 grid = w.first ? dim3(items) : dim3(d.Ppad / PTILE, njobs), block(64 * d.C);
 #define PREM_V(M, K, A, MODE)                                                                  \
-    hipLaunchKernelGGL((k_prem<M, K, A, MODE>), grid, block, 0, st, djobs, d, pack, w);
+    hipLaunchKernelGGL((k_prem<M, K, A, MODE>), grid, block, 0, st, djobs, d, pack, w, cw, pw,   \
+                       partial);
 #define PREM(M, K, MODE)                                                                       \
     case K:                                                                                    \
         if (d.autoScale) { PREM_V(M, K, true, MODE) } else { PREM_V(M, K, false, MODE) }       \
@@ -683,11 +765,6 @@ void launch_pre_mfma(const PreJob* djobs, int njobs, const Dims& d, const float*
 // accumulator layout, per category in FP32, combined over categories in
 // FP64 through LDS.  Tile sums of w r1, w r1^2 (and w r2) per node.
 
-__device__ __forceinline__ double half_wave_sum(double v) {
-#pragma unroll
-    for (int o = 16; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
 
 // MODE and KX as k_prem.
 template <int MB, int KX, int ND, int MODE>

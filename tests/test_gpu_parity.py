@@ -423,15 +423,20 @@ def test_mfma_adjoint_double_matches_oracle(amd, oracle, S, tips_as, scaling, tr
 def test_mfma_fused_gradient_single(amd, oracle, S, transpose, terms):
     """beagleExtCalculateLogLikelihoodAndDerivatives for S > 4 in FP32
     (matrix-core pre-order, then both derivative matrices in one pass; FP32
-    or, terms = 3, split-BF16 matrix cores) against the oracle's
-    beagleUpdatePrePartials + two beagleCalculateEdgeDerivatives."""
+    or, terms = 3, split-BF16 matrix cores; the tips' operations without a
+    destination, their branch sums formed in the pre-order kernel) against
+    the oracle's beagleUpdatePrePartials + two beagleCalculateEdgeDerivatives."""
     prob = synth.make_problem(37 + S, 9, 300, S=S, missing=0.05, mean_length=0.2)
     ev = driver.Evaluator(amd, prob, precision="single", scaling="auto", matrices="explicit",
                           transpose=transpose)
     if terms:
         ev.inst.set_split_bf16(terms)
     lg, g1, g2 = ev.fused_gradient()
+    # tips' pre-order partials kept: the same sums (a tip branch's come from
+    # the pre-order kernel either way)
+    lk, k1, k2 = ev.fused_gradient(keep_tips=True)
     ev.close()
+    assert lk == lg and np.array_equal(k1, g1) and np.array_equal(k2, g2)
     ev = driver.Evaluator(oracle, prob, scaling="manual", matrices="explicit")
     lo, o1, o2 = ev.gradient()
     ev.close()
