@@ -1360,6 +1360,7 @@ void Instance::launchPreMfma(const int* ops, int stride, int count, const TipDer
     for (int o = 0; o < count; ++o) {
         PreJob& J = hPre_[o];
         J.pkT = J.pkM1 = J.pkM2 = -1;
+        const bool bf = splitBf16_ && !dp_;
         if (td && !dp_ && dStates_[td->postIdx[o]]) {
             // the tip's columns of P_self and of P_self D1, P_self D2
             J.sself = dStates_[td->postIdx[o]];
@@ -1370,10 +1371,19 @@ void Instance::launchPreMfma(const int* ops, int stride, int count, const TipDer
             pj.push_back(PackJob{J.moSelf, J.pkT, PACK_TIP_COLUMN | tin, 0});
             pj.push_back(PackJob{J.moSelf, J.pkM1, PACK_TIP_PRODUCT | tin, (int)(td->d1[o] * mstride)});
             pj.push_back(PackJob{J.moSelf, J.pkM2, PACK_TIP_PRODUCT | tin, (int)(td->d2[o] * mstride)});
+        } else if (td && !dp_) {
+            // an internal branch: v's post-order partials, D1 and D2 packed
+            // as P_self is (their transposes applied to pre(v) in registers)
+            J.xself = dPartials_[td->postIdx[o]];
+            const int base = (int)pj.size();
+            J.pkM1 = (int)(base * per);
+            J.pkM2 = (int)((base + 1) * per);
+            const int kd = (bf ? PACK_PRE_SELF_BF16 : PACK_PRE_SELF) | tin;
+            pj.push_back(PackJob{(int)(td->d1[o] * mstride), J.pkM1, kd, 0});
+            pj.push_back(PackJob{(int)(td->d2[o] * mstride), J.pkM2, kd, 0});
         }
         J.pkSelf = (int)(2 * o * per);
         J.pkSib = (int)((2 * o + 1) * per);
-        const bool bf = splitBf16_ && !dp_;
         pj[2 * o] = PackJob{J.moSelf, J.pkSelf, (bf ? PACK_PRE_SELF_BF16 : PACK_PRE_SELF) | tin};
         pj[2 * o + 1] =
             PackJob{J.moSib, J.pkSib, J.ss ? PACK_ROWS : bf ? PACK_OPERAND_BF16 : PACK_OPERAND};
@@ -1612,17 +1622,20 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
     const bool fused = dims_.S == 4 && dims_.C <= 8 && !preTransposed_ && d2Idx != nullptr;
     if (!fused && mfmaAdjoint() && d2Idx != nullptr) {
         // S > 4: MFMA pre-order, then both derivative matrices in one pass.
-        // FP32 / split BF16: a tip branch's sums come from the pre-order
-        // kernel, from the operands of the tip's pre-order partials, which a
-        // NONE destination leaves unwritten; the derivative pass takes the
-        // other branches.
-        std::vector<char> tip(count, 0);
+        // FP32 / split BF16: every branch's sums come from the pre-order
+        // kernel (a tip branch's from the operands of the tip's pre-order
+        // partials, which a NONE destination leaves unwritten; an internal
+        // branch's from pre(v) in registers), the derivative pass only
+        // reduces them.  FP64: the derivative pass.
+        std::vector<char> inPre(count, 0);
         for (int o = 0; o < count; ++o) {
             const int po = postIdx[o];
-            tip[o] = !dp_ && validBuffer(po) && dStates_[po] != nullptr;
-            if (ops[(size_t)o * BEAGLE_OP_COUNT] == BEAGLE_OP_NONE && !tip[o])
+            if (!validBuffer(po) || (!dStates_[po] && !dPartials_[po]) ||
+                !validMatrix(d1Idx[o]) || !validMatrix(d2Idx[o]))
                 return BEAGLE_ERROR_OUT_OF_RANGE;
-            if (!validMatrix(d1Idx[o]) || !validMatrix(d2Idx[o])) return BEAGLE_ERROR_OUT_OF_RANGE;
+            inPre[o] = !dp_;
+            if (ops[(size_t)o * BEAGLE_OP_COUNT] == BEAGLE_OP_NONE && (dp_ || !dStates_[po]))
+                return BEAGLE_ERROR_OUT_OF_RANGE;
         }
         int rc = buildPreJobs(ops, count, false, true);
         if (rc != BEAGLE_SUCCESS || count == 0) return rc;
@@ -1644,7 +1657,7 @@ int Instance::updatePrePartialsWithDerivatives(const int* ops, int count, const 
         for (int o = 0; o < count; ++o) pre[o] = ops[(size_t)o * BEAGLE_OP_COUNT];
         double* sums = nullptr;
         rc = enqueueDerivMfma(postIdx, pre.data(), d1Idx, d2Idx, wIdx, count, nullptr, &sums,
-                              tip.data(), wk);
+                              inPre.data(), wk);
         if (rc != BEAGLE_SUCCESS) return rc;
         return deliverDerivativeSums(sums, count, outD1, outD1Sq, outD2);
     }

@@ -633,9 +633,49 @@ __device__ __forceinline__ double half_wave_sum(double v) {
     return v;
 }
 
+// Derivative tile sums of one branch from its per-category dot products
+// s3 = (pre . x, pre . (D1 x), pre . (D2 x)) of the lane's pattern (lane pair
+// h = 0, 1 already added): categories combined in FP64 through LDS, r_k =
+// N_k / D, tile sums of w r1, w r1^2 and w r2 into rows row * 3 + k of
+// `partial` (ceil(P/32) tiles), as the edge-derivative kernel writes them.
+__device__ __forceinline__ void branch_sums(const float (&s3)[3], const double* cw,
+                                            const double* pw, double* partial, int row, int tile,
+                                            const Dims& d, int c, int n, int h, int l, int p,
+                                            double (*dred)[3][32]) {
+    const double wc = cw[c];
+    if (h == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) dred[c][k][n] = wc * (double)s3[k];
+    }
+    __syncthreads();
+    const int ntp = (d.P + PTILE - 1) / PTILE;
+    if (c == 0) {
+        double D = 0.0, N1 = 0.0, N2 = 0.0;
+        for (int cc = 0; cc < d.C; ++cc) {
+            D += dred[cc][0][n];
+            N1 += dred[cc][1][n];
+            N2 += dred[cc][2][n];
+        }
+        const bool use = p < d.P && h == 0;
+        const double r1 = use ? N1 / D : 0.0, r2 = use ? N2 / D : 0.0;
+        const double wp = use ? pw[p] : 0.0;
+        const double s1 = half_wave_sum(wp * r1), sq = half_wave_sum(wp * r1 * r1);
+        const double s2 = half_wave_sum(wp * r2);
+        if (l == 0 && tile < ntp) {
+            GLOBAL double* pt = gout<double>(partial);
+            pt[((long)row * 3 + 0) * ntp + tile] = s1;
+            pt[((long)row * 3 + 1) * ntp + tile] = sq;
+            pt[((long)row * 3 + 2) * ntp + tile] = s2;
+        }
+    }
+}
+
 // MODE 0: FP32 matrix cores (KX = k groups of 8); 3, 4: split BF16 with 3
-// or 4 products (KX = k steps of 16).
-template <int MB, int KX, bool AUTO, int MODE>
+// or 4 products (KX = k steps of 16).  DV: the fused gradient's launch,
+// every job also forms its branch's derivative sums (pkT >= 0: a tip
+// branch, from u; else from pre(v) in registers and v's post-order
+// partials, with D1 and D2 packed as P_self is).
+template <int MB, int KX, bool AUTO, int MODE, bool DV>
 __global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, Dims d,
                                               const float* __restrict__ pack, TileList w,
                                               const double* __restrict__ cw,
@@ -668,13 +708,11 @@ __global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, D
     }
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) y[0][mb] *= par[mb];
-    if (J.pkT >= 0) {
+    if (DV && J.pkT >= 0) {
         // tip branch of the fused gradient, from u = y[0] without forming
         // pre(t) = P_t^T u: with the tip's state s, pre(t)_{s mod S} =
         // u . P_t[., s mod S] and pre(t) . (D_k x) = u . (P_t D_k)[., s]
-        // (BeagleCPUImpl.hpp:2086-2119 forms the same ratios from pre(t));
-        // categories combined in FP64 through LDS, tile sums of w r1, w r1^2
-        // and w r2 as the edge-derivative kernel writes them
+        // (BeagleCPUImpl.hpp:2086-2119 forms the same ratios from pre(t))
         float s3[3] = {0.f, 0.f, 0.f};
         const int pks[3] = {J.pkT, J.pkM1, J.pkM2};
 #pragma unroll
@@ -687,38 +725,37 @@ __global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, D
                 for (int r = 0; r < 16; ++r) s3[k] += y[0][mb][r] * t[mb][r];
             s3[k] += __shfl_xor(s3[k], 32);
         }
-        const double wc = cw[c];
-        if (h == 0) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) dred[c][k][n] = wc * (double)s3[k];
-        }
-        __syncthreads();
-        const int ntp = (d.P + PTILE - 1) / PTILE, tile = tp0 / PTILE;
-        if (c == 0) {
-            double D = 0.0, N1 = 0.0, N2 = 0.0;
-            for (int cc = 0; cc < d.C; ++cc) {
-                D += dred[cc][0][n];
-                N1 += dred[cc][1][n];
-                N2 += dred[cc][2][n];
-            }
-            const bool use = p < d.P && h == 0;
-            const double r1 = use ? N1 / D : 0.0, r2 = use ? N2 / D : 0.0;
-            const double wp = use ? pw[p] : 0.0;
-            const double s1 = half_wave_sum(wp * r1), sq = half_wave_sum(wp * r1 * r1);
-            const double s2 = half_wave_sum(wp * r2);
-            if (l == 0 && tile < ntp) {
-                GLOBAL double* pt = gout<double>(partial);
-                const long row = (long)J.oidx * 3;
-                pt[(row + 0) * ntp + tile] = s1;
-                pt[(row + 1) * ntp + tile] = sq;
-                pt[(row + 2) * ntp + tile] = s2;
-            }
-        }
+        branch_sums(s3, cw, pw, partial, J.oidx, tp0 / PTILE, d, c, n, h, l, p, dred);
     }
     if (!J.dest) return;   // no pre-order partials kept (uniform)
     const Rsrc aself = rsrc(pack + J.pkSelf + c * per, -1);
     if (MODE == 0) gemm_from_acc<MB>(aself, 16 * l, y[0], y[1]);
     else gemm_from_acc16<MB, MODE>(aself, 16 * l, y[0], y[1]);
+    if (DV && J.pkT < 0) {
+        // internal branch: pre(v) . (D_k x) = x . (D_k^T pre(v)) with
+        // pre(v) = y[1] (before its rescaling, which the ratios do not see)
+        // and x = v's post-order partials in the accumulator layout
+        floatx16 x[MB], t[MB];
+        load_acc<MB>(tile_of(J.xself, tbase, S), 4 * n + 512 * h, x);
+        float s3[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) s3[0] += y[1][mb][r] * x[mb][r];
+#pragma unroll
+        for (int k = 1; k < 3; ++k) {
+            const Rsrc ad = rsrc(pack + (k == 1 ? J.pkM1 : J.pkM2) + c * per, -1);
+            if (MODE == 0) gemm_from_acc<MB>(ad, 16 * l, y[1], t);
+            else gemm_from_acc16<MB, MODE>(ad, 16 * l, y[1], t);
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) s3[k] += t[mb][r] * x[mb][r];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s3[k] += __shfl_xor(s3[k], 32);
+        branch_sums(s3, cw, pw, partial, J.oidx, tp0 / PTILE, d, c, n, h, l, p, dred);
+    }
     const ScaleOut so{J.mode, J.logScalers, nullptr, J.scaleWrite, nullptr, J.accA,
                       J.accB, J.e1, J.e2, J.ed, J.dest};
     rescale_store<MB, AUTO>(y[1], so, tbase, S, d.C, c, n, h, p, in, red);
@@ -730,8 +767,12 @@ void launch_pre_mfma(const PreJob* djobs, int njobs, const Dims& d, const float*
     if (njobs <= 0 || (w.first && items <= 0)) return;
     const dim3 grid = w.first ? dim3(items) : dim3(d.Ppad / PTILE, njobs), block(64 * d.C);
 #define PREM_V(M, K, A, MODE)                                                                  \
-    hipLaunchKernelGGL((k_prem<M, K, A, MODE>), grid, block, 0, st, djobs, d, pack, w, cw, pw,   \
-                       partial);
+    if (partial)                                                                               \
+        hipLaunchKernelGGL((k_prem<M, K, A, MODE, true>), grid, block, 0, st, djobs, d, pack, w, \
+                           cw, pw, partial);                                                   \
+    else                                                                                       \
+        hipLaunchKernelGGL((k_prem<M, K, A, MODE, false>), grid, block, 0, st, djobs, d, pack, \
+                           w, cw, pw, partial);
 #define PREM(M, K, MODE)                                                                       \
     case K:                                                                                    \
         if (d.autoScale) { PREM_V(M, K, true, MODE) } else { PREM_V(M, K, false, MODE) }       \
