@@ -154,7 +154,10 @@ int beagleExtDestroyGraph(int instance, int graph);
  * derivative sums and keeps no pre-order partials (for branches to tips,
  * whose pre-order partials no later operation reads); this needs the
  * fused 4-state path (instances with 4 states and at most 8 categories)
- * and returns BEAGLE_ERROR_OUT_OF_RANGE elsewhere.
+ * or, for 5 to 64 states, a single-precision instance and a tip set with
+ * beagleSetTipStates as the operation's post-order buffer (its derivative
+ * sums are then formed from the tip's state without the tip's pre-order
+ * partials), and returns BEAGLE_ERROR_OUT_OF_RANGE elsewhere.
  * No counterpart in the reference. */
 int beagleExtUpdatePrePartialsWithDerivatives(int instance, const BeagleOperation* operations,
                                               int operationCount, const int* postBufferIndices,
