@@ -104,3 +104,25 @@ def test_allocation_failure_gives_out_of_memory(amd):
     ev = driver.Evaluator(amd, prob)
     assert np.isfinite(ev.log_likelihood())
     ev.close()
+
+
+@pytest.mark.parametrize("precision,tips_as", [("double", "states"), ("single", "partials")])
+def test_no_destination_pre_order_rules(amd, precision, tips_as):
+    """A pre-order operation without a destination in the derivative
+    extension, 20 states: refused (BEAGLE_ERROR_OUT_OF_RANGE) in double
+    precision and for a tip set as partials (its derivative sums need the
+    tip's state); a single-precision instance with the tip set as states
+    accepts it (test_gpu_parity checks those sums)."""
+    prob = synth.make_problem(5, 7, 90, S=20, missing=0.05)
+    ev = driver.Evaluator(amd, prob, precision=precision, tips_as=tips_as)
+    ops, post, d1, d2 = ev.fused_operations(keep_tips=True)
+    ops = np.array(ops, copy=True)
+    ops[post < ev.n, 0] = api.OP_NONE
+    ev.update_matrices()
+    ev.post_order()
+    root = prob.tree.root
+    with pytest.raises(api.BeagleError) as e:
+        ev.inst.log_likelihood_and_derivatives(root, ev.pre_base + root, ops, post, d1, d2,
+                                               cumulative=api.OP_NONE)
+    assert e.value.code == api.ERROR_OUT_OF_RANGE
+    ev.close()
