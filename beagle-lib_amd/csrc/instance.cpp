@@ -1408,6 +1408,17 @@ void Instance::launchPreMfma(const int* ops, int stride, int count, const TipDer
     auto dpj = reinterpret_cast<const PackJob*>(uploadJobs(pj.data(), pj.size() * sizeof(PackJob)));
     if (dp_) launch_pack_matrices64(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
     else launch_pack_matrices<float>(dpj, (int)pj.size(), dims_.S, dims_.C, dMatrices_, pack, stream_);
+    if (td && !dp_) {
+        std::vector<PackJob> prod;
+        for (const PackJob& q : pj)
+            if ((q.kind & ~PACK_TRANSPOSED_INPUT) == PACK_TIP_PRODUCT) prod.push_back(q);
+        if (!prod.empty()) {
+            auto dpp = reinterpret_cast<const PackJob*>(
+                uploadJobs(prod.data(), prod.size() * sizeof(PackJob)));
+            launch_pack_tip_products<float>(dpp, (int)prod.size(), dims_.S, dims_.C, dMatrices_,
+                                            pack, stream_);
+        }
+    }
     auto d = reinterpret_cast<const PreJob*>(uploadJobs(sorted.data(), count * sizeof(PreJob)));
     // operations by partition: each level's (operation, 32-pattern tile) list
     std::vector<int> lints, loff, litems;

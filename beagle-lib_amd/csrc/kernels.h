@@ -242,6 +242,10 @@ __host__ __device__ inline bool mfma_post_supported(int S, int C) { return S > 4
 template <typename R>
 void launch_pack_matrices(const PackJob* jobs, int njobs, int S, int C, const void* mats,
                           float* pack, hipStream_t st);
+// The PACK_TIP_PRODUCT jobs (which launch_pack_matrices skips), S <= 64.
+template <typename R>
+void launch_pack_tip_products(const PackJob* jobs, int njobs, int S, int C, const void* mats,
+                              float* pack, hipStream_t st);
 // One launch per dependency level: grid (pattern tiles, njobs).  variant 0:
 // partials loaded a few k groups ahead; 1: the whole partials tile up front.
 // w.first: a work list of `items` 32-pattern tiles (TileList).

@@ -148,28 +148,15 @@ __global__ __launch_bounds__(WG) void k_pack_mats(const PackJob* __restrict__ jo
             }
             v = __builtin_bit_cast(float, (unsigned)half[0] | ((unsigned)half[1] << 16));
         }
-    } else if (kind == PACK_TIP_COLUMN || kind == PACK_TIP_PRODUCT) {
+    } else if (kind == PACK_TIP_PRODUCT) {
+        return;   // k_pack_tip_products
+    } else if (kind == PACK_TIP_COLUMN) {
         // tip branch of the fused gradient, [state s (0..S)][row j]:
-        // COLUMN P[j][s mod S]; PRODUCT (P D)[j][s] = sum_i P[j][i] D[i][s]
-        // (FP64 sum; s = S is the derivative matrix's pad column, 0 for a
-        // transposed input as in the rows-by-state pack)
+        // P[j][s mod S]
         const int W = MB * 32;
         if (e < (S + 1) * W) {
             const int s = e / W, j = e - s * W;
-            if (j < S) {
-                if (kind == PACK_TIP_COLUMN) {
-                    v = at(j, s % S);
-                } else {
-                    const R* D = mats + J.src2 + (long)c * S * (S + 1);
-                    double acc = 0.0;
-                    for (int i = 0; i < S; ++i) {
-                        const double dv = tin ? (s < S ? (double)D[s * (S + 1) + i] : 0.0)
-                                              : (double)D[i * (S + 1) + s];
-                        acc += (double)at(j, i) * dv;
-                    }
-                    v = (float)acc;
-                }
-            }
+            if (j < S) v = at(j, s % S);
         }
     } else if (kind == PACK_PRE_SELF_BF16) {
         // M^T u with u in the accumulator layout on v_mfma_f32_32x32x16_bf16:
@@ -206,6 +193,48 @@ __global__ __launch_bounds__(WG) void k_pack_mats(const PackJob* __restrict__ jo
         }
     }
     pack[J.dst + (long)c * per + e] = v;
+}
+
+// PACK_TIP_PRODUCT blocks, one workgroup per (job, category): P and D as
+// used staged in LDS, then [state s (0..S)][row j] = (P D)[j][s] = sum_i
+// P[j][i] D[i][s] summed in FP64 (s = S: the derivative matrix's pad column,
+// 0 for a transposed input as in the rows-by-state pack).
+template <typename R>
+__global__ __launch_bounds__(WG) void k_pack_tip_products(const PackJob* __restrict__ jobs,
+                                                          int S, int C, const R* __restrict__ mats,
+                                                          float* __restrict__ pack) {
+    __shared__ float sm[64 * 64], sd[64 * 65];
+    const PackJob J = jobs[blockIdx.x];
+    const int c = blockIdx.y;
+    const bool tin = (J.kind & PACK_TRANSPOSED_INPUT) != 0;
+    const R* M = mats + J.src + (long)c * S * (S + 1);
+    const R* D = mats + J.src2 + (long)c * S * (S + 1);
+    for (int e = threadIdx.x; e < S * S; e += WG) {
+        const int j = e / S, i = e - j * S;
+        sm[j * 64 + i] = (float)(tin ? M[i * (S + 1) + j] : M[j * (S + 1) + i]);
+    }
+    for (int e = threadIdx.x; e < S * (S + 1); e += WG) {
+        const int i = e / (S + 1), s = e - i * (S + 1);
+        sd[i * 65 + s] = (float)(tin ? (s < S ? D[s * (S + 1) + i] : (R)0) : D[i * (S + 1) + s]);
+    }
+    __syncthreads();
+    const int W = mfma_mblocks(S) * 32, per = mfma_pack_reals(S);
+    float* out = pack + J.dst + (long)c * per;
+    for (int e = threadIdx.x; e < (S + 1) * W; e += WG) {
+        const int s = e / W, j = e - s * W;
+        double acc = 0.0;
+        if (j < S)
+            for (int i = 0; i < S; ++i) acc += (double)sm[j * 64 + i] * (double)sd[i * 65 + s];
+        out[e] = (float)acc;
+    }
+}
+
+template <typename R>
+void launch_pack_tip_products(const PackJob* jobs, int njobs, int S, int C, const void* mats,
+                              float* pack, hipStream_t st) {
+    if (njobs <= 0) return;
+    hipLaunchKernelGGL(k_pack_tip_products<R>, dim3(njobs, C), dim3(WG), 0, st, jobs, S, C,
+                       reinterpret_cast<const R*>(mats), pack);
 }
 
 template <typename R>
@@ -1585,6 +1614,8 @@ void launch_deriv_mfma64(const DerivJob* djobs, int njobs, int nd, const Dims& d
 #undef DERIVM64
 }
 
+template void launch_pack_tip_products<float>(const PackJob*, int, int, int, const void*, float*,
+                                              hipStream_t);
 template void launch_pack_matrices<float>(const PackJob*, int, int, int, const void*, float*,
                                           hipStream_t);
 
