@@ -483,16 +483,18 @@ def test_mfma_post_order_partitions(amd, oracle):
     assert rel(out[0][1], out[1][1]) < 1e-6
 
 
-@pytest.mark.parametrize("S,traversal", [(4, "1"), (4, None), (20, None), (61, None)])
-def test_partition_work_lists_match_oracle(amd, oracle, monkeypatch, S, traversal):
+@pytest.mark.parametrize("S,traversal,precision", [(4, "1", "single"), (4, None, "single"),
+                                                  (20, None, "single"), (61, None, "single"),
+                                                  (4, "1", "double"), (20, None, "double")])
+def test_partition_work_lists_match_oracle(amd, oracle, monkeypatch, S, traversal, precision):
     """Operations by partition over ten uneven partitions (one without
     patterns, some smaller than a tile, ranges that start and end inside
     tiles): every level launch over pattern ranges runs one workgroup per
     (operation, tile) item of its work list.  Post-order, root
     log-likelihoods by partition and pre-order partials by partition against
-    the oracle: logL 1e-6 relative (per partition and total), pre-order
-    partials 1e-5 of each row's maximum.  traversal "1" forces the 4-state
-    level plan (batched levels)."""
+    the oracle: logL 1e-6 relative (FP64 1e-10) per partition and total,
+    pre-order partials 1e-5 (FP64 1e-10) of each row's maximum.  traversal
+    "1" forces the 4-state level plan (batched levels)."""
     if traversal:
         monkeypatch.setenv("BEAGLE_AMD_TRAVERSAL", traversal)
     sizes = [0, 3, 40, 70, 1, 150, 33, 64, 5, 200]
@@ -500,7 +502,7 @@ def test_partition_work_lists_match_oracle(amd, oracle, monkeypatch, S, traversa
     prob = synth.make_problem(41 + S, 9, sum(sizes), S=S, missing=0.05, mean_length=0.2)
     parts = np.repeat(np.arange(K), sizes).astype(np.int32)
     out = []
-    for lib, prec in ((amd, "single"), (oracle, "double")):
+    for lib, prec in ((amd, precision), (oracle, "double")):
         ev = driver.Evaluator(lib, prob, precision=prec, matrices="explicit")
         ev.update_matrices()
         ev.inst.set_pattern_partitions(K, parts)
@@ -518,11 +520,12 @@ def test_partition_work_lists_match_oracle(amd, oracle, monkeypatch, S, traversa
         out.append((np.asarray(by), tot, pp))
         ev.close()
     (bg, tg, pg), (bo, to, po) = out
-    assert rel(tg, to) < 1e-6
-    close(bg, bo, 1e-6, 0)
+    tl, tp = (1e-6, 1e-5) if precision == "single" else (1e-10, 1e-10)
+    assert rel(tg, to) < tl
+    close(bg, bo, tl, 0)
     for a, b in zip(pg, po):
         keep = b.max(axis=1) > 1e-30
-        close_rows(a[keep], b[keep], S, 1e-5)
+        close_rows(a[keep], b[keep], S, tp)
 
 
 def test_matrices_match_oracle(amd, oracle):
