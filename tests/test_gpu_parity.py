@@ -485,7 +485,8 @@ def test_mfma_post_order_partitions(amd, oracle):
 
 @pytest.mark.parametrize("S,traversal,precision", [(4, "1", "single"), (4, None, "single"),
                                                   (20, None, "single"), (61, None, "single"),
-                                                  (4, "1", "double"), (20, None, "double")])
+                                                  (4, "1", "double"), (20, None, "double"),
+                                                  (61, None, "bf16x3")])
 def test_partition_work_lists_match_oracle(amd, oracle, monkeypatch, S, traversal, precision):
     """Operations by partition over ten uneven partitions (one without
     patterns, some smaller than a tile, ranges that start and end inside
@@ -494,7 +495,8 @@ def test_partition_work_lists_match_oracle(amd, oracle, monkeypatch, S, traversa
     log-likelihoods by partition and pre-order partials by partition against
     the oracle: logL 1e-6 relative (FP64 1e-10) per partition and total,
     pre-order partials 1e-5 (FP64 1e-10) of each row's maximum.  traversal
-    "1" forces the 4-state level plan (batched levels)."""
+    "1" forces the 4-state level plan (batched levels); bf16x3: the split-BF16
+    engine (pre-order partials 1e-4 of each row's maximum)."""
     if traversal:
         monkeypatch.setenv("BEAGLE_AMD_TRAVERSAL", traversal)
     sizes = [0, 3, 40, 70, 1, 150, 33, 64, 5, 200]
@@ -503,7 +505,10 @@ def test_partition_work_lists_match_oracle(amd, oracle, monkeypatch, S, traversa
     parts = np.repeat(np.arange(K), sizes).astype(np.int32)
     out = []
     for lib, prec in ((amd, precision), (oracle, "double")):
-        ev = driver.Evaluator(lib, prob, precision=prec, matrices="explicit")
+        ev = driver.Evaluator(lib, prob, precision="single" if prec == "bf16x3" else prec,
+                              matrices="explicit")
+        if prec == "bf16x3":
+            ev.inst.set_split_bf16(3)
         ev.update_matrices()
         ev.inst.set_pattern_partitions(K, parts)
         pops = [list(o) + [q, -1] for q in range(K) for o in synth.postorder_ops(prob.tree)]
@@ -520,7 +525,7 @@ def test_partition_work_lists_match_oracle(amd, oracle, monkeypatch, S, traversa
         out.append((np.asarray(by), tot, pp))
         ev.close()
     (bg, tg, pg), (bo, to, po) = out
-    tl, tp = (1e-6, 1e-5) if precision == "single" else (1e-10, 1e-10)
+    tl, tp = {"single": (1e-6, 1e-5), "bf16x3": (1e-6, 1e-4), "double": (1e-10, 1e-10)}[precision]
     assert rel(tg, to) < tl
     close(bg, bo, tl, 0)
     for a, b in zip(pg, po):
