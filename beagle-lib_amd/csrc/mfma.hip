@@ -979,8 +979,8 @@ void launch_deriv_mfma(const DerivJob* djobs, int njobs, int nd, const Dims& d,
 // and B[k][j] = X[32 jb + l % 32][k0 + l / 32].  X_c is summed in FP32 over
 // the chunk (every term is non-negative) and written as FP64 to
 // out[o][c][chunk][S^2]; the chunks are added in FP64 by k_qgx_eigen.  At
-// most 4 categories (one wave per SIMD, so the accumulators may take the
-// AGPRs beside 256 VGPRs).
+// most 4 categories; two workgroups per CU (256 registers per wave with no
+// spills, and the LDS of two workgroups).
 constexpr int QGM_LD = PTILE + 1;   // row stride of the staged tiles (floats)
 
 __host__ __device__ constexpr size_t qgm_lds_bytes(int MB, int C) {
@@ -988,7 +988,7 @@ __host__ __device__ constexpr size_t qgm_lds_bytes(int MB, int C) {
 }
 
 template <int MB, int KG>
-__global__ __launch_bounds__(256) void k_qgm(const PreJob* __restrict__ jobs, Dims d,
+__global__ __launch_bounds__(256, 2) void k_qgm(const PreJob* __restrict__ jobs, Dims d,
                                              const float* __restrict__ pack,
                                              const double* __restrict__ cw,
                                              const double* __restrict__ pw, int tilesPerChunk,
