@@ -996,6 +996,31 @@ def test_rate_matrix_gradient_matrix_cores_against_fp64(amd, S, C, tips_as):
     assert np.abs(p32 - p64).max() <= 1e-6 * np.abs(p64).max()
 
 
+@pytest.mark.parametrize("S,terms", [(61, 3), (20, 4)])
+def test_rate_matrix_gradient_split_bf16_error_bound(amd, S, terms):
+    """The rate-matrix gradient of a split-BF16 instance (post-order,
+    pre-order and derivatives on the BF16 matrix cores) against the FP64
+    instance's: within 2e-5 of the largest dQ entry (the BF16 engine's
+    pre-order partials carry up to 2^-17 relative rounding per split
+    operand; the FP32 engine meets 1e-6), dlogL/dpi_root likewise, and the
+    branch gradient within 1e-6."""
+    prob = synth.make_problem(410 + S, 24, 1900, S=S, C=4, shape="yule", missing=0.05,
+                              mean_length=0.2)
+    res = []
+    for precision, scaling, t in (("single", "auto", terms), ("double", "none", 0)):
+        ev = driver.Evaluator(amd, prob, scaling=scaling, precision=precision)
+        if t:
+            ev.inst.set_split_bf16(t)
+        g = ev.fused_gradient(keep_tips=False)
+        res.append((g, ev.rate_matrix_gradient()))
+        ev.close()
+    (gb, (qb, pb)), (g64, (q64, p64)) = res
+    assert np.abs(qb - q64).max() <= 2e-5 * np.abs(q64).max()
+    assert np.abs(pb - p64).max() <= 2e-5 * np.abs(p64).max()
+    for a, b in zip(gb[1:], g64[1:]):
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
+
+
 @pytest.mark.parametrize("precision,S", [("double", 4), ("single", 4), ("double", 20)])
 def test_torch_autograd_binding(amd, oracle, precision, S):
     """beagle_amd.torch_fn: logL of (branch lengths, GTR(exch, freqs)) and
