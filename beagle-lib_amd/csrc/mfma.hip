@@ -705,7 +705,7 @@ __device__ __forceinline__ void branch_sums(const float (&s3)[3], const double* 
 // branch, from u; else from pre(v) in registers and v's post-order
 // partials, with D1 and D2 packed as P_self is).
 template <int MB, int KX, bool AUTO, int MODE, bool DV>
-__global__ __launch_bounds__(512) void k_prem(const PreJob* __restrict__ jobs, Dims d,
+__global__ __launch_bounds__(512, 3) void k_prem(const PreJob* __restrict__ jobs, Dims d,
                                               const float* __restrict__ pack, TileList w,
                                               const double* __restrict__ cw,
                                               const double* __restrict__ pw, double* partial) {
